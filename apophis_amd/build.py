@@ -1,0 +1,52 @@
+"""Build the native library in-tree: apophis_amd/_build/libapophis_hydro.so (gfx950).
+
+Every translation unit is compiled with -ffp-contract=off (no fused multiply-add, so the
+FP64 results keep the reference's rounding) and with packed FP32 instructions disabled.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SRC = os.path.join(HERE, "csrc")
+OUT = os.path.join(HERE, "_build")
+SOURCES = ["aph_runtime.hip", "aph_plain.hip", "aph_fused.hip"]
+ARCH = os.environ.get("APH_OFFLOAD_ARCH", "gfx950")
+FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+
+def build(verbose=False, jobs=3):
+    os.makedirs(OUT, exist_ok=True)
+    objs, procs = [], []
+    for s in SOURCES:
+        o = os.path.join(OUT, s.replace(".hip", ".o"))
+        objs.append(o)
+        cmd = ["hipcc", *FLAGS, "-c", os.path.join(SRC, s), "-o", o]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append(subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT))
+        if len(procs) >= jobs:
+            _wait(procs)
+            procs = []
+    _wait(procs)
+    lib = os.path.join(OUT, "libapophis_hydro.so")
+    tmp = lib + ".tmp"
+    subprocess.run(["hipcc", *FLAGS, "-shared", "-o", tmp, *objs], check=True)
+    os.replace(tmp, lib)
+    return lib
+
+
+def _wait(procs):
+    for p in procs:
+        out, _ = p.communicate()
+        txt = "\n".join(l for l in out.decode().splitlines() if "packed-fp32-ops" not in l)
+        if p.returncode != 0:
+            sys.stderr.write(txt + "\n")
+            raise RuntimeError("hipcc failed: " + " ".join(map(str, p.args)))
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

@@ -1,0 +1,696 @@
+// aph_fused.hip -- the batched form: one persistent launch per RK2 stage over every
+// block of a block set (uneven sizes allowed), fusing ConsToPrim, reconstruction,
+// the Riemann solve, the flux-divergence update and (optionally) the gravity source.
+//
+// Work item = one tile of one block.  A 64-lane workgroup takes items in a static
+// persistent schedule; for each item it sweeps x, then y, then z:
+//   * a lane owns one (pencil, segment) of the tile and marches it (aph_march.hpp),
+//     loading conserved variables straight from HBM -- from the neighbouring block
+//     or through the physical-boundary rule when the stencil leaves the block, so no
+//     ghost exchange pass and no ghost storage reads are needed;
+//   * the x and y sweeps accumulate du = A*F(+) - A*F(-) per cell in LDS in the
+//     reference's order (x first, then y, then z, FluxDivHelper upstream);
+//   * the last sweep finishes du, applies gam0*u + gam1*u_n + beta*dt*(-du/V), the
+//     gravity source (gravity.cpp:386-407) and, in stage 2, the time-step estimate of
+//     the new state (hydro.cpp:369-419) min-reduced on the device.
+// Floors are applied on the fly every time a conserved state is read, once per
+// stage, which is where the reference's ConsToPrim applies them.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "aph_device.hpp"
+#include "aph_march.hpp"
+#include "aph_runtime.hpp"
+
+namespace aph {
+
+struct TileCfg {
+  int t[3];  // tile extent per axis (cells)
+  int s[3];  // segment length per axis (cells one lane marches)
+};
+
+constexpr int kMaxClasses = 8;
+
+struct FusedArgs {
+  const aph_block *blocks;
+  const int4 *items;  // {block | cls << 24, t0x, t0y, t0z}; block < 0: padding
+  int nitems;
+  int ndim;
+  TileCfg cfg[kMaxClasses];
+  aph_eos eos;
+  double g[3];
+  int grav;
+  const double *u_in;
+  const double *u_n;
+  double *u_out;
+  const double *dt;
+  double gam0, gam1, beta;
+  double *dt_min;  // stage 2: atomic min of cfl * dt candidate; stage 1: reset here
+  double cfl;
+  int stage;
+};
+
+namespace {
+
+// Bases (element offset of component 0) of the block and its two neighbours along
+// axis D, or -1 for a physical boundary.
+struct AxisNbrs {
+  int64_t self, lo, hi;
+  int bclo, bchi;
+};
+
+template <int NV>
+__device__ __forceinline__ void load_cell(const FusedArgs &a, const aph_block &bk,
+                                          const AxisNbrs &nb, int D, int n_d, int c,
+                                          int64_t tbase, int64_t sd, int64_t nc, double *u) {
+  int64_t base = nb.self;
+  int idx = c;
+  bool flip = false;
+  if (c < 0) {
+    if (nb.lo >= 0) {
+      base = nb.lo;
+      idx = c + n_d;
+    } else if (nb.bclo == APH_BC_REFLECT) {
+      idx = -1 - c;
+      flip = true;
+    } else {
+      idx = 0;
+    }
+  } else if (c >= n_d) {
+    if (nb.hi >= 0) {
+      base = nb.hi;
+      idx = c - n_d;
+    } else if (nb.bchi == APH_BC_REFLECT) {
+      idx = 2 * n_d - 1 - c;
+      flip = true;
+    } else {
+      idx = n_d - 1;
+    }
+  }
+  const double *p = a.u_in + base + tbase + (int64_t)idx * sd;
+#pragma unroll
+  for (int v = 0; v < NV; ++v) u[v] = p[v * nc];
+  if (flip) u[1 + D] = -1.0 * u[1 + D];
+}
+
+template <int RECON, int RS, int NV, int NDIM, int D>
+__device__ __forceinline__ void sweep(const FusedArgs &a, const aph_block &bk, int bidx,
+                                      const TileCfg &cfg, const int *t0, double *du_s,
+                                      double &dtmin) {
+  constexpr bool LAST = (D == NDIM - 1);
+  constexpr int A = (D == 0) ? 1 : 0;  // first transverse axis (fastest across lanes)
+  constexpr int B = (D == 2) ? 1 : 2;  // second transverse axis
+  const int ndim = NDIM;
+  const int N0 = ext(bk, 0, ndim), N1 = ext(bk, 1, ndim), N2 = ext(bk, 2, ndim);
+  const int64_t nc = (int64_t)N0 * N1 * N2;
+  const int64_t st[3] = {1, N0, (int64_t)N0 * N1};
+  const int g[3] = {gst(bk, 0, ndim), gst(bk, 1, ndim), gst(bk, 2, ndim)};
+  constexpr int sdn = (D == 0) ? 1 : (D == 1 ? 3 : 9);
+  AxisNbrs nb;
+  nb.self = (int64_t)NV * bk.cell_base;
+  {
+    const int l = bk.nbr[13 - sdn], h = bk.nbr[13 + sdn];
+    nb.lo = l >= 0 ? (int64_t)NV * a.blocks[l].cell_base : -1;
+    nb.hi = h >= 0 ? (int64_t)NV * a.blocks[h].cell_base : -1;
+    nb.bclo = bk.bc[2 * D];
+    nb.bchi = bk.bc[2 * D + 1];
+  }
+  const int n_d = bk.nx[D];
+  const int nseg = cfg.t[D] / cfg.s[D];
+  const int units = cfg.t[A] * cfg.t[B] * nseg;
+  const double area = bk.area[D];
+  const double vol = bk.vol;
+  const double beta_dt = a.beta * a.dt[0];
+  const int tx = cfg.t[0], ty = cfg.t[1];
+  for (int unit = threadIdx.x; unit < units; unit += 64) {
+    int seg, pa, pb;
+    if (D == 0) {  // segments of one row on adjacent lanes
+      seg = unit % nseg;
+      const int p = unit / nseg;
+      pa = p % cfg.t[A];
+      pb = p / cfg.t[A];
+    } else {  // adjacent lanes on adjacent x (coalesced)
+      pa = unit % cfg.t[A];
+      const int p = unit / cfg.t[A];
+      seg = p % nseg;
+      pb = p / nseg;
+    }
+    const int ca = t0[A] + pa, cb = t0[B] + pb;  // interior coordinates
+    const int64_t tbase = (int64_t)(ca + g[A]) * st[A] + (int64_t)(cb + g[B]) * st[B] +
+                          (int64_t)g[D] * st[D];
+    const int f_lo = t0[D] + seg * cfg.s[D];
+    const int f_hi = f_lo + cfg.s[D];
+    double fprev[NV];
+    auto load = [&](int c, double *dst) {
+      double u[NV];
+      load_cell<NV>(a, bk, nb, D, n_d, c, tbase, st[D], nc, u);
+      cons_to_prim_pt<NV>(u, a.eos, dst, dst + NV);
+    };
+    auto emit = [&](int f, const double *fo, const double *, const double *) {
+      if (f > f_lo) {
+        const int m = f - 1;  // cell whose upper face is f
+        int lc[3];
+        lc[D] = m - t0[D];
+        lc[A] = pa;
+        lc[B] = pb;
+        const int l = (lc[2] * ty + lc[1]) * tx + lc[0];
+        const int tcells = tx * ty * cfg.t[2];
+        if constexpr (!LAST) {
+#pragma unroll
+          for (int v = 0; v < NV; ++v) {
+            const double d = area * fo[v] - area * fprev[v];
+            if (D == 0)
+              du_s[v * tcells + l] = d;
+            else
+              du_s[v * tcells + l] = du_s[v * tcells + l] + d;
+          }
+        } else {
+          const int64_t cidx = tbase + (int64_t)m * st[D];
+          double u[NV], w[NV], gm[2];
+          const double *pin = a.u_in + nb.self + cidx;
+#pragma unroll
+          for (int v = 0; v < NV; ++v) u[v] = pin[v * nc];
+          cons_to_prim_pt<NV>(u, a.eos, w, gm);
+          double un[NV];
+          if (a.stage == 2) {
+            const double *pn = a.u_n + nb.self + cidx;
+            double wn[NV], gn[2];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) un[v] = pn[v * nc];
+            cons_to_prim_pt<NV>(un, a.eos, wn, gn);
+          } else {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) un[v] = u[v];
+          }
+          double out[NV];
+#pragma unroll
+          for (int v = 0; v < NV; ++v) {
+            double du;
+            if (NDIM == 1)
+              du = (area * fo[v] - area * fprev[v]);
+            else
+              du = du_s[v * tcells + l] + (area * fo[v] - area * fprev[v]);
+            const double div = -du / vol;
+            out[v] = a.gam0 * u[v] + a.gam1 * un[v] + beta_dt * div;
+          }
+          if (a.grav) {  // ApplyGravity with the stage-start primitives
+            const double gdt = beta_dt;
+            out[1] += a.g[0] * w[0] * gdt;
+            double de = a.g[0] * w[1];
+            if (NDIM >= 2) {
+              out[2] += a.g[1] * w[0] * gdt;
+              de += a.g[1] * w[2];
+            }
+            if (NDIM >= 3) {
+              out[3] += a.g[2] * w[0] * gdt;
+              de += a.g[2] * w[3];
+            }
+            out[4] += de * w[0] * gdt;
+          }
+          double *po = a.u_out + nb.self + cidx;
+#pragma unroll
+          for (int v = 0; v < NV; ++v) po[v * nc] = out[v];
+          if (a.stage == 2) {  // EstimateTimestep of the new state (after its floors)
+            double wq[NV], gq[2];
+            cons_to_prim_pt<NV>(out, a.eos, wq, gq);
+            dtmin = fmin(dtmin, dt_point(wq, out[4], a.eos, bk.dx, NDIM));
+          }
+        }
+      }
+#pragma unroll
+      for (int v = 0; v < NV; ++v) fprev[v] = fo[v];
+    };
+    march<RECON, RS, NV, 1 + D>(f_lo, f_hi, load, emit);
+  }
+}
+
+template <int RECON, int RS, int NV, int NDIM>
+__global__ void __launch_bounds__(64) k_fused(FusedArgs a) {
+  extern __shared__ double du_s[];
+  if (a.stage == 1 && blockIdx.x == 0 && threadIdx.x == 0 && a.dt_min)
+    *a.dt_min = 1.79769313486231570815e308;
+  double dtmin = 1.79769313486231570815e308;
+  for (int it = blockIdx.x; it < a.nitems; it += gridDim.x) {
+    const int4 item = a.items[it];
+    if (item.x < 0) continue;
+    const int b = item.x & 0xFFFFFF;
+    const int cls = item.x >> 24;
+    const aph_block &bk = a.blocks[b];
+    const TileCfg cfg = a.cfg[cls];
+    const int t0[3] = {item.y, item.z, item.w};
+    sweep<RECON, RS, NV, NDIM, 0>(a, bk, b, cfg, t0, du_s, dtmin);
+    if constexpr (NDIM >= 2) {
+      __syncthreads();
+      sweep<RECON, RS, NV, NDIM, 1>(a, bk, b, cfg, t0, du_s, dtmin);
+    }
+    if constexpr (NDIM >= 3) {
+      __syncthreads();
+      sweep<RECON, RS, NV, NDIM, 2>(a, bk, b, cfg, t0, du_s, dtmin);
+    }
+    __syncthreads();
+  }
+  if (a.stage == 2 && a.dt_min) {
+    for (int o = 32; o > 0; o >>= 1) dtmin = fmin(dtmin, __shfl_xor(dtmin, o));
+    if (threadIdx.x == 0) atomic_min_pos(a.dt_min, a.cfl * dtmin);
+  }
+}
+
+// Initial time step of a raw conserved state (floors applied on the fly).
+template <int NV, int NDIM>
+__global__ void __launch_bounds__(256)
+    k_plan_dt(const aph_block *__restrict__ blocks, aph_eos eos, const double *__restrict__ cons,
+              double cfl, double *__restrict__ out) {
+  const aph_block &bk = blocks[blockIdx.y];
+  const int N0 = ext(bk, 0, NDIM), N1 = ext(bk, 1, NDIM), N2 = ext(bk, 2, NDIM);
+  const int64_t nc = (int64_t)N0 * N1 * N2;
+  const int gx = gst(bk, 0, NDIM), gy = gst(bk, 1, NDIM), gz = gst(bk, 2, NDIM);
+  const int64_t nint = (int64_t)bk.nx[0] * bk.nx[1] * bk.nx[2];
+  const double *u = cons + (int64_t)NV * bk.cell_base;
+  double m = 1.79769313486231570815e308;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nint;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(t % bk.nx[0]);
+    const int j = (int)((t / bk.nx[0]) % bk.nx[1]);
+    const int k = (int)(t / ((int64_t)bk.nx[0] * bk.nx[1]));
+    const int64_t c = ((int64_t)(k + gz) * N1 + (j + gy)) * N0 + (i + gx);
+    double uu[NV], w[NV], gm[2];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) uu[v] = u[v * nc + c];
+    cons_to_prim_pt<NV>(uu, eos, w, gm);
+    m = fmin(m, dt_point(w, uu[4], eos, bk.dx, NDIM));
+  }
+  for (int o = 32; o > 0; o >>= 1) m = fmin(m, __shfl_xor(m, o));
+  if ((threadIdx.x & 63) == 0) atomic_min_pos(out, cfl * m);
+}
+
+__global__ void k_fill(double *p, double v) { *p = v; }
+
+// dt policy of the upstream EvolutionDriver: time += dt; dt <- min(2 dt, estimate),
+// clipped so that the step does not pass tlim.
+__global__ void k_dt_finalize(double *dt, double *time, const double *est, double tlim,
+                              int fixed) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double d = *dt;
+  const double t = *time + d;
+  *time = t;
+  if (!fixed) {
+    double nd = (d < 0.1 * 1.79769313486231570815e308) ? 2.0 * d : d;
+    const double e = *est;
+    nd = (e < nd) ? e : nd;
+    if (t < tlim && (tlim - t) < nd) nd = tlim - t;
+    if (!(t < tlim)) nd = 0.0;
+    *dt = nd;
+  }
+}
+
+}  // namespace
+}  // namespace aph
+
+using namespace aph;
+
+struct aph_plan {
+  const aph_blockset *bs = nullptr;
+  int recon = 0, rsolver = 0;
+  aph_eos eos{};
+  double g[3] = {0, 0, 0};
+  int grav = 0;
+  std::vector<aph_tile> tiles;  // per size class (keyed by nx[0])
+  std::vector<int> class_n;     // class index -> nx[0]
+  int4 *d_items = nullptr;
+  int nitems = 0;
+  int grid = 0;
+  size_t lds = 0;
+  double *d_est = nullptr;  // device dt estimate accumulator
+  bool dirty = true;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+using KernelFn = void (*)(FusedArgs);
+
+template <int RECON, int RS, int NV>
+KernelFn pick_dim(int ndim) {
+  switch (ndim) {
+    case 1: return k_fused<RECON, RS, NV, 1>;
+    case 2: return k_fused<RECON, RS, NV, 2>;
+    default: return k_fused<RECON, RS, NV, 3>;
+  }
+}
+
+template <int RECON, int RS>
+KernelFn pick_nv(int nv, int ndim) {
+  switch (nv) {
+    case 5: return pick_dim<RECON, RS, 5>(ndim);
+    case 6: return pick_dim<RECON, RS, 6>(ndim);
+    default: return nullptr;
+  }
+}
+
+KernelFn pick_kernel(int recon, int rs, int nv, int ndim) {
+  if (recon == RECON_PPM && rs == RS_HLLC) return pick_nv<RECON_PPM, RS_HLLC>(nv, ndim);
+  if (recon == RECON_PPM && rs == RS_HLLE) return pick_nv<RECON_PPM, RS_HLLE>(nv, ndim);
+  if (recon == RECON_PLM && rs == RS_HLLC) return pick_nv<RECON_PLM, RS_HLLC>(nv, ndim);
+  if (recon == RECON_PLM && rs == RS_HLLE) return pick_nv<RECON_PLM, RS_HLLE>(nv, ndim);
+  return nullptr;
+}
+
+aph_tile default_tile(int n, int ndim) {
+  aph_tile t{n, 1, 1, 1};
+  if (ndim == 3) {
+    int e = std::min(n, 8);
+    t.tx = t.ty = t.tz = e;
+  } else if (ndim == 2) {
+    int e = std::min(n, 16);
+    t.tx = t.ty = e;
+  } else {
+    t.tx = std::min(n, 256);
+  }
+  return t;
+}
+
+// Segment length along an axis: the largest divisor of the tile extent that keeps
+// at least 64 (pencil, segment) units per sweep, so every lane of the wave works.
+int seg_len(int t_d, int pencils) {
+  int best = 1;
+  for (int s = 1; s <= t_d; ++s)
+    if (t_d % s == 0 && (int64_t)pencils * (t_d / s) >= 64) best = s;
+  return best;
+}
+
+int build_items(aph_context *ctx, aph_plan *p) {
+  const aph_blockset *bs = p->bs;
+  const int ndim = bs->ndim;
+  p->class_n.clear();
+  std::vector<int> cls_of(bs->nblocks);
+  for (int b = 0; b < bs->nblocks; ++b) {
+    const aph_block &k = bs->h[b];
+    int n = k.nx[0];
+    auto it = std::find(p->class_n.begin(), p->class_n.end(), n);
+    int c;
+    if (it == p->class_n.end()) {
+      if ((int)p->class_n.size() >= kMaxClasses)
+        return set_error(ctx, APH_E_INVALID, "more than %d block size classes", kMaxClasses);
+      p->class_n.push_back(n);
+      c = (int)p->class_n.size() - 1;
+    } else {
+      c = (int)(it - p->class_n.begin());
+    }
+    cls_of[b] = c;
+  }
+  // tile per class
+  std::vector<aph_tile> tl(p->class_n.size());
+  for (size_t c = 0; c < p->class_n.size(); ++c) {
+    int n = p->class_n[c];
+    aph_tile t = default_tile(n, ndim);
+    for (const auto &u : p->tiles)
+      if (u.n == n) t = u;
+    tl[c] = t;
+  }
+  // per XCD lists, greedy by cost (cells) so every L2 sees whole blocks
+  constexpr int kXcd = 8;
+  std::vector<std::vector<int4>> lists(kXcd);
+  std::vector<int64_t> load(kXcd, 0);
+  size_t maxcells = 0;
+  for (int b = 0; b < bs->nblocks; ++b) {
+    const aph_block &k = bs->h[b];
+    const aph_tile &t = tl[cls_of[b]];
+    const int tt[3] = {t.tx, t.ty, t.tz};
+    for (int d = 0; d < 3; ++d)
+      if (tt[d] < 1 || k.nx[d] % tt[d] != 0)
+        return set_error(ctx, APH_E_INVALID, "tile %dx%dx%d does not divide block %d (%dx%dx%d)",
+                         t.tx, t.ty, t.tz, b, k.nx[0], k.nx[1], k.nx[2]);
+    maxcells = std::max(maxcells, (size_t)t.tx * t.ty * t.tz);
+    int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    for (int z = 0; z < k.nx[2]; z += t.tz)
+      for (int y = 0; y < k.nx[1]; y += t.ty)
+        for (int xx = 0; xx < k.nx[0]; xx += t.tx)
+          lists[x].push_back(make_int4(b | (cls_of[b] << 24), xx, y, z));
+    load[x] += (int64_t)k.nx[0] * k.nx[1] * k.nx[2];
+  }
+  size_t maxlen = 0;
+  for (auto &l : lists) maxlen = std::max(maxlen, l.size());
+  std::vector<int4> items(maxlen * kXcd, make_int4(-1, 0, 0, 0));
+  for (int x = 0; x < kXcd; ++x)
+    for (size_t r = 0; r < lists[x].size(); ++r) items[r * kXcd + x] = lists[x][r];
+  if (p->d_items) (void)hipFree(p->d_items);
+  p->d_items = nullptr;
+  p->nitems = (int)items.size();
+  if (!items.empty()) {
+    if (hipMalloc(&p->d_items, sizeof(int4) * items.size()) != hipSuccess)
+      return set_error(ctx, APH_E_NOMEM, "plan item list allocation failed");
+    APH_HIP_CHECK(ctx, hipMemcpy(p->d_items, items.data(), sizeof(int4) * items.size(),
+                                 hipMemcpyHostToDevice));
+  }
+  p->lds = (ndim >= 2) ? maxcells * bs->nvar * sizeof(double) : 0;
+  if (p->lds > 64 * 1024)
+    return set_error(ctx, APH_E_INVALID, "tile too large for LDS (%zu bytes)", p->lds);
+  // resolved tile configs
+  p->tiles.assign(tl.begin(), tl.end());
+  // persistent grid: resident workgroups, a multiple of the XCD count
+  KernelFn fn = pick_kernel(p->recon, p->rsolver, bs->nvar, ndim);
+  int dev = ctx->device, ncu = 0, occ = 0;
+  APH_HIP_CHECK(ctx, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  APH_HIP_CHECK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                         &occ, reinterpret_cast<const void *>(fn), 64, p->lds));
+  occ = std::max(occ, 1);
+  int grid = ncu * occ;
+  grid = std::max(kXcd, (grid / kXcd) * kXcd);
+  p->grid = std::min<int>(grid, std::max(kXcd, p->nitems));
+  p->dirty = false;
+  return APH_OK;
+}
+
+FusedArgs make_args(aph_plan *p, int stage, double *u_in, const double *u_n, double *u_out,
+                    const double *dt_dev, double cfl) {
+  FusedArgs a{};
+  a.blocks = p->bs->d;
+  a.items = p->d_items;
+  a.nitems = p->nitems;
+  a.ndim = p->bs->ndim;
+  for (size_t c = 0; c < p->tiles.size(); ++c) {
+    const aph_tile &t = p->tiles[c];
+    TileCfg &k = a.cfg[c];
+    k.t[0] = t.tx;
+    k.t[1] = t.ty;
+    k.t[2] = t.tz;
+    for (int d = 0; d < 3; ++d) {
+      const int A = (d == 0) ? 1 : 0, B = (d == 2) ? 1 : 2;
+      k.s[d] = (d < a.ndim) ? seg_len(k.t[d], k.t[A] * k.t[B]) : 1;
+    }
+  }
+  a.eos = p->eos;
+  a.g[0] = p->g[0];
+  a.g[1] = p->g[1];
+  a.g[2] = p->g[2];
+  a.grav = p->grav;
+  a.u_in = u_in;
+  a.u_n = u_n;
+  a.u_out = u_out;
+  a.dt = dt_dev;
+  a.gam0 = stage == 1 ? 0.0 : 0.5;
+  a.gam1 = stage == 1 ? 1.0 : 0.5;
+  a.beta = stage == 1 ? 1.0 : 0.5;
+  a.dt_min = p->d_est;
+  a.cfl = cfl;
+  a.stage = stage;
+  return a;
+}
+
+int launch_stage(aph_context *ctx, aph_plan *p, int stage, double *u_in, const double *u_n,
+                 double *u_out, const double *dt_dev, double cfl) {
+  if (p->dirty) {
+    int rc = build_items(ctx, p);
+    if (rc) return rc;
+  }
+  if (p->nitems == 0) return APH_OK;
+  KernelFn fn = pick_kernel(p->recon, p->rsolver, p->bs->nvar, p->bs->ndim);
+  FusedArgs a = make_args(p, stage, u_in, u_n, u_out, dt_dev, cfl);
+  hipLaunchKernelGGL(fn, dim3(p->grid), dim3(64), p->lds, ctx->stream, a);
+  APH_LAUNCH_CHECK(ctx);
+  return APH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int aph_plan_create(aph_context *ctx, const aph_blockset *bs, int32_t recon, int32_t rsolver,
+                    const aph_eos *eos, const double g[3], aph_plan **out) {
+  if (!ctx) return APH_E_INVALID;
+  if (!bs || !eos || !out) return set_error(ctx, APH_E_INVALID, "aph_plan_create: null");
+  if (!pick_kernel(recon, rsolver, bs->nvar, bs->ndim))
+    return set_error(ctx, APH_E_INVALID,
+                     "fused form: unsupported (reconstruction=%d, riemann=%d, nvar=%d)", recon,
+                     rsolver, bs->nvar);
+  const int need = recon == APH_RECON_PPM ? 3 : 2;
+  for (int b = 0; b < bs->nblocks; ++b)
+    for (int d = 0; d < bs->ndim; ++d)
+      if (bs->h[b].nx[d] < need)
+        return set_error(ctx, APH_E_INVALID,
+                         "fused form reads ghosts from neighbours: block %d axis %d has %d "
+                         "cells, stencil needs %d", b, d, bs->h[b].nx[d], need);
+  aph_plan *p = new (std::nothrow) aph_plan();
+  if (!p) return set_error(ctx, APH_E_NOMEM, "host allocation failed");
+  p->bs = bs;
+  p->recon = recon;
+  p->rsolver = rsolver;
+  p->eos = *eos;
+  if (g) {
+    p->g[0] = g[0];
+    p->g[1] = g[1];
+    p->g[2] = g[2];
+    p->grav = (g[0] != 0.0 || g[1] != 0.0 || g[2] != 0.0);
+  }
+  if (hipMalloc(&p->d_est, sizeof(double)) != hipSuccess) {
+    delete p;
+    return set_error(ctx, APH_E_NOMEM, "plan scratch allocation failed");
+  }
+  int rc = build_items(ctx, p);
+  if (rc) {
+    aph_plan_destroy(ctx, p);
+    return rc;
+  }
+  *out = p;
+  return APH_OK;
+}
+
+int aph_plan_destroy(aph_context *ctx, aph_plan *p) {
+  (void)ctx;
+  if (!p) return APH_OK;
+  if (p->exec) (void)hipGraphExecDestroy(p->exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  if (p->d_items) (void)hipFree(p->d_items);
+  if (p->d_est) (void)hipFree(p->d_est);
+  delete p;
+  return APH_OK;
+}
+
+int aph_plan_set_tile(aph_context *ctx, aph_plan *p, const aph_tile *t) {
+  if (!ctx) return APH_E_INVALID;
+  if (!p || !t) return set_error(ctx, APH_E_INVALID, "aph_plan_set_tile: null");
+  if (t->tx < 1 || t->ty < 1 || t->tz < 1)
+    return set_error(ctx, APH_E_INVALID, "tile extents must be positive");
+  std::vector<aph_tile> keep = p->tiles;
+  bool found = false;
+  for (auto &u : p->tiles)
+    if (u.n == t->n) {
+      u = *t;
+      found = true;
+    }
+  if (!found) p->tiles.push_back(*t);
+  int rc = build_items(ctx, p);
+  if (rc) {
+    p->tiles = keep;
+    build_items(ctx, p);
+    return rc;
+  }
+  if (p->exec) {
+    (void)hipGraphExecDestroy(p->exec);
+    p->exec = nullptr;
+  }
+  if (p->graph) {
+    (void)hipGraphDestroy(p->graph);
+    p->graph = nullptr;
+  }
+  return APH_OK;
+}
+
+int aph_fused_stage(aph_context *ctx, aph_plan *p, int32_t stage, double *u_in,
+                    const double *u_n, double *u_out, const double *dt_dev) {
+  if (!ctx) return APH_E_INVALID;
+  if (!p || !u_in || !u_out || !dt_dev || (stage == 2 && !u_n) || (stage != 1 && stage != 2))
+    return set_error(ctx, APH_E_INVALID, "aph_fused_stage: bad arguments");
+  if (u_in == u_out) return set_error(ctx, APH_E_INVALID, "u_out must not alias u_in");
+  return launch_stage(ctx, p, stage, u_in, u_n, u_out, dt_dev, 0.0);
+}
+
+int aph_plan_step(aph_context *ctx, aph_plan *p, double *u0, double *u1, double *dt_dev,
+                  double *time_dev, double cfl, double tlim, int32_t fixed_dt) {
+  if (!ctx) return APH_E_INVALID;
+  if (!p || !u0 || !u1 || !dt_dev || !time_dev || u0 == u1)
+    return set_error(ctx, APH_E_INVALID, "aph_plan_step: bad arguments");
+  int rc = launch_stage(ctx, p, 1, u0, nullptr, u1, dt_dev, cfl);
+  if (rc) return rc;
+  rc = launch_stage(ctx, p, 2, u1, u0, u0, dt_dev, cfl);
+  if (rc) return rc;
+  k_dt_finalize<<<1, 1, 0, ctx->stream>>>(dt_dev, time_dev, p->d_est, tlim, fixed_dt);
+  APH_LAUNCH_CHECK(ctx);
+  return APH_OK;
+}
+
+int aph_plan_graph_capture(aph_context *ctx, aph_plan *p, int32_t nsteps, double *u0,
+                           double *u1, double *dt_dev, double *time_dev, double cfl,
+                           double tlim, int32_t fixed_dt) {
+  if (!ctx) return APH_E_INVALID;
+  if (!p || nsteps < 1) return set_error(ctx, APH_E_INVALID, "aph_plan_graph_capture: bad args");
+  if (!ctx->stream)
+    return set_error(ctx, APH_E_STATE, "graph capture needs a non-default stream");
+  if (p->dirty) {
+    int rc = build_items(ctx, p);
+    if (rc) return rc;
+  }
+  if (p->exec) {
+    (void)hipGraphExecDestroy(p->exec);
+    p->exec = nullptr;
+  }
+  if (p->graph) {
+    (void)hipGraphDestroy(p->graph);
+    p->graph = nullptr;
+  }
+  APH_HIP_CHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  int rc = APH_OK;
+  for (int s = 0; s < nsteps && rc == APH_OK; ++s)
+    rc = aph_plan_step(ctx, p, u0, u1, dt_dev, time_dev, cfl, tlim, fixed_dt);
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+  if (rc != APH_OK) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess)
+    return set_error(ctx, APH_E_HIP, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
+  e = hipGraphInstantiate(&p->exec, graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    return set_error(ctx, APH_E_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+  }
+  p->graph = graph;
+  return APH_OK;
+}
+
+int aph_plan_graph_launch(aph_context *ctx, aph_plan *p) {
+  if (!ctx) return APH_E_INVALID;
+  if (!p) return set_error(ctx, APH_E_INVALID, "null plan");
+  if (!p->exec) return set_error(ctx, APH_E_STATE, "no captured graph in this plan");
+  APH_HIP_CHECK(ctx, hipGraphLaunch(p->exec, ctx->stream));
+  return APH_OK;
+}
+
+int aph_plan_estimate_timestep(aph_context *ctx, aph_plan *p, const double *cons, double cfl,
+                               double *dt_out) {
+  if (!ctx) return APH_E_INVALID;
+  if (!p || !cons || !dt_out)
+    return set_error(ctx, APH_E_INVALID, "aph_plan_estimate_timestep: null");
+  const aph_blockset *bs = p->bs;
+  k_fill<<<1, 1, 0, ctx->stream>>>(dt_out, 1.79769313486231570815e308);
+  if (bs->nblocks > 0) {
+    int64_t x = std::min<int64_t>(64, (bs->max_interior + 255) / 256);
+    dim3 gr((unsigned)std::max<int64_t>(1, x), (unsigned)bs->nblocks);
+#define APH_PDT(NVv, ND) k_plan_dt<NVv, ND><<<gr, 256, 0, ctx->stream>>>(bs->d, p->eos, cons, cfl, dt_out)
+    if (bs->nvar == 5) {
+      if (bs->ndim == 1) APH_PDT(5, 1); else if (bs->ndim == 2) APH_PDT(5, 2); else APH_PDT(5, 3);
+    } else {
+      if (bs->ndim == 1) APH_PDT(6, 1); else if (bs->ndim == 2) APH_PDT(6, 2); else APH_PDT(6, 3);
+    }
+#undef APH_PDT
+  }
+  APH_LAUNCH_CHECK(ctx);
+  return APH_OK;
+}
+
+}  // extern "C"

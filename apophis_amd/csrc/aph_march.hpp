@@ -1,0 +1,68 @@
+// aph_march.hpp -- one lane sweeps one pencil: sliding-window reconstruction and
+// Riemann solve, shared by the plain flux kernels and the fused stage kernel.
+//
+// The reference reconstructs cells [s-1, e+1] of a pencil into team scratch and then
+// solves every face [s, e+1] (hydro.cpp:596-603, plm/ppm_simple.hpp).  Here one lane
+// walks the pencil holding a window of W = 2*GR+1 cells of all fields in registers:
+// each step loads one new cell, reconstructs the window's centre cell (ql at its upper
+// face, qr at its lower face) and solves the face below it with the ql kept from the
+// previous step.  Every cell is loaded once and reconstructed once per sweep, the
+// face states never leave registers, and no team barrier is needed.
+#pragma once
+
+#include "aph_device.hpp"
+
+namespace aph {
+
+// Sweep faces [f_lo, f_hi] along one axis.  load(c, dst[NF]) must write the NV
+// primitive fields followed by (gamma_c, gamma_e) of cell c (pencil coordinate).
+// emit(f, flux[NV], ql[NF], qr[NF]) receives the face flux in conserved-variable
+// order (IDN, IM1, IM2, IM3, IEN, scalars...) for face f (between cells f-1 and f).
+// IVX = 1 + sweep axis (the reference's ivx, hydro_hllc.hpp:157-158).
+template <int RECON, int RS, int NV, int IVX, class Load, class Emit>
+__device__ __forceinline__ void march(int f_lo, int f_hi, Load &&load, Emit &&emit) {
+  constexpr int GR = ReconTraits<RECON>::GR;
+  constexpr int W = ReconTraits<RECON>::W;
+  constexpr int NF = NV + 2;
+  constexpr int IVY = 1 + IVX % 3;
+  constexpr int IVZ = 1 + (IVX + 1) % 3;
+  double win[W][NF];
+  double qlp[NF];
+#pragma unroll
+  for (int s = 0; s < W - 1; ++s) load(f_lo - 1 - GR + s, win[s + 1]);
+  for (int m = f_lo - 1; m <= f_hi; ++m) {
+#pragma unroll
+    for (int s = 0; s < W - 1; ++s)
+#pragma unroll
+      for (int f = 0; f < NF; ++f) win[s][f] = win[s + 1][f];
+    load(m + GR, win[W - 1]);
+    double ql[NF], qr[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      double col[W];
+#pragma unroll
+      for (int s = 0; s < W; ++s) col[s] = win[s][f];
+      recon1<RECON>(col, ql[f], qr[f]);
+    }
+    if (m >= f_lo) {
+      const double wl[5] = {qlp[0], qlp[IVX], qlp[IVY], qlp[IVZ], qlp[4]};
+      const double wr[5] = {qr[0], qr[IVX], qr[IVY], qr[IVZ], qr[4]};
+      double flx[5];
+      riemann<RS>(wl, wr, &qlp[NV], &qr[NV], flx);
+      double fo[NV];
+      fo[0] = flx[0];
+      fo[IVX] = flx[1];
+      fo[IVY] = flx[2];
+      fo[IVZ] = flx[3];
+      fo[4] = flx[4];
+#pragma unroll
+      for (int n = 5; n < NV; ++n)  // passive scalars: upwind by the mass flux (:606-615)
+        fo[n] = (fo[0] >= 0.0) ? fo[0] * qlp[n] : fo[0] * qr[n];
+      emit(m, fo, qlp, qr);
+    }
+#pragma unroll
+    for (int f = 0; f < NF; ++f) qlp[f] = ql[f];
+  }
+}
+
+}  // namespace aph

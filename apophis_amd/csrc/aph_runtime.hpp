@@ -1,0 +1,56 @@
+// aph_runtime.hpp -- host-side objects behind the C ABI: context, block set, plan.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/apophis_hydro.h"
+
+struct aph_context {
+  hipStream_t stream = nullptr;
+  int device = 0;
+  std::string err;
+  double *d_scalar = nullptr;  // small device scratch for reductions (aph_global_sum)
+  int64_t d_scalar_n = 0;
+};
+
+struct aph_blockset {
+  std::vector<aph_block> h;  // host copy of the table
+  aph_block *d = nullptr;    // device copy
+  int32_t nblocks = 0, ndim = 0, nscalars = 0, nvar = 0;
+  int64_t ncells = 0;
+  int64_t max_ncell = 0;       // max entire cells per block
+  int64_t max_interior = 0;    // max interior cells per block
+  int64_t max_pencils[3] = {0, 0, 0};  // reference loop pencils per block per axis
+  int32_t max_ext[3] = {0, 0, 0};
+};
+
+namespace aph {
+
+int set_error(aph_context *ctx, int code, const char *fmt, ...);
+
+#define APH_HIP_CHECK(ctx, call)                                                      \
+  do {                                                                                \
+    hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return aph::set_error((ctx), APH_E_HIP, "%s failed: %s", #call,                 \
+                            hipGetErrorString(e_));                                   \
+  } while (0)
+
+#define APH_LAUNCH_CHECK(ctx)                                                         \
+  do {                                                                                \
+    hipError_t e_ = hipGetLastError();                                                \
+    if (e_ != hipSuccess)                                                             \
+      return aph::set_error((ctx), APH_E_HIP, "kernel launch failed: %s",             \
+                            hipGetErrorString(e_));                                   \
+  } while (0)
+
+// Resolve the (first_block, nblocks) sub-range convention of the plain entry points.
+int block_range(aph_context *ctx, const aph_blockset *bs, int32_t first, int32_t n,
+                int32_t *lo, int32_t *cnt);
+
+}  // namespace aph

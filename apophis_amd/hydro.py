@@ -1,0 +1,428 @@
+"""Host-side mirror of the reference's Hydro package and stage driver.
+
+This is the Python face of the reference interface around the flux path:
+
+* enums with the reference's values (``src/main.hpp:32-53``);
+* ``initialize_hydro(pin)`` -- option parsing, ghost check and the flux-function
+  registry keyed by (fluid, reconstruction, riemann) (``src/hydro/hydro.cpp:33-349``,
+  ``src/hydro/hydro.hpp:26-37``); invalid options raise :class:`ApophisError`
+  where the reference calls ``PARTHENON_FAIL``;
+* task functions with the reference's shapes, each returning ``TaskStatus`` or a
+  value like the reference: ``calculate_fluxes`` (``hydro.cpp:535-744``),
+  ``cons_to_prim`` (``:422-533``), ``estimate_timestep`` (``:351-420``),
+  ``update_with_flux_divergence`` (upstream Parthenon), ``apply_gravity``
+  (``src/gravity/gravity.cpp:374-410``), ``fill_ghosts`` (boundary exchange + BCs),
+  ``global_quant_hst`` (``hydro.cpp:746-771``);
+* ``ApophisDriver`` -- the RK2 stage sequence of ``ApophisDriver::MakeTaskCollection``
+  (``src/apophis_driver.cpp:27-130``);
+* ``FusedDriver`` -- the batched form: one fused launch per stage, device-resident
+  dt, optionally replayed from a captured hipGraph.
+
+All compute goes through the C ABI (``_lib``); torch only provides device memory and
+the stream.  There is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import enum
+
+import numpy as np
+import torch
+
+from . import _lib
+from .mesh import BlockSet, EOS_DTYPE, make_eos
+
+
+class Fluid(enum.IntEnum):
+    undefined = 0
+    euler = 1
+
+
+class Reconstruction(enum.IntEnum):
+    undefined = 0
+    dc = 1
+    plm = 2
+    ppm = 3
+
+
+class RiemannSolver(enum.IntEnum):
+    undefined = 0
+    none = 1
+    hlle = 2
+    hllc = 3
+    lhllc = 4
+
+
+class Integrator(enum.IntEnum):
+    undefined = 0
+    rk1 = 1
+    rk2 = 2
+    rk3 = 3
+    vl2 = 4
+
+
+class TaskStatus(enum.Enum):
+    complete = 0
+    incomplete = 1
+    fail = 2
+
+
+class ApophisError(RuntimeError):
+    """Raised where the reference aborts with PARTHENON_FAIL."""
+
+
+# upstream "rk2" low-storage coefficients
+RK2 = {"gam0": (0.0, 0.5), "gam1": (1.0, 0.5), "beta": (1.0, 0.5), "nstages": 2}
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    assert t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()
+    return ctypes.c_void_p(t.data_ptr())
+
+
+# ------------------------------------------------------------------ package
+class StateDescriptor:
+    def __init__(self, name):
+        self.name = name
+        self.params = {}
+
+    def add_param(self, key, value):
+        self.params[key] = value
+
+    def param(self, key):
+        return self.params[key]
+
+
+def _get(pin, block, key, default=None, required=False):
+    blk = pin.get(block, {})
+    if key in blk:
+        return blk[key]
+    if required:
+        raise ApophisError(f"[Apophis]: missing required parameter {block}/{key}")
+    return default
+
+
+FLUX_FUNCTIONS = {}
+
+
+def add_flux_fun(fluid, recon, rsolver):
+    """Register the task function for one (fluid, recon, riemann) key (hydro.hpp:33-37)."""
+    FLUX_FUNCTIONS[(fluid, recon, rsolver)] = _make_flux_task(recon, rsolver)
+
+
+def _make_flux_task(recon, rsolver):
+    def calculate_fluxes(md):
+        md.ctx.check(_lib.lib().aph_calculate_fluxes(
+            md.ctx.h, md.dbs, md.first, md.count, int(recon), int(rsolver), _ptr(md.prim),
+            _ptr(md.gamma), _ptr(md.flux[0]), _ptr(md.flux[1]), _ptr(md.flux[2])))
+        return TaskStatus.complete
+
+    calculate_fluxes.__name__ = f"CalculateFluxes<euler,{recon.name},{rsolver.name}>"
+    calculate_fluxes.key = (Fluid.euler, recon, rsolver)
+    return calculate_fluxes
+
+
+# hydro.cpp:103-112 registers these ten combinations; the ones the flux kernels do not
+# implement (LHLLC, none) are registered as functions that fail loudly when called.
+for _r in (Reconstruction.dc, Reconstruction.plm, Reconstruction.ppm):
+    for _s in (RiemannSolver.hllc, RiemannSolver.hlle, RiemannSolver.lhllc):
+        add_flux_fun(Fluid.euler, _r, _s)
+add_flux_fun(Fluid.euler, Reconstruction.dc, RiemannSolver.none)
+
+
+def initialize_hydro(pin):
+    """Mirror of Apophis::InitializeHydro (hydro.cpp:33-349) for the flux path.
+
+    pin: nested dict {block: {key: value}} using the reference's input-deck names."""
+    pkg = StateDescriptor("Hydro")
+    pkg.add_param("cfl", float(_get(pin, "parthenon/time", "cfl", 0.3)))
+    fluid_str = _get(pin, "hydro", "fluid", "euler")
+    if fluid_str != "euler":
+        raise ApophisError("[Apophis]: Fluid not recognized. Exiting.")
+    pkg.add_param("fluid", Fluid.euler)
+    pkg.add_param("nhydro", 5)
+    recon_str = _get(pin, "hydro", "reconstruction", required=True)
+    need = {"dc": 1, "plm": 2, "ppm": 3}
+    if recon_str not in need:
+        raise ApophisError("[Apophis]: Reconstruction not recognized. Exiting.")
+    recon = Reconstruction[recon_str]
+    nghost = int(_get(pin, "parthenon/mesh", "nghost", required=True))
+    if nghost < need[recon_str]:
+        raise ApophisError("[Apophis]: Not enough ghost zones for reconstruction. Exiting.")
+    pkg.add_param("reconstruction", recon)
+    riemann_str = _get(pin, "hydro", "riemann", required=True)
+    if riemann_str == "none" and recon_str != "dc":
+        raise ApophisError(
+            "[Apophis]: 'none' Riemann solver only works with dc reconstruction. Exiting.")
+    if riemann_str not in ("hllc", "hlle", "lhllc", "none"):
+        raise ApophisError("[Apophis]: Riemann solver not recognized. Exiting.")
+    rsolver = RiemannSolver[riemann_str]
+    pkg.add_param("riemann_solver", rsolver)
+    flux_other = FLUX_FUNCTIONS[(Fluid.euler, recon, rsolver)]
+    integ_str = _get(pin, "parthenon/time", "integrator", required=True)
+    if integ_str not in ("rk1", "rk2", "rk3", "vl2"):
+        raise ApophisError("[Apophis]: Integrator not recognized. Exiting.")
+    flux_first = flux_other
+    if integ_str == "vl2":
+        flux_first = FLUX_FUNCTIONS[(Fluid.euler, Reconstruction.dc, rsolver)]
+    pkg.add_param("integrator", Integrator[integ_str])
+    pkg.add_param("flux_first_stage", flux_first)
+    pkg.add_param("flux_other_stage", flux_other)
+    nscalars = int(_get(pin, "hydro", "nscalars", 0))
+    pkg.add_param("nscalars", nscalars)
+    eos_type = _get(pin, "eos", "type", "ideal")
+    if eos_type != "ideal":
+        raise ApophisError("[Apophis]: EOS not recognized. Exiting.")
+    floor = float(np.sqrt(1024.0 * np.finfo(np.float32).tiny))
+    pkg.add_param("eos", make_eos(float(_get(pin, "eos", "gm1", 0.6666667)),
+                                  float(_get(pin, "eos", "cv", 1.5)),
+                                  float(_get(pin, "hydro", "dfloor", floor)),
+                                  float(_get(pin, "hydro", "pfloor", floor))))
+    return pkg
+
+
+# ------------------------------------------------------------------ mesh data
+class DeviceBlockSet:
+    """A BlockSet uploaded through aph_blockset_create."""
+
+    def __init__(self, ctx, bs: BlockSet):
+        self.ctx = ctx
+        self.bs = bs
+        self.h = ctypes.c_void_p()
+        ctx.check(_lib.lib().aph_blockset_create(
+            ctx.h, bs.blocks.ctypes.data_as(ctypes.c_void_p), bs.nblocks, bs.ndim,
+            bs.nscalars, ctypes.byref(self.h)))
+
+    def __del__(self):
+        try:
+            if self.h:
+                _lib.lib().aph_blockset_destroy(self.ctx.h, self.h)
+        except Exception:
+            pass
+
+
+class MeshData:
+    """Device state of one partition: cons (u0), prim, gamma and face fluxes.
+
+    Field tensors use the pack layout of include/apophis_hydro.h."""
+
+    def __init__(self, ctx, bs: BlockSet, cons=None, device="cuda", first=0, count=-1,
+                 dbs=None):
+        self.ctx = ctx
+        self.bs = bs
+        self.device_bs = dbs or DeviceBlockSet(ctx, bs)
+        self.dbs = self.device_bs.h
+        self.first, self.count = first, count
+        n = bs.field_size(bs.nvar)
+        self.cons = torch.zeros(n, dtype=torch.float64, device=device)
+        if cons is not None:
+            self.cons.copy_(torch.from_numpy(np.ascontiguousarray(cons)))
+        self.prim = torch.zeros_like(self.cons)
+        self.gamma = torch.zeros(bs.field_size(2), dtype=torch.float64, device=device)
+        self.flux = [torch.zeros_like(self.cons) if d < bs.ndim else None for d in range(3)]
+
+    def pack(self, first, count):
+        """A view of a contiguous range of blocks (the reference's pack_size)."""
+        md = MeshData.__new__(MeshData)
+        md.__dict__.update(self.__dict__)
+        md.first, md.count = first, count
+        return md
+
+
+# ------------------------------------------------------------------ task functions
+def cons_to_prim(md, eos):
+    md.ctx.check(_lib.lib().aph_cons_to_prim(md.ctx.h, md.dbs, md.first, md.count,
+                                             eos.ctypes.data_as(ctypes.c_void_p),
+                                             _ptr(md.cons), _ptr(md.prim), _ptr(md.gamma)))
+    return TaskStatus.complete
+
+
+def update_with_flux_divergence(md0, u1, gam0, gam1, beta_dt):
+    md0.ctx.check(_lib.lib().aph_update_with_flux_divergence(
+        md0.ctx.h, md0.dbs, md0.first, md0.count, _ptr(md0.cons), _ptr(u1),
+        _ptr(md0.flux[0]), _ptr(md0.flux[1]), _ptr(md0.flux[2]), gam0, gam1, beta_dt))
+    return TaskStatus.complete
+
+
+def apply_gravity(md, g, dt):
+    garr = (ctypes.c_double * 3)(*g)
+    md.ctx.check(_lib.lib().aph_apply_gravity(md.ctx.h, md.dbs, md.first, md.count,
+                                              _ptr(md.cons), _ptr(md.prim), garr, dt))
+    return TaskStatus.complete
+
+
+def fill_ghosts(md):
+    md.ctx.check(_lib.lib().aph_fill_ghosts(md.ctx.h, md.dbs, md.first, md.count,
+                                            _ptr(md.cons)))
+    return TaskStatus.complete
+
+
+def estimate_timestep(md, eos, cfl, out=None):
+    """Returns cfl * min(dx/(|v|+c)) as a 1-element device tensor."""
+    out = out if out is not None else torch.empty(1, dtype=torch.float64, device=md.cons.device)
+    md.ctx.check(_lib.lib().aph_estimate_timestep(
+        md.ctx.h, md.dbs, md.first, md.count, eos.ctypes.data_as(ctypes.c_void_p),
+        _ptr(md.prim), _ptr(md.cons), cfl, _ptr(out)))
+    return out
+
+
+def global_quant_hst(md, var, out=None):
+    out = out if out is not None else torch.empty(1, dtype=torch.float64, device=md.cons.device)
+    md.ctx.check(_lib.lib().aph_global_sum(md.ctx.h, md.dbs, _ptr(md.cons), var, _ptr(out)))
+    return out
+
+
+# ------------------------------------------------------------------ drivers
+class ApophisDriver:
+    """RK2 stage sequence of ApophisDriver::MakeTaskCollection, plain form.
+
+    pack_size: blocks per partition (None = all blocks in one pack, the default
+    Parthenon partitioning); each task is issued once per partition."""
+
+    def __init__(self, pkg, md, g=(0.0, 0.0, 0.0), pack_size=None):
+        self.pkg = pkg
+        self.md = md
+        self.eos = pkg.param("eos")
+        self.cfl = pkg.param("cfl")
+        self.g = tuple(float(v) for v in g)
+        self.u1 = torch.zeros_like(md.cons)
+        nb = md.bs.nblocks
+        ps = pack_size or nb
+        self.parts = [md.pack(s, min(ps, nb - s)) for s in range(0, nb, ps)]
+        self.dt_dev = torch.empty(1, dtype=torch.float64, device=md.cons.device)
+        if pkg.param("integrator") != Integrator.rk2:
+            raise ApophisError("only the rk2 integrator is implemented by this driver")
+
+    def initialize(self):
+        """Ghost fill + FillDerived after the problem generator (PostInitialization)."""
+        for p in self.parts:
+            fill_ghosts(p)
+        for p in self.parts:
+            cons_to_prim(p, self.eos)
+        return self.new_dt()
+
+    def new_dt(self):
+        est = []
+        for p in self.parts:
+            est.append(float(estimate_timestep(p, self.eos, self.cfl, self.dt_dev).item()))
+        return min(est)
+
+    def make_task_collection(self, stage, dt):
+        """Run the tasks of one stage (apophis_driver.cpp:27-130) in dependency order."""
+        s = stage - 1
+        if stage == 1:  # :44-55
+            self.u1.copy_(self.md.cons)
+        flux_fun = self.pkg.param("flux_first_stage" if stage == 1 else "flux_other_stage")
+        grav = any(v != 0.0 for v in self.g)
+        for p in self.parts:
+            flux_fun(p)                                                      # :70-72
+            update_with_flux_divergence(p, self.u1, RK2["gam0"][s], RK2["gam1"][s],
+                                        RK2["beta"][s] * dt)                 # :79-82
+            if grav:
+                apply_gravity(p, self.g, RK2["beta"][s] * dt)                # :84-85
+        for p in self.parts:
+            fill_ghosts(p)                                                   # :94, :103
+        for p in self.parts:
+            cons_to_prim(p, self.eos)                                        # :110-111
+        if stage == RK2["nstages"]:
+            return self.new_dt()                                             # :113-116
+        return None
+
+    def step(self, dt):
+        est = None
+        for stage in (1, 2):
+            est = self.make_task_collection(stage, dt)
+        return est
+
+    def evolve(self, tlim=None, nlim=None, fixed_dt=None):
+        """EvolutionDriver loop with the upstream dt policy (<= 2x growth, clip to tlim)."""
+        t, n = 0.0, 0
+        est = self.initialize()
+        dt = np.inf
+        while (nlim is None or n < nlim) and (tlim is None or t < tlim * (1 - 1e-14)):
+            if fixed_dt is not None:
+                dt = fixed_dt
+            else:
+                dt = min(2.0 * dt, est)
+                if tlim is not None and t < tlim and tlim - t < dt:
+                    dt = tlim - t
+            est = self.step(dt)
+            t += dt
+            n += 1
+        return t, n
+
+
+class FusedDriver:
+    """Batched form: one fused launch per RK stage, device-resident dt.
+
+    u0 is the state (raw conserved variables; floors are applied on the fly, once per
+    stage); u1 holds the stage-1 result.  Ghost cells are never read."""
+
+    def __init__(self, ctx, bs: BlockSet, cons, recon, riemann, eos, cfl, g=(0.0, 0.0, 0.0),
+                 tlim=np.inf, device="cuda", dbs=None):
+        self.ctx = ctx
+        self.bs = bs
+        self.device_bs = dbs or DeviceBlockSet(ctx, bs)
+        self.eos = eos
+        self.cfl = float(cfl)
+        self.tlim = float(tlim)
+        self.u0 = torch.as_tensor(np.ascontiguousarray(cons), dtype=torch.float64).to(device)
+        self.u1 = self.u0.clone()
+        self.dt = torch.zeros(1, dtype=torch.float64, device=device)
+        self.time = torch.zeros(1, dtype=torch.float64, device=device)
+        self.plan = ctypes.c_void_p()
+        garr = (ctypes.c_double * 3)(*[float(v) for v in g])
+        r = recon if isinstance(recon, int) else int(Reconstruction[recon])
+        s = riemann if isinstance(riemann, int) else int(RiemannSolver[riemann])
+        ctx.check(_lib.lib().aph_plan_create(ctx.h, self.device_bs.h, r, s,
+                                             eos.ctypes.data_as(ctypes.c_void_p), garr,
+                                             ctypes.byref(self.plan)))
+        self.graph_steps = 0
+
+    def set_tile(self, n, tx, ty, tz):
+        t = _lib.Tile(n, tx, ty, tz)
+        self.ctx.check(_lib.lib().aph_plan_set_tile(self.ctx.h, self.plan, ctypes.byref(t)))
+        self.graph_steps = 0
+
+    def initial_dt(self):
+        """dt for the first step: the estimate of the initial state, clipped to tlim."""
+        self.ctx.check(_lib.lib().aph_plan_estimate_timestep(self.ctx.h, self.plan,
+                                                             _ptr(self.u0), self.cfl,
+                                                             _ptr(self.dt)))
+        if np.isfinite(self.tlim):
+            self.dt.clamp_(max=self.tlim)
+        return self.dt
+
+    def set_dt(self, dt):
+        self.dt.fill_(float(dt))
+
+    def stage(self, stage, u_in, u_n, u_out):
+        self.ctx.check(_lib.lib().aph_fused_stage(self.ctx.h, self.plan, stage, _ptr(u_in),
+                                                  _ptr(u_n), _ptr(u_out), _ptr(self.dt)))
+
+    def step(self, fixed_dt=False):
+        self.ctx.check(_lib.lib().aph_plan_step(self.ctx.h, self.plan, _ptr(self.u0),
+                                                _ptr(self.u1), _ptr(self.dt), _ptr(self.time),
+                                                self.cfl, self.tlim, int(bool(fixed_dt))))
+
+    def capture(self, nsteps, fixed_dt=False):
+        self.ctx.check(_lib.lib().aph_plan_graph_capture(
+            self.ctx.h, self.plan, int(nsteps), _ptr(self.u0), _ptr(self.u1), _ptr(self.dt),
+            _ptr(self.time), self.cfl, self.tlim, int(bool(fixed_dt))))
+        self.graph_steps = int(nsteps)
+
+    def replay(self):
+        self.ctx.check(_lib.lib().aph_plan_graph_launch(self.ctx.h, self.plan))
+
+    def close(self):
+        if self.plan:
+            _lib.lib().aph_plan_destroy(self.ctx.h, self.plan)
+            self.plan = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,286 @@
+"""Seeded problem generators for the hydro flux path's workloads.
+
+Each generator returns a :class:`Problem`: the block set, the conserved state in
+pack layout (interior filled; ghosts are filled by the caller's first boundary
+exchange), and the run parameters.  Random fields come from a counter-based
+SplitMix64 stream keyed by (seed, counter), so the same inputs are produced on any
+host and in any order.
+
+Formulas follow the reference's problem generators where one exists:
+``src/pgen/sod.cpp:11-46`` (shock tube), ``src/pgen/blast.cpp:16-71`` (blast),
+``src/pgen/kh.cpp:236-262`` (iprob 5 shear layer), ``src/pgen/rt.cpp:18-111``
+(Rayleigh-Taylor), ``src/pgen/linear_wave.cpp`` (sound wave).
+"""
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+
+from .mesh import BlockSet, lattice_mesh, make_eos
+
+RECON = {"dc": 1, "plm": 2, "ppm": 3}
+RIEMANN = {"none": 1, "hlle": 2, "hllc": 3, "lhllc": 4}
+NEED_GHOST = {"dc": 1, "plm": 2, "ppm": 3}  # hydro.cpp:56-67
+
+
+@dataclasses.dataclass
+class Problem:
+    name: str
+    bs: BlockSet
+    cons: np.ndarray  # flat pack layout, nvar components
+    eos: np.ndarray  # EOS_DTYPE record
+    recon: str
+    riemann: str
+    cfl: float
+    tlim: float = np.inf
+    nsteps: int = 0
+    g: tuple = (0.0, 0.0, 0.0)
+    meta: dict = dataclasses.field(default_factory=dict)
+
+
+# ---------------------------------------------------------------- RNG
+_M64 = (1 << 64) - 1
+
+
+def splitmix64(x):
+    """SplitMix64 finaliser on a uint64 array (wrapping arithmetic)."""
+    x = np.asarray(x, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = x + np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def uniform(seed, counter):
+    """U[0,1) doubles for (seed, counter) pairs; counter may be an array."""
+    c = np.asarray(counter, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        key = splitmix64(np.uint64(seed & _M64)) ^ c
+    z = splitmix64(key)
+    return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def _set_interior(bs, cons, b, rho, m1, m2, m3, en, scalars=()):
+    v = bs.interior(cons, b, bs.nvar)
+    v[0] = rho
+    v[1] = m1
+    v[2] = m2
+    v[3] = m3
+    v[4] = en
+    for n, s in enumerate(scalars):
+        v[5 + n] = s
+
+
+# ---------------------------------------------------------------- Sod
+def sod(ncells=256, nblock_cells=None, xmin=0.0, xmax=1.0, x_discont=0.5,
+        left=(1.0, 0.0, 1.0), right=(0.125, 0.0, 0.1), gamma=1.4, gm1=0.4,
+        recon="plm", riemann="hlle", cfl=0.4, tlim=0.2, ng=None):
+    """1D shock tube (sod.cpp:11-46); outflow at both ends.
+
+    left/right = (rho, u, p).  The energy uses ``p / (gamma - 1.0)`` like the
+    reference pgen while the EOS uses ``gm1`` (eos/gm1), as in inputs/sod.in."""
+    nbx = ncells // (nblock_cells or ncells)
+    ng = ng if ng is not None else NEED_GHOST[recon]
+    m = lattice_mesh(1, (nbx, 1, 1), (ncells // nbx, 1, 1), ng, (xmin, -0.5, -0.5),
+                     (xmax, 0.5, 0.5), ["outflow"] * 6)
+    bs = BlockSet(1, [m])
+    cons = np.zeros(bs.field_size(bs.nvar))
+    for b in range(bs.nblocks):
+        x, _, _ = bs.cell_centers(b)
+        lft = x < x_discont
+        rl, ul, pl = left
+        rr, ur, pr = right
+        rho = np.where(lft, rl, rr)
+        mom = np.where(lft, rl * ul, rr * ur)
+        en = np.where(lft, 0.5 * rl * ul * ul + pl / (gamma - 1.0),
+                      0.5 * rr * ur * ur + pr / (gamma - 1.0))
+        _set_interior(bs, cons, b, rho, mom, 0.0, 0.0, en)
+    return Problem("sod", bs, cons, make_eos(gm1), recon, riemann, cfl, tlim=tlim)
+
+
+# ---------------------------------------------------------------- blast
+def blast3d(nblk=8, n=32, seed=20240601, recon="ppm", riemann="hllc", cfl=0.3,
+            nsteps=100, ng=3, noise=0.01, p_amb=0.1, p_in=10.0, radius=0.1):
+    """3D blast wave on [-1/2,1/2]^3, nblk^3 blocks of n^3, periodic (blast.cpp:16-71
+    with radius_inner = radius_outer, i.e. no ramp), plus +-noise uniform density noise
+    keyed by the global cell index."""
+    gm1 = 2.0 / 3.0
+    m = lattice_mesh(3, (nblk,) * 3, (n,) * 3, ng, (-0.5,) * 3, (0.5,) * 3,
+                     ["periodic"] * 6)
+    bs = BlockSet(3, [m])
+    cons = np.zeros(bs.field_size(bs.nvar))
+    N = nblk * n
+    for b in range(bs.nblocks):
+        x, y, z = bs.cell_centers(b)
+        r = bs.blocks[b]
+        lx, ly, lz = (int(round((r["xmin"][d] + 0.5) / (n * r["dx"][d]))) for d in range(3))
+        gi = lx * n + np.arange(n)
+        gj = ly * n + np.arange(n)
+        gk = lz * n + np.arange(n)
+        gid = (gk[:, None, None] * N + gj[None, :, None]) * N + gi[None, None, :]
+        u = uniform(seed, gid)
+        rho = 1.0 * (1.0 + noise * (2.0 * u - 1.0))
+        rad = np.sqrt(x * x + y * y + z * z)
+        pres = np.where(rad < radius, p_in, p_amb)
+        _set_interior(bs, cons, b, rho, 0.0, 0.0, 0.0, pres / gm1)
+    return Problem("blast3d", bs, cons, make_eos(gm1), recon, riemann, cfl,
+                   nsteps=nsteps, meta={"seed": seed, "N": N})
+
+
+# ---------------------------------------------------------------- uneven blocks
+def _fourier_field(seed, ctr0, x, y, z, ndim):
+    """Sum of 4 seeded Fourier modes with integer wave vectors on the unit box,
+    normalised to [-1, 1]."""
+    r = uniform(seed, ctr0 + np.arange(32, dtype=np.uint64))
+    amps = r[0:4] + 0.1
+    amps = amps / amps.sum()
+    s = np.zeros_like(x)
+    for mdx in range(4):
+        kv = np.floor(r[4 + 3 * mdx: 7 + 3 * mdx] * 5.0) - 2.0  # integers in [-2, 2]
+        kv[ndim:] = 0.0
+        if not kv.any():
+            kv[0] = 1.0
+        ph = 2.0 * np.pi * r[16 + mdx]
+        s = s + amps[mdx] * np.sin(2.0 * np.pi * (kv[0] * x + kv[1] * y + kv[2] * z) + ph)
+    return s
+
+
+def uneven_blocks(nblocks=4096, sizes=(8, 16, 32), probs=(0.5, 0.35, 0.15), ng=4,
+                  seed=7, recon="ppm", riemann="hllc", cfl=0.3, nsteps=50,
+                  jump_frac=0.3, ndim=3):
+    """Independent cubic blocks of uneven edge (uneven AMR-like block set).
+
+    Builder-defined rules (the reference has no uneven packs): every block spans the
+    unit cube (dx = 1/n) and is periodic onto itself, so it needs no neighbour; the
+    global dt is the minimum over all blocks.  Each block holds a smooth field from 4
+    seeded Fourier modes (rho in [0.5,2], p in [0.1,1], |u| <= 1) and, in jump_frac of
+    the blocks, a planar discontinuity through the block centre with a random normal
+    (density x4, pressure x10 on the far side)."""
+    gm1 = 2.0 / 3.0
+    r = uniform(seed, np.arange(nblocks, dtype=np.uint64))
+    cum = np.cumsum(probs)
+    which = np.searchsorted(cum, r * cum[-1], side="right")
+    edges = np.asarray(sizes)[np.minimum(which, len(sizes) - 1)]
+    meshes = []
+    for b in range(nblocks):
+        n = int(edges[b])
+        nx = (n, n if ndim > 1 else 1, n if ndim > 2 else 1)
+        meshes.append(lattice_mesh(ndim, (1, 1, 1), nx, ng, (0.0, 0.0, 0.0),
+                                   (1.0, 1.0, 1.0), ["periodic"] * 6, mesh_id=b))
+    bs = BlockSet(ndim, meshes)
+    cons = np.zeros(bs.field_size(bs.nvar))
+    for b in range(nblocks):
+        x, y, z = bs.cell_centers(b)
+        base = np.uint64(1_000_000 + 256 * b)
+        rho = 1.25 + 0.75 * _fourier_field(seed, base, x, y, z, ndim)
+        prs = 0.55 + 0.45 * _fourier_field(seed, base + np.uint64(40), x, y, z, ndim)
+        vel = [_fourier_field(seed, base + np.uint64(80 + 40 * d), x, y, z, ndim) / np.sqrt(3.0)
+               for d in range(3)]
+        rj = uniform(seed, base + np.uint64(250 + np.arange(4, dtype=np.uint64)))
+        if rj[0] < jump_frac:
+            nrm = rj[1:4] * 2.0 - 1.0
+            nrm[ndim:] = 0.0
+            if not nrm.any():
+                nrm[0] = 1.0
+            side = ((x - 0.5) * nrm[0] + (y - 0.5) * nrm[1] + (z - 0.5) * nrm[2]) > 0.0
+            rho = np.where(side, rho * 4.0, rho)
+            prs = np.where(side, prs * 10.0, prs)
+        ek = 0.5 * rho * (vel[0] ** 2 + vel[1] ** 2 + vel[2] ** 2)
+        _set_interior(bs, cons, b, rho, rho * vel[0], rho * vel[1], rho * vel[2],
+                      prs / gm1 + ek)
+    return Problem("uneven", bs, cons, make_eos(gm1), recon, riemann, cfl,
+                   nsteps=nsteps, meta={"edges": edges, "seed": seed})
+
+
+# ---------------------------------------------------------------- KH
+def kh2d(nblk=16, n=32, seed=11, amps=None, a=0.01, sigma2=0.01, drat=2.0,
+         vflow=0.5, p0=2.5, recon="plm", riemann="hllc", cfl=0.3, nsteps=200):
+    """2D Kelvin-Helmholtz on [0,1]x[-1/2,1/2] periodic (kh.cpp:236-262 iprob 5 form):
+    w = (tanh((|y|-1/4)/a)+1)/2, rho = w + (1-w)*drat, v_x = vflow*(w - (1-w)),
+    v_y = sum_k a_k sin(2 pi k x) exp(-(|y|-1/4)^2/sigma2), p = p0, and one passive
+    scalar s = 1-w marking the inner layer (conserved rho*s)."""
+    gm1 = 2.0 / 3.0
+    if amps is None:
+        amps = 0.01 * uniform(seed, np.arange(8, dtype=np.uint64))
+    amps = np.asarray(amps, dtype=np.float64)
+    m = lattice_mesh(2, (nblk, nblk, 1), (n, n, 1), 2, (0.0, -0.5, -0.5),
+                     (1.0, 0.5, 0.5), ["periodic"] * 6)
+    bs = BlockSet(2, [m], nscalars=1)
+    cons = np.zeros(bs.field_size(bs.nvar))
+    for b in range(bs.nblocks):
+        x, y, _ = bs.cell_centers(b)
+        w = (np.tanh((np.abs(y) - 0.25) / a) + 1.0) * 0.5
+        rho = w + (1.0 - w) * drat
+        vx = vflow * w - (1.0 - w) * vflow
+        env = np.exp(-((np.abs(y) - 0.25) ** 2) / sigma2)
+        vy = np.zeros_like(x)
+        for k in range(len(amps)):
+            vy = vy + amps[k] * np.sin(2.0 * np.pi * (k + 1) * x) * env
+        en = p0 / gm1 + 0.5 * rho * (vx * vx + vy * vy)
+        _set_interior(bs, cons, b, rho, rho * vx, rho * vy, 0.0, en, [rho * (1.0 - w)])
+    return Problem("kh2d", bs, cons, make_eos(gm1), recon, riemann, cfl, nsteps=nsteps,
+                   meta={"amps": amps})
+
+
+# ---------------------------------------------------------------- RT
+def rt2d(nx=128, ny=384, nblk=(4, 12), seed=5, amps=None, gamma=1.4, g=-0.1,
+         recon="ppm", riemann="hllc", cfl=0.3, nsteps=300, mesh_id=0):
+    """2D Rayleigh-Taylor on [-1/4,1/4]x[-3/4,3/4]: rho=2 above y=0, 1 below,
+    hydrostatic p = 2.5 + g*rho*y, 6-mode interface perturbation of v_y, periodic in
+    x, reflecting in y, constant gravity g in y (rt.cpp:18-111, gravity.cpp:374-410)."""
+    gm1 = gamma - 1.0
+    if amps is None:
+        amps = 0.01 * uniform(seed, np.arange(6, dtype=np.uint64))
+    amps = np.asarray(amps, dtype=np.float64)
+    m = lattice_mesh(2, (nblk[0], nblk[1], 1), (nx // nblk[0], ny // nblk[1], 1),
+                     NEED_GHOST[recon], (-0.25, -0.75, -0.5), (0.25, 0.75, 0.5),
+                     ["periodic", "periodic", "reflect", "reflect", "periodic", "periodic"],
+                     mesh_id=mesh_id)
+    bs = BlockSet(2, [m])
+    cons = np.zeros(bs.field_size(bs.nvar))
+    for b in range(bs.nblocks):
+        x, y, _ = bs.cell_centers(b)
+        rho = np.where(y < 0.0, 1.0, 2.0)
+        p = 2.5 + g * rho * y
+        vy = np.zeros_like(x)
+        for k in range(len(amps)):
+            vy = vy + amps[k] * np.cos(2.0 * np.pi * (k + 1) * x / 0.5) * np.exp(-(y / 0.05) ** 2)
+        _set_interior(bs, cons, b, rho, 0.0, rho * vy, 0.0, p / gm1 + 0.5 * rho * vy * vy)
+    return Problem("rt2d", bs, cons, make_eos(gm1), recon, riemann, cfl, nsteps=nsteps,
+                   g=(0.0, g, 0.0), meta={"amps": amps})
+
+
+# ---------------------------------------------------------------- linear wave
+def linear_wave(ndim=1, res=32, amp=1.0e-6, recon="plm", riemann="hllc", cfl=0.3,
+                gamma=5.0 / 3.0, ncycles=1.0):
+    """Sound wave (wave_flag 0) along x1 on a periodic [0,1] (x [0,1] ...) box of
+    res^ndim cells in one block, following linear_wave.cpp's hydro eigenvector with
+    d0 = 1, p0 = 1/gamma (sound speed 1).  Returns the problem; meta['exact'] holds
+    the initial interior state, which the wave returns to after whole periods."""
+    gm1 = gamma - 1.0
+    nx = (res, res if ndim > 1 else 1, res if ndim > 2 else 1)
+    ng = NEED_GHOST[recon]
+    m = lattice_mesh(ndim, (1, 1, 1), nx, ng, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0),
+                     ["periodic"] * 6)
+    bs = BlockSet(ndim, [m])
+    cons = np.zeros(bs.field_size(bs.nvar))
+    d0, p0 = 1.0, 1.0 / gamma
+    c0 = np.sqrt(gamma * p0 / d0)
+    h0 = (p0 / gm1 + p0) / d0
+    # right eigenvector of the sound wave moving in -x (Roe form, u0 = 0)
+    rem = np.array([1.0, -c0, 0.0, 0.0, h0 - 0.0 * c0])
+    x, _, _ = bs.cell_centers(0)
+    # cell-average of sin(2 pi x) over each cell
+    dx = bs.blocks[0]["dx"][0]
+    sn = (np.cos(2 * np.pi * (x - dx / 2)) - np.cos(2 * np.pi * (x + dx / 2))) / (2 * np.pi * dx)
+    e0 = p0 / gm1
+    st = [d0 + amp * rem[0] * sn, amp * rem[1] * sn, np.zeros_like(x), np.zeros_like(x),
+          e0 + amp * rem[4] * sn]
+    _set_interior(bs, cons, 0, *st)
+    period = 1.0 / c0
+    return Problem("linear_wave", bs, cons, make_eos(gm1), recon, riemann, cfl,
+                   tlim=ncycles * period,
+                   meta={"exact": np.stack(st), "amp": amp, "period": period})

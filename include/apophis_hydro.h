@@ -1,0 +1,200 @@
+/*
+ * apophis_hydro.h -- C ABI of the MI355X (gfx950) hydro flux path.
+ *
+ * This library replaces the per-meshblock hydro flux path of AlexHls/apophis:
+ *   reconstruction      src/recon/plm_simple.hpp, src/recon/ppm_simple.hpp, dc_simple.hpp
+ *   Riemann solvers     src/hydro/rsolvers/hydro_hlle.hpp, hydro_hllc.hpp
+ *   flux task           Apophis::CalculateFluxes<F,R,S>      src/hydro/hydro.cpp:535-744
+ *   derived fields      Apophis::ConsToPrim<EOS_t>           src/hydro/hydro.cpp:422-533
+ *   time step           Apophis::EstimateTimestep<euler>     src/hydro/hydro.cpp:351-420
+ *   history sums        Apophis::GlobalQuantHst<T,idx>       src/hydro/hydro.cpp:746-771
+ *   gravity source      Apophis::ApplyGravity                src/gravity/gravity.cpp:374-410
+ *   stage sequence      ApophisDriver::MakeTaskCollection    src/apophis_driver.cpp:27-130
+ * and the upstream Parthenon pieces that task graph calls (flux-divergence update,
+ * RK2 register copy, ghost exchange, outflow/periodic boundaries).
+ *
+ * Conventions (all entry points):
+ *  - every array argument is a DEVICE pointer borrowed from the caller;
+ *  - every function returns 0 on success or a negative APH_E* code, and stores a
+ *    message retrievable with aph_last_error(ctx); no C++ exception crosses the ABI;
+ *  - work is enqueued on the context's stream; nothing synchronises implicitly,
+ *    except the functions documented as "synchronous";
+ *  - one context per stream; a context must not be used from two threads at once.
+ *
+ * Field storage ("pack" layout, same as the reference's VariablePack): a field with
+ * NV components over a block set stores block b, component v, cell (k, j, i) at
+ *     NV * blocks[b].cell_base + v * ncell(b) + (k * Ny(b) + j) * Nx(b) + i
+ * where Nx = nx[0] + 2*ng on an active axis and nx[0] on an inactive one (same for
+ * Ny, Nz), and ncell(b) = Nx * Ny * Nz ("entire" cells, ghosts included).
+ * Face arrays (fluxes) use the same layout: face i of axis x1 lies between cells
+ * i-1 and i.  Variable indices follow src/main.hpp:6-20 (IDN=0, IM1..IM3=1..3,
+ * IEN=4; primitive IV1..IV3=1..3, IPR=4); passive scalars follow at 5.. .
+ */
+#ifndef APOPHIS_HYDRO_H_
+#define APOPHIS_HYDRO_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define APH_ABI_VERSION 1
+
+/* error codes */
+#define APH_OK 0
+#define APH_E_INVALID (-1)   /* bad argument / unsupported combination */
+#define APH_E_HIP (-2)       /* a HIP runtime call failed */
+#define APH_E_NOMEM (-3)     /* device allocation failed */
+#define APH_E_STATE (-4)     /* call out of order (e.g. graph launch before capture) */
+
+/* Enumerations carry the values of the reference enums (src/main.hpp:32-53). */
+enum aph_reconstruction { APH_RECON_DC = 1, APH_RECON_PLM = 2, APH_RECON_PPM = 3 };
+enum aph_riemann { APH_RIEMANN_NONE = 1, APH_RIEMANN_HLLE = 2, APH_RIEMANN_HLLC = 3,
+                   APH_RIEMANN_LHLLC = 4 };
+/* Physical boundary applied on a face whose neighbour entry is -1.
+   Periodicity is expressed through the neighbour table instead. */
+enum aph_bc { APH_BC_OUTFLOW = 1, APH_BC_REFLECT = 2 };
+
+/* One meshblock.  240 bytes, no implicit padding (numpy mirrors it field by field). */
+typedef struct aph_block {
+  int32_t nx[3];     /* interior cells per axis, 1 on inactive axes                  */
+  int32_t ng;        /* ghost layers on every active axis                           */
+  int64_t cell_base; /* sum of ncell() of all preceding blocks                      */
+  double dx[3];      /* cell widths (inactive axes: full extent of the domain)      */
+  double area[3];    /* face areas: dx1*dx2 (x1 faces), dx0*dx2 (x2), dx0*dx1 (x3)  */
+  double vol;        /* cell volume dx0*dx1*dx2                                     */
+  double xmin[3];    /* coordinate of the lower interior corner                     */
+  int32_t nbr[27];   /* neighbour block at offset (ox,oy,oz) in {-1,0,1}^3, stored at
+                        (oz+1)*9 + (oy+1)*3 + (ox+1); -1 = physical boundary         */
+  int32_t bc[6];     /* aph_bc for faces -x1,+x1,-x2,+x2,-x3,+x3 when nbr is -1      */
+  int32_t mesh_id;   /* independent mesh (instance) this block belongs to           */
+} aph_block;
+
+/* Ideal-gas EOS and floors (hydro.cpp:260-299). */
+typedef struct aph_eos {
+  double gm1;    /* gamma - 1 (eos/gm1)                     */
+  double cv;     /* specific heat (eos/cv), carried for API parity, unused by ideal gas */
+  double dfloor; /* hydro/dfloor                            */
+  double pfloor; /* hydro/pfloor                            */
+} aph_eos;
+
+typedef struct aph_context aph_context;
+typedef struct aph_blockset aph_blockset;
+typedef struct aph_plan aph_plan;
+
+/* ---------------------------------------------------------------- context */
+int aph_abi_version(void);
+/* stream: a hipStream_t (NULL = the default stream). */
+int aph_context_create(void *stream, aph_context **out);
+int aph_context_destroy(aph_context *ctx);
+const char *aph_last_error(const aph_context *ctx);
+/* Synchronous: waits for all work enqueued on the context's stream. */
+int aph_context_synchronize(aph_context *ctx);
+
+/* ------------------------------------------------------------- block set
+ * Copies the block table to the device.  ndim in {1,2,3}; nscalars >= 0 passive
+ * scalars follow the 5 hydro variables (hydro.cpp:141-202).  Synchronous. */
+int aph_blockset_create(aph_context *ctx, const aph_block *blocks, int32_t nblocks,
+                        int32_t ndim, int32_t nscalars, aph_blockset **out);
+int aph_blockset_destroy(aph_context *ctx, aph_blockset *bs);
+/* Total entire cells (sum of ncell) -- one component of a field has this many doubles. */
+int64_t aph_blockset_ncells(const aph_blockset *bs);
+
+/* ----------------------------------------------------- plain form (per pack)
+ * These mirror the reference task functions one to one and run on every block of
+ * the block set (the reference's MeshData partition).  `first_block`/`nblocks`
+ * select a contiguous sub-range of blocks (the reference's pack_size); pass 0 and
+ * -1 for all blocks. */
+
+/* ConsToPrim (hydro.cpp:422-533): over all cells incl. ghosts, applies the density
+   and pressure floors to `cons` in place, writes prim (5+nscalars) and gamma (2). */
+int aph_cons_to_prim(aph_context *ctx, const aph_blockset *bs, int32_t first_block,
+                     int32_t nblocks, const aph_eos *eos, double *cons, double *prim,
+                     double *gamma);
+
+/* CalculateFluxes<euler,recon,rsolver> (hydro.cpp:535-744): writes cons.flux for
+   every face the reference loops visit, including the one transverse ghost row.
+   flux_x2 / flux_x3 may be NULL when ndim < 2 / < 3. */
+int aph_calculate_fluxes(aph_context *ctx, const aph_blockset *bs, int32_t first_block,
+                         int32_t nblocks, int32_t recon, int32_t rsolver,
+                         const double *prim, const double *gamma, double *flux_x1,
+                         double *flux_x2, double *flux_x3);
+
+/* Parthenon Update::UpdateWithFluxDivergence (call site apophis_driver.cpp:79-82):
+   interior u0 <- gam0*u0 + gam1*u1 + beta_dt * (-div F). */
+int aph_update_with_flux_divergence(aph_context *ctx, const aph_blockset *bs,
+                                    int32_t first_block, int32_t nblocks, double *u0,
+                                    const double *u1, const double *flux_x1,
+                                    const double *flux_x2, const double *flux_x3,
+                                    double gam0, double gam1, double beta_dt);
+
+/* ApplyGravity (gravity.cpp:374-410) for a constant acceleration g[3]. */
+int aph_apply_gravity(aph_context *ctx, const aph_blockset *bs, int32_t first_block,
+                      int32_t nblocks, double *cons, const double *prim,
+                      const double g[3], double dt);
+
+/* Ghost exchange between neighbouring blocks followed by the physical boundary
+   conditions (outflow / reflect, apophis_boundaries.cpp:17-92), axis by axis. */
+int aph_fill_ghosts(aph_context *ctx, const aph_blockset *bs, int32_t first_block,
+                    int32_t nblocks, double *cons);
+
+/* EstimateTimestep<euler> (hydro.cpp:351-420): *dt_out (device) = cfl * min over
+   interior cells and active axes of dx/(|v|+lambda). */
+int aph_estimate_timestep(aph_context *ctx, const aph_blockset *bs, int32_t first_block,
+                          int32_t nblocks, const aph_eos *eos, const double *prim,
+                          const double *cons, double cfl, double *dt_out);
+
+/* GlobalQuantHst<Sum, var> (hydro.cpp:746-771): *out (device) = sum(cons[var]*vol). */
+int aph_global_sum(aph_context *ctx, const aph_blockset *bs, const double *cons,
+                   int32_t var, double *out);
+
+/* ----------------------------------------------------- batched fused form
+ * A plan binds a block set to one (recon, rsolver) pair and a tile shape per block
+ * size.  aph_fused_stage runs ONE persistent launch that, for every block, does
+ * cons->prim (with floors), reconstruction, the Riemann solve and the flux-
+ * divergence update of one RK2 stage, reading ghost values straight from the
+ * neighbouring blocks (or through the physical boundary rule) instead of from
+ * ghost storage:
+ *   stage 1:  u_out = 0*u_in + 1*u_in + dt   * L(u_in)    (u_n unused)
+ *   stage 2:  u_out = .5*u_in + .5*u_n + dt/2 * L(u_in)   (u_out may alias u_n)
+ * Stage 1 also writes the floored interior state back to u_in.  dt is read from
+ * device memory (*dt_dev).  After stage 2 the plan's time-step estimate for the
+ * new state is min-reduced on the device (see aph_plan_step). */
+typedef struct aph_tile {
+  int32_t n;      /* block edge this entry applies to (8, 16, 32, ...)      */
+  int32_t tx, ty, tz; /* tile shape in cells; must divide the block edge    */
+} aph_tile;
+
+int aph_plan_create(aph_context *ctx, const aph_blockset *bs, int32_t recon,
+                    int32_t rsolver, const aph_eos *eos, const double g[3],
+                    aph_plan **out);
+int aph_plan_destroy(aph_context *ctx, aph_plan *plan);
+/* Override the tile shape for blocks of edge t->n (autotuner hook).  Synchronous. */
+int aph_plan_set_tile(aph_context *ctx, aph_plan *plan, const aph_tile *t);
+int aph_fused_stage(aph_context *ctx, aph_plan *plan, int32_t stage, double *u_in,
+                    const double *u_n, double *u_out, const double *dt_dev);
+
+/* Device-resident time state used by aph_plan_step / the captured graph:
+   dt_dev[0] = current dt, time_dev[0] = simulation time.  One RK2 step:
+     stage 1 (U0 -> U1), stage 2 (U1, U0 -> U0), then
+     dt <- min(2*dt, cfl * min_est), clipped so that time + dt <= tlim, time += dt_old.
+   When fixed_dt != 0 the dt is left unchanged (frozen-dt rollouts). */
+int aph_plan_step(aph_context *ctx, aph_plan *plan, double *u0, double *u1,
+                  double *dt_dev, double *time_dev, double cfl, double tlim,
+                  int32_t fixed_dt);
+/* Capture `nsteps` calls of aph_plan_step into a hipGraph (stream capture) and keep
+   it in the plan; aph_plan_graph_launch replays it on the context stream. */
+int aph_plan_graph_capture(aph_context *ctx, aph_plan *plan, int32_t nsteps, double *u0,
+                           double *u1, double *dt_dev, double *time_dev, double cfl,
+                           double tlim, int32_t fixed_dt);
+int aph_plan_graph_launch(aph_context *ctx, aph_plan *plan);
+/* Initial dt estimate for the plan's block set from cons (device), written to *dt_out:
+   cfl * min(dx/(|v|+lambda)) with the ConsToPrim floors applied on the fly. */
+int aph_plan_estimate_timestep(aph_context *ctx, aph_plan *plan, const double *cons,
+                               double cfl, double *dt_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* APOPHIS_HYDRO_H_ */

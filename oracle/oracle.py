@@ -1,0 +1,189 @@
+"""ctypes front end of the CPU oracle (oracle/hydro_oracle.c).
+
+TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The product path never imports it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "libhydro_oracle.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+
+
+def build(force=False):
+    """Compile the oracle with its Makefile (gcc, -ffp-contract=off, OpenMP)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-B" if force else "all"], cwd=_HERE, check=True)
+    return _LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.orc_plm.argtypes = [ctypes.c_double] * 3 + [_dp, _dp]
+        L.orc_ppm.argtypes = [ctypes.c_double] * 5 + [_dp, _dp]
+        for f in (L.orc_hlle, L.orc_hllc):
+            f.argtypes = [_dp] * 5
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_cons_to_prim.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp,
+                                       _dp, _dp, _dp]
+        L.orc_calculate_fluxes.argtypes = [_vp] + [ctypes.c_int] * 5 + [_dp] * 5
+        L.orc_calculate_fluxes.restype = ctypes.c_int
+        L.orc_update.argtypes = [_vp] + [ctypes.c_int] * 3 + [_dp] * 5 + [ctypes.c_double] * 3
+        L.orc_apply_gravity.argtypes = [_vp] + [ctypes.c_int] * 3 + [_dp, _dp, _dp,
+                                                                     ctypes.c_double]
+        L.orc_fill_ghosts.argtypes = [_vp] + [ctypes.c_int] * 3 + [_dp]
+        L.orc_estimate_timestep.argtypes = [_vp] + [ctypes.c_int] * 3 + [_vp, _dp, _dp,
+                                                                         ctypes.c_double]
+        L.orc_estimate_timestep.restype = ctypes.c_double
+        L.orc_global_sum.argtypes = [_vp] + [ctypes.c_int] * 3 + [_dp, ctypes.c_int]
+        L.orc_global_sum.restype = ctypes.c_double
+        L.orc_rk2_step.argtypes = ([_vp] + [ctypes.c_int] * 5 + [_vp, _dp, ctypes.c_double,
+                                                                 ctypes.c_double] + [_dp] * 7)
+        L.orc_rk2_step.restype = ctypes.c_double
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def _bp(bs):
+    return bs.blocks.ctypes.data_as(_vp)
+
+
+# ------------------------------------------------------------ point functions
+def plm(qm1, q0, qp1):
+    a, b = ctypes.c_double(), ctypes.c_double()
+    lib().orc_plm(qm1, q0, qp1, ctypes.byref(a), ctypes.byref(b))
+    return a.value, b.value  # (ql_ip1, qr_i)
+
+
+def ppm(qm2, qm1, q0, qp1, qp2):
+    a, b = ctypes.c_double(), ctypes.c_double()
+    lib().orc_ppm(qm2, qm1, q0, qp1, qp2, ctypes.byref(a), ctypes.byref(b))
+    return a.value, b.value  # (ql_ip1, qr_i)
+
+
+def riemann(kind, wl, wr, ifl, ifr):
+    wl, wr = np.ascontiguousarray(wl, float), np.ascontiguousarray(wr, float)
+    ifl, ifr = np.ascontiguousarray(ifl, float), np.ascontiguousarray(ifr, float)
+    out = np.zeros(5)
+    f = lib().orc_hllc if kind == "hllc" else lib().orc_hlle
+    f(_p(wl), _p(wr), _p(ifl), _p(ifr), _p(out))
+    return out
+
+
+# ------------------------------------------------------------ mesh functions
+RECON = {"dc": 1, "plm": 2, "ppm": 3}
+RIEMANN = {"hlle": 2, "hllc": 3}
+
+
+def set_threads(n):
+    lib().orc_set_threads(int(n))
+
+
+class State:
+    """Oracle-side state of a problem: u0, u1, prim, gamma and flux scratch."""
+
+    def __init__(self, prob):
+        self.prob = prob
+        bs = prob.bs
+        self.bs = bs
+        nv = bs.nvar
+        self.u0 = prob.cons.copy()
+        self.u1 = np.zeros_like(self.u0)
+        self.prim = np.zeros_like(self.u0)
+        self.gamma = np.zeros(bs.field_size(2))
+        self.fx = [np.zeros_like(self.u0) for _ in range(3)]
+        self.g = np.ascontiguousarray(prob.g, dtype=np.float64)
+        fill_ghosts(bs, self.u0)
+        cons_to_prim(bs, prob.eos, self.u0, self.prim, self.gamma)
+
+    def estimate_dt(self):
+        return estimate_timestep(self.bs, self.prob.eos, self.prim, self.u0, self.prob.cfl)
+
+    def step(self, dt):
+        p = self.prob
+        return lib().orc_rk2_step(_bp(self.bs), self.bs.nblocks, self.bs.ndim, self.bs.nvar,
+                                  RECON[p.recon], RIEMANN[p.riemann],
+                                  p.eos.ctypes.data_as(_vp), _p(self.g), p.cfl, dt,
+                                  _p(self.u0), _p(self.u1), _p(self.prim), _p(self.gamma),
+                                  _p(self.fx[0]), _p(self.fx[1]), _p(self.fx[2]))
+
+    def run(self, nsteps=None, tlim=None, fixed_dt=None):
+        """Evolve with the upstream dt policy (<= 2x growth, clip to tlim)."""
+        t = 0.0
+        dt_est = self.estimate_dt()
+        dt = np.inf
+        n = 0
+        while True:
+            if nsteps is not None and n >= nsteps:
+                break
+            if tlim is not None and t >= tlim * (1 - 1e-14):
+                break
+            if fixed_dt is not None:
+                dt = fixed_dt
+            else:
+                dt = min(2.0 * dt, dt_est)
+                if tlim is not None and t < tlim and tlim - t < dt:
+                    dt = tlim - t
+            dt_est = self.step(dt)
+            t += dt
+            n += 1
+        return t, n
+
+
+def cons_to_prim(bs, eos, cons, prim, gamma):
+    lib().orc_cons_to_prim(_bp(bs), bs.nblocks, bs.ndim, bs.nvar, eos.ctypes.data_as(_vp),
+                           _p(cons), _p(prim), _p(gamma))
+
+
+def calculate_fluxes(bs, recon, riemann_, prim, gamma, fx1, fx2, fx3):
+    rc = lib().orc_calculate_fluxes(_bp(bs), bs.nblocks, bs.ndim, bs.nvar, RECON[recon],
+                                    RIEMANN[riemann_], _p(prim), _p(gamma), _p(fx1),
+                                    _p(fx2), _p(fx3))
+    if rc != 0:
+        raise ValueError("unsupported reconstruction/Riemann combination")
+
+
+def update(bs, u0, u1, fx1, fx2, fx3, gam0, gam1, beta_dt):
+    lib().orc_update(_bp(bs), bs.nblocks, bs.ndim, bs.nvar, _p(u0), _p(u1), _p(fx1),
+                     _p(fx2), _p(fx3), gam0, gam1, beta_dt)
+
+
+def apply_gravity(bs, cons, prim, g, dt):
+    g = np.ascontiguousarray(g, dtype=np.float64)
+    lib().orc_apply_gravity(_bp(bs), bs.nblocks, bs.ndim, bs.nvar, _p(cons), _p(prim),
+                            _p(g), dt)
+
+
+def fill_ghosts(bs, cons):
+    lib().orc_fill_ghosts(_bp(bs), bs.nblocks, bs.ndim, bs.nvar, _p(cons))
+
+
+def estimate_timestep(bs, eos, prim, cons, cfl):
+    return lib().orc_estimate_timestep(_bp(bs), bs.nblocks, bs.ndim, bs.nvar,
+                                       eos.ctypes.data_as(_vp), _p(prim), _p(cons), cfl)
+
+
+def global_sum(bs, cons, var):
+    return lib().orc_global_sum(_bp(bs), bs.nblocks, bs.ndim, bs.nvar, _p(cons), var)

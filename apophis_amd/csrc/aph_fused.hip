@@ -317,8 +317,9 @@ struct aph_plan {
   aph_eos eos{};
   double g[3] = {0, 0, 0};
   int grav = 0;
-  std::vector<aph_tile> tiles;  // per size class (keyed by nx[0])
-  std::vector<int> class_n;     // class index -> nx[0]
+  std::vector<aph_tile> user_tiles;  // aph_plan_set_tile overrides, keyed by nx[0]
+  std::vector<aph_tile> tiles;       // resolved tile per class
+  std::vector<int4> class_shape;  // class index -> block shape (nx0, nx1, nx2)
   int4 *d_items = nullptr;
   int nitems = 0;
   int grid = 0;
@@ -359,17 +360,19 @@ KernelFn pick_kernel(int recon, int rs, int nv, int ndim) {
   return nullptr;
 }
 
-aph_tile default_tile(int n, int ndim) {
-  aph_tile t{n, 1, 1, 1};
-  if (ndim == 3) {
-    int e = std::min(n, 8);
-    t.tx = t.ty = t.tz = e;
-  } else if (ndim == 2) {
-    int e = std::min(n, 16);
-    t.tx = t.ty = e;
-  } else {
-    t.tx = std::min(n, 256);
-  }
+// Largest divisor of n that does not exceed cap.
+int divisor_at_most(int n, int cap) {
+  for (int d = std::min(n, cap); d > 1; --d)
+    if (n % d == 0) return d;
+  return 1;
+}
+
+aph_tile default_tile(const aph_block &k, int ndim) {
+  const int cap = (ndim == 3) ? 8 : (ndim == 2 ? 16 : 256);
+  aph_tile t{k.nx[0], 1, 1, 1};
+  t.tx = divisor_at_most(k.nx[0], cap);
+  if (ndim >= 2) t.ty = divisor_at_most(k.nx[1], cap);
+  if (ndim >= 3) t.tz = divisor_at_most(k.nx[2], cap);
   return t;
 }
 
@@ -385,30 +388,30 @@ int seg_len(int t_d, int pencils) {
 int build_items(aph_context *ctx, aph_plan *p) {
   const aph_blockset *bs = p->bs;
   const int ndim = bs->ndim;
-  p->class_n.clear();
+  p->class_shape.clear();
   std::vector<int> cls_of(bs->nblocks);
   for (int b = 0; b < bs->nblocks; ++b) {
     const aph_block &k = bs->h[b];
-    int n = k.nx[0];
-    auto it = std::find(p->class_n.begin(), p->class_n.end(), n);
-    int c;
-    if (it == p->class_n.end()) {
-      if ((int)p->class_n.size() >= kMaxClasses)
-        return set_error(ctx, APH_E_INVALID, "more than %d block size classes", kMaxClasses);
-      p->class_n.push_back(n);
-      c = (int)p->class_n.size() - 1;
-    } else {
-      c = (int)(it - p->class_n.begin());
+    int c = -1;
+    for (size_t q = 0; q < p->class_shape.size(); ++q) {
+      const int4 &sh = p->class_shape[q];
+      if (sh.x == k.nx[0] && sh.y == k.nx[1] && sh.z == k.nx[2]) c = (int)q;
+    }
+    if (c < 0) {
+      if ((int)p->class_shape.size() >= kMaxClasses)
+        return set_error(ctx, APH_E_INVALID, "more than %d block shapes", kMaxClasses);
+      p->class_shape.push_back(make_int4(k.nx[0], k.nx[1], k.nx[2], b));
+      c = (int)p->class_shape.size() - 1;
     }
     cls_of[b] = c;
   }
-  // tile per class
-  std::vector<aph_tile> tl(p->class_n.size());
-  for (size_t c = 0; c < p->class_n.size(); ++c) {
-    int n = p->class_n[c];
-    aph_tile t = default_tile(n, ndim);
-    for (const auto &u : p->tiles)
-      if (u.n == n) t = u;
+  // tile per class: a user tile for edge n = nx0 (aph_plan_set_tile) if one is set
+  std::vector<aph_tile> tl(p->class_shape.size());
+  for (size_t c = 0; c < p->class_shape.size(); ++c) {
+    const aph_block &k = bs->h[p->class_shape[c].w];
+    aph_tile t = default_tile(k, ndim);
+    for (const auto &u : p->user_tiles)
+      if (u.n == k.nx[0]) t = u;
     tl[c] = t;
   }
   // per XCD lists, greedy by cost (cells) so every L2 sees whole blocks
@@ -449,7 +452,6 @@ int build_items(aph_context *ctx, aph_plan *p) {
   p->lds = (ndim >= 2) ? maxcells * bs->nvar * sizeof(double) : 0;
   if (p->lds > 64 * 1024)
     return set_error(ctx, APH_E_INVALID, "tile too large for LDS (%zu bytes)", p->lds);
-  // resolved tile configs
   p->tiles.assign(tl.begin(), tl.end());
   // persistent grid: resident workgroups, a multiple of the XCD count
   KernelFn fn = pick_kernel(p->recon, p->rsolver, bs->nvar, ndim);
@@ -575,17 +577,17 @@ int aph_plan_set_tile(aph_context *ctx, aph_plan *p, const aph_tile *t) {
   if (!p || !t) return set_error(ctx, APH_E_INVALID, "aph_plan_set_tile: null");
   if (t->tx < 1 || t->ty < 1 || t->tz < 1)
     return set_error(ctx, APH_E_INVALID, "tile extents must be positive");
-  std::vector<aph_tile> keep = p->tiles;
+  std::vector<aph_tile> keep = p->user_tiles;
   bool found = false;
-  for (auto &u : p->tiles)
+  for (auto &u : p->user_tiles)
     if (u.n == t->n) {
       u = *t;
       found = true;
     }
-  if (!found) p->tiles.push_back(*t);
+  if (!found) p->user_tiles.push_back(*t);
   int rc = build_items(ctx, p);
   if (rc) {
-    p->tiles = keep;
+    p->user_tiles = keep;
     build_items(ctx, p);
     return rc;
   }

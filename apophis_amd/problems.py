@@ -132,14 +132,15 @@ def blast3d(nblk=8, n=32, seed=20240601, recon="ppm", riemann="hllc", cfl=0.3,
 
 # ---------------------------------------------------------------- uneven blocks
 def _fourier_field(seed, ctr0, x, y, z, ndim):
-    """Sum of 4 seeded Fourier modes with integer wave vectors on the unit box,
+    """Sum of 4 seeded Fourier modes with wave-vector components in {-1, 0, 1} on the
+    unit box (one wavelength per block at most, so an 8-cell block resolves it),
     normalised to [-1, 1]."""
     r = uniform(seed, ctr0 + np.arange(32, dtype=np.uint64))
     amps = r[0:4] + 0.1
     amps = amps / amps.sum()
     s = np.zeros_like(x)
     for mdx in range(4):
-        kv = np.floor(r[4 + 3 * mdx: 7 + 3 * mdx] * 5.0) - 2.0  # integers in [-2, 2]
+        kv = np.floor(r[4 + 3 * mdx: 7 + 3 * mdx] * 3.0) - 1.0  # integers in [-1, 1]
         kv[ndim:] = 0.0
         if not kv.any():
             kv[0] = 1.0
@@ -157,8 +158,8 @@ def uneven_blocks(nblocks=4096, sizes=(8, 16, 32), probs=(0.5, 0.35, 0.15), ng=4
     unit cube (dx = 1/n) and is periodic onto itself, so it needs no neighbour; the
     global dt is the minimum over all blocks.  Each block holds a smooth field from 4
     seeded Fourier modes (rho in [0.5,2], p in [0.1,1], |u| <= 1) and, in jump_frac of
-    the blocks, a planar discontinuity through the block centre with a random normal
-    (density x4, pressure x10 on the far side)."""
+    the blocks, planar discontinuities at a random orientation (density x4, pressure x10
+    inside a slab bounded by two parallel planes, see below)."""
     gm1 = 2.0 / 3.0
     r = uniform(seed, np.arange(nblocks, dtype=np.uint64))
     cum = np.cumsum(probs)
@@ -179,13 +180,17 @@ def uneven_blocks(nblocks=4096, sizes=(8, 16, 32), probs=(0.5, 0.35, 0.15), ng=4
         prs = 0.55 + 0.45 * _fourier_field(seed, base + np.uint64(40), x, y, z, ndim)
         vel = [_fourier_field(seed, base + np.uint64(80 + 40 * d), x, y, z, ndim) / np.sqrt(3.0)
                for d in range(3)]
-        rj = uniform(seed, base + np.uint64(250 + np.arange(4, dtype=np.uint64)))
+        rj = uniform(seed, base + np.uint64(250 + np.arange(5, dtype=np.uint64)))
         if rj[0] < jump_frac:
-            nrm = rj[1:4] * 2.0 - 1.0
+            # a slab between two parallel planes with an integer normal (13 possible
+            # orientations), so the discontinuity is compatible with the block's own
+            # periodicity; the inner side of the slab gets density x4, pressure x10
+            nrm = np.floor(rj[1:4] * 3.0) - 1.0
             nrm[ndim:] = 0.0
             if not nrm.any():
                 nrm[0] = 1.0
-            side = ((x - 0.5) * nrm[0] + (y - 0.5) * nrm[1] + (z - 0.5) * nrm[2]) > 0.0
+            phase = (x * nrm[0] + y * nrm[1] + z * nrm[2] + rj[4]) % 1.0
+            side = phase < 0.5
             rho = np.where(side, rho * 4.0, rho)
             prs = np.where(side, prs * 10.0, prs)
         ek = 0.5 * rho * (vel[0] ** 2 + vel[1] ** 2 + vel[2] ** 2)

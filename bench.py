@@ -1,0 +1,225 @@
+"""Benchmark of the hydro flux path on one MI355X.
+
+Default workload (N=1): 3D blast wave, 256^3 cells as 512 meshblocks of 32^3, FP64,
+gamma=5/3, PPM + HLLC, RK2, CFL 0.3, periodic.  A step is one full RK2 step (2 stages)
+of the batched fused form replayed from a captured hipGraph with device-resident dt.
+
+Printed (one JSON line): zone-cycles/s (interior cells x RK2 steps / s), the roofline
+of the dominant kernel (the fused stage kernel, timed live with HIP events on its own
+stream), the plain form's throughput on the same workload, the uneven-block (batched
+vs per-block plain) comparison, and the oracle's CPU throughput on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def bytes_per_zone_cycle(nv, ndim, g, n):
+    """Algorithmic bytes of one fused RK2 zone-cycle: 8*NV*(2f+3), f = 1 + 2*d*g/n."""
+    f = 1.0 + 2.0 * ndim * g / n
+    return 8.0 * nv * (2.0 * f + 3.0)
+
+
+def stage_bytes(bs, stencil_g):
+    """Algorithmic bytes of one stage launch averaged over the two stages
+    (stage 1: f+1 state reads/writes per cell, stage 2: f+2)."""
+    tot = 0.0
+    for b in range(bs.nblocks):
+        n = int(bs.blocks["nx"][b][0])
+        cells = int(np.prod(bs.blocks["nx"][b][: bs.ndim]))
+        tot += cells * bytes_per_zone_cycle(bs.nvar, bs.ndim, stencil_g, n) / 2.0
+    return tot
+
+
+def event_time_stage(fd, torch, reps=10):
+    """Average duration (ms) of one fused stage launch, HIP events on fd's stream."""
+    s = torch.cuda.current_stream()
+    times = []
+    for r in range(reps):
+        for stage, (a, b, c) in ((1, (fd.u0, None, fd.u1)), (2, (fd.u1, fd.u0, fd.u0))):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(s)
+            fd.stage(stage, a, b, c)
+            e1.record(s)
+            e1.synchronize()
+            times.append(e0.elapsed_time(e1))
+    return float(np.mean(times)), float(np.min(times))
+
+
+def cpu_baseline(threads=None):
+    """Oracle (C restatement, OpenMP over blocks) on a bounded sample of the workload:
+    a 128^3 periodic blast (64 blocks of 32^3, PPM+HLLC), RK2 steps until ~10 s of
+    CPU time have elapsed (at most 40 steps)."""
+    from apophis_amd import problems
+    from oracle import oracle as orc
+
+    ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    threads = threads or min(ncpu, 16)
+    orc.set_threads(threads)
+    p = problems.blast3d(nblk=4, n=32)
+    st = orc.State(p)
+    dt = st.estimate_dt()
+    st.step(dt)  # warm-up
+    nsteps = 0
+    t0 = time.perf_counter()
+    while nsteps < 40 and time.perf_counter() - t0 < 10.0:
+        st.step(dt)
+        nsteps += 1
+    el = time.perf_counter() - t0
+    cells = p.bs.interior_cells()
+    return {"value": cells * nsteps / el, "unit": "zone-cycles/s", "cores": threads,
+            "kind": "port",
+            "sample": f"oracle rk2 step, 128^3 periodic blast (64 blocks of 32^3, PPM+HLLC), "
+                      f"{nsteps} steps after 1 warm-up, {el:.1f} s, OMP threads={threads}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--nblk", type=int, default=8, help="blocks per axis (8 -> 256^3)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip plain/uneven legs")
+    ap.add_argument("--uneven-blocks", type=int, default=4096)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is design-only, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from apophis_amd import hydro, problems
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
+    ctx = hydro._lib.Context(stream.cuda_stream)
+
+    # ------------------------------------------------------------ headline: fused
+    t_setup = time.perf_counter()
+    p = problems.blast3d(nblk=args.nblk, n=32)
+    cells = p.bs.interior_cells()
+    fd = hydro.FusedDriver(ctx, p.bs, p.cons, p.recon, p.riemann, p.eos, p.cfl)
+    fd.initial_dt()
+    for _ in range(args.warmup):
+        fd.step()
+    fd.capture(args.steps)
+    torch.cuda.synchronize()
+    print(f"# setup {time.perf_counter() - t_setup:.1f}s, {cells} cells, "
+          f"{p.bs.nblocks} blocks", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    fd.replay()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    value = cells * args.steps / el
+    u = fd.u0.cpu().numpy()
+    finite = bool(np.isfinite(u[np.tile(p.bs.interior_mask(), p.bs.nvar)]).all())
+    # roofline of the fused stage kernel (HIP events on this stream)
+    avg_ms, min_ms = event_time_stage(fd, torch)
+    sb = stage_bytes(p.bs, 3)
+    achieved = sb / (avg_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "k_fused<PPM,HLLC,NV=5,3D>", "avg_launch_ms": round(avg_ms, 4),
+            "algorithmic_bytes_per_launch": sb}
+    print(f"# fused: {value:.4e} zone-cycles/s, stage {avg_ms:.3f} ms, "
+          f"{achieved:.0f} GB/s", file=sys.stderr, flush=True)
+    del fd
+    torch.cuda.empty_cache()
+
+    extras = {}
+    if not args.no_extras:
+        # plain form on the same workload (reference task structure, HBM round trips)
+        pkg = hydro.initialize_hydro({
+            "parthenon/time": {"cfl": p.cfl, "integrator": "rk2"},
+            "parthenon/mesh": {"nghost": 3},
+            "hydro": {"reconstruction": "ppm", "riemann": "hllc"},
+            "eos": {"gm1": float(p.eos["gm1"][0])}})
+        md = hydro.MeshData(ctx, p.bs, p.cons)
+        drv = hydro.ApophisDriver(pkg, md)
+        dt = drv.initialize()
+        drv.step(dt)
+        torch.cuda.synchronize()
+        ns = max(3, args.steps // 10)
+        t0 = time.perf_counter()
+        for _ in range(ns):
+            dt = drv.step(dt)
+        torch.cuda.synchronize()
+        extras["plain_form"] = {"value": cells * ns / (time.perf_counter() - t0),
+                                "steps": ns, "note": "per-step dt read back to host"}
+        print(f"# plain: {extras['plain_form']['value']:.4e}", file=sys.stderr, flush=True)
+        del drv, md
+        torch.cuda.empty_cache()
+        # uneven AMR-like blocks: batched fused vs per-block plain launches
+        q = problems.uneven_blocks(nblocks=args.uneven_blocks)
+        qc = q.bs.interior_cells()
+        fq = hydro.FusedDriver(ctx, q.bs, q.cons, q.recon, q.riemann, q.eos, q.cfl)
+        fq.initial_dt()
+        for _ in range(2):
+            fq.step()
+        nq = 20
+        fq.capture(nq)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fq.replay()
+        torch.cuda.synchronize()
+        vb = qc * nq / (time.perf_counter() - t0)
+        del fq
+        pkg = hydro.initialize_hydro({
+            "parthenon/time": {"cfl": q.cfl, "integrator": "rk2"},
+            "parthenon/mesh": {"nghost": 4},
+            "hydro": {"reconstruction": "ppm", "riemann": "hllc"},
+            "eos": {"gm1": float(q.eos["gm1"][0])}})
+        mq = hydro.MeshData(ctx, q.bs, q.cons)
+        dq = hydro.ApophisDriver(pkg, mq, pack_size=1)
+        dt = dq.initialize()
+        dq.step(dt)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        npl = 2
+        for _ in range(npl):
+            dt = dq.step(dt)
+        torch.cuda.synchronize()
+        vp = qc * npl / (time.perf_counter() - t0)
+        extras["uneven_blocks"] = {"blocks": q.bs.nblocks, "cells": qc,
+                                   "batched_value": vb, "plain_per_block_value": vp,
+                                   "batched_over_plain": vb / vp}
+        print(f"# uneven: batched {vb:.4e}, plain per-block {vp:.4e}", file=sys.stderr,
+              flush=True)
+        del dq, mq
+
+    cpu = None if args.no_cpu else cpu_baseline()
+    out = {
+        "metric": "zone-cycles/s + % HBM roofline on 1 GPU (batched vs plain, FP64); "
+                  "adjoint/forward time",
+        "value": value, "unit": "zone-cycles/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded blast wave)",
+        "config": {"workload": f"blast3d {args.nblk * 32}^3 periodic, {p.bs.nblocks} blocks "
+                               "of 32^3, PPM+HLLC, RK2, CFL 0.3, gamma 5/3",
+                   "form": "batched fused, hipGraph-replayed, device dt",
+                   "finite": finite},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "extras": extras,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

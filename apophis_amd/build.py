@@ -36,6 +36,12 @@ def build(verbose=False, jobs=3):
     tmp = lib + ".tmp"
     subprocess.run(["hipcc", *FLAGS, "-shared", "-o", tmp, *objs], check=True)
     os.replace(tmp, lib)
+    # C++ host program using the reference-shaped adapter (host code only)
+    exe = os.path.join(OUT, "apophis_run")
+    subprocess.run(["hipcc", "-O2", "-std=c++17", os.path.join(SRC, "host", "apophis_run.cpp"),
+                    "-o", exe + ".tmp", "-L" + OUT, "-lapophis_hydro",
+                    "-Wl,-rpath,$ORIGIN"], check=True)
+    os.replace(exe + ".tmp", exe)
     return lib
 
 

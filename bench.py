@@ -42,6 +42,19 @@ def stage_bytes(bs, stencil_g):
     return tot
 
 
+def measured_traffic(name="k_fused_blast256_ppm_hllc.md"):
+    """HBM bytes per launch of the fused kernel from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE, each scaled by a 1 GiB copy's reading), or None."""
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def event_time_stage(fd, torch, reps=10):
     """Average duration (ms) of one fused stage launch, HIP events on fd's stream."""
     s = torch.cuda.current_stream()
@@ -85,6 +98,38 @@ def cpu_baseline(threads=None):
                       f"{nsteps} steps after 1 warm-up, {el:.1f} s, OMP threads={threads}"}
 
 
+def profile_run(args, ctx, hydro, problems, torch):
+    """Eager launches for rocprofv3: 3 RK2 steps of the chosen form on the headline
+    workload (or the uneven block set), then a 1 GiB device-to-device copy repeated 3
+    times, whose FETCH_SIZE / WRITE_SIZE calibrate the counters (2^30 bytes each way)."""
+    if args.profile_run == "uneven":
+        p = problems.uneven_blocks(nblocks=args.uneven_blocks)
+    else:
+        p = problems.blast3d(nblk=args.nblk, n=32)
+    if args.profile_run == "plain":
+        pkg = hydro.initialize_hydro({
+            "parthenon/time": {"cfl": p.cfl, "integrator": "rk2"},
+            "parthenon/mesh": {"nghost": 3},
+            "hydro": {"reconstruction": p.recon, "riemann": p.riemann},
+            "eos": {"gm1": float(p.eos["gm1"][0])}})
+        drv = hydro.ApophisDriver(pkg, hydro.MeshData(ctx, p.bs, p.cons))
+        dt = drv.initialize()
+        for _ in range(3):
+            dt = drv.step(dt)
+    else:
+        fd = hydro.FusedDriver(ctx, p.bs, p.cons, p.recon, p.riemann, p.eos, p.cfl)
+        fd.initial_dt()
+        for _ in range(3):
+            fd.step()
+    torch.cuda.synchronize()
+    a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(f"profile-run {args.profile_run}: done", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,6 +139,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip plain/uneven legs")
     ap.add_argument("--uneven-blocks", type=int, default=4096)
+    ap.add_argument("--profile-run", choices=["fused", "plain", "uneven"], default=None,
+                    help="short eager run for rocprofv3 (no timing, no JSON)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -108,6 +155,9 @@ def main():
     stream = torch.cuda.Stream()
     torch.cuda.set_stream(stream)
     ctx = hydro._lib.Context(stream.cuda_stream)
+    if args.profile_run:
+        profile_run(args, ctx, hydro, problems, torch)
+        return
 
     # ------------------------------------------------------------ headline: fused
     t_setup = time.perf_counter()
@@ -133,7 +183,8 @@ def main():
     sb = stage_bytes(p.bs, 3)
     achieved = sb / (avg_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic(),
             "kernel": "k_fused<PPM,HLLC,NV=5,3D>", "avg_launch_ms": round(avg_ms, 4),
             "algorithmic_bytes_per_launch": sb}
     print(f"# fused: {value:.4e} zone-cycles/s, stage {avg_ms:.3f} ms, "

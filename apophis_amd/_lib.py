@@ -60,6 +60,8 @@ _SIGS = {
     "aph_plan_graph_capture": (_i32, [_vp, _vp, _i32, _vp, _vp, _vp, _vp, _d, _d, _i32]),
     "aph_plan_graph_launch": (_i32, [_vp, _vp]),
     "aph_plan_estimate_timestep": (_i32, [_vp, _vp, _vp, _d, _vp]),
+    "aph_adjoint_stage": (_i32, [_vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "aph_adjoint_step": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp]),
 }
 
 _lib = None

@@ -1,0 +1,139 @@
+"""Differentiable rollouts of the batched fused form (torch.autograd over the C ABI).
+
+``rollout(u0, stepper, nsteps, checkpoint_every)`` advances the conserved state nsteps
+RK2 steps with a frozen dt through ``aph_plan_step`` and returns the final state; its
+backward pass runs the hand-written adjoint kernels (``aph_adjoint_step``) with step
+checkpointing: the forward keeps every ``checkpoint_every``-th state, the backward
+recomputes the states of one checkpoint segment at a time and walks them backwards.
+
+Also here: the Kelvin-Helmholtz initial state as a torch function of the perturbation
+amplitudes a_k (so gradients reach a_k through autograd) and the scalar-variance loss.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from .hydro import FusedDriver, _ptr
+
+
+class Stepper:
+    """A FusedDriver used as a frozen-dt step function on caller-provided states."""
+
+    def __init__(self, driver: FusedDriver, dt: float):
+        self.drv = driver
+        self.dt = float(dt)
+        self.drv.set_dt(self.dt)
+        self.scratch = torch.empty_like(driver.u0)
+
+    def step_(self, u):
+        """Advance u (device tensor, pack layout) one RK2 step in place."""
+        d = self.drv
+        d.ctx.check(_lib.lib().aph_plan_step(d.ctx.h, d.plan, _ptr(u), _ptr(self.scratch),
+                                             _ptr(d.dt), _ptr(d.time), d.cfl, d.tlim, 1))
+
+    def adjoint_step_(self, u_start, lam):
+        """lam <- (d step(u_start) / d u_start)^T lam, in place."""
+        d = self.drv
+        d.ctx.check(_lib.lib().aph_adjoint_step(d.ctx.h, d.plan, _ptr(u_start),
+                                                _ptr(self.scratch), _ptr(d.dt), _ptr(lam)))
+
+
+class _Rollout(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, u0, stepper, nsteps, every):
+        u = u0.detach().clone().contiguous()
+        checkpoints = []
+        for n in range(nsteps):
+            if n % every == 0:
+                checkpoints.append(u.clone())
+            stepper.step_(u)
+        ctx.stepper, ctx.nsteps, ctx.every = stepper, nsteps, every
+        ctx.checkpoints = checkpoints
+        return u
+
+    @staticmethod
+    def backward(ctx, grad):
+        st, nsteps, every = ctx.stepper, ctx.nsteps, ctx.every
+        lam = grad.detach().clone().contiguous()
+        mask = interior_mask_tensor(st.drv.bs, lam.device)
+        lam.mul_(mask)  # ghost entries carry no cotangent
+        for k in range(len(ctx.checkpoints) - 1, -1, -1):
+            n0 = k * every
+            n1 = min(nsteps, n0 + every)
+            states = [ctx.checkpoints[k]]
+            u = ctx.checkpoints[k].clone()
+            for _ in range(n0, n1 - 1):  # recompute the segment's step inputs
+                st.step_(u)
+                states.append(u.clone())
+            for n in range(n1 - 1, n0 - 1, -1):
+                st.adjoint_step_(states[n - n0], lam)
+        return lam * mask, None, None, None
+
+
+def rollout(u0, stepper, nsteps, checkpoint_every=20):
+    return _Rollout.apply(u0, stepper, int(nsteps), int(checkpoint_every))
+
+
+_MASKS = {}
+
+
+def interior_mask_tensor(bs, device):
+    key = (id(bs), str(device))
+    if key not in _MASKS:
+        m = np.tile(bs.interior_mask(), bs.nvar).astype(np.float64)
+        _MASKS[key] = torch.from_numpy(m).to(device)
+    return _MASKS[key]
+
+
+# ------------------------------------------------------------------ KH as f(a)
+def kh_state(bs, amps, a=0.01, sigma2=0.01, drat=2.0, vflow=0.5, p0=2.5, gm1=2.0 / 3.0):
+    """Kelvin-Helmholtz conserved state (problems.kh2d) as a differentiable torch
+    function of the amplitudes (a 1-D float64 tensor on the target device)."""
+    dev = amps.device
+    parts = []
+    cols = []
+    for b in range(bs.nblocks):
+        x, y, _ = bs.cell_centers(b)
+        parts.append((x[0], y[0]))
+    x = torch.from_numpy(np.stack([p[0] for p in parts])).to(dev)
+    y = torch.from_numpy(np.stack([p[1] for p in parts])).to(dev)
+    w = (torch.tanh((torch.abs(y) - 0.25) / a) + 1.0) * 0.5
+    rho = w + (1.0 - w) * drat
+    vx = vflow * w - (1.0 - w) * vflow
+    env = torch.exp(-((torch.abs(y) - 0.25) ** 2) / sigma2)
+    vy = torch.zeros_like(x)
+    for k in range(amps.shape[0]):
+        vy = vy + amps[k] * torch.sin(2.0 * np.pi * (k + 1) * x) * env
+    en = p0 / gm1 + 0.5 * rho * (vx * vx + vy * vy)
+    fields = [rho, rho * vx, rho * vy, torch.zeros_like(x), en, rho * (1.0 - w)]
+    # scatter into the pack layout (interior cells)
+    idx = torch.from_numpy(interior_index(bs)).to(dev)
+    out = torch.zeros(bs.field_size(bs.nvar), dtype=torch.float64, device=dev)
+    stacked = torch.stack([f.reshape(bs.nblocks, -1) for f in fields], dim=1)  # (nb, nv, cells)
+    return out.index_put((idx.reshape(-1),), stacked.reshape(-1))
+
+
+def interior_index(bs):
+    """Flat pack indices of the interior cells, shaped (nblocks, nvar, cells_per_block)
+    for blocks of equal shape."""
+    out = []
+    for b in range(bs.nblocks):
+        view_idx = np.arange(bs.nvar * int(bs.ncell[b])) + bs.nvar * int(bs.blocks["cell_base"][b])
+        v = view_idx.reshape((bs.nvar,) + tuple(int(e) for e in bs.extents()[b][::-1]))
+        out.append(v[(slice(None),) + bs.interior_slices(b)].reshape(bs.nvar, -1))
+    return np.stack(out)
+
+
+def scalar_variance(u, bs):
+    """Domain-integrated variance of the passive-scalar concentration s = (rho s)/rho:
+    sum V (s - <s>)^2 with <s> the volume average, over interior cells."""
+    idx = torch.from_numpy(interior_index(bs)).to(u.device)
+    v = u[idx]  # (nb, nv, cells)
+    s = v[:, 5] / v[:, 0]
+    vol = torch.from_numpy(bs.blocks["vol"].astype(np.float64)).to(u.device)[:, None]
+    mean = (vol * s).sum() / (vol.expand_as(s)).sum()
+    return (vol * (s - mean) ** 2).sum()

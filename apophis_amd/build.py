@@ -12,13 +12,13 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "_build")
-SOURCES = ["aph_runtime.hip", "aph_plain.hip", "aph_fused.hip"]
+SOURCES = ["aph_runtime.hip", "aph_plain.hip", "aph_fused.hip", "aph_adjoint.hip"]
 ARCH = os.environ.get("APH_OFFLOAD_ARCH", "gfx950")
 FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
-def build(verbose=False, jobs=3):
+def build(verbose=False, jobs=4):
     os.makedirs(OUT, exist_ok=True)
     objs, procs = [], []
     for s in SOURCES:
@@ -42,6 +42,13 @@ def build(verbose=False, jobs=3):
                     "-o", exe + ".tmp", "-L" + OUT, "-lapophis_hydro",
                     "-Wl,-rpath,$ORIGIN"], check=True)
     os.replace(exe + ".tmp", exe)
+    # host build of the point/adjoint functions for the CPU adjoint checks (tests/native)
+    tdir = os.path.join(os.path.dirname(HERE), "tests", "native")
+    if os.path.isdir(tdir):
+        os.makedirs(os.path.join(tdir, "_build"), exist_ok=True)
+        subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fPIC", "-shared",
+                        os.path.join(tdir, "adj_check.cpp"), "-o",
+                        os.path.join(tdir, "_build", "libadj_check.so")], check=True)
     return lib
 
 

@@ -1,0 +1,29 @@
+// aph_adjoint_host.hpp -- host interface of the adjoint stage kernels (aph_adjoint.hip).
+#pragma once
+
+#include <cstdint>
+
+#include "aph_runtime.hpp"
+
+namespace aph {
+
+// Device scratch of the adjoint: primitive-space cotangents (NV+2 fields, pack layout),
+// per-axis spill records and their per-block offsets, and two cotangent registers used
+// by a whole RK2 step.  Allocated on first use, released with the plan.
+struct AdjScratch {
+  double *pbar = nullptr;
+  double *spill[3] = {nullptr, nullptr, nullptr};
+  int64_t *spill_base = nullptr;
+  double *lam1 = nullptr, *lam0p = nullptr;
+  int64_t ncells = 0;
+};
+
+int adj_stage(aph_context *ctx, const aph_blockset *bs, int recon, int rs, const aph_eos &eos,
+              const double *g, int grav, AdjScratch &s, int stage, const double *u_in,
+              const double *dt_dev, const double *lam_out, const double *lam_acc,
+              double *lam_in, double *lam_n);
+int adj_prepare(aph_context *ctx, const aph_blockset *bs, AdjScratch &s);
+void adj_release(AdjScratch &s);
+bool adj_supported(int recon, int rs, int nv, int nd);
+
+}  // namespace aph

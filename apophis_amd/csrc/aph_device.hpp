@@ -1,37 +1,13 @@
-// aph_device.hpp -- device-side numerics of the hydro flux path (gfx950, FP64).
-//
-// Point functions restate the reference's arithmetic in its operand order; the
-// library is compiled with -ffp-contract=off so that no multiply-add is fused and
-// results are bitwise reproducible against the CPU restatement.
+// aph_device.hpp -- device-side helpers of the hydro flux path (gfx950, FP64):
+// pack-layout arithmetic and reductions, on top of the point functions.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#include "../../include/apophis_hydro.h"
+#include "aph_point.hpp"
 
 namespace aph {
-
-// Parthenon macros used by the reference (upstream): SIGN, SQR, TINY_NUMBER.
-__device__ __forceinline__ double SIGN(double x) { return (x < 0.0) ? -1.0 : 1.0; }
-__device__ __forceinline__ double SQR(double x) { return x * x; }
-constexpr double TINY_NUMBER = 1.0e-20;
-constexpr double C_LIGHT = 2.99792458e10;  // hydro_hlle.hpp:26, hydro_hllc.hpp:159
-constexpr double C_LIGHT2 = C_LIGHT * C_LIGHT;
-constexpr double EOS_SMALL = 10.0 * 2.2250738585072014e-308;  // singularity robust::SMALL()
-
-// std::min / std::max semantics
-__device__ __forceinline__ double mn(double a, double b) { return (b < a) ? b : a; }
-__device__ __forceinline__ double mx(double a, double b) { return (a < b) ? b : a; }
-
-constexpr int RECON_DC = APH_RECON_DC, RECON_PLM = APH_RECON_PLM, RECON_PPM = APH_RECON_PPM;
-constexpr int RS_HLLE = APH_RIEMANN_HLLE, RS_HLLC = APH_RIEMANN_HLLC;
-
-template <int RECON>
-struct ReconTraits {
-  static constexpr int GR = (RECON == RECON_PPM) ? 2 : ((RECON == RECON_PLM) ? 1 : 0);
-  static constexpr int W = 2 * GR + 1;  // window of cells one reconstruction reads
-};
 
 // ------------------------------------------------------------ layout
 __host__ __device__ __forceinline__ int ext(const aph_block &b, int d, int ndim) {
@@ -42,274 +18,6 @@ __host__ __device__ __forceinline__ int gst(const aph_block &b, int d, int ndim)
 }
 __host__ __device__ __forceinline__ int64_t ncell(const aph_block &b, int ndim) {
   return (int64_t)ext(b, 0, ndim) * ext(b, 1, ndim) * ext(b, 2, ndim);
-}
-
-// ------------------------------------------------------------ PLM
-// src/recon/plm_simple.hpp:14-31
-__device__ __forceinline__ void plm(double q_im1, double q_i, double q_ip1, double &ql_ip1,
-                                    double &qr_i) {
-  double dql = (q_i - q_im1);
-  double dqr = (q_ip1 - q_i);
-  double dq2 = dql * dqr;
-  double dqm = 0.0;
-  if (dq2 > 0.0) dqm = dq2 / (dql + dqr);
-  ql_ip1 = q_i + dqm;
-  qr_i = q_i - dqm;
-}
-
-// ------------------------------------------------------------ PPM
-// src/recon/ppm_simple.hpp:31-161
-__device__ __forceinline__ void ppm(double q_im2, double q_im1, double q_i, double q_ip1,
-                                    double q_ip2, double &ql_ip1, double &qr_i) {
-  const double C2 = 1.25;
-  double qa = (q_i - q_im1);
-  double qb = (q_ip1 - q_i);
-  const double dd_im1 = 0.5 * qa + 0.5 * (q_im1 - q_im2);
-  const double dd = 0.5 * qb + 0.5 * qa;
-  const double dd_ip1 = 0.5 * (q_ip2 - q_ip1) + 0.5 * qb;
-  double dph = 0.5 * (q_im1 + q_i) + (dd_im1 - dd) / 6.0;
-  double dph_ip1 = 0.5 * (q_i + q_ip1) + (dd - dd_ip1) / 6.0;
-  const double d2qc_im1 = q_im2 + q_i - 2.0 * q_im1;
-  const double d2qc = q_im1 + q_ip1 - 2.0 * q_i;
-  const double d2qc_ip1 = q_i + q_ip2 - 2.0 * q_ip1;
-  double qa_tmp = dph - q_im1;
-  double qb_tmp = q_i - dph;
-  qa = 3.0 * (q_im1 + q_i - 2.0 * dph);
-  qb = d2qc_im1;
-  double qc = d2qc;
-  double qd = 0.0;
-  if (SIGN(qa) == SIGN(qb) && SIGN(qa) == SIGN(qc))
-    qd = SIGN(qa) * mn(C2 * fabs(qb), mn(C2 * fabs(qc), fabs(qa)));
-  double dph_tmp = 0.5 * (q_im1 + q_i) - qd / 6.0;
-  if (qa_tmp * qb_tmp < 0.0) dph = dph_tmp;
-  qa_tmp = dph_ip1 - q_i;
-  qb_tmp = q_ip1 - dph_ip1;
-  qa = 3.0 * (q_i + q_ip1 - 2.0 * dph_ip1);
-  qb = d2qc;
-  qc = d2qc_ip1;
-  qd = 0.0;
-  if (SIGN(qa) == SIGN(qb) && SIGN(qa) == SIGN(qc))
-    qd = SIGN(qa) * mn(C2 * fabs(qb), mn(C2 * fabs(qc), fabs(qa)));
-  double dphip1_tmp = 0.5 * (q_i + q_ip1) - qd / 6.0;
-  if (qa_tmp * qb_tmp < 0.0) dph_ip1 = dphip1_tmp;
-  const double d2qf = 6.0 * (dph + dph_ip1 - 2.0 * q_i);
-  double qr = dph;
-  double ql = dph_ip1;
-  const double dqf_minus = q_i - qr;
-  const double dqf_plus = ql - q_i;
-  qa_tmp = dqf_minus * dqf_plus;
-  qb_tmp = (q_ip1 - q_i) * (q_i - q_im1);
-  qa = d2qc_im1;
-  qb = d2qc;
-  qc = d2qc_ip1;
-  qd = d2qf;
-  double qe = 0.0;
-  if (SIGN(qa) == SIGN(qb) && SIGN(qa) == SIGN(qc) && SIGN(qa) == SIGN(qd))
-    qe = SIGN(qd) * mn(mn(C2 * fabs(qa), C2 * fabs(qb)), mn(C2 * fabs(qc), fabs(qd)));
-  qa = mx(fabs(q_im1), fabs(q_im2));
-  qb = mx(mx(fabs(q_i), fabs(q_ip1)), fabs(q_ip2));
-  double rho = 0.0;
-  if (fabs(qd) > (1.0e-12) * mx(qa, qb)) rho = qe / qd;
-  double tmp_m = q_i - rho * dqf_minus;
-  double tmp_p = q_i + rho * dqf_plus;
-  double tmp2_m = q_i - 2.0 * dqf_plus;
-  double tmp2_p = q_i + 2.0 * dqf_minus;
-  if ((qa_tmp <= 0.0 || qb_tmp <= 0.0)) {
-    if (rho <= (1.0 - (1.0e-12))) {
-      qr = tmp_m;
-      ql = tmp_p;
-    }
-  } else {
-    if (fabs(dqf_minus) >= 2.0 * fabs(dqf_plus)) qr = tmp2_m;
-    if (fabs(dqf_plus) >= 2.0 * fabs(dqf_minus)) ql = tmp2_p;
-  }
-  qr_i = qr;
-  ql_ip1 = ql;
-}
-
-// One reconstruction over a window w[0..W-1] (cells m-GR .. m+GR) of field values.
-template <int RECON>
-__device__ __forceinline__ void recon1(const double *w, double &ql_ip1, double &qr_i) {
-  if constexpr (RECON == RECON_PPM) {
-    ppm(w[0], w[1], w[2], w[3], w[4], ql_ip1, qr_i);
-  } else if constexpr (RECON == RECON_PLM) {
-    plm(w[0], w[1], w[2], ql_ip1, qr_i);
-  } else {
-    ql_ip1 = w[0];
-    qr_i = w[0];
-  }
-}
-
-// ------------------------------------------------------------ sound speed
-__device__ __forceinline__ double sound_speed(double gamma, double igm1, double rho,
-                                              double p) {
-  return C_LIGHT * sqrt(gamma / (1 + (igm1 + rho * C_LIGHT2 / p)));
-}
-
-// ------------------------------------------------------------ HLLE
-// src/hydro/rsolvers/hydro_hlle.hpp:28-133; w = (rho, vn, vt1, vt2, p)
-__device__ __forceinline__ void hlle(const double *wl, const double *wr, const double *ifl,
-                                     const double *ifr, double *flx) {
-  double gammal = ifl[0], gammar = ifr[0];
-  double gm1l = ifl[1] - 1, gm1r = ifr[1] - 1;
-  double igm1l = 1.0 / gm1l, igm1r = 1.0 / gm1r;
-  double cl = sound_speed(gammal, igm1l, wl[0], wl[4]);
-  double cr = sound_speed(gammar, igm1r, wr[0], wr[4]);
-  double el = wl[4] * igm1l + 0.5 * wl[0] * (SQR(wl[1]) + SQR(wl[2]) + SQR(wl[3]));
-  double er = wr[4] * igm1r + 0.5 * wr[0] * (SQR(wr[1]) + SQR(wr[2]) + SQR(wr[3]));
-  double rhoa = .5 * (wl[0] + wr[0]);
-  double ca = .5 * (cl + cr);
-  double pmid = .5 * (wl[4] + wr[4] + (wl[1] - wr[1]) * rhoa * ca);
-  double ql = (pmid <= wl[4])
-                  ? 1.0
-                  : (1.0 + (gammal + 1) / sqrt(2 * gammal) * (pmid / wl[4] - 1.0));
-  double qr = (pmid <= wr[4])
-                  ? 1.0
-                  : (1.0 + (gammar + 1) / sqrt(2 * gammar) * (pmid / wr[4] - 1.0));
-  double al = wl[1] - cl * ql;
-  double ar = wr[1] + cr * qr;
-  double bp = ar > 0.0 ? ar : 0.0;
-  double bm = al < 0.0 ? al : 0.0;
-  double vxl = wl[1] - bm;
-  double vxr = wr[1] - bp;
-  double fl[5], fr[5];
-  fl[0] = wl[0] * vxl;
-  fr[0] = wr[0] * vxr;
-  fl[1] = wl[0] * wl[1] * vxl;
-  fr[1] = wr[0] * wr[1] * vxr;
-  fl[2] = wl[0] * wl[2] * vxl;
-  fr[2] = wr[0] * wr[2] * vxr;
-  fl[3] = wl[0] * wl[3] * vxl;
-  fr[3] = wr[0] * wr[3] * vxr;
-  fl[1] += wl[4];
-  fr[1] += wr[4];
-  fl[4] = el * vxl + wl[4] * wl[1];
-  fr[4] = er * vxr + wr[4] * wr[1];
-  double tmp = 0.0;
-  if (bp != bm) tmp = 0.5 * (bp + bm) / (bp - bm);
-#pragma unroll
-  for (int n = 0; n < 5; ++n) flx[n] = 0.5 * (fl[n] + fr[n]) + (fl[n] - fr[n]) * tmp;
-}
-
-// ------------------------------------------------------------ HLLC
-// src/hydro/rsolvers/hydro_hllc.hpp:161-282
-__device__ __forceinline__ void hllc(const double *wl, const double *wr, const double *ifl,
-                                     const double *ifr, double *flx) {
-  double gammal = ifl[0], gammar = ifr[0];
-  double gm1l = ifl[1] - 1, gm1r = ifr[1] - 1;
-  double igm1l = 1.0 / gm1l, igm1r = 1.0 / gm1r;
-  double cl = sound_speed(gammal, igm1l, wl[0], wl[4]);
-  double cr = sound_speed(gammar, igm1r, wr[0], wr[4]);
-  double el = wl[4] * igm1l + 0.5 * wl[0] * (SQR(wl[1]) + SQR(wl[2]) + SQR(wl[3]));
-  double er = wr[4] * igm1r + 0.5 * wr[0] * (SQR(wr[1]) + SQR(wr[2]) + SQR(wr[3]));
-  double rhoa = .5 * (wl[0] + wr[0]);
-  double ca = .5 * (cl + cr);
-  double pstar = .5 * (wl[4] + wr[4]) - .5 * (wr[1] - wl[1]) * rhoa * ca;
-  double ql = (pstar <= wl[4])
-                  ? 1.0
-                  : sqrt(1.0 + (gammal + 1) / (2 * gammar) * (pstar / wl[4] - 1.0));
-  double qr = (pstar <= wr[4])
-                  ? 1.0
-                  : sqrt(1.0 + (gammal + 1) / (2 * gammar) * (pstar / wr[4] - 1.0));
-  double sl = wl[1] - cl * ql;
-  double sr = wr[1] + cr * qr;
-  double bp = sr > 0.0 ? sr : (TINY_NUMBER);
-  double bm = sl < 0.0 ? sl : -(TINY_NUMBER);
-  double vxl = wl[1] - sl;
-  double vxr = wr[1] - sr;
-  double tl = wl[4] + vxl * wl[0] * wl[1];
-  double tr = wr[4] + vxr * wr[0] * wr[1];
-  double ml = wl[0] * vxl;
-  double mr = -(wr[0] * vxr);
-  double sstar = (tl - tr) / (ml + mr);
-  double cp = (ml * tr + mr * tl) / (ml + mr);
-  cp = cp > 0.0 ? cp : 0.0;
-  vxl = wl[1] - bm;
-  vxr = wr[1] - bp;
-  double fl[5], fr[5];
-  fl[0] = wl[0] * vxl;
-  fr[0] = wr[0] * vxr;
-  fl[1] = wl[0] * wl[1] * vxl + wl[4];
-  fr[1] = wr[0] * wr[1] * vxr + wr[4];
-  fl[2] = wl[0] * wl[2] * vxl;
-  fr[2] = wr[0] * wr[2] * vxr;
-  fl[3] = wl[0] * wl[3] * vxl;
-  fr[3] = wr[0] * wr[3] * vxr;
-  fl[4] = el * vxl + wl[4] * wl[1];
-  fr[4] = er * vxr + wr[4] * wr[1];
-  double ur, ul, ustar;
-  if (sstar >= 0.0) {
-    ul = sstar / (sstar - bm);
-    ur = 0.0;
-    ustar = -bm / (sstar - bm);
-  } else {
-    ul = 0.0;
-    ur = -sstar / (bp - sstar);
-    ustar = bp / (bp - sstar);
-  }
-  flx[0] = ul * fl[0] + ur * fr[0];
-  flx[1] = ul * fl[1] + ur * fr[1] + ustar * cp;
-  flx[2] = ul * fl[2] + ur * fr[2];
-  flx[3] = ul * fl[3] + ur * fr[3];
-  flx[4] = ul * fl[4] + ur * fr[4] + ustar * cp * sstar;
-}
-
-template <int RS>
-__device__ __forceinline__ void riemann(const double *wl, const double *wr, const double *ifl,
-                                        const double *ifr, double *flx) {
-  if constexpr (RS == RS_HLLC)
-    hllc(wl, wr, ifl, ifr, flx);
-  else
-    hlle(wl, wr, ifl, ifr, flx);
-}
-
-// ------------------------------------------------------------ ConsToPrim point
-// src/hydro/hydro.cpp:474-531 with singularity IdealGas closed forms (upstream).
-// u[NV] in (floored in place), w[NV] out, gm[2] out.
-template <int NV>
-__device__ __forceinline__ void cons_to_prim_pt(double *u, const aph_eos &eos, double *w,
-                                                double *gm) {
-  const double gm1e = eos.gm1;
-  double u_d = u[0];
-  u_d = (u_d > eos.dfloor) ? u_d : eos.dfloor;
-  u[0] = u_d;
-  w[0] = u_d;
-  double di = 1.0 / u_d;
-  w[1] = u[1] * di;
-  w[2] = u[2] * di;
-  w[3] = u[3] * di;
-  double e_k = 0.5 * di * (SQR(u[1]) + SQR(u[2]) + SQR(u[3]));
-  double u_e = u[4];
-  double sie = (u_e - e_k) / u_d;
-  double w_p = mx(EOS_SMALL, gm1e * u_d * sie);
-  if (!(w_p > eos.pfloor)) {  // pressure floor, energy fix (:517-521)
-    double gm1 = w_p / (u_d * e_k);
-    u_e = ((eos.pfloor / gm1) + e_k);
-    w_p = eos.pfloor;
-    sie = (u_e - e_k) / u_d;
-  }
-  u[4] = u_e;
-  w[4] = w_p;
-  double bulk = mx(EOS_SMALL, gm1e * (gm1e + 1.0) * u_d * sie);
-  gm[0] = bulk / w_p;
-  gm[1] = w_p / (u_e - e_k) + 1;
-#pragma unroll
-  for (int n = 5; n < NV; ++n) w[n] = u[n] * di;
-}
-
-// EstimateTimestep point (hydro.cpp:384-415): returns min over active axes of
-// dx_d / (|v_d| + lambda).
-__device__ __forceinline__ double dt_point(const double *w, double cons_en, const aph_eos &eos,
-                                           const double *dx, int ndim) {
-  const double gm1e = eos.gm1;
-  double e_internal = (cons_en - 0.5 * w[0] * (SQR(w[1]) + SQR(w[2]) + SQR(w[3]))) / w[0];
-  double bulkmod = mx(EOS_SMALL, gm1e * (gm1e + 1.0) * w[0] * e_internal);
-  const double gamma_c = bulkmod / w[4];
-  double lam = C_LIGHT * sqrt(gamma_c / (1 + (e_internal + C_LIGHT2) * w[0] / w[4]));
-  double m = fmin(1.79769313486231570815e308, dx[0] / (fabs(w[1]) + lam));
-  if (ndim > 1) m = fmin(m, dx[1] / (fabs(w[2]) + lam));
-  if (ndim > 2) m = fmin(m, dx[2] / (fabs(w[3]) + lam));
-  return m;
 }
 
 // Positive-double atomic min through the IEEE bit pattern (monotone for x >= 0).

@@ -21,6 +21,7 @@
 #include <cstring>
 #include <vector>
 
+#include "aph_adjoint_host.hpp"
 #include "aph_device.hpp"
 #include "aph_march.hpp"
 #include "aph_runtime.hpp"
@@ -328,6 +329,7 @@ struct aph_plan {
   bool dirty = true;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  aph::AdjScratch adj;
 };
 
 namespace {
@@ -568,6 +570,7 @@ int aph_plan_destroy(aph_context *ctx, aph_plan *p) {
   if (p->graph) (void)hipGraphDestroy(p->graph);
   if (p->d_items) (void)hipFree(p->d_items);
   if (p->d_est) (void)hipFree(p->d_est);
+  adj_release(p->adj);
   delete p;
   return APH_OK;
 }
@@ -693,6 +696,43 @@ int aph_plan_estimate_timestep(aph_context *ctx, aph_plan *p, const double *cons
   }
   APH_LAUNCH_CHECK(ctx);
   return APH_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int aph_adjoint_stage(aph_context *ctx, aph_plan *p, int32_t stage, const double *u_in,
+                      const double *dt_dev, const double *lam_out, const double *lam_acc,
+                      double *lam_in, double *lam_n) {
+  if (!ctx) return APH_E_INVALID;
+  if (!p || !u_in || !dt_dev || !lam_out || !lam_in || (stage != 1 && stage != 2))
+    return set_error(ctx, APH_E_INVALID, "aph_adjoint_stage: bad arguments");
+  if (lam_in == lam_out || (lam_acc && lam_acc == lam_in))
+    return set_error(ctx, APH_E_INVALID, "aph_adjoint_stage: lam_in must not alias inputs");
+  if (!adj_supported(p->recon, p->rsolver, p->bs->nvar, p->bs->ndim))
+    return set_error(ctx, APH_E_INVALID, "adjoint: unsupported (recon, riemann, nvar)");
+  return adj_stage(ctx, p->bs, p->recon, p->rsolver, p->eos, p->g, p->grav, p->adj, stage,
+                   u_in, dt_dev, lam_out, lam_acc, lam_in, lam_n);
+}
+
+int aph_adjoint_step(aph_context *ctx, aph_plan *p, double *u0, double *u1,
+                     const double *dt_dev, double *lam) {
+  if (!ctx) return APH_E_INVALID;
+  if (!p || !u0 || !u1 || !dt_dev || !lam || u0 == u1)
+    return set_error(ctx, APH_E_INVALID, "aph_adjoint_step: bad arguments");
+  if (!adj_supported(p->recon, p->rsolver, p->bs->nvar, p->bs->ndim))
+    return set_error(ctx, APH_E_INVALID, "adjoint: unsupported (recon, riemann, nvar)");
+  int rc = adj_prepare(ctx, p->bs, p->adj);
+  if (rc) return rc;
+  // recompute the stage-1 state from the step's checkpoint, then stage 2 and stage 1
+  rc = launch_stage(ctx, p, 1, u0, nullptr, u1, dt_dev, 0.0);
+  if (rc) return rc;
+  rc = adj_stage(ctx, p->bs, p->recon, p->rsolver, p->eos, p->g, p->grav, p->adj, 2, u1,
+                 dt_dev, lam, nullptr, p->adj.lam1, p->adj.lam0p);
+  if (rc) return rc;
+  return adj_stage(ctx, p->bs, p->recon, p->rsolver, p->eos, p->g, p->grav, p->adj, 1, u0,
+                   dt_dev, p->adj.lam1, p->adj.lam0p, lam, nullptr);
 }
 
 }  // extern "C"

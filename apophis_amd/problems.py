@@ -211,7 +211,7 @@ def kh2d(nblk=16, n=32, seed=11, amps=None, a=0.01, sigma2=0.01, drat=2.0,
     if amps is None:
         amps = 0.01 * uniform(seed, np.arange(8, dtype=np.uint64))
     amps = np.asarray(amps, dtype=np.float64)
-    m = lattice_mesh(2, (nblk, nblk, 1), (n, n, 1), 2, (0.0, -0.5, -0.5),
+    m = lattice_mesh(2, (nblk, nblk, 1), (n, n, 1), NEED_GHOST[recon], (0.0, -0.5, -0.5),
                      (1.0, 0.5, 0.5), ["periodic"] * 6)
     bs = BlockSet(2, [m], nscalars=1)
     cons = np.zeros(bs.field_size(bs.nvar))

@@ -194,6 +194,24 @@ int aph_plan_graph_launch(aph_context *ctx, aph_plan *plan);
 int aph_plan_estimate_timestep(aph_context *ctx, aph_plan *plan, const double *cons,
                                double cfl, double *dt_out);
 
+/* ----------------------------------------------------- adjoint (frozen dt)
+ * Reverse-mode counterparts of the fused stage (no reference counterpart: the
+ * reference has no adjoint).  Given the cotangent lam_out of a stage's output:
+ *   stage 1:  lam_in = (0 + 1) * lam_out + J^T lam_out [+ lam_acc]
+ *   stage 2:  lam_in = 0.5 * lam_out + J^T lam_out [+ lam_acc],  lam_n = 0.5 * lam_out
+ * where J is the Jacobian of the flux-divergence (+ gravity) term with respect to the
+ * stage input u_in (ConsToPrim, reconstruction through the limiter's active branch, the
+ * Riemann solver's active wave branches).  Only interior cells of the cotangent arrays
+ * are read or written.  lam_acc and lam_n may be NULL. */
+int aph_adjoint_stage(aph_context *ctx, aph_plan *plan, int32_t stage, const double *u_in,
+                      const double *dt_dev, const double *lam_out, const double *lam_acc,
+                      double *lam_in, double *lam_n);
+/* One RK2 step backwards: u0 is the step's input state (checkpoint), u1 scratch of the
+ * same size (receives the recomputed stage-1 state), lam holds the cotangent of the
+ * step's output on entry and that of u0 on return. */
+int aph_adjoint_step(aph_context *ctx, aph_plan *plan, double *u0, double *u1,
+                     const double *dt_dev, double *lam);
+
 #ifdef __cplusplus
 }
 #endif

@@ -412,6 +412,9 @@ int orc_calculate_fluxes(const aph_block *blocks, int nb, int ndim, int nvar, in
                          double *fx1, double *fx2, double *fx3) {
   if (recon != APH_RECON_DC && recon != APH_RECON_PLM && recon != APH_RECON_PPM) return -1;
   if (rsolver != APH_RIEMANN_HLLE && rsolver != APH_RIEMANN_HLLC) return -1;
+  const int need = recon == APH_RECON_PPM ? 3 : (recon == APH_RECON_PLM ? 2 : 1);
+  for (int b = 0; b < nb; ++b)
+    if (blocks[b].ng < need) return -2; /* hydro.cpp:72-75 */
   int nmax = 0;
   for (int b = 0; b < nb; ++b)
     for (int d = 0; d < 3; ++d) {

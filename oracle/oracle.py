@@ -123,6 +123,9 @@ class State:
 
     def step(self, dt):
         p = self.prob
+        need = {"dc": 1, "plm": 2, "ppm": 3}[p.recon]
+        if int(self.bs.blocks["ng"].min()) < need:
+            raise ValueError("Not enough ghost zones for reconstruction")
         return lib().orc_rk2_step(_bp(self.bs), self.bs.nblocks, self.bs.ndim, self.bs.nvar,
                                   RECON[p.recon], RIEMANN[p.riemann],
                                   p.eos.ctypes.data_as(_vp), _p(self.g), p.cfl, dt,
@@ -161,6 +164,8 @@ def calculate_fluxes(bs, recon, riemann_, prim, gamma, fx1, fx2, fx3):
     rc = lib().orc_calculate_fluxes(_bp(bs), bs.nblocks, bs.ndim, bs.nvar, RECON[recon],
                                     RIEMANN[riemann_], _p(prim), _p(gamma), _p(fx1),
                                     _p(fx2), _p(fx3))
+    if rc == -2:
+        raise ValueError("Not enough ghost zones for reconstruction")
     if rc != 0:
         raise ValueError("unsupported reconstruction/Riemann combination")
 

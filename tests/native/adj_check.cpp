@@ -1,0 +1,36 @@
+// Host build of the product's point and adjoint functions (apophis_amd/csrc/aph_point.hpp,
+// aph_adjoint.hpp) with C entry points, so CPU tests can check every adjoint against
+// central finite differences of its forward function.  Test infrastructure only.
+#include "../../apophis_amd/csrc/aph_adjoint.hpp"
+
+using namespace aph;
+
+extern "C" {
+void chk_ppm(const double *q, double *out) { ppm(q[0], q[1], q[2], q[3], q[4], out[0], out[1]); }
+void chk_ppm_adj(const double *q, const double *ob, double *qb) { ppm_adj(q, ob[0], ob[1], qb); }
+void chk_plm(const double *q, double *out) { plm(q[0], q[1], q[2], out[0], out[1]); }
+void chk_plm_adj(const double *q, const double *ob, double *qb) { plm_adj(q, ob[0], ob[1], qb); }
+// x = (wl[5], wr[5], ifl[2], ifr[2]) -> flux[5]
+void chk_riemann(int kind, const double *x, double *f) {
+  if (kind == RS_HLLC)
+    hllc(x, x + 5, x + 10, x + 12, f);
+  else
+    hlle(x, x + 5, x + 10, x + 12, f);
+}
+void chk_riemann_adj(int kind, const double *x, const double *fb, double *xb) {
+  if (kind == RS_HLLC)
+    hllc_adj(x, x + 5, x + 10, x + 12, fb, xb, xb + 5, xb + 10, xb + 12);
+  else
+    hlle_adj(x, x + 5, x + 10, x + 12, fb, xb, xb + 5, xb + 10, xb + 12);
+}
+// u[6] -> (w[6], gamma[2]) ; eos = (gm1, cv, dfloor, pfloor)
+void chk_c2p(const double *u, const double *e, double *out) {
+  aph_eos eos{e[0], e[1], e[2], e[3]};
+  double uu[6] = {u[0], u[1], u[2], u[3], u[4], u[5]};
+  cons_to_prim_pt<6>(uu, eos, out, out + 6);
+}
+void chk_c2p_adj(const double *u, const double *e, const double *ob, double *ub) {
+  aph_eos eos{e[0], e[1], e[2], e[3]};
+  cons_to_prim_adj<6>(u, eos, ob, ob + 6, ub);
+}
+}

@@ -78,15 +78,15 @@ def rollout(u0, stepper, nsteps, checkpoint_every=20):
     return _Rollout.apply(u0, stepper, int(nsteps), int(checkpoint_every))
 
 
-_MASKS = {}
-
-
 def interior_mask_tensor(bs, device):
-    key = (id(bs), str(device))
-    if key not in _MASKS:
+    """1.0 on interior cells, 0.0 on ghost storage, over all nvar components (cached on
+    the block set object)."""
+    cache = bs.__dict__.setdefault("_mask_tensors", {})
+    key = str(device)
+    if key not in cache:
         m = np.tile(bs.interior_mask(), bs.nvar).astype(np.float64)
-        _MASKS[key] = torch.from_numpy(m).to(device)
-    return _MASKS[key]
+        cache[key] = torch.from_numpy(m).to(device)
+    return cache[key]
 
 
 # ------------------------------------------------------------------ KH as f(a)

@@ -257,3 +257,96 @@ def stage(u_in, u_n, gam0, gam1, beta_dt, recon, riemann, eos, dx, ndim):
 def rk2_step(u, dt, recon, riemann, eos, dx, ndim):
     u1 = stage(u, u, 0.0, 1.0, 1.0 * dt, recon, riemann, eos, dx, ndim)
     return stage(u1, u, 0.5, 0.5, 0.5 * dt, recon, riemann, eos, dx, ndim)
+
+
+def _pad(u, ax, G, kind, d):
+    """Pad conserved state u along tensor axis ax by G cells per side with the boundary
+    rule of spatial axis d: periodic, outflow (copy edge) or reflect (mirror, negate the
+    normal momentum)."""
+    n = u.shape[ax]
+    if kind == "periodic":
+        lo, hi = u.narrow(ax, n - G, G), u.narrow(ax, 0, G)
+    elif kind == "outflow":
+        lo = u.narrow(ax, 0, 1).expand(*[G if a == ax else -1 for a in range(u.dim())])
+        hi = u.narrow(ax, n - 1, 1).expand(*[G if a == ax else -1 for a in range(u.dim())])
+    else:
+        sign = torch.ones(u.shape[0], 1, 1, 1, dtype=u.dtype)
+        sign[1 + d] = -1.0
+        lo = torch.flip(u.narrow(ax, 0, G), dims=[ax]) * sign
+        hi = torch.flip(u.narrow(ax, n - G, G), dims=[ax]) * sign
+    return torch.cat([lo, u, hi], dim=ax)
+
+
+def stage_bc(u_in, u_n, gam0, gam1, beta_dt, recon, riemann, eos, dx, ndim, bc, g=None):
+    """One fused stage on a single mesh with per-axis boundary rules bc[d] in
+    {"periodic","outflow","reflect"} (applied to the conserved state before ConsToPrim,
+    like the fused kernel) and an optional uniform gravity g (stage-start primitives)."""
+    gm1e, dfloor, pfloor = eos
+    nv = u_in.shape[0]
+    G = 3
+    area = (dx[1] * dx[2], dx[0] * dx[2], dx[0] * dx[1])
+    vol = dx[0] * dx[1] * dx[2]
+    du = None
+    for d in range(ndim):
+        ax = 3 - d
+        n = u_in.shape[ax]
+        U = _pad(u_in, ax, G, bc[d], d)
+        w, gm = cons_to_prim(U, gm1e, dfloor, pfloor)
+        q = torch.cat([w, gm])
+        tk = lambda k: q.narrow(ax, G - 1 + k, n + 2)  # stencil offset k of cells -1..n
+        if recon == "ppm":
+            ql, qr = ppm(tk(-2), tk(-1), tk(0), tk(1), tk(2))
+        else:
+            ql, qr = plm(tk(-1), tk(0), tk(1))
+        WL = ql.narrow(ax, 0, n + 1)   # ql output of cell f-1, faces f = 0..n
+        WR = qr.narrow(ax, 1, n + 1)   # qr output of cell f
+        ivx, ivy, ivz = 1 + d, 1 + (1 + d) % 3, 1 + (2 + d) % 3
+        wl = [WL[0], WL[ivx], WL[ivy], WL[ivz], WL[4]]
+        wr = [WR[0], WR[ivx], WR[ivy], WR[ivz], WR[4]]
+        f = (hllc if riemann == "hllc" else hlle)(wl, wr, WL[nv:], WR[nv:])
+        F = [None] * nv
+        F[0], F[ivx], F[ivy], F[ivz], F[4] = f
+        for m in range(5, nv):
+            F[m] = torch.where(F[0] >= 0.0, F[0] * WL[m], F[0] * WR[m])
+        F = torch.stack(F)
+        dd = area[d] * F.narrow(ax, 1, n) - area[d] * F.narrow(ax, 0, n)
+        du = dd if du is None else du + dd
+    div = -du / vol
+    out = gam0 * u_in + gam1 * u_n + beta_dt * div
+    if g is not None and any(v != 0.0 for v in g):
+        w, _ = cons_to_prim(u_in, gm1e, dfloor, pfloor)
+        parts = [out[v] for v in range(nv)]
+        parts[1] = parts[1] + g[0] * w[0] * beta_dt
+        de = g[0] * w[1]
+        if ndim >= 2:
+            parts[2] = parts[2] + g[1] * w[0] * beta_dt
+            de = de + g[1] * w[2]
+        if ndim >= 3:
+            parts[3] = parts[3] + g[2] * w[0] * beta_dt
+            de = de + g[2] * w[3]
+        parts[4] = parts[4] + de * w[0] * beta_dt
+        out = torch.stack(parts)
+    return out
+
+
+def rk2_step_bc(u, dt, recon, riemann, eos, dx, ndim, bc, g=None):
+    u1 = stage_bc(u, u, 0.0, 1.0, 1.0 * dt, recon, riemann, eos, dx, ndim, bc, g)
+    return stage_bc(u1, u, 0.5, 0.5, 0.5 * dt, recon, riemann, eos, dx, ndim, bc, g)
+
+
+def lattice_to_global(bs, field):
+    """Interior of a single lattice mesh (pack layout) as one (NV, nz, ny, nx) array."""
+    import numpy as np
+
+    nv = bs.nvar
+    bx = bs.blocks["nx"][0]
+    dx = bs.blocks["dx"][0]
+    x0 = bs.blocks["xmin"].min(axis=0)
+    pos = np.rint((bs.blocks["xmin"] - x0) / (dx * bx)).astype(int)
+    shape = [(pos[:, d].max() + 1) * bx[d] for d in (2, 1, 0)]
+    out = np.zeros((nv, *shape))
+    for b in range(bs.nblocks):
+        lx, ly, lz = pos[b]
+        out[:, lz * bx[2]:(lz + 1) * bx[2], ly * bx[1]:(ly + 1) * bx[1],
+            lx * bx[0]:(lx + 1) * bx[0]] = bs.interior(field, b, nv)
+    return out

@@ -98,10 +98,76 @@ def cpu_baseline(threads=None):
                       f"{nsteps} steps after 1 warm-up, {el:.1f} s, OMP threads={threads}"}
 
 
+def adjoint_leg(ctx, hydro, problems, torch):
+    """Kelvin-Helmholtz 512^2 (16x16 blocks of 32^2, PLM+HLLC, one passive scalar): time
+    of a frozen-dt forward RK2 step vs one adjoint step (stage-1 recompute + both stage
+    adjoints), and of a whole 200-step rollout with checkpoints every 20 steps, forward
+    and backward (segment recomputation included)."""
+    from apophis_amd import adjoint
+    from oracle import oracle as orc
+
+    p = problems.kh2d(nblk=16, n=32)
+    dt = orc.State(problems.kh2d(nblk=1, n=32)).estimate_dt() * 0.5  # frozen, CFL-safe
+    fd = hydro.FusedDriver(ctx, p.bs, p.cons, p.recon, p.riemann, p.eos, p.cfl)
+    st = adjoint.Stepper(fd, dt)
+    u = torch.from_numpy(p.cons.copy()).cuda()
+    n = 20
+    states = []
+    for _ in range(3):
+        st.step_(u)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(n):
+        states.append(u.clone())
+        st.step_(u)
+    torch.cuda.synchronize()
+    t_fwd = (time.perf_counter() - t0) / n
+    lam = torch.ones_like(u) * adjoint.interior_mask_tensor(p.bs, u.device)
+    st.adjoint_step_(states[-1], lam)  # allocate scratch, warm up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(n - 1, -1, -1):
+        st.adjoint_step_(states[k], lam)
+    torch.cuda.synchronize()
+    t_adj = (time.perf_counter() - t0) / n
+    amps = torch.from_numpy(p.meta["amps"]).cuda().requires_grad_(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    u0 = adjoint.kh_state(p.bs, amps)
+    uN = adjoint.rollout(u0, st, 200, 20)
+    loss = adjoint.scalar_variance(uN, p.bs)
+    torch.cuda.synchronize()
+    t_roll_f = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    loss.backward()
+    torch.cuda.synchronize()
+    t_roll_b = time.perf_counter() - t0
+    return {"workload": "KH 512^2 (256 blocks of 32^2), PLM+HLLC + scalar, frozen dt",
+            "forward_step_ms": t_fwd * 1e3, "adjoint_step_ms": t_adj * 1e3,
+            "adjoint_over_forward": t_adj / t_fwd,
+            "rollout_200_forward_s": t_roll_f, "rollout_200_backward_s": t_roll_b,
+            "rollout_backward_over_forward": t_roll_b / t_roll_f,
+            "finite_grad": bool(torch.isfinite(amps.grad).all().item())}
+
+
 def profile_run(args, ctx, hydro, problems, torch):
     """Eager launches for rocprofv3: 3 RK2 steps of the chosen form on the headline
     workload (or the uneven block set), then a 1 GiB device-to-device copy repeated 3
     times, whose FETCH_SIZE / WRITE_SIZE calibrate the counters (2^30 bytes each way)."""
+    if args.profile_run == "adjoint":
+        from apophis_amd import adjoint
+
+        p = problems.kh2d(nblk=16, n=32)
+        fd = hydro.FusedDriver(ctx, p.bs, p.cons, p.recon, p.riemann, p.eos, p.cfl)
+        st = adjoint.Stepper(fd, 1e-4)
+        u = torch.from_numpy(p.cons.copy()).cuda()
+        lam = torch.ones_like(u) * adjoint.interior_mask_tensor(p.bs, u.device)
+        for _ in range(3):
+            st.step_(u)
+            st.adjoint_step_(u, lam)
+        torch.cuda.synchronize()
+        print("profile-run adjoint: done", flush=True)
+        return
     if args.profile_run == "uneven":
         p = problems.uneven_blocks(nblocks=args.uneven_blocks)
     else:
@@ -139,7 +205,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip plain/uneven legs")
     ap.add_argument("--uneven-blocks", type=int, default=4096)
-    ap.add_argument("--profile-run", choices=["fused", "plain", "uneven"], default=None,
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the KH adjoint leg")
+    ap.add_argument("--profile-run", choices=["fused", "plain", "uneven", "adjoint"],
+                    default=None,
                     help="short eager run for rocprofv3 (no timing, no JSON)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -253,6 +321,9 @@ def main():
               flush=True)
         del dq, mq
 
+    if not args.no_adjoint:
+        extras["adjoint_kh"] = adjoint_leg(ctx, hydro, problems, torch)
+        print(f"# adjoint: {extras['adjoint_kh']}", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline()
     out = {
         "metric": "zone-cycles/s + % HBM roofline on 1 GPU (batched vs plain, FP64); "

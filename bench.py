@@ -104,11 +104,10 @@ def adjoint_leg(ctx, hydro, problems, torch):
     adjoints), and of a whole 200-step rollout with checkpoints every 20 steps, forward
     and backward (segment recomputation included)."""
     from apophis_amd import adjoint
-    from oracle import oracle as orc
 
     p = problems.kh2d(nblk=16, n=32)
-    dt = orc.State(problems.kh2d(nblk=1, n=32)).estimate_dt() * 0.5  # frozen, CFL-safe
     fd = hydro.FusedDriver(ctx, p.bs, p.cons, p.recon, p.riemann, p.eos, p.cfl)
+    dt = float(fd.initial_dt().item())  # CFL 0.3 estimate of the initial state, frozen
     st = adjoint.Stepper(fd, dt)
     u = torch.from_numpy(p.cons.copy()).cuda()
     n = 20
@@ -132,8 +131,9 @@ def adjoint_leg(ctx, hydro, problems, torch):
     t_adj = (time.perf_counter() - t0) / n
     amps = torch.from_numpy(p.meta["amps"]).cuda().requires_grad_(True)
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
     u0 = adjoint.kh_state(p.bs, amps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
     uN = adjoint.rollout(u0, st, 200, 20)
     loss = adjoint.scalar_variance(uN, p.bs)
     torch.cuda.synchronize()
@@ -142,7 +142,7 @@ def adjoint_leg(ctx, hydro, problems, torch):
     loss.backward()
     torch.cuda.synchronize()
     t_roll_b = time.perf_counter() - t0
-    return {"workload": "KH 512^2 (256 blocks of 32^2), PLM+HLLC + scalar, frozen dt",
+    return {"workload": "KH 512^2 (256 blocks of 32^2), PLM+HLLC + scalar, frozen dt", "dt": dt,
             "forward_step_ms": t_fwd * 1e3, "adjoint_step_ms": t_adj * 1e3,
             "adjoint_over_forward": t_adj / t_fwd,
             "rollout_200_forward_s": t_roll_f, "rollout_200_backward_s": t_roll_b,

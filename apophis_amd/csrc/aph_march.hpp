@@ -28,8 +28,13 @@ __device__ __forceinline__ void march(int f_lo, int f_hi, Load &&load, Emit &&em
   constexpr int IVZ = 1 + (IVX + 1) % 3;
   double win[W][NF];
   double qlp[NF];
+  double fv[NF];  // PPM: limited value of the face below the cell being reconstructed
 #pragma unroll
   for (int s = 0; s < W - 1; ++s) load(f_lo - 1 - GR + s, win[s + 1]);
+  if constexpr (RECON == RECON_PPM) {
+#pragma unroll
+    for (int f = 0; f < NF; ++f) fv[f] = ppm_face(win[1][f], win[2][f], win[3][f], win[4][f]);
+  }
   for (int m = f_lo - 1; m <= f_hi; ++m) {
 #pragma unroll
     for (int s = 0; s < W - 1; ++s)
@@ -39,10 +44,18 @@ __device__ __forceinline__ void march(int f_lo, int f_hi, Load &&load, Emit &&em
     double ql[NF], qr[NF];
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
-      double col[W];
+      if constexpr (RECON == RECON_PPM) {
+        // face m+1/2 once; cell m's own face m-1/2 was the previous step's upper face
+        const double up = ppm_face(win[1][f], win[2][f], win[3][f], win[4][f]);
+        ppm_cell(win[0][f], win[1][f], win[2][f], win[3][f], win[4][f], fv[f], up, ql[f],
+                 qr[f]);
+        fv[f] = up;
+      } else {
+        double col[W];
 #pragma unroll
-      for (int s = 0; s < W; ++s) col[s] = win[s][f];
-      recon1<RECON>(col, ql[f], qr[f]);
+        for (int s = 0; s < W; ++s) col[s] = win[s][f];
+        recon1<RECON>(col, ql[f], qr[f]);
+      }
     }
     if (m >= f_lo) {
       const double wl[5] = {qlp[0], qlp[IVX], qlp[IVY], qlp[IVZ], qlp[4]};

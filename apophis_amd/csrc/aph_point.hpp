@@ -124,6 +124,71 @@ APH_HD void ppm(double q_im2, double q_im1, double q_i, double q_ip1,
   ql_ip1 = ql;
 }
 
+// Face-shared PPM.  The limited interface value of face i-1/2 (steps 1-2a of
+// ppm_simple.hpp:40-93) uses cells i-2..i+1 only, and cell i-1 (as its "dph_ip1") and
+// cell i (as its "dph") evaluate it with the same operands in the same order, so it is
+// bitwise the same value: ppm_face computes it once, ppm_cell finishes steps 3-4 of one
+// cell from its two face values.  ppm_cell(q, ppm_face(q[0..3]), ppm_face(q[1..4]))
+// equals ppm(q) bitwise.
+APH_HD double ppm_face(double q_im2, double q_im1, double q_i, double q_ip1) {
+  const double C2 = 1.25;
+  const double qa0 = (q_i - q_im1);
+  const double qb0 = (q_ip1 - q_i);
+  const double dd_im1 = 0.5 * qa0 + 0.5 * (q_im1 - q_im2);
+  const double dd = 0.5 * qb0 + 0.5 * qa0;
+  double dph = 0.5 * (q_im1 + q_i) + (dd_im1 - dd) / 6.0;
+  const double d2qc_im1 = q_im2 + q_i - 2.0 * q_im1;
+  const double d2qc = q_im1 + q_ip1 - 2.0 * q_i;
+  const double qa_tmp = dph - q_im1;
+  const double qb_tmp = q_i - dph;
+  const double qa = 3.0 * (q_im1 + q_i - 2.0 * dph);
+  double qd = 0.0;
+  if (SIGN(qa) == SIGN(d2qc_im1) && SIGN(qa) == SIGN(d2qc))
+    qd = SIGN(qa) * mn(C2 * fabs(d2qc_im1), mn(C2 * fabs(d2qc), fabs(qa)));
+  const double dph_tmp = 0.5 * (q_im1 + q_i) - qd / 6.0;
+  if (qa_tmp * qb_tmp < 0.0) dph = dph_tmp;
+  return dph;
+}
+
+APH_HD void ppm_cell(double q_im2, double q_im1, double q_i, double q_ip1, double q_ip2,
+                     double dph, double dph_ip1, double &ql_ip1, double &qr_i) {
+  const double C2 = 1.25;
+  const double d2qc_im1 = q_im2 + q_i - 2.0 * q_im1;
+  const double d2qc = q_im1 + q_ip1 - 2.0 * q_i;
+  const double d2qc_ip1 = q_i + q_ip2 - 2.0 * q_ip1;
+  const double d2qf = 6.0 * (dph + dph_ip1 - 2.0 * q_i);
+  double qr = dph;
+  double ql = dph_ip1;
+  const double dqf_minus = q_i - qr;
+  const double dqf_plus = ql - q_i;
+  const double qa_tmp = dqf_minus * dqf_plus;
+  const double qb_tmp = (q_ip1 - q_i) * (q_i - q_im1);
+  double qe = 0.0;
+  if (SIGN(d2qc_im1) == SIGN(d2qc) && SIGN(d2qc_im1) == SIGN(d2qc_ip1) &&
+      SIGN(d2qc_im1) == SIGN(d2qf))
+    qe = SIGN(d2qf) * mn(mn(C2 * fabs(d2qc_im1), C2 * fabs(d2qc)),
+                         mn(C2 * fabs(d2qc_ip1), fabs(d2qf)));
+  const double qa = mx(fabs(q_im1), fabs(q_im2));
+  const double qb = mx(mx(fabs(q_i), fabs(q_ip1)), fabs(q_ip2));
+  double rho = 0.0;
+  if (fabs(d2qf) > (1.0e-12) * mx(qa, qb)) rho = qe / d2qf;
+  const double tmp_m = q_i - rho * dqf_minus;
+  const double tmp_p = q_i + rho * dqf_plus;
+  const double tmp2_m = q_i - 2.0 * dqf_plus;
+  const double tmp2_p = q_i + 2.0 * dqf_minus;
+  if ((qa_tmp <= 0.0 || qb_tmp <= 0.0)) {
+    if (rho <= (1.0 - (1.0e-12))) {
+      qr = tmp_m;
+      ql = tmp_p;
+    }
+  } else {
+    if (fabs(dqf_minus) >= 2.0 * fabs(dqf_plus)) qr = tmp2_m;
+    if (fabs(dqf_plus) >= 2.0 * fabs(dqf_minus)) ql = tmp2_p;
+  }
+  qr_i = qr;
+  ql_ip1 = ql;
+}
+
 // One reconstruction over a window w[0..W-1] (cells m-GR .. m+GR) of field values.
 template <int RECON>
 APH_HD void recon1(const double *w, double &ql_ip1, double &qr_i) {

@@ -34,3 +34,8 @@ void chk_c2p_adj(const double *u, const double *e, const double *ob, double *ub)
   cons_to_prim_adj<6>(u, eos, ob, ob + 6, ub);
 }
 }
+extern "C" void chk_ppm_shared(const double *q, double *out) {
+  const double lo = ppm_face(q[0], q[1], q[2], q[3]);
+  const double hi = ppm_face(q[1], q[2], q[3], q[4]);
+  ppm_cell(q[0], q[1], q[2], q[3], q[4], lo, hi, out[0], out[1]);
+}

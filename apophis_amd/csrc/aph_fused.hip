@@ -145,9 +145,13 @@ __device__ __forceinline__ void sweep(const FusedArgs &a, const aph_block &bk, i
     const int f_lo = t0[D] + seg * cfg.s[D];
     const int f_hi = f_lo + cfg.s[D];
     double fprev[NV];
-    auto load = [&](int c, double *dst) {
-      double u[NV];
+    auto fetch = [&](int c, double *u) {
       load_cell<NV>(a, bk, nb, D, n_d, c, tbase, st[D], nc, u);
+    };
+    auto convert = [&](const double *raw, double *dst) {
+      double u[NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) u[v] = raw[v];
       cons_to_prim_pt<NV>(u, a.eos, dst, dst + NV);
     };
     auto emit = [&](int f, const double *fo, const double *, const double *) {
@@ -174,14 +178,16 @@ __device__ __forceinline__ void sweep(const FusedArgs &a, const aph_block &bk, i
           const double *pin = a.u_in + nb.self + cidx;
 #pragma unroll
           for (int v = 0; v < NV; ++v) u[v] = pin[v * nc];
-          cons_to_prim_pt<NV>(u, a.eos, w, gm);
+          if (a.grav)  // the gravity source needs the stage-start primitives
+            cons_to_prim_pt<NV>(u, a.eos, w, gm);
+          else
+            floor_cons_pt<NV>(u, a.eos);
           double un[NV];
           if (a.stage == 2) {
             const double *pn = a.u_n + nb.self + cidx;
-            double wn[NV], gn[2];
 #pragma unroll
             for (int v = 0; v < NV; ++v) un[v] = pn[v * nc];
-            cons_to_prim_pt<NV>(un, a.eos, wn, gn);
+            floor_cons_pt<NV>(un, a.eos);
           } else {
 #pragma unroll
             for (int v = 0; v < NV; ++v) un[v] = u[v];
@@ -224,7 +230,7 @@ __device__ __forceinline__ void sweep(const FusedArgs &a, const aph_block &bk, i
 #pragma unroll
       for (int v = 0; v < NV; ++v) fprev[v] = fo[v];
     };
-    march<RECON, RS, NV, 1 + D>(f_lo, f_hi, load, emit);
+    march<RECON, RS, NV, 1 + D, NV>(f_lo, f_hi, fetch, convert, emit);
   }
 }
 

@@ -14,13 +14,17 @@
 
 namespace aph {
 
-// Sweep faces [f_lo, f_hi] along one axis.  load(c, dst[NF]) must write the NV
-// primitive fields followed by (gamma_c, gamma_e) of cell c (pencil coordinate).
+// Sweep faces [f_lo, f_hi] along one axis.  A cell (pencil coordinate c) enters the
+// window in two parts: fetch(c, raw[NR]) issues its loads, convert(raw, dst[NF]) turns
+// them into the NV primitive fields followed by (gamma_c, gamma_e).  The fetch of the
+// next step's cell is issued before the current step's reconstruction and Riemann
+// solve, so its memory latency is hidden behind them.
 // emit(f, flux[NV], ql[NF], qr[NF]) receives the face flux in conserved-variable
 // order (IDN, IM1, IM2, IM3, IEN, scalars...) for face f (between cells f-1 and f).
 // IVX = 1 + sweep axis (the reference's ivx, hydro_hllc.hpp:157-158).
-template <int RECON, int RS, int NV, int IVX, class Load, class Emit>
-__device__ __forceinline__ void march(int f_lo, int f_hi, Load &&load, Emit &&emit) {
+template <int RECON, int RS, int NV, int IVX, int NR, class Fetch, class Convert, class Emit>
+__device__ __forceinline__ void march(int f_lo, int f_hi, Fetch &&fetch, Convert &&convert,
+                                      Emit &&emit) {
   constexpr int GR = ReconTraits<RECON>::GR;
   constexpr int W = ReconTraits<RECON>::W;
   constexpr int NF = NV + 2;
@@ -29,28 +33,72 @@ __device__ __forceinline__ void march(int f_lo, int f_hi, Load &&load, Emit &&em
   double win[W][NF];
   double qlp[NF];
   double fv[NF];  // PPM: limited value of the face below the cell being reconstructed
+  double raw[NR];
+  // PPM: bit s is set when every field of window cell s is +-0 or 2^-900 <= |q| < 2^1000
+  // (frexp exponent in [-899, 1000]); a face whose four cells all qualify may take
+  // ppm_face<true>.  Non-finite values give exponent 0 and qualify: a state holding them
+  // is already invalid.
+  unsigned plain = 0;
+  auto plain_cell = [](const double *c) -> unsigned {
+    int lo = 0, hi = 0;
 #pragma unroll
-  for (int s = 0; s < W - 1; ++s) load(f_lo - 1 - GR + s, win[s + 1]);
+    for (int f = 0; f < NF; ++f) {
+      const int k = __builtin_amdgcn_frexp_exp(c[f]);
+      lo = min(lo, k);
+      hi = max(hi, k);
+    }
+    return (lo >= -899 && hi <= 1000) ? 1u : 0u;
+  };
+#pragma unroll
+  for (int s = 0; s < W - 1; ++s) {
+    fetch(f_lo - 1 - GR + s, raw);
+    convert(raw, win[s + 1]);
+    if constexpr (RECON == RECON_PPM) plain |= plain_cell(win[s + 1]) << (s + 1);
+  }
+  fetch(f_lo - 1 + GR, raw);
   if constexpr (RECON == RECON_PPM) {
+    // lower face of the first reconstructed cell (m = f_lo - 1): cells f_lo-3 .. f_lo
+    if (((plain >> 1) & 0xFu) == 0xFu) {
 #pragma unroll
-    for (int f = 0; f < NF; ++f) fv[f] = ppm_face(win[1][f], win[2][f], win[3][f], win[4][f]);
+      for (int f = 0; f < NF; ++f)
+        fv[f] = ppm_face<true>(win[1][f], win[2][f], win[3][f], win[4][f]);
+    } else {
+#pragma unroll
+      for (int f = 0; f < NF; ++f)
+        fv[f] = ppm_face<false>(win[1][f], win[2][f], win[3][f], win[4][f]);
+    }
   }
   for (int m = f_lo - 1; m <= f_hi; ++m) {
 #pragma unroll
     for (int s = 0; s < W - 1; ++s)
 #pragma unroll
       for (int f = 0; f < NF; ++f) win[s][f] = win[s + 1][f];
-    load(m + GR, win[W - 1]);
+    convert(raw, win[W - 1]);
+    if (m < f_hi) fetch(m + 1 + GR, raw);
     double ql[NF], qr[NF];
+    if constexpr (RECON == RECON_PPM) {
+      // face m+1/2 once (cells m-1 .. m+2 = win[1..4]); cell m's own face m-1/2 was the
+      // previous step's upper face
+      plain = (plain >> 1) | (plain_cell(win[W - 1]) << (W - 1));
+      double up[NF];
+      if (((plain >> 1) & 0xFu) == 0xFu) {
 #pragma unroll
-    for (int f = 0; f < NF; ++f) {
-      if constexpr (RECON == RECON_PPM) {
-        // face m+1/2 once; cell m's own face m-1/2 was the previous step's upper face
-        const double up = ppm_face(win[1][f], win[2][f], win[3][f], win[4][f]);
-        ppm_cell(win[0][f], win[1][f], win[2][f], win[3][f], win[4][f], fv[f], up, ql[f],
-                 qr[f]);
-        fv[f] = up;
+        for (int f = 0; f < NF; ++f)
+          up[f] = ppm_face<true>(win[1][f], win[2][f], win[3][f], win[4][f]);
       } else {
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+          up[f] = ppm_face<false>(win[1][f], win[2][f], win[3][f], win[4][f]);
+      }
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        ppm_cell(win[0][f], win[1][f], win[2][f], win[3][f], win[4][f], fv[f], up[f], ql[f],
+                 qr[f]);
+        fv[f] = up[f];
+      }
+    } else {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
         double col[W];
 #pragma unroll
         for (int s = 0; s < W; ++s) col[s] = win[s][f];

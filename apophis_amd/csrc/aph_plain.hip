@@ -87,19 +87,23 @@ __global__ void __launch_bounds__(64)
   const double *gm = gamma + (int64_t)2 * bk.cell_base;
   double *f = flux + (int64_t)NV * bk.cell_base;
   const int64_t sd = st[DIR];
-  auto load = [&](int c, double *dst) {
+  auto fetch = [&](int c, double *dst) {
     const int64_t o = base + c * sd;
 #pragma unroll
     for (int v = 0; v < NV; ++v) dst[v] = w[v * nc + o];
     dst[NV] = gm[o];
     dst[NV + 1] = gm[nc + o];
   };
+  auto convert = [](const double *src, double *dst) {
+#pragma unroll
+    for (int v = 0; v < NV + 2; ++v) dst[v] = src[v];
+  };
   auto emit = [&](int fc, const double *fo, const double *, const double *) {
     const int64_t o = base + fc * sd;
 #pragma unroll
     for (int v = 0; v < NV; ++v) f[v * nc + o] = fo[v];
   };
-  march<RECON, RS, NV, 1 + DIR>(g[DIR], g[DIR] + bk.nx[DIR], load, emit);
+  march<RECON, RS, NV, 1 + DIR, NV + 2>(g[DIR], g[DIR] + bk.nx[DIR], fetch, convert, emit);
 }
 
 // ---------------------------------------------------------------- update

@@ -17,6 +17,8 @@
 
 namespace aph {
 using std::fabs;
+using std::fma;
+using std::fmax;
 using std::fmin;
 using std::sqrt;
 
@@ -124,28 +126,49 @@ APH_HD void ppm(double q_im2, double q_im1, double q_i, double q_ip1,
   ql_ip1 = ql;
 }
 
-// Face-shared PPM.  The limited interface value of face i-1/2 (steps 1-2a of
-// ppm_simple.hpp:40-93) uses cells i-2..i+1 only, and cell i-1 (as its "dph_ip1") and
-// cell i (as its "dph") evaluate it with the same operands in the same order, so it is
-// bitwise the same value: ppm_face computes it once, ppm_cell finishes steps 3-4 of one
-// cell from its two face values.  ppm_cell(q, ppm_face(q[0..3]), ppm_face(q[1..4]))
-// equals ppm(q) bitwise.
+// x / 6.0 correctly rounded, without a division instruction sequence, for x = +-0 or
+// 2^-1019 <= |x| <= 2^1020 (other x: use the true division).  y = RN(1/6);
+// q0 = RN(x*y) is within one ulp of x/6; r = 6*q0 - x is exact (one fma) and
+// q0 - r*y = x/6 + e*(q0 - x/6) with |e| = |1 - 6y| < 2^-53, i.e. within 2^-52 ulp of x/6
+// before its one rounding.  Since x/6 = (x/2)/3 and a normal quotient by 3 lies 0, 1/3 or
+// 2/3 of an ulp from a representable number, it is never within 1/6 ulp of a rounding
+// midpoint, so that rounding gives RN(x/6).  Signed zeros come out right.
+APH_HD double div6_fast(double x) {
+  constexpr double y = 1.0 / 6.0;
+  const double q0 = x * y;
+  const double r = fma(q0, 6.0, -x);
+  return fma(-r, y, q0);
+}
+
+// Helpers that keep ppm_simple.hpp's results bitwise:  SIGN(a) == SIGN(b) is the same
+// test as (a < 0) == (b < 0), SIGN(a) * m is (a < 0 ? -m : m) (the product by +-1 is
+// exact), and for the non-negative operands of the limiter min/max(a, b) is fmin/fmax(a, b)
+// (they differ only when an operand is NaN, so the results agree for finite data).
+APH_HD bool neg(double x) { return x < 0.0; }
+APH_HD double with_sign_of(double s, double m) { return neg(s) ? -m : m; }
+
+//
+// FAST = true evaluates the two divisions by 6 with div6_fast.  It is exact when every
+// q is +-0 or 2^-900 <= |q| < 2^1000 (a "plain" window): then every intermediate is a
+// multiple of 2^-1008 below 2^1007, so both dividends are 0 or inside div6_fast's range.
+template <bool FAST = false>
 APH_HD double ppm_face(double q_im2, double q_im1, double q_i, double q_ip1) {
   const double C2 = 1.25;
   const double qa0 = (q_i - q_im1);
   const double qb0 = (q_ip1 - q_i);
   const double dd_im1 = 0.5 * qa0 + 0.5 * (q_im1 - q_im2);
   const double dd = 0.5 * qb0 + 0.5 * qa0;
-  double dph = 0.5 * (q_im1 + q_i) + (dd_im1 - dd) / 6.0;
+  const double avg = 0.5 * (q_im1 + q_i);
+  double dph = avg + (FAST ? div6_fast(dd_im1 - dd) : (dd_im1 - dd) / 6.0);
   const double d2qc_im1 = q_im2 + q_i - 2.0 * q_im1;
   const double d2qc = q_im1 + q_ip1 - 2.0 * q_i;
   const double qa_tmp = dph - q_im1;
   const double qb_tmp = q_i - dph;
   const double qa = 3.0 * (q_im1 + q_i - 2.0 * dph);
   double qd = 0.0;
-  if (SIGN(qa) == SIGN(d2qc_im1) && SIGN(qa) == SIGN(d2qc))
-    qd = SIGN(qa) * mn(C2 * fabs(d2qc_im1), mn(C2 * fabs(d2qc), fabs(qa)));
-  const double dph_tmp = 0.5 * (q_im1 + q_i) - qd / 6.0;
+  if (neg(qa) == neg(d2qc_im1) && neg(qa) == neg(d2qc))
+    qd = with_sign_of(qa, fmin(C2 * fabs(d2qc_im1), fmin(C2 * fabs(d2qc), fabs(qa))));
+  const double dph_tmp = avg - (FAST ? div6_fast(qd) : qd / 6.0);
   if (qa_tmp * qb_tmp < 0.0) dph = dph_tmp;
   return dph;
 }
@@ -164,14 +187,14 @@ APH_HD void ppm_cell(double q_im2, double q_im1, double q_i, double q_ip1, doubl
   const double qa_tmp = dqf_minus * dqf_plus;
   const double qb_tmp = (q_ip1 - q_i) * (q_i - q_im1);
   double qe = 0.0;
-  if (SIGN(d2qc_im1) == SIGN(d2qc) && SIGN(d2qc_im1) == SIGN(d2qc_ip1) &&
-      SIGN(d2qc_im1) == SIGN(d2qf))
-    qe = SIGN(d2qf) * mn(mn(C2 * fabs(d2qc_im1), C2 * fabs(d2qc)),
-                         mn(C2 * fabs(d2qc_ip1), fabs(d2qf)));
-  const double qa = mx(fabs(q_im1), fabs(q_im2));
-  const double qb = mx(mx(fabs(q_i), fabs(q_ip1)), fabs(q_ip2));
+  if (neg(d2qc_im1) == neg(d2qc) && neg(d2qc_im1) == neg(d2qc_ip1) &&
+      neg(d2qc_im1) == neg(d2qf))
+    qe = with_sign_of(d2qf, fmin(fmin(C2 * fabs(d2qc_im1), C2 * fabs(d2qc)),
+                                 fmin(C2 * fabs(d2qc_ip1), fabs(d2qf))));
+  const double qa = fmax(fabs(q_im1), fabs(q_im2));
+  const double qb = fmax(fmax(fabs(q_i), fabs(q_ip1)), fabs(q_ip2));
   double rho = 0.0;
-  if (fabs(d2qf) > (1.0e-12) * mx(qa, qb)) rho = qe / d2qf;
+  if (fabs(d2qf) > (1.0e-12) * fmax(qa, qb)) rho = qe / d2qf;
   const double tmp_m = q_i - rho * dqf_minus;
   const double tmp_p = q_i + rho * dqf_plus;
   const double tmp2_m = q_i - 2.0 * dqf_plus;
@@ -267,12 +290,14 @@ APH_HD void hllc(const double *wl, const double *wr, const double *ifl,
   double rhoa = .5 * (wl[0] + wr[0]);
   double ca = .5 * (cl + cr);
   double pstar = .5 * (wl[4] + wr[4]) - .5 * (wr[1] - wl[1]) * rhoa * ca;
-  double ql = (pstar <= wl[4])
-                  ? 1.0
-                  : sqrt(1.0 + (gammal + 1) / (2 * gammar) * (pstar / wl[4] - 1.0));
-  double qr = (pstar <= wr[4])
-                  ? 1.0
-                  : sqrt(1.0 + (gammal + 1) / (2 * gammar) * (pstar / wr[4] - 1.0));
+  // (gammal + 1) / (2 * gammar) is the same quotient on both sides: evaluated once
+  double ql = 1.0, qr = 1.0;
+  const bool shl = !(pstar <= wl[4]), shr = !(pstar <= wr[4]);
+  if (shl || shr) {
+    const double gfac = (gammal + 1) / (2 * gammar);
+    if (shl) ql = sqrt(1.0 + gfac * (pstar / wl[4] - 1.0));
+    if (shr) qr = sqrt(1.0 + gfac * (pstar / wr[4] - 1.0));
+  }
   double sl = wl[1] - cl * ql;
   double sr = wr[1] + cr * qr;
   double bp = sr > 0.0 ? sr : (TINY_NUMBER);
@@ -299,16 +324,14 @@ APH_HD void hllc(const double *wl, const double *wr, const double *ifl,
   fr[3] = wr[0] * wr[3] * vxr;
   fl[4] = el * vxl + wl[4] * wl[1];
   fr[4] = er * vxr + wr[4] * wr[1];
-  double ur, ul, ustar;
-  if (sstar >= 0.0) {
-    ul = sstar / (sstar - bm);
-    ur = 0.0;
-    ustar = -bm / (sstar - bm);
-  } else {
-    ul = 0.0;
-    ur = -sstar / (bp - sstar);
-    ustar = bp / (bp - sstar);
-  }
+  // the reference's two branches divide by one denominator each; selecting the operands
+  // first gives every lane the same two quotients with two divisions instead of four
+  const bool pos = (sstar >= 0.0);
+  const double den = pos ? (sstar - bm) : (bp - sstar);
+  const double uw = (pos ? sstar : -sstar) / den;
+  const double ustar = (pos ? -bm : bp) / den;
+  const double ul = pos ? uw : 0.0;
+  const double ur = pos ? 0.0 : uw;
   flx[0] = ul * fl[0] + ur * fr[0];
   flx[1] = ul * fl[1] + ur * fr[1] + ustar * cp;
   flx[2] = ul * fl[2] + ur * fr[2];
@@ -357,6 +380,25 @@ APH_HD void cons_to_prim_pt(double *u, const aph_eos &eos, double *w,
   gm[1] = w_p / (u_e - e_k) + 1;
 #pragma unroll
   for (int n = 5; n < NV; ++n) w[n] = u[n] * di;
+}
+
+// The in-place part of cons_to_prim_pt only: the density floor and the pressure-floor
+// energy fix, decided by the same arithmetic (so the same branch), without the
+// primitives and adiabatic indices.
+template <int NV>
+APH_HD void floor_cons_pt(double *u, const aph_eos &eos) {
+  const double gm1e = eos.gm1;
+  double u_d = u[0];
+  u_d = (u_d > eos.dfloor) ? u_d : eos.dfloor;
+  u[0] = u_d;
+  double di = 1.0 / u_d;
+  double e_k = 0.5 * di * (SQR(u[1]) + SQR(u[2]) + SQR(u[3]));
+  double sie = (u[4] - e_k) / u_d;
+  double w_p = mx(EOS_SMALL, gm1e * u_d * sie);
+  if (!(w_p > eos.pfloor)) {
+    double gm1 = w_p / (u_d * e_k);
+    u[4] = ((eos.pfloor / gm1) + e_k);
+  }
 }
 
 // EstimateTimestep point (hydro.cpp:384-415): returns min over active axes of

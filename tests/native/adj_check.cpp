@@ -39,3 +39,23 @@ extern "C" void chk_ppm_shared(const double *q, double *out) {
   const double hi = ppm_face(q[1], q[2], q[3], q[4]);
   ppm_cell(q[0], q[1], q[2], q[3], q[4], lo, hi, out[0], out[1]);
 }
+extern "C" void chk_ppm_face(const double *q, long n, int fast, double *out) {
+  for (long i = 0; i < n; ++i)
+    out[i] = fast ? ppm_face<true>(q[4 * i], q[4 * i + 1], q[4 * i + 2], q[4 * i + 3])
+                  : ppm_face<false>(q[4 * i], q[4 * i + 1], q[4 * i + 2], q[4 * i + 3]);
+}
+extern "C" void chk_div6_fast(const double *x, long n, double *out) {
+  for (long i = 0; i < n; ++i) out[i] = div6_fast(x[i]);
+}
+// u[5] floored in place by floor_cons_pt (out[0..4]) and by cons_to_prim_pt (out[5..9])
+extern "C" void chk_floor(const double *u, const double *e, double *out) {
+  aph_eos eos{e[0], e[1], e[2], e[3]};
+  double a[5] = {u[0], u[1], u[2], u[3], u[4]}, b[5] = {u[0], u[1], u[2], u[3], u[4]};
+  double w[5], g[2];
+  floor_cons_pt<5>(a, eos);
+  cons_to_prim_pt<5>(b, eos, w, g);
+  for (int i = 0; i < 5; ++i) {
+    out[i] = a[i];
+    out[5 + i] = b[i];
+  }
+}

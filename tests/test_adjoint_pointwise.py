@@ -119,17 +119,3 @@ def test_cons_to_prim_adjoint_matches_fd(lib):
         checked += fd_check(f, adj, u, 8, rng)
     assert checked > 900
 
-
-def test_face_shared_ppm_is_bitwise_ppm(lib):
-    """ppm_cell(q, ppm_face(q[0..3]), ppm_face(q[1..4])) == ppm(q) bitwise (the face value
-    of i-1/2 is the same whichever adjacent cell computes it)."""
-    rng = np.random.default_rng(9)
-    for trial in range(20000):
-        if trial % 2:
-            t = np.arange(5) - 2.0
-            c = rng.normal(size=3)
-            q = c[0] + c[1] * t + c[2] * t * t + 1e-6 * rng.normal(size=5)
-        else:
-            q = rng.normal(size=5) * rng.choice([1e-8, 1.0, 1e8])
-        got = _call(lib.chk_ppm_shared, 2, q)
-        assert tuple(got) == orc.ppm(*q)

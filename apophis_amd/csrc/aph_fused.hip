@@ -18,7 +18,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "aph_adjoint_host.hpp"
@@ -325,6 +327,8 @@ struct aph_plan {
   double g[3] = {0, 0, 0};
   int grav = 0;
   std::vector<aph_tile> user_tiles;  // aph_plan_set_tile overrides, keyed by nx[0]
+  std::vector<std::pair<int4, aph_tile>> tuned;  // autotuner choices, keyed by block shape
+  int only_class = -1;               // autotuner: item list restricted to one class
   std::vector<aph_tile> tiles;       // resolved tile per class
   std::vector<int4> class_shape;  // class index -> block shape (nx0, nx1, nx2)
   int4 *d_items = nullptr;
@@ -413,11 +417,14 @@ int build_items(aph_context *ctx, aph_plan *p) {
     }
     cls_of[b] = c;
   }
-  // tile per class: a user tile for edge n = nx0 (aph_plan_set_tile) if one is set
+  // tile per class: a user tile for edge n = nx0 (aph_plan_set_tile) if one is set,
+  // else the autotuner's choice for the shape, else the default
   std::vector<aph_tile> tl(p->class_shape.size());
   for (size_t c = 0; c < p->class_shape.size(); ++c) {
     const aph_block &k = bs->h[p->class_shape[c].w];
     aph_tile t = default_tile(k, ndim);
+    for (const auto &u : p->tuned)
+      if (u.first.x == k.nx[0] && u.first.y == k.nx[1] && u.first.z == k.nx[2]) t = u.second;
     for (const auto &u : p->user_tiles)
       if (u.n == k.nx[0]) t = u;
     tl[c] = t;
@@ -428,6 +435,7 @@ int build_items(aph_context *ctx, aph_plan *p) {
   std::vector<int64_t> load(kXcd, 0);
   size_t maxcells = 0;
   for (int b = 0; b < bs->nblocks; ++b) {
+    if (p->only_class >= 0 && cls_of[b] != p->only_class) continue;
     const aph_block &k = bs->h[b];
     const aph_tile &t = tl[cls_of[b]];
     const int tt[3] = {t.tx, t.ty, t.tz};
@@ -525,6 +533,46 @@ int launch_stage(aph_context *ctx, aph_plan *p, int stage, double *u_in, const d
   return APH_OK;
 }
 
+// Candidate tiles for one block shape: divisors of the block edges, x extent at least
+// 4 where the block allows (coalesced rows), 64..1024 cells (a whole small block is a
+// candidate too), du tile within the LDS limit.
+std::vector<aph_tile> tile_candidates(const int4 &sh, int ndim, int nvar) {
+  auto divs = [](int n) {
+    std::vector<int> d;
+    for (int i = 1; i <= n; ++i)
+      if (n % i == 0) d.push_back(i);
+    return d;
+  };
+  const int64_t lds_cells = 64 * 1024 / (8 * nvar);
+  const int64_t whole = (int64_t)sh.x * sh.y * sh.z;
+  std::vector<int> dy = ndim >= 2 ? divs(sh.y) : std::vector<int>{1};
+  std::vector<int> dz = ndim >= 3 ? divs(sh.z) : std::vector<int>{1};
+  std::vector<aph_tile> out;
+  for (int tx : divs(sh.x)) {
+    if (tx < std::min(4, sh.x)) continue;
+    for (int ty : dy)
+      for (int tz : dz) {
+        const int64_t cells = (int64_t)tx * ty * tz;
+        if (ndim >= 2 && cells > lds_cells) continue;
+        if (cells > 1024 && cells != whole) continue;
+        if (cells < 64 && cells != whole) continue;
+        out.push_back(aph_tile{sh.x, tx, ty, tz});
+      }
+  }
+  return out;
+}
+
+void drop_graph(aph_plan *p) {
+  if (p->exec) {
+    (void)hipGraphExecDestroy(p->exec);
+    p->exec = nullptr;
+  }
+  if (p->graph) {
+    (void)hipGraphDestroy(p->graph);
+    p->graph = nullptr;
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -600,15 +648,113 @@ int aph_plan_set_tile(aph_context *ctx, aph_plan *p, const aph_tile *t) {
     build_items(ctx, p);
     return rc;
   }
-  if (p->exec) {
-    (void)hipGraphExecDestroy(p->exec);
-    p->exec = nullptr;
-  }
-  if (p->graph) {
-    (void)hipGraphDestroy(p->graph);
-    p->graph = nullptr;
-  }
+  drop_graph(p);
   return APH_OK;
+}
+
+int aph_plan_autotune(aph_context *ctx, aph_plan *p, const double *u_in, double *u_scratch,
+                      const double *dt_dev, int32_t reps, char *report, size_t report_len) {
+  if (!ctx) return APH_E_INVALID;
+  if (!p || !u_in || !u_scratch || !dt_dev || reps < 1 || (const double *)u_scratch == u_in)
+    return set_error(ctx, APH_E_INVALID, "aph_plan_autotune: bad arguments");
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  APH_HIP_CHECK(ctx, hipStreamIsCapturing(ctx->stream, &cs));
+  if (cs != hipStreamCaptureStatusNone)
+    return set_error(ctx, APH_E_STATE, "aph_plan_autotune: stream is capturing a graph");
+  if (p->dirty) {
+    int rc = build_items(ctx, p);
+    if (rc) return rc;
+  }
+  const std::vector<int4> shapes = p->class_shape;
+  const int ndim = p->bs->ndim;
+  std::string rep;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  APH_HIP_CHECK(ctx, hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return set_error(ctx, APH_E_HIP, "hipEventCreate failed");
+  }
+  int rc = APH_OK;
+  char line[160];
+  for (size_t c = 0; c < shapes.size() && rc == APH_OK; ++c) {
+    const int4 sh = shapes[c];
+    bool user = false;
+    for (const auto &u : p->user_tiles) user = user || (u.n == sh.x);
+    snprintf(line, sizeof line, "block %dx%dx%d:%s\n", sh.x, sh.y, sh.z,
+             user ? " tile set by aph_plan_set_tile, not tuned" : "");
+    rep += line;
+    if (user) continue;
+    // drop an earlier choice for this shape, then time every candidate on this class only
+    auto &tv = p->tuned;
+    tv.erase(std::remove_if(tv.begin(), tv.end(),
+                            [&](const std::pair<int4, aph_tile> &e) {
+                              return e.first.x == sh.x && e.first.y == sh.y && e.first.z == sh.z;
+                            }),
+             tv.end());
+    tv.push_back({sh, aph_tile{}});
+    double best = 1e300;
+    aph_tile bt = default_tile(p->bs->h[sh.w], ndim);
+    for (const aph_tile &t : tile_candidates(sh, ndim, p->bs->nvar)) {
+      tv.back().second = t;
+      p->only_class = (int)c;
+      if (build_items(ctx, p) != APH_OK) continue;  // e.g. LDS too small for this tile
+      rc = launch_stage(ctx, p, 1, const_cast<double *>(u_in), nullptr, u_scratch, dt_dev, 0.0);
+      if (rc) break;
+      if (hipEventRecord(e0, ctx->stream) != hipSuccess) { rc = APH_E_HIP; break; }
+      for (int r = 0; r < reps && rc == APH_OK; ++r)
+        rc = launch_stage(ctx, p, 1, const_cast<double *>(u_in), nullptr, u_scratch, dt_dev, 0.0);
+      if (rc) break;
+      float ms = 0.f;
+      if (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+        rc = set_error(ctx, APH_E_HIP, "autotune: event timing failed");
+        break;
+      }
+      const double avg = ms / reps;
+      snprintf(line, sizeof line, "  %3dx%3dx%3d  %.4f ms\n", t.tx, t.ty, t.tz, avg);
+      rep += line;
+      if (avg < best) {
+        best = avg;
+        bt = t;
+      }
+    }
+    tv.back().second = bt;
+    snprintf(line, sizeof line, "  chosen %dx%dx%d\n", bt.tx, bt.ty, bt.tz);
+    rep += line;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  p->only_class = -1;
+  p->dirty = true;
+  const int rb = build_items(ctx, p);
+  drop_graph(p);
+  if (report && report_len > 0) {
+    const size_t n = std::min(report_len - 1, rep.size());
+    memcpy(report, rep.data(), n);
+    report[n] = '\0';
+  }
+  if (rc) return rc;
+  return rb;
+}
+
+int aph_plan_get_tile(aph_context *ctx, aph_plan *p, int32_t block, aph_tile *out) {
+  if (!ctx) return APH_E_INVALID;
+  if (!p || !out || block < 0 || block >= p->bs->nblocks)
+    return set_error(ctx, APH_E_INVALID, "aph_plan_get_tile: bad arguments");
+  if (p->dirty) {
+    int rc = build_items(ctx, p);
+    if (rc) return rc;
+  }
+  const aph_block &k = p->bs->h[block];
+  for (size_t c = 0; c < p->class_shape.size(); ++c) {
+    const int4 &sh = p->class_shape[c];
+    if (sh.x == k.nx[0] && sh.y == k.nx[1] && sh.z == k.nx[2]) {
+      *out = p->tiles[c];
+      out->n = k.nx[0];
+      return APH_OK;
+    }
+  }
+  return set_error(ctx, APH_E_STATE, "aph_plan_get_tile: block shape not in plan");
 }
 
 int aph_fused_stage(aph_context *ctx, aph_plan *p, int32_t stage, double *u_in,

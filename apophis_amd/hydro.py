@@ -386,6 +386,29 @@ class FusedDriver:
         self.ctx.check(_lib.lib().aph_plan_set_tile(self.ctx.h, self.plan, ctypes.byref(t)))
         self.graph_steps = 0
 
+    def autotune(self, reps=3):
+        """Time the candidate tiles of every block shape on the current state and keep
+        the fastest (aph_plan_autotune; u1 is used as scratch).  Returns the timing
+        report."""
+        buf = ctypes.create_string_buffer(1 << 16)
+        self.ctx.check(_lib.lib().aph_plan_autotune(self.ctx.h, self.plan, _ptr(self.u0),
+                                                    _ptr(self.u1), _ptr(self.dt), int(reps),
+                                                    buf, len(buf)))
+        self.graph_steps = 0
+        return buf.value.decode()
+
+    def tiles(self):
+        """{block shape (nx, ny, nz): tile (tx, ty, tz)} in use."""
+        out = {}
+        for b in range(self.bs.nblocks):
+            shape = tuple(int(v) for v in self.bs.blocks["nx"][b])
+            if shape not in out:
+                t = _lib.Tile()
+                self.ctx.check(_lib.lib().aph_plan_get_tile(self.ctx.h, self.plan, b,
+                                                            ctypes.byref(t)))
+                out[shape] = (t.tx, t.ty, t.tz)
+        return out
+
     def initial_dt(self):
         """dt for the first step: the estimate of the initial state, clipped to tlim."""
         self.ctx.check(_lib.lib().aph_plan_estimate_timestep(self.ctx.h, self.plan,

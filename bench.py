@@ -233,6 +233,11 @@ def main():
     cells = p.bs.interior_cells()
     fd = hydro.FusedDriver(ctx, p.bs, p.cons, p.recon, p.riemann, p.eos, p.cfl)
     fd.initial_dt()
+    t_tune = time.perf_counter()
+    report = fd.autotune(reps=3)
+    tiles = {"x".join(map(str, k)): "x".join(map(str, v)) for k, v in fd.tiles().items()}
+    print(f"# autotune {time.perf_counter() - t_tune:.1f}s: {tiles}", file=sys.stderr,
+          flush=True)
     for _ in range(args.warmup):
         fd.step()
     fd.capture(args.steps)
@@ -288,6 +293,9 @@ def main():
         qc = q.bs.interior_cells()
         fq = hydro.FusedDriver(ctx, q.bs, q.cons, q.recon, q.riemann, q.eos, q.cfl)
         fq.initial_dt()
+        fq.autotune(reps=3)
+        uneven_tiles = {"x".join(map(str, k)): "x".join(map(str, v))
+                        for k, v in fq.tiles().items()}
         for _ in range(2):
             fq.step()
         nq = 20
@@ -314,7 +322,7 @@ def main():
             dt = dq.step(dt)
         torch.cuda.synchronize()
         vp = qc * npl / (time.perf_counter() - t0)
-        extras["uneven_blocks"] = {"blocks": q.bs.nblocks, "cells": qc,
+        extras["uneven_blocks"] = {"blocks": q.bs.nblocks, "cells": qc, "tiles": uneven_tiles,
                                    "batched_value": vb, "plain_per_block_value": vp,
                                    "batched_over_plain": vb / vp}
         print(f"# uneven: batched {vb:.4e}, plain per-block {vp:.4e}", file=sys.stderr,
@@ -335,7 +343,7 @@ def main():
         "config": {"workload": f"blast3d {args.nblk * 32}^3 periodic, {p.bs.nblocks} blocks "
                                "of 32^3, PPM+HLLC, RK2, CFL 0.3, gamma 5/3",
                    "form": "batched fused, hipGraph-replayed, device dt",
-                   "finite": finite},
+                   "tiles": tiles, "finite": finite},
         "roofline": roof,
         "cpu_baseline": cpu,
         "extras": extras,

@@ -33,6 +33,7 @@
 #ifndef APOPHIS_HYDRO_H_
 #define APOPHIS_HYDRO_H_
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -172,6 +173,19 @@ int aph_plan_create(aph_context *ctx, const aph_blockset *bs, int32_t recon,
 int aph_plan_destroy(aph_context *ctx, aph_plan *plan);
 /* Override the tile shape for blocks of edge t->n (autotuner hook).  Synchronous. */
 int aph_plan_set_tile(aph_context *ctx, aph_plan *plan, const aph_tile *t);
+/* Autotuner (no counterpart in the reference, whose flux kernels take one fixed team
+ * shape, hydro.cpp:580-583).  For every block shape of the plan it times `reps` stage-1
+ * launches (u_in -> u_scratch, *dt_dev on the device) over the blocks of that shape for
+ * each candidate tile (divisors of the block edges, 64..1024 cells, within LDS), and keeps
+ * the fastest; tiles set with aph_plan_set_tile are kept as they are.  Synchronous; u_in
+ * is only read, u_scratch is overwritten.  If `report` is non-null, a text table of the
+ * candidates and their average launch times (ms) is written to it (at most report_len
+ * bytes, NUL-terminated).  Fails with APH_E_STATE while the context's stream is
+ * capturing a graph. */
+int aph_plan_autotune(aph_context *ctx, aph_plan *plan, const double *u_in, double *u_scratch,
+                      const double *dt_dev, int32_t reps, char *report, size_t report_len);
+/* Tile currently used for block `block` of the plan's block set. */
+int aph_plan_get_tile(aph_context *ctx, aph_plan *plan, int32_t block, aph_tile *out);
 int aph_fused_stage(aph_context *ctx, aph_plan *plan, int32_t stage, double *u_in,
                     const double *u_n, double *u_out, const double *dt_dev);
 

@@ -6,8 +6,9 @@ backward pass runs the hand-written adjoint kernels (``aph_adjoint_step``) with 
 checkpointing: the forward keeps every ``checkpoint_every``-th state, the backward
 recomputes the states of one checkpoint segment at a time and walks them backwards.
 
-Also here: the Kelvin-Helmholtz initial state as a torch function of the perturbation
-amplitudes a_k (so gradients reach a_k through autograd) and the scalar-variance loss.
+Also here: the Kelvin-Helmholtz and batched Rayleigh-Taylor initial states as torch
+functions of their perturbation amplitudes (so gradients reach them through autograd),
+and the losses of those workloads (scalar variance, mixing-layer width).
 """
 from __future__ import annotations
 
@@ -120,12 +121,7 @@ def kh_state(bs, amps, a=0.01, sigma2=0.01, drat=2.0, vflow=0.5, p0=2.5, gm1=2.0
 def interior_index(bs):
     """Flat pack indices of the interior cells, shaped (nblocks, nvar, cells_per_block)
     for blocks of equal shape."""
-    out = []
-    for b in range(bs.nblocks):
-        view_idx = np.arange(bs.nvar * int(bs.ncell[b])) + bs.nvar * int(bs.blocks["cell_base"][b])
-        v = view_idx.reshape((bs.nvar,) + tuple(int(e) for e in bs.extents()[b][::-1]))
-        out.append(v[(slice(None),) + bs.interior_slices(b)].reshape(bs.nvar, -1))
-    return np.stack(out)
+    return bs.interior_index()
 
 
 def scalar_variance(u, bs):
@@ -137,3 +133,48 @@ def scalar_variance(u, bs):
     vol = torch.from_numpy(bs.blocks["vol"].astype(np.float64)).to(u.device)[:, None]
     mean = (vol * s).sum() / (vol.expand_as(s)).sum()
     return (vol * (s - mean) ** 2).sum()
+
+
+# ------------------------------------------------------------------ RT batch as f(a)
+def rt_state(bs, amps, g=-0.1, gm1=0.4, width=0.0):
+    """Batched Rayleigh-Taylor conserved state (problems.rt_batch) as a differentiable
+    torch function of the amplitudes, a (ninst, 6) float64 tensor on the target device;
+    block b takes the amplitudes of its instance mesh_id[b]."""
+    dev = amps.device
+    xa, ya, _ = bs.cell_centers_all()
+    x = torch.from_numpy(np.ascontiguousarray(xa)).to(dev)
+    y = torch.from_numpy(np.ascontiguousarray(ya)).to(dev)
+    mid = torch.from_numpy(bs.blocks["mesh_id"].astype(np.int64)).to(dev)
+    a = amps[mid]  # (nblocks, 6)
+    if width > 0.0:
+        rho = 1.5 + 0.5 * torch.tanh(y / width)
+        p = 2.5 + g * (1.5 * y + 0.5 * width * torch.log(torch.cosh(y / width)))
+    else:
+        rho = torch.where(y < 0.0, torch.ones_like(y), 2.0 * torch.ones_like(y))
+        p = 2.5 + g * rho * y
+    vy = torch.zeros_like(x)
+    for k in range(amps.shape[1]):
+        vy = vy + a[:, k, None, None, None] * torch.cos(2.0 * np.pi * (k + 1) * x / 0.5) \
+            * torch.exp(-((y / 0.05) ** 2))
+    zero = torch.zeros_like(x)
+    fields = [rho, zero, rho * vy, zero, p / gm1 + 0.5 * rho * vy * vy]
+    idx = torch.from_numpy(bs.interior_index()).to(dev)
+    out = torch.zeros(bs.field_size(bs.nvar), dtype=torch.float64, device=dev)
+    stacked = torch.stack([f.reshape(bs.nblocks, -1) for f in fields], dim=1)
+    return out.index_put((idx.reshape(-1),), stacked.reshape(-1))
+
+
+def mixing_width(u, bs, rho_light=1.0, rho_heavy=2.0, lx=0.5, per_instance=False):
+    """Mixing-layer width of each instance, W = (1/Lx) * sum_cells 4 f (1 - f) dx dy
+    with the heavy-fluid fraction f = (rho - rho_light) / (rho_heavy - rho_light); the
+    sum over instances unless per_instance."""
+    dev = u.device
+    idx = torch.from_numpy(bs.interior_index()[:, 0, :]).to(dev)
+    f = (u[idx] - rho_light) / (rho_heavy - rho_light)  # (nblocks, cells)
+    area = torch.from_numpy((bs.blocks["dx"][:, 0] * bs.blocks["dx"][:, 1]).astype(np.float64))
+    wb = (4.0 * f * (1.0 - f)).sum(dim=1) * area.to(dev) / lx  # per block
+    if not per_instance:
+        return wb.sum()
+    mid = torch.from_numpy(bs.blocks["mesh_id"].astype(np.int64)).to(dev)
+    n = int(bs.blocks["mesh_id"].max()) + 1
+    return torch.zeros(n, dtype=torch.float64, device=dev).index_add(0, mid, wb)

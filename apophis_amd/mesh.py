@@ -185,6 +185,39 @@ class BlockSet:
         z, y, x = np.meshgrid(c[2], c[1], c[0], indexing="ij")
         return x, y, z
 
+    def _uniform_shape(self):
+        nx, ng = self.blocks["nx"], self.blocks["ng"]
+        if self.nblocks == 0 or not ((nx == nx[0]).all() and (ng == ng[0]).all()):
+            raise ValueError("blocks of one shape (and ghost width) required")
+        return [int(v) for v in nx[0]], int(ng[0])
+
+    def interior_index(self, nv=None):
+        """Flat pack indices of every interior cell as (nblocks, nv, nz*ny*nx), for block
+        sets whose blocks share one shape."""
+        nv = self.nvar if nv is None else int(nv)
+        n, ng = self._uniform_shape()
+        Nx, Ny, Nz = (int(v) for v in self.extents()[0])
+        g = [ng if d < self.ndim else 0 for d in range(3)]
+        k, j, i = np.meshgrid(np.arange(n[2]) + g[2], np.arange(n[1]) + g[1],
+                              np.arange(n[0]) + g[0], indexing="ij")
+        local = ((k * Ny + j) * Nx + i).reshape(-1).astype(np.int64)
+        base = nv * self.blocks["cell_base"].astype(np.int64)
+        comp = np.arange(nv, dtype=np.int64) * (Nx * Ny * Nz)
+        return base[:, None, None] + comp[None, :, None] + local[None, None, :]
+
+    def cell_centers_all(self):
+        """cell_centers of every block at once, each (nblocks, nz, ny, nx), for block sets
+        whose blocks share one shape (same arithmetic, so the same values)."""
+        n, _ = self._uniform_shape()
+        c = []
+        for d in range(3):
+            c.append(self.blocks["xmin"][:, d][:, None] +
+                     (np.arange(n[d]) + 0.5)[None, :] * self.blocks["dx"][:, d][:, None])
+        x = np.broadcast_to(c[0][:, None, None, :], (self.nblocks, n[2], n[1], n[0]))
+        y = np.broadcast_to(c[1][:, None, :, None], (self.nblocks, n[2], n[1], n[0]))
+        z = np.broadcast_to(c[2][:, :, None, None], (self.nblocks, n[2], n[1], n[0]))
+        return x, y, z
+
     def interior_mask(self):
         """Boolean mask over one component (ncells_total) selecting interior cells."""
         m = np.zeros(self.ncells_total, dtype=bool)

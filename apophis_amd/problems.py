@@ -232,10 +232,12 @@ def kh2d(nblk=16, n=32, seed=11, amps=None, a=0.01, sigma2=0.01, drat=2.0,
 
 # ---------------------------------------------------------------- RT
 def rt2d(nx=128, ny=384, nblk=(4, 12), seed=5, amps=None, gamma=1.4, g=-0.1,
-         recon="ppm", riemann="hllc", cfl=0.3, nsteps=300, mesh_id=0):
+         recon="ppm", riemann="hllc", cfl=0.3, nsteps=300, mesh_id=0, width=0.0):
     """2D Rayleigh-Taylor on [-1/4,1/4]x[-3/4,3/4]: rho=2 above y=0, 1 below,
     hydrostatic p = 2.5 + g*rho*y, 6-mode interface perturbation of v_y, periodic in
-    x, reflecting in y, constant gravity g in y (rt.cpp:18-111, gravity.cpp:374-410)."""
+    x, reflecting in y, constant gravity g in y (rt.cpp:18-111, gravity.cpp:374-410).
+    width > 0 replaces the sharp interface by rho = 1.5 + 0.5 tanh(y/width) with the
+    hydrostatic p = 2.5 + g (1.5 y + 0.5 width log cosh(y/width))."""
     gm1 = gamma - 1.0
     if amps is None:
         amps = 0.01 * uniform(seed, np.arange(6, dtype=np.uint64))
@@ -248,14 +250,60 @@ def rt2d(nx=128, ny=384, nblk=(4, 12), seed=5, amps=None, gamma=1.4, g=-0.1,
     cons = np.zeros(bs.field_size(bs.nvar))
     for b in range(bs.nblocks):
         x, y, _ = bs.cell_centers(b)
-        rho = np.where(y < 0.0, 1.0, 2.0)
-        p = 2.5 + g * rho * y
-        vy = np.zeros_like(x)
-        for k in range(len(amps)):
-            vy = vy + amps[k] * np.cos(2.0 * np.pi * (k + 1) * x / 0.5) * np.exp(-(y / 0.05) ** 2)
+        rho, vy, p = _rt_fields(x, y, amps, g, width)
         _set_interior(bs, cons, b, rho, 0.0, rho * vy, 0.0, p / gm1 + 0.5 * rho * vy * vy)
     return Problem("rt2d", bs, cons, make_eos(gm1), recon, riemann, cfl, nsteps=nsteps,
                    g=(0.0, g, 0.0), meta={"amps": amps})
+
+
+RT_GRIDS = ((128, 384), (256, 768), (512, 1536))
+
+
+def rt_batch(ninst=64, seed=11, grids=RT_GRIDS, block=32, gamma=1.4, g=-0.1, recon="ppm",
+             riemann="hllc", cfl=0.3, nsteps=300, width=0.0):
+    """Batched Rayleigh-Taylor: `ninst` independent rt2d instances (same domain, EOS and
+    gravity) in one block set, instance i = mesh_id i.  Each instance draws its grid
+    uniformly from `grids` and six mode amplitudes U(0, 0.01) from the SplitMix64 stream
+    (seed, 7*i .. 7*i+6); every mesh is split into blocks of block x block cells.  The
+    state of instance i equals rt2d(nx, ny, ..., amps=amps[i], mesh_id=i) bitwise."""
+    gm1 = gamma - 1.0
+    r = uniform(seed, np.arange(7 * ninst, dtype=np.uint64)).reshape(ninst, 7)
+    shapes, amps, meshes = [], [], []
+    for i in range(ninst):
+        nx, ny = grids[min(int(r[i, 0] * len(grids)), len(grids) - 1)]
+        shapes.append((nx, ny))
+        amps.append(0.01 * r[i, 1:7])
+        meshes.append(lattice_mesh(2, (nx // block, ny // block, 1), (block, block, 1),
+                                   NEED_GHOST[recon], (-0.25, -0.75, -0.5), (0.25, 0.75, 0.5),
+                                   ["periodic", "periodic", "reflect", "reflect", "periodic",
+                                    "periodic"], mesh_id=i))
+    amps = np.array(amps)
+    bs = BlockSet(2, meshes)
+    cons = np.zeros(bs.field_size(bs.nvar))
+    x, y, _ = bs.cell_centers_all()
+    ab = amps[bs.blocks["mesh_id"].astype(np.int64)].T[:, :, None, None, None]  # (6, nb, 1,1,1)
+    rho, vy, p = _rt_fields(x, y, ab, g, width)
+    fields = [rho, 0.0 * rho, rho * vy, 0.0 * rho, p / gm1 + 0.5 * rho * vy * vy]
+    idx = bs.interior_index()
+    nb = bs.nblocks
+    for v, f in enumerate(fields):
+        cons[idx[:, v, :]] = np.reshape(f, (nb, -1))
+    return Problem("rt_batch", bs, cons, make_eos(gm1), recon, riemann, cfl, nsteps=nsteps,
+                   g=(0.0, g, 0.0), meta={"amps": amps, "grids": shapes, "width": width})
+
+
+def _rt_fields(x, y, amps, g, width=0.0):
+    """rho, v_y, p of the rt2d initial state at cell centres (x, y)."""
+    if width > 0.0:
+        rho = 1.5 + 0.5 * np.tanh(y / width)
+        p = 2.5 + g * (1.5 * y + 0.5 * width * np.log(np.cosh(y / width)))
+    else:
+        rho = np.where(y < 0.0, 1.0, 2.0)
+        p = 2.5 + g * rho * y
+    vy = np.zeros_like(x)
+    for k in range(len(amps)):
+        vy = vy + amps[k] * np.cos(2.0 * np.pi * (k + 1) * x / 0.5) * np.exp(-(y / 0.05) ** 2)
+    return rho, vy, p
 
 
 # ---------------------------------------------------------------- linear wave

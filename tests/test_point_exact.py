@@ -149,3 +149,22 @@ def test_floor_cons_matches_cons_to_prim(lib):
         assert (_bits(out[:5]) == _bits(out[5:])).all(), (u, out)
         fired += out[4] != u[4]
     assert fired > 1000  # the pressure-floor branch was exercised
+
+
+def test_ppm_limiter_is_discontinuous_plm_is_lipschitz():
+    """Along a line in input space the reference PPM (ppm_simple.hpp) jumps by ~1 where a
+    limiter branch switches -- the jump does not shrink with the sampling step -- while
+    PLM's van Leer limiter is Lipschitz.  Central finite differences of a PPM rollout
+    therefore do not converge to the (exact, active-branch) adjoint gradient when a jump
+    lies within +-h; gradients of PPM runs are checked against autograd instead."""
+    q0 = np.array([0.3, -1.2, 0.7, 0.9, -0.4])
+    d = np.array([1.0, 0.2, -0.5, 0.3, 0.8])
+    jumps = []
+    for n in (2001, 20001, 200001):
+        t = np.linspace(-3.0, 3.0, n)
+        v = np.array([orc.ppm(*(q0 + s * d)) for s in t])
+        jumps.append(np.abs(np.diff(v, axis=0)).max())
+    assert min(jumps) > 0.9 and max(jumps) - min(jumps) < 0.01, jumps
+    t = np.linspace(-3.0, 3.0, 200001)
+    v = np.array([orc.plm(*(q0[:3] + s * d[:3])) for s in t])
+    assert np.abs(np.diff(v, axis=0)).max() / (t[1] - t[0]) < 10.0

@@ -65,6 +65,7 @@ _SIGS = {
     "aph_plan_estimate_timestep": (_i32, [_vp, _vp, _vp, _d, _vp]),
     "aph_adjoint_stage": (_i32, [_vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp]),
     "aph_adjoint_step": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "aph_adjoint_step_stored": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp]),
 }
 
 _lib = None

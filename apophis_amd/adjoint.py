@@ -36,16 +36,33 @@ class Stepper:
         d.ctx.check(_lib.lib().aph_plan_step(d.ctx.h, d.plan, _ptr(u), _ptr(self.scratch),
                                              _ptr(d.dt), _ptr(d.time), d.cfl, d.tlim, 1))
 
-    def adjoint_step_(self, u_start, lam):
-        """lam <- (d step(u_start) / d u_start)^T lam, in place."""
+    def stage1(self):
+        """The stage-1 state of the last step_ / stage1_ (valid until the next call)."""
+        return self.scratch
+
+    def stage1_(self, u):
+        """Only the first RK stage of a step from u (u is not modified); see stage1()."""
         d = self.drv
-        d.ctx.check(_lib.lib().aph_adjoint_step(d.ctx.h, d.plan, _ptr(u_start),
-                                                _ptr(self.scratch), _ptr(d.dt), _ptr(lam)))
+        d.ctx.check(_lib.lib().aph_fused_stage(d.ctx.h, d.plan, 1, _ptr(u), None,
+                                               _ptr(self.scratch), _ptr(d.dt)))
+
+    def adjoint_step_(self, u_start, lam, stage1=None):
+        """lam <- (d step(u_start) / d u_start)^T lam, in place.  With stage1 (the step's
+        stage-1 state, e.g. a copy of stage1() after step_(u_start)) no stage is
+        recomputed."""
+        d = self.drv
+        if stage1 is None:
+            d.ctx.check(_lib.lib().aph_adjoint_step(d.ctx.h, d.plan, _ptr(u_start),
+                                                    _ptr(self.scratch), _ptr(d.dt), _ptr(lam)))
+        else:
+            d.ctx.check(_lib.lib().aph_adjoint_step_stored(d.ctx.h, d.plan, _ptr(u_start),
+                                                           _ptr(stage1), _ptr(d.dt),
+                                                           _ptr(lam)))
 
 
 class _Rollout(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, u0, stepper, nsteps, every):
+    def forward(ctx, u0, stepper, nsteps, every, keep_stage1):
         u = u0.detach().clone().contiguous()
         checkpoints = []
         for n in range(nsteps):
@@ -53,6 +70,7 @@ class _Rollout(torch.autograd.Function):
                 checkpoints.append(u.clone())
             stepper.step_(u)
         ctx.stepper, ctx.nsteps, ctx.every = stepper, nsteps, every
+        ctx.keep_stage1 = keep_stage1
         ctx.checkpoints = checkpoints
         return u
 
@@ -65,18 +83,28 @@ class _Rollout(torch.autograd.Function):
         for k in range(len(ctx.checkpoints) - 1, -1, -1):
             n0 = k * every
             n1 = min(nsteps, n0 + every)
-            states = [ctx.checkpoints[k]]
+            # recompute the segment's step inputs (and, if kept, their stage-1 states)
+            states, stage1 = [], []
             u = ctx.checkpoints[k].clone()
-            for _ in range(n0, n1 - 1):  # recompute the segment's step inputs
-                st.step_(u)
+            for n in range(n0, n1):
                 states.append(u.clone())
+                if n < n1 - 1:
+                    st.step_(u)
+                elif ctx.keep_stage1:
+                    st.stage1_(u)  # the last input needs only its stage 1
+                if ctx.keep_stage1:
+                    stage1.append(st.stage1().clone())
             for n in range(n1 - 1, n0 - 1, -1):
-                st.adjoint_step_(states[n - n0], lam)
-        return lam * mask, None, None, None
+                s1 = stage1[n - n0] if ctx.keep_stage1 else None
+                st.adjoint_step_(states[n - n0], lam, stage1=s1)
+        return lam * mask, None, None, None, None
 
 
-def rollout(u0, stepper, nsteps, checkpoint_every=20):
-    return _Rollout.apply(u0, stepper, int(nsteps), int(checkpoint_every))
+def rollout(u0, stepper, nsteps, checkpoint_every=20, keep_stage1=True):
+    """Differentiable nsteps-step rollout.  The backward pass keeps every checkpoint_every
+    -th state; with keep_stage1 it also keeps the stage-1 states of the segment being
+    reversed (twice the segment's memory) so that no stage is computed twice."""
+    return _Rollout.apply(u0, stepper, int(nsteps), int(checkpoint_every), bool(keep_stage1))
 
 
 def interior_mask_tensor(bs, device):

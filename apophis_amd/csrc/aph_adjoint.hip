@@ -30,10 +30,11 @@
 
 namespace aph {
 
-// Segment length of the adjoint sweeps: the largest divisor of n not above 8; at least 3
-// (a spill reaches 3 cells), else the whole pencil.
-__host__ __device__ inline int adj_seg(int n) {
-  for (int d = (n < 8 ? n : 8); d >= 3; --d)
+// Segment length of the adjoint sweeps: the largest divisor of n not above cap; at least
+// 3 (a spill reaches 3 cells), else the whole pencil.  cap is 8 unless the block set is
+// too small to give every SIMD two waves with it (adj_prepare).
+__host__ __device__ inline int adj_seg(int n, int cap) {
+  for (int d = (n < cap ? n : cap); d >= 3; --d)
     if (n % d == 0) return d;
   return n;
 }
@@ -55,6 +56,7 @@ struct AdjArgs {
   double *pbar;                // NF x ncells, pack layout
   double *spill[3];            // per axis
   const int64_t *spill_base;   // [3][nblocks] element offsets
+  int segcap[3];               // adj_seg cap per axis
 };
 
 namespace {
@@ -110,7 +112,7 @@ __global__ void __launch_bounds__(64) k_adj_axis(AdjArgs a) {
   const int b = blockIdx.y;
   const aph_block &bk = a.blocks[b];
   const int n_d = bk.nx[D];
-  const int S = adj_seg(n_d);
+  const int S = adj_seg(n_d, a.segcap[D]);
   const int nseg = n_d / S;
   const int nA = bk.nx[A], nB = bk.nx[B];
   const int u = blockIdx.x * blockDim.x + threadIdx.x;
@@ -377,7 +379,7 @@ __global__ void __launch_bounds__(256) k_adj_gather(AdjArgs a) {
     for (int d = 0; d < NDIM; ++d) {
       const int Ad = (d == 0) ? 1 : 0, Bd = (d == 2) ? 1 : 2;
       const int n_d = bk.nx[d];
-      const int S = adj_seg(n_d), nseg = n_d / S;
+      const int S = adj_seg(n_d, a.segcap[d]), nseg = n_d / S;
       const int pencil = cc[Ad] + cc[Bd] * bk.nx[Ad];
       const int s = cc[d] / S, off = cc[d] % S;
       const int sdn = (d == 0) ? 1 : (d == 1 ? 3 : 9);
@@ -472,13 +474,37 @@ GatherFn pick_gather(int nv, int nd) {
 int adj_prepare(aph_context *ctx, const aph_blockset *bs, AdjScratch &s) {
   if (s.pbar) return APH_OK;
   const int nv = bs->nvar, nf = nv + 2, nd = bs->ndim, nb = bs->nblocks;
+  // segment cap per axis: the longest segments (fewest recomputed halo cells) that still
+  // give >= 2 waves of 64 lanes per SIMD (MI355X: 256 CUs x 4 SIMDs)
+  constexpr int64_t kTargetLanes = 2LL * 64 * 1024;
+  for (int d = 0; d < nd; ++d) {
+    const int A = (d == 0) ? 1 : 0, B = (d == 2) ? 1 : 2;
+    int best = 8;
+    int64_t best_lanes = -1;
+    for (int cap = 8; cap >= 3; --cap) {
+      int64_t lanes = 0;
+      for (int b = 0; b < nb; ++b) {
+        const aph_block &k = bs->h[b];
+        lanes += (int64_t)k.nx[A] * k.nx[B] * (k.nx[d] / adj_seg(k.nx[d], cap));
+      }
+      if (lanes > best_lanes) {
+        best_lanes = lanes;
+        best = cap;
+      }
+      if (lanes >= kTargetLanes) {
+        best = cap;
+        break;
+      }
+    }
+    s.segcap[d] = best;
+  }
   std::vector<int64_t> base(3 * (size_t)nb, 0);
   int64_t tot[3] = {0, 0, 0};
   for (int d = 0; d < nd; ++d)
     for (int b = 0; b < nb; ++b) {
       const aph_block &k = bs->h[b];
       const int A = (d == 0) ? 1 : 0, B = (d == 2) ? 1 : 2;
-      const int nseg = k.nx[d] / adj_seg(k.nx[d]);
+      const int nseg = k.nx[d] / adj_seg(k.nx[d], s.segcap[d]);
       base[(size_t)d * nb + b] = tot[d];
       tot[d] += (int64_t)k.nx[A] * k.nx[B] * nseg * 6 * nv;
     }
@@ -536,6 +562,7 @@ int adj_stage(aph_context *ctx, const aph_blockset *bs, int recon, int rs, const
   a.pbar = s.pbar;
   for (int d = 0; d < 3; ++d) a.spill[d] = s.spill[d];
   a.spill_base = s.spill_base;
+  for (int d = 0; d < 3; ++d) a.segcap[d] = s.segcap[d];
   const int nd = bs->ndim;
   for (int d = 0; d < nd; ++d) {
     AxisFn f = pick_adj_axis(recon, rs, bs->nvar, nd, d);
@@ -543,7 +570,8 @@ int adj_stage(aph_context *ctx, const aph_blockset *bs, int recon, int rs, const
     int64_t maxu = 0;
     for (const auto &k : bs->h) {
       const int A = (d == 0) ? 1 : 0, B = (d == 2) ? 1 : 2;
-      maxu = std::max<int64_t>(maxu, (int64_t)k.nx[A] * k.nx[B] * (k.nx[d] / adj_seg(k.nx[d])));
+      maxu = std::max<int64_t>(maxu, (int64_t)k.nx[A] * k.nx[B] *
+                                         (k.nx[d] / adj_seg(k.nx[d], s.segcap[d])));
     }
     dim3 gr((unsigned)((maxu + 63) / 64), (unsigned)bs->nblocks);
     hipLaunchKernelGGL(f, gr, dim3(64), 0, ctx->stream, a);

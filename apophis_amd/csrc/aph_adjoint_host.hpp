@@ -16,6 +16,7 @@ struct AdjScratch {
   int64_t *spill_base = nullptr;
   double *lam1 = nullptr, *lam0p = nullptr;
   int64_t ncells = 0;
+  int segcap[3] = {8, 8, 8};  // longest segment per axis (shorter when blocks are few)
 };
 
 int adj_stage(aph_context *ctx, const aph_blockset *bs, int recon, int rs, const aph_eos &eos,

@@ -880,6 +880,19 @@ int aph_adjoint_step(aph_context *ctx, aph_plan *p, double *u0, double *u1,
   // recompute the stage-1 state from the step's checkpoint, then stage 2 and stage 1
   rc = launch_stage(ctx, p, 1, u0, nullptr, u1, dt_dev, 0.0);
   if (rc) return rc;
+  return aph_adjoint_step_stored(ctx, p, u0, u1, dt_dev, lam);
+}
+
+int aph_adjoint_step_stored(aph_context *ctx, aph_plan *p, const double *u0, const double *u1,
+                            const double *dt_dev, double *lam) {
+  if (!ctx) return APH_E_INVALID;
+  if (!p || !u0 || !u1 || !dt_dev || !lam || u0 == u1 || lam == u0 || lam == u1)
+    return set_error(ctx, APH_E_INVALID, "aph_adjoint_step_stored: bad arguments");
+  if (!adj_supported(p->recon, p->rsolver, p->bs->nvar, p->bs->ndim))
+    return set_error(ctx, APH_E_INVALID, "adjoint: unsupported (recon, riemann, nvar)");
+  int rc = adj_prepare(ctx, p->bs, p->adj);
+  if (rc) return rc;
+  // stage 2 (input u1) then stage 1 (input u0)
   rc = adj_stage(ctx, p->bs, p->recon, p->rsolver, p->eos, p->g, p->grav, p->adj, 2, u1,
                  dt_dev, lam, nullptr, p->adj.lam1, p->adj.lam0p);
   if (rc) return rc;

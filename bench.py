@@ -100,9 +100,10 @@ def cpu_baseline(threads=None):
 
 def adjoint_leg(ctx, hydro, problems, torch):
     """Kelvin-Helmholtz 512^2 (16x16 blocks of 32^2, PLM+HLLC, one passive scalar): time
-    of a frozen-dt forward RK2 step vs one adjoint step (stage-1 recompute + both stage
-    adjoints), and of a whole 200-step rollout with checkpoints every 20 steps, forward
-    and backward (segment recomputation included)."""
+    of a frozen-dt forward RK2 step vs one adjoint step (both stage adjoints, given the
+    step's input and stage-1 states, as the rollout's backward pass runs it; also without
+    the stage-1 state, which is then recomputed), and of a whole 200-step rollout with
+    checkpoints every 20 steps, forward and backward (segment recomputation included)."""
     from apophis_amd import adjoint
 
     p = problems.kh2d(nblk=16, n=32)
@@ -121,14 +122,23 @@ def adjoint_leg(ctx, hydro, problems, torch):
         st.step_(u)
     torch.cuda.synchronize()
     t_fwd = (time.perf_counter() - t0) / n
+    stage1 = []
+    for k in range(n):
+        st.stage1_(states[k])
+        stage1.append(st.stage1().clone())
     lam = torch.ones_like(u) * adjoint.interior_mask_tensor(p.bs, u.device)
     st.adjoint_step_(states[-1], lam)  # allocate scratch, warm up
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(n - 1, -1, -1):
-        st.adjoint_step_(states[k], lam)
+        st.adjoint_step_(states[k], lam, stage1=stage1[k])
     torch.cuda.synchronize()
     t_adj = (time.perf_counter() - t0) / n
+    t0 = time.perf_counter()
+    for k in range(n - 1, -1, -1):
+        st.adjoint_step_(states[k], lam)
+    torch.cuda.synchronize()
+    t_adj_rc = (time.perf_counter() - t0) / n
     amps = torch.from_numpy(p.meta["amps"]).cuda().requires_grad_(True)
     torch.cuda.synchronize()
     u0 = adjoint.kh_state(p.bs, amps)
@@ -145,20 +155,62 @@ def adjoint_leg(ctx, hydro, problems, torch):
     return {"workload": "KH 512^2 (256 blocks of 32^2), PLM+HLLC + scalar, frozen dt", "dt": dt,
             "forward_step_ms": t_fwd * 1e3, "adjoint_step_ms": t_adj * 1e3,
             "adjoint_over_forward": t_adj / t_fwd,
+            "adjoint_step_recompute_ms": t_adj_rc * 1e3,
             "rollout_200_forward_s": t_roll_f, "rollout_200_backward_s": t_roll_b,
             "rollout_backward_over_forward": t_roll_b / t_roll_f,
             "finite_grad": bool(torch.isfinite(amps.grad).all().item())}
+
+
+def rt_batch_leg(ctx, hydro, problems, torch, nsteps=300, every=20):
+    """Batched Rayleigh-Taylor with adjoint: 64 independent instances (grids drawn from
+    128x384 / 256x768 / 512x1536, blocks of 32^2), PPM+HLLC, gravity, reflecting walls,
+    all instances in one fused launch per stage.  A 300-step rollout at the frozen
+    initial CFL dt with checkpoints every 20 steps, then the gradient of the summed
+    mixing-layer width with respect to the 64x6 mode amplitudes."""
+    from apophis_amd import adjoint
+
+    p = problems.rt_batch()
+    cells = p.bs.interior_cells()
+    fd = hydro.FusedDriver(ctx, p.bs, p.cons, p.recon, p.riemann, p.eos, p.cfl, g=p.g)
+    dt = float(fd.initial_dt().item())
+    fd.autotune(reps=2)
+    st = adjoint.Stepper(fd, dt)
+    a = torch.from_numpy(p.meta["amps"]).cuda().requires_grad_(True)
+    u0 = adjoint.rt_state(p.bs, a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    uN = adjoint.rollout(u0, st, nsteps, every)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    loss = adjoint.mixing_width(uN, p.bs)
+    loss.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    g = a.grad
+    return {"workload": "64 RT instances (0.5x1.5, grids 128x384/256x768/512x1536, blocks "
+                        "of 32^2), PPM+HLLC, g=-0.1, frozen dt, checkpoint every 20",
+            "instances": int(p.meta["amps"].shape[0]), "blocks": p.bs.nblocks, "cells": cells,
+            "steps": nsteps, "dt": dt,
+            "tiles": {"x".join(map(str, k)): "x".join(map(str, v))
+                      for k, v in fd.tiles().items()},
+            "forward_s": t1 - t0, "forward_zone_cycles_per_s": cells * nsteps / (t1 - t0),
+            "backward_s": t2 - t1, "backward_over_forward": (t2 - t1) / (t1 - t0),
+            "loss": float(loss.item()), "grad_finite": bool(torch.isfinite(g).all().item()),
+            "grad_max": float(g.abs().max().item())}
 
 
 def profile_run(args, ctx, hydro, problems, torch):
     """Eager launches for rocprofv3: 3 RK2 steps of the chosen form on the headline
     workload (or the uneven block set), then a 1 GiB device-to-device copy repeated 3
     times, whose FETCH_SIZE / WRITE_SIZE calibrate the counters (2^30 bytes each way)."""
-    if args.profile_run == "adjoint":
+    if args.profile_run in ("adjoint", "rt_adjoint"):
         from apophis_amd import adjoint
 
-        p = problems.kh2d(nblk=16, n=32)
-        fd = hydro.FusedDriver(ctx, p.bs, p.cons, p.recon, p.riemann, p.eos, p.cfl)
+        if args.profile_run == "adjoint":
+            p = problems.kh2d(nblk=16, n=32)
+        else:  # 8 RT instances, PPM+HLLC + gravity
+            p = problems.rt_batch(ninst=8)
+        fd = hydro.FusedDriver(ctx, p.bs, p.cons, p.recon, p.riemann, p.eos, p.cfl, g=p.g)
         st = adjoint.Stepper(fd, 1e-4)
         u = torch.from_numpy(p.cons.copy()).cuda()
         lam = torch.ones_like(u) * adjoint.interior_mask_tensor(p.bs, u.device)
@@ -166,7 +218,7 @@ def profile_run(args, ctx, hydro, problems, torch):
             st.step_(u)
             st.adjoint_step_(u, lam)
         torch.cuda.synchronize()
-        print("profile-run adjoint: done", flush=True)
+        print(f"profile-run {args.profile_run}: done", flush=True)
         return
     if args.profile_run == "uneven":
         p = problems.uneven_blocks(nblocks=args.uneven_blocks)
@@ -206,7 +258,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="skip plain/uneven legs")
     ap.add_argument("--uneven-blocks", type=int, default=4096)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the KH adjoint leg")
-    ap.add_argument("--profile-run", choices=["fused", "plain", "uneven", "adjoint"],
+    ap.add_argument("--no-rt", action="store_true", help="skip the batched RT adjoint leg")
+    ap.add_argument("--profile-run", choices=["fused", "plain", "uneven", "adjoint", "rt_adjoint"],
                     default=None,
                     help="short eager run for rocprofv3 (no timing, no JSON)")
     args = ap.parse_args()
@@ -332,6 +385,10 @@ def main():
     if not args.no_adjoint:
         extras["adjoint_kh"] = adjoint_leg(ctx, hydro, problems, torch)
         print(f"# adjoint: {extras['adjoint_kh']}", file=sys.stderr, flush=True)
+    if not args.no_rt:
+        extras["rt_batch_adjoint"] = rt_batch_leg(ctx, hydro, problems, torch)
+        print(f"# rt batch: {extras['rt_batch_adjoint']}", file=sys.stderr, flush=True)
+        torch.cuda.empty_cache()
     cpu = None if args.no_cpu else cpu_baseline()
     out = {
         "metric": "zone-cycles/s + % HBM roofline on 1 GPU (batched vs plain, FP64); "

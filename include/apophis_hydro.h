@@ -225,6 +225,11 @@ int aph_adjoint_stage(aph_context *ctx, aph_plan *plan, int32_t stage, const dou
  * step's output on entry and that of u0 on return. */
 int aph_adjoint_step(aph_context *ctx, aph_plan *plan, double *u0, double *u1,
                      const double *dt_dev, double *lam);
+/* The same with the step's stage-1 state u1 = stage1(u0) supplied by the caller (kept
+ * when the forward step was taken or recomputed), so no stage is recomputed.  u0 and u1
+ * are only read. */
+int aph_adjoint_step_stored(aph_context *ctx, aph_plan *plan, const double *u0,
+                            const double *u1, const double *dt_dev, double *lam);
 
 #ifdef __cplusplus
 }

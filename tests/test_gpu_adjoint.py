@@ -11,7 +11,7 @@
   piecewise smooth (limiter, wave-speed and upwind switches), so central FD differs from
   the exact derivative by O(h) kink crossings: the test checks the h = 1e-6 agreement
   stated below and that the difference shrinks as h does;
-* checkpoint interval invariance (bitwise).
+* checkpoint interval invariance, and keeping vs recomputing stage-1 states (bitwise).
 """
 import numpy as np
 import pytest
@@ -189,14 +189,15 @@ def test_adjoint_dot_product_fd_smooth(gpu_ctx, name):
     assert abs(ad - fd) <= 1e-5 * max(abs(fd), abs(ad))
 
 
-def kh_loss_and_grad(ctx, amps, nsteps, every, nblk=2, n=16, dt=None, width=0.01):
+def kh_loss_and_grad(ctx, amps, nsteps, every, nblk=2, n=16, dt=None, width=0.01,
+                     keep_stage1=True):
     hydro, adjoint = _mods()
     p = problems.kh2d(nblk=nblk, n=n, amps=amps.detach().cpu().numpy(), a=width)
     dt = dt if dt is not None else dt_of(p)
     st = make_stepper(ctx, p, dt)
     a = amps.clone().cuda().requires_grad_(True)
     u0 = adjoint.kh_state(p.bs, a, a=width)
-    uN = adjoint.rollout(u0, st, nsteps, every)
+    uN = adjoint.rollout(u0, st, nsteps, every, keep_stage1=keep_stage1)
     loss = adjoint.scalar_variance(uN, p.bs)
     loss.backward()
     return float(loss.item()), a.grad.detach().cpu().numpy(), dt, p
@@ -238,3 +239,7 @@ def test_checkpoint_interval_invariance(gpu_ctx):
     _, g3, _, _ = kh_loss_and_grad(gpu_ctx, amps, 6, 3, nblk=1, n=16, dt=dt)
     _, g6, _, _ = kh_loss_and_grad(gpu_ctx, amps, 6, 6, nblk=1, n=16, dt=dt)
     assert np.array_equal(g1, g3) and np.array_equal(g1, g6)
+    # recomputing each step's stage 1 instead of keeping it gives the same bits
+    _, g3r, _, _ = kh_loss_and_grad(gpu_ctx, amps, 6, 3, nblk=1, n=16, dt=dt,
+                                    keep_stage1=False)
+    assert np.array_equal(g1, g3r)

@@ -51,9 +51,10 @@ struct FusedArgs {
   double *u_out;
   const double *dt;
   double gam0, gam1, beta;
-  double *dt_min;  // stage 2: atomic min of cfl * dt candidate; stage 1: reset here
+  double *dt_min;  // last stage: atomic min of cfl * dt candidate (reset before stage 1)
   double cfl;
-  int stage;
+  int has_un;      // u_n is a separate register (every stage but the first)
+  int last;        // last stage of the step: estimate the next dt
 };
 
 namespace {
@@ -185,7 +186,7 @@ __device__ __forceinline__ void sweep(const FusedArgs &a, const aph_block &bk, i
           else
             floor_cons_pt<NV>(u, a.eos);
           double un[NV];
-          if (a.stage == 2) {
+          if (a.has_un) {
             const double *pn = a.u_n + nb.self + cidx;
 #pragma unroll
             for (int v = 0; v < NV; ++v) un[v] = pn[v * nc];
@@ -222,7 +223,7 @@ __device__ __forceinline__ void sweep(const FusedArgs &a, const aph_block &bk, i
           double *po = a.u_out + nb.self + cidx;
 #pragma unroll
           for (int v = 0; v < NV; ++v) po[v * nc] = out[v];
-          if (a.stage == 2) {  // EstimateTimestep of the new state (after its floors)
+          if (a.last) {  // EstimateTimestep of the new state (after its floors)
             double wq[NV], gq[2];
             cons_to_prim_pt<NV>(out, a.eos, wq, gq);
             dtmin = fmin(dtmin, dt_point(wq, out[4], a.eos, bk.dx, NDIM));
@@ -239,8 +240,6 @@ __device__ __forceinline__ void sweep(const FusedArgs &a, const aph_block &bk, i
 template <int RECON, int RS, int NV, int NDIM>
 __global__ void __launch_bounds__(64) k_fused(FusedArgs a) {
   extern __shared__ double du_s[];
-  if (a.stage == 1 && blockIdx.x == 0 && threadIdx.x == 0 && a.dt_min)
-    *a.dt_min = 1.79769313486231570815e308;
   double dtmin = 1.79769313486231570815e308;
   for (int it = blockIdx.x; it < a.nitems; it += gridDim.x) {
     const int4 item = a.items[it];
@@ -261,7 +260,7 @@ __global__ void __launch_bounds__(64) k_fused(FusedArgs a) {
     }
     __syncthreads();
   }
-  if (a.stage == 2 && a.dt_min) {
+  if (a.last && a.dt_min) {
     for (int o = 32; o > 0; o >>= 1) dtmin = fmin(dtmin, __shfl_xor(dtmin, o));
     if (threadIdx.x == 0) atomic_min_pos(a.dt_min, a.cfl * dtmin);
   }
@@ -336,6 +335,8 @@ struct aph_plan {
   int grid = 0;
   size_t lds = 0;
   double *d_est = nullptr;  // device dt estimate accumulator
+  int integrator = APH_INTEGRATOR_RK2;
+  double *d_u2 = nullptr;   // third register (rk3)
   bool dirty = true;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -365,11 +366,40 @@ KernelFn pick_nv(int nv, int ndim) {
 }
 
 KernelFn pick_kernel(int recon, int rs, int nv, int ndim) {
-  if (recon == RECON_PPM && rs == RS_HLLC) return pick_nv<RECON_PPM, RS_HLLC>(nv, ndim);
-  if (recon == RECON_PPM && rs == RS_HLLE) return pick_nv<RECON_PPM, RS_HLLE>(nv, ndim);
-  if (recon == RECON_PLM && rs == RS_HLLC) return pick_nv<RECON_PLM, RS_HLLC>(nv, ndim);
-  if (recon == RECON_PLM && rs == RS_HLLE) return pick_nv<RECON_PLM, RS_HLLE>(nv, ndim);
+#define APH_PICK(R, S) \
+  if (recon == R && rs == S) return pick_nv<R, S>(nv, ndim);
+  APH_PICK(RECON_PPM, RS_HLLC)
+  APH_PICK(RECON_PPM, RS_HLLE)
+  APH_PICK(RECON_PPM, RS_LHLLC)
+  APH_PICK(RECON_PLM, RS_HLLC)
+  APH_PICK(RECON_PLM, RS_HLLE)
+  APH_PICK(RECON_PLM, RS_LHLLC)
+  APH_PICK(RECON_DC, RS_HLLC)
+  APH_PICK(RECON_DC, RS_HLLE)
+  APH_PICK(RECON_DC, RS_LHLLC)
+#undef APH_PICK
   return nullptr;
+}
+
+// Low-storage coefficients of stage s (1-based) of an integrator (upstream Parthenon:
+// rk1, rk2, rk3, vl2) and the stage's reconstruction (vl2 predicts with DC,
+// hydro.cpp:114-139).  Returns the number of stages (0: unknown integrator).
+int stage_coef(int integrator, int s, int recon, double &gam0, double &gam1, double &beta,
+               int &srecon) {
+  static const double kC[4][3][3] = {
+      {{0.0, 1.0, 1.0}, {0, 0, 0}, {0, 0, 0}},                           // rk1
+      {{0.0, 1.0, 1.0}, {0.5, 0.5, 0.5}, {0, 0, 0}},                     // rk2
+      {{0.0, 1.0, 1.0}, {0.25, 0.75, 0.25}, {2.0 / 3.0, 1.0 / 3.0, 2.0 / 3.0}},  // rk3
+      {{0.0, 1.0, 0.5}, {0.0, 1.0, 1.0}, {0, 0, 0}}};                    // vl2
+  static const int kN[4] = {1, 2, 3, 2};
+  if (integrator < 1 || integrator > 4) return 0;
+  const int n = kN[integrator - 1];
+  if (s < 1 || s > n) return 0;
+  gam0 = kC[integrator - 1][s - 1][0];
+  gam1 = kC[integrator - 1][s - 1][1];
+  beta = kC[integrator - 1][s - 1][2];
+  srecon = (integrator == APH_INTEGRATOR_VL2 && s == 1) ? RECON_DC : recon;
+  return n;
 }
 
 // Largest divisor of n that does not exceed cap.
@@ -483,7 +513,8 @@ int build_items(aph_context *ctx, aph_plan *p) {
   return APH_OK;
 }
 
-FusedArgs make_args(aph_plan *p, int stage, double *u_in, const double *u_n, double *u_out,
+FusedArgs make_args(aph_plan *p, int stage, int nstages, double gam0, double gam1,
+                    double beta, double *u_in, const double *u_n, double *u_out,
                     const double *dt_dev, double cfl) {
   FusedArgs a{};
   a.blocks = p->bs->d;
@@ -510,27 +541,52 @@ FusedArgs make_args(aph_plan *p, int stage, double *u_in, const double *u_n, dou
   a.u_n = u_n;
   a.u_out = u_out;
   a.dt = dt_dev;
-  a.gam0 = stage == 1 ? 0.0 : 0.5;
-  a.gam1 = stage == 1 ? 1.0 : 0.5;
-  a.beta = stage == 1 ? 1.0 : 0.5;
+  a.gam0 = gam0;
+  a.gam1 = gam1;
+  a.beta = beta;
   a.dt_min = p->d_est;
   a.cfl = cfl;
-  a.stage = stage;
+  a.has_un = stage > 1;
+  a.last = stage == nstages;
   return a;
 }
 
+int launch_kernel(aph_context *ctx, aph_plan *p, int recon, int stage, int nst, double gam0,
+                  double gam1, double beta, double *u_in, const double *u_n, double *u_out,
+                  const double *dt_dev, double cfl) {
+  if (p->dirty) {
+    int rc = build_items(ctx, p);
+    if (rc) return rc;
+  }
+  if (p->nitems == 0) return APH_OK;
+  KernelFn fn = pick_kernel(recon, p->rsolver, p->bs->nvar, p->bs->ndim);
+  if (!fn) return set_error(ctx, APH_E_INVALID, "no fused kernel for this stage");
+  FusedArgs a = make_args(p, stage, nst, gam0, gam1, beta, u_in, u_n, u_out, dt_dev, cfl);
+  hipLaunchKernelGGL(fn, dim3(p->grid), dim3(64), p->lds, ctx->stream, a);
+  APH_LAUNCH_CHECK(ctx);
+  return APH_OK;
+}
+
+// Stage `stage` (1-based) of the plan's integrator: u_out = gam0 u_in + gam1 u_n + beta dt
+// (-div F(u_in)) [+ gravity]; u_n is ignored in stage 1 (it is u_in there).
 int launch_stage(aph_context *ctx, aph_plan *p, int stage, double *u_in, const double *u_n,
                  double *u_out, const double *dt_dev, double cfl) {
   if (p->dirty) {
     int rc = build_items(ctx, p);
     if (rc) return rc;
   }
-  if (p->nitems == 0) return APH_OK;
-  KernelFn fn = pick_kernel(p->recon, p->rsolver, p->bs->nvar, p->bs->ndim);
-  FusedArgs a = make_args(p, stage, u_in, u_n, u_out, dt_dev, cfl);
-  hipLaunchKernelGGL(fn, dim3(p->grid), dim3(64), p->lds, ctx->stream, a);
-  APH_LAUNCH_CHECK(ctx);
-  return APH_OK;
+  double gam0, gam1, beta;
+  int srecon;
+  const int nst = stage_coef(p->integrator, stage, p->recon, gam0, gam1, beta, srecon);
+  if (!nst) return set_error(ctx, APH_E_INVALID, "stage %d out of range", stage);
+  return launch_kernel(ctx, p, srecon, stage, nst, gam0, gam1, beta, u_in, u_n, u_out, dt_dev,
+                       cfl);
+}
+
+int plan_nstages(const aph_plan *p) {
+  double g0, g1, b;
+  int r;
+  return stage_coef(p->integrator, 1, p->recon, g0, g1, b, r);
 }
 
 // Candidate tiles for one block shape: divisors of the block edges, x extent at least
@@ -624,6 +680,7 @@ int aph_plan_destroy(aph_context *ctx, aph_plan *p) {
   if (p->graph) (void)hipGraphDestroy(p->graph);
   if (p->d_items) (void)hipFree(p->d_items);
   if (p->d_est) (void)hipFree(p->d_est);
+  if (p->d_u2) (void)hipFree(p->d_u2);
   adj_release(p->adj);
   delete p;
   return APH_OK;
@@ -698,11 +755,13 @@ int aph_plan_autotune(aph_context *ctx, aph_plan *p, const double *u_in, double 
       tv.back().second = t;
       p->only_class = (int)c;
       if (build_items(ctx, p) != APH_OK) continue;  // e.g. LDS too small for this tile
-      rc = launch_stage(ctx, p, 1, const_cast<double *>(u_in), nullptr, u_scratch, dt_dev, 0.0);
+      rc = launch_kernel(ctx, p, p->recon, 1, 2, 0.0, 1.0, 1.0, const_cast<double *>(u_in),
+                         nullptr, u_scratch, dt_dev, 0.0);
       if (rc) break;
       if (hipEventRecord(e0, ctx->stream) != hipSuccess) { rc = APH_E_HIP; break; }
       for (int r = 0; r < reps && rc == APH_OK; ++r)
-        rc = launch_stage(ctx, p, 1, const_cast<double *>(u_in), nullptr, u_scratch, dt_dev, 0.0);
+        rc = launch_kernel(ctx, p, p->recon, 1, 2, 0.0, 1.0, 1.0, const_cast<double *>(u_in),
+                         nullptr, u_scratch, dt_dev, 0.0);
       if (rc) break;
       float ms = 0.f;
       if (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
@@ -757,12 +816,37 @@ int aph_plan_get_tile(aph_context *ctx, aph_plan *p, int32_t block, aph_tile *ou
   return set_error(ctx, APH_E_STATE, "aph_plan_get_tile: block shape not in plan");
 }
 
+int aph_plan_set_integrator(aph_context *ctx, aph_plan *p, int32_t integrator) {
+  if (!ctx) return APH_E_INVALID;
+  if (!p) return set_error(ctx, APH_E_INVALID, "aph_plan_set_integrator: null plan");
+  if (integrator < APH_INTEGRATOR_RK1 || integrator > APH_INTEGRATOR_VL2)
+    return set_error(ctx, APH_E_INVALID, "[Apophis]: Integrator not recognized. Exiting.");
+  if (integrator == APH_INTEGRATOR_VL2 &&
+      !pick_kernel(RECON_DC, p->rsolver, p->bs->nvar, p->bs->ndim))
+    return set_error(ctx, APH_E_INVALID, "vl2: no first-order kernel for this Riemann solver");
+  if (integrator == APH_INTEGRATOR_RK3 && !p->d_u2) {
+    if (hipMalloc(&p->d_u2, sizeof(double) * p->bs->nvar * p->bs->ncells) != hipSuccess) {
+      p->d_u2 = nullptr;
+      return set_error(ctx, APH_E_NOMEM, "rk3 register allocation failed");
+    }
+  }
+  p->integrator = integrator;
+  drop_graph(p);
+  return APH_OK;
+}
+
 int aph_fused_stage(aph_context *ctx, aph_plan *p, int32_t stage, double *u_in,
                     const double *u_n, double *u_out, const double *dt_dev) {
   if (!ctx) return APH_E_INVALID;
-  if (!p || !u_in || !u_out || !dt_dev || (stage == 2 && !u_n) || (stage != 1 && stage != 2))
+  if (!p) return set_error(ctx, APH_E_INVALID, "aph_fused_stage: null plan");
+  const int nst = plan_nstages(p);
+  if (!u_in || !u_out || !dt_dev || stage < 1 || stage > nst || (stage > 1 && !u_n))
     return set_error(ctx, APH_E_INVALID, "aph_fused_stage: bad arguments");
   if (u_in == u_out) return set_error(ctx, APH_E_INVALID, "u_out must not alias u_in");
+  if (stage == 1) {  // the step's dt estimate starts here
+    k_fill<<<1, 1, 0, ctx->stream>>>(p->d_est, 1.79769313486231570815e308);
+    APH_LAUNCH_CHECK(ctx);
+  }
   return launch_stage(ctx, p, stage, u_in, u_n, u_out, dt_dev, 0.0);
 }
 
@@ -771,9 +855,25 @@ int aph_plan_step(aph_context *ctx, aph_plan *p, double *u0, double *u1, double 
   if (!ctx) return APH_E_INVALID;
   if (!p || !u0 || !u1 || !dt_dev || !time_dev || u0 == u1)
     return set_error(ctx, APH_E_INVALID, "aph_plan_step: bad arguments");
+  k_fill<<<1, 1, 0, ctx->stream>>>(p->d_est, 1.79769313486231570815e308);
+  APH_LAUNCH_CHECK(ctx);
+  // registers: U0 holds the step's start state until the last stage overwrites it
+  // (a lane reads its cell's U0 before writing it); stage states ping-pong in U1 / U2
   int rc = launch_stage(ctx, p, 1, u0, nullptr, u1, dt_dev, cfl);
   if (rc) return rc;
-  rc = launch_stage(ctx, p, 2, u1, u0, u0, dt_dev, cfl);
+  switch (p->integrator) {
+    case APH_INTEGRATOR_RK1:
+      APH_HIP_CHECK(ctx, hipMemcpyAsync(u0, u1, sizeof(double) * p->bs->nvar * p->bs->ncells,
+                                        hipMemcpyDeviceToDevice, ctx->stream));
+      break;
+    case APH_INTEGRATOR_RK3:
+      rc = launch_stage(ctx, p, 2, u1, u0, p->d_u2, dt_dev, cfl);
+      if (rc) return rc;
+      rc = launch_stage(ctx, p, 3, p->d_u2, u0, u0, dt_dev, cfl);
+      break;
+    default:  // rk2, vl2
+      rc = launch_stage(ctx, p, 2, u1, u0, u0, dt_dev, cfl);
+  }
   if (rc) return rc;
   k_dt_finalize<<<1, 1, 0, ctx->stream>>>(dt_dev, time_dev, p->d_est, tlim, fixed_dt);
   APH_LAUNCH_CHECK(ctx);
@@ -858,6 +958,8 @@ int aph_adjoint_stage(aph_context *ctx, aph_plan *p, int32_t stage, const double
                       const double *dt_dev, const double *lam_out, const double *lam_acc,
                       double *lam_in, double *lam_n) {
   if (!ctx) return APH_E_INVALID;
+  if (p && p->integrator != APH_INTEGRATOR_RK2)
+    return set_error(ctx, APH_E_INVALID, "adjoint: the plan's integrator must be rk2");
   if (!p || !u_in || !dt_dev || !lam_out || !lam_in || (stage != 1 && stage != 2))
     return set_error(ctx, APH_E_INVALID, "aph_adjoint_stage: bad arguments");
   if (lam_in == lam_out || (lam_acc && lam_acc == lam_in))
@@ -871,6 +973,8 @@ int aph_adjoint_stage(aph_context *ctx, aph_plan *p, int32_t stage, const double
 int aph_adjoint_step(aph_context *ctx, aph_plan *p, double *u0, double *u1,
                      const double *dt_dev, double *lam) {
   if (!ctx) return APH_E_INVALID;
+  if (p && p->integrator != APH_INTEGRATOR_RK2)
+    return set_error(ctx, APH_E_INVALID, "adjoint: the plan's integrator must be rk2");
   if (!p || !u0 || !u1 || !dt_dev || !lam || u0 == u1)
     return set_error(ctx, APH_E_INVALID, "aph_adjoint_step: bad arguments");
   if (!adj_supported(p->recon, p->rsolver, p->bs->nvar, p->bs->ndim))
@@ -886,6 +990,8 @@ int aph_adjoint_step(aph_context *ctx, aph_plan *p, double *u0, double *u1,
 int aph_adjoint_step_stored(aph_context *ctx, aph_plan *p, const double *u0, const double *u1,
                             const double *dt_dev, double *lam) {
   if (!ctx) return APH_E_INVALID;
+  if (p && p->integrator != APH_INTEGRATOR_RK2)
+    return set_error(ctx, APH_E_INVALID, "adjoint: the plan's integrator must be rk2");
   if (!p || !u0 || !u1 || !dt_dev || !lam || u0 == u1 || lam == u0 || lam == u1)
     return set_error(ctx, APH_E_INVALID, "aph_adjoint_step_stored: bad arguments");
   if (!adj_supported(p->recon, p->rsolver, p->bs->nvar, p->bs->ndim))

@@ -391,6 +391,10 @@ int aph_calculate_fluxes(aph_context *ctx, const aph_blockset *bs, int32_t first
   APH_FLUX_CASE(RECON_PPM, RS_HLLC)
   APH_FLUX_CASE(RECON_DC, RS_HLLE)
   APH_FLUX_CASE(RECON_DC, RS_HLLC)
+  APH_FLUX_CASE(RECON_PLM, RS_LHLLC)
+  APH_FLUX_CASE(RECON_PPM, RS_LHLLC)
+  APH_FLUX_CASE(RECON_DC, RS_LHLLC)
+  APH_FLUX_CASE(RECON_DC, RS_NONE)  // the only "none" pairing registered (hydro.cpp:112)
 #undef APH_FLUX_CASE
   return set_error(ctx, APH_E_INVALID, "unsupported (reconstruction=%d, riemann=%d)", recon,
                    rsolver);

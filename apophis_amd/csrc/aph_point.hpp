@@ -36,6 +36,7 @@ APH_HD double mx(double a, double b) { return (a < b) ? b : a; }
 
 constexpr int RECON_DC = APH_RECON_DC, RECON_PLM = APH_RECON_PLM, RECON_PPM = APH_RECON_PPM;
 constexpr int RS_HLLE = APH_RIEMANN_HLLE, RS_HLLC = APH_RIEMANN_HLLC;
+constexpr int RS_LHLLC = APH_RIEMANN_LHLLC, RS_NONE = APH_RIEMANN_NONE;
 
 template <int RECON>
 struct ReconTraits {
@@ -276,8 +277,11 @@ APH_HD void hlle(const double *wl, const double *wr, const double *ifl,
   for (int n = 0; n < 5; ++n) flx[n] = 0.5 * (fl[n] + fr[n]) + (fl[n] - fr[n]) * tmp;
 }
 
-// ------------------------------------------------------------ HLLC
-// src/hydro/rsolvers/hydro_hllc.hpp:161-282
+// ------------------------------------------------------------ HLLC / LHLLC
+// src/hydro/rsolvers/hydro_hllc.hpp:161-282; with LOWMACH, hydro_lhllc.hpp:12-158, whose
+// only change is the factor phi = chi (2 - chi), chi = max(|v_L|, |v_R|) (a velocity
+// magnitude, not a Mach number, and not clipped), on the PVRS term of p*.
+template <bool LOWMACH = false>
 APH_HD void hllc(const double *wl, const double *wr, const double *ifl,
                                      const double *ifr, double *flx) {
   double gammal = ifl[0], gammar = ifr[0];
@@ -289,7 +293,14 @@ APH_HD void hllc(const double *wl, const double *wr, const double *ifl,
   double er = wr[4] * igm1r + 0.5 * wr[0] * (SQR(wr[1]) + SQR(wr[2]) + SQR(wr[3]));
   double rhoa = .5 * (wl[0] + wr[0]);
   double ca = .5 * (cl + cr);
-  double pstar = .5 * (wl[4] + wr[4]) - .5 * (wr[1] - wl[1]) * rhoa * ca;
+  double pvrs = .5 * (wr[1] - wl[1]) * rhoa * ca;
+  if constexpr (LOWMACH) {
+    const double machl = sqrt(wl[1] * wl[1] + wl[2] * wl[2] + wl[3] * wl[3]);
+    const double machr = sqrt(wr[1] * wr[1] + wr[2] * wr[2] + wr[3] * wr[3]);
+    const double chi = mx(machl, machr);
+    pvrs = pvrs * (chi * (2.0 - chi));
+  }
+  double pstar = .5 * (wl[4] + wr[4]) - pvrs;
   // (gammal + 1) / (2 * gammar) is the same quotient on both sides: evaluated once
   double ql = 1.0, qr = 1.0;
   const bool shl = !(pstar <= wl[4]), shr = !(pstar <= wr[4]);
@@ -342,10 +353,16 @@ APH_HD void hllc(const double *wl, const double *wr, const double *ifl,
 template <int RS>
 APH_HD void riemann(const double *wl, const double *wr, const double *ifl,
                                         const double *ifr, double *flx) {
-  if constexpr (RS == RS_HLLC)
-    hllc(wl, wr, ifl, ifr, flx);
-  else
+  if constexpr (RS == RS_HLLC) {
+    hllc<false>(wl, wr, ifl, ifr, flx);
+  } else if constexpr (RS == RS_LHLLC) {
+    hllc<true>(wl, wr, ifl, ifr, flx);
+  } else if constexpr (RS == RS_HLLE) {
     hlle(wl, wr, ifl, ifr, flx);
+  } else {  // none (hydro_none.hpp:17-35): the hydro fluxes are zero
+#pragma unroll
+    for (int n = 0; n < 5; ++n) flx[n] = 0.0;
+  }
 }
 
 // ------------------------------------------------------------ ConsToPrim point

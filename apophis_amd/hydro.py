@@ -72,7 +72,16 @@ class ApophisError(RuntimeError):
 
 
 # upstream "rk2" low-storage coefficients
-RK2 = {"gam0": (0.0, 0.5), "gam1": (1.0, 0.5), "beta": (1.0, 0.5), "nstages": 2}
+# Upstream Parthenon low-storage integrators (stage k: u0 <- gam0 u0 + gam1 u1 +
+# beta dt div F(u0), with u1 the step's starting state).
+INTEGRATORS = {
+    "rk1": {"gam0": (0.0,), "gam1": (1.0,), "beta": (1.0,), "nstages": 1},
+    "rk2": {"gam0": (0.0, 0.5), "gam1": (1.0, 0.5), "beta": (1.0, 0.5), "nstages": 2},
+    "rk3": {"gam0": (0.0, 0.25, 2.0 / 3.0), "gam1": (1.0, 0.75, 1.0 / 3.0),
+            "beta": (1.0, 0.25, 2.0 / 3.0), "nstages": 3},
+    "vl2": {"gam0": (0.0, 0.0), "gam1": (1.0, 1.0), "beta": (0.5, 1.0), "nstages": 2},
+}
+RK2 = INTEGRATORS["rk2"]
 
 
 def _ptr(t):
@@ -124,8 +133,7 @@ def _make_flux_task(recon, rsolver):
     return calculate_fluxes
 
 
-# hydro.cpp:103-112 registers these ten combinations; the ones the flux kernels do not
-# implement (LHLLC, none) are registered as functions that fail loudly when called.
+# hydro.cpp:103-112 registers these ten combinations.
 for _r in (Reconstruction.dc, Reconstruction.plm, Reconstruction.ppm):
     for _s in (RiemannSolver.hllc, RiemannSolver.hlle, RiemannSolver.lhllc):
         add_flux_fun(Fluid.euler, _r, _s)
@@ -276,7 +284,8 @@ def global_quant_hst(md, var, out=None):
 
 # ------------------------------------------------------------------ drivers
 class ApophisDriver:
-    """RK2 stage sequence of ApophisDriver::MakeTaskCollection, plain form.
+    """Stage sequence of ApophisDriver::MakeTaskCollection, plain form, for the
+    package's integrator (rk1 / rk2 / rk3 / vl2).
 
     pack_size: blocks per partition (None = all blocks in one pack, the default
     Parthenon partitioning); each task is issued once per partition."""
@@ -292,8 +301,7 @@ class ApophisDriver:
         ps = pack_size or nb
         self.parts = [md.pack(s, min(ps, nb - s)) for s in range(0, nb, ps)]
         self.dt_dev = torch.empty(1, dtype=torch.float64, device=md.cons.device)
-        if pkg.param("integrator") != Integrator.rk2:
-            raise ApophisError("only the rk2 integrator is implemented by this driver")
+        self.rk = INTEGRATORS[pkg.param("integrator").name]
 
     def initialize(self):
         """Ghost fill + FillDerived after the problem generator (PostInitialization)."""
@@ -316,23 +324,24 @@ class ApophisDriver:
             self.u1.copy_(self.md.cons)
         flux_fun = self.pkg.param("flux_first_stage" if stage == 1 else "flux_other_stage")
         grav = any(v != 0.0 for v in self.g)
+        rk = self.rk
         for p in self.parts:
             flux_fun(p)                                                      # :70-72
-            update_with_flux_divergence(p, self.u1, RK2["gam0"][s], RK2["gam1"][s],
-                                        RK2["beta"][s] * dt)                 # :79-82
+            update_with_flux_divergence(p, self.u1, rk["gam0"][s], rk["gam1"][s],
+                                        rk["beta"][s] * dt)                  # :79-82
             if grav:
-                apply_gravity(p, self.g, RK2["beta"][s] * dt)                # :84-85
+                apply_gravity(p, self.g, rk["beta"][s] * dt)                 # :84-85
         for p in self.parts:
             fill_ghosts(p)                                                   # :94, :103
         for p in self.parts:
             cons_to_prim(p, self.eos)                                        # :110-111
-        if stage == RK2["nstages"]:
+        if stage == rk["nstages"]:
             return self.new_dt()                                             # :113-116
         return None
 
     def step(self, dt):
         est = None
-        for stage in (1, 2):
+        for stage in range(1, self.rk["nstages"] + 1):
             est = self.make_task_collection(stage, dt)
         return est
 
@@ -361,7 +370,7 @@ class FusedDriver:
     stage); u1 holds the stage-1 result.  Ghost cells are never read."""
 
     def __init__(self, ctx, bs: BlockSet, cons, recon, riemann, eos, cfl, g=(0.0, 0.0, 0.0),
-                 tlim=np.inf, device="cuda", dbs=None):
+                 tlim=np.inf, device="cuda", dbs=None, integrator="rk2"):
         self.ctx = ctx
         self.bs = bs
         self.device_bs = dbs or DeviceBlockSet(ctx, bs)
@@ -379,6 +388,15 @@ class FusedDriver:
         ctx.check(_lib.lib().aph_plan_create(ctx.h, self.device_bs.h, r, s,
                                              eos.ctypes.data_as(ctypes.c_void_p), garr,
                                              ctypes.byref(self.plan)))
+        self.graph_steps = 0
+        if integrator != "rk2":
+            self.set_integrator(integrator)
+
+    def set_integrator(self, name):
+        """rk1 / rk2 / rk3 / vl2 (aph_plan_set_integrator)."""
+        self.ctx.check(_lib.lib().aph_plan_set_integrator(self.ctx.h, self.plan,
+                                                          int(Integrator[name])))
+        self.integrator = name
         self.graph_steps = 0
 
     def set_tile(self, n, tx, ty, tz):

@@ -37,6 +37,7 @@ class Problem:
     nsteps: int = 0
     g: tuple = (0.0, 0.0, 0.0)
     meta: dict = dataclasses.field(default_factory=dict)
+    integrator: str = "rk2"  # rk1 / rk2 / rk3 / vl2 (main.hpp Integrator)
 
 
 # ---------------------------------------------------------------- RNG

@@ -53,6 +53,8 @@ extern "C" {
 enum aph_reconstruction { APH_RECON_DC = 1, APH_RECON_PLM = 2, APH_RECON_PPM = 3 };
 enum aph_riemann { APH_RIEMANN_NONE = 1, APH_RIEMANN_HLLE = 2, APH_RIEMANN_HLLC = 3,
                    APH_RIEMANN_LHLLC = 4 };
+enum aph_integrator { APH_INTEGRATOR_RK1 = 1, APH_INTEGRATOR_RK2 = 2, APH_INTEGRATOR_RK3 = 3,
+                      APH_INTEGRATOR_VL2 = 4 };
 /* Physical boundary applied on a face whose neighbour entry is -1.
    Periodicity is expressed through the neighbour table instead. */
 enum aph_bc { APH_BC_OUTFLOW = 1, APH_BC_REFLECT = 2 };
@@ -186,12 +188,21 @@ int aph_plan_autotune(aph_context *ctx, aph_plan *plan, const double *u_in, doub
                       const double *dt_dev, int32_t reps, char *report, size_t report_len);
 /* Tile currently used for block `block` of the plan's block set. */
 int aph_plan_get_tile(aph_context *ctx, aph_plan *plan, int32_t block, aph_tile *out);
+/* Time integrator of the plan (default rk2): the upstream Parthenon low-storage
+ * schemes rk1 / rk2 / rk3 / vl2 that ApophisDriver uses (apophis_driver.cpp:78-85,
+ * hydro.cpp:114-139; vl2 predicts with DC reconstruction).  rk3 allocates one more
+ * state register inside the plan.  Synchronous.  The adjoint entry points need rk2. */
+int aph_plan_set_integrator(aph_context *ctx, aph_plan *plan, int32_t integrator);
+/* Stage `stage` (1..number of stages of the plan's integrator):
+ * u_out = gam0*u_in + gam1*u_n + beta*dt*(-div F(u_in)) [+ gravity]; u_n (the step's
+ * start state) is unused in stage 1. */
 int aph_fused_stage(aph_context *ctx, aph_plan *plan, int32_t stage, double *u_in,
                     const double *u_n, double *u_out, const double *dt_dev);
 
 /* Device-resident time state used by aph_plan_step / the captured graph:
-   dt_dev[0] = current dt, time_dev[0] = simulation time.  One RK2 step:
-     stage 1 (U0 -> U1), stage 2 (U1, U0 -> U0), then
+   dt_dev[0] = current dt, time_dev[0] = simulation time.  One step of the plan's
+   integrator -- rk2 / vl2: stage 1 (U0 -> U1), stage 2 (U1, U0 -> U0); rk3: U0 -> U1,
+   (U1, U0) -> U2 (inside the plan), (U2, U0) -> U0; rk1: U0 -> U1 copied to U0 -- then
      dt <- min(2*dt, cfl * min_est), clipped so that time + dt <= tlim, time += dt_old.
    When fixed_dt != 0 the dt is left unchanged (frozen-dt rollouts). */
 int aph_plan_step(aph_context *ctx, aph_plan *plan, double *u0, double *u1,

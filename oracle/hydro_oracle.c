@@ -269,6 +269,75 @@ void orc_hllc(const double *wl, const double *wr, const double *ifl, const doubl
   flx[4] = ul * fl[4] + ur * fr[4] + ustar * cp * sstar;
 }
 
+/* ----------------------------------------------------------- LHLLC (a5, f)
+ * src/hydro/rsolvers/hydro_lhllc.hpp:12-158: HLLC with a low-Mach factor on the PVRS
+ * middle pressure.  "mach" there is the velocity magnitude |v| (not |v|/c) and
+ * phi = chi (2 - chi) is not clipped: both quirks are kept. */
+void orc_lhllc(const double *wl, const double *wr, const double *ifl, const double *ifr,
+               double *flx) {
+  double gammal = ifl[0], gammar = ifr[0];
+  double gm1l = ifl[1] - 1, gm1r = ifr[1] - 1;
+  double igm1l = 1.0 / gm1l, igm1r = 1.0 / gm1r;
+  double cl = C_LIGHT * sqrt(gammal / (1 + (igm1l + wl[0] * SQR(C_LIGHT) / wl[4])));
+  double cr = C_LIGHT * sqrt(gammar / (1 + (igm1r + wr[0] * SQR(C_LIGHT) / wr[4])));
+  double el = wl[4] * igm1l + 0.5 * wl[0] * (SQR(wl[1]) + SQR(wl[2]) + SQR(wl[3]));
+  double er = wr[4] * igm1r + 0.5 * wr[0] * (SQR(wr[1]) + SQR(wr[2]) + SQR(wr[3]));
+  double rhoa = .5 * (wl[0] + wr[0]);
+  double ca = .5 * (cl + cr);
+  double machl = sqrt(wl[1] * wl[1] + wl[2] * wl[2] + wl[3] * wl[3]);
+  double machr = sqrt(wr[1] * wr[1] + wr[2] * wr[2] + wr[3] * wr[3]);
+  double chi = mx(machl, machr);
+  double phi = chi * (2.0 - chi);
+  double pstar = .5 * (wl[4] + wr[4]) - .5 * (wr[1] - wl[1]) * rhoa * ca * phi;
+  double ql = (pstar <= wl[4])
+                  ? 1.0
+                  : sqrt(1.0 + (gammal + 1) / (2 * gammar) * (pstar / wl[4] - 1.0));
+  double qr = (pstar <= wr[4])
+                  ? 1.0
+                  : sqrt(1.0 + (gammal + 1) / (2 * gammar) * (pstar / wr[4] - 1.0));
+  double sl = wl[1] - cl * ql;
+  double sr = wr[1] + cr * qr;
+  double bp = sr > 0.0 ? sr : (TINY_NUMBER);
+  double bm = sl < 0.0 ? sl : -(TINY_NUMBER);
+  double vxl = wl[1] - sl;
+  double vxr = wr[1] - sr;
+  double tl = wl[4] + vxl * wl[0] * wl[1];
+  double tr = wr[4] + vxr * wr[0] * wr[1];
+  double ml = wl[0] * vxl;
+  double mr = -(wr[0] * vxr);
+  double sstar = (tl - tr) / (ml + mr);
+  double cp = (ml * tr + mr * tl) / (ml + mr);
+  cp = cp > 0.0 ? cp : 0.0;
+  vxl = wl[1] - bm;
+  vxr = wr[1] - bp;
+  double fl[5], fr[5];
+  fl[0] = wl[0] * vxl;
+  fr[0] = wr[0] * vxr;
+  fl[1] = wl[0] * wl[1] * vxl + wl[4];
+  fr[1] = wr[0] * wr[1] * vxr + wr[4];
+  fl[2] = wl[0] * wl[2] * vxl;
+  fr[2] = wr[0] * wr[2] * vxr;
+  fl[3] = wl[0] * wl[3] * vxl;
+  fr[3] = wr[0] * wr[3] * vxr;
+  fl[4] = el * vxl + wl[4] * wl[1];
+  fr[4] = er * vxr + wr[4] * wr[1];
+  double ur, ul, ustar;
+  if (sstar >= 0.0) {
+    ul = sstar / (sstar - bm);
+    ur = 0.0;
+    ustar = -bm / (sstar - bm);
+  } else {
+    ul = 0.0;
+    ur = -sstar / (bp - sstar);
+    ustar = bp / (bp - sstar);
+  }
+  flx[0] = ul * fl[0] + ur * fr[0];
+  flx[1] = ul * fl[1] + ur * fr[1] + ustar * cp;
+  flx[2] = ul * fl[2] + ur * fr[2];
+  flx[3] = ul * fl[3] + ur * fr[3];
+  flx[4] = ul * fl[4] + ur * fr[4] + ustar * cp * sstar;
+}
+
 /* ------------------------------------------------------- ConsToPrim (a9)
  * src/hydro/hydro.cpp:450-532 with the singularity-eos IdealGas closed forms
  * (upstream): P = max(SMALL, gm1*rho*sie), K = max(SMALL, gm1*(gm1+1)*rho*sie). */
@@ -388,8 +457,12 @@ static void flux_block_dir(const aph_block *bk, int ndim, int nvar, int recon,
         }
         if (rsolver == APH_RIEMANN_HLLC)
           orc_hllc(wl, wr, ifl, ifr, fx);
-        else
+        else if (rsolver == APH_RIEMANN_LHLLC)
+          orc_lhllc(wl, wr, ifl, ifr, fx);
+        else if (rsolver == APH_RIEMANN_HLLE)
           orc_hlle(wl, wr, ifl, ifr, fx);
+        else /* none: hydro_none.hpp:17-35 zeroes the hydro fluxes */
+          for (int n = 0; n < 5; ++n) fx[n] = 0.0;
         long off = base + (long)c * st[dir];
         f[off] = fx[0];
         f[(long)ivx * nc + off] = fx[1];
@@ -411,7 +484,9 @@ int orc_calculate_fluxes(const aph_block *blocks, int nb, int ndim, int nvar, in
                          int rsolver, const double *prim, const double *gamma,
                          double *fx1, double *fx2, double *fx3) {
   if (recon != APH_RECON_DC && recon != APH_RECON_PLM && recon != APH_RECON_PPM) return -1;
-  if (rsolver != APH_RIEMANN_HLLE && rsolver != APH_RIEMANN_HLLC) return -1;
+  if (rsolver != APH_RIEMANN_HLLE && rsolver != APH_RIEMANN_HLLC &&
+      rsolver != APH_RIEMANN_LHLLC && rsolver != APH_RIEMANN_NONE)
+    return -1;
   const int need = recon == APH_RECON_PPM ? 3 : (recon == APH_RECON_PLM ? 2 : 1);
   for (int b = 0; b < nb; ++b)
     if (blocks[b].ng < need) return -2; /* hydro.cpp:72-75 */
@@ -617,20 +692,50 @@ double orc_global_sum(const aph_block *blocks, int nb, int ndim, int nvar,
   return s;
 }
 
-/* ------------------------------------------------------- RK2 step (a8)
+/* ------------------------------------------------------- RK step (a8, f)
  * ApophisDriver::MakeTaskCollection (apophis_driver.cpp:27-130) with the upstream
- * "rk2" low-storage coefficients gam0 = {0, 1/2}, gam1 = {1, 1/2}, beta = {1, 1/2}. */
-double orc_rk2_step(const aph_block *blocks, int nb, int ndim, int nvar, int recon,
-                    int rsolver, const aph_eos *eos, const double *g, double cfl,
-                    double dt, double *u0, double *u1, double *prim, double *gamma,
-                    double *fx1, double *fx2, double *fx3) {
-  static const double gam0[2] = {0.0, 0.5}, gam1[2] = {1.0, 0.5}, beta[2] = {1.0, 0.5};
+ * Parthenon low-storage coefficients of each integrator (integrator enum = main.hpp):
+ *   rk1: gam0 = {0},          gam1 = {1},            beta = {1}
+ *   rk2: gam0 = {0, 1/2},     gam1 = {1, 1/2},       beta = {1, 1/2}
+ *   rk3: gam0 = {0, 1/4, 2/3}, gam1 = {1, 3/4, 1/3}, beta = {1, 1/4, 2/3}
+ *   vl2: gam0 = {0, 0},       gam1 = {1, 1},         beta = {1/2, 1}
+ * and the first stage's flux function: DC reconstruction for vl2 (hydro.cpp:114-139). */
+int orc_integrator_stages(int integrator, double *gam0, double *gam1, double *beta) {
+  switch (integrator) {
+    case 1:
+      gam0[0] = 0.0; gam1[0] = 1.0; beta[0] = 1.0;
+      return 1;
+    case 2:
+      gam0[0] = 0.0; gam1[0] = 1.0; beta[0] = 1.0;
+      gam0[1] = 0.5; gam1[1] = 0.5; beta[1] = 0.5;
+      return 2;
+    case 3:
+      gam0[0] = 0.0; gam1[0] = 1.0; beta[0] = 1.0;
+      gam0[1] = 0.25; gam1[1] = 0.75; beta[1] = 0.25;
+      gam0[2] = 2.0 / 3.0; gam1[2] = 1.0 / 3.0; beta[2] = 2.0 / 3.0;
+      return 3;
+    case 4:
+      gam0[0] = 0.0; gam1[0] = 1.0; beta[0] = 0.5;
+      gam0[1] = 0.0; gam1[1] = 1.0; beta[1] = 1.0;
+      return 2;
+    default:
+      return 0;
+  }
+}
+
+double orc_step(const aph_block *blocks, int nb, int ndim, int nvar, int integrator,
+                int recon, int rsolver, const aph_eos *eos, const double *g, double cfl,
+                double dt, double *u0, double *u1, double *prim, double *gamma, double *fx1,
+                double *fx2, double *fx3) {
+  double gam0[3], gam1[3], beta[3];
+  const int nstages = orc_integrator_stages(integrator, gam0, gam1, beta);
   long ntot = 0;
   for (int b = 0; b < nb; ++b) ntot += ncell(&blocks[b], ndim);
   memcpy(u1, u0, sizeof(double) * (size_t)nvar * ntot); /* :44-55 */
   const int grav = g && (g[0] != 0.0 || g[1] != 0.0 || g[2] != 0.0);
-  for (int s = 0; s < 2; ++s) {
-    orc_calculate_fluxes(blocks, nb, ndim, nvar, recon, rsolver, prim, gamma, fx1, fx2,
+  for (int s = 0; s < nstages; ++s) {
+    const int rec = (s == 0 && integrator == 4) ? APH_RECON_DC : recon;
+    orc_calculate_fluxes(blocks, nb, ndim, nvar, rec, rsolver, prim, gamma, fx1, fx2,
                          fx3);                                           /* :70-72 */
     orc_update(blocks, nb, ndim, nvar, u0, u1, fx1, fx2, fx3, gam0[s], gam1[s],
                beta[s] * dt);                                            /* :79-82 */
@@ -639,4 +744,12 @@ double orc_rk2_step(const aph_block *blocks, int nb, int ndim, int nvar, int rec
     orc_cons_to_prim(blocks, nb, ndim, nvar, eos, u0, prim, gamma);      /* :110-111 */
   }
   return orc_estimate_timestep(blocks, nb, ndim, nvar, eos, prim, u0, cfl); /* :113-116 */
+}
+
+double orc_rk2_step(const aph_block *blocks, int nb, int ndim, int nvar, int recon,
+                    int rsolver, const aph_eos *eos, const double *g, double cfl,
+                    double dt, double *u0, double *u1, double *prim, double *gamma,
+                    double *fx1, double *fx2, double *fx3) {
+  return orc_step(blocks, nb, ndim, nvar, 2, recon, rsolver, eos, g, cfl, dt, u0, u1, prim,
+                  gamma, fx1, fx2, fx3);
 }

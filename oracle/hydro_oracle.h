@@ -26,6 +26,8 @@ void orc_ppm(double q_im2, double q_im1, double q_i, double q_ip1, double q_ip2,
 /* w* = (rho, v_normal, v_t1, v_t2, p) already permuted; if* = (gamma_c, gamma_e). */
 void orc_hlle(const double *wl, const double *wr, const double *ifl, const double *ifr,
               double *flx);
+void orc_lhllc(const double *wl, const double *wr, const double *ifl, const double *ifr,
+               double *flx);
 void orc_hllc(const double *wl, const double *wr, const double *ifl, const double *ifr,
               double *flx);
 
@@ -56,6 +58,14 @@ double orc_rk2_step(const aph_block *blocks, int nb, int ndim, int nvar, int rec
                     int rsolver, const aph_eos *eos, const double *g, double cfl,
                     double dt, double *u0, double *u1, double *prim, double *gamma,
                     double *fx1, double *fx2, double *fx3);
+/* One step of `integrator` (1 rk1, 2 rk2, 3 rk3, 4 vl2 -- main.hpp values); the same
+ * contract as orc_rk2_step.  orc_integrator_stages fills the per-stage coefficients and
+ * returns the stage count (0 for an unknown integrator). */
+double orc_step(const aph_block *blocks, int nb, int ndim, int nvar, int integrator,
+                int recon, int rsolver, const aph_eos *eos, const double *g, double cfl,
+                double dt, double *u0, double *u1, double *prim, double *gamma, double *fx1,
+                double *fx2, double *fx3);
+int orc_integrator_stages(int integrator, double *gam0, double *gam1, double *beta);
 
 #ifdef __cplusplus
 }

@@ -36,7 +36,7 @@ def lib():
         L = ctypes.CDLL(_LIB_PATH)
         L.orc_plm.argtypes = [ctypes.c_double] * 3 + [_dp, _dp]
         L.orc_ppm.argtypes = [ctypes.c_double] * 5 + [_dp, _dp]
-        for f in (L.orc_hlle, L.orc_hllc):
+        for f in (L.orc_hlle, L.orc_hllc, L.orc_lhllc):
             f.argtypes = [_dp] * 5
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_cons_to_prim.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp,
@@ -55,6 +55,11 @@ def lib():
         L.orc_rk2_step.argtypes = ([_vp] + [ctypes.c_int] * 5 + [_vp, _dp, ctypes.c_double,
                                                                  ctypes.c_double] + [_dp] * 7)
         L.orc_rk2_step.restype = ctypes.c_double
+        L.orc_step.argtypes = ([_vp] + [ctypes.c_int] * 6 + [_vp, _dp, ctypes.c_double,
+                                                             ctypes.c_double] + [_dp] * 7)
+        L.orc_step.restype = ctypes.c_double
+        L.orc_integrator_stages.argtypes = [ctypes.c_int, _dp, _dp, _dp]
+        L.orc_integrator_stages.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -87,14 +92,22 @@ def riemann(kind, wl, wr, ifl, ifr):
     wl, wr = np.ascontiguousarray(wl, float), np.ascontiguousarray(wr, float)
     ifl, ifr = np.ascontiguousarray(ifl, float), np.ascontiguousarray(ifr, float)
     out = np.zeros(5)
-    f = lib().orc_hllc if kind == "hllc" else lib().orc_hlle
+    f = {"hllc": lib().orc_hllc, "lhllc": lib().orc_lhllc, "hlle": lib().orc_hlle}[kind]
     f(_p(wl), _p(wr), _p(ifl), _p(ifr), _p(out))
     return out
 
 
+def integrator_stages(name):
+    """(gam0, gam1, beta) of the named integrator's stages."""
+    g0, g1, be = np.zeros(3), np.zeros(3), np.zeros(3)
+    n = lib().orc_integrator_stages(INTEGRATOR[name], _p(g0), _p(g1), _p(be))
+    return g0[:n], g1[:n], be[:n]
+
+
 # ------------------------------------------------------------ mesh functions
 RECON = {"dc": 1, "plm": 2, "ppm": 3}
-RIEMANN = {"hlle": 2, "hllc": 3}
+RIEMANN = {"none": 1, "hlle": 2, "hllc": 3, "lhllc": 4}
+INTEGRATOR = {"rk1": 1, "rk2": 2, "rk3": 3, "vl2": 4}
 
 
 def set_threads(n):
@@ -126,11 +139,11 @@ class State:
         need = {"dc": 1, "plm": 2, "ppm": 3}[p.recon]
         if int(self.bs.blocks["ng"].min()) < need:
             raise ValueError("Not enough ghost zones for reconstruction")
-        return lib().orc_rk2_step(_bp(self.bs), self.bs.nblocks, self.bs.ndim, self.bs.nvar,
-                                  RECON[p.recon], RIEMANN[p.riemann],
-                                  p.eos.ctypes.data_as(_vp), _p(self.g), p.cfl, dt,
-                                  _p(self.u0), _p(self.u1), _p(self.prim), _p(self.gamma),
-                                  _p(self.fx[0]), _p(self.fx[1]), _p(self.fx[2]))
+        return lib().orc_step(_bp(self.bs), self.bs.nblocks, self.bs.ndim, self.bs.nvar,
+                              INTEGRATOR[p.integrator], RECON[p.recon], RIEMANN[p.riemann],
+                              p.eos.ctypes.data_as(_vp), _p(self.g), p.cfl, dt,
+                              _p(self.u0), _p(self.u1), _p(self.prim), _p(self.gamma),
+                              _p(self.fx[0]), _p(self.fx[1]), _p(self.fx[2]))
 
     def run(self, nsteps=None, tlim=None, fixed_dt=None):
         """Evolve with the upstream dt policy (<= 2x growth, clip to tlim)."""

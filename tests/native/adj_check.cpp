@@ -13,9 +13,13 @@ void chk_plm_adj(const double *q, const double *ob, double *qb) { plm_adj(q, ob[
 // x = (wl[5], wr[5], ifl[2], ifr[2]) -> flux[5]
 void chk_riemann(int kind, const double *x, double *f) {
   if (kind == RS_HLLC)
-    hllc(x, x + 5, x + 10, x + 12, f);
+    riemann<RS_HLLC>(x, x + 5, x + 10, x + 12, f);
+  else if (kind == RS_LHLLC)
+    riemann<RS_LHLLC>(x, x + 5, x + 10, x + 12, f);
+  else if (kind == RS_HLLE)
+    riemann<RS_HLLE>(x, x + 5, x + 10, x + 12, f);
   else
-    hlle(x, x + 5, x + 10, x + 12, f);
+    riemann<RS_NONE>(x, x + 5, x + 10, x + 12, f);
 }
 void chk_riemann_adj(int kind, const double *x, const double *fb, double *xb) {
   if (kind == RS_HLLC)

@@ -110,7 +110,7 @@ def euler_flux(w, gamma):
     return np.array([rho * u, rho * u * u + p, rho * u * v, rho * u * w3, (E + p) * u])
 
 
-@pytest.mark.parametrize("kind", ["hlle", "hllc"])
+@pytest.mark.parametrize("kind", ["hlle", "hllc", "lhllc"])
 def test_riemann_consistency_identical_states(kind):
     for _ in range(200):
         g = rng.choice([1.4, 5.0 / 3.0])
@@ -121,11 +121,13 @@ def test_riemann_consistency_identical_states(kind):
                                    atol=1e-13 * np.abs(euler_flux(w, g)).max())
 
 
-@pytest.mark.parametrize("kind", ["hlle", "hllc"])
+@pytest.mark.parametrize("kind", ["hlle", "hllc", "lhllc"])
 def test_riemann_supersonic_upwind(kind):
     g = 1.4
-    wl = np.array([1.0, 5.0, 0.3, -0.2, 1.0])   # u = 5 >> c ~ 1.18
-    wr = np.array([0.5, 4.0, 0.1, 0.0, 0.4])
+    # cold flow: u = 1.5 >> c ~ 0.12 (speeds below 2 keep LHLLC's phi = chi (2 - chi)
+    # positive; see test_oracle_variants.py)
+    wl = np.array([1.0, 1.5, 0.3, -0.2, 0.01])
+    wr = np.array([0.5, 1.2, 0.1, 0.0, 0.004])
     f = orc.riemann(kind, wl, wr, [g, g], [g, g])
     np.testing.assert_allclose(f, euler_flux(wl, g), rtol=1e-13)
     f = orc.riemann(kind, wl * [1, -1, 1, 1, 1], wr * [1, -1, 1, 1, 1], [g, g], [g, g])
@@ -143,7 +145,7 @@ def test_hllc_resolves_stationary_contact_exactly():
     assert abs(orc.riemann("hlle", wl, wr, [g, g], [g, g])[0]) > 1e-3
 
 
-@pytest.mark.parametrize("kind", ["hlle", "hllc"])
+@pytest.mark.parametrize("kind", ["hlle", "hllc", "lhllc"])
 def test_riemann_mirror_symmetry(kind):
     g = 5.0 / 3.0
     for _ in range(300):

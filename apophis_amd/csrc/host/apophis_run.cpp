@@ -51,8 +51,18 @@ int main(int argc, char **argv) {
     auto pkg = InitializeHydro(&pin);
     const aph_eos eos = pkg->Param<aph_eos>("eos");
     const Real cfl = pkg->Param<Real>("cfl");
-    if (pkg->Param<Integrator>("integrator") != Integrator::rk2)
-      throw ApophisError("this driver implements the rk2 integrator");
+    // upstream low-storage stage tables: {gam0, gam1, beta} per stage
+    struct Stage { Real gam0, gam1, beta; };
+    std::vector<Stage> stages;
+    switch (pkg->Param<Integrator>("integrator")) {
+      case Integrator::rk1: stages = {{0.0, 1.0, 1.0}}; break;
+      case Integrator::rk2: stages = {{0.0, 1.0, 1.0}, {0.5, 0.5, 0.5}}; break;
+      case Integrator::rk3:
+        stages = {{0.0, 1.0, 1.0}, {0.25, 0.75, 0.25}, {2.0 / 3.0, 1.0 / 3.0, 2.0 / 3.0}};
+        break;
+      case Integrator::vl2: stages = {{0.0, 1.0, 0.5}, {0.0, 1.0, 1.0}}; break;
+      default: throw ApophisError("[Apophis]: Integrator not recognized. Exiting.");
+    }
 
     // mesh: 1D lattice of nx1/mb_nx1 blocks
     const int ng = pin.GetInteger("parthenon/mesh", "nghost");
@@ -144,7 +154,6 @@ int main(int argc, char **argv) {
     Real est = EstimateTimestep(md.get(), eos, cfl, dt_dev);
     const Real tlim = pin.GetOrAddReal("parthenon/time", "tlim", 0.2);
     const int nlim = pin.GetOrAddInteger("parthenon/time", "nlim", 100000);
-    static const Real gam0[2] = {0.0, 0.5}, gam1[2] = {1.0, 0.5}, beta[2] = {1.0, 0.5};
     Real t = 0.0, dt = std::numeric_limits<Real>::infinity();
     int n = 0;
     while (t < tlim * (1 - 1e-14) && n < nlim) {
@@ -152,10 +161,11 @@ int main(int argc, char **argv) {
       if (t < tlim && tlim - t < dt) dt = tlim - t;
       hip_ok(hipMemcpyAsync(u1, md->cons, sizeof(Real) * nfield, hipMemcpyDeviceToDevice,
                             stream), "u1 copy");
-      for (int s = 0; s < 2; ++s) {
+      for (size_t s = 0; s < stages.size(); ++s) {
         auto *flux = pkg->Param<FluxFun_t *>(s == 0 ? "flux_first_stage" : "flux_other_stage");
         flux(md);
-        UpdateWithFluxDivergence(md.get(), u1, gam0[s], gam1[s], beta[s] * dt);
+        UpdateWithFluxDivergence(md.get(), u1, stages[s].gam0, stages[s].gam1,
+                                 stages[s].beta * dt);
         FillGhosts(md);
         ConsToPrim(md.get(), eos);
       }

@@ -1,6 +1,7 @@
 """The C++ host adapter (reference task shapes: CalculateFluxes<F,R,S>(shared_ptr<MeshData>&),
 FluxFun_t registry, InitializeHydro from an input deck) driving the Sod workload through
 the C ABI matches the oracle."""
+import dataclasses
 import os
 import subprocess
 
@@ -26,13 +27,18 @@ def _run(tmp_path, *overrides):
     return r, out
 
 
-@pytest.mark.parametrize("recon,riemann,nblock", [("plm", "hlle", 256), ("ppm", "hllc", 64)])
-def test_cpp_driver_sod_matches_oracle(tmp_path, recon, riemann, nblock):
+@pytest.mark.parametrize("recon,riemann,nblock,integrator",
+                         [("plm", "hlle", 256, "rk2"), ("ppm", "hllc", 64, "rk2"),
+                          ("plm", "lhllc", 64, "rk3"), ("ppm", "hllc", 128, "vl2"),
+                          ("dc", "hlle", 256, "rk1")])
+def test_cpp_driver_sod_matches_oracle(tmp_path, recon, riemann, nblock, integrator):
     ng = 3 if recon == "ppm" else 2
     r, out = _run(tmp_path, f"hydro/reconstruction={recon}", f"hydro/riemann={riemann}",
-                  f"parthenon/mesh/nghost={ng}", f"parthenon/meshblock/nx1={nblock}")
+                  f"parthenon/mesh/nghost={ng}", f"parthenon/meshblock/nx1={nblock}",
+                  f"parthenon/time/integrator={integrator}")
     assert r.returncode == 0, r.stderr
-    p = problems.sod(recon=recon, riemann=riemann, nblock_cells=nblock, ng=ng)
+    p = dataclasses.replace(problems.sod(recon=recon, riemann=riemann, nblock_cells=nblock,
+                                         ng=ng), integrator=integrator)
     ref = orc.State(p)
     t, n = ref.run(tlim=p.tlim)
     assert f"steps={n} " in r.stdout

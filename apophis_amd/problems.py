@@ -338,3 +338,66 @@ def linear_wave(ndim=1, res=32, amp=1.0e-6, recon="plm", riemann="hllc", cfl=0.3
     return Problem("linear_wave", bs, cons, make_eos(gm1), recon, riemann, cfl,
                    tlim=ncycles * period,
                    meta={"exact": np.stack(st), "amp": amp, "period": period})
+
+
+def linear_wave3d(res=64, recon="plm", riemann="hllc", integrator="vl2", amp=1.0e-6,
+                  wave_flag=0, gamma=1.666666666666667, eos_gm1=0.66666666667, cfl=0.3,
+                  tlim_periods=2.0, block=None):
+    """The reference's 3D linear-wave convergence problem (inputs/linear_wave3d.in with
+    the overrides of convergence.py:55-80): a (2 res) x res x res periodic mesh on
+    [0,3]x[0,1.5]x[0,1.5], a hydro wave along the grid diagonal (linear_wave.cpp:42-147,
+    304-338), amplitude 1e-6, run for tlim_periods wave periods.  The problem generator
+    uses hydro/gamma = 1.666666666666667 and the EOS eos/gm1 = 0.66666666667, as the deck
+    does.  block = meshblock extent (default: the whole mesh, as convergence.py sets for
+    one rank); meta['exact'] holds the analytic state the errors are measured against."""
+    nx = (2 * res, res, res)
+    block = tuple(block) if block is not None else nx
+    ng = 3 if recon == "ppm" else 2
+    size = (3.0, 1.5, 1.5)
+    gm1 = gamma - 1.0
+    ang_3 = np.arctan(size[0] / size[1])
+    sin_a3, cos_a3 = np.sin(ang_3), np.cos(ang_3)
+    ang_2 = np.arctan(0.5 * (size[0] * cos_a3 + size[1] * sin_a3) / size[2])
+    sin_a2, cos_a2 = np.sin(ang_2), np.cos(ang_2)
+    x1 = size[0] * cos_a2 * cos_a3
+    x2 = size[1] * cos_a2 * sin_a3
+    x3 = size[2] * sin_a2
+    lam = x1
+    if ang_3 != 0.0:
+        lam = min(lam, x2)
+    if ang_2 != 0.0:
+        lam = min(lam, x3)
+    k_par = 2.0 * np.pi / lam
+    d0, u0 = 1.0, 0.0
+    p0 = 1.0 / gamma
+    h0 = ((p0 / gm1 + 0.5 * d0 * (u0 * u0)) + p0) / d0
+    a = np.sqrt(gm1 * max(h0 - 0.5 * u0 * u0, 1.0e-20))
+    ev = [u0 - a, u0, u0, u0, u0 + a]
+    rem = {0: [1.0, u0 - a, 0.0, 0.0, h0 - u0 * a], 4: [1.0, u0 + a, 0.0, 0.0, h0 + u0 * a]}
+    if wave_flag not in rem:
+        raise ValueError("sound waves only (wave_flag 0 or 4)")
+    r = rem[wave_flag]
+    nblk = tuple(n // b for n, b in zip(nx, block))
+    m = lattice_mesh(3, nblk, block, ng, (0.0, 0.0, 0.0), size, ["periodic"] * 6)
+    bs = BlockSet(3, [m])
+    cons = np.zeros(bs.field_size(bs.nvar))
+    exact = []
+    for b in range(bs.nblocks):
+        xc, yc, zc = bs.cell_centers(b)
+        x = cos_a2 * (xc * cos_a3 + yc * sin_a3) + zc * sin_a2
+        sn = np.sin(k_par * x)
+        d1 = d0 + amp * sn * r[0]
+        mx = d0 * u0 + amp * sn * r[1]
+        my = amp * sn * r[2]
+        mz = amp * sn * r[3]
+        m1 = mx * cos_a2 * cos_a3 - my * sin_a3 - mz * sin_a2 * cos_a3
+        m2 = mx * cos_a2 * sin_a3 + my * cos_a3 - mz * sin_a2 * sin_a3
+        m3 = mx * sin_a2 + mz * cos_a2
+        e0 = p0 / gm1 + 0.5 * d0 * u0 * u0 + amp * sn * r[4]
+        _set_interior(bs, cons, b, d1, m1, m2, m3, e0)
+        exact.append(np.stack([d1, m1, m2, m3, e0]))
+    tlim = lam / abs(ev[wave_flag]) * tlim_periods
+    return Problem("linear_wave3d", bs, cons, make_eos(eos_gm1), recon, riemann, cfl,
+                   tlim=tlim, integrator=integrator,
+                   meta={"exact": exact, "nx": nx, "domain_volume": size[0] * size[1] * size[2],
+                         "lambda": lam})

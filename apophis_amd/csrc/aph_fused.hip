@@ -66,8 +66,12 @@ struct AxisNbrs {
   int bclo, bchi;
 };
 
+// Issues the loads of cell c of the pencil (neighbour block, reflected or clamped
+// ghost) and returns whether its normal momentum must change sign (reflecting
+// boundary).  The sign change is left to the caller: applied here it would make the
+// loads complete before the function returns and defeat the march's prefetch.
 template <int NV>
-__device__ __forceinline__ void load_cell(const FusedArgs &a, const aph_block &bk,
+__device__ __forceinline__ bool load_cell(const FusedArgs &a, const aph_block &bk,
                                           const AxisNbrs &nb, int D, int n_d, int c,
                                           int64_t tbase, int64_t sd, int64_t nc, double *u) {
   int64_t base = nb.self;
@@ -97,7 +101,7 @@ __device__ __forceinline__ void load_cell(const FusedArgs &a, const aph_block &b
   const double *p = a.u_in + base + tbase + (int64_t)idx * sd;
 #pragma unroll
   for (int v = 0; v < NV; ++v) u[v] = p[v * nc];
-  if (flip) u[1 + D] = -1.0 * u[1 + D];
+  return flip;
 }
 
 template <int RECON, int RS, int NV, int NDIM, int D>
@@ -148,13 +152,15 @@ __device__ __forceinline__ void sweep(const FusedArgs &a, const aph_block &bk, i
     const int f_lo = t0[D] + seg * cfg.s[D];
     const int f_hi = f_lo + cfg.s[D];
     double fprev[NV];
+    bool flip = false;  // sign change of the fetched cell, applied by convert
     auto fetch = [&](int c, double *u) {
-      load_cell<NV>(a, bk, nb, D, n_d, c, tbase, st[D], nc, u);
+      flip = load_cell<NV>(a, bk, nb, D, n_d, c, tbase, st[D], nc, u);
     };
     auto convert = [&](const double *raw, double *dst) {
       double u[NV];
 #pragma unroll
       for (int v = 0; v < NV; ++v) u[v] = raw[v];
+      if (flip) u[1 + D] = -1.0 * u[1 + D];
       cons_to_prim_pt<NV>(u, a.eos, dst, dst + NV);
     };
     auto emit = [&](int f, const double *fo, const double *, const double *) {

@@ -6,18 +6,33 @@
 // L_out of U_out this computes
 //   L_in = (gam0 [+ gam1 in stage 1]) L_out + (stencil part) [+ L_acc],
 //   L_n  = gam1 L_out                                   (stage 2).
-// Structure (deterministic, no atomics):
-//   k_adj_axis<D>  one lane per (block, pencil, segment) per axis re-marches the forward
-//                  (ConsToPrim, reconstruction, Riemann) over its segment, back-propagates
-//                  the face cotangents mu_f = beta dt A/V (L_out(f) - L_out(f-1)) through
-//                  the Riemann solver and the reconstruction of both adjacent cells, and
-//                  writes the primitive-space cotangent of its own cells to Pbar (x writes,
-//                  y and z add) and the cotangent of the <= 3 cells it reads beyond its
-//                  segment on each side -- already mapped to conserved variables at that
-//                  cell's (possibly reflected) state -- to a 6-slot spill record;
-//   k_adj_gather   per interior cell: gam terms, gravity adjoint, J_prim^T (Pbar), plus the
-//                  spills of the neighbouring segments / blocks / boundary images that read
-//                  this cell, in a fixed order.
+// Structure (deterministic, no atomics), per stage:
+//   PPM, split form (the PPM+HLLC adjoint does not fit one kernel's registers at two waves
+//   per SIMD):
+//     k_adj_prim       P = ConsToPrim(U) of every interior cell, in the pack layout and in
+//                      the x-transposed layout, and L_out x-transposed;
+//     per axis D:
+//     k_adj_face<D>    one lane per (block, pencil, segment) re-marches the forward
+//                      reconstruction over its faces from P and back-propagates the face
+//                      cotangents mu_f = beta dt A/V (L_out(f) - L_out(f-1)) through the
+//                      Riemann solver: QL_bar / QR_bar, the cotangents of every cell's two
+//                      reconstructed face states;
+//     k_adj_recon<D>   one lane per (block, pencil, segment, field) applies the adjoint of
+//                      that field's reconstruction of each cell to them.
+//   PLM, one march per axis:
+//     k_adj_axis<D>    one lane per (block, pencil, segment) re-marches the forward
+//                      (ConsToPrim, reconstruction, Riemann) and back-propagates mu_f
+//                      through the Riemann solver and both adjacent reconstructions.
+//   Both forms write the primitive-space cotangent of a segment's own cells to Pbar (x
+//   writes, y and z add) and that of the <= 3 cells it reads beyond the segment on each
+//   side to a 6-slot spill record (primitive space; at a reflecting-boundary image the
+//   normal velocity's cotangent is flipped back to the interior cell's sign).
+//   k_adj_gather     per interior cell: Pbar plus the spills of the segments / blocks /
+//                    boundary images that read this cell (fixed order), the gravity
+//                    adjoint, J_prim^T, and the gam terms.
+// The x sweeps of the split form keep P, L_out and QL_bar / QR_bar in the x-transposed
+// layout (y fastest, then z, then x): the lanes of a wave take adjacent pencils, so every
+// load of the march reads adjacent addresses.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -54,7 +69,12 @@ struct AdjArgs {
   double gam0, gam1, beta;
   int stage;
   double *pbar;                // NF x ncells, pack layout
-  double *spill[3];            // per axis
+  double *prim, *prim_t;       // split form: NF x ncells, P of the interior cells in the
+                               // pack / x-transposed layout
+  double *lam_t;               // split form: NV x ncells, L_out x-transposed
+  double *qlb, *qrb;           // split form: NF x ncells, cotangents of each cell's ql / qr
+                               // output, in the layout of the axis being swept
+  double *spill[3];            // per axis, 6 slots x NF per (pencil, segment)
   const int64_t *spill_base;   // [3][nblocks] element offsets
   int segcap[3];               // adj_seg cap per axis
 };
@@ -100,6 +120,475 @@ __device__ __forceinline__ bool map_load(const AdjArgs &a, const AxisMap &m, int
   if (flip) u[1 + D] = -1.0 * u[1 + D];
   return flip;
 }
+
+
+// Split form ---------------------------------------------------------------------
+
+// Where pencil coordinate c of the current block lives along axis D: the block itself,
+// a neighbour, or a boundary image (reflect: mirrored, normal velocity negated; outflow:
+// the edge cell).  Returns that block's cell_base and sets the index along D.
+struct CellMap {
+  int64_t self, lo, hi;  // block / neighbour cell_base, -1 = none
+  int bclo, bchi;
+};
+
+__device__ __forceinline__ int64_t map_cell(const CellMap &m, int n_d, int c, int &idx,
+                                            bool &flip) {
+  int64_t base = m.self;
+  idx = c;
+  flip = false;
+  if (c < 0) {
+    if (m.lo >= 0) {
+      base = m.lo;
+      idx = c + n_d;
+    } else if (m.bclo == APH_BC_REFLECT) {
+      idx = -1 - c;
+      flip = true;
+    } else {
+      idx = 0;
+    }
+  } else if (c >= n_d) {
+    if (m.hi >= 0) {
+      base = m.hi;
+      idx = c - n_d;
+    } else if (m.bchi == APH_BC_REFLECT) {
+      idx = 2 * n_d - 1 - c;
+      flip = true;
+    } else {
+      idx = n_d - 1;
+    }
+  }
+  return base;
+}
+
+template <int D>
+__device__ __forceinline__ CellMap cell_map(const AdjArgs &a, const aph_block &bk) {
+  constexpr int sdn = (D == 0) ? 1 : (D == 1 ? 3 : 9);
+  CellMap m;
+  m.self = bk.cell_base;
+  const int nl = bk.nbr[13 - sdn], nh = bk.nbr[13 + sdn];
+  m.lo = nl >= 0 ? a.blocks[nl].cell_base : -1;
+  m.hi = nh >= 0 ? a.blocks[nh].cell_base : -1;
+  m.bclo = bk.bc[2 * D];
+  m.bchi = bk.bc[2 * D + 1];
+  return m;
+}
+
+// Unit u of axis D -> (segment, pencil coordinates pa, pb), pa fastest: adjacent lanes
+// take adjacent pencils.
+__device__ __forceinline__ void split_geom(int u, int nA, int nseg, int &seg, int &pa,
+                                           int &pb) {
+  pa = u % nA;
+  const int r = u / nA;
+  seg = r % nseg;
+  pb = r / nseg;
+}
+
+// Offset of pencil (pa, pb) at axis coordinate 0 and the stride along axis D, in the pack
+// layout (x fastest) or, XT, the x-transposed layout (y fastest, then z, then x) of the
+// same ghost-inclusive N0 x N1 x N2 box.
+struct Lay {
+  int64_t base, sd;
+};
+template <int D, bool XT>
+__device__ __forceinline__ Lay axis_lay(int pa, int pb, int N0, int N1, int N2,
+                                        const int *g) {
+  constexpr int A = (D == 0) ? 1 : 0, B = (D == 2) ? 1 : 2;
+  int64_t st[3];
+  if (XT) {
+    st[0] = (int64_t)N1 * N2;
+    st[1] = 1;
+    st[2] = N1;
+  } else {
+    st[0] = 1;
+    st[1] = N0;
+    st[2] = (int64_t)N0 * N1;
+  }
+  Lay l;
+  l.base = (int64_t)(pa + g[A]) * st[A] + (int64_t)(pb + g[B]) * st[B] + (int64_t)g[D] * st[D];
+  l.sd = st[D];
+  return l;
+}
+
+// P = (w[NV], gamma_c, gamma_e) = ConsToPrim(U) of every interior cell (with the floors, as
+// the forward applies them when it reads a cell), in the pack and x-transposed layouts,
+// and L_out in the x-transposed layout.  A workgroup takes 16 x 16 (x, y) tiles of one
+// z-plane: it reads U and L_out and writes P along x, then writes the transposed copies
+// along y through LDS, so every access runs over 16 consecutive doubles.
+template <int NV, int NDIM>
+__global__ void __launch_bounds__(256) k_adj_prim(AdjArgs a) {
+  constexpr int NF = NV + 2;
+  constexpr int T = 16;
+  __shared__ double tile[NF + NV][T][T + 1];
+  const int b = blockIdx.y;
+  const aph_block &bk = a.blocks[b];
+  const int N0 = ext(bk, 0, NDIM), N1 = ext(bk, 1, NDIM), N2 = ext(bk, 2, NDIM);
+  const int64_t nc = (int64_t)N0 * N1 * N2;
+  const int g[3] = {gst(bk, 0, NDIM), gst(bk, 1, NDIM), gst(bk, 2, NDIM)};
+  const int nx0 = bk.nx[0], nx1 = bk.nx[1], nx2 = bk.nx[2];
+  const int nti = (nx0 + T - 1) / T, ntj = (nx1 + T - 1) / T;
+  const int ntiles = nti * ntj * nx2;
+  const int64_t o = (int64_t)NV * bk.cell_base;
+  double *pp = a.prim + (int64_t)NF * bk.cell_base;
+  double *pt = a.prim_t + (int64_t)NF * bk.cell_base;
+  const int tx = threadIdx.x % T, ty = threadIdx.x / T;
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const int k = t / (nti * ntj), r = t % (nti * ntj);
+    const int i0 = (r % nti) * T, j0 = (r / nti) * T;
+    {  // along x: thread (tx, ty) = cell (i0 + tx, j0 + ty)
+      const int i = i0 + tx, j = j0 + ty;
+      if (i < nx0 && j < nx1) {
+        const int64_t cell = ((int64_t)(k + g[2]) * N1 + (j + g[1])) * N0 + (i + g[0]);
+        double u[NV], w[NV], gm[2];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          u[v] = a.u_in[o + v * nc + cell];
+          tile[NF + v][ty][tx] = a.lam_out[o + v * nc + cell];
+        }
+        cons_to_prim_pt<NV>(u, a.eos, w, gm);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+          pp[v * nc + cell] = w[v];
+          tile[v][ty][tx] = w[v];
+        }
+        pp[NV * nc + cell] = gm[0];
+        pp[(NV + 1) * nc + cell] = gm[1];
+        tile[NV][ty][tx] = gm[0];
+        tile[NV + 1][ty][tx] = gm[1];
+      }
+    }
+    __syncthreads();
+    {  // along y: thread (tx, ty) = cell (i0 + ty, j0 + tx)
+      const int i = i0 + ty, j = j0 + tx;
+      if (i < nx0 && j < nx1) {
+        const int64_t cellt = ((int64_t)(i + g[0]) * N2 + (k + g[2])) * N1 + (j + g[1]);
+#pragma unroll
+        for (int f = 0; f < NF; ++f) pt[f * nc + cellt] = tile[f][tx][ty];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) a.lam_t[o + v * nc + cellt] = tile[NF + v][tx][ty];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Face adjoint of axis D: one lane per (block, pencil, segment) re-marches the forward
+// reconstruction over its faces [a0, fend) from P and back-propagates the face cotangent
+// mu_f = beta dt A/V (L_out(f) - L_out(f-1)) (only this block's cells use this block's
+// copy of a face) through the Riemann solver and the passive-scalar upwind rule.  It
+// writes QL_bar(f-1) (cotangent of the ql output of cell f-1) and QR_bar(f) (of the qr
+// output of cell f), NF fields each; the block-local copies for the cells -1 and n go to
+// the ghost slots.
+template <int RECON, int RS, int NV, int NDIM, int D>
+__global__ void __launch_bounds__(64, 2) k_adj_face(AdjArgs a) {
+  constexpr int GR = ReconTraits<RECON>::GR;
+  constexpr int W = ReconTraits<RECON>::W;
+  constexpr int NF = NV + 2;
+  constexpr int IVX = 1 + D, IVY = 1 + IVX % 3, IVZ = 1 + (IVX + 1) % 3;
+  constexpr int A = (D == 0) ? 1 : 0, B = (D == 2) ? 1 : 2;
+  constexpr bool XT = (D == 0);
+  const int b = blockIdx.y;
+  const aph_block &bk = a.blocks[b];
+  const int n_d = bk.nx[D];
+  const int S = adj_seg(n_d, a.segcap[D]);
+  const int nseg = n_d / S;
+  const int nA = bk.nx[A], nB = bk.nx[B];
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= nA * nB * nseg) return;
+  int seg, pa, pb;
+  split_geom(u, nA, nseg, seg, pa, pb);
+  const int N0 = ext(bk, 0, NDIM), N1 = ext(bk, 1, NDIM), N2 = ext(bk, 2, NDIM);
+  const int64_t nc = (int64_t)N0 * N1 * N2;
+  const int g[3] = {gst(bk, 0, NDIM), gst(bk, 1, NDIM), gst(bk, 2, NDIM)};
+  const CellMap m = cell_map<D>(a, bk);
+  const Lay L = axis_lay<D, XT>(pa, pb, N0, N1, N2, g);
+  const int a0 = seg * S;
+  const int fend = (seg == nseg - 1) ? n_d + 1 : a0 + S;  // faces [a0, fend)
+  const double coef = a.beta * a.dt[0] * bk.area[D] / bk.vol;
+  const double *prim = XT ? a.prim_t : a.prim;
+  const double *lo_ptr = (XT ? a.lam_t : a.lam_out) + (int64_t)NV * bk.cell_base + L.base;
+  double *qlb_ptr = a.qlb + (int64_t)NF * bk.cell_base + L.base;
+  double *qrb_ptr = a.qrb + (int64_t)NF * bk.cell_base + L.base;
+  auto load = [&](int c, double *dst) {
+    int idx;
+    bool flip;
+    const int64_t base = map_cell(m, n_d, c, idx, flip);
+    const double *p = prim + (int64_t)NF * base + L.base + (int64_t)idx * L.sd;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) dst[f] = p[f * nc];
+    if (flip) dst[IVX] = -1.0 * dst[IVX];
+  };
+  double win[W][NF];  // win[k] = cell mm-GR+k
+  // PPM as in the forward march: bit k of `plain` says every field of win[k] is +-0 or
+  // 2^-900 <= |.| < 2^1000; faces over four such cells divide by 6 with div6_fast, and
+  // each limited face value is formed once (fv: face mm-1/2)
+  auto plain_cell = [](const double *c) -> unsigned {
+    int lo = 0, hi = 0;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      const int k = __builtin_amdgcn_frexp_exp(c[f]);
+      lo = min(lo, k);
+      hi = max(hi, k);
+    }
+    return (lo >= -899 && hi <= 1000) ? 1u : 0u;
+  };
+  auto faces = [&](unsigned pl, double *out) {  // face between win[2] and win[3]
+    if (((pl >> 1) & 0xFu) == 0xFu) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f)
+        out[f] = ppm_face<true>(win[1][f], win[2][f], win[3][f], win[4][f]);
+    } else {
+#pragma unroll
+      for (int f = 0; f < NF; ++f)
+        out[f] = ppm_face<false>(win[1][f], win[2][f], win[3][f], win[4][f]);
+    }
+  };
+  unsigned plain = 0;
+  double fv[NF];
+#pragma unroll
+  for (int s = 0; s < W - 1; ++s) {
+    load(a0 - 1 - GR + s, win[s + 1]);
+    if constexpr (RECON == RECON_PPM) plain |= plain_cell(win[s + 1]) << (s + 1);
+  }
+  if constexpr (RECON == RECON_PPM) faces(plain, fv);  // face a0-3/2
+  double qlp[NF], lprev[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v)
+    lprev[v] = (a0 >= 1) ? lo_ptr[v * nc + (int64_t)(a0 - 1) * L.sd] : 0.0;
+  for (int mm = a0 - 1; mm <= fend - 1; ++mm) {
+#pragma unroll
+    for (int s = 0; s < W - 1; ++s)
+#pragma unroll
+      for (int f = 0; f < NF; ++f) win[s][f] = win[s + 1][f];
+    load(mm + GR, win[W - 1]);
+    double ql[NF], qr[NF];
+    if constexpr (RECON == RECON_PPM) {
+      plain = (plain >> 1) | (plain_cell(win[W - 1]) << (W - 1));
+      double up[NF];
+      faces(plain, up);  // face mm+1/2
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        ppm_cell(win[0][f], win[1][f], win[2][f], win[3][f], win[4][f], fv[f], up[f], ql[f],
+                 qr[f]);
+        fv[f] = up[f];
+      }
+    } else {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        double col[W];
+#pragma unroll
+        for (int s = 0; s < W; ++s) col[s] = win[s][f];
+        recon1<RECON>(col, ql[f], qr[f]);
+      }
+    }
+    if (mm >= a0) {  // face mm between cells mm-1 (ql = qlp) and mm (qr)
+      const int f = mm;
+      double lcur[NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v)
+        lcur[v] = (f < n_d) ? lo_ptr[v * nc + (int64_t)f * L.sd] : 0.0;
+      double mu[NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) mu[v] = coef * (lcur[v] - lprev[v]);
+#pragma unroll
+      for (int v = 0; v < NV; ++v) lprev[v] = lcur[v];
+      const double wl[5] = {qlp[0], qlp[IVX], qlp[IVY], qlp[IVZ], qlp[4]};
+      const double wr[5] = {qr[0], qr[IVX], qr[IVY], qr[IVZ], qr[4]};
+      double flx[5];
+      riemann<RS>(wl, wr, &qlp[NV], &qr[NV], flx);  // forward flux for the scalar upwind
+      double fb[5] = {mu[0], mu[IVX], mu[IVY], mu[IVZ], mu[4]};
+      double qlb[NF], qrb[NF];
+#pragma unroll
+      for (int k = 0; k < NF; ++k) {
+        qlb[k] = 0.0;
+        qrb[k] = 0.0;
+      }
+#pragma unroll
+      for (int n = 5; n < NV; ++n) {  // F_s = F_0 * (F_0 >= 0 ? ql_s : qr_s)
+        if (flx[0] >= 0.0) {
+          fb[0] += mu[n] * qlp[n];
+          qlb[n] += mu[n] * flx[0];
+        } else {
+          fb[0] += mu[n] * qr[n];
+          qrb[n] += mu[n] * flx[0];
+        }
+      }
+      double wlb[5] = {0, 0, 0, 0, 0}, wrb[5] = {0, 0, 0, 0, 0};
+      double iflb[2] = {0, 0}, ifrb[2] = {0, 0};
+      riemann_adj<RS>(wl, wr, &qlp[NV], &qr[NV], fb, wlb, wrb, iflb, ifrb);
+      qlb[0] += wlb[0];
+      qlb[IVX] += wlb[1];
+      qlb[IVY] += wlb[2];
+      qlb[IVZ] += wlb[3];
+      qlb[4] += wlb[4];
+      qlb[NV] += iflb[0];
+      qlb[NV + 1] += iflb[1];
+      qrb[0] += wrb[0];
+      qrb[IVX] += wrb[1];
+      qrb[IVY] += wrb[2];
+      qrb[IVZ] += wrb[3];
+      qrb[4] += wrb[4];
+      qrb[NV] += ifrb[0];
+      qrb[NV + 1] += ifrb[1];
+      double *pl = qlb_ptr + (int64_t)(f - 1) * L.sd;
+      double *pr = qrb_ptr + (int64_t)f * L.sd;
+#pragma unroll
+      for (int k = 0; k < NF; ++k) {
+        pl[k * nc] = qlb[k];
+        pr[k * nc] = qrb[k];
+      }
+    }
+#pragma unroll
+    for (int f = 0; f < NF; ++f) qlp[f] = ql[f];
+  }
+}
+
+// Reconstruction (PPM) adjoint of axis D, one field per lane: lane (block, pencil,
+// segment, field) takes the cells m of its segment (segment 0 also m = -1, the last also
+// m = n) and applies the adjoint of field `fld`'s reconstruction of cell m to
+// (QL_bar(m), QR_bar(m)), recomputing the forward from P.  Faces are shared as in the
+// forward march: each step forms the limited value of face m+1/2 once (with its branch
+// tape), runs the cell part's adjoint, and then propagates face m-1/2's cotangent --
+// cell m-1's share plus cell m's -- once.  A face between two segments gets each
+// segment's share from that segment's lane (the face adjoint is linear in its cotangent).
+// The contributions to cells m-2 .. m+2 are summed in a window; the segment's own cells
+// go to Pbar (axis x writes, y and z add), the <= 3 cells on either side to the segment's
+// 6-slot spill record.
+template <int RECON, int NV, int NDIM, int D>
+__global__ void __launch_bounds__(64) k_adj_recon(AdjArgs a) {
+  static_assert(RECON == RECON_PPM, "the split form is the PPM adjoint");
+  constexpr int GR = ReconTraits<RECON>::GR;
+  constexpr int W = ReconTraits<RECON>::W;
+  constexpr int NF = NV + 2;
+  constexpr int IVX = 1 + D;
+  constexpr int A = (D == 0) ? 1 : 0, B = (D == 2) ? 1 : 2;
+  constexpr bool XT = (D == 0);
+  const int b = blockIdx.y;
+  const aph_block &bk = a.blocks[b];
+  const int n_d = bk.nx[D];
+  const int S = adj_seg(n_d, a.segcap[D]);
+  const int nseg = n_d / S;
+  const int nA = bk.nx[A], nB = bk.nx[B];
+  const int per_field = nA * nB * nseg;
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= per_field * NF) return;
+  const int fld = u / per_field;
+  int seg, pa, pb;
+  split_geom(u - fld * per_field, nA, nseg, seg, pa, pb);
+  const int pencil = pa + pb * nA;
+  const int N0 = ext(bk, 0, NDIM), N1 = ext(bk, 1, NDIM), N2 = ext(bk, 2, NDIM);
+  const int64_t nc = (int64_t)N0 * N1 * N2;
+  const int g[3] = {gst(bk, 0, NDIM), gst(bk, 1, NDIM), gst(bk, 2, NDIM)};
+  const CellMap m = cell_map<D>(a, bk);
+  const Lay L = axis_lay<D, XT>(pa, pb, N0, N1, N2, g);      // P, QL_bar, QR_bar
+  const Lay Lp = axis_lay<D, false>(pa, pb, N0, N1, N2, g);  // Pbar
+  const int a0 = seg * S;
+  const int m_lo = (seg == 0) ? -1 : a0;
+  const int m_hi = (seg == nseg - 1) ? n_d : a0 + S - 1;
+  const double *prim = (XT ? a.prim_t : a.prim) + fld * nc + L.base;
+  const double *qlb_ptr = a.qlb + (int64_t)NF * bk.cell_base + fld * nc + L.base;
+  const double *qrb_ptr = a.qrb + (int64_t)NF * bk.cell_base + fld * nc + L.base;
+  double *pb_ptr = a.pbar + (int64_t)NF * bk.cell_base + fld * nc + Lp.base;
+  double *sp = a.spill[D] + a.spill_base[D * a.nblocks + b] +
+               ((int64_t)pencil * nseg + seg) * 6 * NF + fld;
+  // every halo cell is emitted exactly once: clear the record, then write each slot once
+#pragma unroll
+  for (int s = 0; s < 6; ++s) sp[s * NF] = 0.0;
+  const bool vel = (fld == IVX);
+  auto load = [&](int c) -> double {
+    int idx;
+    bool flip;
+    const int64_t base = map_cell(m, n_d, c, idx, flip);
+    const double v = prim[(int64_t)NF * base + (int64_t)idx * L.sd];
+    return (flip && vel) ? -1.0 * v : v;
+  };
+  // the next step's operands are loaded one step ahead; the sign change of a reflected
+  // value is applied when it is used, so the loads stay in flight across a step
+  auto fetch = [&](int c, double &v, bool &ng) {
+    int idx;
+    bool flip;
+    const int64_t base = map_cell(m, n_d, c, idx, flip);
+    v = prim[(int64_t)NF * base + (int64_t)idx * L.sd];
+    ng = flip && vel;
+  };
+  auto emit = [&](int c, double v) {
+    if (c >= a0 && c < a0 + S) {
+      double *q = pb_ptr + (int64_t)c * Lp.sd;
+      if (D == 0)
+        *q = v;
+      else
+        *q += v;
+    } else {
+      const bool below = c < a0;
+      const int slot = below ? c - (a0 - 3) : 3 + (c - (a0 + S));
+      const bool image = below ? (c < 0 && m.lo < 0 && m.bclo == APH_BC_REFLECT)
+                               : (c >= n_d && m.hi < 0 && m.bchi == APH_BC_REFLECT);
+      sp[slot * NF] = (image && vel) ? -1.0 * v : v;
+    }
+  };
+  // bit k of `plain`: col[k] is +-0 or 2^-900 <= |.| < 2^1000, so a face over four such
+  // cells may divide by 6 with div6_fast (exact there, as in the forward march)
+  auto plain1 = [](double x) -> unsigned {
+    const int k = __builtin_amdgcn_frexp_exp(x);
+    return (k >= -899 && k <= 1000) ? 1u : 0u;
+  };
+  auto face = [](const double *c, unsigned pl, PpmFaceTape &t) -> double {
+    return ((pl >> 1) & 0xFu) == 0xFu ? ppm_face_tape<true>(c[1], c[2], c[3], c[4], t)
+                                       : ppm_face_tape<false>(c[1], c[2], c[3], c[4], t);
+  };
+  double col[W];  // col[k] = field value of cell mm-GR+k
+  double cw[W];   // cw[k] = cotangent of cell mm-GR+k
+  unsigned plain = 0;
+#pragma unroll
+  for (int s = 0; s < W - 1; ++s) {
+    col[s + 1] = load(m_lo - GR + s);
+    plain |= plain1(col[s + 1]) << (s + 1);
+  }
+#pragma unroll
+  for (int s = 0; s < W; ++s) cw[s] = 0.0;
+  PpmFaceTape tlo;                         // face mm-1/2 (cells mm-2 .. mm+1)
+  double flo = face(col, plain, tlo);
+  double fb_lo = 0.0;                      // its cotangent from cell mm-1 (this lane's)
+  double nv, nql, nqr;
+  bool nng;
+  fetch(m_lo + GR, nv, nng);
+  nql = (m_lo <= n_d - 1) ? qlb_ptr[(int64_t)m_lo * L.sd] : 0.0;
+  nqr = (m_lo >= 0) ? qrb_ptr[(int64_t)m_lo * L.sd] : 0.0;
+  for (int mm = m_lo; mm <= m_hi; ++mm) {
+#pragma unroll
+    for (int s = 0; s < W - 1; ++s) col[s] = col[s + 1];
+    col[W - 1] = nng ? -1.0 * nv : nv;
+    const double ql_b = nql, qr_b = nqr;
+    if (mm < m_hi) {
+      fetch(mm + 1 + GR, nv, nng);
+      nql = (mm + 1 <= n_d - 1) ? qlb_ptr[(int64_t)(mm + 1) * L.sd] : 0.0;
+      nqr = qrb_ptr[(int64_t)(mm + 1) * L.sd];  // mm + 1 >= 0
+    }
+    plain = (plain >> 1) | (plain1(col[W - 1]) << (W - 1));
+    PpmFaceTape thi;                       // face mm+1/2 (cells mm-1 .. mm+2)
+    const double fhi = face(col, plain, thi);
+    double colb[W];
+#pragma unroll
+    for (int s = 0; s < W; ++s) colb[s] = 0.0;
+    double flob, fhib;
+    ppm_cell_adj(col, flo, fhi, ql_b, qr_b, colb, flob, fhib);
+    ppm_face_adj(tlo, fb_lo + flob, colb);  // face mm-1/2 -> cells mm-2 .. mm+1
+    fb_lo = fhib;
+    flo = fhi;
+    tlo = thi;
+#pragma unroll
+    for (int s = 0; s < W; ++s) cw[s] += colb[s];
+    emit(mm - GR, cw[0]);  // no later cell or face reaches mm-GR
+#pragma unroll
+    for (int s = 0; s < W - 1; ++s) cw[s] = cw[s + 1];
+    cw[W - 1] = 0.0;
+  }
+  ppm_face_adj(tlo, fb_lo, cw);  // face m_hi+1/2 -> cells m_hi-1 .. m_hi+2 = cw[0..3]
+#pragma unroll
+  for (int s = 0; s < W - 1; ++s) emit(m_hi - GR + 1 + s, cw[s]);
+}
+
+// One-march form (PLM) -------------------------------------------------------------
 
 template <int RECON, int RS, int NV, int NDIM, int D>
 __global__ void __launch_bounds__(64) k_adj_axis(AdjArgs a) {
@@ -149,9 +638,9 @@ __global__ void __launch_bounds__(64) k_adj_axis(AdjArgs a) {
   const double *lo_ptr = a.lam_out + m.self + tbase;
   double *pb_ptr = a.pbar + (int64_t)NF * bk.cell_base + tbase;
   double *sp = a.spill[D] + a.spill_base[D * a.nblocks + b] +
-               ((int64_t)pencil * nseg + seg) * 6 * NV;
+               ((int64_t)pencil * nseg + seg) * 6 * NF;
   // every halo cell is emitted exactly once: clear the record, then write each slot once
-  for (int s = 0; s < 6 * NV; ++s) sp[s] = 0.0;
+  for (int s = 0; s < 6 * NF; ++s) sp[s] = 0.0;
 
   auto load = [&](int c, double *dst) {
     double uu[NV];
@@ -170,16 +659,13 @@ __global__ void __launch_bounds__(64) k_adj_axis(AdjArgs a) {
           q[f * nc] += pbv[f];
       }
     } else {
-      const int slot = (c < a0) ? c - (a0 - 3) : 3 + (c - (a0 + S));
-      double uu[NV];
-      const bool flip = map_load<NV>(a, m, D, n_d, c, tbase, st[D], nc, uu);
-      double ub[NV];
+      const bool below = c < a0;
+      const int slot = below ? c - (a0 - 3) : 3 + (c - (a0 + S));
+      const bool image = below ? (c < 0 && m.lo < 0 && m.bclo == APH_BC_REFLECT)
+                               : (c >= n_d && m.hi < 0 && m.bchi == APH_BC_REFLECT);
 #pragma unroll
-      for (int v = 0; v < NV; ++v) ub[v] = 0.0;
-      cons_to_prim_adj<NV>(uu, a.eos, pbv, pbv + NV, ub);
-      if (flip) ub[1 + D] = -1.0 * ub[1 + D];
-#pragma unroll
-      for (int v = 0; v < NV; ++v) sp[slot * NV + v] = ub[v];
+      for (int f = 0; f < NF; ++f)
+        sp[slot * NF + f] = (image && f == IVX) ? -1.0 * pbv[f] : pbv[f];
     }
   };
 
@@ -322,13 +808,13 @@ __global__ void __launch_bounds__(64) k_adj_axis(AdjArgs a) {
   }
 }
 
-template <int NV>
+template <int NF>
 __device__ __forceinline__ void add_slot(const AdjArgs &a, int d, int blk, int pencil,
                                          int seg, int nseg, int slot, double *acc) {
   const double *p = a.spill[d] + a.spill_base[d * a.nblocks + blk] +
-                    (((int64_t)pencil * nseg + seg) * 6 + slot) * NV;
+                    (((int64_t)pencil * nseg + seg) * 6 + slot) * NF;
 #pragma unroll
-  for (int v = 0; v < NV; ++v) acc[v] += p[v];
+  for (int v = 0; v < NF; ++v) acc[v] += p[v];
 }
 
 template <int NV, int NDIM>
@@ -361,6 +847,45 @@ __global__ void __launch_bounds__(256) k_adj_gather(AdjArgs a) {
     const double *pp = a.pbar + (int64_t)NF * bk.cell_base + cell;
 #pragma unroll
     for (int f = 0; f < NF; ++f) pbv[f] = pp[f * nc];
+    // spills (primitive space) of the segments / blocks / boundary images that read
+    // this cell
+    for (int d = 0; d < NDIM; ++d) {
+      const int Ad = (d == 0) ? 1 : 0, Bd = (d == 2) ? 1 : 2;
+      const int n_d = bk.nx[d];
+      const int S = adj_seg(n_d, a.segcap[d]), nseg = n_d / S;
+      const int pencil = cc[Ad] + cc[Bd] * bk.nx[Ad];
+      const int s = cc[d] / S, off = cc[d] % S;
+      const int sdn = (d == 0) ? 1 : (d == 1 ? 3 : 9);
+      if (off < 3) {  // from below
+        if (s > 0) {
+          add_slot<NF>(a, d, b, pencil, s - 1, nseg, 3 + off, pbv);
+        } else {
+          const int nb = bk.nbr[13 - sdn];
+          if (nb >= 0) {
+            add_slot<NF>(a, d, nb, pencil, nseg - 1, nseg, 3 + off, pbv);
+          } else if (bk.bc[2 * d] == APH_BC_REFLECT) {
+            add_slot<NF>(a, d, b, pencil, 0, nseg, 2 - off, pbv);
+          } else if (cc[d] == 0) {
+            for (int sl = 0; sl < 3; ++sl) add_slot<NF>(a, d, b, pencil, 0, nseg, sl, pbv);
+          }
+        }
+      }
+      if (off >= S - 3) {  // from above
+        if (s < nseg - 1) {
+          add_slot<NF>(a, d, b, pencil, s + 1, nseg, off - (S - 3), pbv);
+        } else {
+          const int nb = bk.nbr[13 + sdn];
+          if (nb >= 0) {
+            add_slot<NF>(a, d, nb, pencil, 0, nseg, off - (S - 3), pbv);
+          } else if (bk.bc[2 * d + 1] == APH_BC_REFLECT) {
+            add_slot<NF>(a, d, b, pencil, nseg - 1, nseg, 3 + (n_d - 1 - cc[d]), pbv);
+          } else if (cc[d] == n_d - 1) {
+            for (int sl = 3; sl < 6; ++sl)
+              add_slot<NF>(a, d, b, pencil, nseg - 1, nseg, sl, pbv);
+          }
+        }
+      }
+    }
     if (a.grav) {  // out[1+d] += g_d w0 bdt ; out[4] += (sum g_d w_{1+d}) w0 bdt
       double uu[NV], w[NV], gm[2];
 #pragma unroll
@@ -375,44 +900,6 @@ __global__ void __launch_bounds__(256) k_adj_gather(AdjArgs a) {
       pbv[0] += de * beta_dt * lo[4];
     }
     cons_to_prim_adj<NV>(u, a.eos, pbv, pbv + NV, acc);
-    // spills of the segments / blocks / boundary images that read this cell
-    for (int d = 0; d < NDIM; ++d) {
-      const int Ad = (d == 0) ? 1 : 0, Bd = (d == 2) ? 1 : 2;
-      const int n_d = bk.nx[d];
-      const int S = adj_seg(n_d, a.segcap[d]), nseg = n_d / S;
-      const int pencil = cc[Ad] + cc[Bd] * bk.nx[Ad];
-      const int s = cc[d] / S, off = cc[d] % S;
-      const int sdn = (d == 0) ? 1 : (d == 1 ? 3 : 9);
-      if (off < 3) {  // from below
-        if (s > 0) {
-          add_slot<NV>(a, d, b, pencil, s - 1, nseg, 3 + off, acc);
-        } else {
-          const int nb = bk.nbr[13 - sdn];
-          if (nb >= 0) {
-            add_slot<NV>(a, d, nb, pencil, nseg - 1, nseg, 3 + off, acc);
-          } else if (bk.bc[2 * d] == APH_BC_REFLECT) {
-            add_slot<NV>(a, d, b, pencil, 0, nseg, 2 - off, acc);
-          } else if (cc[d] == 0) {
-            for (int sl = 0; sl < 3; ++sl) add_slot<NV>(a, d, b, pencil, 0, nseg, sl, acc);
-          }
-        }
-      }
-      if (off >= S - 3) {  // from above
-        if (s < nseg - 1) {
-          add_slot<NV>(a, d, b, pencil, s + 1, nseg, off - (S - 3), acc);
-        } else {
-          const int nb = bk.nbr[13 + sdn];
-          if (nb >= 0) {
-            add_slot<NV>(a, d, nb, pencil, 0, nseg, off - (S - 3), acc);
-          } else if (bk.bc[2 * d + 1] == APH_BC_REFLECT) {
-            add_slot<NV>(a, d, b, pencil, nseg - 1, nseg, 3 + (n_d - 1 - cc[d]), acc);
-          } else if (cc[d] == n_d - 1) {
-            for (int sl = 3; sl < 6; ++sl)
-              add_slot<NV>(a, d, b, pencil, nseg - 1, nseg, sl, acc);
-          }
-        }
-      }
-    }
 #pragma unroll
     for (int v = 0; v < NV; ++v) a.lam_in[o + v * nc] = acc[v];
     if (a.lam_n && a.stage == 2) {
@@ -426,54 +913,86 @@ __global__ void __launch_bounds__(256) k_adj_gather(AdjArgs a) {
 
 // host side ------------------------------------------------------------------------
 
-using AxisFn = void (*)(AdjArgs);
-using GatherFn = void (*)(AdjArgs);
+using KFn = void (*)(AdjArgs);
+
+// Kernels of one axis: PPM takes the split form (face, recon), PLM the one-march form.
+struct AxisKernels {
+  KFn face = nullptr, recon = nullptr, axis = nullptr;
+};
+
+template <int RECON, int RS, int NV, int ND, int D>
+void set_axis(AxisKernels &k) {
+  if constexpr (RECON == RECON_PPM) {
+    k.face = k_adj_face<RECON, RS, NV, ND, D>;
+    k.recon = k_adj_recon<RECON, NV, ND, D>;
+  } else {
+    k.axis = k_adj_axis<RECON, RS, NV, ND, D>;
+  }
+}
 
 template <int RECON, int RS, int NV, int ND>
-void pick_axis(int d, AxisFn &f) {
-  if (d == 0) f = k_adj_axis<RECON, RS, NV, ND, 0>;
+void pick_axis(int d, AxisKernels &k) {
+  if (d == 0) set_axis<RECON, RS, NV, ND, 0>(k);
   if constexpr (ND >= 2)
-    if (d == 1) f = k_adj_axis<RECON, RS, NV, ND, 1>;
+    if (d == 1) set_axis<RECON, RS, NV, ND, 1>(k);
   if constexpr (ND >= 3)
-    if (d == 2) f = k_adj_axis<RECON, RS, NV, ND, 2>;
+    if (d == 2) set_axis<RECON, RS, NV, ND, 2>(k);
 }
 
 template <int RECON, int RS, int NV>
-AxisFn axis_nd(int nd, int d) {
-  AxisFn f = nullptr;
-  if (nd == 1) pick_axis<RECON, RS, NV, 1>(d, f);
-  if (nd == 2) pick_axis<RECON, RS, NV, 2>(d, f);
-  if (nd == 3) pick_axis<RECON, RS, NV, 3>(d, f);
-  return f;
+void axis_nd(int nd, int d, AxisKernels &k) {
+  if (nd == 1) pick_axis<RECON, RS, NV, 1>(d, k);
+  if (nd == 2) pick_axis<RECON, RS, NV, 2>(d, k);
+  if (nd == 3) pick_axis<RECON, RS, NV, 3>(d, k);
 }
 
-AxisFn pick_adj_axis(int recon, int rs, int nv, int nd, int d) {
+AxisKernels pick_adj_axis(int recon, int rs, int nv, int nd, int d) {
+  AxisKernels k;
 #define APH_AX(R, S)                                                             \
   if (recon == R && rs == S) {                                                    \
-    if (nv == 5) return axis_nd<R, S, 5>(nd, d);                                  \
-    if (nv == 6) return axis_nd<R, S, 6>(nd, d);                                  \
+    if (nv == 5) axis_nd<R, S, 5>(nd, d, k);                                      \
+    if (nv == 6) axis_nd<R, S, 6>(nd, d, k);                                      \
   }
   APH_AX(RECON_PPM, RS_HLLC)
   APH_AX(RECON_PPM, RS_HLLE)
   APH_AX(RECON_PLM, RS_HLLC)
   APH_AX(RECON_PLM, RS_HLLE)
 #undef APH_AX
-  return nullptr;
+  return k;
 }
 
 bool adj_supported(int recon, int rs, int nv, int nd) {
-  return pick_adj_axis(recon, rs, nv, nd, 0) != nullptr;
+  const AxisKernels k = pick_adj_axis(recon, rs, nv, nd, 0);
+  return k.face != nullptr || k.axis != nullptr;
 }
 
-GatherFn pick_gather(int nv, int nd) {
-  if (nv == 5) return nd == 1 ? k_adj_gather<5, 1> : (nd == 2 ? k_adj_gather<5, 2> : k_adj_gather<5, 3>);
-  if (nv == 6) return nd == 1 ? k_adj_gather<6, 1> : (nd == 2 ? k_adj_gather<6, 2> : k_adj_gather<6, 3>);
+KFn pick_nd(int nd, KFn k1, KFn k2, KFn k3) {
+  return nd == 1 ? k1 : (nd == 2 ? k2 : k3);
+}
+
+KFn pick_gather(int nv, int nd) {
+  if (nv == 5) return pick_nd(nd, k_adj_gather<5, 1>, k_adj_gather<5, 2>, k_adj_gather<5, 3>);
+  if (nv == 6) return pick_nd(nd, k_adj_gather<6, 1>, k_adj_gather<6, 2>, k_adj_gather<6, 3>);
   return nullptr;
 }
 
-int adj_prepare(aph_context *ctx, const aph_blockset *bs, AdjScratch &s) {
-  if (s.pbar) return APH_OK;
+KFn pick_prim(int nv, int nd) {
+  if (nv == 5) return pick_nd(nd, k_adj_prim<5, 1>, k_adj_prim<5, 2>, k_adj_prim<5, 3>);
+  if (nv == 6) return pick_nd(nd, k_adj_prim<6, 1>, k_adj_prim<6, 2>, k_adj_prim<6, 3>);
+  return nullptr;
+}
+
+int adj_prepare(aph_context *ctx, const aph_blockset *bs, int recon, AdjScratch &s) {
   const int nv = bs->nvar, nf = nv + 2, nd = bs->ndim, nb = bs->nblocks;
+  if (recon == RECON_PPM && !s.prim) {  // buffers of the split form
+    if (hipMalloc(&s.prim, sizeof(double) * nf * bs->ncells) != hipSuccess ||
+        hipMalloc(&s.prim_t, sizeof(double) * nf * bs->ncells) != hipSuccess ||
+        hipMalloc(&s.lam_t, sizeof(double) * nv * bs->ncells) != hipSuccess ||
+        hipMalloc(&s.qlb, sizeof(double) * nf * bs->ncells) != hipSuccess ||
+        hipMalloc(&s.qrb, sizeof(double) * nf * bs->ncells) != hipSuccess)
+      return set_error(ctx, APH_E_NOMEM, "adjoint scratch allocation failed");
+  }
+  if (s.pbar) return APH_OK;
   // segment cap per axis: the longest segments (fewest recomputed halo cells) that still
   // give >= 2 waves of 64 lanes per SIMD (MI355X: 256 CUs x 4 SIMDs)
   constexpr int64_t kTargetLanes = 2LL * 64 * 1024;
@@ -506,7 +1025,7 @@ int adj_prepare(aph_context *ctx, const aph_blockset *bs, AdjScratch &s) {
       const int A = (d == 0) ? 1 : 0, B = (d == 2) ? 1 : 2;
       const int nseg = k.nx[d] / adj_seg(k.nx[d], s.segcap[d]);
       base[(size_t)d * nb + b] = tot[d];
-      tot[d] += (int64_t)k.nx[A] * k.nx[B] * nseg * 6 * nv;
+      tot[d] += (int64_t)k.nx[A] * k.nx[B] * nseg * 6 * nf;
     }
   if (hipMalloc(&s.pbar, sizeof(double) * nf * bs->ncells) != hipSuccess ||
       hipMalloc(&s.lam1, sizeof(double) * nv * bs->ncells) != hipSuccess ||
@@ -526,12 +1045,11 @@ int adj_prepare(aph_context *ctx, const aph_blockset *bs, AdjScratch &s) {
 }
 
 void adj_release(AdjScratch &s) {
-  if (s.pbar) (void)hipFree(s.pbar);
+  for (double *p : {s.pbar, s.lam1, s.lam0p, s.prim, s.prim_t, s.lam_t, s.qlb, s.qrb})
+    if (p) (void)hipFree(p);
   for (auto &p : s.spill)
     if (p) (void)hipFree(p);
   if (s.spill_base) (void)hipFree(s.spill_base);
-  if (s.lam1) (void)hipFree(s.lam1);
-  if (s.lam0p) (void)hipFree(s.lam0p);
   s = AdjScratch{};
 }
 
@@ -539,8 +1057,15 @@ int adj_stage(aph_context *ctx, const aph_blockset *bs, int recon, int rs, const
               const double *g, int grav, AdjScratch &s, int stage, const double *u_in,
               const double *dt_dev, const double *lam_out, const double *lam_acc,
               double *lam_in, double *lam_n) {
-  int rc = adj_prepare(ctx, bs, s);
+  int rc = adj_prepare(ctx, bs, recon, s);
   if (rc) return rc;
+  const int nd = bs->ndim;
+  AxisKernels ks[3];
+  for (int d = 0; d < nd; ++d) {
+    ks[d] = pick_adj_axis(recon, rs, bs->nvar, nd, d);
+    if (!ks[d].face && !ks[d].axis)
+      return set_error(ctx, APH_E_INVALID, "adjoint: unsupported combination");
+  }
   AdjArgs a{};
   a.blocks = bs->d;
   a.nblocks = bs->nblocks;
@@ -560,26 +1085,37 @@ int adj_stage(aph_context *ctx, const aph_blockset *bs, int recon, int rs, const
   a.beta = stage == 1 ? 1.0 : 0.5;
   a.stage = stage;
   a.pbar = s.pbar;
+  a.prim = s.prim;
+  a.prim_t = s.prim_t;
+  a.lam_t = s.lam_t;
+  a.qlb = s.qlb;
+  a.qrb = s.qrb;
   for (int d = 0; d < 3; ++d) a.spill[d] = s.spill[d];
   a.spill_base = s.spill_base;
   for (int d = 0; d < 3; ++d) a.segcap[d] = s.segcap[d];
-  const int nd = bs->ndim;
+  const int nf = bs->nvar + 2;
+  const dim3 gc((unsigned)std::max<int64_t>(1, std::min<int64_t>(64, (bs->max_interior + 255) / 256)),
+                (unsigned)bs->nblocks);
+  if (ks[0].face) hipLaunchKernelGGL(pick_prim(bs->nvar, nd), gc, dim3(256), 0, ctx->stream, a);
   for (int d = 0; d < nd; ++d) {
-    AxisFn f = pick_adj_axis(recon, rs, bs->nvar, nd, d);
-    if (!f) return set_error(ctx, APH_E_INVALID, "adjoint: unsupported combination");
     int64_t maxu = 0;
     for (const auto &k : bs->h) {
       const int A = (d == 0) ? 1 : 0, B = (d == 2) ? 1 : 2;
       maxu = std::max<int64_t>(maxu, (int64_t)k.nx[A] * k.nx[B] *
                                          (k.nx[d] / adj_seg(k.nx[d], s.segcap[d])));
     }
-    dim3 gr((unsigned)((maxu + 63) / 64), (unsigned)bs->nblocks);
-    hipLaunchKernelGGL(f, gr, dim3(64), 0, ctx->stream, a);
+    const unsigned nbk = (unsigned)bs->nblocks;
+    if (ks[d].face) {
+      hipLaunchKernelGGL(ks[d].face, dim3((unsigned)((maxu + 63) / 64), nbk), dim3(64), 0,
+                         ctx->stream, a);
+      hipLaunchKernelGGL(ks[d].recon, dim3((unsigned)((maxu * nf + 63) / 64), nbk), dim3(64),
+                         0, ctx->stream, a);
+    } else {
+      hipLaunchKernelGGL(ks[d].axis, dim3((unsigned)((maxu + 63) / 64), nbk), dim3(64), 0,
+                         ctx->stream, a);
+    }
   }
-  GatherFn gf = pick_gather(bs->nvar, nd);
-  dim3 gg((unsigned)std::max<int64_t>(1, std::min<int64_t>(64, (bs->max_interior + 255) / 256)),
-          (unsigned)bs->nblocks);
-  hipLaunchKernelGGL(gf, gg, dim3(256), 0, ctx->stream, a);
+  hipLaunchKernelGGL(pick_gather(bs->nvar, nd), gc, dim3(256), 0, ctx->stream, a);
   APH_LAUNCH_CHECK(ctx);
   return APH_OK;
 }

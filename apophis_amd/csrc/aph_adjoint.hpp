@@ -220,6 +220,181 @@ APH_HD void ppm_adj(const double *q, double qlb, double qrb, double *qb) {
   qb[4] += q4b;
 }
 
+// ---- face-shared PPM (ppm_face / ppm_cell of aph_point.hpp): the limited value of a
+// face is formed once and its cotangent, summed over the two cells that use it, is
+// propagated once.  The reverse sweep multiplies by 1/6 where the forward divides by 6
+// (a cotangent needs no bitwise match; the forward values do, and are recomputed
+// exactly).
+constexpr double kSixth = 1.0 / 6.0;
+
+// What the reverse of one face needs from its forward.
+struct PpmFaceTape {
+  double qa, d2qc_im1, d2qc;
+  bool lim;  // the extremum-limited value dph_tmp was taken
+  bool sgn;  // the three signs agreed (qd != 0)
+};
+
+// ppm_face of cells (q_im2 .. q_ip1), bitwise, recording its branches.
+template <bool FAST>
+APH_HD double ppm_face_tape(double q_im2, double q_im1, double q_i, double q_ip1,
+                            PpmFaceTape &t) {
+  const double C2 = 1.25;
+  const double qa0 = (q_i - q_im1);
+  const double qb0 = (q_ip1 - q_i);
+  const double dd_im1 = 0.5 * qa0 + 0.5 * (q_im1 - q_im2);
+  const double dd = 0.5 * qb0 + 0.5 * qa0;
+  const double avg = 0.5 * (q_im1 + q_i);
+  double dph = avg + (FAST ? div6_fast(dd_im1 - dd) : (dd_im1 - dd) / 6.0);
+  t.d2qc_im1 = q_im2 + q_i - 2.0 * q_im1;
+  t.d2qc = q_im1 + q_ip1 - 2.0 * q_i;
+  const double qa_tmp = dph - q_im1;
+  const double qb_tmp = q_i - dph;
+  t.qa = 3.0 * (q_im1 + q_i - 2.0 * dph);
+  double qd = 0.0;
+  t.sgn = (neg(t.qa) == neg(t.d2qc_im1) && neg(t.qa) == neg(t.d2qc));
+  if (t.sgn)
+    qd = with_sign_of(t.qa, fmin(C2 * fabs(t.d2qc_im1), fmin(C2 * fabs(t.d2qc), fabs(t.qa))));
+  const double dph_tmp = avg - (FAST ? div6_fast(qd) : qd / 6.0);
+  t.lim = (qa_tmp * qb_tmp < 0.0);
+  if (t.lim) dph = dph_tmp;
+  return dph;
+}
+
+// Cotangent Fb of a face value -> cells (q_im2 .. q_ip1), added into qb[0..3].
+APH_HD void ppm_face_adj(const PpmFaceTape &t, double Fb, double *qb) {
+  double dph0b = 0.0, avgb = 0.0;
+  double b0 = 0.0, b1 = 0.0, b2 = 0.0, b3 = 0.0;  // q_im2, q_im1, q_i, q_ip1
+  if (t.lim) {  // dph = avg - qd/6
+    avgb += Fb;
+    if (t.sgn) {
+      double ab = 0.0, bb = 0.0, cb = 0.0;
+      ppm_qd_adj(t.qa, t.d2qc_im1, t.d2qc, -Fb * kSixth, ab, bb, cb);
+      // qa = 3 (q_im1 + q_i - 2 dph0)
+      b1 += 3.0 * ab;
+      b2 += 3.0 * ab;
+      dph0b -= 6.0 * ab;
+      // d2qc_im1 = q_im2 + q_i - 2 q_im1 ; d2qc = q_im1 + q_ip1 - 2 q_i
+      b0 += bb;
+      b2 += bb;
+      b1 -= 2.0 * bb;
+      b1 += cb;
+      b3 += cb;
+      b2 -= 2.0 * cb;
+    }
+  } else {
+    dph0b += Fb;
+  }
+  // dph0 = avg + (dd_im1 - dd)/6 ; avg = (q_im1 + q_i)/2
+  avgb += dph0b;
+  const double dd_im1b = dph0b * kSixth, ddb = -dd_im1b;
+  b1 += 0.5 * avgb;
+  b2 += 0.5 * avgb;
+  // dd_im1 = qa0/2 + (q_im1 - q_im2)/2 ; dd = qb0/2 + qa0/2 ; qa0 = q_i - q_im1 ;
+  // qb0 = q_ip1 - q_i
+  const double qa0b = 0.5 * dd_im1b + 0.5 * ddb;
+  const double qb0b = 0.5 * ddb;
+  b1 += 0.5 * dd_im1b;
+  b0 -= 0.5 * dd_im1b;
+  b2 += qa0b - qb0b;
+  b1 -= qa0b;
+  b3 += qb0b;
+  qb[0] += b0;
+  qb[1] += b1;
+  qb[2] += b2;
+  qb[3] += b3;
+}
+
+// Cotangents (ql_ip1b, qr_ib) of ppm_cell -> cells q[0..4] (added into qb) and the two
+// face values (dphb, dph_ip1b, set).  The forward is recomputed bitwise from the given
+// face values.
+APH_HD void ppm_cell_adj(const double *q, double dph, double dph_ip1, double qlb, double qrb,
+                         double *qb, double &dphb, double &dph_ip1b) {
+  const double C2 = 1.25;
+  const double q_im2 = q[0], q_im1 = q[1], q_i = q[2], q_ip1 = q[3], q_ip2 = q[4];
+  const double d2qc_im1 = q_im2 + q_i - 2.0 * q_im1;
+  const double d2qc = q_im1 + q_ip1 - 2.0 * q_i;
+  const double d2qc_ip1 = q_i + q_ip2 - 2.0 * q_ip1;
+  const double d2qf = 6.0 * (dph + dph_ip1 - 2.0 * q_i);
+  const double dqf_minus = q_i - dph;
+  const double dqf_plus = dph_ip1 - q_i;
+  const double qa_tmp = dqf_minus * dqf_plus;
+  const double qb_tmp = (q_ip1 - q_i) * (q_i - q_im1);
+  const bool s4 = (neg(d2qc_im1) == neg(d2qc) && neg(d2qc_im1) == neg(d2qc_ip1) &&
+                   neg(d2qc_im1) == neg(d2qf));
+  const double ea = C2 * fabs(d2qc_im1), eb = C2 * fabs(d2qc), ec = C2 * fabs(d2qc_ip1),
+               ed = fabs(d2qf);
+  const bool m1_is_b = eb < ea;
+  const double m1 = m1_is_b ? eb : ea;
+  const bool m2_is_d = ed < ec;
+  const double m2 = m2_is_d ? ed : ec;
+  const bool sel_m2 = m2 < m1;
+  double qe = 0.0;
+  if (s4) qe = with_sign_of(d2qf, fmin(fmin(ea, eb), fmin(ec, ed)));
+  const double qa = fmax(fabs(q_im1), fabs(q_im2));
+  const double qbm = fmax(fmax(fabs(q_i), fabs(q_ip1)), fabs(q_ip2));
+  const bool rho_on = fabs(d2qf) > (1.0e-12) * fmax(qa, qbm);
+  const double rho = rho_on ? qe / d2qf : 0.0;
+  // ---- reverse
+  double dphb_ = 0.0, dphpb = 0.0, dqf_minusb = 0.0, dqf_plusb = 0.0, rhob = 0.0;
+  double q2b = 0.0;
+  if ((qa_tmp <= 0.0 || qb_tmp <= 0.0)) {
+    if (rho <= (1.0 - (1.0e-12))) {  // qr = q_i - rho dqf_minus, ql = q_i + rho dqf_plus
+      q2b += qrb + qlb;
+      rhob += -qrb * dqf_minus + qlb * dqf_plus;
+      dqf_minusb += -qrb * rho;
+      dqf_plusb += qlb * rho;
+    } else {
+      dphb_ += qrb;
+      dphpb += qlb;
+    }
+  } else {
+    if (fabs(dqf_minus) >= 2.0 * fabs(dqf_plus)) {  // qr = q_i - 2 dqf_plus
+      q2b += qrb;
+      dqf_plusb -= 2.0 * qrb;
+    } else {
+      dphb_ += qrb;
+    }
+    if (fabs(dqf_plus) >= 2.0 * fabs(dqf_minus)) {  // ql = q_i + 2 dqf_minus
+      q2b += qlb;
+      dqf_minusb += 2.0 * qlb;
+    } else {
+      dphpb += qlb;
+    }
+  }
+  double d2qfb = 0.0, d2qc_im1b = 0.0, d2qcb = 0.0, d2qc_ip1b = 0.0;
+  if (rho_on) {  // rho = qe / d2qf
+    const double qeb = rhob / d2qf;
+    d2qfb -= qeb * rho;
+    if (s4) {
+      const double sg = with_sign_of(d2qf, qeb);
+      if (sel_m2) {
+        if (m2_is_d)
+          d2qfb += sg * dabs(d2qf);
+        else
+          d2qc_ip1b += sg * C2 * dabs(d2qc_ip1);
+      } else {
+        if (m1_is_b)
+          d2qcb += sg * C2 * dabs(d2qc);
+        else
+          d2qc_im1b += sg * C2 * dabs(d2qc_im1);
+      }
+    }
+  }
+  q2b += dqf_minusb - dqf_plusb;
+  dphb_ -= dqf_minusb;
+  dphpb += dqf_plusb;
+  dphb_ += 6.0 * d2qfb;
+  dphpb += 6.0 * d2qfb;
+  q2b -= 12.0 * d2qfb;
+  qb[0] += d2qc_im1b;
+  qb[1] += d2qcb - 2.0 * d2qc_im1b;
+  qb[2] += q2b + d2qc_im1b + d2qc_ip1b - 2.0 * d2qcb;
+  qb[3] += d2qcb - 2.0 * d2qc_ip1b;
+  qb[4] += d2qc_ip1b;
+  dphb = dphb_;
+  dph_ip1b = dphpb;
+}
+
 template <int RECON>
 APH_HD void recon1_adj(const double *w, double qlb, double qrb, double *wb) {
   if constexpr (RECON == RECON_PPM) {

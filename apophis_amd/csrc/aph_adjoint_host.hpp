@@ -8,10 +8,13 @@
 namespace aph {
 
 // Device scratch of the adjoint: primitive-space cotangents (NV+2 fields, pack layout),
-// per-axis spill records and their per-block offsets, and two cotangent registers used
-// by a whole RK2 step.  Allocated on first use, released with the plan.
+// per-axis spill records and their per-block offsets, two cotangent registers used by a
+// whole RK2 step, and for the split (PPM) form the primitive state in two layouts, the
+// x-transposed L_out and the face-state cotangents.  Allocated on first use, released
+// with the plan.
 struct AdjScratch {
   double *pbar = nullptr;
+  double *prim = nullptr, *prim_t = nullptr, *lam_t = nullptr, *qlb = nullptr, *qrb = nullptr;
   double *spill[3] = {nullptr, nullptr, nullptr};
   int64_t *spill_base = nullptr;
   double *lam1 = nullptr, *lam0p = nullptr;
@@ -23,7 +26,7 @@ int adj_stage(aph_context *ctx, const aph_blockset *bs, int recon, int rs, const
               const double *g, int grav, AdjScratch &s, int stage, const double *u_in,
               const double *dt_dev, const double *lam_out, const double *lam_acc,
               double *lam_in, double *lam_n);
-int adj_prepare(aph_context *ctx, const aph_blockset *bs, AdjScratch &s);
+int adj_prepare(aph_context *ctx, const aph_blockset *bs, int recon, AdjScratch &s);
 void adj_release(AdjScratch &s);
 bool adj_supported(int recon, int rs, int nv, int nd);
 

@@ -985,7 +985,7 @@ int aph_adjoint_step(aph_context *ctx, aph_plan *p, double *u0, double *u1,
     return set_error(ctx, APH_E_INVALID, "aph_adjoint_step: bad arguments");
   if (!adj_supported(p->recon, p->rsolver, p->bs->nvar, p->bs->ndim))
     return set_error(ctx, APH_E_INVALID, "adjoint: unsupported (recon, riemann, nvar)");
-  int rc = adj_prepare(ctx, p->bs, p->adj);
+  int rc = adj_prepare(ctx, p->bs, p->recon, p->adj);
   if (rc) return rc;
   // recompute the stage-1 state from the step's checkpoint, then stage 2 and stage 1
   rc = launch_stage(ctx, p, 1, u0, nullptr, u1, dt_dev, 0.0);
@@ -1002,7 +1002,7 @@ int aph_adjoint_step_stored(aph_context *ctx, aph_plan *p, const double *u0, con
     return set_error(ctx, APH_E_INVALID, "aph_adjoint_step_stored: bad arguments");
   if (!adj_supported(p->recon, p->rsolver, p->bs->nvar, p->bs->ndim))
     return set_error(ctx, APH_E_INVALID, "adjoint: unsupported (recon, riemann, nvar)");
-  int rc = adj_prepare(ctx, p->bs, p->adj);
+  int rc = adj_prepare(ctx, p->bs, p->recon, p->adj);
   if (rc) return rc;
   // stage 2 (input u1) then stage 1 (input u0)
   rc = adj_stage(ctx, p->bs, p->recon, p->rsolver, p->eos, p->g, p->grav, p->adj, 2, u1,

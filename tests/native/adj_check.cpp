@@ -63,3 +63,16 @@ extern "C" void chk_floor(const double *u, const double *e, double *out) {
     out[5 + i] = b[i];
   }
 }
+// Face-shared PPM adjoint of one cell: its two faces (taped), the cell part, then each
+// face's cotangent through ppm_face_adj.  Also returns the taped face values (out[5..6]).
+extern "C" void chk_ppm_split_adj(const double *q, const double *ob, double *qb) {
+  PpmFaceTape tlo, thi;
+  const double lo = ppm_face_tape<false>(q[0], q[1], q[2], q[3], tlo);
+  const double hi = ppm_face_tape<false>(q[1], q[2], q[3], q[4], thi);
+  double lob = 0.0, hib = 0.0;
+  ppm_cell_adj(q, lo, hi, ob[0], ob[1], qb, lob, hib);
+  ppm_face_adj(tlo, lob, qb);
+  ppm_face_adj(thi, hib, qb + 1);
+  qb[5] = lo;
+  qb[6] = hi;
+}

@@ -119,3 +119,28 @@ def test_cons_to_prim_adjoint_matches_fd(lib):
         checked += fd_check(f, adj, u, 8, rng)
     assert checked > 900
 
+
+
+def test_face_shared_ppm_adjoint_matches_ppm_adj(lib):
+    """The split reverse (ppm_face_tape / ppm_cell_adj / ppm_face_adj, as the batched
+    adjoint's reconstruction kernel composes them) against the one-cell ppm_adj: the same
+    branches, so the same cotangents up to rounding; the taped faces are ppm_face bitwise."""
+    rng = np.random.default_rng(5)
+    worst = 0.0
+    for trial in range(4000):
+        if trial % 2 == 0:
+            c = rng.normal(size=3)
+            t = np.arange(5) - 2
+            q = c[0] + c[1] * t + c[2] * t * t + 1e-3 * rng.normal(size=5)
+        else:
+            q = rng.normal(size=5)
+        ob = rng.normal(size=2)
+        a = _call(lib.chk_ppm_adj, 5, q, ob)
+        b = _call(lib.chk_ppm_split_adj, 7, q, ob)
+        shared = _call(lib.chk_ppm_shared, 2, q)
+        lo = _call(lib.chk_ppm_face, 1, q[:4], ctypes.c_long(1), 0)
+        assert b[5] == lo[0]
+        assert tuple(shared) == orc.ppm(*q)
+        scale = max(np.abs(a).max(), np.abs(ob).max(), 1e-300)
+        worst = max(worst, np.abs(a - b[:5]).max() / scale)
+    assert worst < 1e-13, worst

@@ -24,281 +24,11 @@
 #include <vector>
 
 #include "aph_adjoint_host.hpp"
-#include "aph_device.hpp"
-#include "aph_march.hpp"
-#include "aph_runtime.hpp"
+#include "aph_fused_kernel.hpp"
 
 namespace aph {
 
-struct TileCfg {
-  int t[3];  // tile extent per axis (cells)
-  int s[3];  // segment length per axis (cells one lane marches)
-};
-
-constexpr int kMaxClasses = 8;
-
-struct FusedArgs {
-  const aph_block *blocks;
-  const int4 *items;  // {block | cls << 24, t0x, t0y, t0z}; block < 0: padding
-  int nitems;
-  int ndim;
-  TileCfg cfg[kMaxClasses];
-  aph_eos eos;
-  double g[3];
-  int grav;
-  const double *u_in;
-  const double *u_n;
-  double *u_out;
-  const double *dt;
-  double gam0, gam1, beta;
-  double *dt_min;  // last stage: atomic min of cfl * dt candidate (reset before stage 1)
-  double cfl;
-  int has_un;      // u_n is a separate register (every stage but the first)
-  int last;        // last stage of the step: estimate the next dt
-};
-
 namespace {
-
-// Bases (element offset of component 0) of the block and its two neighbours along
-// axis D, or -1 for a physical boundary.
-struct AxisNbrs {
-  int64_t self, lo, hi;
-  int bclo, bchi;
-};
-
-// Issues the loads of cell c of the pencil (neighbour block, reflected or clamped
-// ghost) and returns whether its normal momentum must change sign (reflecting
-// boundary).  The sign change is left to the caller: applied here it would make the
-// loads complete before the function returns and defeat the march's prefetch.
-template <int NV>
-__device__ __forceinline__ bool load_cell(const FusedArgs &a, const aph_block &bk,
-                                          const AxisNbrs &nb, int D, int n_d, int c,
-                                          int64_t tbase, int64_t sd, int64_t nc, double *u) {
-  int64_t base = nb.self;
-  int idx = c;
-  bool flip = false;
-  if (c < 0) {
-    if (nb.lo >= 0) {
-      base = nb.lo;
-      idx = c + n_d;
-    } else if (nb.bclo == APH_BC_REFLECT) {
-      idx = -1 - c;
-      flip = true;
-    } else {
-      idx = 0;
-    }
-  } else if (c >= n_d) {
-    if (nb.hi >= 0) {
-      base = nb.hi;
-      idx = c - n_d;
-    } else if (nb.bchi == APH_BC_REFLECT) {
-      idx = 2 * n_d - 1 - c;
-      flip = true;
-    } else {
-      idx = n_d - 1;
-    }
-  }
-  const double *p = a.u_in + base + tbase + (int64_t)idx * sd;
-#pragma unroll
-  for (int v = 0; v < NV; ++v) u[v] = p[v * nc];
-  return flip;
-}
-
-template <int RECON, int RS, int NV, int NDIM, int D>
-__device__ __forceinline__ void sweep(const FusedArgs &a, const aph_block &bk, int bidx,
-                                      const TileCfg &cfg, const int *t0, double *du_s,
-                                      double &dtmin) {
-  constexpr bool LAST = (D == NDIM - 1);
-  constexpr int A = (D == 0) ? 1 : 0;  // first transverse axis (fastest across lanes)
-  constexpr int B = (D == 2) ? 1 : 2;  // second transverse axis
-  const int ndim = NDIM;
-  const int N0 = ext(bk, 0, ndim), N1 = ext(bk, 1, ndim), N2 = ext(bk, 2, ndim);
-  const int64_t nc = (int64_t)N0 * N1 * N2;
-  const int64_t st[3] = {1, N0, (int64_t)N0 * N1};
-  const int g[3] = {gst(bk, 0, ndim), gst(bk, 1, ndim), gst(bk, 2, ndim)};
-  constexpr int sdn = (D == 0) ? 1 : (D == 1 ? 3 : 9);
-  AxisNbrs nb;
-  nb.self = (int64_t)NV * bk.cell_base;
-  {
-    const int l = bk.nbr[13 - sdn], h = bk.nbr[13 + sdn];
-    nb.lo = l >= 0 ? (int64_t)NV * a.blocks[l].cell_base : -1;
-    nb.hi = h >= 0 ? (int64_t)NV * a.blocks[h].cell_base : -1;
-    nb.bclo = bk.bc[2 * D];
-    nb.bchi = bk.bc[2 * D + 1];
-  }
-  const int n_d = bk.nx[D];
-  const int nseg = cfg.t[D] / cfg.s[D];
-  const int units = cfg.t[A] * cfg.t[B] * nseg;
-  const double area = bk.area[D];
-  const double vol = bk.vol;
-  const double beta_dt = a.beta * a.dt[0];
-  const int tx = cfg.t[0], ty = cfg.t[1];
-  for (int unit = threadIdx.x; unit < units; unit += 64) {
-    int seg, pa, pb;
-    if (D == 0) {  // segments of one row on adjacent lanes
-      seg = unit % nseg;
-      const int p = unit / nseg;
-      pa = p % cfg.t[A];
-      pb = p / cfg.t[A];
-    } else {  // adjacent lanes on adjacent x (coalesced)
-      pa = unit % cfg.t[A];
-      const int p = unit / cfg.t[A];
-      seg = p % nseg;
-      pb = p / nseg;
-    }
-    const int ca = t0[A] + pa, cb = t0[B] + pb;  // interior coordinates
-    const int64_t tbase = (int64_t)(ca + g[A]) * st[A] + (int64_t)(cb + g[B]) * st[B] +
-                          (int64_t)g[D] * st[D];
-    const int f_lo = t0[D] + seg * cfg.s[D];
-    const int f_hi = f_lo + cfg.s[D];
-    double fprev[NV];
-    bool flip = false;  // sign change of the fetched cell, applied by convert
-    auto fetch = [&](int c, double *u) {
-      flip = load_cell<NV>(a, bk, nb, D, n_d, c, tbase, st[D], nc, u);
-    };
-    auto convert = [&](const double *raw, double *dst) {
-      double u[NV];
-#pragma unroll
-      for (int v = 0; v < NV; ++v) u[v] = raw[v];
-      if (flip) u[1 + D] = -1.0 * u[1 + D];
-      cons_to_prim_pt<NV>(u, a.eos, dst, dst + NV);
-    };
-    auto emit = [&](int f, const double *fo, const double *, const double *) {
-      if (f > f_lo) {
-        const int m = f - 1;  // cell whose upper face is f
-        int lc[3];
-        lc[D] = m - t0[D];
-        lc[A] = pa;
-        lc[B] = pb;
-        const int l = (lc[2] * ty + lc[1]) * tx + lc[0];
-        const int tcells = tx * ty * cfg.t[2];
-        if constexpr (!LAST) {
-#pragma unroll
-          for (int v = 0; v < NV; ++v) {
-            const double d = area * fo[v] - area * fprev[v];
-            if (D == 0)
-              du_s[v * tcells + l] = d;
-            else
-              du_s[v * tcells + l] = du_s[v * tcells + l] + d;
-          }
-        } else {
-          const int64_t cidx = tbase + (int64_t)m * st[D];
-          double u[NV], w[NV], gm[2];
-          const double *pin = a.u_in + nb.self + cidx;
-#pragma unroll
-          for (int v = 0; v < NV; ++v) u[v] = pin[v * nc];
-          if (a.grav)  // the gravity source needs the stage-start primitives
-            cons_to_prim_pt<NV>(u, a.eos, w, gm);
-          else
-            floor_cons_pt<NV>(u, a.eos);
-          double un[NV];
-          if (a.has_un) {
-            const double *pn = a.u_n + nb.self + cidx;
-#pragma unroll
-            for (int v = 0; v < NV; ++v) un[v] = pn[v * nc];
-            floor_cons_pt<NV>(un, a.eos);
-          } else {
-#pragma unroll
-            for (int v = 0; v < NV; ++v) un[v] = u[v];
-          }
-          double out[NV];
-#pragma unroll
-          for (int v = 0; v < NV; ++v) {
-            double du;
-            if (NDIM == 1)
-              du = (area * fo[v] - area * fprev[v]);
-            else
-              du = du_s[v * tcells + l] + (area * fo[v] - area * fprev[v]);
-            const double div = -du / vol;
-            out[v] = a.gam0 * u[v] + a.gam1 * un[v] + beta_dt * div;
-          }
-          if (a.grav) {  // ApplyGravity with the stage-start primitives
-            const double gdt = beta_dt;
-            out[1] += a.g[0] * w[0] * gdt;
-            double de = a.g[0] * w[1];
-            if (NDIM >= 2) {
-              out[2] += a.g[1] * w[0] * gdt;
-              de += a.g[1] * w[2];
-            }
-            if (NDIM >= 3) {
-              out[3] += a.g[2] * w[0] * gdt;
-              de += a.g[2] * w[3];
-            }
-            out[4] += de * w[0] * gdt;
-          }
-          double *po = a.u_out + nb.self + cidx;
-#pragma unroll
-          for (int v = 0; v < NV; ++v) po[v * nc] = out[v];
-          if (a.last) {  // EstimateTimestep of the new state (after its floors)
-            double wq[NV], gq[2];
-            cons_to_prim_pt<NV>(out, a.eos, wq, gq);
-            dtmin = fmin(dtmin, dt_point(wq, out[4], a.eos, bk.dx, NDIM));
-          }
-        }
-      }
-#pragma unroll
-      for (int v = 0; v < NV; ++v) fprev[v] = fo[v];
-    };
-    march<RECON, RS, NV, 1 + D, NV>(f_lo, f_hi, fetch, convert, emit);
-  }
-}
-
-template <int RECON, int RS, int NV, int NDIM>
-__global__ void __launch_bounds__(64) k_fused(FusedArgs a) {
-  extern __shared__ double du_s[];
-  double dtmin = 1.79769313486231570815e308;
-  for (int it = blockIdx.x; it < a.nitems; it += gridDim.x) {
-    const int4 item = a.items[it];
-    if (item.x < 0) continue;
-    const int b = item.x & 0xFFFFFF;
-    const int cls = item.x >> 24;
-    const aph_block &bk = a.blocks[b];
-    const TileCfg cfg = a.cfg[cls];
-    const int t0[3] = {item.y, item.z, item.w};
-    sweep<RECON, RS, NV, NDIM, 0>(a, bk, b, cfg, t0, du_s, dtmin);
-    if constexpr (NDIM >= 2) {
-      __syncthreads();
-      sweep<RECON, RS, NV, NDIM, 1>(a, bk, b, cfg, t0, du_s, dtmin);
-    }
-    if constexpr (NDIM >= 3) {
-      __syncthreads();
-      sweep<RECON, RS, NV, NDIM, 2>(a, bk, b, cfg, t0, du_s, dtmin);
-    }
-    __syncthreads();
-  }
-  if (a.last && a.dt_min) {
-    for (int o = 32; o > 0; o >>= 1) dtmin = fmin(dtmin, __shfl_xor(dtmin, o));
-    if (threadIdx.x == 0) atomic_min_pos(a.dt_min, a.cfl * dtmin);
-  }
-}
-
-// Initial time step of a raw conserved state (floors applied on the fly).
-template <int NV, int NDIM>
-__global__ void __launch_bounds__(256)
-    k_plan_dt(const aph_block *__restrict__ blocks, aph_eos eos, const double *__restrict__ cons,
-              double cfl, double *__restrict__ out) {
-  const aph_block &bk = blocks[blockIdx.y];
-  const int N0 = ext(bk, 0, NDIM), N1 = ext(bk, 1, NDIM), N2 = ext(bk, 2, NDIM);
-  const int64_t nc = (int64_t)N0 * N1 * N2;
-  const int gx = gst(bk, 0, NDIM), gy = gst(bk, 1, NDIM), gz = gst(bk, 2, NDIM);
-  const int64_t nint = (int64_t)bk.nx[0] * bk.nx[1] * bk.nx[2];
-  const double *u = cons + (int64_t)NV * bk.cell_base;
-  double m = 1.79769313486231570815e308;
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nint;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    const int i = (int)(t % bk.nx[0]);
-    const int j = (int)((t / bk.nx[0]) % bk.nx[1]);
-    const int k = (int)(t / ((int64_t)bk.nx[0] * bk.nx[1]));
-    const int64_t c = ((int64_t)(k + gz) * N1 + (j + gy)) * N0 + (i + gx);
-    double uu[NV], w[NV], gm[2];
-#pragma unroll
-    for (int v = 0; v < NV; ++v) uu[v] = u[v * nc + c];
-    cons_to_prim_pt<NV>(uu, eos, w, gm);
-    m = fmin(m, dt_point(w, uu[4], eos, bk.dx, NDIM));
-  }
-  for (int o = 32; o > 0; o >>= 1) m = fmin(m, __shfl_xor(m, o));
-  if ((threadIdx.x & 63) == 0) atomic_min_pos(out, cfl * m);
-}
 
 __global__ void k_fill(double *p, double v) { *p = v; }
 
@@ -351,7 +81,6 @@ struct aph_plan {
 
 namespace {
 
-using KernelFn = void (*)(FusedArgs);
 
 template <int RECON, int RS, int NV>
 KernelFn pick_dim(int ndim) {
@@ -372,6 +101,7 @@ KernelFn pick_nv(int nv, int ndim) {
 }
 
 KernelFn pick_kernel(int recon, int rs, int nv, int ndim) {
+  if (nv != 5 && nv != 6) return pick_kernel_wide(recon, rs, nv, ndim);
 #define APH_PICK(R, S) \
   if (recon == R && rs == S) return pick_nv<R, S>(nv, ndim);
   APH_PICK(RECON_PPM, RS_HLLC)
@@ -947,8 +677,12 @@ int aph_plan_estimate_timestep(aph_context *ctx, aph_plan *p, const double *cons
 #define APH_PDT(NVv, ND) k_plan_dt<NVv, ND><<<gr, 256, 0, ctx->stream>>>(bs->d, p->eos, cons, cfl, dt_out)
     if (bs->nvar == 5) {
       if (bs->ndim == 1) APH_PDT(5, 1); else if (bs->ndim == 2) APH_PDT(5, 2); else APH_PDT(5, 3);
-    } else {
+    } else if (bs->nvar == 6) {
       if (bs->ndim == 1) APH_PDT(6, 1); else if (bs->ndim == 2) APH_PDT(6, 2); else APH_PDT(6, 3);
+    } else {
+      PlanDtFn f = pick_plan_dt_wide(bs->nvar, bs->ndim);
+      if (!f) return set_error(ctx, APH_E_INVALID, "aph_plan_estimate_timestep: unsupported nvar");
+      hipLaunchKernelGGL(f, gr, dim3(256), 0, ctx->stream, bs->d, p->eos, cons, cfl, dt_out);
     }
 #undef APH_PDT
   }

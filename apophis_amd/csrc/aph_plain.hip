@@ -312,6 +312,8 @@ inline dim3 grid_for(int64_t work, int threads, int nblk, int64_t cap = 1024) {
   switch (nv) {                                                                        \
     case 5: { constexpr int NV = 5; KCALL; } break;                                    \
     case 6: { constexpr int NV = 6; KCALL; } break;                                    \
+    case 12: { constexpr int NV = 12; KCALL; } break;                                  \
+    case 14: { constexpr int NV = 14; KCALL; } break;                                  \
     default: return set_error(ctx, APH_E_INVALID, "unsupported nvar %d", nv);          \
   }
 

@@ -78,8 +78,10 @@ int aph_blockset_create(aph_context *ctx, const aph_block *blocks, int32_t nbloc
   if (!ctx || !out || (!blocks && nblocks > 0) || nblocks < 0)
     return set_error(ctx, APH_E_INVALID, "aph_blockset_create: bad arguments");
   if (ndim < 1 || ndim > 3) return set_error(ctx, APH_E_INVALID, "ndim must be 1..3");
-  if (nscalars < 0 || nscalars > 1)
-    return set_error(ctx, APH_E_INVALID, "nscalars=%d unsupported (0..1)", nscalars);
+  if (nscalars != 0 && nscalars != 1 && nscalars != 7 && nscalars != 9)
+    return set_error(ctx, APH_E_INVALID,
+                     "nscalars=%d unsupported (0, 1, 7 = ONe composition, 9 = ONe + one level set)",
+                     nscalars);
   aph_blockset *bs = new (std::nothrow) aph_blockset();
   if (!bs) return set_error(ctx, APH_E_NOMEM, "host allocation failed");
   bs->h.assign(blocks, blocks + nblocks);

@@ -178,9 +178,49 @@ def initialize_hydro(pin):
     pkg.add_param("integrator", Integrator[integ_str])
     pkg.add_param("flux_first_stage", flux_first)
     pkg.add_param("flux_other_stage", flux_other)
-    nscalars = int(_get(pin, "hydro", "nscalars", 0))
+    # passive scalars (hydro.cpp:141-177): the ONe composition (6 species densities and
+    # Ye), then two fields per level set; "nscalars" adds generic scalars (an extension:
+    # the reference has no such key, the KH workload's marker scalar needs it)
+    nscalars = 0
+    comp_str = _get(pin, "hydro", "composition", "unset")
+    ncomp, labels, abar, zbar, ebind = 0, [], [], [], []
+    if comp_str == "ONe":
+        nscalars += 7
+        ncomp = 6
+        labels = ["alpha", "C12", "O16", "Ne20", "IME", "IGE"]
+        abar = [4.0, 12.0, 16.0, 20.0, 30.0, 56.0]
+        zbar = [2.0, 6.0, 8.0, 10.0, 15.0, 28.0]
+        ebind = [6.8266e+18, 7.41121e+18, 7.69691e+18, 7.74994e+18, 8.17906e+18, 8.34e+18]
+    pkg.add_param("ncomp", ncomp)
+    pkg.add_param("comp_labels", labels)
+    pkg.add_param("comp_abar", abar)
+    pkg.add_param("comp_zbar", zbar)
+    pkg.add_param("comp_ebind", ebind)
+    nlset = int(_get(pin, "hydro", "nlset", 0))
+    if nlset > 0 and ncomp < 1:
+        raise ApophisError(
+            "[Apophis]: Levelset is enabled but no composition is set. Exiting.")
+    nscalars += 2 * nlset
+    pkg.add_param("nlset", nlset)
+    nscalars += int(_get(pin, "hydro", "nscalars", 0))
     pkg.add_param("nscalars", nscalars)
+    cons_labels = ["Density", "MomentumDensity1", "MomentumDensity2", "MomentumDensity3",
+                   "TotalEnergyDensity"]
+    prim_labels = ["Density", "Velocity1", "Velocity2", "Velocity3", "Pressure"]
+    extra = labels + (["ye"] if ncomp > 0 else [])
+    for i in range(nlset):
+        extra += [f"lset{i}", f"xfuel{i}"]
+    cons_labels += ["scalar_density_" + x for x in extra]
+    prim_labels += ["scalar_" + x for x in extra]
+    pkg.add_param("cons_labels", cons_labels)
+    pkg.add_param("prim_labels", prim_labels)
     eos_type = _get(pin, "eos", "type", "ideal")
+    if eos_type == "helm":
+        if ncomp < 1:
+            raise ApophisError("[Apophis]: Helmholtz EOS is enabled but no composition is "
+                               "set. Exiting.")
+        raise ApophisError("[Apophis]: Helmholtz EOS (singularity-eos tables) is not "
+                           "available in this build.")
     if eos_type != "ideal":
         raise ApophisError("[Apophis]: EOS not recognized. Exiting.")
     floor = float(np.sqrt(1024.0 * np.finfo(np.float32).tiny))

@@ -401,3 +401,75 @@ def linear_wave3d(res=64, recon="plm", riemann="hllc", integrator="vl2", amp=1.0
                    tlim=tlim, integrator=integrator,
                    meta={"exact": exact, "nx": nx, "domain_volume": size[0] * size[1] * size[2],
                          "lambda": lam})
+
+
+# ---------------------------------------------------------------- KH with composition
+def std_uniform_m05_05(seed, n):
+    """The first n values of std::uniform_real_distribution<double>(-0.5, 0.5) driven by
+    std::mt19937(seed), as libstdc++ computes them (the reference's pgen, kh.cpp:38-39):
+    generate_canonical<double, 53> takes two 32-bit words w1, w2 and forms
+    (w1 + w2 * 2^32) / 2^64 (one rounding, in the sum; at 1.0 it steps down to the largest
+    double below 1), then the distribution returns (b - a) * that + a.  numpy's legacy
+    RandomState seeds MT19937 with the same init_genrand and randint over the full
+    uint32 range returns its raw words in order."""
+    words = np.random.RandomState(seed & 0xFFFFFFFF).randint(
+        0, 2 ** 32, size=2 * n, dtype=np.uint64).astype(np.float64)
+    c = (words[0::2] + words[1::2] * 4294967296.0) / 18446744073709551616.0
+    c = np.where(c >= 1.0, np.nextafter(1.0, 0.0), c)
+    return 1.0 * c + -0.5
+
+
+def kh_comp(nx=(256, 128), block=(64, 128), vflow=1.0, drat=0.1, amp=0.01, ofrac=0.65,
+            ye=0.4983, pamp=2.5, gamma=1.666666666666667, eos_gm1=0.66666666667,
+            nlset=1, recon="plm", riemann="hllc", cfl=0.4, tlim=1.0, nsteps=100):
+    """The reference deck inputs/kh-comp-advection.in: Kelvin-Helmholtz slip surfaces with
+    random perturbations and the ONe composition (kh.cpp:268-313, iprob = 6) on
+    [-1,1]x[-1/2,1/2], periodic, nghost 2 (3 for PPM), PLM + HLLC, cfl 0.4, ideal gas with
+    gm1 = 0.66666666667 (the pgen's pressure uses hydro/gamma).  Scalars (hydro.cpp:
+    187-198): 6 species densities (alpha, C12, O16, Ne20, IME, IGE), Ye, then per level
+    set (lset, xfuel); the pgen sets O16, Ne20, Ye and lset0 and leaves the rest 0.
+    Block b draws its perturbations from std::mt19937(b) (Parthenon's gid; the blocks of
+    one level are numbered along x first) in the pgen's loop order k, j, i."""
+    gm1 = gamma - 1.0
+    nbx, nby = nx[0] // block[0], nx[1] // block[1]
+    m = lattice_mesh(2, (nbx, nby, 1), (block[0], block[1], 1), max(2, NEED_GHOST[recon]),
+                     (-1.0, -0.5, -0.5), (1.0, 0.5, 0.5), ["periodic"] * 6)
+    nsc = 7 + 2 * nlset
+    bs = BlockSet(2, [m], nscalars=nsc)
+    cons = np.zeros(bs.field_size(bs.nvar))
+    rho = 1.0
+    for b in range(bs.nblocks):
+        _, y, _ = bs.cell_centers(b)
+        inner = np.abs(y) < 0.25
+        ncell = y.size
+        # draws per cell in loop order: IM1, IM2, and for the inner layer IM1, IM2 again
+        per = np.where(inner.ravel(), 4, 2)
+        r = std_uniform_m05_05(b, int(per.sum()))
+        first = np.concatenate([[0], np.cumsum(per)[:-1]])
+        r1, r2 = r[first], r[first + 1]
+        r3 = np.where(inner.ravel(), r[np.minimum(first + 2, r.size - 1)], 0.0)
+        r4 = np.where(inner.ravel(), r[np.minimum(first + 3, r.size - 1)], 0.0)
+        shp = y.shape
+        d = np.full(ncell, rho)
+        m1 = vflow + amp * r1
+        m2 = amp * r2
+        o16 = np.full(ncell, ofrac * rho)
+        ne20 = np.full(ncell, (1.0 - ofrac) * rho)
+        yes = np.full(ncell, ye * rho)
+        ii = inner.ravel()
+        d[ii] = drat * rho
+        m1[ii] = -drat * (vflow + amp * r3[ii]) * rho
+        m2[ii] = drat * amp * r4[ii] * rho
+        o16[ii] = 1.0 * drat * rho
+        ne20[ii] = 0.0 * drat * rho
+        yes[ii] = 0.5 * drat * rho
+        lset = (0.25 - np.abs(y.ravel())) * d
+        en = pamp / gm1 + 0.5 * (m1 * m1 + m2 * m2) / d
+        z = np.zeros(ncell)
+        sc = [z, z, o16, ne20, z, z, yes]
+        for i in range(nlset):
+            sc += [lset if i == 0 else z, z]
+        _set_interior(bs, cons, b, d.reshape(shp), m1.reshape(shp), m2.reshape(shp), 0.0,
+                      en.reshape(shp), [s.reshape(shp) for s in sc])
+    return Problem("kh_comp", bs, cons, make_eos(eos_gm1), recon, riemann, cfl, tlim=tlim,
+                   nsteps=nsteps, meta={"nlset": nlset, "composition": "ONe"})

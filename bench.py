@@ -199,6 +199,32 @@ def rt_batch_leg(ctx, hydro, problems, torch, nsteps=300, every=20):
             "grad_max": float(g.abs().max().item())}
 
 
+def composition_leg(ctx, hydro, problems, torch, nsteps=20):
+    """The reference deck inputs/kh-comp-advection.in (ONe composition + one level set:
+    NV = 14, PLM + HLLC, blocks of 64 x 128) at its own 256 x 128 mesh and widened to
+    4096 x 2048 (1024 blocks): zone-cycles/s of the batched fused form, graph-replayed."""
+    out = {}
+    for name, nx in (("deck_256x128", (256, 128)), ("wide_4096x2048", (4096, 2048))):
+        p = problems.kh_comp(nx=nx)
+        cells = p.bs.interior_cells()
+        fd = hydro.FusedDriver(ctx, p.bs, p.cons, p.recon, p.riemann, p.eos, p.cfl)
+        fd.initial_dt()
+        fd.autotune(reps=3)
+        for _ in range(2):
+            fd.step()
+        fd.capture(nsteps)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fd.replay()
+        torch.cuda.synchronize()
+        out[name] = {"cells": cells, "blocks": p.bs.nblocks, "nvar": p.bs.nvar,
+                     "zone_cycles_per_s": cells * nsteps / (time.perf_counter() - t0),
+                     "tiles": {"x".join(map(str, k)): "x".join(map(str, v))
+                               for k, v in fd.tiles().items()}}
+        del fd
+    return out
+
+
 def profile_run(args, ctx, hydro, problems, torch):
     """Eager launches for rocprofv3: 3 RK2 steps of the chosen form on the headline
     workload (or the uneven block set), then a 1 GiB device-to-device copy repeated 3
@@ -382,6 +408,9 @@ def main():
               flush=True)
         del dq, mq
 
+    if not args.no_extras:
+        extras["kh_comp_ONe_lset"] = composition_leg(ctx, hydro, problems, torch)
+        print(f"# composition: {extras['kh_comp_ONe_lset']}", file=sys.stderr, flush=True)
     if not args.no_adjoint:
         extras["adjoint_kh"] = adjoint_leg(ctx, hydro, problems, torch)
         print(f"# adjoint: {extras['adjoint_kh']}", file=sys.stderr, flush=True)

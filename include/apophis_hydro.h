@@ -96,8 +96,11 @@ const char *aph_last_error(const aph_context *ctx);
 int aph_context_synchronize(aph_context *ctx);
 
 /* ------------------------------------------------------------- block set
- * Copies the block table to the device.  ndim in {1,2,3}; nscalars >= 0 passive
- * scalars follow the 5 hydro variables (hydro.cpp:141-202).  Synchronous. */
+ * Copies the block table to the device.  ndim in {1,2,3}; nscalars passive scalars
+ * follow the 5 hydro variables (hydro.cpp:141-202): 0, 1 (a generic scalar), 7 (the
+ * ONe composition: 6 species densities and Ye, hydro.cpp:154-162) or 9 (ONe plus one
+ * level set's two fields, hydro.cpp:170-174); other counts fail with APH_E_INVALID.
+ * Synchronous. */
 int aph_blockset_create(aph_context *ctx, const aph_block *blocks, int32_t nblocks,
                         int32_t ndim, int32_t nscalars, aph_blockset **out);
 int aph_blockset_destroy(aph_context *ctx, aph_blockset *bs);

@@ -29,10 +29,15 @@ def _hydro():
 
 
 def _pin(p):
+    hyd = {"reconstruction": p.recon, "riemann": p.riemann}
+    if p.meta.get("composition") == "ONe":  # the reference's own scalar options
+        hyd["composition"] = "ONe"
+        hyd["nlset"] = p.meta["nlset"]
+    else:
+        hyd["nscalars"] = p.bs.nscalars
     return {"parthenon/time": {"cfl": p.cfl, "integrator": p.integrator},
             "parthenon/mesh": {"nghost": int(p.bs.blocks["ng"].min())},
-            "hydro": {"reconstruction": p.recon, "riemann": p.riemann,
-                      "nscalars": p.bs.nscalars},
+            "hydro": hyd,
             "eos": {"gm1": float(p.eos["gm1"][0])}}
 
 
@@ -75,6 +80,10 @@ def small_cases():
                                                 integrator="vl2"),
         "rt_ppm_hllc_gravity_rk3": dataclasses.replace(problems.rt2d(nx=32, ny=96, nblk=(2, 4)),
                                                        integrator="rk3"),
+        # the ONe composition (NV = 12) and ONe + one level set (NV = 14)
+        "khcomp_plm_hllc_lset": problems.kh_comp(nx=(64, 32), block=(16, 16)),
+        "khcomp_ppm_hlle": problems.kh_comp(nx=(32, 32), block=(16, 16), nlset=0,
+                                            recon="ppm", riemann="hlle"),
     }
 
 
@@ -239,6 +248,12 @@ def test_unsupported_options_fail_loudly(gpu_ctx):
         hydro.FusedDriver(gpu_ctx, p.bs, p.cons, "dc", "none", p.eos, p.cfl)
     u = torch.from_numpy(p.cons.copy()).cuda()
     lam = torch.zeros_like(u)
+    # the adjoint covers NV = 5, 6; the composition counts run forward only
+    q = problems.kh_comp(nx=(32, 32), block=(16, 16))
+    fq = hydro.FusedDriver(gpu_ctx, q.bs, q.cons, q.recon, q.riemann, q.eos, q.cfl)
+    uq = torch.from_numpy(q.cons.copy()).cuda()
+    with pytest.raises(hydro._lib.HydroError):
+        adjoint.Stepper(fq, 1e-3).adjoint_step_(uq, torch.zeros_like(uq))
     for riemann, integ in (("lhllc", "rk2"), ("hllc", "rk3")):
         fd = hydro.FusedDriver(gpu_ctx, p.bs, p.cons, p.recon, riemann, p.eos, p.cfl,
                                integrator=integ)

@@ -55,6 +55,24 @@ def measured_traffic(name="k_fused_blast256_ppm_hllc.md"):
     return None
 
 
+def profiled_valu_per_cell(name="k_fused_blast256_ppm_hllc.md"):
+    """FP64 VALU lane-instructions per cell per stage of the fused kernel, from the
+    committed rocprofv3 SQ_INSTS_VALU summary (wave instructions x 64 / cells), or None."""
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        for line in open(path):
+            if line.startswith("| VALU lane-instructions per cell per stage |"):
+                return float(line.split("|")[2])
+    except (OSError, ValueError):
+        pass
+    return None
+
+
+# MI355X FP64 vector peak: 78.6 TFLOP/s counting an FMA as 2 -> 39.3e12 lane-instr/s
+# (256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz)
+VALU_PEAK_LANE_INSTR = 39.3e12
+
+
 def event_time_stage(fd, torch, reps=10):
     """Average duration (ms) of one fused stage launch, HIP events on fd's stream."""
     s = torch.cuda.current_stream()
@@ -339,6 +357,14 @@ def main():
             "traffic": measured_traffic(),
             "kernel": "k_fused<PPM,HLLC,NV=5,3D>", "avg_launch_ms": round(avg_ms, 4),
             "algorithmic_bytes_per_launch": sb}
+    vpc = profiled_valu_per_cell()
+    # the bound that applies: FP64 VALU issue, the profiled instructions per cell at the
+    # event-timed launch duration
+    valu_roof = None if vpc is None else {
+        "bound": "valu_fp64", "unit": "lane-instr/s",
+        "achieved": vpc * cells / (avg_ms * 1e-3), "peak": VALU_PEAK_LANE_INSTR,
+        "frac": round(vpc * cells / (avg_ms * 1e-3) / VALU_PEAK_LANE_INSTR, 3),
+        "lane_instr_per_cell_stage": vpc}
     print(f"# fused: {value:.4e} zone-cycles/s, stage {avg_ms:.3f} ms, "
           f"{achieved:.0f} GB/s", file=sys.stderr, flush=True)
     del fd
@@ -431,6 +457,7 @@ def main():
                    "form": "batched fused, hipGraph-replayed, device dt",
                    "tiles": tiles, "finite": finite},
         "roofline": roof,
+        "valu_roofline": valu_roof,
         "cpu_baseline": cpu,
         "extras": extras,
     }

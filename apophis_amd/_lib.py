@@ -52,6 +52,10 @@ _SIGS = {
     "aph_fill_ghosts": (_i32, [_vp, _vp, _i32, _i32, _vp]),
     "aph_estimate_timestep": (_i32, [_vp, _vp, _i32, _i32, _vp, _vp, _vp, _d, _vp]),
     "aph_global_sum": (_i32, [_vp, _vp, _vp, _i32, _vp]),
+    "aph_amr_plan_create": (_i32, [_vp, _vp, _vp, _i64, _vp, _i64, _vp, _i64, _pp]),
+    "aph_amr_plan_destroy": (_i32, [_vp, _vp]),
+    "aph_amr_fill": (_i32, [_vp, _vp, _vp]),
+    "aph_amr_flux_correct": (_i32, [_vp, _vp, _vp, _vp, _vp]),
     "aph_plan_create": (_i32, [_vp, _vp, _i32, _i32, _vp, _vp, _pp]),
     "aph_plan_destroy": (_i32, [_vp, _vp]),
     "aph_plan_set_tile": (_i32, [_vp, _vp, ctypes.POINTER(Tile)]),

@@ -1,0 +1,251 @@
+// aph_amr.hip -- multilevel meshes in the plain form: cross-level ghost values and flux
+// correction, executed from host-built plans.
+//
+// The reference leaves both to Parthenon (apophis_driver.cpp:68,74-76,94): the boundary
+// exchange of the cons field restricts fine data into coarse ghosts (RestrictAverage) and
+// prolongates coarse data into fine ghosts (ProlongateSharedMinMod), and the flux
+// correction replaces a coarse face flux by the area-weighted mean of the fine fluxes
+// across it.  Here the host resolves the mesh tree once into flat op lists; each op is one
+// ghost cell (or one coarse face) whose sources are a handful of interior cells (fine
+// faces), so both passes are a single gather kernel over the ops: every op reads cells
+// no op writes, and the result does not depend on the order the ops run in.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <set>
+#include <vector>
+
+#include "aph_device.hpp"
+#include "aph_runtime.hpp"
+
+struct aph_amr_plan {
+  aph_context *ctx = nullptr;
+  int32_t nvar = 0, ndim = 0;
+  int64_t nops = 0, nfixes = 0, nrefs = 0;
+  aph_amr_op *ops = nullptr;
+  aph_flux_fix *fixes = nullptr;
+  aph_amr_ref *refs = nullptr;
+};
+
+namespace aph {
+namespace {
+
+constexpr int kThreads = 256;
+
+template <int NV>
+__device__ __forceinline__ double amr_value(const aph_amr_ref *__restrict__ refs,
+                                            aph_amr_val v, int comp,
+                                            const double *__restrict__ u) {
+  aph_amr_ref r = refs[v.first];
+  double s = u[r.offset + comp * r.stride];
+  for (int k = 1; k < v.count; ++k) {
+    r = refs[v.first + k];
+    s += u[r.offset + comp * r.stride];
+  }
+  return v.count == 1 ? s : s * (1.0 / v.count);
+}
+
+__device__ __forceinline__ double sgn1(double x) { return x < 0.0 ? -1.0 : 1.0; }
+
+template <int NV>
+__global__ void __launch_bounds__(kThreads)
+    k_amr_fill(const aph_amr_op *__restrict__ ops, int64_t nops,
+               const aph_amr_ref *__restrict__ refs, int ndim, double *u) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nops;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const aph_amr_op op = ops[t];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      const double c = amr_value<NV>(refs, op.val[0], v, u);
+      double out = c;
+      if (op.kind == APH_AMR_PROLONG) {
+        for (int d = 0; d < ndim; ++d) {
+          const double a = c - amr_value<NV>(refs, op.val[1 + 2 * d], v, u);
+          const double b = amr_value<NV>(refs, op.val[2 + 2 * d], v, u) - c;
+          const double g = (0.5 * (sgn1(a) + sgn1(b))) * fmin(fabs(a), fabs(b));
+          out = out + (op.sign[d] * 0.25) * g;
+        }
+      }
+      u[op.dst + v * op.dst_stride] = out;
+    }
+  }
+}
+
+template <int NV>
+__global__ void __launch_bounds__(kThreads)
+    k_flux_fix(const aph_flux_fix *__restrict__ fixes, int64_t nfix,
+               const aph_amr_ref *__restrict__ refs, double *f1, double *f2, double *f3) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nfix;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const aph_flux_fix fx = fixes[t];
+    double *f = fx.dir == 0 ? f1 : (fx.dir == 1 ? f2 : f3);
+#pragma unroll
+    for (int v = 0; v < NV; ++v) f[fx.dst + v * fx.dst_stride] = amr_value<NV>(refs, fx.val, v, f);
+  }
+}
+
+// Host-side checks that keep every device access inside the fields.
+struct Locator {
+  const aph_blockset *bs;
+  std::vector<int64_t> base;  // nvar * cell_base per block
+  explicit Locator(const aph_blockset *b) : bs(b) {
+    for (const auto &k : b->h) base.push_back((int64_t)b->nvar * k.cell_base);
+  }
+  // block holding component-0 offset `off` with component stride `stride`, or -1
+  int find(int64_t off, int64_t stride) const {
+    auto it = std::upper_bound(base.begin(), base.end(), off);
+    if (it == base.begin()) return -1;
+    const int b = (int)(it - base.begin()) - 1;
+    const aph_block &k = bs->h[b];
+    int64_t nc = 1;
+    for (int d = 0; d < 3; ++d) nc *= (d < bs->ndim) ? k.nx[d] + 2 * k.ng : k.nx[d];
+    if (stride != nc || off - base[b] >= nc) return -1;
+    return b;
+  }
+  // whether the cell at `off` of block b is an interior cell
+  bool interior(int b, int64_t off) const {
+    const aph_block &k = bs->h[b];
+    int N[3], g[3];
+    for (int d = 0; d < 3; ++d) {
+      g[d] = (d < bs->ndim) ? k.ng : 0;
+      N[d] = k.nx[d] + 2 * g[d];
+    }
+    int64_t c = off - base[b];
+    const int i = (int)(c % N[0]);
+    c /= N[0];
+    const int j = (int)(c % N[1]);
+    const int kk = (int)(c / N[1]);
+    const int id[3] = {i, j, kk};
+    for (int d = 0; d < 3; ++d)
+      if (id[d] < g[d] || id[d] >= g[d] + k.nx[d]) return false;
+    return true;
+  }
+};
+
+}  // namespace
+}  // namespace aph
+
+using namespace aph;
+
+extern "C" {
+
+int aph_amr_plan_create(aph_context *ctx, const aph_blockset *bs, const aph_amr_op *ops,
+                        int64_t nops, const aph_flux_fix *fixes, int64_t nfixes,
+                        const aph_amr_ref *refs, int64_t nrefs, aph_amr_plan **out) {
+  if (!ctx) return APH_E_INVALID;
+  if (!bs || !out || nops < 0 || nfixes < 0 || nrefs < 0 || (nops && !ops) ||
+      (nfixes && !fixes) || (nrefs && !refs))
+    return set_error(ctx, APH_E_INVALID, "aph_amr_plan_create: bad arguments");
+  if (bs->ndim < 2) return set_error(ctx, APH_E_INVALID, "aph_amr_plan_create: ndim must be 2 or 3");
+  const Locator loc(bs);
+  auto check_val = [&](const aph_amr_val &v, bool want_interior, int *blk) -> bool {
+    if (v.count < 1 || v.count > 8 || v.first < 0 || (int64_t)v.first + v.count > nrefs)
+      return false;
+    for (int k = 0; k < v.count; ++k) {
+      const aph_amr_ref &r = refs[v.first + k];
+      const int b = loc.find(r.offset, r.stride);
+      if (b < 0) return false;
+      if (want_interior && !loc.interior(b, r.offset)) return false;
+      if (blk) *blk = b;
+    }
+    return true;
+  };
+  for (int64_t t = 0; t < nops; ++t) {
+    const aph_amr_op &op = ops[t];
+    const int b = loc.find(op.dst, op.dst_stride);
+    if (b < 0 || loc.interior(b, op.dst))
+      return set_error(ctx, APH_E_INVALID, "amr op %lld: destination is not a ghost cell",
+                       (long long)t);
+    if (op.kind != APH_AMR_MEAN && op.kind != APH_AMR_PROLONG)
+      return set_error(ctx, APH_E_INVALID, "amr op %lld: unknown kind %d", (long long)t, op.kind);
+    const int nval = op.kind == APH_AMR_MEAN ? 1 : 1 + 2 * bs->ndim;
+    for (int k = 0; k < nval; ++k)
+      if (!check_val(op.val[k], true, nullptr))
+        return set_error(ctx, APH_E_INVALID, "amr op %lld: value %d does not reference "
+                         "interior cells", (long long)t, k);
+    if (op.kind == APH_AMR_PROLONG)
+      for (int d = 0; d < bs->ndim; ++d)
+        if (op.sign[d] != -1 && op.sign[d] != 1)
+          return set_error(ctx, APH_E_INVALID, "amr op %lld: sign must be +-1", (long long)t);
+  }
+  std::set<int> dst_blocks, src_blocks;
+  for (int64_t t = 0; t < nfixes; ++t) {
+    const aph_flux_fix &fx = fixes[t];
+    const int b = loc.find(fx.dst, fx.dst_stride);
+    int sb = -1;
+    if (b < 0 || fx.dir < 0 || fx.dir >= bs->ndim || !check_val(fx.val, false, &sb))
+      return set_error(ctx, APH_E_INVALID, "flux fix %lld: bad face references", (long long)t);
+    dst_blocks.insert(b);
+    for (int k = 0; k < fx.val.count; ++k)
+      src_blocks.insert(loc.find(refs[fx.val.first + k].offset, refs[fx.val.first + k].stride));
+  }
+  for (int b : dst_blocks)
+    if (src_blocks.count(b))
+      return set_error(ctx, APH_E_INVALID, "flux fix: block %d is both corrected and a source", b);
+  aph_amr_plan *p = new (std::nothrow) aph_amr_plan();
+  if (!p) return set_error(ctx, APH_E_NOMEM, "host allocation failed");
+  p->ctx = ctx;
+  p->nvar = bs->nvar;
+  p->ndim = bs->ndim;
+  p->nops = nops;
+  p->nfixes = nfixes;
+  p->nrefs = nrefs;
+  auto up = [&](void **dst, const void *src, size_t bytes) -> bool {
+    if (!bytes) return true;
+    if (hipMalloc(dst, bytes) != hipSuccess) return false;
+    return hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
+  };
+  if (!up((void **)&p->ops, ops, sizeof(aph_amr_op) * nops) ||
+      !up((void **)&p->fixes, fixes, sizeof(aph_flux_fix) * nfixes) ||
+      !up((void **)&p->refs, refs, sizeof(aph_amr_ref) * nrefs)) {
+    aph_amr_plan_destroy(ctx, p);
+    return set_error(ctx, APH_E_NOMEM, "amr plan device allocation failed");
+  }
+  *out = p;
+  return APH_OK;
+}
+
+int aph_amr_plan_destroy(aph_context *ctx, aph_amr_plan *p) {
+  if (!ctx) return APH_E_INVALID;
+  if (!p) return APH_OK;
+  if (p->ops) (void)hipFree(p->ops);
+  if (p->fixes) (void)hipFree(p->fixes);
+  if (p->refs) (void)hipFree(p->refs);
+  delete p;
+  return APH_OK;
+}
+
+#define APH_AMR_NV(nv, CALL)                                                          \
+  switch (nv) {                                                                       \
+    case 5: { constexpr int NV = 5; CALL; } break;                                    \
+    case 6: { constexpr int NV = 6; CALL; } break;                                    \
+    case 12: { constexpr int NV = 12; CALL; } break;                                  \
+    case 14: { constexpr int NV = 14; CALL; } break;                                  \
+    default: return set_error(ctx, APH_E_INVALID, "unsupported nvar %d", nv);         \
+  }
+
+int aph_amr_fill(aph_context *ctx, const aph_amr_plan *p, double *cons) {
+  if (!ctx) return APH_E_INVALID;
+  if (!p || !cons) return set_error(ctx, APH_E_INVALID, "aph_amr_fill: null");
+  if (!p->nops) return APH_OK;
+  const unsigned grid = (unsigned)std::min<int64_t>((p->nops + kThreads - 1) / kThreads, 4096);
+  APH_AMR_NV(p->nvar, (k_amr_fill<NV><<<grid, kThreads, 0, ctx->stream>>>(
+                          p->ops, p->nops, p->refs, p->ndim, cons)));
+  APH_LAUNCH_CHECK(ctx);
+  return APH_OK;
+}
+
+int aph_amr_flux_correct(aph_context *ctx, const aph_amr_plan *p, double *f1, double *f2,
+                         double *f3) {
+  if (!ctx) return APH_E_INVALID;
+  if (!p || !f1 || (p->ndim >= 2 && !f2) || (p->ndim >= 3 && !f3))
+    return set_error(ctx, APH_E_INVALID, "aph_amr_flux_correct: null");
+  if (!p->nfixes) return APH_OK;
+  const unsigned grid = (unsigned)std::min<int64_t>((p->nfixes + kThreads - 1) / kThreads, 4096);
+  APH_AMR_NV(p->nvar, (k_flux_fix<NV><<<grid, kThreads, 0, ctx->stream>>>(
+                          p->fixes, p->nfixes, p->refs, f1, f2, f3)));
+  APH_LAUNCH_CHECK(ctx);
+  return APH_OK;
+}
+
+}  // extern "C"

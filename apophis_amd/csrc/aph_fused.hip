@@ -379,6 +379,12 @@ int aph_plan_create(aph_context *ctx, const aph_blockset *bs, int32_t recon, int
                      rsolver, bs->nvar);
   const int need = recon == APH_RECON_PPM ? 3 : 2;
   for (int b = 0; b < bs->nblocks; ++b)
+    for (int f = 0; f < 2 * bs->ndim; ++f)
+      if (bs->h[b].bc[f] == APH_BC_AMR)
+        return set_error(ctx, APH_E_INVALID,
+                         "fused form: block %d has a face to another refinement level "
+                         "(multilevel meshes run in the plain form)", b);
+  for (int b = 0; b < bs->nblocks; ++b)
     for (int d = 0; d < bs->ndim; ++d)
       if (bs->h[b].nx[d] < need)
         return set_error(ctx, APH_E_INVALID,

@@ -223,6 +223,7 @@ __global__ void __launch_bounds__(kThreads)
     if (side ? !outer : !inner) continue;
     if (side) idx[D] = g + bk.nx[D] + (idx[D] - g);
     const int type = bk.bc[2 * D + (side ? 1 : 0)];
+    if (type == APH_BC_AMR) continue;  // set from the other level (aph_amr_fill)
     const int ref = side ? g + bk.nx[D] - 1 : g;
     const int offset = 2 * ref + (side ? 1 : -1);
     int s[3] = {idx[0], idx[1], idx[2]};

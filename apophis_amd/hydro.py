@@ -330,9 +330,10 @@ class ApophisDriver:
     pack_size: blocks per partition (None = all blocks in one pack, the default
     Parthenon partitioning); each task is issued once per partition."""
 
-    def __init__(self, pkg, md, g=(0.0, 0.0, 0.0), pack_size=None):
+    def __init__(self, pkg, md, g=(0.0, 0.0, 0.0), pack_size=None, amr=None):
         self.pkg = pkg
         self.md = md
+        self.amr = amr  # amr.AmrPlan of a two-level mesh, or None
         self.eos = pkg.param("eos")
         self.cfl = pkg.param("cfl")
         self.g = tuple(float(v) for v in g)
@@ -347,6 +348,8 @@ class ApophisDriver:
         """Ghost fill + FillDerived after the problem generator (PostInitialization)."""
         for p in self.parts:
             fill_ghosts(p)
+        if self.amr is not None:
+            self.amr.fill(self.md.cons)
         for p in self.parts:
             cons_to_prim(p, self.eos)
         return self.new_dt()
@@ -367,12 +370,17 @@ class ApophisDriver:
         rk = self.rk
         for p in self.parts:
             flux_fun(p)                                                      # :70-72
+        if self.amr is not None:
+            self.amr.flux_correct(self.md.flux)                              # :74-76
+        for p in self.parts:
             update_with_flux_divergence(p, self.u1, rk["gam0"][s], rk["gam1"][s],
                                         rk["beta"][s] * dt)                  # :79-82
             if grav:
                 apply_gravity(p, self.g, rk["beta"][s] * dt)                 # :84-85
         for p in self.parts:
             fill_ghosts(p)                                                   # :94, :103
+        if self.amr is not None:
+            self.amr.fill(self.md.cons)                                      # :94
         for p in self.parts:
             cons_to_prim(p, self.eos)                                        # :110-111
         if stage == rk["nstages"]:

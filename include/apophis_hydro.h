@@ -57,7 +57,9 @@ enum aph_integrator { APH_INTEGRATOR_RK1 = 1, APH_INTEGRATOR_RK2 = 2, APH_INTEGR
                       APH_INTEGRATOR_VL2 = 4 };
 /* Physical boundary applied on a face whose neighbour entry is -1.
    Periodicity is expressed through the neighbour table instead. */
-enum aph_bc { APH_BC_OUTFLOW = 1, APH_BC_REFLECT = 2 };
+/* APH_BC_AMR: the region beyond this face belongs to blocks of another refinement level;
+   aph_fill_ghosts leaves its ghost cells to aph_amr_fill (multilevel meshes below). */
+enum aph_bc { APH_BC_OUTFLOW = 1, APH_BC_REFLECT = 2, APH_BC_AMR = 3 };
 
 /* One meshblock.  240 bytes, no implicit padding (numpy mirrors it field by field). */
 typedef struct aph_block {
@@ -154,6 +156,68 @@ int aph_estimate_timestep(aph_context *ctx, const aph_blockset *bs, int32_t firs
 /* GlobalQuantHst<Sum, var> (hydro.cpp:746-771): *out (device) = sum(cons[var]*vol). */
 int aph_global_sum(aph_context *ctx, const aph_blockset *bs, const double *cons,
                    int32_t var, double *out);
+
+/* ------------------------------------------- multilevel meshes (plain form)
+ * Blocks of two refinement levels in one mesh (a 2:1 jump).  nbr[] links blocks of the
+ * same level only; a face whose neighbour region belongs to the other level has nbr = -1
+ * and bc = APH_BC_AMR.  The ghost cells there, and the coarse fluxes on faces shared
+ * with finer blocks, come from a plan the host builds from the mesh tree.  This replaces
+ * Parthenon's multilevel boundary exchange (RestrictAverage / ProlongateSharedMinMod
+ * for the cons field) and its flux correction (LoadAndSendFluxCorrections /
+ * SetFluxCorrections), the tasks at apophis_driver.cpp:68,74-76 and :94.
+ *
+ * A cell is an aph_amr_ref: its component v is at element offset + v * stride (stride =
+ * its block's ncell).  A value is an aph_amr_val: the mean of refs[first .. first+count)
+ * (count 1: a copy; 2^ndim: the restriction of a coarse cell's children), summed in
+ * order and scaled by 1/count. */
+typedef struct aph_amr_ref {
+  int64_t offset;
+  int64_t stride;
+} aph_amr_ref;
+typedef struct aph_amr_val {
+  int32_t first;
+  int32_t count;
+} aph_amr_val;
+enum aph_amr_kind { APH_AMR_MEAN = 0, APH_AMR_PROLONG = 1 };
+/* One ghost cell of cons (all components):
+ *   APH_AMR_MEAN:    value(val[0]);
+ *   APH_AMR_PROLONG: c = value(val[0]); for each active axis d in order,
+ *                    g_d = minmod(c - value(val[1+2d]), value(val[2+2d]) - c),
+ *                    minmod(a, b) = (SIGN(a) + SIGN(b))/2 * min(|a|, |b|), SIGN(0) = 1;
+ *                    result = (((c + sign[0]/4 g_0) + sign[1]/4 g_1) + sign[2]/4 g_2),
+ *                    sign[d] = -1 / +1 for the lower / upper child of the coarse cell. */
+typedef struct aph_amr_op {
+  int64_t dst;        /* component-0 element offset of the ghost cell   */
+  int64_t dst_stride; /* its block's ncell                               */
+  int32_t kind;
+  int32_t sign[3];
+  aph_amr_val val[7];
+} aph_amr_op;
+/* One coarse face of flux array `dir` (0: flux_x1, ...): all components set to the mean
+   of the fine faces val refers to (offsets into the same flux array). */
+typedef struct aph_flux_fix {
+  int64_t dst;
+  int64_t dst_stride;
+  int32_t dir;
+  int32_t reserved;
+  aph_amr_val val;
+} aph_flux_fix;
+typedef struct aph_amr_plan aph_amr_plan;
+
+/* Validates every op, fix and ref against the block set (offsets inside the fields,
+   value ranges inside refs, at most 8 cells per value) and copies them to the device.
+   Synchronous. */
+int aph_amr_plan_create(aph_context *ctx, const aph_blockset *bs, const aph_amr_op *ops,
+                        int64_t nops, const aph_flux_fix *fixes, int64_t nfixes,
+                        const aph_amr_ref *refs, int64_t nrefs, aph_amr_plan **out);
+int aph_amr_plan_destroy(aph_context *ctx, aph_amr_plan *plan);
+/* Cross-level ghost values of cons; run after aph_fill_ghosts.  Reads interior cells
+   only, writes ghost cells only. */
+int aph_amr_fill(aph_context *ctx, const aph_amr_plan *plan, double *cons);
+/* Flux correction before the update: every coarse face shared with finer blocks takes
+   the mean of the fine fluxes across it.  flux_x2 / flux_x3 may be NULL when unused. */
+int aph_amr_flux_correct(aph_context *ctx, const aph_amr_plan *plan, double *flux_x1,
+                         double *flux_x2, double *flux_x3);
 
 /* ----------------------------------------------------- batched fused form
  * A plan binds a block set to one (recon, rsolver) pair and a tile shape per block

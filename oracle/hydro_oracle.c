@@ -617,6 +617,7 @@ void orc_fill_ghosts(const aph_block *blocks, int nb, int ndim, int nvar, double
         const int face = 13 + (side ? sd : -sd);
         if (bk->nbr[face] >= 0) continue;
         const int type = bk->bc[2 * d + side];
+        if (type == APH_BC_AMR) continue; /* filled from the other level */
         const int ref = side ? g[d] + bk->nx[d] - 1 : g[d];
         const int offset = 2 * ref + (side ? 1 : -1);
         const int glo = side ? g[d] + bk->nx[d] : 0;

@@ -1,0 +1,97 @@
+"""Two-level meshes on the GPU through the C ABI: aph_amr_fill / aph_amr_flux_correct
+against the oracle's whole-level restatement (oracle/amr_oracle.py) bitwise, whole plain
+steps with the flux correction against the oracle's step bitwise, conservation, and the
+error paths (a corrupt plan, the fused form on a multilevel set)."""
+import numpy as np
+import pytest
+import torch
+
+from apophis_amd import amr, hydro
+from oracle import amr_oracle as ao
+from oracle import oracle as orc
+from tests.test_amr import MESHES, _mesh, _problem, _totals
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("name", list(MESHES))
+def test_amr_fill_and_flux_correct_match_oracle(gpu_ctx, name):
+    print(f"[case] amr fill {name}", flush=True)
+    m = _mesh(name)
+    bs = m.block_set()
+    u = np.random.default_rng(5).normal(size=bs.field_size(5))
+    md = hydro.MeshData(gpu_ctx, bs, u)
+    plan = amr.AmrPlan(gpu_ctx, md.dbs, m, bs)
+    hydro.fill_ghosts(md)
+    plan.fill(md.cons)
+    want = u.copy()
+    orc.fill_ghosts(bs, want)
+    ao.amr_fill(m, bs, want)
+    torch.cuda.synchronize()
+    assert np.array_equal(md.cons.cpu().numpy(), want)
+    fx = [np.random.default_rng(9 + d).normal(size=bs.field_size(5)) for d in range(3)]
+    for d in range(3):
+        if md.flux[d] is not None:
+            md.flux[d].copy_(torch.from_numpy(fx[d]))
+    plan.flux_correct(md.flux)
+    ao.flux_correct(m, bs, fx)
+    torch.cuda.synchronize()
+    for d in range(m.ndim):
+        assert np.array_equal(md.flux[d].cpu().numpy(), fx[d]), d
+
+
+@pytest.mark.parametrize("name,recon,riemann", [("2d_L", "plm", "hllc"),
+                                                ("2d_ppm", "ppm", "hllc"),
+                                                ("3d", "plm", "hlle")])
+def test_plain_amr_steps_match_oracle(gpu_ctx, name, recon, riemann):
+    print(f"[case] amr steps {name} {recon} {riemann}", flush=True)
+    m = _mesh(name)
+    p = _problem(m, recon=recon)
+    p.riemann = riemann
+    ref = ao.AmrState(p, m)
+    pkg = hydro.initialize_hydro({
+        "parthenon/time": {"cfl": p.cfl, "integrator": "rk2"},
+        "parthenon/mesh": {"nghost": m.ng},
+        "hydro": {"reconstruction": recon, "riemann": riemann},
+        "eos": {"gm1": float(p.eos["gm1"][0])}})
+    md = hydro.MeshData(gpu_ctx, p.bs, p.cons)
+    plan = amr.AmrPlan(gpu_ctx, md.dbs, m, p.bs)
+    drv = hydro.ApophisDriver(pkg, md, amr=plan)
+    dt = drv.initialize()
+    assert dt == ref.estimate_dt()
+    t0 = _totals(p.bs, md.cons.cpu().numpy())
+    for _ in range(2):
+        est = drv.step(dt)
+        est_ref = ref.step(dt)
+        assert est == est_ref
+    torch.cuda.synchronize()
+    got = md.cons.cpu().numpy()
+    assert np.array_equal(got, ref.u0)
+    t1 = _totals(p.bs, got)
+    assert np.all(np.abs(t1 - t0) <= 1e-13 * (np.abs(t0) + 1.0))
+
+
+def test_amr_error_paths(gpu_ctx):
+    m = _mesh("2d_L")
+    bs = m.block_set()
+    md = hydro.MeshData(gpu_ctx, bs, np.ones(bs.field_size(5)))
+    ops, fixes, refs = amr.amr_plan(m, bs)
+    bad = ops.copy()
+    bad["dst"][0] = refs["offset"][0]  # an interior cell
+    import ctypes
+    from apophis_amd import _lib
+    h = ctypes.c_void_p()
+    vp = ctypes.c_void_p
+    rc = _lib.lib().aph_amr_plan_create(gpu_ctx.h, md.dbs, bad.ctypes.data_as(vp), len(bad),
+                                        fixes.ctypes.data_as(vp), len(fixes),
+                                        refs.ctypes.data_as(vp), len(refs), ctypes.byref(h))
+    assert rc != 0 and not h.value
+    bad = ops.copy()
+    bad["val"]["count"][0, 0] = 9
+    rc = _lib.lib().aph_amr_plan_create(gpu_ctx.h, md.dbs, bad.ctypes.data_as(vp), len(bad),
+                                        fixes.ctypes.data_as(vp), len(fixes),
+                                        refs.ctypes.data_as(vp), len(refs), ctypes.byref(h))
+    assert rc != 0
+    p = _problem(m)
+    with pytest.raises(hydro._lib.HydroError, match="refinement level"):
+        hydro.FusedDriver(gpu_ctx, p.bs, p.cons, "plm", "hllc", p.eos, p.cfl)

@@ -252,6 +252,59 @@ def amr_plan(mesh: TwoLevelMesh, bs: BlockSet):
             np.array(refs.rows, dtype=AMR_REF_DTYPE))
 
 
+AMR_FACE_DTYPE = np.dtype([("base", "<i8"), ("role", "<i4"), ("qa", "<i4"), ("qb", "<i4"),
+                           ("na", "<i4")])
+assert AMR_FACE_DTYPE.itemsize == 24
+
+
+def fused_faces(mesh: TwoLevelMesh, bs: BlockSet):
+    """The fused form's interface table (aph_plan_set_amr): (faces [nblocks*6], level,
+    buffer length).  Each coarse face to the fine level owns a region of the buffer with
+    one slot per (coarse face cell, child) of nvar doubles, transverse axes ascending and
+    the first slowest; each fine face there points into that region with its offset in
+    fine cells."""
+    nd, nx, nv = mesh.ndim, mesh.nx, bs.nvar
+    nch = 2 ** (nd - 1)
+    faces = np.zeros(bs.nblocks * 6, dtype=AMR_FACE_DTYPE)
+    region = {}
+    total = 0
+    trans = lambda d: ((1 if d == 0 else 0), (1 if d == 2 else 2))
+    for b in range(bs.nblocks):
+        if mesh.level[b] != 0:
+            continue
+        for d in range(nd):
+            A, B = trans(d)
+            for side in (0, 1):
+                if bs.blocks["bc"][b][2 * d + side] != BC_AMR:
+                    continue
+                region[(b, d, side)] = total
+                f = faces[b * 6 + 2 * d + side]
+                f["base"], f["role"], f["na"] = total, 2, nx[A]
+                total += nx[A] * (nx[B] if B < nd else 1) * nch * nv
+    for b in range(bs.nblocks):
+        if mesh.level[b] != 1:
+            continue
+        lf = mesh.loc[b]
+        for d in range(nd):
+            A, B = trans(d)
+            for side in (0, 1):
+                if bs.blocks["bc"][b][2 * d + side] != BC_AMR:
+                    continue
+                # the coarse cell across the face, and the coarse block holding it
+                G = [int(lf[t]) * nx[t] for t in range(3)]
+                G[d] = int(lf[d]) * nx[d] + (nx[d] if side else -1)
+                I = [(G[t] % mesh.ncells_axis(1, t)) // 2 if t < nd else 0 for t in range(3)]
+                lc = tuple(I[t] // nx[t] if t < nd else 0 for t in range(3))
+                cb = mesh.index[(0, lc)]
+                f = faces[b * 6 + 2 * d + side]
+                f["base"] = region[(cb, d, 1 - side)]
+                f["role"] = 1
+                f["na"] = nx[A]
+                f["qa"] = int(lf[A]) * nx[A] - 2 * lc[A] * nx[A]
+                f["qb"] = (int(lf[B]) * nx[B] - 2 * lc[B] * nx[B]) if B < nd else 0
+    return faces, mesh.level.astype(np.int32), total
+
+
 class AmrPlan:
     """The plan on the device (aph_amr_plan_create)."""
 

@@ -77,6 +77,14 @@ struct aph_plan {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   aph::AdjScratch adj;
+  // two-level mesh (aph_plan_set_amr): items [0, nitems1) are the level-1 blocks, run
+  // before the level-0 blocks [nitems1, nitems) so the coarse side finds the fine fluxes
+  const aph_amr_plan *amr = nullptr;
+  aph_amr_face *d_amr_face = nullptr;
+  double *d_amr_buf = nullptr;
+  std::vector<int32_t> level;
+  int nitems1 = 0;
+  bool amr_faces = false;  // the block set has faces to another level: needs set_amr
 };
 
 namespace {
@@ -197,31 +205,40 @@ int build_items(aph_context *ctx, aph_plan *p) {
   }
   // per XCD lists, greedy by cost (cells) so every L2 sees whole blocks
   constexpr int kXcd = 8;
-  std::vector<std::vector<int4>> lists(kXcd);
-  std::vector<int64_t> load(kXcd, 0);
   size_t maxcells = 0;
-  for (int b = 0; b < bs->nblocks; ++b) {
-    if (p->only_class >= 0 && cls_of[b] != p->only_class) continue;
-    const aph_block &k = bs->h[b];
-    const aph_tile &t = tl[cls_of[b]];
-    const int tt[3] = {t.tx, t.ty, t.tz};
-    for (int d = 0; d < 3; ++d)
-      if (tt[d] < 1 || k.nx[d] % tt[d] != 0)
-        return set_error(ctx, APH_E_INVALID, "tile %dx%dx%d does not divide block %d (%dx%dx%d)",
-                         t.tx, t.ty, t.tz, b, k.nx[0], k.nx[1], k.nx[2]);
-    maxcells = std::max(maxcells, (size_t)t.tx * t.ty * t.tz);
-    int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-    for (int z = 0; z < k.nx[2]; z += t.tz)
-      for (int y = 0; y < k.nx[1]; y += t.ty)
-        for (int xx = 0; xx < k.nx[0]; xx += t.tx)
-          lists[x].push_back(make_int4(b | (cls_of[b] << 24), xx, y, z));
-    load[x] += (int64_t)k.nx[0] * k.nx[1] * k.nx[2];
+  std::vector<int4> items;
+  // one phase without a multilevel mesh; with one, the level-1 blocks, then level 0
+  const bool two = !p->level.empty();
+  for (int phase = 0; phase < (two ? 2 : 1); ++phase) {
+    std::vector<std::vector<int4>> lists(kXcd);
+    std::vector<int64_t> load(kXcd, 0);
+    for (int b = 0; b < bs->nblocks; ++b) {
+      if (p->only_class >= 0 && cls_of[b] != p->only_class) continue;
+      if (two && p->level[b] != 1 - phase) continue;
+      const aph_block &k = bs->h[b];
+      const aph_tile &t = tl[cls_of[b]];
+      const int tt[3] = {t.tx, t.ty, t.tz};
+      for (int d = 0; d < 3; ++d)
+        if (tt[d] < 1 || k.nx[d] % tt[d] != 0)
+          return set_error(ctx, APH_E_INVALID,
+                           "tile %dx%dx%d does not divide block %d (%dx%dx%d)", t.tx, t.ty,
+                           t.tz, b, k.nx[0], k.nx[1], k.nx[2]);
+      maxcells = std::max(maxcells, (size_t)t.tx * t.ty * t.tz);
+      int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      for (int z = 0; z < k.nx[2]; z += t.tz)
+        for (int y = 0; y < k.nx[1]; y += t.ty)
+          for (int xx = 0; xx < k.nx[0]; xx += t.tx)
+            lists[x].push_back(make_int4(b | (cls_of[b] << 24), xx, y, z));
+      load[x] += (int64_t)k.nx[0] * k.nx[1] * k.nx[2];
+    }
+    size_t maxlen = 0;
+    for (auto &l : lists) maxlen = std::max(maxlen, l.size());
+    const size_t at = items.size();
+    items.resize(at + maxlen * kXcd, make_int4(-1, 0, 0, 0));
+    for (int x = 0; x < kXcd; ++x)
+      for (size_t r = 0; r < lists[x].size(); ++r) items[at + r * kXcd + x] = lists[x][r];
+    if (phase == 0) p->nitems1 = (int)items.size();
   }
-  size_t maxlen = 0;
-  for (auto &l : lists) maxlen = std::max(maxlen, l.size());
-  std::vector<int4> items(maxlen * kXcd, make_int4(-1, 0, 0, 0));
-  for (int x = 0; x < kXcd; ++x)
-    for (size_t r = 0; r < lists[x].size(); ++r) items[r * kXcd + x] = lists[x][r];
   if (p->d_items) (void)hipFree(p->d_items);
   p->d_items = nullptr;
   p->nitems = (int)items.size();
@@ -284,6 +301,8 @@ FusedArgs make_args(aph_plan *p, int stage, int nstages, double gam0, double gam
   a.cfl = cfl;
   a.has_un = stage > 1;
   a.last = stage == nstages;
+  a.amr_face = p->d_amr_face;
+  a.amr_buf = p->d_amr_buf;
   return a;
 }
 
@@ -295,11 +314,30 @@ int launch_kernel(aph_context *ctx, aph_plan *p, int recon, int stage, int nst, 
     if (rc) return rc;
   }
   if (p->nitems == 0) return APH_OK;
+  if (p->amr_faces && !p->amr)
+    return set_error(ctx, APH_E_INVALID,
+                     "fused form: the block set has faces to another refinement level; "
+                     "call aph_plan_set_amr first");
   KernelFn fn = pick_kernel(recon, p->rsolver, p->bs->nvar, p->bs->ndim);
   if (!fn) return set_error(ctx, APH_E_INVALID, "no fused kernel for this stage");
   FusedArgs a = make_args(p, stage, nst, gam0, gam1, beta, u_in, u_n, u_out, dt_dev, cfl);
-  hipLaunchKernelGGL(fn, dim3(p->grid), dim3(64), p->lds, ctx->stream, a);
-  APH_LAUNCH_CHECK(ctx);
+  if (!p->amr) {
+    hipLaunchKernelGGL(fn, dim3(p->grid), dim3(64), p->lds, ctx->stream, a);
+    APH_LAUNCH_CHECK(ctx);
+    return APH_OK;
+  }
+  // two-level mesh: cross-level ghosts of the stage input, then the fine blocks (which
+  // record their interface fluxes), then the coarse blocks (which read them)
+  int rc = aph_amr_fill(ctx, p->amr, u_in);
+  if (rc) return rc;
+  const int n[2] = {p->nitems1, p->nitems - p->nitems1};
+  for (int ph = 0; ph < 2; ++ph) {
+    if (!n[ph]) continue;
+    a.items = p->d_items + (ph ? p->nitems1 : 0);
+    a.nitems = n[ph];
+    hipLaunchKernelGGL(fn, dim3(p->grid), dim3(64), p->lds, ctx->stream, a);
+    APH_LAUNCH_CHECK(ctx);
+  }
   return APH_OK;
 }
 
@@ -378,12 +416,10 @@ int aph_plan_create(aph_context *ctx, const aph_blockset *bs, int32_t recon, int
                      "fused form: unsupported (reconstruction=%d, riemann=%d, nvar=%d)", recon,
                      rsolver, bs->nvar);
   const int need = recon == APH_RECON_PPM ? 3 : 2;
+  bool amr_faces = false;
   for (int b = 0; b < bs->nblocks; ++b)
     for (int f = 0; f < 2 * bs->ndim; ++f)
-      if (bs->h[b].bc[f] == APH_BC_AMR)
-        return set_error(ctx, APH_E_INVALID,
-                         "fused form: block %d has a face to another refinement level "
-                         "(multilevel meshes run in the plain form)", b);
+      if (bs->h[b].bc[f] == APH_BC_AMR) amr_faces = true;
   for (int b = 0; b < bs->nblocks; ++b)
     for (int d = 0; d < bs->ndim; ++d)
       if (bs->h[b].nx[d] < need)
@@ -393,6 +429,7 @@ int aph_plan_create(aph_context *ctx, const aph_blockset *bs, int32_t recon, int
   aph_plan *p = new (std::nothrow) aph_plan();
   if (!p) return set_error(ctx, APH_E_NOMEM, "host allocation failed");
   p->bs = bs;
+  p->amr_faces = amr_faces;
   p->recon = recon;
   p->rsolver = rsolver;
   p->eos = *eos;
@@ -423,8 +460,72 @@ int aph_plan_destroy(aph_context *ctx, aph_plan *p) {
   if (p->d_items) (void)hipFree(p->d_items);
   if (p->d_est) (void)hipFree(p->d_est);
   if (p->d_u2) (void)hipFree(p->d_u2);
+  if (p->d_amr_face) (void)hipFree(p->d_amr_face);
+  if (p->d_amr_buf) (void)hipFree(p->d_amr_buf);
   adj_release(p->adj);
   delete p;
+  return APH_OK;
+}
+
+int aph_plan_set_amr(aph_context *ctx, aph_plan *p, const aph_amr_plan *amr,
+                     const aph_amr_face *faces, const int32_t *level,
+                     int64_t buffer_doubles) {
+  if (!ctx) return APH_E_INVALID;
+  if (!p || !amr || !faces || !level || buffer_doubles < 0)
+    return set_error(ctx, APH_E_INVALID, "aph_plan_set_amr: bad arguments");
+  const aph_blockset *bs = p->bs;
+  const int nd = bs->ndim, nv = bs->nvar;
+  if (nd < 2) return set_error(ctx, APH_E_INVALID, "aph_plan_set_amr: ndim must be 2 or 3");
+  const int64_t nch = (nd == 3) ? 4 : 2;
+  for (int b = 0; b < bs->nblocks; ++b) {
+    if (level[b] != 0 && level[b] != 1)
+      return set_error(ctx, APH_E_INVALID, "aph_plan_set_amr: block %d level %d", b, level[b]);
+    const aph_block &k = bs->h[b];
+    for (int f = 0; f < 6; ++f) {
+      const aph_amr_face &af = faces[b * 6 + f];
+      const int d = f / 2;
+      const bool amr_face = d < nd && k.bc[f] == APH_BC_AMR;
+      if (!amr_face) {
+        if (af.role != 0)
+          return set_error(ctx, APH_E_INVALID, "block %d face %d: role on a non-AMR face", b, f);
+        continue;
+      }
+      if (af.role != (level[b] == 1 ? 1 : 2))
+        return set_error(ctx, APH_E_INVALID, "block %d face %d: role %d does not match level %d",
+                         b, f, af.role, level[b]);
+      const int A = (d == 0) ? 1 : 0, B = (d == 2) ? 1 : 2;
+      const int64_t n_a = k.nx[A], n_b = (B < nd) ? k.nx[B] : 1;
+      int64_t last;  // last element the face touches
+      if (af.role == 1) {
+        if (af.qa < 0 || af.qb < 0 || af.na < 1 || (af.qa + n_a + 1) / 2 > af.na)
+          return set_error(ctx, APH_E_INVALID, "block %d face %d: bad fine offsets", b, f);
+        last = af.base + (((af.qb + n_b - 1) / 2) * af.na + (af.qa + n_a - 1) / 2) * nch * nv +
+               nch * nv - 1;
+      } else {
+        if (af.na != n_a || af.qa != 0 || af.qb != 0)
+          return set_error(ctx, APH_E_INVALID, "block %d face %d: bad coarse region", b, f);
+        last = af.base + ((n_b - 1) * af.na + (n_a - 1)) * nch * nv + nch * nv - 1;
+      }
+      if (af.base < 0 || last >= buffer_doubles)
+        return set_error(ctx, APH_E_INVALID, "block %d face %d: outside the interface buffer",
+                         b, f);
+    }
+  }
+  if (p->d_amr_face) (void)hipFree(p->d_amr_face);
+  if (p->d_amr_buf) (void)hipFree(p->d_amr_buf);
+  p->d_amr_face = nullptr;
+  p->d_amr_buf = nullptr;
+  if (hipMalloc(&p->d_amr_face, sizeof(aph_amr_face) * 6 * bs->nblocks) != hipSuccess ||
+      hipMalloc(&p->d_amr_buf, sizeof(double) * std::max<int64_t>(buffer_doubles, 1)) !=
+          hipSuccess)
+    return set_error(ctx, APH_E_NOMEM, "aph_plan_set_amr: device allocation failed");
+  APH_HIP_CHECK(ctx, hipMemcpy(p->d_amr_face, faces, sizeof(aph_amr_face) * 6 * bs->nblocks,
+                               hipMemcpyHostToDevice));
+  APH_HIP_CHECK(ctx, hipMemset(p->d_amr_buf, 0, sizeof(double) * std::max<int64_t>(buffer_doubles, 1)));
+  p->amr = amr;
+  p->level.assign(level, level + bs->nblocks);
+  p->dirty = true;
+  drop_graph(p);
   return APH_OK;
 }
 
@@ -708,6 +809,8 @@ int aph_adjoint_stage(aph_context *ctx, aph_plan *p, int32_t stage, const double
     return set_error(ctx, APH_E_INVALID, "adjoint: the plan's integrator must be rk2");
   if (!p || !u_in || !dt_dev || !lam_out || !lam_in || (stage != 1 && stage != 2))
     return set_error(ctx, APH_E_INVALID, "aph_adjoint_stage: bad arguments");
+  if (p->amr)
+    return set_error(ctx, APH_E_INVALID, "adjoint: multilevel meshes are not supported");
   if (lam_in == lam_out || (lam_acc && lam_acc == lam_in))
     return set_error(ctx, APH_E_INVALID, "aph_adjoint_stage: lam_in must not alias inputs");
   if (!adj_supported(p->recon, p->rsolver, p->bs->nvar, p->bs->ndim))
@@ -723,6 +826,8 @@ int aph_adjoint_step(aph_context *ctx, aph_plan *p, double *u0, double *u1,
     return set_error(ctx, APH_E_INVALID, "adjoint: the plan's integrator must be rk2");
   if (!p || !u0 || !u1 || !dt_dev || !lam || u0 == u1)
     return set_error(ctx, APH_E_INVALID, "aph_adjoint_step: bad arguments");
+  if (p->amr)
+    return set_error(ctx, APH_E_INVALID, "adjoint: multilevel meshes are not supported");
   if (!adj_supported(p->recon, p->rsolver, p->bs->nvar, p->bs->ndim))
     return set_error(ctx, APH_E_INVALID, "adjoint: unsupported (recon, riemann, nvar)");
   int rc = adj_prepare(ctx, p->bs, p->recon, p->adj);
@@ -740,6 +845,8 @@ int aph_adjoint_step_stored(aph_context *ctx, aph_plan *p, const double *u0, con
     return set_error(ctx, APH_E_INVALID, "adjoint: the plan's integrator must be rk2");
   if (!p || !u0 || !u1 || !dt_dev || !lam || u0 == u1 || lam == u0 || lam == u1)
     return set_error(ctx, APH_E_INVALID, "aph_adjoint_step_stored: bad arguments");
+  if (p->amr)
+    return set_error(ctx, APH_E_INVALID, "adjoint: multilevel meshes are not supported");
   if (!adj_supported(p->recon, p->rsolver, p->bs->nvar, p->bs->ndim))
     return set_error(ctx, APH_E_INVALID, "adjoint: unsupported (recon, riemann, nvar)");
   int rc = adj_prepare(ctx, p->bs, p->recon, p->adj);

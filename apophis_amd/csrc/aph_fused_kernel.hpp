@@ -39,6 +39,8 @@ struct FusedArgs {
   double cfl;
   int has_un;      // u_n is a separate register (every stage but the first)
   int last;        // last stage of the step: estimate the next dt
+  const aph_amr_face *amr_face;  // two-level mesh: [block][6] interface roles, or null
+  double *amr_buf;               // interface flux buffer (fine side writes, coarse reads)
 };
 
 namespace {
@@ -68,6 +70,8 @@ __device__ __forceinline__ bool load_cell(const FusedArgs &a, const aph_block &b
     } else if (nb.bclo == APH_BC_REFLECT) {
       idx = -1 - c;
       flip = true;
+    } else if (nb.bclo == APH_BC_AMR) {
+      idx = c;  // the block's own ghost cells, set from the other level before the stage
     } else {
       idx = 0;
     }
@@ -78,6 +82,8 @@ __device__ __forceinline__ bool load_cell(const FusedArgs &a, const aph_block &b
     } else if (nb.bchi == APH_BC_REFLECT) {
       idx = 2 * n_d - 1 - c;
       flip = true;
+    } else if (nb.bchi == APH_BC_AMR) {
+      idx = c;
     } else {
       idx = n_d - 1;
     }
@@ -147,7 +153,33 @@ __device__ __forceinline__ void sweep(const FusedArgs &a, const aph_block &bk, i
       if (flip) u[1 + D] = -1.0 * u[1 + D];
       cons_to_prim_pt<NV>(u, a.eos, dst, dst + NV);
     };
-    auto emit = [&](int f, const double *fo, const double *, const double *) {
+    auto emit = [&](int f, const double *fo_in, const double *, const double *) {
+      const double *fo = fo_in;
+      double fc[NV];
+      if (a.amr_face && (f == 0 || f == n_d)) {
+        // a block face to the other level: the fine side records its flux, the coarse
+        // side takes the mean of the fine fluxes across (flux correction)
+        const aph_amr_face &af = a.amr_face[bidx * 6 + 2 * D + (f == 0 ? 0 : 1)];
+        constexpr int NCH = (NDIM == 3) ? 4 : 2;
+        if (af.role == 1) {
+          const int fa = af.qa + ca, fb = af.qb + cb;
+          const int child = (NDIM == 3) ? (fa & 1) * 2 + (fb & 1) : (fa & 1);
+          double *q = a.amr_buf + af.base +
+                      ((int64_t)((fb >> 1) * af.na + (fa >> 1)) * NCH + child) * NV;
+#pragma unroll
+          for (int v = 0; v < NV; ++v) q[v] = fo_in[v];
+        } else if (af.role == 2) {
+          const double *q = a.amr_buf + af.base + ((int64_t)cb * af.na + ca) * NCH * NV;
+#pragma unroll
+          for (int v = 0; v < NV; ++v) {
+            double sum = q[v];
+#pragma unroll
+            for (int k = 1; k < NCH; ++k) sum += q[k * NV + v];
+            fc[v] = sum * (1.0 / NCH);
+          }
+          fo = fc;
+        }
+      }
       if (f > f_lo) {
         const int m = f - 1;  // cell whose upper face is f
         int lc[3];

@@ -418,7 +418,7 @@ class FusedDriver:
     stage); u1 holds the stage-1 result.  Ghost cells are never read."""
 
     def __init__(self, ctx, bs: BlockSet, cons, recon, riemann, eos, cfl, g=(0.0, 0.0, 0.0),
-                 tlim=np.inf, device="cuda", dbs=None, integrator="rk2"):
+                 tlim=np.inf, device="cuda", dbs=None, integrator="rk2", amr_mesh=None):
         self.ctx = ctx
         self.bs = bs
         self.device_bs = dbs or DeviceBlockSet(ctx, bs)
@@ -439,6 +439,15 @@ class FusedDriver:
         self.graph_steps = 0
         if integrator != "rk2":
             self.set_integrator(integrator)
+        self.amr = None
+        if amr_mesh is not None:  # a two-level mesh (amr.TwoLevelMesh) of this block set
+            from . import amr as _amr
+            self.amr = _amr.AmrPlan(ctx, self.device_bs.h, amr_mesh, bs)
+            faces, level, nbuf = _amr.fused_faces(amr_mesh, bs)
+            self._amr_faces, self._amr_level = faces, level
+            ctx.check(_lib.lib().aph_plan_set_amr(
+                ctx.h, self.plan, self.amr.h, faces.ctypes.data_as(ctypes.c_void_p),
+                level.ctypes.data_as(ctypes.c_void_p), nbuf))
 
     def set_integrator(self, name):
         """rk1 / rk2 / rk3 / vl2 (aph_plan_set_integrator)."""

@@ -286,6 +286,28 @@ int aph_plan_graph_launch(aph_context *ctx, aph_plan *plan);
 int aph_plan_estimate_timestep(aph_context *ctx, aph_plan *plan, const double *cons,
                                double cfl, double *dt_out);
 
+/* Two-level meshes in the fused form.  Per block face, for faces with bc = APH_BC_AMR
+   (zeros elsewhere): the fine side writes the flux of each of its boundary faces to an
+   interface buffer, ordered per coarse face cell (transverse axes in ascending order,
+   the first slowest) and child (the same order); the coarse side replaces each boundary
+   face flux by the mean of its children there, in that order -- the flux correction of
+   aph_amr_flux_correct, bitwise.  A stage then runs aph_amr_fill on its input, the kernel
+   over the level-1 blocks, and the kernel over the level-0 blocks. */
+typedef struct aph_amr_face {
+  int64_t base;  /* element offset of the coarse face region in the interface buffer */
+  int32_t role;  /* 0 none, 1 fine side (writes), 2 coarse side (reads)               */
+  int32_t qa, qb;/* fine side: offset of this block's face inside the coarse face
+                    region, in fine cells along the two transverse axes              */
+  int32_t na;    /* coarse face cells along the first transverse axis               */
+} aph_amr_face;
+/* faces: nblocks*6 entries (face order -x1,+x1,-x2,+x2,-x3,+x3); level: nblocks entries
+   (0 coarse, 1 fine); buffer_doubles: interface buffer length (>= every base + its
+   region's cells x 2^(ndim-1) x nvar).  The amr plan must outlive the plan.  The adjoint
+   entry points reject a plan with a multilevel mesh.  Synchronous. */
+int aph_plan_set_amr(aph_context *ctx, aph_plan *plan, const aph_amr_plan *amr,
+                     const aph_amr_face *faces, const int32_t *level,
+                     int64_t buffer_doubles);
+
 /* ----------------------------------------------------- adjoint (frozen dt)
  * Reverse-mode counterparts of the fused stage (no reference counterpart: the
  * reference has no adjoint).  Given the cotangent lam_out of a stage's output:

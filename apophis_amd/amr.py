@@ -179,41 +179,41 @@ def amr_plan(mesh: TwoLevelMesh, bs: BlockSet):
         lv = int(mesh.level[b])
         lc = mesh.loc[b]
         nbr = bs.blocks["nbr"][b]
-        rng = [range(int(N[b][d])) for d in range(3)]
-        for k in rng[2]:
-            for j in rng[1]:
-                for i in rng[0]:
-                    loc = (i, j, k)
-                    o = [0, 0, 0]
-                    for d in act:
-                        o[d] = -1 if loc[d] < g else (1 if loc[d] >= g + nx[d] else 0)
-                    if o == [0, 0, 0]:
-                        continue
-                    if nbr[(o[2] + 1) * 9 + (o[1] + 1) * 3 + (o[0] + 1)] >= 0:
-                        continue  # same-level exchange
-                    G = tuple(int(lc[d]) * nx[d] + loc[d] - g if d < nd else 0
-                              for d in range(3))
-                    dst, stride = _cell(bs, b, loc)
-                    rec = np.zeros(1, dtype=AMR_OP_DTYPE)[0]
-                    rec["dst"], rec["dst_stride"] = dst, stride
-                    if lv == 0:  # coarse ghost over a refined region: restriction
-                        rec["kind"] = AMR_MEAN
-                        rec["val"][0] = coarse_value(wrap(0, G))
-                    else:  # fine ghost over a coarse region: prolongation
-                        Gw = wrap(1, G)
-                        I = tuple(Gw[d] // 2 if d < nd else 0 for d in range(3))
-                        assert interior(0, I) is not None, "two levels only"
-                        rec["kind"] = AMR_PROLONG
-                        rec["val"][0] = coarse_value(I)
-                        for d in act:
-                            e = [0, 0, 0]
-                            e[d] = 1
-                            rec["sign"][d] = -1 if Gw[d] % 2 == 0 else 1
-                            rec["val"][1 + 2 * d] = coarse_value(
-                                tuple(I[t] - e[t] for t in range(3)))
-                            rec["val"][2 + 2 * d] = coarse_value(
-                                tuple(I[t] + e[t] for t in range(3)))
-                    ops.append(rec)
+        # ghost cells whose direction has no same-level neighbour (the rest is exchanged)
+        E = [int(N[b][d]) for d in range(3)]
+        kk, jj, ii = np.meshgrid(np.arange(E[2]), np.arange(E[1]), np.arange(E[0]),
+                                 indexing="ij")
+        o3 = [np.zeros_like(ii) for _ in range(3)]
+        for d in act:
+            c = (ii, jj, kk)[d]
+            o3[d] = np.where(c < g, -1, np.where(c >= g + nx[d], 1, 0))
+        ghost = (o3[0] != 0) | (o3[1] != 0) | (o3[2] != 0)
+        need = ghost & (nbr[(o3[2] + 1) * 9 + (o3[1] + 1) * 3 + (o3[0] + 1)] < 0)
+        for k, j, i in zip(kk[need].tolist(), jj[need].tolist(), ii[need].tolist()):
+            loc = (i, j, k)
+            G = tuple(int(lc[d]) * nx[d] + loc[d] - g if d < nd else 0
+                      for d in range(3))
+            dst, stride = _cell(bs, b, loc)
+            rec = np.zeros(1, dtype=AMR_OP_DTYPE)[0]
+            rec["dst"], rec["dst_stride"] = dst, stride
+            if lv == 0:  # coarse ghost over a refined region: restriction
+                rec["kind"] = AMR_MEAN
+                rec["val"][0] = coarse_value(wrap(0, G))
+            else:  # fine ghost over a coarse region: prolongation
+                Gw = wrap(1, G)
+                I = tuple(Gw[d] // 2 if d < nd else 0 for d in range(3))
+                assert interior(0, I) is not None, "two levels only"
+                rec["kind"] = AMR_PROLONG
+                rec["val"][0] = coarse_value(I)
+                for d in act:
+                    e = [0, 0, 0]
+                    e[d] = 1
+                    rec["sign"][d] = -1 if Gw[d] % 2 == 0 else 1
+                    rec["val"][1 + 2 * d] = coarse_value(
+                        tuple(I[t] - e[t] for t in range(3)))
+                    rec["val"][2 + 2 * d] = coarse_value(
+                        tuple(I[t] + e[t] for t in range(3)))
+            ops.append(rec)
     fixes = []
     for b in range(bs.nblocks):
         if mesh.level[b] != 0:

@@ -473,3 +473,31 @@ def kh_comp(nx=(256, 128), block=(64, 128), vflow=1.0, drat=0.1, amp=0.01, ofrac
                       en.reshape(shp), [s.reshape(shp) for s in sc])
     return Problem("kh_comp", bs, cons, make_eos(eos_gm1), recon, riemann, cfl, tlim=tlim,
                    nsteps=nsteps, meta={"nlset": nlset, "composition": "ONe"})
+
+
+# ---------------------------------------------------------------- two-level blast
+def blast2d_two_level(nbase=48, n=32, nref=16, seed=20240601, recon="ppm", riemann="hllc",
+                      cfl=0.3, noise=0.01, p_amb=0.1, p_in=10.0, radius=0.1):
+    """2D blast wave (blast.cpp:16-71, no ramp) on [-1/2,1/2]^2, periodic, on a two-level
+    mesh: a base of nbase^2 blocks of n^2 whose central nref^2 base blocks are refined once
+    (each replaced by 4 blocks of n^2 at half the spacing).  Density noise is keyed by the
+    block and the cell's index in it.  Returns (Problem, amr.TwoLevelMesh)."""
+    from .amr import TwoLevelMesh
+
+    gm1 = 2.0 / 3.0
+    lo = (nbase - nref) // 2
+    refined = [(x, y) for x in range(lo, lo + nref) for y in range(lo, lo + nref)]
+    m = TwoLevelMesh(2, (nbase, nbase), (n, n), NEED_GHOST[recon], (-0.5, -0.5, -0.5),
+                     (0.5, 0.5, 0.5), refined)
+    bs = m.block_set()
+    cons = np.zeros(bs.field_size(bs.nvar))
+    for b in range(bs.nblocks):
+        x, y, _ = bs.cell_centers(b)
+        key = (np.uint64(b) << np.uint64(20)) + np.arange(x.size, dtype=np.uint64)
+        u = uniform(seed, key).reshape(x.shape)
+        rho = 1.0 * (1.0 + noise * (2.0 * u - 1.0))
+        pres = np.where(np.sqrt(x * x + y * y) < radius, p_in, p_amb)
+        _set_interior(bs, cons, b, rho, 0.0, 0.0, 0.0, pres / gm1)
+    p = Problem("blast2d_two_level", bs, cons, make_eos(gm1), recon, riemann, cfl,
+                meta={"nbase": nbase, "nref": nref})
+    return p, m

@@ -243,6 +243,50 @@ def composition_leg(ctx, hydro, problems, torch, nsteps=20):
     return out
 
 
+def two_level_leg(ctx, hydro, problems, torch, nsteps=20):
+    """A two-level mesh (2D blast, 48^2 base blocks of 32^2 with the central 16^2 refined:
+    3.1 M cells, 3072 blocks): zone-cycles/s of the batched form (cross-level ghosts, fine
+    blocks first, flux correction from the interface buffer; graph-replayed) and of the
+    plain task form (aph_amr_fill / aph_amr_flux_correct between the tasks)."""
+    from apophis_amd import amr
+    t_plan = time.perf_counter()
+    p, m = problems.blast2d_two_level()
+    cells = p.bs.interior_cells()
+    fd = hydro.FusedDriver(ctx, p.bs, p.cons, p.recon, p.riemann, p.eos, p.cfl, amr_mesh=m)
+    t_plan = time.perf_counter() - t_plan
+    fd.initial_dt()
+    for _ in range(2):
+        fd.step()
+    fd.capture(nsteps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fd.replay()
+    torch.cuda.synchronize()
+    vf = cells * nsteps / (time.perf_counter() - t0)
+    del fd
+    pkg = hydro.initialize_hydro({
+        "parthenon/time": {"cfl": p.cfl, "integrator": "rk2"},
+        "parthenon/mesh": {"nghost": int(p.bs.blocks["ng"][0])},
+        "hydro": {"reconstruction": p.recon, "riemann": p.riemann},
+        "eos": {"gm1": float(p.eos["gm1"][0])}})
+    md = hydro.MeshData(ctx, p.bs, p.cons)
+    plan = amr.AmrPlan(ctx, md.dbs, m, p.bs)
+    drv = hydro.ApophisDriver(pkg, md, amr=plan)
+    dt = drv.initialize()
+    dt = drv.step(dt)
+    torch.cuda.synchronize()
+    npl = 5
+    t0 = time.perf_counter()
+    for _ in range(npl):
+        dt = drv.step(dt)
+    torch.cuda.synchronize()
+    vp = cells * npl / (time.perf_counter() - t0)
+    return {"cells": cells, "blocks": p.bs.nblocks, "fine_blocks": int((m.level == 1).sum()),
+            "ghost_ops": len(plan.ops), "flux_fixes": len(plan.fixes),
+            "plan_build_s": round(t_plan, 2), "fused_zone_cycles_per_s": vf,
+            "plain_zone_cycles_per_s": vp, "fused_over_plain": vf / vp}
+
+
 def profile_run(args, ctx, hydro, problems, torch):
     """Eager launches for rocprofv3: 3 RK2 steps of the chosen form on the headline
     workload (or the uneven block set), then a 1 GiB device-to-device copy repeated 3
@@ -435,6 +479,8 @@ def main():
         del dq, mq
 
     if not args.no_extras:
+        extras["two_level_blast2d"] = two_level_leg(ctx, hydro, problems, torch)
+        print(f"# two-level: {extras['two_level_blast2d']}", file=sys.stderr, flush=True)
         extras["kh_comp_ONe_lset"] = composition_leg(ctx, hydro, problems, torch)
         print(f"# composition: {extras['kh_comp_ONe_lset']}", file=sys.stderr, flush=True)
     if not args.no_adjoint:

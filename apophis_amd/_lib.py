@@ -57,6 +57,7 @@ _SIGS = {
     "aph_amr_fill": (_i32, [_vp, _vp, _vp]),
     "aph_amr_flux_correct": (_i32, [_vp, _vp, _vp, _vp, _vp]),
     "aph_plan_set_amr": (_i32, [_vp, _vp, _vp, _vp, _vp, _i64]),
+    "aph_refinement_tag": (_i32, [_vp, _vp, _i32, _vp, _vp, _vp, _vp]),
     "aph_plan_create": (_i32, [_vp, _vp, _i32, _i32, _vp, _vp, _pp]),
     "aph_plan_destroy": (_i32, [_vp, _vp]),
     "aph_plan_set_tile": (_i32, [_vp, _vp, ctypes.POINTER(Tile)]),

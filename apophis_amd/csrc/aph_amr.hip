@@ -122,12 +122,118 @@ struct Locator {
   }
 };
 
+// Refinement criteria (src/refinement/gradient.cpp, other.cpp): one workgroup per block
+// reduces the criterion's quantity over the reference's index ranges with max, which is
+// independent of the order, so the result is the same bits as any serial loop.
+__global__ void __launch_bounds__(kThreads)
+    k_refine_tag(const aph_block *__restrict__ blocks, int ndim, int nvar, int crit, double p0,
+                 double p1, const double *__restrict__ prim, int32_t *tags, double *values) {
+  const aph_block &bk = blocks[blockIdx.x];
+  const int N0 = ext(bk, 0, ndim), N1 = ext(bk, 1, ndim), N2 = ext(bk, 2, ndim);
+  const int g[3] = {gst(bk, 0, ndim), gst(bk, 1, ndim), gst(bk, 2, ndim)};
+  const int64_t nc = (int64_t)N0 * N1 * N2;
+  const double *w = prim + (int64_t)nvar * bk.cell_base;
+  auto at = [&](int v, int i, int j, int k) -> double {
+    return w[v * nc + ((int64_t)k * N1 + j) * N0 + i];
+  };
+  // inclusive ghost-inclusive ranges
+  int lo[3], hi[3];
+  for (int d = 0; d < 3; ++d) {
+    lo[d] = g[d];
+    hi[d] = g[d] + bk.nx[d] - 1;
+  }
+  if (crit == APH_REFINE_PRESSURE_GRADIENT) {
+    for (int d = 0; d < ndim; ++d) {
+      --lo[d];
+      ++hi[d];
+    }
+  } else if (crit == APH_REFINE_VELOCITY_GRADIENT) {
+    --lo[0];
+    ++hi[0];
+    --lo[1];
+    ++hi[1];
+  } else {
+    ++hi[0];  // the reference's i range runs to ib.e + 1
+  }
+  const int n0 = hi[0] - lo[0] + 1, n1 = hi[1] - lo[1] + 1, n2 = hi[2] - lo[2] + 1;
+  const int64_t n = (int64_t)n0 * n1 * n2;
+  double m = 0.0;
+  if (!(crit == APH_REFINE_PRESSURE_GRADIENT && ndim == 1)) {
+    for (int64_t t = threadIdx.x; t < n; t += blockDim.x) {
+      const int i = lo[0] + (int)(t % n0);
+      const int j = lo[1] + (int)((t / n0) % n1);
+      const int k = lo[2] + (int)(t / ((int64_t)n0 * n1));
+      double q;
+      if (crit == APH_REFINE_PRESSURE_GRADIENT) {
+        const double a = 0.5 * (at(4, i + 1, j, k) - at(4, i - 1, j, k));
+        const double b = 0.5 * (at(4, i, j + 1, k) - at(4, i, j - 1, k));
+        double s2 = a * a + b * b;
+        if (ndim == 3) {
+          const double c = 0.5 * (at(4, i, j, k + 1) - at(4, i, j, k - 1));
+          s2 = s2 + c * c;
+        }
+        q = sqrt(s2) / at(4, i, j, k);
+      } else if (crit == APH_REFINE_VELOCITY_GRADIENT) {
+        const double vgy = fabs(at(2, i + 1, j, k) - at(2, i - 1, j, k)) * 0.5;
+        const double vgx = fabs(at(1, i, j + 1, k) - at(1, i, j - 1, k)) * 0.5;
+        q = sqrt(vgx * vgx + vgy * vgy);
+      } else {
+        q = at(0, i, j, k);
+      }
+      m = (m < q) ? q : m;  // std::max(m, q)
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const double r = __shfl_xor(m, o);
+    m = (m < r) ? r : m;
+  }
+  __shared__ double part[kThreads / 64];
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < (int)(blockDim.x >> 6); ++k) m = (m < part[k]) ? part[k] : m;
+    int tag = 0;
+    if (crit == APH_REFINE_PRESSURE_GRADIENT) {
+      if (ndim > 1) tag = (m > p0) ? 1 : ((m < 0.25 * p0) ? -1 : 0);
+    } else if (crit == APH_REFINE_VELOCITY_GRADIENT) {
+      tag = (m > p0) ? 1 : ((m < 0.5 * p0) ? -1 : 0);
+    } else {
+      tag = (m > p1) ? 1 : ((m < p0) ? -1 : 0);
+    }
+    tags[blockIdx.x] = tag;
+    if (values) values[blockIdx.x] = m;
+  }
+}
+
 }  // namespace
 }  // namespace aph
 
 using namespace aph;
 
 extern "C" {
+
+int aph_refinement_tag(aph_context *ctx, const aph_blockset *bs, int32_t criterion,
+                       const double params[2], const double *prim, int32_t *tags,
+                       double *values) {
+  if (!ctx) return APH_E_INVALID;
+  if (!bs || !params || !prim || !tags)
+    return set_error(ctx, APH_E_INVALID, "aph_refinement_tag: null");
+  if (criterion < APH_REFINE_PRESSURE_GRADIENT || criterion > APH_REFINE_MAX_DENSITY)
+    return set_error(ctx, APH_E_INVALID, "aph_refinement_tag: unknown criterion %d", criterion);
+  if (criterion == APH_REFINE_VELOCITY_GRADIENT && bs->ndim < 2)
+    return set_error(ctx, APH_E_INVALID, "aph_refinement_tag: the velocity gradient needs ndim >= 2");
+  const int need = criterion == APH_REFINE_MAX_DENSITY ? 1 : 2;
+  for (int b = 0; b < bs->nblocks; ++b)
+    if (bs->ndim > 1 && bs->h[b].ng < need)
+      return set_error(ctx, APH_E_INVALID, "aph_refinement_tag: block %d has %d ghost layers, "
+                       "the criterion reads %d", b, bs->h[b].ng, need);
+  if (!bs->nblocks) return APH_OK;
+  k_refine_tag<<<bs->nblocks, kThreads, 0, ctx->stream>>>(bs->d, bs->ndim, bs->nvar, criterion,
+                                                          params[0], params[1], prim, tags,
+                                                          values);
+  APH_LAUNCH_CHECK(ctx);
+  return APH_OK;
+}
 
 int aph_amr_plan_create(aph_context *ctx, const aph_blockset *bs, const aph_amr_op *ops,
                         int64_t nops, const aph_flux_fix *fixes, int64_t nfixes,

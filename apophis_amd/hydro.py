@@ -214,6 +214,34 @@ def initialize_hydro(pin):
     prim_labels += ["scalar_" + x for x in extra]
     pkg.add_param("cons_labels", cons_labels)
     pkg.add_param("prim_labels", prim_labels)
+    # refinement criterion (hydro.cpp:303-333)
+    refine_str = _get(pin, "refinement", "type", "unset")
+    pkg.add_param("refinement/type", refine_str)
+    if refine_str == "pressure_gradient":
+        thr = float(_get(pin, "refinement", "threshold_pressure_gradient", 0.0))
+        if not thr > 0.0:
+            raise ApophisError("Make sure to set refinement/threshold_pressure_gradient >0.")
+        pkg.add_param("refinement/threshold_pressure_gradient", thr)
+        pkg.add_param("refinement/criterion", (REFINE_CRITERIA[refine_str], thr, 0.0))
+    elif refine_str == "xyvelocity_gradient":
+        thr = float(_get(pin, "refinement", "threshold_xyvelocity_gradient", 0.0))
+        if not thr > 0.0:
+            raise ApophisError("Make sure to set refinement/threshold_xyvelocity_gradient >0.")
+        pkg.add_param("refinement/threshold_xyvelocity_gradient", thr)
+        pkg.add_param("refinement/criterion", (REFINE_CRITERIA[refine_str], thr, 0.0))
+    elif refine_str == "maxdensity":
+        lo = float(_get(pin, "refinement", "maxdensity_deref_below", 0.0))
+        hi = float(_get(pin, "refinement", "maxdensity_refine_above", 0.0))
+        if not lo > 0.0:
+            raise ApophisError("Make sure to set refinement/maxdensity_deref_below > 0.")
+        if not hi > 0.0:
+            raise ApophisError("Make sure to set refinement/maxdensity_refine_above > 0.")
+        if not lo < hi:
+            raise ApophisError("Make sure to set refinement/maxdensity_deref_below < "
+                               "refinement/maxdensity_refine_above")
+        pkg.add_param("refinement/maxdensity_deref_below", lo)
+        pkg.add_param("refinement/maxdensity_refine_above", hi)
+        pkg.add_param("refinement/criterion", (REFINE_CRITERIA[refine_str], lo, hi))
     eos_type = _get(pin, "eos", "type", "ideal")
     if eos_type == "helm":
         if ncomp < 1:
@@ -314,6 +342,25 @@ def estimate_timestep(md, eos, cfl, out=None):
         md.ctx.h, md.dbs, md.first, md.count, eos.ctypes.data_as(ctypes.c_void_p),
         _ptr(md.prim), _ptr(md.cons), cfl, _ptr(out)))
     return out
+
+
+# aph_refine_criterion codes, keyed by the reference's refinement/type names
+REFINE_CRITERIA = {"pressure_gradient": 1, "xyvelocity_gradient": 2, "maxdensity": 3}
+
+
+def refinement_tag(md, pkg):
+    """The package's CheckRefinementBlock over every block of md (Refinement::Tag,
+    apophis_driver.cpp:120-127): (tags, values) device tensors, tags -1 / 0 / 1 =
+    derefine / same / refine.  prim must be current (after ConsToPrim)."""
+    crit, a, b = pkg.param("refinement/criterion")
+    n = md.bs.nblocks
+    tags = torch.zeros(n, dtype=torch.int32, device=md.cons.device)
+    vals = torch.zeros(n, dtype=torch.float64, device=md.cons.device)
+    params = (ctypes.c_double * 2)(a, b)
+    assert tags.is_cuda and tags.is_contiguous()
+    md.ctx.check(_lib.lib().aph_refinement_tag(md.ctx.h, md.dbs, crit, params, _ptr(md.prim),
+                                               ctypes.c_void_p(tags.data_ptr()), _ptr(vals)))
+    return tags, vals
 
 
 def global_quant_hst(md, var, out=None):

@@ -157,6 +157,29 @@ int aph_estimate_timestep(aph_context *ctx, const aph_blockset *bs, int32_t firs
 int aph_global_sum(aph_context *ctx, const aph_blockset *bs, const double *cons,
                    int32_t var, double *out);
 
+/* Refinement criteria of the hydro package (hydro.cpp:303-333, src/refinement/), one
+ * batched launch over the blocks: per block, the reference's reduction over its own index
+ * ranges of prim (which must hold valid ghost cells, as after aph_cons_to_prim):
+ *   APH_REFINE_PRESSURE_GRADIENT (gradient.cpp:11-53): eps = max sqrt(sum_d (dp_d/2)^2)/p
+ *     over the interior grown by one cell on every active axis (1D: tag 0);
+ *     refine if eps > params[0], derefine if eps < params[0]/4;
+ *   APH_REFINE_VELOCITY_GRADIENT (gradient.cpp:56-84, ndim >= 2): vg = max sqrt(vgx^2 +
+ *     vgy^2), vgy = |v2(i+1) - v2(i-1)|/2, vgx = |v1(j+1) - v1(j-1)|/2, over i and j grown
+ *     by one, k interior; refine if vg > params[0], derefine if vg < params[0]/2;
+ *   APH_REFINE_MAX_DENSITY (other.cpp:11-34): max rho over k, j interior and i from the
+ *     first interior cell to one past the last; refine if > params[1], derefine if <
+ *     params[0].
+ * tags (device, nblocks): -1 derefine, 0 same, 1 refine (Parthenon's AmrTag); values
+ * (device, nblocks, may be NULL): the reduced quantity. */
+enum aph_refine_criterion {
+  APH_REFINE_PRESSURE_GRADIENT = 1,
+  APH_REFINE_VELOCITY_GRADIENT = 2,
+  APH_REFINE_MAX_DENSITY = 3
+};
+int aph_refinement_tag(aph_context *ctx, const aph_blockset *bs, int32_t criterion,
+                       const double params[2], const double *prim, int32_t *tags,
+                       double *values);
+
 /* ------------------------------------------- multilevel meshes (plain form)
  * Blocks of two refinement levels in one mesh (a 2:1 jump).  nbr[] links blocks of the
  * same level only; a face whose neighbour region belongs to the other level has nbr = -1

@@ -6,8 +6,8 @@ The reference takes these from Parthenon (apophis_driver.cpp:68,74-76,94), which
 the reference tree; what is restated here is its published behaviour for a Cartesian
 mesh with a 2:1 jump (RestrictAverage, ProlongateSharedMinMod, flux correction by the
 mean of the fine fluxes), with the operation order the build defines
-(apophis_amd/amr.py, include/apophis_hydro.h): restriction sums the 2^d children with x
-fastest, then z, y, x outermost ... i.e. in (dz, dy, dx) order, and scales by 2^-d;
+(apophis_amd/amr.py, include/apophis_hydro.h): restriction sums the 2^d children in
+(dz, dy, dx) order, x fastest, and scales by 2^-d;
 prolongation is c + sum_d (sign_d / 4) minmod(c - c_-d, c_+d - c) accumulated in axis
 order; a corrected coarse face is the mean of its fine faces, the transverse child index
 of the lower axis varying slowest.  Parity with the reference is unpinned (no Parthenon
@@ -16,6 +16,11 @@ linear fields, conservation, uniform states and an op-by-op cross-check of the h
 
 This module works from whole-level arrays (every level-0 cell of the domain, the
 refined ones holding the restriction of their children), not from the plan's op lists.
+
+refinement_tag restates the reference's own refinement criteria
+(src/refinement/gradient.cpp, other.cpp) with their index ranges; tests/test_refinement.py
+pins it by closed forms (linear pressure and velocity fields, uniform states) and the
+ranges' edge cells.
 """
 from __future__ import annotations
 
@@ -218,3 +223,61 @@ class AmrState:
             orc.update(bs, self.u0, self.u1, *self.fx, g0[s], g1[s], be[s] * dt)
             self._ghosts()
         return self.estimate_dt()
+
+
+def refinement_tag(bs, prim, crit, p0, p1):
+    """Per block (tag, value) of the reference's refinement criteria
+    (src/refinement/gradient.cpp:11-84, other.cpp:11-34) on prim with its ghost cells:
+    crit 1 pressure gradient, 2 xy-velocity gradient, 3 max density (see
+    include/apophis_hydro.h for the ranges; the density's i range runs to ib.e + 1)."""
+    nd = bs.ndim
+    tags = np.zeros(bs.nblocks, dtype=np.int32)
+    vals = np.zeros(bs.nblocks)
+    for b in range(bs.nblocks):
+        w = bs.block_view(prim, b, bs.nvar)  # (nv, Nz, Ny, Nx)
+        g = int(bs.blocks["ng"][b])
+        nx = [int(v) for v in bs.blocks["nx"][b]]
+        gg = [g if d < nd else 0 for d in range(3)]
+        lo = [gg[d] for d in range(3)]
+        hi = [gg[d] + nx[d] - 1 for d in range(3)]
+        if crit == 1:
+            for d in range(nd):
+                lo[d] -= 1
+                hi[d] += 1
+        elif crit == 2:
+            for d in (0, 1):
+                lo[d] -= 1
+                hi[d] += 1
+        else:
+            hi[0] += 1
+        K = np.arange(lo[2], hi[2] + 1)[:, None, None]
+        J = np.arange(lo[1], hi[1] + 1)[None, :, None]
+        I = np.arange(lo[0], hi[0] + 1)[None, None, :]
+        if crit == 1:
+            if nd == 1:
+                continue
+            p = w[4]
+            a = 0.5 * (p[K, J, I + 1] - p[K, J, I - 1])
+            c = 0.5 * (p[K, J + 1, I] - p[K, J - 1, I])
+            s2 = a * a + c * c
+            if nd == 3:
+                e = 0.5 * (p[K + 1, J, I] - p[K - 1, J, I])
+                s2 = s2 + e * e
+            q = np.sqrt(s2) / p[K, J, I]
+        elif crit == 2:
+            vgy = np.abs(w[2][K, J, I + 1] - w[2][K, J, I - 1]) * 0.5
+            vgx = np.abs(w[1][K, J + 1, I] - w[1][K, J - 1, I]) * 0.5
+            q = np.sqrt(vgx * vgx + vgy * vgy)
+        else:
+            q = w[0][K, J, I] + 0.0 * I  # broadcast to the range
+        m = 0.0
+        for v in q.ravel():  # std::max, in any order
+            m = v if m < v else m
+        vals[b] = m
+        if crit == 1:
+            tags[b] = 1 if m > p0 else (-1 if m < 0.25 * p0 else 0)
+        elif crit == 2:
+            tags[b] = 1 if m > p0 else (-1 if m < 0.5 * p0 else 0)
+        else:
+            tags[b] = 1 if m > p1 else (-1 if m < p0 else 0)
+    return tags, vals

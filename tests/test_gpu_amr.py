@@ -129,3 +129,43 @@ def test_fused_amr_steps_match_oracle(gpu_ctx, name, recon, riemann):
     got = fd.u0.cpu().numpy()
     mask = np.tile(p.bs.interior_mask(), 5)
     assert np.array_equal(got[mask], ref.u0[mask])
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_refinement_tags_match_oracle(gpu_ctx, ndim):
+    """aph_refinement_tag on an evolved blast state (prim with ghosts from the plain form)
+    against the oracle's restatement of the three criteria: the same reduced values,
+    bitwise, and the same tags, with thresholds that make blocks refine, stay and
+    derefine."""
+    from apophis_amd import problems
+    print(f"[case] refinement tags {ndim}d", flush=True)
+    if ndim == 3:
+        p = problems.blast3d(nblk=4, n=8, recon="plm")
+    else:
+        p, _ = problems.blast2d_two_level(nbase=8, n=16, nref=0, recon="plm")
+    ref = orc.State(p)
+    dt = ref.estimate_dt()
+    for _ in range(6):
+        dt = ref.step(dt)
+    pin = {"parthenon/time": {"cfl": 0.3, "integrator": "rk2"},
+           "parthenon/mesh": {"nghost": int(p.bs.blocks["ng"].min())},
+           "hydro": {"reconstruction": p.recon, "riemann": p.riemann},
+           "eos": {"gm1": float(p.eos["gm1"][0])}}
+    md = hydro.MeshData(gpu_ctx, p.bs, ref.u0)
+    hydro.fill_ghosts(md)
+    hydro.cons_to_prim(md, p.eos)
+    vals0 = {c: ao.refinement_tag(p.bs, ref.prim, c, 1.0, 2.0)[1] for c in (1, 2, 3)}
+    q = lambda c, f: float(np.quantile(vals0[c], f))
+    for refine in ({"type": "pressure_gradient", "threshold_pressure_gradient": q(1, 0.7)},
+                   {"type": "xyvelocity_gradient",
+                    "threshold_xyvelocity_gradient": q(2, 0.7)},
+                   {"type": "maxdensity", "maxdensity_deref_below": q(3, 0.3),
+                    "maxdensity_refine_above": q(3, 0.7)}):
+        pkg = hydro.initialize_hydro(dict(pin, refinement=refine))
+        tags, vals = hydro.refinement_tag(md, pkg)
+        crit, a, b = pkg.param("refinement/criterion")
+        want_t, want_v = ao.refinement_tag(p.bs, ref.prim, crit, a, b)
+        torch.cuda.synchronize()
+        assert np.array_equal(vals.cpu().numpy(), want_v), refine["type"]
+        assert np.array_equal(tags.cpu().numpy(), want_t), refine["type"]
+        assert len(set(want_t.tolist())) >= 2, (refine["type"], want_t)

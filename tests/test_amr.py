@@ -106,6 +106,7 @@ def test_linear_field_exact_across_levels(name):
     orc.fill_ghosts(bs, u)
     ao.amr_fill(m, bs, u)
     ops, _, _ = amr.amr_plan(m, bs)
+    dst = set(int(v) for v in ops["dst"])
     checked = 0
     ext = bs.extents()
     for b in range(bs.nblocks):
@@ -118,7 +119,7 @@ def test_linear_field_exact_across_levels(name):
                 for i in range(int(E[0])):
                     loc = (i, j, k)
                     off = 5 * int(r["cell_base"]) + (k * int(E[1]) + j) * int(E[0]) + i
-                    if off not in set_dst(ops):
+                    if off not in dst:
                         continue
                     xc = [r["xmin"][d] + (loc[d] - g + 0.5) * r["dx"][d] if d < nd else 0.5
                           for d in range(3)]
@@ -131,17 +132,6 @@ def test_linear_field_exact_across_levels(name):
                         assert view[n, k, j, i] == pytest.approx(want, rel=1e-13, abs=1e-13)
                     checked += 1
     assert checked > 20
-
-
-_DST = {}
-
-
-def set_dst(ops):
-    key = id(ops)
-    if key not in _DST:
-        _DST.clear()
-        _DST[key] = set(int(v) for v in ops["dst"])
-    return _DST[key]
 
 
 def _problem(m, recon="plm", riemann="hllc", seed=3, amp=0.1, uniform=False):

@@ -54,6 +54,9 @@ class TwoLevelMesh:
         if ndim not in (2, 3):
             raise ValueError("two-level meshes are 2D or 3D")
         self.ndim = ndim
+        self.xmin = tuple(float(v) for v in xmin)
+        self.xmax = tuple(float(v) for v in xmax)
+        self.mesh_id = int(mesh_id)
         self.nbase = tuple(int(v) for v in nbase) + (1,) * (3 - len(nbase))
         self.nx = tuple(int(v) for v in block_nx) + (1,) * (3 - len(block_nx))
         self.ng = int(ng)
@@ -331,3 +334,114 @@ class AmrPlan:
                 _lib.lib().aph_amr_plan_destroy(self.ctx.h, self.h)
         except Exception:
             pass
+
+
+def regrid(mesh: TwoLevelMesh, bs: BlockSet, cons, tags):
+    """One regrid of a two-level mesh from per-block tags (-1 / 0 / 1, as
+    aph_refinement_tag returns them): a level-0 block tagged 1 is refined, a level-1
+    sibling group whose 2^ndim blocks are all tagged -1 is merged back; nothing else
+    changes (two levels: level-1 blocks do not refine further).  Returns
+    (new_mesh, new_bs, new_cons) with the interiors remapped on the host:
+    * kept blocks: copied;
+    * merged groups: the mean of each coarse cell's children, (dz, dy, dx) order;
+    * new children: c + sum_d (sign_d / 4) minmod(c - c_-d, c_+d - c) of their parent's
+      cells, the neighbours taken at level 0 (a cell, or the mean of its children).
+    Ghost cells are left to the next ghost fill.  Totals are conserved to round-off and
+    refining then merging returns the coarse data."""
+    nd, nx = mesh.ndim, mesh.nx
+    nv = bs.nvar
+    tags = np.asarray(tags)
+    old_ref = {tuple(int(v) // 2 for v in mesh.loc[b])[:3] for b in range(bs.nblocks)
+               if mesh.level[b] == 1}
+    old_ref = {t if nd == 3 else (t[0], t[1], 0) for t in old_ref}
+    groups = {}
+    for b in range(bs.nblocks):
+        if mesh.level[b] == 1:
+            key = tuple(int(mesh.loc[b][d]) // 2 if d < nd else 0 for d in range(3))
+            groups.setdefault(key, []).append(int(tags[b]))
+    new_ref = set(old_ref)
+    for key, ts in groups.items():
+        if len(ts) == 2 ** nd and all(t == -1 for t in ts):
+            new_ref.discard(key)
+    for b in range(bs.nblocks):
+        if mesh.level[b] == 0 and tags[b] == 1:
+            new_ref.add(tuple(int(mesh.loc[b][d]) if d < nd else 0 for d in range(3)))
+    refined = [t[:nd] for t in sorted(new_ref)]
+    new = TwoLevelMesh(nd, mesh.nbase[:nd], nx[:nd], mesh.ng, mesh.xmin, mesh.xmax, refined,
+                       mesh_id=mesh.mesh_id)
+    nbs = new.block_set(nscalars=bs.nscalars)
+    out = np.zeros(nbs.field_size(nv))
+    # level-0 values over the whole domain (refined regions: the mean of their children)
+    shape0 = tuple(mesh.ncells_axis(0, d) for d in (2, 1, 0))
+    coarse = np.zeros((nv,) + shape0)
+    fine = {}
+    for b in range(bs.nblocks):
+        lv, lc = int(mesh.level[b]), mesh.loc[b]
+        v = bs.interior(cons, b, nv)
+        if lv == 0:
+            sl = tuple(slice(int(lc[d]) * nx[d], (int(lc[d]) + 1) * nx[d]) for d in (2, 1, 0))
+            coarse[(slice(None),) + sl] = v
+        else:
+            fine[tuple(int(x) for x in lc)] = v
+    for key in old_ref:
+        kids = []
+        for dz in ((0, 1) if nd == 3 else (0,)):
+            for dy in (0, 1):
+                for dxx in (0, 1):
+                    k = (2 * key[0] + dxx, 2 * key[1] + dy, (2 * key[2] + dz) if nd == 3 else 0)
+                    kids.append(fine[k])
+        # the parent's cells: children of parent cell (i, j, k) are (2i+dx, 2j+dy, 2k+dz)
+        # across the 2^nd child blocks; assemble the 2n fine cells first
+        fz = 2 * nx[2] if nd == 3 else 1
+        full = np.zeros((nv, fz, 2 * nx[1], 2 * nx[0]))
+        idx = 0
+        for dz in ((0, 1) if nd == 3 else (0,)):
+            for dy in (0, 1):
+                for dxx in (0, 1):
+                    zs = slice(dz * nx[2], (dz + 1) * nx[2]) if nd == 3 else slice(0, 1)
+                    full[:, zs, dy * nx[1]:(dy + 1) * nx[1], dxx * nx[0]:(dxx + 1) * nx[0]] = kids[idx]
+                    idx += 1
+        parts = []
+        for dz in ((0, 1) if nd == 3 else (0,)):
+            for dy in (0, 1):
+                for dxx in (0, 1):
+                    parts.append(full[:, dz::2, dy::2, dxx::2] if nd == 3 else full[:, :, dy::2, dxx::2])
+        s = parts[0].copy()
+        for q in parts[1:]:
+            s = s + q
+        s = s * (1.0 / len(parts))
+        sl = tuple(slice(key[d] * nx[d], (key[d] + 1) * nx[d]) if d < nd else slice(0, 1)
+                   for d in (2, 1, 0))
+        coarse[(slice(None),) + sl] = s
+    N0 = [mesh.ncells_axis(0, d) for d in range(3)]
+
+    def sgn(x):
+        return np.where(x < 0.0, -1.0, 1.0)
+
+    for b in range(nbs.nblocks):
+        lv, lc = int(new.level[b]), new.loc[b]
+        dst = nbs.interior(out, b, nv)
+        if lv == 0:
+            sl = tuple(slice(int(lc[d]) * nx[d], (int(lc[d]) + 1) * nx[d]) for d in (2, 1, 0))
+            dst[:] = coarse[(slice(None),) + sl]
+            continue
+        key = tuple(int(lc[d]) for d in range(3))
+        if key in fine:  # a level-1 block of the old mesh, kept
+            dst[:] = fine[key]
+            continue
+        # a new child: prolongate from level 0
+        G = np.meshgrid(*[int(lc[d]) * nx[d] + np.arange(nx[d]) if d < nd else np.zeros(1, int)
+                          for d in (2, 1, 0)], indexing="ij")
+        G = [G[2], G[1], G[0]]  # x, y, z index arrays, shape (nz, ny, nx)
+        I = [G[d] // 2 if d < nd else G[d] for d in range(3)]
+        at = lambda J: coarse[(slice(None),) + tuple(J[d] % N0[d] for d in (2, 1, 0))]
+        c = at(I)
+        val = c.copy()
+        for d in range(nd):
+            lo = [I[t] - (1 if t == d else 0) for t in range(3)]
+            hi = [I[t] + (1 if t == d else 0) for t in range(3)]
+            a, bb = c - at(lo), at(hi) - c
+            g = (0.5 * (sgn(a) + sgn(bb))) * np.minimum(np.abs(a), np.abs(bb))
+            val = val + (np.where(G[d] % 2 == 0, -1.0, 1.0) * 0.25) * g
+        dst[:] = val
+    return new, nbs, out

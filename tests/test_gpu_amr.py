@@ -169,3 +169,42 @@ def test_refinement_tags_match_oracle(gpu_ctx, ndim):
         assert np.array_equal(vals.cpu().numpy(), want_v), refine["type"]
         assert np.array_equal(tags.cpu().numpy(), want_t), refine["type"]
         assert len(set(want_t.tolist())) >= 2, (refine["type"], want_t)
+
+
+def test_adaptive_cycle_matches_oracle(gpu_ctx):
+    """Three cycles of: two plain steps on the two-level mesh, aph_refinement_tag on the
+    device, regrid on the host, new plans -- the same bits as the oracle's cycle."""
+    from apophis_amd import problems
+    print("[case] adaptive cycle", flush=True)
+    p, m = problems.blast2d_two_level(nbase=6, n=8, nref=2, recon="plm")
+    ref_p, ref_m = p, m
+    ref = ao.AmrState(ref_p, ref_m)
+    pin = {"parthenon/time": {"cfl": p.cfl, "integrator": "rk2"},
+           "parthenon/mesh": {"nghost": m.ng},
+           "hydro": {"reconstruction": p.recon, "riemann": p.riemann},
+           "eos": {"gm1": float(p.eos["gm1"][0])},
+           "refinement": {"type": "pressure_gradient", "threshold_pressure_gradient": 0.5}}
+    pkg = hydro.initialize_hydro(pin)
+    cons = p.cons
+    dt = ref.estimate_dt()
+    for cycle in range(3):
+        md = hydro.MeshData(gpu_ctx, p.bs, cons)
+        plan = amr.AmrPlan(gpu_ctx, md.dbs, m, p.bs)
+        drv = hydro.ApophisDriver(pkg, md, amr=plan)
+        assert drv.initialize() == ref.estimate_dt()
+        for _ in range(2):
+            est = drv.step(dt)
+            assert est == ref.step(dt)
+            dt = min(dt, est)
+        tags, _ = hydro.refinement_tag(md, pkg)
+        want, _ = ao.refinement_tag(ref_p.bs, ref.prim, 1, 0.5, 0.0)
+        got_u = md.cons.cpu().numpy()
+        assert np.array_equal(got_u, ref.u0)
+        assert np.array_equal(tags.cpu().numpy(), want)
+        m, nbs, cons = amr.regrid(m, p.bs, got_u, tags.cpu().numpy())
+        p = problems.Problem("adaptive", nbs, cons, p.eos, p.recon, p.riemann, p.cfl)
+        ref_m, rbs, rc = amr.regrid(ref_m, ref_p.bs, ref.u0, want)
+        ref_p = problems.Problem("adaptive", rbs, rc, p.eos, p.recon, p.riemann, p.cfl)
+        ref = ao.AmrState(ref_p, ref_m)
+        del drv, plan, md
+    assert (m.level == 1).sum() > 0

@@ -12,14 +12,14 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "_build")
-SOURCES = ["aph_fused.hip", "aph_fused_wide.hip", "aph_adjoint.hip", "aph_plain.hip",
-           "aph_runtime.hip", "aph_amr.hip"]
+SOURCES = ["aph_fused.hip", "aph_fused_wide.hip", "aph_fused_amr.hip", "aph_adjoint.hip",
+           "aph_plain.hip", "aph_runtime.hip", "aph_amr.hip"]
 ARCH = os.environ.get("APH_OFFLOAD_ARCH", "gfx950")
 FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
-def build(verbose=False, jobs=5):
+def build(verbose=False, jobs=7):
     os.makedirs(OUT, exist_ok=True)
     objs, procs = [], []
     for s in SOURCES:

@@ -253,7 +253,9 @@ int build_items(aph_context *ctx, aph_plan *p) {
     return set_error(ctx, APH_E_INVALID, "tile too large for LDS (%zu bytes)", p->lds);
   p->tiles.assign(tl.begin(), tl.end());
   // persistent grid: resident workgroups, a multiple of the XCD count
-  KernelFn fn = pick_kernel(p->recon, p->rsolver, bs->nvar, ndim);
+  KernelFn fn = p->amr ? pick_kernel_amr(p->recon, p->rsolver, bs->nvar, ndim)
+                       : pick_kernel(p->recon, p->rsolver, bs->nvar, ndim);
+  if (!fn) return set_error(ctx, APH_E_INVALID, "no fused kernel for this plan");
   int dev = ctx->device, ncu = 0, occ = 0;
   APH_HIP_CHECK(ctx, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
   APH_HIP_CHECK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(
@@ -318,7 +320,8 @@ int launch_kernel(aph_context *ctx, aph_plan *p, int recon, int stage, int nst, 
     return set_error(ctx, APH_E_INVALID,
                      "fused form: the block set has faces to another refinement level; "
                      "call aph_plan_set_amr first");
-  KernelFn fn = pick_kernel(recon, p->rsolver, p->bs->nvar, p->bs->ndim);
+  KernelFn fn = p->amr ? pick_kernel_amr(recon, p->rsolver, p->bs->nvar, p->bs->ndim)
+                       : pick_kernel(recon, p->rsolver, p->bs->nvar, p->bs->ndim);
   if (!fn) return set_error(ctx, APH_E_INVALID, "no fused kernel for this stage");
   FusedArgs a = make_args(p, stage, nst, gam0, gam1, beta, u_in, u_n, u_out, dt_dev, cfl);
   if (!p->amr) {
@@ -476,6 +479,9 @@ int aph_plan_set_amr(aph_context *ctx, aph_plan *p, const aph_amr_plan *amr,
   const aph_blockset *bs = p->bs;
   const int nd = bs->ndim, nv = bs->nvar;
   if (nd < 2) return set_error(ctx, APH_E_INVALID, "aph_plan_set_amr: ndim must be 2 or 3");
+  if (!pick_kernel_amr(p->recon, p->rsolver, nv, nd))
+    return set_error(ctx, APH_E_INVALID, "aph_plan_set_amr: the fused form runs two-level "
+                     "meshes with nvar 5 or 6 (this block set has %d)", nv);
   const int64_t nch = (nd == 3) ? 4 : 2;
   for (int b = 0; b < bs->nblocks; ++b) {
     if (level[b] != 0 && level[b] != 1)

@@ -94,7 +94,7 @@ __device__ __forceinline__ bool load_cell(const FusedArgs &a, const aph_block &b
   return flip;
 }
 
-template <int RECON, int RS, int NV, int NDIM, int D>
+template <int RECON, int RS, int NV, int NDIM, int D, bool AMR>
 __device__ __forceinline__ void sweep(const FusedArgs &a, const aph_block &bk, int bidx,
                                       const TileCfg &cfg, const int *t0, double *du_s,
                                       double &dtmin) {
@@ -156,7 +156,7 @@ __device__ __forceinline__ void sweep(const FusedArgs &a, const aph_block &bk, i
     auto emit = [&](int f, const double *fo_in, const double *, const double *) {
       const double *fo = fo_in;
       double fc[NV];
-      if (a.amr_face && (f == 0 || f == n_d)) {
+      if (AMR && (f == 0 || f == n_d)) {
         // a block face to the other level: the fine side records its flux, the coarse
         // side takes the mean of the fine fluxes across (flux correction)
         const aph_amr_face &af = a.amr_face[bidx * 6 + 2 * D + (f == 0 ? 0 : 1)];
@@ -259,7 +259,9 @@ __device__ __forceinline__ void sweep(const FusedArgs &a, const aph_block &bk, i
   }
 }
 
-template <int RECON, int RS, int NV, int NDIM>
+// AMR: the two-level-mesh variant (interface fluxes recorded / corrected at block faces
+// to the other level); the single-level kernel carries none of that code.
+template <int RECON, int RS, int NV, int NDIM, bool AMR = false>
 __global__ void __launch_bounds__(64) k_fused(FusedArgs a) {
   extern __shared__ double du_s[];
   double dtmin = 1.79769313486231570815e308;
@@ -271,14 +273,14 @@ __global__ void __launch_bounds__(64) k_fused(FusedArgs a) {
     const aph_block &bk = a.blocks[b];
     const TileCfg cfg = a.cfg[cls];
     const int t0[3] = {item.y, item.z, item.w};
-    sweep<RECON, RS, NV, NDIM, 0>(a, bk, b, cfg, t0, du_s, dtmin);
+    sweep<RECON, RS, NV, NDIM, 0, AMR>(a, bk, b, cfg, t0, du_s, dtmin);
     if constexpr (NDIM >= 2) {
       __syncthreads();
-      sweep<RECON, RS, NV, NDIM, 1>(a, bk, b, cfg, t0, du_s, dtmin);
+      sweep<RECON, RS, NV, NDIM, 1, AMR>(a, bk, b, cfg, t0, du_s, dtmin);
     }
     if constexpr (NDIM >= 3) {
       __syncthreads();
-      sweep<RECON, RS, NV, NDIM, 2>(a, bk, b, cfg, t0, du_s, dtmin);
+      sweep<RECON, RS, NV, NDIM, 2, AMR>(a, bk, b, cfg, t0, du_s, dtmin);
     }
     __syncthreads();
   }
@@ -324,6 +326,8 @@ using PlanDtFn = void (*)(const aph_block *, aph_eos, const double *, double, do
 // The stage kernel / initial time-step kernel for NV = 12 or 14 (aph_fused_wide.hip);
 // null for an unsupported combination.
 KernelFn pick_kernel_wide(int recon, int rs, int nv, int ndim);
+// The two-level-mesh stage kernel, NV = 5 or 6, ndim 2 or 3 (aph_fused_amr.hip).
+KernelFn pick_kernel_amr(int recon, int rs, int nv, int ndim);
 PlanDtFn pick_plan_dt_wide(int nv, int ndim);
 
 }  // namespace aph

@@ -1057,6 +1057,7 @@ int adj_stage(aph_context *ctx, const aph_blockset *bs, int recon, int rs, const
               const double *g, int grav, AdjScratch &s, int stage, const double *u_in,
               const double *dt_dev, const double *lam_out, const double *lam_acc,
               double *lam_in, double *lam_n) {
+  if (bs->nblocks == 0) return APH_OK;
   int rc = adj_prepare(ctx, bs, recon, s);
   if (rc) return rc;
   const int nd = bs->ndim;

@@ -689,7 +689,9 @@ int aph_fused_stage(aph_context *ctx, aph_plan *p, int32_t stage, double *u_in,
   if (!ctx) return APH_E_INVALID;
   if (!p) return set_error(ctx, APH_E_INVALID, "aph_fused_stage: null plan");
   const int nst = plan_nstages(p);
-  if (!u_in || !u_out || !dt_dev || stage < 1 || stage > nst || (stage > 1 && !u_n))
+  const bool empty = p->bs->nblocks == 0;  // an empty block set: state pointers may be null
+  if ((!empty && (!u_in || !u_out || (stage > 1 && !u_n))) || !dt_dev || stage < 1 ||
+      stage > nst)
     return set_error(ctx, APH_E_INVALID, "aph_fused_stage: bad arguments");
   if (u_in == u_out) return set_error(ctx, APH_E_INVALID, "u_out must not alias u_in");
   if (stage == 1) {  // the step's dt estimate starts here
@@ -702,7 +704,8 @@ int aph_fused_stage(aph_context *ctx, aph_plan *p, int32_t stage, double *u_in,
 int aph_plan_step(aph_context *ctx, aph_plan *p, double *u0, double *u1, double *dt_dev,
                   double *time_dev, double cfl, double tlim, int32_t fixed_dt) {
   if (!ctx) return APH_E_INVALID;
-  if (!p || !u0 || !u1 || !dt_dev || !time_dev || u0 == u1)
+  if (!p || !dt_dev || !time_dev ||
+      (p->bs->nblocks > 0 && (!u0 || !u1 || u0 == u1)))  // empty set: states may be null
     return set_error(ctx, APH_E_INVALID, "aph_plan_step: bad arguments");
   k_fill<<<1, 1, 0, ctx->stream>>>(p->d_est, 1.79769313486231570815e308);
   APH_LAUNCH_CHECK(ctx);
@@ -780,7 +783,7 @@ int aph_plan_graph_launch(aph_context *ctx, aph_plan *p) {
 int aph_plan_estimate_timestep(aph_context *ctx, aph_plan *p, const double *cons, double cfl,
                                double *dt_out) {
   if (!ctx) return APH_E_INVALID;
-  if (!p || !cons || !dt_out)
+  if (!p || !dt_out || (!cons && p->bs->nblocks > 0))
     return set_error(ctx, APH_E_INVALID, "aph_plan_estimate_timestep: null");
   const aph_blockset *bs = p->bs;
   k_fill<<<1, 1, 0, ctx->stream>>>(dt_out, 1.79769313486231570815e308);

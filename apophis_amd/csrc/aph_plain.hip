@@ -357,9 +357,9 @@ int aph_cons_to_prim(aph_context *ctx, const aph_blockset *bs, int32_t first_blo
   int lo, cnt;
   int rc = block_range(ctx, bs, first_block, nblocks, &lo, &cnt);
   if (rc) return rc;
+  if (cnt == 0) return APH_OK;  // an empty range: nothing to read or write
   if (!eos || !cons || !prim || !gamma)
     return set_error(ctx, APH_E_INVALID, "aph_cons_to_prim: null pointer");
-  if (cnt == 0) return APH_OK;
   dim3 gr = grid_for(bs->max_ncell, kThreads, cnt, 256);
   APH_NV_DISPATCH(bs->nvar, (k_cons_to_prim<NV><<<gr, kThreads, 0, ctx->stream>>>(
                                 bs->d, lo, bs->ndim, *eos, cons, prim, gamma)));
@@ -375,6 +375,7 @@ int aph_calculate_fluxes(aph_context *ctx, const aph_blockset *bs, int32_t first
   int lo, cnt;
   int rc = block_range(ctx, bs, first_block, nblocks, &lo, &cnt);
   if (rc) return rc;
+  if (cnt == 0) return APH_OK;  // an empty range: nothing to read or write
   if (!prim || !gamma || !flux_x1 || (bs->ndim >= 2 && !flux_x2) ||
       (bs->ndim >= 3 && !flux_x3))
     return set_error(ctx, APH_E_INVALID, "aph_calculate_fluxes: null pointer");
@@ -384,7 +385,6 @@ int aph_calculate_fluxes(aph_context *ctx, const aph_blockset *bs, int32_t first
       return set_error(ctx, APH_E_INVALID,
                        "Not enough ghost zones for reconstruction (block %d has %d, need %d)",
                        b, bs->h[b].ng, need);
-  if (cnt == 0) return APH_OK;
 #define APH_FLUX_CASE(R, S)                                                            \
   if (recon == R && rsolver == S)                                                      \
     return launch_flux_nv<R, S>(ctx, bs, lo, cnt, prim, gamma, flux_x1, flux_x2, flux_x3);
@@ -412,9 +412,9 @@ int aph_update_with_flux_divergence(aph_context *ctx, const aph_blockset *bs,
   int lo, cnt;
   int rc = block_range(ctx, bs, first_block, nblocks, &lo, &cnt);
   if (rc) return rc;
+  if (cnt == 0) return APH_OK;  // an empty range: nothing to read or write
   if (!u0 || !u1 || !flux_x1 || (bs->ndim >= 2 && !flux_x2) || (bs->ndim >= 3 && !flux_x3))
     return set_error(ctx, APH_E_INVALID, "aph_update_with_flux_divergence: null pointer");
-  if (cnt == 0) return APH_OK;
   dim3 gr = grid_for(bs->max_interior, kThreads, cnt, 256);
   APH_NV_DISPATCH(bs->nvar, (k_update<NV><<<gr, kThreads, 0, ctx->stream>>>(
                                 bs->d, lo, bs->ndim, u0, u1, flux_x1, flux_x2, flux_x3, gam0,
@@ -430,8 +430,8 @@ int aph_apply_gravity(aph_context *ctx, const aph_blockset *bs, int32_t first_bl
   int lo, cnt;
   int rc = block_range(ctx, bs, first_block, nblocks, &lo, &cnt);
   if (rc) return rc;
+  if (cnt == 0) return APH_OK;  // an empty range: nothing to read or write
   if (!cons || !prim || !g) return set_error(ctx, APH_E_INVALID, "aph_apply_gravity: null");
-  if (cnt == 0) return APH_OK;
   dim3 gr = grid_for(bs->max_interior, kThreads, cnt, 256);
   APH_NV_DISPATCH(bs->nvar, (k_gravity<NV><<<gr, kThreads, 0, ctx->stream>>>(
                                 bs->d, lo, bs->ndim, cons, prim, g[0], g[1], g[2], dt)));
@@ -445,8 +445,8 @@ int aph_fill_ghosts(aph_context *ctx, const aph_blockset *bs, int32_t first_bloc
   int lo, cnt;
   int rc = block_range(ctx, bs, first_block, nblocks, &lo, &cnt);
   if (rc) return rc;
+  if (cnt == 0) return APH_OK;  // an empty range: nothing to read or write
   if (!cons) return set_error(ctx, APH_E_INVALID, "aph_fill_ghosts: null");
-  if (cnt == 0) return APH_OK;
   dim3 gr = grid_for(bs->max_ncell, kThreads, cnt, 256);
   const int nd = bs->ndim;
   int64_t slab = 0;
@@ -471,7 +471,8 @@ int aph_estimate_timestep(aph_context *ctx, const aph_blockset *bs, int32_t firs
   int lo, cnt;
   int rc = block_range(ctx, bs, first_block, nblocks, &lo, &cnt);
   if (rc) return rc;
-  if (!eos || !prim || !cons || !dt_out)
+  if (!dt_out) return set_error(ctx, APH_E_INVALID, "aph_estimate_timestep: null pointer");
+  if (cnt > 0 && (!eos || !prim || !cons))
     return set_error(ctx, APH_E_INVALID, "aph_estimate_timestep: null pointer");
   k_set<<<1, 1, 0, ctx->stream>>>(dt_out, 1.79769313486231570815e308);
   if (cnt > 0) {
@@ -486,7 +487,7 @@ int aph_estimate_timestep(aph_context *ctx, const aph_blockset *bs, int32_t firs
 int aph_global_sum(aph_context *ctx, const aph_blockset *bs, const double *cons,
                    int32_t var, double *out) {
   if (!ctx) return APH_E_INVALID;
-  if (!bs || !cons || !out || var < 0 || var >= bs->nvar)
+  if (!bs || !out || var < 0 || var >= bs->nvar || (!cons && bs->nblocks > 0))
     return set_error(ctx, APH_E_INVALID, "aph_global_sum: bad arguments");
   if (bs->nblocks == 0) {
     k_set<<<1, 1, 0, ctx->stream>>>(out, 0.0);

@@ -386,8 +386,9 @@ class ApophisDriver:
         self.g = tuple(float(v) for v in g)
         self.u1 = torch.zeros_like(md.cons)
         nb = md.bs.nblocks
-        ps = pack_size or nb
-        self.parts = [md.pack(s, min(ps, nb - s)) for s in range(0, nb, ps)]
+        ps = pack_size or max(nb, 1)
+        # an empty block set is one empty partition
+        self.parts = [md.pack(s, min(ps, nb - s)) for s in range(0, nb, ps)] or [md.pack(0, 0)]
         self.dt_dev = torch.empty(1, dtype=torch.float64, device=md.cons.device)
         self.rk = INTEGRATORS[pkg.param("integrator").name]
 

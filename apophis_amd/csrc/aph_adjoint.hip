@@ -511,13 +511,16 @@ __global__ void __launch_bounds__(64) k_adj_recon(AdjArgs a) {
     v = prim[(int64_t)NF * base + (int64_t)idx * L.sd];
     ng = flip && vel;
   };
-  auto emit = [&](int c, double v) {
-    if (c >= a0 && c < a0 + S) {
+  // y and z add into Pbar.  Each cell's value has one owner lane here, so the march loads
+  // it a step before it adds (pre), keeping the load off the step's critical path.
+  auto own = [&](int c) { return c >= a0 && c < a0 + S; };
+  auto emit = [&](int c, double v, const double *pre) {
+    if (own(c)) {
       double *q = pb_ptr + (int64_t)c * Lp.sd;
       if (D == 0)
         *q = v;
       else
-        *q += v;
+        *q = (pre ? *pre : *q) + v;
     } else {
       const bool below = c < a0;
       const int slot = below ? c - (a0 - 3) : 3 + (c - (a0 + S));
@@ -532,9 +535,9 @@ __global__ void __launch_bounds__(64) k_adj_recon(AdjArgs a) {
     const int k = __builtin_amdgcn_frexp_exp(x);
     return (k >= -899 && k <= 1000) ? 1u : 0u;
   };
-  auto face = [](const double *c, unsigned pl, PpmFaceTape &t) -> double {
-    return ((pl >> 1) & 0xFu) == 0xFu ? ppm_face_tape<true>(c[1], c[2], c[3], c[4], t)
-                                       : ppm_face_tape<false>(c[1], c[2], c[3], c[4], t);
+  auto face = [](const double *c, unsigned pl) -> PpmFace {
+    return ((pl >> 1) & 0xFu) == 0xFu ? ppm_face_taped<true>(c[1], c[2], c[3], c[4])
+                                       : ppm_face_taped<false>(c[1], c[2], c[3], c[4]);
   };
   double col[W];  // col[k] = field value of cell mm-GR+k
   double cw[W];   // cw[k] = cotangent of cell mm-GR+k
@@ -546,15 +549,18 @@ __global__ void __launch_bounds__(64) k_adj_recon(AdjArgs a) {
   }
 #pragma unroll
   for (int s = 0; s < W; ++s) cw[s] = 0.0;
-  PpmFaceTape tlo;                         // face mm-1/2 (cells mm-2 .. mm+1)
-  double flo = face(col, plain, tlo);
+  PpmFace flo = face(col, plain);          // face mm-1/2 (cells mm-2 .. mm+1)
   double fb_lo = 0.0;                      // its cotangent from cell mm-1 (this lane's)
   double nv, nql, nqr;
   bool nng;
   fetch(m_lo + GR, nv, nng);
   nql = (m_lo <= n_d - 1) ? qlb_ptr[(int64_t)m_lo * L.sd] : 0.0;
   nqr = (m_lo >= 0) ? qrb_ptr[(int64_t)m_lo * L.sd] : 0.0;
+  double npb = 0.0;  // Pbar of the cell the next step emits (y, z)
   for (int mm = m_lo; mm <= m_hi; ++mm) {
+    const double pb_cur = npb;  // cell mm - GR
+    // (the first step's cell, m_lo - GR, is never the segment's own: no load needed)
+    if (D > 0 && own(mm + 1 - GR)) npb = pb_ptr[(int64_t)(mm + 1 - GR) * Lp.sd];
 #pragma unroll
     for (int s = 0; s < W - 1; ++s) col[s] = col[s + 1];
     col[W - 1] = nng ? -1.0 * nv : nv;
@@ -565,27 +571,25 @@ __global__ void __launch_bounds__(64) k_adj_recon(AdjArgs a) {
       nqr = qrb_ptr[(int64_t)(mm + 1) * L.sd];  // mm + 1 >= 0
     }
     plain = (plain >> 1) | (plain1(col[W - 1]) << (W - 1));
-    PpmFaceTape thi;                       // face mm+1/2 (cells mm-1 .. mm+2)
-    const double fhi = face(col, plain, thi);
+    const PpmFace fhi = face(col, plain);  // face mm+1/2 (cells mm-1 .. mm+2)
     double colb[W];
 #pragma unroll
     for (int s = 0; s < W; ++s) colb[s] = 0.0;
     double flob, fhib;
-    ppm_cell_adj(col, flo, fhi, ql_b, qr_b, colb, flob, fhib);
-    ppm_face_adj(tlo, fb_lo + flob, colb);  // face mm-1/2 -> cells mm-2 .. mm+1
+    ppm_cell_adj(col, flo.dph, fhi.dph, ql_b, qr_b, colb, flob, fhib);
+    ppm_face_adj(flo.t, fb_lo + flob, colb);  // face mm-1/2 -> cells mm-2 .. mm+1
     fb_lo = fhib;
     flo = fhi;
-    tlo = thi;
 #pragma unroll
     for (int s = 0; s < W; ++s) cw[s] += colb[s];
-    emit(mm - GR, cw[0]);  // no later cell or face reaches mm-GR
+    emit(mm - GR, cw[0], &pb_cur);  // no later cell or face reaches mm-GR
 #pragma unroll
     for (int s = 0; s < W - 1; ++s) cw[s] = cw[s + 1];
     cw[W - 1] = 0.0;
   }
-  ppm_face_adj(tlo, fb_lo, cw);  // face m_hi+1/2 -> cells m_hi-1 .. m_hi+2 = cw[0..3]
+  ppm_face_adj(flo.t, fb_lo, cw);  // face m_hi+1/2 -> cells m_hi-1 .. m_hi+2 = cw[0..3]
 #pragma unroll
-  for (int s = 0; s < W - 1; ++s) emit(m_hi - GR + 1 + s, cw[s]);
+  for (int s = 0; s < W - 1; ++s) emit(m_hi - GR + 1 + s, cw[s], nullptr);
 }
 
 // One-march form (PLM) -------------------------------------------------------------
@@ -648,7 +652,17 @@ __global__ void __launch_bounds__(64) k_adj_axis(AdjArgs a) {
     cons_to_prim_pt<NV>(uu, a.eos, dst, dst + NV);
   };
   // final primitive-space cotangent of cell c from this segment
-  auto emit = [&](int c, const double *pbv) {
+  // y and z add into Pbar; every cell has one owner lane, so the value of the next cell
+  // to emit is loaded right after the previous emit (pre), a step before it is needed
+  double npb[NF];
+  auto prefetch = [&](int c) {
+    if (D > 0 && c >= a0 && c < a0 + S) {
+      const double *q = pb_ptr + (int64_t)c * st[D];
+#pragma unroll
+      for (int f = 0; f < NF; ++f) npb[f] = q[f * nc];
+    }
+  };
+  auto emit = [&](int c, const double *pbv, bool pre) {
     if (c >= a0 && c < a0 + S) {
       double *q = pb_ptr + (int64_t)c * st[D];
 #pragma unroll
@@ -656,7 +670,7 @@ __global__ void __launch_bounds__(64) k_adj_axis(AdjArgs a) {
         if (D == 0)
           q[f * nc] = pbv[f];
         else
-          q[f * nc] += pbv[f];
+          q[f * nc] = (pre ? npb[f] : q[f * nc]) + pbv[f];
       }
     } else {
       const bool below = c < a0;
@@ -769,8 +783,9 @@ __global__ void __launch_bounds__(64) k_adj_axis(AdjArgs a) {
         for (int s = 0; s < W; ++s) cw[s][fld] += colb[s];
       }
       // cell mm-1-GR is complete: emit it and shift the cotangent window
-      emit(emitted, cw[0]);
+      emit(emitted, cw[0], true);
       ++emitted;
+      prefetch(emitted);
 #pragma unroll
       for (int s = 0; s < W - 1; ++s)
 #pragma unroll
@@ -781,6 +796,7 @@ __global__ void __launch_bounds__(64) k_adj_axis(AdjArgs a) {
       // mm == a0-1: no face yet; cells up to mm-1+GR stay pending.  The window of
       // cotangents starts at cell mm-GR from the next step on.
       emitted = mm - GR;
+      prefetch(emitted);
     }
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
@@ -804,7 +820,7 @@ __global__ void __launch_bounds__(64) k_adj_axis(AdjArgs a) {
       for (int s = 0; s < W; ++s) cw[s][fld] += colb[s];
     }
 #pragma unroll
-    for (int s = 0; s < W; ++s) emit(emitted + s, cw[s]);
+    for (int s = 0; s < W; ++s) emit(emitted + s, cw[s], s == 0);
   }
 }
 

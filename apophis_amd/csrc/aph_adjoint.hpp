@@ -234,10 +234,16 @@ struct PpmFaceTape {
   bool sgn;  // the three signs agreed (qd != 0)
 };
 
-// ppm_face of cells (q_im2 .. q_ip1), bitwise, recording its branches.
+// ppm_face of cells (q_im2 .. q_ip1), bitwise, recording its branches (returned with
+// the value, so the tape stays in registers).
+struct PpmFace {
+  double dph;
+  PpmFaceTape t;
+};
 template <bool FAST>
-APH_HD double ppm_face_tape(double q_im2, double q_im1, double q_i, double q_ip1,
-                            PpmFaceTape &t) {
+APH_HD PpmFace ppm_face_taped(double q_im2, double q_im1, double q_i, double q_ip1) {
+  PpmFace r;
+  PpmFaceTape &t = r.t;
   const double C2 = 1.25;
   const double qa0 = (q_i - q_im1);
   const double qb0 = (q_ip1 - q_i);
@@ -257,7 +263,16 @@ APH_HD double ppm_face_tape(double q_im2, double q_im1, double q_i, double q_ip1
   const double dph_tmp = avg - (FAST ? div6_fast(qd) : qd / 6.0);
   t.lim = (qa_tmp * qb_tmp < 0.0);
   if (t.lim) dph = dph_tmp;
-  return dph;
+  r.dph = dph;
+  return r;
+}
+
+template <bool FAST>
+APH_HD double ppm_face_tape(double q_im2, double q_im1, double q_i, double q_ip1,
+                            PpmFaceTape &t) {
+  const PpmFace r = ppm_face_taped<FAST>(q_im2, q_im1, q_i, q_ip1);
+  t = r.t;
+  return r.dph;
 }
 
 // Cotangent Fb of a face value -> cells (q_im2 .. q_ip1), added into qb[0..3].
@@ -267,8 +282,18 @@ APH_HD void ppm_face_adj(const PpmFaceTape &t, double Fb, double *qb) {
   if (t.lim) {  // dph = avg - qd/6
     avgb += Fb;
     if (t.sgn) {
-      double ab = 0.0, bb = 0.0, cb = 0.0;
-      ppm_qd_adj(t.qa, t.d2qc_im1, t.d2qc, -Fb * kSixth, ab, bb, cb);
+      // ppm_qd_adj(t.qa, t.d2qc_im1, t.d2qc, -Fb/6, ab, bb, cb) written with selects:
+      // adding into one of three references by branch made the compiler index a stack
+      // array on every face
+      const double C2 = 1.25;
+      const double qdb = -Fb * kSixth;
+      const double sa = SIGN(t.qa);
+      const double ea = fabs(t.qa), eb = C2 * fabs(t.d2qc_im1), ec = C2 * fabs(t.d2qc);
+      const bool inner_is_a = ea < ec;
+      const bool use_inner = (inner_is_a ? ea : ec) < eb;
+      const double ab = (use_inner && inner_is_a) ? qdb * sa * dabs(t.qa) : 0.0;
+      const double cb = (use_inner && !inner_is_a) ? qdb * sa * C2 * dabs(t.d2qc) : 0.0;
+      const double bb = use_inner ? 0.0 : qdb * sa * C2 * dabs(t.d2qc_im1);
       // qa = 3 (q_im1 + q_i - 2 dph0)
       b1 += 3.0 * ab;
       b2 += 3.0 * ab;

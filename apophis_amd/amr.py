@@ -1,9 +1,10 @@
-"""Two-level meshes for the plain form: a 2:1 refinement jump, as Parthenon's AMR produces.
+"""Multilevel meshes for the plain and fused forms: a periodic block tree with 2:1 jumps,
+as Parthenon's AMR produces.
 
 The reference leaves multilevel meshes to Parthenon (apophis_driver.cpp:68,74-76,94):
-* the boundary exchange of `cons` fills a coarse block's ghost cells over a refined region
-  with the mean of the fine children (RestrictAverage) and a fine block's ghost cells over
-  a coarse region by limited linear prolongation (ProlongateSharedMinMod);
+* the boundary exchange of `cons` fills a block's ghost cells over a finer region with the
+  mean of the fine children (RestrictAverage) and over a coarser region by limited linear
+  prolongation (ProlongateSharedMinMod);
 * the flux correction replaces every coarse face flux on a coarse-fine face by the
   area-weighted mean of the fine fluxes across it, before the update, so the levels
   exchange exactly the same fluxes and the total is conserved.
@@ -11,9 +12,11 @@ The reference leaves multilevel meshes to Parthenon (apophis_driver.cpp:68,74-76
 Here the host resolves the mesh tree once into flat op lists (``amr_plan``), which the
 library validates and runs as two gather kernels (``aph_amr_fill``,
 ``aph_amr_flux_correct``).  Same-level neighbours keep using the block table's ``nbr``
-(``aph_fill_ghosts``); a face whose neighbour region is on the other level has nbr = -1 and
-bc = APH_BC_AMR.  Meshes are periodic; each block keeps the same cells per block at every
-level (spacing halves), so a coarse cell's children always lie in one fine block.
+(``aph_fill_ghosts``); a face whose neighbour region is on another level has nbr = -1 and
+bc = APH_BC_AMR.  Leaves that touch differ by at most one level (2:1 balance across
+faces, edges and corners), so every cross-level value is one level away.  Meshes are
+periodic; each block keeps the same cells per block at every level (spacing halves), so a
+coarse cell's children always lie in one fine block.
 
 Prolongation of coarse cell c to the child on the lower / upper side (sign -1 / +1) of axis
 d uses g_d = minmod(c - c_{-d}, c_{+d} - c) and gives c + sum_d sign_d g_d / 4, the
@@ -43,16 +46,24 @@ assert AMR_REF_DTYPE.itemsize == 16 and AMR_OP_DTYPE.itemsize == 88
 assert FLUX_FIX_DTYPE.itemsize == 32
 
 
-class TwoLevelMesh:
-    """Leaf blocks of a periodic mesh whose base lattice ``nbase`` has the blocks at the
-    logical positions ``refined`` replaced by their 2^ndim children (level 1, the same
-    ``block_nx`` cells at half the spacing).  ``blocks`` is a BLOCK_DTYPE table (nbr links
-    same-level leaves, with periodic wrap; faces to the other level carry BC_AMR);
-    ``level[b]`` and ``loc[b]`` give each leaf's level and logical position there."""
+class AmrMesh:
+    """Leaf blocks of a periodic block tree, as Parthenon's AMR builds it: a base lattice
+    ``nbase`` of level-0 blocks, where a refined node at level L is replaced by its 2^ndim
+    children at level L+1 (the same ``block_nx`` cells at half the spacing).  ``refined``
+    lists the refined nodes as (level, loc); it holds every ancestor of its members, and
+    the leaves are 2:1 balanced: leaves that touch (across a face, an edge or a corner,
+    with periodic wrap) differ by at most one level.  ``max_level`` is the deepest level a
+    regrid may create (numlevel - 1).
 
-    def __init__(self, ndim, nbase, block_nx, ng, xmin, xmax, refined, mesh_id=0):
+    ``blocks`` is a BLOCK_DTYPE table, leaves in depth-first order (base lattice z, y, x;
+    children z, y, x): nbr links same-level leaves, faces with no same-level neighbour
+    carry BC_AMR.  ``level[b]``, ``loc[b]`` give each leaf's level and logical position;
+    ``index[(level, loc)]`` is the leaf's block number."""
+
+    def __init__(self, ndim, nbase, block_nx, ng, xmin, xmax, refined, mesh_id=0,
+                 max_level=None):
         if ndim not in (2, 3):
-            raise ValueError("two-level meshes are 2D or 3D")
+            raise ValueError("multilevel meshes are 2D or 3D")
         self.ndim = ndim
         self.xmin = tuple(float(v) for v in xmin)
         self.xmax = tuple(float(v) for v in xmax)
@@ -62,22 +73,39 @@ class TwoLevelMesh:
         self.ng = int(ng)
         if any(self.nx[d] < 2 * self.ng for d in range(ndim)):
             raise ValueError("blocks need at least 2*ng cells per axis")
-        refined = {tuple(int(v) for v in r) + (0,) * (3 - len(r)) for r in refined}
-        span = [float(xmax[d]) - float(xmin[d]) for d in range(3)]
-        leaves = []  # (level, loc)
+        if any(self.nx[d] % 2 for d in range(ndim)):
+            raise ValueError("multilevel meshes need an even number of cells per block axis")
+        self.refined = set()
+        for lv, lc in refined:
+            lc = tuple(int(v) for v in lc) + (0,) * (3 - len(lc))
+            self.refined.add((int(lv), lc))
+        for lv, lc in self.refined:
+            if lv < 0 or any(not 0 <= lc[d] < self.nlat(lv, d) for d in range(3)):
+                raise ValueError(f"refined node {(lv, lc)} is outside the mesh")
+            if lv > 0 and (lv - 1, self.parent(lc)) not in self.refined:
+                raise ValueError(f"refined node {(lv, lc)} has an unrefined parent")
+        deepest = 1 + max((lv for lv, _ in self.refined), default=-1)
+        self.max_level = deepest if max_level is None else int(max_level)
+        if self.max_level < deepest:
+            raise ValueError(f"the tree is {deepest} levels deep, max_level {self.max_level}")
+        leaves = []  # (level, loc), depth first
+
+        def visit(lv, lc):
+            if (lv, lc) not in self.refined:
+                leaves.append((lv, lc))
+                return
+            for c in self.children(lc):
+                visit(lv + 1, c)
+
         for lz in range(self.nbase[2]):
             for ly in range(self.nbase[1]):
                 for lx in range(self.nbase[0]):
-                    if (lx, ly, lz) not in refined:
-                        leaves.append((0, (lx, ly, lz)))
-                        continue
-                    for cz in (range(2) if ndim == 3 else (0,)):
-                        for cy in range(2):
-                            for cx in range(2):
-                                leaves.append((1, (2 * lx + cx, 2 * ly + cy, 2 * lz + cz)))
+                    visit(0, (lx, ly, lz))
         self.level = np.array([lv for lv, _ in leaves], dtype=np.int32)
-        self.loc = np.array([lc for _, lc in leaves], dtype=np.int64)
+        self.loc = np.array([lc for _, lc in leaves], dtype=np.int64).reshape(-1, 3)
         self.index = {(lv, lc): b for b, (lv, lc) in enumerate(leaves)}
+        self._check_balance()
+        span = [float(xmax[d]) - float(xmin[d]) for d in range(3)]
         out = np.zeros(len(leaves), dtype=BLOCK_DTYPE)
         for b, (lv, lc) in enumerate(leaves):
             r = out[b]
@@ -93,11 +121,9 @@ class TwoLevelMesh:
                          for d in range(3)]
             r["mesh_id"] = mesh_id
             nbr = np.full(27, -1, dtype=np.int32)
-            for o in itertools.product((-1, 0, 1), repeat=3):
-                if o == (0, 0, 0) or any(o[d] != 0 for d in range(ndim, 3)):
-                    continue
-                p = tuple((lc[d] + o[d]) % self.nlat(lv, d) for d in range(3))
-                nbr[(o[2] + 1) * 9 + (o[1] + 1) * 3 + (o[0] + 1)] = self.index.get((lv, p), -1)
+            for o in self.offsets():
+                nbr[(o[2] + 1) * 9 + (o[1] + 1) * 3 + (o[0] + 1)] = self.index.get(
+                    (lv, self.shift(lv, lc, o)), -1)
             r["nbr"] = nbr
             bc = [1] * 6  # outflow placeholder on faces with a same-level neighbour
             for d in range(ndim):
@@ -109,15 +135,76 @@ class TwoLevelMesh:
             r["bc"] = bc
         self.blocks = out
 
+    # -- tree geometry
     def nlat(self, level, d):
-        """Blocks per axis d at a level."""
+        """Nodes per axis d at a level."""
         return self.nbase[d] * (2 ** level if d < self.ndim else 1)
 
     def ncells_axis(self, level, d):
         return self.nlat(level, d) * self.nx[d]
 
+    @property
+    def nlevels(self):
+        return int(self.level.max()) + 1 if len(self.level) else 1
+
+    def offsets(self):
+        """The 3^ndim - 1 neighbour offsets."""
+        return [o for o in itertools.product((-1, 0, 1), repeat=3)
+                if o != (0, 0, 0) and all(o[d] == 0 for d in range(self.ndim, 3))]
+
+    def shift(self, level, lc, o):
+        return tuple((lc[d] + o[d]) % self.nlat(level, d) for d in range(3))
+
+    def parent(self, lc):
+        return tuple(lc[d] // 2 if d < self.ndim else 0 for d in range(3))
+
+    def children(self, lc):
+        """The 2^ndim child positions, z, y, x order (x fastest)."""
+        nd = self.ndim
+        return [(2 * lc[0] + cx, 2 * lc[1] + cy, 2 * lc[2] + cz if nd == 3 else 0)
+                for cz in ((0, 1) if nd == 3 else (0,)) for cy in (0, 1) for cx in (0, 1)]
+
+    def is_node(self, level, lc):
+        """Whether (level, lc) is a leaf or a refined node of the tree."""
+        return level == 0 or (level - 1, self.parent(lc)) in self.refined
+
+    def touching_children(self, lc, o):
+        """Children of node lc that touch the node at offset -o from it (the neighbour
+        that sees lc at offset o)."""
+        out = []
+        for c in self.children(lc):
+            if all(o[d] == 0 or (c[d] % 2) == (1 if o[d] < 0 else 0)
+                   for d in range(self.ndim)):
+                out.append(c)
+        return out
+
+    def _check_balance(self):
+        for lv, lc in self.index:
+            for o in self.offsets():
+                p = self.shift(lv, lc, o)
+                if (lv, p) in self.index:
+                    continue
+                if (lv, p) in self.refined:
+                    for c in self.touching_children(p, o):
+                        if (lv + 1, c) in self.refined:
+                            raise ValueError(f"leaf {(lv, lc)} touches level {lv + 2} "
+                                             "(2:1 balance)")
+                    continue
+                if (lv - 1, self.parent(p)) not in self.index:
+                    raise ValueError(f"leaf {(lv, lc)} touches a level below {lv - 1} "
+                                     "(2:1 balance)")
+
     def block_set(self, nscalars=0):
         return BlockSet(self.ndim, [self.blocks], nscalars=nscalars)
+
+
+class TwoLevelMesh(AmrMesh):
+    """An AmrMesh of two levels: the base blocks at the logical positions ``refined`` are
+    replaced by their 2^ndim children (level 1), and regrids stay within the two levels."""
+
+    def __init__(self, ndim, nbase, block_nx, ng, xmin, xmax, refined, mesh_id=0):
+        super().__init__(ndim, nbase, block_nx, ng, xmin, xmax,
+                         [(0, tuple(r)) for r in refined], mesh_id=mesh_id, max_level=1)
 
 
 class _Refs:
@@ -142,8 +229,16 @@ def _cell(bs, b, local, _cache={}):
     return (bs.nvar * int(cb[b]) + c, int(bs.ncell[b]))
 
 
-def amr_plan(mesh: TwoLevelMesh, bs: BlockSet):
-    """(ops, fixes, refs) for ``bs`` (a block set of ``mesh``'s table)."""
+def amr_plan(mesh: AmrMesh, bs: BlockSet):
+    """(ops, fixes, refs) for ``bs`` (a block set of ``mesh``'s table).
+
+    Every ghost cell whose direction has no same-level neighbour gets one op:
+    * over a refined region (the leaves there are one level finer): MEAN of the 2^ndim
+      children of the ghost cell;
+    * over a coarser leaf: PROLONG from the coarse cell holding it, the slope stencil's
+      neighbours taken at the coarse level (a coarse cell, or the mean of its children).
+    Every coarse face cell on a face to a refined region gets one flux fix (the mean of
+    the 2^(ndim-1) fine faces across it).  2:1 balance keeps every value one level away."""
     nd, g, nx = mesh.ndim, mesh.ng, mesh.nx
     refs = _Refs()
     act = range(nd)
@@ -151,29 +246,33 @@ def amr_plan(mesh: TwoLevelMesh, bs: BlockSet):
     def wrap(level, G):
         return tuple(G[d] % mesh.ncells_axis(level, d) if d < nd else 0 for d in range(3))
 
+    def node(G):
+        return tuple(G[d] // nx[d] if d < nd else 0 for d in range(3))
+
     def interior(level, G):
         """(block, ghost-inclusive local index) of the level-`level` leaf holding global
         cell G, or None when that region is not a leaf at this level."""
         G = wrap(level, G)
-        lc = tuple(G[d] // nx[d] if d < nd else 0 for d in range(3))
+        lc = node(G)
         b = mesh.index.get((level, lc))
         if b is None:
             return None
         return b, tuple(G[d] - lc[d] * nx[d] + (g if d < nd else 0) for d in range(3))
 
-    def coarse_value(I):
-        """Level-0 value of global coarse cell I: the cell, or the mean of its children."""
-        hit = interior(0, I)
+    def value(level, I):
+        """Level-`level` value of global cell I: the leaf cell, or the mean of its 2^ndim
+        children, which are leaf cells one level finer."""
+        hit = interior(level, I)
         if hit is not None:
             return refs.add([_cell(bs, *hit)])
+        I = wrap(level, I)
         kids = []
-        for dz in ((0, 1) if nd == 3 else (0,)):
-            for dy in (0, 1):
-                for dx_ in (0, 1):
-                    F = (2 * I[0] + dx_, 2 * I[1] + dy, 2 * I[2] + dz if nd == 3 else 0)
-                    hit = interior(1, F)
-                    assert hit is not None, "two levels only"
-                    kids.append(_cell(bs, *hit))
+        for F in mesh.children(I):
+            hit = interior(level + 1, F)
+            if hit is None:
+                raise ValueError(f"cell {I} of level {level} is neither a leaf cell nor "
+                                 "the parent of leaf cells (2:1 balance)")
+            kids.append(_cell(bs, *hit))
         return refs.add(kids)
 
     ops = []
@@ -194,39 +293,41 @@ def amr_plan(mesh: TwoLevelMesh, bs: BlockSet):
         need = ghost & (nbr[(o3[2] + 1) * 9 + (o3[1] + 1) * 3 + (o3[0] + 1)] < 0)
         for k, j, i in zip(kk[need].tolist(), jj[need].tolist(), ii[need].tolist()):
             loc = (i, j, k)
-            G = tuple(int(lc[d]) * nx[d] + loc[d] - g if d < nd else 0
-                      for d in range(3))
+            Gw = wrap(lv, tuple(int(lc[d]) * nx[d] + loc[d] - g if d < nd else 0
+                                for d in range(3)))
             dst, stride = _cell(bs, b, loc)
             rec = np.zeros(1, dtype=AMR_OP_DTYPE)[0]
             rec["dst"], rec["dst_stride"] = dst, stride
-            if lv == 0:  # coarse ghost over a refined region: restriction
+            if (lv, node(Gw)) in mesh.refined:  # over a finer region: restriction
                 rec["kind"] = AMR_MEAN
-                rec["val"][0] = coarse_value(wrap(0, G))
-            else:  # fine ghost over a coarse region: prolongation
-                Gw = wrap(1, G)
+                rec["val"][0] = value(lv, Gw)
+            else:  # over a coarser leaf: prolongation
                 I = tuple(Gw[d] // 2 if d < nd else 0 for d in range(3))
-                assert interior(0, I) is not None, "two levels only"
+                if lv == 0 or interior(lv - 1, I) is None:
+                    raise ValueError(f"block {b}: ghost cell {loc} is over neither a "
+                                     "finer region nor a coarser leaf (2:1 balance)")
                 rec["kind"] = AMR_PROLONG
-                rec["val"][0] = coarse_value(I)
+                rec["val"][0] = value(lv - 1, I)
                 for d in act:
                     e = [0, 0, 0]
                     e[d] = 1
                     rec["sign"][d] = -1 if Gw[d] % 2 == 0 else 1
-                    rec["val"][1 + 2 * d] = coarse_value(
-                        tuple(I[t] - e[t] for t in range(3)))
-                    rec["val"][2 + 2 * d] = coarse_value(
-                        tuple(I[t] + e[t] for t in range(3)))
+                    rec["val"][1 + 2 * d] = value(lv - 1, tuple(I[t] - e[t] for t in range(3)))
+                    rec["val"][2 + 2 * d] = value(lv - 1, tuple(I[t] + e[t] for t in range(3)))
             ops.append(rec)
     fixes = []
     for b in range(bs.nblocks):
-        if mesh.level[b] != 0:
-            continue
+        lv = int(mesh.level[b])
         lc = mesh.loc[b]
         bc = bs.blocks["bc"][b]
         for d in act:
             for side in (0, 1):
                 if bc[2 * d + side] != BC_AMR:
                     continue
+                o = [0, 0, 0]
+                o[d] = 1 if side else -1
+                if (lv, mesh.shift(lv, lc, o)) not in mesh.refined:
+                    continue  # the other side is coarser: this block is the fine side
                 tr = [t for t in act if t != d]
                 for tcell in itertools.product(*[range(nx[t]) for t in tr]):
                     loc = [0, 0, 0]
@@ -242,7 +343,7 @@ def amr_plan(mesh: TwoLevelMesh, bs: BlockSet):
                         F[d] = Fd
                         for t, c, q in zip(tr, tcell, kid):
                             F[t] = 2 * (int(lc[t]) * nx[t] + c) + q
-                        fb, floc = interior(1, tuple(F))
+                        fb, floc = interior(lv + 1, tuple(F))
                         floc = list(floc)
                         if not side:
                             floc[d] += 1  # upper face of the last fine cell below
@@ -260,58 +361,60 @@ AMR_FACE_DTYPE = np.dtype([("base", "<i8"), ("role", "<i4"), ("qa", "<i4"), ("qb
 assert AMR_FACE_DTYPE.itemsize == 24
 
 
-def fused_faces(mesh: TwoLevelMesh, bs: BlockSet):
+def fused_faces(mesh: AmrMesh, bs: BlockSet):
     """The fused form's interface table (aph_plan_set_amr): (faces [nblocks*6], level,
-    buffer length).  Each coarse face to the fine level owns a region of the buffer with
-    one slot per (coarse face cell, child) of nvar doubles, transverse axes ascending and
-    the first slowest; each fine face there points into that region with its offset in
-    fine cells."""
+    buffer length).  Each face to a finer region (role 2) owns a region of the buffer
+    with one slot per (coarse face cell, child) of nvar doubles, transverse axes
+    ascending and the first slowest; each face to a coarser leaf (role 1) points into the
+    region of the coarse leaf's face across it, with its offset in fine cells.  A block
+    can hold faces of both roles (a level between two others)."""
     nd, nx, nv = mesh.ndim, mesh.nx, bs.nvar
     nch = 2 ** (nd - 1)
     faces = np.zeros(bs.nblocks * 6, dtype=AMR_FACE_DTYPE)
     region = {}
     total = 0
     trans = lambda d: ((1 if d == 0 else 0), (1 if d == 2 else 2))
-    for b in range(bs.nblocks):
-        if mesh.level[b] != 0:
+
+    def across(b, d, side):
+        o = [0, 0, 0]
+        o[d] = 1 if side else -1
+        return mesh.shift(int(mesh.level[b]), tuple(int(v) for v in mesh.loc[b]), o)
+
+    amr_faces = [(b, d, side) for b in range(bs.nblocks) for d in range(nd)
+                 for side in (0, 1) if bs.blocks["bc"][b][2 * d + side] == BC_AMR]
+    for b, d, side in amr_faces:
+        if (int(mesh.level[b]), across(b, d, side)) not in mesh.refined:
             continue
-        for d in range(nd):
-            A, B = trans(d)
-            for side in (0, 1):
-                if bs.blocks["bc"][b][2 * d + side] != BC_AMR:
-                    continue
-                region[(b, d, side)] = total
-                f = faces[b * 6 + 2 * d + side]
-                f["base"], f["role"], f["na"] = total, 2, nx[A]
-                total += nx[A] * (nx[B] if B < nd else 1) * nch * nv
-    for b in range(bs.nblocks):
-        if mesh.level[b] != 1:
+        A, B = trans(d)
+        region[(b, d, side)] = total
+        f = faces[b * 6 + 2 * d + side]
+        f["base"], f["role"], f["na"] = total, 2, nx[A]
+        total += nx[A] * (nx[B] if B < nd else 1) * nch * nv
+    for b, d, side in amr_faces:
+        lv = int(mesh.level[b])
+        if (lv, across(b, d, side)) in mesh.refined:
             continue
+        A, B = trans(d)
         lf = mesh.loc[b]
-        for d in range(nd):
-            A, B = trans(d)
-            for side in (0, 1):
-                if bs.blocks["bc"][b][2 * d + side] != BC_AMR:
-                    continue
-                # the coarse cell across the face, and the coarse block holding it
-                G = [int(lf[t]) * nx[t] for t in range(3)]
-                G[d] = int(lf[d]) * nx[d] + (nx[d] if side else -1)
-                I = [(G[t] % mesh.ncells_axis(1, t)) // 2 if t < nd else 0 for t in range(3)]
-                lc = tuple(I[t] // nx[t] if t < nd else 0 for t in range(3))
-                cb = mesh.index[(0, lc)]
-                f = faces[b * 6 + 2 * d + side]
-                f["base"] = region[(cb, d, 1 - side)]
-                f["role"] = 1
-                f["na"] = nx[A]
-                f["qa"] = int(lf[A]) * nx[A] - 2 * lc[A] * nx[A]
-                f["qb"] = (int(lf[B]) * nx[B] - 2 * lc[B] * nx[B]) if B < nd else 0
+        # the coarse cell across the face, and the coarse leaf holding it
+        G = [int(lf[t]) * nx[t] for t in range(3)]
+        G[d] = int(lf[d]) * nx[d] + (nx[d] if side else -1)
+        I = [(G[t] % mesh.ncells_axis(lv, t)) // 2 if t < nd else 0 for t in range(3)]
+        lc = tuple(I[t] // nx[t] if t < nd else 0 for t in range(3))
+        cb = mesh.index[(lv - 1, lc)]
+        f = faces[b * 6 + 2 * d + side]
+        f["base"] = region[(cb, d, 1 - side)]
+        f["role"] = 1
+        f["na"] = nx[A]
+        f["qa"] = int(lf[A]) * nx[A] - 2 * lc[A] * nx[A]
+        f["qb"] = (int(lf[B]) * nx[B] - 2 * lc[B] * nx[B]) if B < nd else 0
     return faces, mesh.level.astype(np.int32), total
 
 
 class AmrPlan:
     """The plan on the device (aph_amr_plan_create)."""
 
-    def __init__(self, ctx, dbs, mesh: TwoLevelMesh, bs: BlockSet):
+    def __init__(self, ctx, dbs, mesh: AmrMesh, bs: BlockSet):
         self.ctx = ctx
         self.ops, self.fixes, self.refs = amr_plan(mesh, bs)
         self.h = ctypes.c_void_p()
@@ -336,112 +439,135 @@ class AmrPlan:
             pass
 
 
-def regrid(mesh: TwoLevelMesh, bs: BlockSet, cons, tags):
-    """One regrid of a two-level mesh from per-block tags (-1 / 0 / 1, as
-    aph_refinement_tag returns them): a level-0 block tagged 1 is refined, a level-1
-    sibling group whose 2^ndim blocks are all tagged -1 is merged back; nothing else
-    changes (two levels: level-1 blocks do not refine further).  Returns
-    (new_mesh, new_bs, new_cons) with the interiors remapped on the host:
-    * kept blocks: copied;
-    * merged groups: the mean of each coarse cell's children, (dz, dy, dx) order;
-    * new children: c + sum_d (sign_d / 4) minmod(c - c_-d, c_+d - c) of their parent's
-      cells, the neighbours taken at level 0 (a cell, or the mean of its children).
-    Ghost cells are left to the next ghost fill.  Totals are conserved to round-off and
-    refining then merging returns the coarse data."""
-    nd, nx = mesh.ndim, mesh.nx
-    nv = bs.nvar
-    tags = np.asarray(tags)
-    old_ref = {tuple(int(v) // 2 for v in mesh.loc[b])[:3] for b in range(bs.nblocks)
-               if mesh.level[b] == 1}
-    old_ref = {t if nd == 3 else (t[0], t[1], 0) for t in old_ref}
-    groups = {}
-    for b in range(bs.nblocks):
-        if mesh.level[b] == 1:
-            key = tuple(int(mesh.loc[b][d]) // 2 if d < nd else 0 for d in range(3))
-            groups.setdefault(key, []).append(int(tags[b]))
-    new_ref = set(old_ref)
-    for key, ts in groups.items():
-        if len(ts) == 2 ** nd and all(t == -1 for t in ts):
-            new_ref.discard(key)
-    for b in range(bs.nblocks):
-        if mesh.level[b] == 0 and tags[b] == 1:
-            new_ref.add(tuple(int(mesh.loc[b][d]) if d < nd else 0 for d in range(3)))
-    refined = [t[:nd] for t in sorted(new_ref)]
-    new = TwoLevelMesh(nd, mesh.nbase[:nd], nx[:nd], mesh.ng, mesh.xmin, mesh.xmax, refined,
-                       mesh_id=mesh.mesh_id)
-    nbs = new.block_set(nscalars=bs.nscalars)
-    out = np.zeros(nbs.field_size(nv))
-    # level-0 values over the whole domain (refined regions: the mean of their children)
-    shape0 = tuple(mesh.ncells_axis(0, d) for d in (2, 1, 0))
-    coarse = np.zeros((nv,) + shape0)
-    fine = {}
-    for b in range(bs.nblocks):
-        lv, lc = int(mesh.level[b]), mesh.loc[b]
-        v = bs.interior(cons, b, nv)
-        if lv == 0:
-            sl = tuple(slice(int(lc[d]) * nx[d], (int(lc[d]) + 1) * nx[d]) for d in (2, 1, 0))
-            coarse[(slice(None),) + sl] = v
-        else:
-            fine[tuple(int(x) for x in lc)] = v
-    for key in old_ref:
-        kids = []
-        for dz in ((0, 1) if nd == 3 else (0,)):
-            for dy in (0, 1):
-                for dxx in (0, 1):
-                    k = (2 * key[0] + dxx, 2 * key[1] + dy, (2 * key[2] + dz) if nd == 3 else 0)
-                    kids.append(fine[k])
-        # the parent's cells: children of parent cell (i, j, k) are (2i+dx, 2j+dy, 2k+dz)
-        # across the 2^nd child blocks; assemble the 2n fine cells first
-        fz = 2 * nx[2] if nd == 3 else 1
-        full = np.zeros((nv, fz, 2 * nx[1], 2 * nx[0]))
-        idx = 0
-        for dz in ((0, 1) if nd == 3 else (0,)):
-            for dy in (0, 1):
-                for dxx in (0, 1):
-                    zs = slice(dz * nx[2], (dz + 1) * nx[2]) if nd == 3 else slice(0, 1)
-                    full[:, zs, dy * nx[1]:(dy + 1) * nx[1], dxx * nx[0]:(dxx + 1) * nx[0]] = kids[idx]
-                    idx += 1
-        parts = []
-        for dz in ((0, 1) if nd == 3 else (0,)):
-            for dy in (0, 1):
-                for dxx in (0, 1):
-                    parts.append(full[:, dz::2, dy::2, dxx::2] if nd == 3 else full[:, :, dy::2, dxx::2])
-        s = parts[0].copy()
-        for q in parts[1:]:
-            s = s + q
-        s = s * (1.0 / len(parts))
-        sl = tuple(slice(key[d] * nx[d], (key[d] + 1) * nx[d]) if d < nd else slice(0, 1)
-                   for d in (2, 1, 0))
-        coarse[(slice(None),) + sl] = s
-    N0 = [mesh.ncells_axis(0, d) for d in range(3)]
+def _restrict(a, nd):
+    """Mean of the 2^nd children of every cell of a level array (nv, Z, Y, X), summed in
+    (dz, dy, dx) order, x fastest, and scaled by 2^-nd."""
+    parts = [a[:, dz::2, dy::2, dx::2] if nd == 3 else a[:, :, dy::2, dx::2]
+             for dz in ((0, 1) if nd == 3 else (0,)) for dy in (0, 1) for dx in (0, 1)]
+    s = parts[0].copy()
+    for q in parts[1:]:
+        s = s + q
+    return s * (1.0 / len(parts))
 
+
+def _prolong(c, nd):
+    """Limited linear prolongation of a whole periodic level array (nv, Z, Y, X) to the
+    next level: child = c + sum_d (sign_d / 4) minmod(c - c_-d, c_+d - c), axes in order."""
     def sgn(x):
         return np.where(x < 0.0, -1.0, 1.0)
 
+    ax = {0: 3, 1: 2, 2: 1}
+    val = c
+    for d in range(nd):
+        val = np.repeat(val, 2, axis=ax[d])
+    for d in range(nd):
+        a = c - np.roll(c, 1, axis=ax[d])
+        b = np.roll(c, -1, axis=ax[d]) - c
+        g = (0.5 * (sgn(a) + sgn(b))) * np.minimum(np.abs(a), np.abs(b))
+        for t in range(nd):
+            g = np.repeat(g, 2, axis=ax[t])
+        shape = [1, 1, 1, 1]
+        shape[ax[d]] = g.shape[ax[d]]
+        sign = np.where(np.arange(g.shape[ax[d]]) % 2 == 0, -1.0, 1.0).reshape(shape)
+        val = val + (sign * 0.25) * g
+    return val
+
+
+def level_values(mesh: AmrMesh, bs: BlockSet, cons, nlevels=None):
+    """Whole-domain arrays (nv, Z, Y, X) of every level 0 .. nlevels-1: the leaf cells at
+    their level; a refined region the mean of its children (finest levels first); a region
+    under a coarser leaf the prolongation of the level above it (as the ghost exchange
+    fills a block's ghost cells over a coarser neighbour)."""
+    nd, nx, nv = mesh.ndim, mesh.nx, bs.nvar
+    nl = max(mesh.nlevels, nlevels or 0)
+    arr = [np.full((nv,) + tuple(mesh.ncells_axis(L, d) for d in (2, 1, 0)), np.nan)
+           for L in range(nl)]
+    for b in range(bs.nblocks):
+        lv, lc = int(mesh.level[b]), mesh.loc[b]
+        sl = tuple(slice(int(lc[d]) * nx[d], (int(lc[d]) + 1) * nx[d]) for d in (2, 1, 0))
+        arr[lv][(slice(None),) + sl] = bs.interior(cons, b, nv)
+    for L in range(nl - 2, -1, -1):
+        r = _restrict(arr[L + 1], nd)
+        hole = np.isnan(arr[L]) & ~np.isnan(r)
+        arr[L][hole] = r[hole]
+    for L in range(1, nl):
+        hole = np.isnan(arr[L])
+        if hole.any():
+            arr[L][hole] = _prolong(arr[L - 1], nd)[hole]
+    return arr
+
+
+def regrid(mesh: AmrMesh, bs: BlockSet, cons, tags):
+    """One regrid from per-block tags (-1 / 0 / 1, as aph_refinement_tag returns them),
+    Parthenon's rules on a periodic tree:
+    * a leaf tagged 1 below ``mesh.max_level`` is refined;
+    * then every leaf that touches a leaf two levels finer is refined (2:1 balance);
+    * a sibling group whose 2^ndim leaves were all tagged -1 (and none was refined) is
+      merged into its parent, unless the parent would touch a leaf two levels finer;
+      groups are taken in (level, position) order.
+    Returns (new_mesh, new_bs, new_cons): each new leaf takes its region of the per-level
+    arrays of ``level_values`` -- a kept leaf its data, a merged parent the mean of its
+    children (dz, dy, dx order), a new child the minmod prolongation of its parent's
+    level (the neighbours a cell, a mean of children or a prolongation from the level
+    above).  Ghost cells are left to the next ghost fill.  Totals are conserved to
+    round-off and refining then merging returns the data."""
+    nd = mesh.ndim
+    tags = np.asarray(tags)
+    ref = set(mesh.refined)
+    leaves = [(int(mesh.level[b]), tuple(int(v) for v in mesh.loc[b]))
+              for b in range(bs.nblocks)]
+    for b, (lv, lc) in enumerate(leaves):
+        if tags[b] == 1 and lv < mesh.max_level:
+            ref.add((lv, lc))
+
+    def tree_leaves(ref):
+        out = []
+        stack = [(0, (x, y, z)) for z in range(mesh.nbase[2]) for y in range(mesh.nbase[1])
+                 for x in range(mesh.nbase[0])]
+        while stack:
+            lv, lc = stack.pop()
+            if (lv, lc) in ref:
+                stack.extend((lv + 1, c) for c in mesh.children(lc))
+            else:
+                out.append((lv, lc))
+        return out
+
+    def touches_finer(lv, lc, ref):
+        """Whether leaf (lv, lc) of tree `ref` touches a leaf at level lv + 2 or deeper."""
+        for o in mesh.offsets():
+            p = mesh.shift(lv, lc, o)
+            if (lv, p) in ref and any((lv + 1, c) in ref
+                                      for c in mesh.touching_children(p, o)):
+                return True
+        return False
+
+    changed = True
+    while changed:
+        changed = False
+        for lv, lc in tree_leaves(ref):
+            if touches_finer(lv, lc, ref):
+                ref.add((lv, lc))
+                changed = True
+    old_index = mesh.index
+    groups = sorted({(lv - 1, mesh.parent(lc)) for b, (lv, lc) in enumerate(leaves)
+                     if lv > 0 and tags[b] == -1})
+    for P in groups:
+        lv = P[0] + 1
+        kids = mesh.children(P[1])
+        if any((lv, c) not in old_index or tags[old_index[(lv, c)]] != -1 or (lv, c) in ref
+               for c in kids):
+            continue
+        ref.discard(P)
+        if touches_finer(P[0], P[1], ref):
+            ref.add(P)
+    new = AmrMesh(nd, mesh.nbase[:nd], mesh.nx[:nd], mesh.ng, mesh.xmin, mesh.xmax, ref,
+                  mesh_id=mesh.mesh_id, max_level=mesh.max_level)
+    nbs = new.block_set(nscalars=bs.nscalars)
+    nv, nx = bs.nvar, mesh.nx
+    arr = level_values(mesh, bs, cons, nlevels=new.nlevels)
+    out = np.zeros(nbs.field_size(nv))
     for b in range(nbs.nblocks):
         lv, lc = int(new.level[b]), new.loc[b]
-        dst = nbs.interior(out, b, nv)
-        if lv == 0:
-            sl = tuple(slice(int(lc[d]) * nx[d], (int(lc[d]) + 1) * nx[d]) for d in (2, 1, 0))
-            dst[:] = coarse[(slice(None),) + sl]
-            continue
-        key = tuple(int(lc[d]) for d in range(3))
-        if key in fine:  # a level-1 block of the old mesh, kept
-            dst[:] = fine[key]
-            continue
-        # a new child: prolongate from level 0
-        G = np.meshgrid(*[int(lc[d]) * nx[d] + np.arange(nx[d]) if d < nd else np.zeros(1, int)
-                          for d in (2, 1, 0)], indexing="ij")
-        G = [G[2], G[1], G[0]]  # x, y, z index arrays, shape (nz, ny, nx)
-        I = [G[d] // 2 if d < nd else G[d] for d in range(3)]
-        at = lambda J: coarse[(slice(None),) + tuple(J[d] % N0[d] for d in (2, 1, 0))]
-        c = at(I)
-        val = c.copy()
-        for d in range(nd):
-            lo = [I[t] - (1 if t == d else 0) for t in range(3)]
-            hi = [I[t] + (1 if t == d else 0) for t in range(3)]
-            a, bb = c - at(lo), at(hi) - c
-            g = (0.5 * (sgn(a) + sgn(bb))) * np.minimum(np.abs(a), np.abs(bb))
-            val = val + (np.where(G[d] % 2 == 0, -1.0, 1.0) * 0.25) * g
-        dst[:] = val
+        sl = tuple(slice(int(lc[d]) * nx[d], (int(lc[d]) + 1) * nx[d]) for d in (2, 1, 0))
+        nbs.interior(out, b, nv)[:] = arr[lv][(slice(None),) + sl]
     return new, nbs, out

@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <set>
+#include <utility>
 #include <vector>
 
 #include "aph_device.hpp"
@@ -274,20 +275,26 @@ int aph_amr_plan_create(aph_context *ctx, const aph_blockset *bs, const aph_amr_
         if (op.sign[d] != -1 && op.sign[d] != 1)
           return set_error(ctx, APH_E_INVALID, "amr op %lld: sign must be +-1", (long long)t);
   }
-  std::set<int> dst_blocks, src_blocks;
+  // faces (flux array, component-0 offset) written and read: a face written twice, or
+  // both written and read, would race inside the one fix launch (on a mesh of three or
+  // more levels a block can be corrected on one face and a source on another)
+  std::set<std::pair<int, int64_t>> dst_faces, src_faces;
   for (int64_t t = 0; t < nfixes; ++t) {
     const aph_flux_fix &fx = fixes[t];
     const int b = loc.find(fx.dst, fx.dst_stride);
     int sb = -1;
     if (b < 0 || fx.dir < 0 || fx.dir >= bs->ndim || !check_val(fx.val, false, &sb))
       return set_error(ctx, APH_E_INVALID, "flux fix %lld: bad face references", (long long)t);
-    dst_blocks.insert(b);
+    if (!dst_faces.emplace(fx.dir, fx.dst).second)
+      return set_error(ctx, APH_E_INVALID, "flux fix %lld: face corrected twice", (long long)t);
     for (int k = 0; k < fx.val.count; ++k)
-      src_blocks.insert(loc.find(refs[fx.val.first + k].offset, refs[fx.val.first + k].stride));
+      src_faces.emplace(fx.dir, refs[fx.val.first + k].offset);
   }
-  for (int b : dst_blocks)
-    if (src_blocks.count(b))
-      return set_error(ctx, APH_E_INVALID, "flux fix: block %d is both corrected and a source", b);
+  for (const auto &f : dst_faces)
+    if (src_faces.count(f))
+      return set_error(ctx, APH_E_INVALID,
+                       "flux fix: face %lld of flux array %d is both corrected and a source",
+                       (long long)f.second, f.first);
   aph_amr_plan *p = new (std::nothrow) aph_amr_plan();
   if (!p) return set_error(ctx, APH_E_NOMEM, "host allocation failed");
   p->ctx = ctx;

@@ -21,6 +21,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <map>
 #include <vector>
 
 #include "aph_adjoint_host.hpp"
@@ -77,13 +78,14 @@ struct aph_plan {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   aph::AdjScratch adj;
-  // two-level mesh (aph_plan_set_amr): items [0, nitems1) are the level-1 blocks, run
-  // before the level-0 blocks [nitems1, nitems) so the coarse side finds the fine fluxes
+  // multilevel mesh (aph_plan_set_amr): one phase of items per level, the finest first,
+  // so the coarse side of every level jump finds the fine fluxes; phase k is items
+  // [phase_start[k], phase_start[k + 1])
   const aph_amr_plan *amr = nullptr;
   aph_amr_face *d_amr_face = nullptr;
   double *d_amr_buf = nullptr;
   std::vector<int32_t> level;
-  int nitems1 = 0;
+  std::vector<int> phase_start;
   bool amr_faces = false;  // the block set has faces to another level: needs set_amr
 };
 
@@ -207,14 +209,16 @@ int build_items(aph_context *ctx, aph_plan *p) {
   constexpr int kXcd = 8;
   size_t maxcells = 0;
   std::vector<int4> items;
-  // one phase without a multilevel mesh; with one, the level-1 blocks, then level 0
-  const bool two = !p->level.empty();
-  for (int phase = 0; phase < (two ? 2 : 1); ++phase) {
+  // one phase without a multilevel mesh; with one, a phase per level, the finest first
+  const bool multi = !p->level.empty();
+  const int top = multi ? *std::max_element(p->level.begin(), p->level.end()) : 0;
+  p->phase_start.assign(1, 0);
+  for (int phase = 0; phase <= top; ++phase) {
     std::vector<std::vector<int4>> lists(kXcd);
     std::vector<int64_t> load(kXcd, 0);
     for (int b = 0; b < bs->nblocks; ++b) {
       if (p->only_class >= 0 && cls_of[b] != p->only_class) continue;
-      if (two && p->level[b] != 1 - phase) continue;
+      if (multi && p->level[b] != top - phase) continue;
       const aph_block &k = bs->h[b];
       const aph_tile &t = tl[cls_of[b]];
       const int tt[3] = {t.tx, t.ty, t.tz};
@@ -237,7 +241,7 @@ int build_items(aph_context *ctx, aph_plan *p) {
     items.resize(at + maxlen * kXcd, make_int4(-1, 0, 0, 0));
     for (int x = 0; x < kXcd; ++x)
       for (size_t r = 0; r < lists[x].size(); ++r) items[at + r * kXcd + x] = lists[x][r];
-    if (phase == 0) p->nitems1 = (int)items.size();
+    p->phase_start.push_back((int)items.size());
   }
   if (p->d_items) (void)hipFree(p->d_items);
   p->d_items = nullptr;
@@ -329,15 +333,16 @@ int launch_kernel(aph_context *ctx, aph_plan *p, int recon, int stage, int nst, 
     APH_LAUNCH_CHECK(ctx);
     return APH_OK;
   }
-  // two-level mesh: cross-level ghosts of the stage input, then the fine blocks (which
-  // record their interface fluxes), then the coarse blocks (which read them)
+  // multilevel mesh: cross-level ghosts of the stage input, then the levels finest first
+  // (each records the fluxes on its faces to the level above and takes the mean of the
+  // recorded fluxes on its faces to the level below)
   int rc = aph_amr_fill(ctx, p->amr, u_in);
   if (rc) return rc;
-  const int n[2] = {p->nitems1, p->nitems - p->nitems1};
-  for (int ph = 0; ph < 2; ++ph) {
-    if (!n[ph]) continue;
-    a.items = p->d_items + (ph ? p->nitems1 : 0);
-    a.nitems = n[ph];
+  for (size_t ph = 0; ph + 1 < p->phase_start.size(); ++ph) {
+    const int n = p->phase_start[ph + 1] - p->phase_start[ph];
+    if (!n) continue;
+    a.items = p->d_items + p->phase_start[ph];
+    a.nitems = n;
     hipLaunchKernelGGL(fn, dim3(p->grid), dim3(64), p->lds, ctx->stream, a);
     APH_LAUNCH_CHECK(ctx);
   }
@@ -480,11 +485,20 @@ int aph_plan_set_amr(aph_context *ctx, aph_plan *p, const aph_amr_plan *amr,
   const int nd = bs->ndim, nv = bs->nvar;
   if (nd < 2) return set_error(ctx, APH_E_INVALID, "aph_plan_set_amr: ndim must be 2 or 3");
   if (!pick_kernel_amr(p->recon, p->rsolver, nv, nd))
-    return set_error(ctx, APH_E_INVALID, "aph_plan_set_amr: the fused form runs two-level "
+    return set_error(ctx, APH_E_INVALID, "aph_plan_set_amr: the fused form runs multilevel "
                      "meshes with nvar 5 or 6 (this block set has %d)", nv);
   const int64_t nch = (nd == 3) ? 4 : 2;
+  auto extents = [&](int b, int d, int64_t &n_a, int64_t &n_b) {
+    const aph_block &k = bs->h[b];
+    const int A = (d == 0) ? 1 : 0, B = (d == 2) ? 1 : 2;
+    n_a = k.nx[A];
+    n_b = (B < nd) ? k.nx[B] : 1;
+  };
+  // the regions: a face to a finer region (role 2) owns [base, base + extent)
+  struct Region { int block, face; int64_t end; };
+  std::map<int64_t, Region> regions;
   for (int b = 0; b < bs->nblocks; ++b) {
-    if (level[b] != 0 && level[b] != 1)
+    if (level[b] < 0 || level[b] > 30)
       return set_error(ctx, APH_E_INVALID, "aph_plan_set_amr: block %d level %d", b, level[b]);
     const aph_block &k = bs->h[b];
     for (int f = 0; f < 6; ++f) {
@@ -496,25 +510,56 @@ int aph_plan_set_amr(aph_context *ctx, aph_plan *p, const aph_amr_plan *amr,
           return set_error(ctx, APH_E_INVALID, "block %d face %d: role on a non-AMR face", b, f);
         continue;
       }
-      if (af.role != (level[b] == 1 ? 1 : 2))
-        return set_error(ctx, APH_E_INVALID, "block %d face %d: role %d does not match level %d",
-                         b, f, af.role, level[b]);
-      const int A = (d == 0) ? 1 : 0, B = (d == 2) ? 1 : 2;
-      const int64_t n_a = k.nx[A], n_b = (B < nd) ? k.nx[B] : 1;
-      int64_t last;  // last element the face touches
-      if (af.role == 1) {
-        if (af.qa < 0 || af.qb < 0 || af.na < 1 || (af.qa + n_a + 1) / 2 > af.na)
-          return set_error(ctx, APH_E_INVALID, "block %d face %d: bad fine offsets", b, f);
-        last = af.base + (((af.qb + n_b - 1) / 2) * af.na + (af.qa + n_a - 1) / 2) * nch * nv +
-               nch * nv - 1;
-      } else {
-        if (af.na != n_a || af.qa != 0 || af.qb != 0)
-          return set_error(ctx, APH_E_INVALID, "block %d face %d: bad coarse region", b, f);
-        last = af.base + ((n_b - 1) * af.na + (n_a - 1)) * nch * nv + nch * nv - 1;
-      }
-      if (af.base < 0 || last >= buffer_doubles)
+      if (af.role != 1 && af.role != 2)
+        return set_error(ctx, APH_E_INVALID, "block %d face %d: role %d on an AMR face", b, f,
+                         af.role);
+      if (af.role != 2) continue;
+      int64_t n_a, n_b;
+      extents(b, d, n_a, n_b);
+      if (af.na != n_a || af.qa != 0 || af.qb != 0)
+        return set_error(ctx, APH_E_INVALID, "block %d face %d: bad coarse region", b, f);
+      const int64_t end = af.base + n_a * n_b * nch * nv;
+      if (af.base < 0 || end > buffer_doubles)
         return set_error(ctx, APH_E_INVALID, "block %d face %d: outside the interface buffer",
                          b, f);
+      if (!regions.emplace(af.base, Region{b, f, end}).second)
+        return set_error(ctx, APH_E_INVALID, "block %d face %d: region shared with another face",
+                         b, f);
+    }
+  }
+  int64_t prev_end = 0;
+  for (const auto &r : regions) {
+    if (r.first < prev_end)
+      return set_error(ctx, APH_E_INVALID, "block %d face %d: overlapping interface regions",
+                       r.second.block, r.second.face);
+    prev_end = r.second.end;
+  }
+  // a face to a coarser leaf (role 1) writes into the region of the opposite face of a
+  // block one level coarser, which runs in the next phase
+  for (int b = 0; b < bs->nblocks; ++b) {
+    for (int f = 0; f < 2 * nd; ++f) {
+      const aph_amr_face &af = faces[b * 6 + f];
+      if (af.role != 1) continue;
+      const auto it = regions.find(af.base);
+      if (it == regions.end())
+        return set_error(ctx, APH_E_INVALID, "block %d face %d: no coarse region at %lld", b, f,
+                         (long long)af.base);
+      const Region &r = it->second;
+      if (r.face != (f ^ 1) || level[r.block] != level[b] - 1)
+        return set_error(ctx, APH_E_INVALID, "block %d face %d (level %d): region of block %d "
+                         "face %d (level %d) is not the opposite face one level coarser", b, f,
+                         level[b], r.block, r.face, level[r.block]);
+      int64_t n_a, n_b;
+      extents(b, f / 2, n_a, n_b);
+      if (af.qa < 0 || af.qb < 0 || af.na != faces[r.block * 6 + r.face].na ||
+          (af.qa + n_a + 1) / 2 > af.na)
+        return set_error(ctx, APH_E_INVALID, "block %d face %d: bad fine offsets", b, f);
+      const int64_t last = af.base +
+                           (((af.qb + n_b - 1) / 2) * af.na + (af.qa + n_a - 1) / 2) * nch * nv +
+                           nch * nv - 1;  // last element the face touches
+      if (last >= r.end)
+        return set_error(ctx, APH_E_INVALID, "block %d face %d: outside its coarse region", b,
+                         f);
     }
   }
   if (p->d_amr_face) (void)hipFree(p->d_amr_face);

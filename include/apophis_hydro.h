@@ -181,9 +181,9 @@ int aph_refinement_tag(aph_context *ctx, const aph_blockset *bs, int32_t criteri
                        double *values);
 
 /* ------------------------------------------- multilevel meshes (plain form)
- * Blocks of two refinement levels in one mesh (a 2:1 jump).  nbr[] links blocks of the
- * same level only; a face whose neighbour region belongs to the other level has nbr = -1
- * and bc = APH_BC_AMR.  The ghost cells there, and the coarse fluxes on faces shared
+ * Blocks of several refinement levels in one mesh, 2:1 balanced (leaves that touch differ
+ * by at most one level).  nbr[] links blocks of the same level only; a face whose
+ * neighbour region belongs to another level has nbr = -1 and bc = APH_BC_AMR.  The ghost cells there, and the coarse fluxes on faces shared
  * with finer blocks, come from a plan the host builds from the mesh tree.  This replaces
  * Parthenon's multilevel boundary exchange (RestrictAverage / ProlongateSharedMinMod
  * for the cons field) and its flux correction (LoadAndSendFluxCorrections /
@@ -309,13 +309,14 @@ int aph_plan_graph_launch(aph_context *ctx, aph_plan *plan);
 int aph_plan_estimate_timestep(aph_context *ctx, aph_plan *plan, const double *cons,
                                double cfl, double *dt_out);
 
-/* Two-level meshes in the fused form.  Per block face, for faces with bc = APH_BC_AMR
-   (zeros elsewhere): the fine side writes the flux of each of its boundary faces to an
-   interface buffer, ordered per coarse face cell (transverse axes in ascending order,
-   the first slowest) and child (the same order); the coarse side replaces each boundary
-   face flux by the mean of its children there, in that order -- the flux correction of
-   aph_amr_flux_correct, bitwise.  A stage then runs aph_amr_fill on its input, the kernel
-   over the level-1 blocks, and the kernel over the level-0 blocks. */
+/* Multilevel meshes in the fused form.  Per block face, for faces with bc = APH_BC_AMR
+   (zeros elsewhere): the fine side of every level jump writes the flux of each of its
+   boundary faces to an interface buffer, ordered per coarse face cell (transverse axes in
+   ascending order, the first slowest) and child (the same order); the coarse side
+   replaces each boundary face flux by the mean of its children there, in that order --
+   the flux correction of aph_amr_flux_correct, bitwise.  A block between two levels can
+   hold faces of both roles.  A stage runs aph_amr_fill on its input, then the kernel
+   once per level, the finest first. */
 typedef struct aph_amr_face {
   int64_t base;  /* element offset of the coarse face region in the interface buffer */
   int32_t role;  /* 0 none, 1 fine side (writes), 2 coarse side (reads)               */
@@ -324,8 +325,10 @@ typedef struct aph_amr_face {
   int32_t na;    /* coarse face cells along the first transverse axis               */
 } aph_amr_face;
 /* faces: nblocks*6 entries (face order -x1,+x1,-x2,+x2,-x3,+x3); level: nblocks entries
-   (0 coarse, 1 fine); buffer_doubles: interface buffer length (>= every base + its
-   region's cells x 2^(ndim-1) x nvar).  The amr plan must outlive the plan.  The adjoint
+   (0 the base level, up to 30); buffer_doubles: interface buffer length.  Checked: each
+   role-2 face owns [base, base + its face cells x 2^(ndim-1) x nvar) inside the buffer,
+   regions do not overlap, and each role-1 face lies inside the region of the opposite
+   face of a block one level coarser.  The amr plan must outlive the plan.  The adjoint
    entry points reject a plan with a multilevel mesh.  Synchronous. */
 int aph_plan_set_amr(aph_context *ctx, aph_plan *plan, const aph_amr_plan *amr,
                      const aph_amr_face *faces, const int32_t *level,

@@ -1,10 +1,10 @@
-"""Two-level meshes in the oracle: cross-level ghost values and flux correction.
+"""Multilevel meshes in the oracle: cross-level ghost values and flux correction.
 
 TEST INFRASTRUCTURE: imported only by tests/ as the checker.
 
 The reference takes these from Parthenon (apophis_driver.cpp:68,74-76,94), which is not in
 the reference tree; what is restated here is its published behaviour for a Cartesian
-mesh with a 2:1 jump (RestrictAverage, ProlongateSharedMinMod, flux correction by the
+mesh with 2:1 jumps (RestrictAverage, ProlongateSharedMinMod, flux correction by the
 mean of the fine fluxes), with the operation order the build defines
 (apophis_amd/amr.py, include/apophis_hydro.h): restriction sums the 2^d children in
 (dz, dy, dx) order, x fastest, and scales by 2^-d;
@@ -14,7 +14,7 @@ of the lower axis varying slowest.  Parity with the reference is unpinned (no Pa
 source or output is available); tests/test_amr.py pins these functions by exactness on
 linear fields, conservation, uniform states and an op-by-op cross-check of the host plan.
 
-This module works from whole-level arrays (every level-0 cell of the domain, the
+This module works from whole-level arrays (every cell of the domain at each level, the
 refined ones holding the restriction of their children), not from the plan's op lists.
 
 refinement_tag restates the reference's own refinement criteria
@@ -43,44 +43,43 @@ def _nzyx(mesh, level):
     return tuple(mesh.ncells_axis(level, d) for d in (2, 1, 0))
 
 
+def _kids(a, nd):
+    """The 2^nd strided child views of a level array, (dz, dy, dx) order."""
+    zs = (0, 1) if nd == 3 else (0,)
+    return [a[:, dz::2, dy::2, dx::2] if nd == 3 else a[:, :, dy::2, dx::2]
+            for dz in zs for dy in (0, 1) for dx in (0, 1)]
+
+
 def level_arrays(mesh, bs, u, nv):
-    """(coarse, fine): (nv, Nz, Ny, Nx) arrays of each level's cells over the whole
-    domain; coarse cells under a refined region hold the mean of their children, fine
-    cells outside refined regions are NaN."""
-    coarse = np.full((nv,) + _nzyx(mesh, 0), np.nan)
-    fine = np.full((nv,) + _nzyx(mesh, 1), np.nan)
+    """[level] -> (nv, Nz, Ny, Nx) array of that level's cells over the whole domain: leaf
+    cells hold their values, cells of a refined region the mean of their children (the
+    deepest levels first), cells under a coarser leaf NaN."""
+    nl = int(mesh.level.max()) + 1
+    arrs = [np.full((nv,) + _nzyx(mesh, L), np.nan) for L in range(nl)]
     nx = mesh.nx
     for b in range(bs.nblocks):
         lv = int(mesh.level[b])
         lc = mesh.loc[b]
         sl = tuple(slice(int(lc[d]) * nx[d], (int(lc[d]) + 1) * nx[d]) for d in (2, 1, 0))
-        (coarse if lv == 0 else fine)[(slice(None),) + sl] = bs.interior(u, b, nv)
+        arrs[lv][(slice(None),) + sl] = bs.interior(u, b, nv)
     nd = mesh.ndim
-    zs = (0, 1) if nd == 3 else (0,)
-    kids = []
-    for dz in zs:
-        for dy in (0, 1):
-            for dx in (0, 1):
-                if nd == 3:
-                    kids.append(fine[:, dz::2, dy::2, dx::2])
-                else:
-                    kids.append(fine[:, :, dy::2, dx::2])
-    s = kids[0].copy()
-    for k in kids[1:]:
-        s = s + k
-    s = s * (1.0 / len(kids))
-    refined = ~np.isnan(s)
-    coarse[refined] = s[refined]
-    return coarse, fine
+    for L in range(nl - 2, -1, -1):
+        kids = _kids(arrs[L + 1], nd)
+        s = kids[0].copy()
+        for k in kids[1:]:
+            s = s + k
+        s = s * (1.0 / len(kids))
+        refined = ~np.isnan(s)
+        arrs[L][refined] = s[refined]
+    return arrs
 
 
 def amr_fill(mesh, bs, u):
-    """Ghost cells of u on faces / edges / corners whose neighbour region is on the other
-    level (nbr = -1), after the same-level exchange."""
+    """Ghost cells of u on faces / edges / corners whose neighbour region is on another
+    level (nbr = -1), after the same-level exchange: over a finer region the level's
+    restriction of it, over a coarser leaf the prolongation of the level above."""
     nv, nd, g, nx = bs.nvar, mesh.ndim, mesh.ng, mesh.nx
-    coarse, _ = level_arrays(mesh, bs, u, nv)
-    N0 = [mesh.ncells_axis(0, d) for d in range(3)]
-    N1 = [mesh.ncells_axis(1, d) for d in range(3)]
+    arrs = level_arrays(mesh, bs, u, nv)
     ext = bs.extents()
     for b in range(bs.nblocks):
         lv = int(mesh.level[b])
@@ -98,97 +97,109 @@ def amr_fill(mesh, bs, u):
         sel = ghost & (nbr[code] < 0)
         if not sel.any():
             continue
-        G = [(int(lc[d]) * nx[d] + loc[d][sel] - g) if d < nd else np.zeros(sel.sum(), int)
-             for d in range(3)]
+        NL = [mesh.ncells_axis(lv, d) for d in range(3)]
+        Gw = [(int(lc[d]) * nx[d] + loc[d][sel] - g) % NL[d] if d < nd
+              else np.zeros(sel.sum(), int) for d in range(3)]
         view = bs.block_view(u, b, nv)
-        if lv == 0:
-            idx = tuple(G[d] % N0[d] for d in (2, 1, 0))
-            val = coarse[(slice(None),) + idx]
-        else:
-            Gw = [G[d] % N1[d] for d in range(3)]
-            I = [Gw[d] // 2 if d < nd else np.zeros_like(Gw[d]) for d in range(3)]
-            at = lambda J: coarse[(slice(None),) + tuple(J[d] % N0[d] for d in (2, 1, 0))]
+        own = arrs[lv][(slice(None),) + (Gw[2], Gw[1], Gw[0])]
+        finer = ~np.isnan(own[0])  # a refined region: the level holds its restriction
+        val = own.copy()
+        coarser = ~finer
+        if coarser.any():
+            assert lv > 0, "a level-0 ghost over no level-0 region"
+            C = arrs[lv - 1]
+            NC = [mesh.ncells_axis(lv - 1, d) for d in range(3)]
+            Gc = [Gw[d][coarser] for d in range(3)]
+            I = [Gc[d] // 2 if d < nd else np.zeros_like(Gc[d]) for d in range(3)]
+            at = lambda J: C[(slice(None),) + tuple(J[d] % NC[d] for d in (2, 1, 0))]
             c = at(I)
-            val = c.copy()
+            pv = c.copy()
             for d in range(nd):
                 lo = [I[t] - (1 if t == d else 0) for t in range(3)]
                 hi = [I[t] + (1 if t == d else 0) for t in range(3)]
                 gd = _minmod(c - at(lo), at(hi) - c)
-                sign = np.where(Gw[d] % 2 == 0, -1.0, 1.0)
-                val = val + (sign * 0.25) * gd
+                sign = np.where(Gc[d] % 2 == 0, -1.0, 1.0)
+                pv = pv + (sign * 0.25) * gd
+            val[:, coarser] = pv
+        assert not np.isnan(val).any(), "a cross-level value two levels away"
         view[(slice(None),) + (kk[sel], jj[sel], ii[sel])] = val
 
 
 def flux_correct(mesh, bs, fx):
-    """Coarse fluxes on faces shared with the finer level <- the mean of the fine
-    fluxes across each (fx: the three flux arrays)."""
+    """Coarse fluxes on faces shared with a finer level <- the mean of the fine fluxes
+    across each (fx: the three flux arrays)."""
     nv, nd, g, nx = bs.nvar, mesh.ndim, mesh.ng, mesh.nx
-    N1 = [mesh.ncells_axis(1, d) for d in range(3)]
-    for d in range(nd):
-        # fine faces along d over the whole domain: face G between fine cells G-1 and G
-        shape = [N1[t] if t != d else N1[t] + 1 for t in range(3)]
-        ff = np.full((nv, shape[2], shape[1], shape[0]), np.nan)
-        for b in range(bs.nblocks):
-            if mesh.level[b] != 1:
-                continue
-            lc = mesh.loc[b]
-            view = bs.block_view(fx[d], b, nv)
-            sl_src, sl_dst = [], []
-            for t in (2, 1, 0):
-                if t >= nd:
-                    sl_src.append(slice(0, 1))
-                    sl_dst.append(slice(0, 1))
+    nl = int(mesh.level.max()) + 1
+    for L in range(nl - 1):
+        N1 = [mesh.ncells_axis(L + 1, d) for d in range(3)]
+        for d in range(nd):
+            # level-(L+1) faces along d over the whole domain: face G between cells G-1, G
+            shape = [N1[t] if t != d else N1[t] + 1 for t in range(3)]
+            ff = np.full((nv, shape[2], shape[1], shape[0]), np.nan)
+            for b in range(bs.nblocks):
+                if mesh.level[b] != L + 1:
                     continue
-                lo = int(lc[t]) * nx[t]
-                if t == d:
-                    sl_src.append(slice(g, g + nx[t] + 1))
-                    sl_dst.append(slice(lo, lo + nx[t] + 1))
-                else:
-                    sl_src.append(slice(g, g + nx[t]))
-                    sl_dst.append(slice(lo, lo + nx[t]))
-            ff[(slice(None),) + tuple(sl_dst)] = view[(slice(None),) + tuple(sl_src)]
-        # periodic: face N1[d] is face 0
-        edge = [slice(None)] * 4
-        edge[3 - d] = slice(0, 1)
-        last = [slice(None)] * 4
-        last[3 - d] = slice(shape[d] - 1, shape[d])
-        f0, fN = ff[tuple(edge)], ff[tuple(last)]
-        ff[tuple(edge)] = np.where(np.isnan(f0), fN, f0)
-        tr = [t for t in range(nd) if t != d]
-        for b in range(bs.nblocks):
-            if mesh.level[b] != 0:
-                continue
-            lc = mesh.loc[b]
-            bc = bs.blocks["bc"][b]
-            view = bs.block_view(fx[d], b, nv)
-            for side in (0, 1):
-                if bc[2 * d + side] != BC_AMR:
+                lc = mesh.loc[b]
+                view = bs.block_view(fx[d], b, nv)
+                sl_src, sl_dst = [], []
+                for t in (2, 1, 0):
+                    if t >= nd:
+                        sl_src.append(slice(0, 1))
+                        sl_dst.append(slice(0, 1))
+                        continue
+                    lo = int(lc[t]) * nx[t]
+                    if t == d:
+                        sl_src.append(slice(g, g + nx[t] + 1))
+                        sl_dst.append(slice(lo, lo + nx[t] + 1))
+                    else:
+                        sl_src.append(slice(g, g + nx[t]))
+                        sl_dst.append(slice(lo, lo + nx[t]))
+                ff[(slice(None),) + tuple(sl_dst)] = view[(slice(None),) + tuple(sl_src)]
+            # periodic: face N1[d] is face 0
+            edge = [slice(None)] * 4
+            edge[3 - d] = slice(0, 1)
+            last = [slice(None)] * 4
+            last[3 - d] = slice(shape[d] - 1, shape[d])
+            f0, fN = ff[tuple(edge)], ff[tuple(last)]
+            ff[tuple(edge)] = np.where(np.isnan(f0), fN, f0)
+            tr = [t for t in range(nd) if t != d]
+            for b in range(bs.nblocks):
+                if mesh.level[b] != L:
                     continue
-                Fd = 2 * (int(lc[d]) * nx[d] + (nx[d] if side else 0)) % N1[d]
-                # coarse face cells (transverse interior) and their fine children
-                tc = np.meshgrid(*[np.arange(nx[t]) for t in tr], indexing="ij")
-                acc = None
-                for kid in np.ndindex(*([2] * len(tr))):
-                    F = [None, None, None]
-                    F[d] = np.full(tc[0].shape, Fd)
-                    for t, c, q in zip(tr, tc, kid):
-                        F[t] = 2 * (int(lc[t]) * nx[t] + c) + q
+                lc = mesh.loc[b]
+                bc = bs.blocks["bc"][b]
+                view = bs.block_view(fx[d], b, nv)
+                for side in (0, 1):
+                    if bc[2 * d + side] != BC_AMR:
+                        continue
+                    Fd = 2 * (int(lc[d]) * nx[d] + (nx[d] if side else 0)) % N1[d]
+                    # coarse face cells (transverse interior) and their fine children
+                    tc = np.meshgrid(*[np.arange(nx[t]) for t in tr], indexing="ij")
+                    acc = None
+                    for kid in np.ndindex(*([2] * len(tr))):
+                        F = [None, None, None]
+                        F[d] = np.full(tc[0].shape, Fd)
+                        for t, c, q in zip(tr, tc, kid):
+                            F[t] = 2 * (int(lc[t]) * nx[t] + c) + q
+                        for t in range(nd, 3):
+                            F[t] = np.zeros(tc[0].shape, int)
+                        v = ff[(slice(None),) + (F[2], F[1], F[0])]
+                        acc = v.copy() if acc is None else acc + v
+                    if np.isnan(acc).all():
+                        continue  # the other side is coarser: this block is the fine side
+                    assert not np.isnan(acc).any(), "a face half on the finer level"
+                    acc = acc * (1.0 / 2 ** len(tr))
+                    dst = [None, None, None]
+                    dst[d] = np.full(tc[0].shape, g + (nx[d] if side else 0))
+                    for t, c in zip(tr, tc):
+                        dst[t] = g + c
                     for t in range(nd, 3):
-                        F[t] = np.zeros(tc[0].shape, int)
-                    v = ff[(slice(None),) + (F[2], F[1], F[0])]
-                    acc = v.copy() if acc is None else acc + v
-                acc = acc * (1.0 / 2 ** len(tr))
-                dst = [None, None, None]
-                dst[d] = np.full(tc[0].shape, g + (nx[d] if side else 0))
-                for t, c in zip(tr, tc):
-                    dst[t] = g + c
-                for t in range(nd, 3):
-                    dst[t] = np.zeros(tc[0].shape, int)
-                view[(slice(None),) + (dst[2], dst[1], dst[0])] = acc
+                        dst[t] = np.zeros(tc[0].shape, int)
+                    view[(slice(None),) + (dst[2], dst[1], dst[0])] = acc
 
 
 class AmrState:
-    """The oracle State of a problem on a two-level mesh: the stage sequence of the plain
+    """The oracle State of a problem on a multilevel mesh: the stage sequence of the plain
     driver (fluxes, flux correction, update, ghost exchange + cross-level fill,
     ConsToPrim) composed from the oracle's task functions."""
 

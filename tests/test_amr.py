@@ -1,4 +1,4 @@
-"""Two-level meshes (a 2:1 jump): the host plan (apophis_amd/amr.py) and the oracle's
+"""Multilevel meshes (2:1 jumps, two and three levels): the host plan (apophis_amd/amr.py) and the oracle's
 restatement (oracle/amr_oracle.py) of Parthenon's multilevel exchange and flux correction.
 
 The reference delegates both to Parthenon, whose source is not in the reference tree, so
@@ -24,11 +24,25 @@ MESHES = {
     "2d_L": dict(ndim=2, nbase=(4, 4), block_nx=(8, 8), ng=2, refined=[(1, 1), (2, 1), (1, 2)]),
     "2d_ppm": dict(ndim=2, nbase=(4, 3), block_nx=(8, 8), ng=3, refined=[(1, 1), (2, 1)]),
     "3d": dict(ndim=3, nbase=(3, 3, 2), block_nx=(6, 6, 6), ng=2, refined=[(1, 1, 0)]),
+    # three levels: a level-2 region inside a level-1 region (refined as (level, loc)),
+    # with blocks that hold faces to both other levels
+    "2d_3lev": dict(ndim=2, nbase=(4, 4), block_nx=(8, 8), ng=2,
+                    nodes=[(0, (1, 1)), (0, (2, 1)), (0, (1, 2)), (0, (2, 2)),
+                           (1, (3, 3)), (1, (4, 4))]),
+    "2d_3lev_ppm": dict(ndim=2, nbase=(4, 4), block_nx=(6, 6), ng=3,
+                        nodes=[(0, (1, 1)), (0, (2, 1)), (0, (1, 2)), (0, (2, 2)),
+                               (1, (3, 4))]),
+    "3d_3lev": dict(ndim=3, nbase=(4, 4, 2), block_nx=(4, 4, 4), ng=2,
+                    nodes=[(0, (x, y, z)) for x in (1, 2) for y in (1, 2) for z in (0, 1)]
+                    + [(1, (3, 3, 1))]),
 }
 
 
 def _mesh(name):
     k = MESHES[name]
+    if "nodes" in k:
+        return amr.AmrMesh(k["ndim"], k["nbase"], k["block_nx"], k["ng"], (0.0, 0.0, 0.0),
+                           (1.0, 1.0, 1.0), k["nodes"], max_level=2)
     return amr.TwoLevelMesh(k["ndim"], k["nbase"], k["block_nx"], k["ng"], (0.0, 0.0, 0.0),
                             (1.0, 1.0, 1.0), k["refined"])
 
@@ -172,7 +186,9 @@ def _totals(bs, u):
     return t
 
 
-@pytest.mark.parametrize("name,recon", [("2d_L", "plm"), ("2d_ppm", "ppm"), ("3d", "plm")])
+@pytest.mark.parametrize("name,recon", [("2d_L", "plm"), ("2d_ppm", "ppm"), ("3d", "plm"),
+                                        ("2d_3lev", "plm"), ("2d_3lev_ppm", "ppm"),
+                                        ("3d_3lev", "plm")])
 def test_flux_correction_conserves_across_levels(name, recon):
     m = _mesh(name)
     p = _problem(m, recon=recon)
@@ -213,10 +229,77 @@ def test_ops_write_ghosts_and_read_interiors(name):
             v = op["val"][k]
             used.update(range(int(v["first"]), int(v["first"]) + int(v["count"])))
     assert all(_is_interior(bs, refs["offset"][i]) for i in used)
-    # fixes write level-0 blocks and read level-1 blocks only
+    # a fix writes a face of a level-L block and reads faces of level-(L+1) blocks only
     blk = lambda off: int(np.searchsorted(bs.nvar * bs.blocks["cell_base"], off,
                                           side="right")) - 1
-    assert all(m.level[blk(f)] == 0 for f in fixes["dst"])
     for f in fixes:
         r = refs[f["val"]["first"]: f["val"]["first"] + f["val"]["count"]]
-        assert all(m.level[blk(o)] == 1 for o in r["offset"])
+        assert all(m.level[blk(o)] == m.level[blk(f["dst"])] + 1 for o in r["offset"])
+    assert len(fixes) > 0 and len(np.unique(fixes["dst"] * 3 + fixes["dir"])) == len(fixes)
+
+
+@pytest.mark.parametrize("name", list(MESHES))
+def test_fused_interface_table_covers_every_slot_once(name):
+    """The fused form's interface table (amr.fused_faces), read with the kernel's slot
+    formula: every fine face cell on a face to a coarser leaf writes one slot of the
+    region owned by the coarse face across it, at its own child position, and every
+    slot of every region is written exactly once."""
+    m = _mesh(name)
+    bs = m.block_set()
+    nd, nx, nv = m.ndim, m.nx, bs.nvar
+    faces, level, total = amr.fused_faces(m, bs)
+    nch = 2 ** (nd - 1)
+    owner = {}  # slot -> (coarse block, face, coarse global transverse cell (A, B), child)
+    trans = lambda d: ((1 if d == 0 else 0), (1 if d == 2 else 2))
+    for b in range(bs.nblocks):
+        for f in range(6):
+            af = faces[b * 6 + f]
+            d = f // 2
+            is_amr = d < nd and bs.blocks["bc"][b][f] == amr.BC_AMR
+            assert (af["role"] != 0) == is_amr
+            if af["role"] != 2:
+                continue
+            A, B = trans(d)
+            for cb in range(nx[B] if B < nd else 1):
+                for ca in range(nx[A]):
+                    for k in range(nch):
+                        slot = ((cb * af["na"] + ca) * nch + k) * nv + af["base"]
+                        assert slot not in owner
+                        ga = int(m.loc[b][A]) * nx[A] + ca
+                        gb = int(m.loc[b][B]) * nx[B] + cb if B < nd else 0
+                        owner[slot] = (b, f, (ga, gb), k)
+    assert len(owner) * nv == total
+    written = set()
+    for b in range(bs.nblocks):
+        for f in range(6):
+            af = faces[b * 6 + f]
+            if af["role"] != 1:
+                continue
+            d = f // 2
+            A, B = trans(d)
+            for cb in range(nx[B] if B < nd else 1):
+                for ca in range(nx[A]):
+                    fa, fb = af["qa"] + ca, af["qb"] + cb
+                    child = (fa & 1) * 2 + (fb & 1) if nd == 3 else (fa & 1)
+                    slot = (((fb >> 1) * af["na"] + (fa >> 1)) * nch + child) * nv + af["base"]
+                    assert slot in owner and slot not in written
+                    written.add(slot)
+                    cblk, cf, (ga, gb), k = owner[slot]
+                    # one level coarser, the opposite face of the same axis, above this cell
+                    assert level[cblk] == level[b] - 1 and cf == (f ^ 1)
+                    assert k == child
+                    assert (int(m.loc[b][A]) * nx[A] + ca) // 2 == ga
+                    if B < nd:
+                        assert (int(m.loc[b][B]) * nx[B] + cb) // 2 == gb
+    assert written == set(owner)
+
+
+def test_unbalanced_or_orphan_trees_are_refused():
+    base = dict(ndim=2, nbase=(4, 4), block_nx=(8, 8), ng=2, xmin=(0, 0, 0), xmax=(1, 1, 1))
+    with pytest.raises(ValueError, match="balance"):
+        # a level-2 region next to a level-0 leaf
+        amr.AmrMesh(**base, refined=[(0, (1, 1)), (1, (2, 2))])
+    with pytest.raises(ValueError, match="parent"):
+        amr.AmrMesh(**base, refined=[(1, (2, 2))])
+    with pytest.raises(ValueError, match="max_level"):
+        amr.AmrMesh(**base, refined=[(0, (1, 1))], max_level=0)

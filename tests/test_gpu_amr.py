@@ -1,4 +1,4 @@
-"""Two-level meshes on the GPU through the C ABI: aph_amr_fill / aph_amr_flux_correct
+"""Multilevel meshes (two and three levels) on the GPU through the C ABI: aph_amr_fill / aph_amr_flux_correct
 against the oracle's whole-level restatement (oracle/amr_oracle.py) bitwise, whole plain
 steps with the flux correction against the oracle's step bitwise, conservation, and the
 error paths (a corrupt plan, the fused form on a multilevel set)."""
@@ -42,7 +42,10 @@ def test_amr_fill_and_flux_correct_match_oracle(gpu_ctx, name):
 
 @pytest.mark.parametrize("name,recon,riemann", [("2d_L", "plm", "hllc"),
                                                 ("2d_ppm", "ppm", "hllc"),
-                                                ("3d", "plm", "hlle")])
+                                                ("3d", "plm", "hlle"),
+                                                ("2d_3lev", "plm", "hllc"),
+                                                ("2d_3lev_ppm", "ppm", "hllc"),
+                                                ("3d_3lev", "plm", "hlle")])
 def test_plain_amr_steps_match_oracle(gpu_ctx, name, recon, riemann):
     print(f"[case] amr steps {name} {recon} {riemann}", flush=True)
     m = _mesh(name)
@@ -92,10 +95,43 @@ def test_amr_error_paths(gpu_ctx):
                                         fixes.ctypes.data_as(vp), len(fixes),
                                         refs.ctypes.data_as(vp), len(refs), ctypes.byref(h))
     assert rc != 0
+    # a face corrected twice, and a corrected face that is another fix's source
+    def create(fx, rf):
+        return _lib.lib().aph_amr_plan_create(gpu_ctx.h, md.dbs, ops.ctypes.data_as(vp),
+                                              len(ops), fx.ctypes.data_as(vp), len(fx),
+                                              rf.ctypes.data_as(vp), len(rf), ctypes.byref(h))
+    assert create(np.concatenate([fixes, fixes[:1]]), refs) != 0
+    assert "twice" in _lib.lib().aph_last_error(gpu_ctx.h).decode()
+    j = int(np.flatnonzero(fixes["dir"] == fixes["dir"][0])[1])
+    rf = np.concatenate([refs, np.array([(fixes["dst"][0], fixes["dst_stride"][0])],
+                                        dtype=refs.dtype)])
+    bad = fixes.copy()
+    bad["val"]["first"][j], bad["val"]["count"][j] = len(refs), 1
+    assert create(bad, rf) != 0
+    assert "both corrected and a source" in _lib.lib().aph_last_error(gpu_ctx.h).decode()
     p = _problem(m)
     fd = hydro.FusedDriver(gpu_ctx, p.bs, p.cons, "plm", "hllc", p.eos, p.cfl)
     with pytest.raises(hydro._lib.HydroError, match="aph_plan_set_amr"):
         fd.step(fixed_dt=True)
+    # an interface table whose fine side does not sit one level below its coarse region
+    import ctypes as ct
+    m3 = _mesh("2d_3lev")
+    p3 = _problem(m3)
+    f3 = hydro.FusedDriver(gpu_ctx, p3.bs, p3.cons, "plm", "hllc", p3.eos, p3.cfl)
+    plan3 = amr.AmrPlan(gpu_ctx, f3.device_bs.h, m3, p3.bs)
+    faces, level, nbuf = amr.fused_faces(m3, p3.bs)
+    args = lambda fc, lv, n: (gpu_ctx.h, f3.plan, plan3.h, fc.ctypes.data_as(ct.c_void_p),
+                              lv.ctypes.data_as(ct.c_void_p), ct.c_int64(n))
+    assert _lib.lib().aph_plan_set_amr(*args(faces, level, nbuf)) == 0
+    bad = level.copy()
+    bad[bad == 2] = 0
+    assert _lib.lib().aph_plan_set_amr(*args(faces, bad, nbuf)) != 0
+    assert "one level coarser" in _lib.lib().aph_last_error(gpu_ctx.h).decode()
+    bad = faces.copy()
+    r1 = np.flatnonzero(bad["role"] == 1)[0]
+    bad["qa"][r1] = bad["na"][r1] + 2  # past the end of its coarse region
+    assert _lib.lib().aph_plan_set_amr(*args(bad, level, nbuf)) != 0
+    assert _lib.lib().aph_plan_set_amr(*args(faces, level, nbuf - 1)) != 0
     # and the adjoint refuses a multilevel plan
     from apophis_amd import adjoint
     fa = hydro.FusedDriver(gpu_ctx, p.bs, p.cons, "plm", "hllc", p.eos, p.cfl, amr_mesh=m)
@@ -106,11 +142,14 @@ def test_amr_error_paths(gpu_ctx):
 
 @pytest.mark.parametrize("name,recon,riemann", [("2d_L", "plm", "hllc"),
                                                 ("2d_ppm", "ppm", "hllc"),
-                                                ("3d", "plm", "hlle")])
+                                                ("3d", "plm", "hlle"),
+                                                ("2d_3lev", "plm", "hllc"),
+                                                ("2d_3lev_ppm", "ppm", "hllc"),
+                                                ("3d_3lev", "plm", "hlle")])
 def test_fused_amr_steps_match_oracle(gpu_ctx, name, recon, riemann):
-    """The batched form on a two-level mesh: cross-level ghosts from aph_amr_fill, fine
-    blocks first, coarse faces corrected from the interface buffer -- the same bits as
-    the plain form's step with aph_amr_flux_correct."""
+    """The batched form on a multilevel mesh: cross-level ghosts from aph_amr_fill, one
+    launch per level, the finest first, coarse faces corrected from the interface buffer
+    -- the same bits as the plain form's step with aph_amr_flux_correct."""
     print(f"[case] fused amr steps {name} {recon} {riemann}", flush=True)
     m = _mesh(name)
     p = _problem(m, recon=recon)

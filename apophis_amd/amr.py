@@ -504,7 +504,9 @@ def regrid(mesh: AmrMesh, bs: BlockSet, cons, tags):
     * then every leaf that touches a leaf two levels finer is refined (2:1 balance);
     * a sibling group whose 2^ndim leaves were all tagged -1 (and none was refined) is
       merged into its parent, unless the parent would touch a leaf two levels finer;
-      groups are taken in (level, position) order.
+      groups are taken finest level first, then by position, so a merge on a finer level
+      can clear the way for one on a coarser level in the same regrid (a block loses at
+      most one level per regrid).
     Returns (new_mesh, new_bs, new_cons): each new leaf takes its region of the per-level
     arrays of ``level_values`` -- a kept leaf its data, a merged parent the mean of its
     children (dz, dy, dx order), a new child the minmod prolongation of its parent's
@@ -550,7 +552,7 @@ def regrid(mesh: AmrMesh, bs: BlockSet, cons, tags):
                 changed = True
     old_index = mesh.index
     groups = sorted({(lv - 1, mesh.parent(lc)) for b, (lv, lc) in enumerate(leaves)
-                     if lv > 0 and tags[b] == -1})
+                     if lv > 0 and tags[b] == -1}, key=lambda P: (-P[0], P[1]))
     for P in groups:
         lv = P[0] + 1
         kids = mesh.children(P[1])

@@ -1,4 +1,4 @@
-// aph_fused_amr.hip -- the two-level-mesh variant of k_fused (block faces to the other
+// aph_fused_amr.hip -- the multilevel-mesh variant of k_fused (block faces to another
 // level record or correct their interface fluxes), NV = 5 and 6, ndim 2 and 3.  Compiled
 // apart from aph_fused.hip so the single-level kernels carry none of that code and the
 // instantiation sets build in parallel.

@@ -39,7 +39,7 @@ struct FusedArgs {
   double cfl;
   int has_un;      // u_n is a separate register (every stage but the first)
   int last;        // last stage of the step: estimate the next dt
-  const aph_amr_face *amr_face;  // two-level mesh: [block][6] interface roles, or null
+  const aph_amr_face *amr_face;  // multilevel mesh: [block][6] interface roles, or null
   double *amr_buf;               // interface flux buffer (fine side writes, coarse reads)
 };
 
@@ -259,7 +259,7 @@ __device__ __forceinline__ void sweep(const FusedArgs &a, const aph_block &bk, i
   }
 }
 
-// AMR: the two-level-mesh variant (interface fluxes recorded / corrected at block faces
+// AMR: the multilevel-mesh variant (interface fluxes recorded / corrected at block faces
 // to the other level); the single-level kernel carries none of that code.
 template <int RECON, int RS, int NV, int NDIM, bool AMR = false>
 __global__ void __launch_bounds__(64) k_fused(FusedArgs a) {
@@ -326,7 +326,7 @@ using PlanDtFn = void (*)(const aph_block *, aph_eos, const double *, double, do
 // The stage kernel / initial time-step kernel for NV = 12 or 14 (aph_fused_wide.hip);
 // null for an unsupported combination.
 KernelFn pick_kernel_wide(int recon, int rs, int nv, int ndim);
-// The two-level-mesh stage kernel, NV = 5 or 6, ndim 2 or 3 (aph_fused_amr.hip).
+// The multilevel-mesh stage kernel, NV = 5 or 6, ndim 2 or 3 (aph_fused_amr.hip).
 KernelFn pick_kernel_amr(int recon, int rs, int nv, int ndim);
 PlanDtFn pick_plan_dt_wide(int nv, int ndim);
 

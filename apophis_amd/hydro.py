@@ -380,7 +380,7 @@ class ApophisDriver:
     def __init__(self, pkg, md, g=(0.0, 0.0, 0.0), pack_size=None, amr=None):
         self.pkg = pkg
         self.md = md
-        self.amr = amr  # amr.AmrPlan of a two-level mesh, or None
+        self.amr = amr  # amr.AmrPlan of a multilevel mesh, or None
         self.eos = pkg.param("eos")
         self.cfl = pkg.param("cfl")
         self.g = tuple(float(v) for v in g)
@@ -488,7 +488,7 @@ class FusedDriver:
         if integrator != "rk2":
             self.set_integrator(integrator)
         self.amr = None
-        if amr_mesh is not None:  # a two-level mesh (amr.TwoLevelMesh) of this block set
+        if amr_mesh is not None:  # a multilevel mesh (amr.AmrMesh) of this block set
             from . import amr as _amr
             self.amr = _amr.AmrPlan(ctx, self.device_bs.h, amr_mesh, bs)
             faces, level, nbuf = _amr.fused_faces(amr_mesh, bs)

@@ -501,3 +501,70 @@ def blast2d_two_level(nbase=48, n=32, nref=16, seed=20240601, recon="ppm", riema
     p = Problem("blast2d_two_level", bs, cons, make_eos(gm1), recon, riemann, cfl,
                 meta={"nbase": nbase, "nref": nref})
     return p, m
+
+
+# ---------------------------------------------------------------- the adaptive blast deck
+# inputs/blast.in: a 128^2 mesh of 16^2 blocks on [-1/2, 1/2]^2, periodic, three levels
+# (refinement = adaptive, numlevel = 3) on the pressure-gradient criterion (threshold 0.1),
+# PLM + HLLC, vl2, cfl 0.4, tlim 1e-2; <eos> is not set, so the EOS takes the default
+# gm1 = 0.6666667 (hydro.cpp:262) while the problem generator uses hydro/gamma.
+BLAST_DECK = dict(nx=(128, 128), block_nx=(16, 16), numlevel=3, xmin=(-0.5, -0.5, -0.5),
+                  xmax=(0.5, 0.5, 0.5), recon="plm", riemann="hllc", integrator="vl2",
+                  cfl=0.4, tlim=1.0e-2, gamma=1.666666666666667, eos_gm1=0.6666667,
+                  threshold=0.1, pamb=1.0e-3, prat=1.6e8, radius_outer=0.1,
+                  radius_inner=0.0, damb=1.0, drat=1.0, x0=(0.0, 0.0, 0.0))
+
+
+def blast_deck_pin(deck=None):
+    """The deck's parameter input for hydro.initialize_hydro."""
+    k = dict(BLAST_DECK, **(deck or {}))
+    return {"parthenon/time": {"cfl": k["cfl"], "integrator": k["integrator"]},
+            "parthenon/mesh": {"nghost": NEED_GHOST[k["recon"]]},
+            "hydro": {"reconstruction": k["recon"], "riemann": k["riemann"],
+                      "gamma": k["gamma"]},
+            "eos": {"gm1": k["eos_gm1"]},
+            "refinement": {"type": "pressure_gradient",
+                           "threshold_pressure_gradient": k["threshold"]}}
+
+
+def blast_pgen(deck=None):
+    """The blast problem generator (blast.cpp:16-71) as a function (mesh, bs) -> cons:
+    uniform (damb, pamb) outside radius_outer, (drat damb, prat pamb) inside
+    rin = radius_outer - radius_inner, and between them the logarithmic ramp."""
+    k = dict(BLAST_DECK, **(deck or {}))
+    rout = k["radius_outer"]
+    rin = rout - k["radius_inner"]
+    da, pa, drat, prat = k["damb"], k["pamb"], k["drat"], k["prat"]
+    gm1 = k["gamma"] - 1.0
+
+    def pgen(mesh, bs):
+        cons = np.zeros(bs.field_size(bs.nvar))
+        for b in range(bs.nblocks):
+            x, y, z = bs.cell_centers(b)
+            if bs.ndim < 3:
+                z = np.zeros_like(x)
+            rad = np.sqrt((x - k["x0"][0]) ** 2 + (y - k["x0"][1]) ** 2 + (z - k["x0"][2]) ** 2)
+            den = np.full(x.shape, da)
+            pres = np.full(x.shape, pa)
+            inner = rad < rin
+            den[inner], pres[inner] = drat * da, prat * pa
+            ramp = (rad < rout) & ~inner
+            if ramp.any():
+                f = (rad[ramp] - rin) / (rout - rin)
+                den[ramp] = np.exp((1.0 - f) * np.log(drat * da) + f * np.log(da))
+                pres[ramp] = np.exp((1.0 - f) * np.log(prat * pa) + f * np.log(pa))
+            _set_interior(bs, cons, b, den, 0.0, 0.0, 0.0, pres / gm1)
+        return cons
+
+    return pgen
+
+
+def blast_deck_base_mesh(deck=None):
+    """The deck's root-level mesh (no block refined yet) as an amr.AmrMesh."""
+    from .amr import AmrMesh
+
+    k = dict(BLAST_DECK, **(deck or {}))
+    nd = len(k["nx"])
+    nbase = tuple(k["nx"][d] // k["block_nx"][d] for d in range(nd))
+    return AmrMesh(nd, nbase, k["block_nx"], NEED_GHOST[k["recon"]], k["xmin"], k["xmax"],
+                   [], max_level=k["numlevel"] - 1)

@@ -1,0 +1,108 @@
+"""The adaptive loop on the GPU (apophis_amd/adaptive.py) on a reduced blast deck
+(inputs/blast.in at 64^2 with 8^2 blocks: three levels, PLM + HLLC, vl2): the adaptive
+start with the device criterion builds the same mesh and state as with the oracle's
+criterion; the plain driver's cycles (step, device tags, derefinement delay, regrid, new
+plans, dt policy) follow the oracle's cycles bitwise through refinements and merges; the
+fused driver follows the plain one bitwise."""
+import numpy as np
+import pytest
+import torch
+
+from apophis_amd import adaptive, amr, hydro, problems
+from oracle import amr_oracle as ao
+from tests.test_adaptive import oracle_tagger
+
+pytestmark = pytest.mark.gpu
+
+DECK = dict(nx=(64, 64), block_nx=(8, 8))
+NCYCLES = 8
+DEREF = 2
+
+
+def _start(tagger):
+    return adaptive.initial_mesh(problems.blast_deck_base_mesh(DECK),
+                                 problems.blast_pgen(DECK), tagger)
+
+
+def _pkg():
+    return hydro.initialize_hydro(problems.blast_deck_pin(DECK))
+
+
+def _oracle_cycles(pkg, m, bs, cons):
+    """The cycle of the adaptive driver written out with the oracle's step, criterion
+    and ghost fill, and the host regrid."""
+    k = dict(problems.BLAST_DECK, **DECK)
+    mk = lambda bs_, c_: problems.Problem("deck", bs_, c_, pkg.param("eos"), k["recon"],
+                                          k["riemann"], k["cfl"], integrator="vl2")
+    st = ao.AmrState(mk(bs, cons), m)
+    dt = st.estimate_dt()
+    cnt = adaptive.DerefineCounter(DEREF)
+    out = []
+    for _ in range(NCYCLES):
+        est = st.step(dt)
+        tags = ao.refinement_tag(st.bs, st.prim, *pkg.param("refinement/criterion"))[0]
+        fl = cnt.flags(m, tags)
+        fl[(fl > 0) & (m.level >= m.max_level)] = 0
+        fl[(fl < 0) & (m.level == 0)] = 0
+        if fl.any():
+            new, nbs, nc = amr.regrid(m, st.bs, st.u0, fl)
+            if new.refined != m.refined:
+                m = new
+                st = ao.AmrState(mk(nbs, nc), m)
+                est = st.estimate_dt()
+        out.append((dt, set(m.refined), st.bs, st.u0.copy()))
+        dt = min(2.0 * dt, est)
+    return out
+
+
+def test_adaptive_start_matches_oracle_criterion(gpu_ctx):
+    print("[case] adaptive start", flush=True)
+    pkg = _pkg()
+    m, bs, cons = _start(lambda mm, bb, cc: adaptive.device_tags(gpu_ctx, pkg, mm, bb, cc))
+    rm, rbs, rc = _start(oracle_tagger(pkg))
+    assert m.refined == rm.refined and m.nlevels == 3
+    assert np.array_equal(cons, rc)
+
+
+@pytest.fixture(scope="module")
+def reference_run():
+    pkg = _pkg()
+    m, bs, cons = _start(oracle_tagger(pkg))
+    return pkg, m, bs, cons, _oracle_cycles(pkg, m, bs, cons)
+
+
+def _run(gpu_ctx, form, reference_run):
+    pkg, m, bs, cons, ref = reference_run
+    drv = adaptive.AdaptiveDriver(gpu_ctx, pkg, m, bs, cons, form=form, derefine_count=DEREF)
+    got = []
+    for _ in range(NCYCLES):
+        dt = drv.cycle()
+        torch.cuda.synchronize()
+        got.append((dt, set(drv.mesh.refined), drv.bs, drv.state().cpu().numpy()))
+    return drv, got
+
+
+def test_plain_adaptive_cycles_match_oracle(gpu_ctx, reference_run):
+    print("[case] adaptive cycles plain", flush=True)
+    drv, got = _run(gpu_ctx, "plain", reference_run)
+    ref = reference_run[-1]
+    assert drv.stats["regrids"] >= 2
+    levels = set()
+    for c, ((dt, refd, bs, u), (rdt, rrefd, rbs, ru)) in enumerate(zip(got, ref)):
+        assert dt == rdt, c
+        assert refd == rrefd, c
+        mask = np.tile(bs.interior_mask(), bs.nvar)
+        assert np.array_equal(u[mask], ru[mask]), c
+        levels.add(len(refd))
+    assert len(levels) >= 2  # the tree changed along the run
+
+
+def test_fused_adaptive_cycles_match_plain(gpu_ctx, reference_run):
+    print("[case] adaptive cycles fused", flush=True)
+    drv, got = _run(gpu_ctx, "fused", reference_run)
+    ref = reference_run[-1]
+    for c, ((dt, refd, bs, u), (rdt, rrefd, rbs, ru)) in enumerate(zip(got, ref)):
+        assert refd == rrefd, c
+        assert dt == rdt, c
+        mask = np.tile(bs.interior_mask(), bs.nvar)
+        assert np.array_equal(u[mask], ru[mask]), c

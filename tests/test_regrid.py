@@ -87,3 +87,108 @@ def test_adaptive_run_conserves_across_regrids():
     np.testing.assert_allclose(_tot(p.bs, st.u0), t0, rtol=1e-13,
                                atol=1e-13 * np.abs(t0).max())
     assert (m.level == 1).sum() > 0
+
+
+def _three_level(nd=2):
+    """A level-2 region inside a level-1 region on a 4^nd base of 8^nd-cell blocks."""
+    if nd == 2:
+        nodes = [(0, (x, y)) for x in (1, 2) for y in (1, 2)] + [(1, (3, 3))]
+        return amr.AmrMesh(2, (4, 4), (8, 8), 2, (0, 0, 0), (1, 1, 1), nodes, max_level=2)
+    nodes = [(0, (x, y, z)) for x in (1, 2) for y in (1, 2) for z in (0, 1)] + [(1, (3, 3, 1))]
+    return amr.AmrMesh(3, (4, 4, 2), (4, 4, 4), 2, (0, 0, 0), (1, 1, 1), nodes, max_level=2)
+
+
+def _linear(bs, nd, nv=5):
+    u = np.zeros(bs.field_size(nv))
+    for b in range(bs.nblocks):
+        x, y, z = bs.cell_centers(b)
+        v = bs.interior(u, b, nv)
+        for n in range(nv):
+            v[n] = 1.0 + (n + 1) * 0.1 * x - 0.3 * y + (0.2 * z if nd == 3 else 0.0)
+    return u
+
+
+def test_refinement_next_to_a_coarser_leaf_refines_it_too():
+    """2:1 balance: refining a level-1 leaf that touches level-0 leaves refines those
+    leaves as well, and the result is a valid (balanced) tree."""
+    m = _three_level()
+    bs = m.block_set()
+    b = m.index[(1, (2, 2, 0))]  # level-1 corner of the refined region
+    tags = np.zeros(bs.nblocks, np.int32)
+    tags[b] = 1
+    nm, nbs, nc = amr.regrid(m, bs, _linear(bs, 2), tags)
+    assert (1, (2, 2, 0)) in nm.refined
+    # the level-0 leaves around base node (1, 1) had to refine
+    for lc in [(0, 0, 0), (1, 0, 0), (0, 1, 0)]:
+        assert (0, lc) in nm.refined and (0, lc) not in m.refined
+    assert nm.nlevels == 3
+    np.testing.assert_allclose(_tot(nbs, nc), _tot(bs, _linear(bs, 2)), rtol=1e-14)
+
+
+def test_merge_blocked_by_balance_and_levels_capped():
+    m = _three_level()
+    bs = m.block_set()
+    # merging the level-1 group of base (1, 1) would put level 0 next to level 2
+    tags = np.zeros(bs.nblocks, np.int32)
+    for c in [(2, 2, 0), (3, 2, 0), (2, 3, 0)]:
+        tags[m.index[(1, c)]] = -1
+    for c in m.children((3, 3, 0)):
+        tags[m.index[(2, c)]] = 1  # at max_level: no effect
+    nm, nbs, _ = amr.regrid(m, bs, _linear(bs, 2), tags)
+    assert nm.refined == m.refined   # (3, 3) is refined, so the group is not all leaves
+    # a group of leaves whose merge would touch level 2: base (2, 1) next to node (1,(3,3))
+    tags[:] = 0
+    for c in m.children((2, 1, 0)):
+        tags[m.index[(1, c)]] = -1
+    nm, _, _ = amr.regrid(m, bs, _linear(bs, 2), tags)
+    assert (0, (2, 1, 0)) in nm.refined
+    # a group away from level 2 merges: drop level 2 first, then the base (2, 2) group
+    tags[:] = 0
+    for c in m.children((3, 3, 0)):
+        tags[m.index[(2, c)]] = -1
+    m2, bs2, u2 = amr.regrid(m, bs, _linear(bs, 2), tags)
+    assert (1, (3, 3, 0)) not in m2.refined and m2.nlevels == 2
+    t2 = np.zeros(bs2.nblocks, np.int32)
+    for c in m2.children((2, 1, 0)):
+        t2[m2.index[(1, c)]] = -1
+    m3, _, _ = amr.regrid(m2, bs2, u2, t2)
+    assert (0, (2, 1, 0)) not in m3.refined
+
+
+@pytest.mark.parametrize("nd", [2, 3])
+def test_three_level_refine_merge_round_trip_and_linear_exactness(nd):
+    m = _three_level(nd)
+    bs = m.block_set()
+    u = _linear(bs, nd)
+    rng = np.random.default_rng(7)
+    noisy = u + 0.01 * rng.random(u.shape)
+    # refine every level-1 leaf that stays balanced after refinement: whole level 2 ring
+    tags = np.zeros(bs.nblocks, np.int32)
+    lvl1 = [b for b in range(bs.nblocks) if m.level[b] == 1]
+    tags[lvl1] = 1
+    nm, nbs, nc = amr.regrid(m, bs, noisy, tags)
+    assert nm.nlevels == 3 and (nm.level == 2).sum() > (m.level == 2).sum()
+    np.testing.assert_allclose(_tot(nbs, nc), _tot(bs, noisy), rtol=1e-13)
+    # linear data: every new child (prolongated through one or two levels) is exact
+    _, lbs, lc = amr.regrid(m, bs, u, tags)
+    for b in range(lbs.nblocks):
+        x, y, z = lbs.cell_centers(b)
+        v = lbs.interior(lc, b, 5)
+        h = 2.5 * max(lbs.blocks["dx"][b][:nd]) * 4
+        inner = (x > h) & (x < 1 - h) & (y > h) & (y < 1 - h)
+        if nd == 3:
+            inner &= (z > h) & (z < 1 - h)
+        for n in range(5):
+            want = 1.0 + (n + 1) * 0.1 * x - 0.3 * y + (0.2 * z if nd == 3 else 0.0)
+            np.testing.assert_allclose(v[n][inner], want[inner], rtol=1e-13, atol=1e-13)
+    # merge every new leaf back (the level-2 ones first, which frees the balance-refined
+    # level-1 groups): the original mesh and data come back
+    back = np.zeros(nbs.nblocks, np.int32)
+    for b in range(nbs.nblocks):
+        lv = int(nm.level[b])
+        if lv > 0 and (lv - 1, nm.parent(tuple(int(v) for v in nm.loc[b]))) not in m.refined:
+            back[b] = -1
+    m3, bs3, c3 = amr.regrid(nm, nbs, nc, back)
+    assert m3.refined == m.refined
+    mask = np.tile(bs.interior_mask(), bs.nvar)
+    np.testing.assert_allclose(c3[mask], noisy[mask], rtol=2e-15, atol=0)

@@ -54,6 +54,8 @@ _SIGS = {
     "aph_global_sum": (_i32, [_vp, _vp, _vp, _i32, _vp]),
     "aph_amr_plan_create": (_i32, [_vp, _vp, _vp, _i64, _vp, _i64, _vp, _i64, _pp]),
     "aph_amr_plan_destroy": (_i32, [_vp, _vp]),
+    "aph_amr_build": (_i32, [_vp, _i32, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                             _vp, ctypes.c_char_p, ctypes.c_size_t]),
     "aph_amr_fill": (_i32, [_vp, _vp, _vp]),
     "aph_amr_flux_correct": (_i32, [_vp, _vp, _vp, _vp, _vp]),
     "aph_plan_set_amr": (_i32, [_vp, _vp, _vp, _vp, _vp, _i64]),

@@ -140,6 +140,13 @@ class AmrMesh:
         """Nodes per axis d at a level."""
         return self.nbase[d] * (2 ** level if d < self.ndim else 1)
 
+    def _lattice(self, level):
+        cache = self.__dict__.setdefault("_lat", {})
+        n = cache.get(level)
+        if n is None:
+            n = cache[level] = tuple(self.nlat(level, d) for d in range(3))
+        return n
+
     def ncells_axis(self, level, d):
         return self.nlat(level, d) * self.nx[d]
 
@@ -149,11 +156,14 @@ class AmrMesh:
 
     def offsets(self):
         """The 3^ndim - 1 neighbour offsets."""
-        return [o for o in itertools.product((-1, 0, 1), repeat=3)
-                if o != (0, 0, 0) and all(o[d] == 0 for d in range(self.ndim, 3))]
+        if getattr(self, "_offsets", None) is None:
+            self._offsets = [o for o in itertools.product((-1, 0, 1), repeat=3)
+                             if o != (0, 0, 0) and all(o[d] == 0 for d in range(self.ndim, 3))]
+        return self._offsets
 
     def shift(self, level, lc, o):
-        return tuple((lc[d] + o[d]) % self.nlat(level, d) for d in range(3))
+        n = self._lattice(level)
+        return ((lc[0] + o[0]) % n[0], (lc[1] + o[1]) % n[1], (lc[2] + o[2]) % n[2])
 
     def parent(self, lc):
         return tuple(lc[d] // 2 if d < self.ndim else 0 for d in range(3))
@@ -411,12 +421,42 @@ def fused_faces(mesh: AmrMesh, bs: BlockSet):
     return faces, mesh.level.astype(np.int32), total
 
 
+def native_plan(mesh: AmrMesh, bs: BlockSet):
+    """(ops, fixes, refs) of ``amr_plan``, built by the library's tree (aph_amr_build):
+    the same lists, in milliseconds rather than seconds on large meshes."""
+    L = _lib.lib()
+    vp = ctypes.c_void_p
+    nb = bs.nblocks
+    blocks = np.ascontiguousarray(bs.blocks)
+    nbase = np.array(mesh.nbase, dtype=np.int32)
+    level = np.ascontiguousarray(mesh.level, dtype=np.int32)
+    loc = np.ascontiguousarray(mesh.loc, dtype=np.int64)
+    err = ctypes.create_string_buffer(512)
+    n = [ctypes.c_int64(0) for _ in range(3)]
+    args = lambda o, f, r: (blocks.ctypes.data_as(vp), nb, mesh.ndim, bs.nvar,
+                            nbase.ctypes.data_as(vp), level.ctypes.data_as(vp),
+                            loc.ctypes.data_as(vp), o, ctypes.byref(n[0]), f,
+                            ctypes.byref(n[1]), r, ctypes.byref(n[2]), err, len(err))
+    rc = L.aph_amr_build(*args(None, None, None))
+    if rc != _lib.APH_OK:
+        raise _lib.HydroError(rc, err.value.decode())
+    ops = np.zeros(n[0].value, dtype=AMR_OP_DTYPE)
+    fixes = np.zeros(n[1].value, dtype=FLUX_FIX_DTYPE)
+    refs = np.zeros(n[2].value, dtype=AMR_REF_DTYPE)
+    rc = L.aph_amr_build(*args(ops.ctypes.data_as(vp), fixes.ctypes.data_as(vp),
+                               refs.ctypes.data_as(vp)))
+    if rc != _lib.APH_OK:
+        raise _lib.HydroError(rc, err.value.decode())
+    return ops, fixes, refs
+
+
 class AmrPlan:
-    """The plan on the device (aph_amr_plan_create)."""
+    """The plan on the device (aph_amr_plan_create), its lists built by the library's
+    tree (aph_amr_build)."""
 
     def __init__(self, ctx, dbs, mesh: AmrMesh, bs: BlockSet):
         self.ctx = ctx
-        self.ops, self.fixes, self.refs = amr_plan(mesh, bs)
+        self.ops, self.fixes, self.refs = native_plan(mesh, bs)
         self.h = ctypes.c_void_p()
         vp = ctypes.c_void_p
         ctx.check(_lib.lib().aph_amr_plan_create(
@@ -450,51 +490,86 @@ def _restrict(a, nd):
     return s * (1.0 / len(parts))
 
 
-def _prolong(c, nd):
-    """Limited linear prolongation of a whole periodic level array (nv, Z, Y, X) to the
-    next level: child = c + sum_d (sign_d / 4) minmod(c - c_-d, c_+d - c), axes in order."""
-    def sgn(x):
-        return np.where(x < 0.0, -1.0, 1.0)
-
-    ax = {0: 3, 1: 2, 2: 1}
-    val = c
-    for d in range(nd):
-        val = np.repeat(val, 2, axis=ax[d])
-    for d in range(nd):
-        a = c - np.roll(c, 1, axis=ax[d])
-        b = np.roll(c, -1, axis=ax[d]) - c
-        g = (0.5 * (sgn(a) + sgn(b))) * np.minimum(np.abs(a), np.abs(b))
-        for t in range(nd):
-            g = np.repeat(g, 2, axis=ax[t])
-        shape = [1, 1, 1, 1]
-        shape[ax[d]] = g.shape[ax[d]]
-        sign = np.where(np.arange(g.shape[ax[d]]) % 2 == 0, -1.0, 1.0).reshape(shape)
-        val = val + (sign * 0.25) * g
-    return val
+def _segments(lo, n, nx, nlat):
+    """The node-aligned pieces of the cell range [lo, lo + n) of one axis (periodic):
+    (node, first, end, offset) -- cells [first, end) of node `node`, at `offset` in the
+    range."""
+    out = []
+    c = lo
+    while c < lo + n:
+        node = c // nx
+        end = min((node + 1) * nx, lo + n)
+        out.append((node % nlat, c - node * nx, end - node * nx, c - lo))
+        c = end
+    return out
 
 
-def level_values(mesh: AmrMesh, bs: BlockSet, cons, nlevels=None):
-    """Whole-domain arrays (nv, Z, Y, X) of every level 0 .. nlevels-1: the leaf cells at
-    their level; a refined region the mean of its children (finest levels first); a region
-    under a coarser leaf the prolongation of the level above it (as the ghost exchange
-    fills a block's ghost cells over a coarser neighbour)."""
-    nd, nx, nv = mesh.ndim, mesh.nx, bs.nvar
-    nl = max(mesh.nlevels, nlevels or 0)
-    arr = [np.full((nv,) + tuple(mesh.ncells_axis(L, d) for d in (2, 1, 0)), np.nan)
-           for L in range(nl)]
-    for b in range(bs.nblocks):
-        lv, lc = int(mesh.level[b]), mesh.loc[b]
-        sl = tuple(slice(int(lc[d]) * nx[d], (int(lc[d]) + 1) * nx[d]) for d in (2, 1, 0))
-        arr[lv][(slice(None),) + sl] = bs.interior(cons, b, nv)
-    for L in range(nl - 2, -1, -1):
-        r = _restrict(arr[L + 1], nd)
-        hole = np.isnan(arr[L]) & ~np.isnan(r)
-        arr[L][hole] = r[hole]
-    for L in range(1, nl):
-        hole = np.isnan(arr[L])
-        if hole.any():
-            arr[L][hole] = _prolong(arr[L - 1], nd)[hole]
-    return arr
+class _Sampler:
+    """Values of a mesh's state at any level over any box of cells: a leaf's cells where
+    a leaf of that level holds them, the mean of their children where the node is refined,
+    and the minmod prolongation of the level above where a coarser leaf covers them (the
+    ghost exchange's rule) -- each box built from the leaves around it only."""
+
+    def __init__(self, mesh: AmrMesh, bs: BlockSet, cons):
+        self.m, self.nd, self.nv = mesh, mesh.ndim, bs.nvar
+        self.leaf = {(int(mesh.level[b]), tuple(int(v) for v in mesh.loc[b])):
+                     bs.interior(cons, b, bs.nvar) for b in range(bs.nblocks)}
+
+    def box(self, L, lo, n):
+        """(nv, n_z, n_y, n_x) values of level L over cells lo .. lo + n (x, y, z)."""
+        m, nd, nx = self.m, self.nd, self.m.nx
+        out = np.empty((self.nv, n[2], n[1], n[0]))
+        segs = [_segments(lo[d], n[d], nx[d], m.nlat(L, d)) if d < nd else [(0, 0, 1, 0)]
+                for d in range(3)]
+        for sz in segs[2]:
+            for sy in segs[1]:
+                for sx in segs[0]:
+                    sg = (sx, sy, sz)
+                    node = tuple(q[0] for q in sg)
+                    cnt = [q[2] - q[1] for q in sg]
+                    dst = (slice(None),) + tuple(slice(sg[d][3], sg[d][3] + cnt[d])
+                                                 for d in (2, 1, 0))
+                    v = self.leaf.get((L, node))
+                    if v is not None:
+                        out[dst] = v[(slice(None),) + tuple(slice(sg[d][1], sg[d][2])
+                                                            for d in (2, 1, 0))]
+                        continue
+                    g0 = [node[d] * nx[d] + sg[d][1] if d < nd else 0 for d in range(3)]
+                    if (L, node) in m.refined:
+                        fine = self.box(L + 1, [2 * g0[d] for d in range(3)],
+                                        [2 * cnt[d] if d < nd else 1 for d in range(3)])
+                        out[dst] = _restrict(fine, nd)
+                    else:
+                        out[dst] = self._prolong(L, g0, cnt)
+        return out
+
+    def _prolong(self, L, g0, cnt):
+        """Level-L cells g0 .. g0 + cnt under a coarser leaf: c + sum_d (sign_d / 4)
+        minmod(c - c_-d, c_+d - c) of level L - 1, axes in order."""
+        nd = self.nd
+        if L == 0:
+            raise ValueError("a level-0 region outside every level-0 node")
+        clo = [g0[d] // 2 - 1 if d < nd else 0 for d in range(3)]
+        chi = [(g0[d] + cnt[d] - 1) // 2 + 1 if d < nd else 0 for d in range(3)]
+        C = self.box(L - 1, clo, [chi[d] - clo[d] + 1 for d in range(3)])
+        G = [np.arange(g0[d], g0[d] + cnt[d]) if d < nd else np.zeros(1, int) for d in range(3)]
+        I = [G[d] // 2 - clo[d] for d in range(3)]
+        ix = lambda J: (slice(None), J[2][:, None, None], J[1][None, :, None], J[0][None, None, :])
+        c = C[ix(I)]
+        val = c
+        for d in range(nd):
+            lo = list(I)
+            hi = list(I)
+            lo[d], hi[d] = I[d] - 1, I[d] + 1
+            a = c - C[ix(lo)]
+            b = C[ix(hi)] - c
+            g = (0.5 * (np.where(a < 0.0, -1.0, 1.0) + np.where(b < 0.0, -1.0, 1.0))) * \
+                np.minimum(np.abs(a), np.abs(b))
+            shape = [1, 1, 1, 1]
+            shape[3 - d] = len(G[d])
+            sign = np.where(G[d] % 2 == 0, -1.0, 1.0).reshape(shape)
+            val = val + (sign * 0.25) * g
+        return val
 
 
 def regrid(mesh: AmrMesh, bs: BlockSet, cons, tags):
@@ -507,11 +582,10 @@ def regrid(mesh: AmrMesh, bs: BlockSet, cons, tags):
       groups are taken finest level first, then by position, so a merge on a finer level
       can clear the way for one on a coarser level in the same regrid (a block loses at
       most one level per regrid).
-    Returns (new_mesh, new_bs, new_cons): each new leaf takes its region of the per-level
-    arrays of ``level_values`` -- a kept leaf its data, a merged parent the mean of its
-    children (dz, dy, dx order), a new child the minmod prolongation of its parent's
-    level (the neighbours a cell, a mean of children or a prolongation from the level
-    above).  Ghost cells are left to the next ghost fill.  Totals are conserved to
+    Returns (new_mesh, new_bs, new_cons): a kept leaf keeps its data, a merged parent
+    takes the mean of its children (dz, dy, dx order), a new child the minmod prolongation
+    of its parent's level (the neighbours a cell, a mean of children or a prolongation
+    from the level above, as the ghost exchange fills them).  Ghost cells are left to the next ghost fill.  Totals are conserved to
     round-off and refining then merging returns the data."""
     nd = mesh.ndim
     tags = np.asarray(tags)
@@ -566,10 +640,24 @@ def regrid(mesh: AmrMesh, bs: BlockSet, cons, tags):
                   mesh_id=mesh.mesh_id, max_level=mesh.max_level)
     nbs = new.block_set(nscalars=bs.nscalars)
     nv, nx = bs.nvar, mesh.nx
-    arr = level_values(mesh, bs, cons, nlevels=new.nlevels)
+    old = _Sampler(mesh, bs, cons)
     out = np.zeros(nbs.field_size(nv))
+    groups = {}  # new children of an old leaf: prolongated together
     for b in range(nbs.nblocks):
-        lv, lc = int(new.level[b]), new.loc[b]
-        sl = tuple(slice(int(lc[d]) * nx[d], (int(lc[d]) + 1) * nx[d]) for d in (2, 1, 0))
-        nbs.interior(out, b, nv)[:] = arr[lv][(slice(None),) + sl]
+        lv, lc = int(new.level[b]), tuple(int(v) for v in new.loc[b])
+        dst = nbs.interior(out, b, nv)
+        src = old.leaf.get((lv, lc))
+        if src is not None:
+            dst[:] = src
+        elif lv > 0 and (lv - 1, new.parent(lc)) in old.leaf:
+            groups.setdefault((lv, new.parent(lc)), []).append((b, lc))
+        else:
+            dst[:] = old.box(lv, [lc[d] * nx[d] for d in range(3)], list(nx))
+    for (lv, pc), kids in groups.items():
+        g0 = [2 * pc[d] * nx[d] if d < nd else 0 for d in range(3)]
+        v = old._prolong(lv, g0, [2 * nx[d] if d < nd else 1 for d in range(3)])
+        for b, lc in kids:
+            sl = tuple(slice(lc[d] * nx[d] - g0[d], (lc[d] + 1) * nx[d] - g0[d])
+                       for d in (2, 1, 0))
+            nbs.interior(out, b, nv)[:] = v[(slice(None),) + sl]
     return new, nbs, out

@@ -1,0 +1,298 @@
+// aph_amr_tree.hip -- the multilevel mesh tree on the host: from the leaf list of a block
+// set (each block's level and logical position in a periodic base lattice) to the op lists
+// that aph_amr_plan_create takes (cross-level ghost values and flux fixes).
+//
+// The reference gets these from Parthenon's mesh tree (neighbour finding across levels for
+// the boundary exchange and the flux correction, apophis_driver.cpp:68,74-76,94).  Here the
+// tree is a hash of leaves and of refined nodes (the ancestors of leaves); every ghost cell
+// with no same-level neighbour in its direction becomes one op:
+//   * over a refined node: MEAN of the 2^ndim children of the ghost cell (one level finer);
+//   * over a coarser leaf: PROLONG from the coarse cell holding it, the slope stencil's
+//     neighbours taken at the coarse level (a coarse cell, or the mean of its children);
+// and every coarse face cell on a block face whose neighbour node is refined becomes one
+// flux fix (the mean of the 2^(ndim-1) fine faces across it).  Lists are emitted in block
+// order, ghost cells k, j, i ascending, children (cz, cy, cx) with x fastest, transverse
+// axes ascending with the first slowest -- the order the Python statement of the same
+// builder (apophis_amd/amr.py amr_plan) uses, so both give identical lists.
+// Host code only; no device work.
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <unordered_map>
+#include <unordered_set>
+#include <vector>
+
+#include "../../include/apophis_hydro.h"
+
+namespace {
+
+constexpr int kMaxLevel = 30;
+constexpr int64_t kMaxCoord = (int64_t)1 << 19;  // nodes per axis the key can hold
+
+struct BuildError {
+  char msg[256];
+};
+
+[[noreturn]] void fail(const char *fmt, ...) {
+  BuildError e;
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(e.msg, sizeof(e.msg), fmt, ap);
+  va_end(ap);
+  throw e;
+}
+
+inline uint64_t key(int lv, int64_t x, int64_t y, int64_t z) {
+  return ((uint64_t)lv << 58) | ((uint64_t)x << 38) | ((uint64_t)y << 19) | (uint64_t)z;
+}
+
+struct Tree {
+  int nd = 0, ng = 0, nvar = 0, nb = 0;
+  int nx[3] = {1, 1, 1}, nbase[3] = {1, 1, 1}, E[3] = {1, 1, 1};
+  const aph_block *blk = nullptr;
+  const int32_t *level = nullptr;
+  const int64_t *loc = nullptr;
+  std::unordered_map<uint64_t, int> leaf;
+  std::unordered_set<uint64_t> refined;
+  std::vector<int64_t> base;  // nvar * cell_base per block
+  int64_t ncell = 0;
+  std::vector<aph_amr_ref> refs;
+
+  int64_t nlat(int lv, int d) const { return d < nd ? (int64_t)nbase[d] << lv : 1; }
+  int64_t wrap(int64_t g, int lv, int d) const {
+    if (d >= nd) return 0;
+    const int64_t n = nlat(lv, d) * nx[d];
+    return ((g % n) + n) % n;
+  }
+  bool is_refined(int lv, const int64_t node[3]) const {
+    return refined.count(key(lv, node[0], node[1], node[2])) != 0;
+  }
+  // component-0 offset of block b's cell at ghost-inclusive local index l
+  int64_t cell(int b, const int64_t l[3]) const {
+    return base[b] + (l[2] * E[1] + l[1]) * E[0] + l[0];
+  }
+  // the leaf of level lv holding global cell G (wrapped): its block and local index
+  bool interior(int lv, const int64_t G[3], int &b, int64_t l[3]) const {
+    int64_t W[3], node[3];
+    for (int d = 0; d < 3; ++d) {
+      W[d] = wrap(G[d], lv, d);
+      node[d] = d < nd ? W[d] / nx[d] : 0;
+    }
+    auto it = leaf.find(key(lv, node[0], node[1], node[2]));
+    if (it == leaf.end()) return false;
+    b = it->second;
+    for (int d = 0; d < 3; ++d) l[d] = W[d] - node[d] * nx[d] + (d < nd ? ng : 0);
+    return true;
+  }
+  aph_amr_val add(const std::vector<int64_t> &offs) {
+    aph_amr_val v;
+    v.first = (int32_t)refs.size();
+    v.count = (int32_t)offs.size();
+    for (int64_t o : offs) refs.push_back(aph_amr_ref{o, ncell});
+    if (refs.size() > (size_t)INT32_MAX) fail("too many cell references");
+    return v;
+  }
+  // level-lv value of global cell I: the leaf cell, or the mean of its children
+  aph_amr_val value(int lv, const int64_t I[3]) {
+    int b;
+    int64_t l[3];
+    if (interior(lv, I, b, l)) return add({cell(b, l)});
+    int64_t W[3];
+    for (int d = 0; d < 3; ++d) W[d] = wrap(I[d], lv, d);
+    std::vector<int64_t> kids;
+    for (int cz = 0; cz < (nd == 3 ? 2 : 1); ++cz)
+      for (int cy = 0; cy < 2; ++cy)
+        for (int cx = 0; cx < 2; ++cx) {
+          const int64_t F[3] = {2 * W[0] + cx, 2 * W[1] + cy, nd == 3 ? 2 * W[2] + cz : 0};
+          if (!interior(lv + 1, F, b, l))
+            fail("cell (%lld, %lld, %lld) of level %d is neither a leaf cell nor the parent "
+                 "of leaf cells (2:1 balance)", (long long)W[0], (long long)W[1],
+                 (long long)W[2], lv);
+          kids.push_back(cell(b, l));
+        }
+    return add(kids);
+  }
+};
+
+void setup(Tree &t, const aph_block *blocks, int32_t nblocks, int32_t ndim, int32_t nvar,
+           const int32_t nbase[3], const int32_t *level, const int64_t *loc) {
+  if (!blocks || nblocks < 1 || ndim < 2 || ndim > 3 || nvar < 1 || !nbase || !level || !loc)
+    fail("bad arguments");
+  t.nd = ndim;
+  t.nvar = nvar;
+  t.nb = nblocks;
+  t.blk = blocks;
+  t.level = level;
+  t.loc = loc;
+  t.ng = blocks[0].ng;
+  for (int d = 0; d < 3; ++d) {
+    t.nx[d] = d < ndim ? blocks[0].nx[d] : 1;
+    t.nbase[d] = d < ndim ? nbase[d] : 1;
+    if (t.nbase[d] < 1) fail("nbase[%d] = %d", d, nbase[d]);
+    t.E[d] = t.nx[d] + (d < ndim ? 2 * t.ng : 0);
+  }
+  t.ncell = (int64_t)t.E[0] * t.E[1] * t.E[2];
+  for (int b = 0; b < nblocks; ++b) {
+    const aph_block &k = blocks[b];
+    if (k.ng != t.ng) fail("block %d: ng %d differs from block 0", b, k.ng);
+    for (int d = 0; d < ndim; ++d)
+      if (k.nx[d] != t.nx[d] || k.nx[d] % 2 || k.nx[d] < 2 * k.ng)
+        fail("block %d: nx must be even, at least 2*ng and the same for every block", b);
+    if (level[b] < 0 || level[b] > kMaxLevel) fail("block %d: level %d", b, level[b]);
+    int64_t c[3];
+    for (int d = 0; d < 3; ++d) {
+      c[d] = d < ndim ? loc[3 * b + d] : 0;
+      if (c[d] < 0 || c[d] >= t.nlat(level[b], d) || c[d] >= kMaxCoord)
+        fail("block %d: logical position outside level %d", b, level[b]);
+    }
+    if (!t.leaf.emplace(key(level[b], c[0], c[1], c[2]), b).second)
+      fail("block %d: two blocks at the same node", b);
+    t.base.push_back((int64_t)nvar * k.cell_base);
+  }
+  // refined nodes: the ancestors of the leaves
+  for (int b = 0; b < nblocks; ++b) {
+    int64_t c[3] = {t.nd > 0 ? loc[3 * b] : 0, t.nd > 1 ? loc[3 * b + 1] : 0,
+                    t.nd > 2 ? loc[3 * b + 2] : 0};
+    for (int lv = level[b] - 1; lv >= 0; --lv) {
+      for (int d = 0; d < t.nd; ++d) c[d] >>= 1;
+      const uint64_t k = key(lv, c[0], c[1], c[2]);
+      if (t.leaf.count(k)) fail("block %d lies inside block %d", b, t.leaf[k]);
+      if (!t.refined.insert(k).second) break;
+    }
+  }
+}
+
+void build(Tree &t, std::vector<aph_amr_op> &ops, std::vector<aph_flux_fix> &fixes) {
+  const int nd = t.nd, g = t.ng;
+  for (int b = 0; b < t.nb; ++b) {
+    const int lv = t.level[b];
+    const int64_t lc[3] = {t.loc[3 * b], nd > 1 ? t.loc[3 * b + 1] : 0,
+                           nd > 2 ? t.loc[3 * b + 2] : 0};
+    const int32_t *nbr = t.blk[b].nbr;
+    for (int k = 0; k < t.E[2]; ++k)
+      for (int j = 0; j < t.E[1]; ++j)
+        for (int i = 0; i < t.E[0]; ++i) {
+          const int64_t l[3] = {i, j, k};
+          int o[3] = {0, 0, 0};
+          for (int d = 0; d < nd; ++d) o[d] = l[d] < g ? -1 : (l[d] >= g + t.nx[d] ? 1 : 0);
+          if (!o[0] && !o[1] && !o[2]) continue;
+          if (nbr[(o[2] + 1) * 9 + (o[1] + 1) * 3 + (o[0] + 1)] >= 0) continue;
+          int64_t Gw[3], node[3];
+          for (int d = 0; d < 3; ++d) {
+            Gw[d] = d < nd ? t.wrap(lc[d] * t.nx[d] + l[d] - g, lv, d) : 0;
+            node[d] = d < nd ? Gw[d] / t.nx[d] : 0;
+          }
+          aph_amr_op op;
+          std::memset(&op, 0, sizeof(op));
+          op.dst = t.cell(b, l);
+          op.dst_stride = t.ncell;
+          if (t.is_refined(lv, node)) {  // over a finer region: restriction
+            op.kind = APH_AMR_MEAN;
+            op.val[0] = t.value(lv, Gw);
+          } else {  // over a coarser leaf: prolongation
+            const int64_t I[3] = {Gw[0] / 2, nd > 1 ? Gw[1] / 2 : 0, nd > 2 ? Gw[2] / 2 : 0};
+            int cb;
+            int64_t cl[3];
+            if (lv == 0 || !t.interior(lv - 1, I, cb, cl))
+              fail("block %d: ghost cell (%d, %d, %d) is over neither a finer region nor a "
+                   "coarser leaf (2:1 balance)", b, i, j, k);
+            op.kind = APH_AMR_PROLONG;
+            op.val[0] = t.value(lv - 1, I);
+            for (int d = 0; d < nd; ++d) {
+              op.sign[d] = Gw[d] % 2 == 0 ? -1 : 1;
+              int64_t lo[3] = {I[0], I[1], I[2]}, hi[3] = {I[0], I[1], I[2]};
+              lo[d] -= 1;
+              hi[d] += 1;
+              op.val[1 + 2 * d] = t.value(lv - 1, lo);
+              op.val[2 + 2 * d] = t.value(lv - 1, hi);
+            }
+          }
+          ops.push_back(op);
+        }
+  }
+  for (int b = 0; b < t.nb; ++b) {
+    const int lv = t.level[b];
+    const int64_t lc[3] = {t.loc[3 * b], nd > 1 ? t.loc[3 * b + 1] : 0,
+                           nd > 2 ? t.loc[3 * b + 2] : 0};
+    for (int d = 0; d < nd; ++d)
+      for (int side = 0; side < 2; ++side) {
+        if (t.blk[b].bc[2 * d + side] != APH_BC_AMR) continue;
+        int64_t across[3] = {lc[0], lc[1], lc[2]};
+        across[d] = ((across[d] + (side ? 1 : -1)) % t.nlat(lv, d) + t.nlat(lv, d)) %
+                    t.nlat(lv, d);
+        if (!t.is_refined(lv, across)) continue;  // the other side is coarser
+        int tr[2], ntr = 0;
+        for (int q = 0; q < nd; ++q)
+          if (q != d) tr[ntr++] = q;
+        const int64_t Fd = 2 * (lc[d] * t.nx[d] + (side ? t.nx[d] : 0)) - (side ? 0 : 1);
+        const int n0 = t.nx[tr[0]], n1 = ntr > 1 ? t.nx[tr[1]] : 1;
+        for (int c0 = 0; c0 < n0; ++c0)
+          for (int c1 = 0; c1 < n1; ++c1) {
+            const int tc[2] = {c0, c1};
+            int64_t l[3] = {0, 0, 0};
+            l[d] = g + (side ? t.nx[d] : 0);
+            for (int q = 0; q < ntr; ++q) l[tr[q]] = g + tc[q];
+            std::vector<int64_t> cells;
+            for (int k0 = 0; k0 < 2; ++k0)
+              for (int k1 = 0; k1 < (ntr > 1 ? 2 : 1); ++k1) {
+                const int kid[2] = {k0, k1};
+                int64_t F[3] = {0, 0, 0};
+                F[d] = Fd;
+                for (int q = 0; q < ntr; ++q)
+                  F[tr[q]] = 2 * (lc[tr[q]] * t.nx[tr[q]] + tc[q]) + kid[q];
+                int fb;
+                int64_t fl[3];
+                if (!t.interior(lv + 1, F, fb, fl))
+                  fail("block %d face %d: fine face outside the finer level", b, 2 * d + side);
+                if (!side) fl[d] += 1;  // upper face of the last fine cell below
+                cells.push_back(t.cell(fb, fl));
+              }
+            aph_flux_fix fx;
+            std::memset(&fx, 0, sizeof(fx));
+            fx.dst = t.cell(b, l);
+            fx.dst_stride = t.ncell;
+            fx.dir = d;
+            fx.val = t.add(cells);
+            fixes.push_back(fx);
+          }
+      }
+  }
+}
+
+}  // namespace
+
+extern "C" int aph_amr_build(const aph_block *blocks, int32_t nblocks, int32_t ndim,
+                             int32_t nvar, const int32_t nbase[3], const int32_t *level,
+                             const int64_t *loc, aph_amr_op *ops, int64_t *nops,
+                             aph_flux_fix *fixes, int64_t *nfixes, aph_amr_ref *refs,
+                             int64_t *nrefs, char *err, size_t err_len) {
+  auto report = [&](int code, const char *msg) {
+    if (err && err_len) snprintf(err, err_len, "aph_amr_build: %s", msg);
+    return code;
+  };
+  if (!nops || !nfixes || !nrefs) return report(APH_E_INVALID, "null length");
+  try {
+    Tree t;
+    setup(t, blocks, nblocks, ndim, nvar, nbase, level, loc);
+    std::vector<aph_amr_op> o;
+    std::vector<aph_flux_fix> f;
+    build(t, o, f);
+    const int64_t cap[3] = {*nops, *nfixes, *nrefs};
+    *nops = (int64_t)o.size();
+    *nfixes = (int64_t)f.size();
+    *nrefs = (int64_t)t.refs.size();
+    if ((ops && cap[0] < *nops) || (fixes && cap[1] < *nfixes) || (refs && cap[2] < *nrefs))
+      return report(APH_E_INVALID, "output arrays too short");
+    if (ops && !o.empty()) std::memcpy(ops, o.data(), sizeof(aph_amr_op) * o.size());
+    if (fixes && !f.empty()) std::memcpy(fixes, f.data(), sizeof(aph_flux_fix) * f.size());
+    if (refs && !t.refs.empty())
+      std::memcpy(refs, t.refs.data(), sizeof(aph_amr_ref) * t.refs.size());
+  } catch (const BuildError &e) {
+    return report(APH_E_INVALID, e.msg);
+  } catch (const std::bad_alloc &) {
+    return report(APH_E_NOMEM, "host allocation failed");
+  }
+  if (err && err_len) err[0] = 0;
+  return APH_OK;
+}

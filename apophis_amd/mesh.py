@@ -130,6 +130,7 @@ class BlockSet:
         self.blocks = np.concatenate(parts) if parts else np.zeros(0, dtype=BLOCK_DTYPE)
         self.nblocks = len(self.blocks)
         ext = self.extents()
+        self._ext = ext  # fixed with ncell and cell_base below
         nc = ext.prod(axis=1)
         self.ncell = nc
         cb = np.zeros(self.nblocks, dtype=np.int64)
@@ -155,7 +156,7 @@ class BlockSet:
 
     def block_view(self, field, b, nv):
         """View of block b of a flat field with nv components as (nv, Nz, Ny, Nx)."""
-        Nx, Ny, Nz = (int(v) for v in self.extents()[b])
+        Nx, Ny, Nz = (int(v) for v in self._ext[b])
         base = nv * int(self.blocks["cell_base"][b])
         n = int(self.ncell[b])
         return field[base: base + nv * n].reshape(nv, Nz, Ny, Nx)

@@ -234,6 +234,21 @@ int aph_amr_plan_create(aph_context *ctx, const aph_blockset *bs, const aph_amr_
                         int64_t nops, const aph_flux_fix *fixes, int64_t nfixes,
                         const aph_amr_ref *refs, int64_t nrefs, aph_amr_plan **out);
 int aph_amr_plan_destroy(aph_context *ctx, aph_amr_plan *plan);
+/* The op lists of a periodic multilevel mesh from its leaves (the Parthenon mesh tree's
+   neighbour finding across levels; host only, no context or device needed).  blocks: the
+   block set's table (every block the same nx, even, and ng; nbr / bc as described above);
+   level[b] and loc[3*b .. 3*b+2]: each block's level and logical position at its level in
+   a base lattice of nbase[0..ndim-1] blocks; the tree must be 2:1 balanced.  Ops: one per
+   ghost cell with no same-level neighbour in its direction (MEAN over a refined node,
+   PROLONG from a coarser leaf); fixes: one per coarse face cell on a face to a refined
+   node; in block order, ghost cells k, j, i ascending, children (cz, cy, cx) x fastest.
+   On input *nops, *nfixes, *nrefs are the capacities of the arrays (an array may be NULL:
+   then only its length is returned); on output the lengths.  On failure the message goes
+   to err (err_len bytes, may be NULL). */
+int aph_amr_build(const aph_block *blocks, int32_t nblocks, int32_t ndim, int32_t nvar,
+                  const int32_t nbase[3], const int32_t *level, const int64_t *loc,
+                  aph_amr_op *ops, int64_t *nops, aph_flux_fix *fixes, int64_t *nfixes,
+                  aph_amr_ref *refs, int64_t *nrefs, char *err, size_t err_len);
 /* Cross-level ghost values of cons; run after aph_fill_ghosts.  Reads interior cells
    only, writes ghost cells only. */
 int aph_amr_fill(aph_context *ctx, const aph_amr_plan *plan, double *cons);

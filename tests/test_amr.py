@@ -303,3 +303,48 @@ def test_unbalanced_or_orphan_trees_are_refused():
         amr.AmrMesh(**base, refined=[(1, (2, 2))])
     with pytest.raises(ValueError, match="max_level"):
         amr.AmrMesh(**base, refined=[(0, (1, 1))], max_level=0)
+
+
+def _deck_mesh():
+    from apophis_amd import adaptive, hydro
+    from tests.test_adaptive import oracle_tagger
+    pkg = hydro.initialize_hydro(problems.blast_deck_pin())
+    m, bs, _ = adaptive.initial_mesh(problems.blast_deck_base_mesh(), problems.blast_pgen(),
+                                     oracle_tagger(pkg))
+    return m, bs
+
+
+@pytest.mark.parametrize("name", list(MESHES) + ["blast_deck"])
+def test_native_tree_builds_the_same_lists(name):
+    """aph_amr_build (the library's tree, host only) gives the op lists of the Python
+    statement, byte for byte."""
+    if name == "blast_deck":
+        m, bs = _deck_mesh()
+    else:
+        m = _mesh(name)
+        bs = m.block_set()
+    want = amr.amr_plan(m, bs)
+    got = amr.native_plan(m, bs)
+    for w, g in zip(want, got):
+        assert w.dtype == g.dtype and len(w) == len(g)
+        assert w.tobytes() == g.tobytes()
+
+
+def test_native_tree_refuses_bad_trees():
+    from apophis_amd import _lib
+    m = _mesh("2d_3lev")
+    bs = m.block_set()
+    # a level-2 block moved next to level-0 leaves: not balanced
+    bad = amr.AmrMesh.__new__(amr.AmrMesh)
+    bad.__dict__.update(m.__dict__)
+    bad.loc = m.loc.copy()
+    bad.level = m.level.copy()
+    b2 = int(np.flatnonzero(m.level == 2)[0])
+    b0 = int(np.flatnonzero(m.level == 0)[0])
+    bad.level[b0], bad.loc[b0] = 2, (0, 0, 0)  # inside a level-0 leaf's region
+    with pytest.raises(_lib.HydroError, match="aph_amr_build"):
+        amr.native_plan(bad, bs)
+    bad.level, bad.loc = m.level.copy(), m.loc.copy()
+    bad.loc[b2] = bad.loc[b2 + 1]  # two blocks at one node
+    with pytest.raises(_lib.HydroError, match="same node"):
+        amr.native_plan(bad, bs)

@@ -287,6 +287,46 @@ def two_level_leg(ctx, hydro, problems, torch, nsteps=20):
             "plain_zone_cycles_per_s": vp, "fused_over_plain": vf / vp}
 
 
+def blast_deck_leg(ctx, hydro, problems, torch, ncycles=300):
+    """The reference's adaptive deck (inputs/blast.in: 128^2 of 16^2 blocks, three levels
+    on the pressure-gradient criterion, PLM + HLLC, vl2, cfl 0.4, derefine_count 10):
+    the adaptive start, then `ncycles` cycles of step + device tags + regrid in each form.
+    Reports zone-cycles/s of the whole loop (host regrids included) and of the steps alone
+    on the final mesh, the regrids and the host seconds they took."""
+    from apophis_amd import adaptive
+    pkg = hydro.initialize_hydro(problems.blast_deck_pin())
+    t0 = time.perf_counter()
+    m, bs, cons = adaptive.initial_mesh(
+        problems.blast_deck_base_mesh(), problems.blast_pgen(),
+        lambda mm, bb, cc: adaptive.device_tags(ctx, pkg, mm, bb, cc))
+    out = {"start_s": round(time.perf_counter() - t0, 3), "start_blocks": bs.nblocks,
+           "start_levels": [int(v) for v in np.bincount(m.level)], "cycles": ncycles}
+    for form in ("fused", "plain"):
+        drv = adaptive.AdaptiveDriver(ctx, pkg, m, bs, cons, form=form)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        drv.evolve(ncycles=ncycles)
+        w = time.perf_counter() - t0
+        st = drv.stats
+        r = {"zone_cycles_per_s": st["zone_cycles"] / w, "wall_s": round(w, 3),
+             "regrids": st["regrids"], "regrid_host_s": round(st["regrid_s"], 3),
+             "sim_time": drv.time, "end_blocks": drv.bs.nblocks,
+             "end_levels": [int(v) for v in np.bincount(drv.mesh.level)]}
+        if form == "fused":  # the steps alone, on the final mesh
+            fd = drv.fd
+            fd.set_dt(drv.dt)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            for _ in range(50):
+                fd.step(fixed_dt=True)
+            torch.cuda.synchronize()
+            r["steps_only_zone_cycles_per_s"] = drv.bs.interior_cells() * 50 / (
+                time.perf_counter() - t1)
+        out[form] = r
+        del drv
+    return out
+
+
 def profile_run(args, ctx, hydro, problems, torch):
     """Eager launches for rocprofv3: 3 RK2 steps of the chosen form on the headline
     workload (or the uneven block set), then a 1 GiB device-to-device copy repeated 3
@@ -483,6 +523,8 @@ def main():
         print(f"# two-level: {extras['two_level_blast2d']}", file=sys.stderr, flush=True)
         extras["kh_comp_ONe_lset"] = composition_leg(ctx, hydro, problems, torch)
         print(f"# composition: {extras['kh_comp_ONe_lset']}", file=sys.stderr, flush=True)
+        extras["blast_deck_adaptive"] = blast_deck_leg(ctx, hydro, problems, torch)
+        print(f"# blast deck: {extras['blast_deck_adaptive']}", file=sys.stderr, flush=True)
     if not args.no_adjoint:
         extras["adjoint_kh"] = adjoint_leg(ctx, hydro, problems, torch)
         print(f"# adjoint: {extras['adjoint_kh']}", file=sys.stderr, flush=True)

@@ -1,5 +1,6 @@
-"""The adaptive loop on the GPU (apophis_amd/adaptive.py) on a reduced blast deck
-(inputs/blast.in at 64^2 with 8^2 blocks: three levels, PLM + HLLC, vl2): the adaptive
+"""The adaptive loop on the GPU (apophis_amd/adaptive.py) on reduced blast decks
+(inputs/blast.in at 64^2 with 8^2 blocks, and a 3D variant at 16^3 with 4^3 blocks: three
+levels, PLM + HLLC, vl2): the adaptive
 start with the device criterion builds the same mesh and state as with the oracle's
 criterion; the plain driver's cycles (step, device tags, derefinement delay, regrid, new
 plans, dt policy) follow the oracle's cycles bitwise through refinements and merges; the
@@ -14,24 +15,26 @@ from tests.test_adaptive import oracle_tagger
 
 pytestmark = pytest.mark.gpu
 
-DECK = dict(nx=(64, 64), block_nx=(8, 8))
+DECKS = {"2d": dict(nx=(64, 64), block_nx=(8, 8)),
+         "3d": dict(nx=(16, 16, 16), block_nx=(4, 4, 4), xmin=(-0.5,) * 3, xmax=(0.5,) * 3,
+                    radius_outer=0.15)}
 NCYCLES = 8
 DEREF = 2
 
 
-def _start(tagger):
-    return adaptive.initial_mesh(problems.blast_deck_base_mesh(DECK),
-                                 problems.blast_pgen(DECK), tagger)
+def _start(deck, tagger):
+    return adaptive.initial_mesh(problems.blast_deck_base_mesh(deck),
+                                 problems.blast_pgen(deck), tagger)
 
 
-def _pkg():
-    return hydro.initialize_hydro(problems.blast_deck_pin(DECK))
+def _pkg(deck):
+    return hydro.initialize_hydro(problems.blast_deck_pin(deck))
 
 
-def _oracle_cycles(pkg, m, bs, cons):
+def _oracle_cycles(deck, pkg, m, bs, cons):
     """The cycle of the adaptive driver written out with the oracle's step, criterion
     and ghost fill, and the host regrid."""
-    k = dict(problems.BLAST_DECK, **DECK)
+    k = dict(problems.BLAST_DECK, **deck)
     mk = lambda bs_, c_: problems.Problem("deck", bs_, c_, pkg.param("eos"), k["recon"],
                                           k["riemann"], k["cfl"], integrator="vl2")
     st = ao.AmrState(mk(bs, cons), m)
@@ -55,20 +58,23 @@ def _oracle_cycles(pkg, m, bs, cons):
     return out
 
 
-def test_adaptive_start_matches_oracle_criterion(gpu_ctx):
-    print("[case] adaptive start", flush=True)
-    pkg = _pkg()
-    m, bs, cons = _start(lambda mm, bb, cc: adaptive.device_tags(gpu_ctx, pkg, mm, bb, cc))
-    rm, rbs, rc = _start(oracle_tagger(pkg))
+@pytest.mark.parametrize("deck", list(DECKS))
+def test_adaptive_start_matches_oracle_criterion(gpu_ctx, deck):
+    print(f"[case] adaptive start {deck}", flush=True)
+    d = DECKS[deck]
+    pkg = _pkg(d)
+    m, bs, cons = _start(d, lambda mm, bb, cc: adaptive.device_tags(gpu_ctx, pkg, mm, bb, cc))
+    rm, rbs, rc = _start(d, oracle_tagger(pkg))
     assert m.refined == rm.refined and m.nlevels == 3
     assert np.array_equal(cons, rc)
 
 
-@pytest.fixture(scope="module")
-def reference_run():
-    pkg = _pkg()
-    m, bs, cons = _start(oracle_tagger(pkg))
-    return pkg, m, bs, cons, _oracle_cycles(pkg, m, bs, cons)
+@pytest.fixture(scope="module", params=list(DECKS))
+def reference_run(request):
+    d = DECKS[request.param]
+    pkg = _pkg(d)
+    m, bs, cons = _start(d, oracle_tagger(pkg))
+    return pkg, m, bs, cons, _oracle_cycles(d, pkg, m, bs, cons)
 
 
 def _run(gpu_ctx, form, reference_run):

@@ -286,7 +286,7 @@ __global__ void __launch_bounds__(64) k_fused(FusedArgs a) {
   }
   if (a.last && a.dt_min) {
     for (int o = 32; o > 0; o >>= 1) dtmin = fmin(dtmin, __shfl_xor(dtmin, o));
-    if (threadIdx.x == 0) atomic_min_pos(a.dt_min, a.cfl * dtmin);
+    if (threadIdx.x == 0) fold_min_pos(a.dt_min, a.cfl * dtmin);
   }
 }
 
@@ -314,8 +314,7 @@ __global__ void __launch_bounds__(256)
     cons_to_prim_pt<NV>(uu, eos, w, gm);
     m = fmin(m, dt_point(w, uu[4], eos, bk.dx, NDIM));
   }
-  for (int o = 32; o > 0; o >>= 1) m = fmin(m, __shfl_xor(m, o));
-  if ((threadIdx.x & 63) == 0) atomic_min_pos(out, cfl * m);
+  workgroup_min_pos<256>(out, cfl * m);
 }
 
 }  // namespace

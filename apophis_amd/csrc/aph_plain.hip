@@ -264,8 +264,7 @@ __global__ void __launch_bounds__(kThreads)
     double ww[5] = {w[c], w[nc + c], w[2 * nc + c], w[3 * nc + c], w[4 * nc + c]};
     m = fmin(m, dt_point(ww, u[4 * nc + c], eos, bk.dx, ndim));
   }
-  for (int o = 32; o > 0; o >>= 1) m = fmin(m, __shfl_xor(m, o));
-  if ((threadIdx.x & 63) == 0) atomic_min_pos(out, cfl * m);
+  workgroup_min_pos<kThreads>(out, cfl * m);
 }
 
 template <int NV>

@@ -348,6 +348,27 @@ def profile_run(args, ctx, hydro, problems, torch):
         torch.cuda.synchronize()
         print(f"profile-run {args.profile_run}: done", flush=True)
         return
+    if args.profile_run == "amr":
+        # multilevel fused form: 3 RK2 steps of the two-level blast (3072 blocks of 32^2),
+        # then 3 of the blast deck's three-level start mesh (148 blocks of 16^2, vl2)
+        from apophis_amd import adaptive
+        p, m = problems.blast2d_two_level()
+        fd = hydro.FusedDriver(ctx, p.bs, p.cons, p.recon, p.riemann, p.eos, p.cfl, amr_mesh=m)
+        fd.initial_dt()
+        for _ in range(3):
+            fd.step()
+        torch.cuda.synchronize()
+        del fd
+        pkg = hydro.initialize_hydro(problems.blast_deck_pin())
+        m, bs, cons = adaptive.initial_mesh(
+            problems.blast_deck_base_mesh(), problems.blast_pgen(),
+            lambda mm, bb, cc: adaptive.device_tags(ctx, pkg, mm, bb, cc))
+        drv = adaptive.AdaptiveDriver(ctx, pkg, m, bs, cons, form="fused")
+        for _ in range(3):
+            drv.fd.step()
+        torch.cuda.synchronize()
+        print(f"profile-run {args.profile_run}: done", flush=True)
+        return
     if args.profile_run == "uneven":
         p = problems.uneven_blocks(nblocks=args.uneven_blocks)
     else:
@@ -387,13 +408,14 @@ def main():
     ap.add_argument("--uneven-blocks", type=int, default=4096)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the KH adjoint leg")
     ap.add_argument("--no-rt", action="store_true", help="skip the batched RT adjoint leg")
-    ap.add_argument("--profile-run", choices=["fused", "plain", "uneven", "adjoint", "rt_adjoint"],
+    ap.add_argument("--profile-run", choices=["fused", "plain", "uneven", "adjoint", "rt_adjoint",
+                                              "amr"],
                     default=None,
                     help="short eager run for rocprofv3 (no timing, no JSON)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(multi-GPU sharding is design-only, see DESIGN.md)", file=sys.stderr)
+              "(multi-GPU sharding is a design, not built)", file=sys.stderr)
         sys.exit(2)
 
     import torch

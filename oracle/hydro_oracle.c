@@ -7,9 +7,9 @@
  * Every expression keeps the reference's operand order so that, with contraction
  * off, results are bitwise reproducible against the GPU path.
  *
- * What pins each part is listed in tests/test_oracle_*.py and DESIGN.md; functions
- * restated from upstream Parthenon / singularity-eos (not vendored in the reference)
- * are marked "upstream".
+ * What pins each part is listed in tests/test_oracle_*.py; functions restated from
+ * upstream Parthenon / singularity-eos (not vendored in the reference) are marked
+ * "upstream".
  */
 #include "hydro_oracle.h"
 

@@ -136,11 +136,13 @@ class AdaptiveDriver:
         tags, _ = hydro.refinement_tag(self.md, self.pkg)  # prim is current after a step
         return tags.cpu().numpy()
 
-    def cycle(self):
-        """One step at the current dt, then the refinement check; returns the dt used."""
+    def cycle(self, tlim=None):
+        """One step at the current dt (clipped to end at tlim, default the driver's), then
+        the refinement check; returns the dt used."""
         dt = self.dt
-        if self.tlim - self.time < dt:
-            dt = self.tlim - self.time
+        end = self.tlim if tlim is None else float(tlim)
+        if end - self.time < dt:
+            dt = end - self.time
         if self.form == "fused":
             self.fd.set_dt(dt)
             self.fd.step(fixed_dt=True)
@@ -179,7 +181,7 @@ class AdaptiveDriver:
         tl = self.tlim if tlim is None else float(tlim)
         n = 0
         while (ncycles is None or n < ncycles) and self.time < tl * (1 - 1e-14):
-            self.cycle()
+            self.cycle(tl)
             n += 1
         torch.cuda.synchronize()
         return n

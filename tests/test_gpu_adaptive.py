@@ -112,3 +112,22 @@ def test_fused_adaptive_cycles_match_plain(gpu_ctx, reference_run):
         assert dt == rdt, c
         mask = np.tile(bs.interior_mask(), bs.nvar)
         assert np.array_equal(u[mask], ru[mask]), c
+
+
+def test_evolve_ends_at_tlim(gpu_ctx):
+    """evolve(tlim=...) clips the last step so the run ends at tlim, in both forms, and
+    the two forms take the same steps."""
+    print("[case] adaptive evolve to tlim", flush=True)
+    d = DECKS["2d"]
+    pkg = _pkg(d)
+    m, bs, cons = _start(d, lambda mm, bb, cc: adaptive.device_tags(gpu_ctx, pkg, mm, bb, cc))
+    ends = []
+    for form in ("plain", "fused"):
+        drv = adaptive.AdaptiveDriver(gpu_ctx, pkg, m, bs, cons, form=form)
+        tl = 3.7 * drv.dt
+        n = drv.evolve(tlim=tl)
+        assert n >= 2 and drv.time == pytest.approx(tl, rel=1e-15, abs=0)
+        ends.append((set(drv.mesh.refined), drv.bs, drv.state().cpu().numpy()))
+    (r0, b0, u0), (r1, b1, u1) = ends
+    mask = np.tile(b0.interior_mask(), b0.nvar)
+    assert r0 == r1 and np.array_equal(u0[mask], u1[mask])

@@ -33,6 +33,15 @@ namespace {
 
 __global__ void k_fill(double *p, double v) { *p = v; }
 
+// launch epoch of a multilevel stage (the interface counts are compared against it)
+__global__ void k_bump(unsigned *epoch) { *epoch = *epoch + 1u; }
+
+// A multilevel stage whose cross-level wait ran out (its fine tiles were not all
+// resident): the result is wrong, so make it visibly so.
+__global__ void k_amr_check(const int *err, double *u_out) {
+  if (*err) *u_out = __builtin_nan("");
+}
+
 // dt policy of the upstream EvolutionDriver: time += dt; dt <- min(2 dt, estimate),
 // clipped so that the step does not pass tlim.
 __global__ void k_dt_finalize(double *dt, double *time, const double *est, double tlim,
@@ -78,14 +87,22 @@ struct aph_plan {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   aph::AdjScratch adj;
-  // multilevel mesh (aph_plan_set_amr): one phase of items per level, the finest first,
-  // so the coarse side of every level jump finds the fine fluxes; phase k is items
-  // [phase_start[k], phase_start[k + 1])
+  // multilevel mesh (aph_plan_set_amr): one launch per stage whose items run the finest
+  // level first; a coarse tile waits for the count of fine tiles that wrote its interface
+  // region (per block face: region index; per region: fine tiles per launch from the item
+  // list, a running count, and the launch epoch)
   const aph_amr_plan *amr = nullptr;
   aph_amr_face *d_amr_face = nullptr;
   double *d_amr_buf = nullptr;
   std::vector<int32_t> level;
-  std::vector<int> phase_start;
+  std::vector<aph_amr_face> faces_h;
+  std::vector<int32_t> face_region;
+  int nregions = 0;
+  int *d_region = nullptr;
+  unsigned long long *d_count = nullptr;
+  unsigned *d_expect = nullptr;
+  unsigned *d_epoch = nullptr;
+  int *d_err = nullptr;
   bool amr_faces = false;  // the block set has faces to another level: needs set_amr
 };
 
@@ -212,7 +229,6 @@ int build_items(aph_context *ctx, aph_plan *p) {
   // one phase without a multilevel mesh; with one, a phase per level, the finest first
   const bool multi = !p->level.empty();
   const int top = multi ? *std::max_element(p->level.begin(), p->level.end()) : 0;
-  p->phase_start.assign(1, 0);
   for (int phase = 0; phase <= top; ++phase) {
     std::vector<std::vector<int4>> lists(kXcd);
     std::vector<int64_t> load(kXcd, 0);
@@ -241,7 +257,27 @@ int build_items(aph_context *ctx, aph_plan *p) {
     items.resize(at + maxlen * kXcd, make_int4(-1, 0, 0, 0));
     for (int x = 0; x < kXcd; ++x)
       for (size_t r = 0; r < lists[x].size(); ++r) items[at + r * kXcd + x] = lists[x][r];
-    p->phase_start.push_back((int)items.size());
+  }
+  if (p->amr) {
+    // fine tiles per interface region and launch: tiles on a face to a coarser leaf
+    std::vector<unsigned> expect(std::max(p->nregions, 1), 0u);
+    for (const int4 &it : items) {
+      if (it.x < 0) continue;
+      const int b = it.x & 0xFFFFFF;
+      const aph_tile &t = tl[it.x >> 24];
+      const int tt[3] = {t.tx, t.ty, t.tz}, t0[3] = {it.y, it.z, it.w};
+      for (int d = 0; d < ndim; ++d)
+        for (int side = 0; side < 2; ++side) {
+          const int fi = b * 6 + 2 * d + side;
+          const bool on = side ? (t0[d] + tt[d] == bs->h[b].nx[d]) : (t0[d] == 0);
+          if (on && p->faces_h[fi].role == 1) ++expect[p->face_region[fi]];
+        }
+    }
+    APH_HIP_CHECK(ctx, hipMemcpy(p->d_expect, expect.data(), sizeof(unsigned) * expect.size(),
+                                 hipMemcpyHostToDevice));
+    APH_HIP_CHECK(ctx, hipMemset(p->d_count, 0, sizeof(unsigned long long) * expect.size()));
+    APH_HIP_CHECK(ctx, hipMemset(p->d_epoch, 0, sizeof(unsigned)));
+    APH_HIP_CHECK(ctx, hipMemset(p->d_err, 0, sizeof(int)));
   }
   if (p->d_items) (void)hipFree(p->d_items);
   p->d_items = nullptr;
@@ -309,6 +345,11 @@ FusedArgs make_args(aph_plan *p, int stage, int nstages, double gam0, double gam
   a.last = stage == nstages;
   a.amr_face = p->d_amr_face;
   a.amr_buf = p->d_amr_buf;
+  a.amr_region = p->d_region;
+  a.amr_count = p->d_count;
+  a.amr_expect = p->d_expect;
+  a.amr_epoch = p->d_epoch;
+  a.amr_err = p->d_err;
   return a;
 }
 
@@ -333,19 +374,16 @@ int launch_kernel(aph_context *ctx, aph_plan *p, int recon, int stage, int nst, 
     APH_LAUNCH_CHECK(ctx);
     return APH_OK;
   }
-  // multilevel mesh: cross-level ghosts of the stage input, then the levels finest first
-  // (each records the fluxes on its faces to the level above and takes the mean of the
-  // recorded fluxes on its faces to the level below)
+  // multilevel mesh: cross-level ghosts of the stage input, then one launch whose items
+  // run the finest level first (each tile records the fluxes on its faces to a coarser
+  // leaf; a tile with a face to a finer region waits for them, then takes their mean)
   int rc = aph_amr_fill(ctx, p->amr, u_in);
   if (rc) return rc;
-  for (size_t ph = 0; ph + 1 < p->phase_start.size(); ++ph) {
-    const int n = p->phase_start[ph + 1] - p->phase_start[ph];
-    if (!n) continue;
-    a.items = p->d_items + p->phase_start[ph];
-    a.nitems = n;
-    hipLaunchKernelGGL(fn, dim3(p->grid), dim3(64), p->lds, ctx->stream, a);
-    APH_LAUNCH_CHECK(ctx);
-  }
+  k_bump<<<1, 1, 0, ctx->stream>>>(p->d_epoch);
+  hipLaunchKernelGGL(fn, dim3(p->grid), dim3(64), p->lds, ctx->stream, a);
+  APH_LAUNCH_CHECK(ctx);
+  k_amr_check<<<1, 1, 0, ctx->stream>>>(p->d_err, u_out);
+  APH_LAUNCH_CHECK(ctx);
   return APH_OK;
 }
 
@@ -470,6 +508,9 @@ int aph_plan_destroy(aph_context *ctx, aph_plan *p) {
   if (p->d_u2) (void)hipFree(p->d_u2);
   if (p->d_amr_face) (void)hipFree(p->d_amr_face);
   if (p->d_amr_buf) (void)hipFree(p->d_amr_buf);
+  for (void *q : {(void *)p->d_region, (void *)p->d_count, (void *)p->d_expect,
+                  (void *)p->d_epoch, (void *)p->d_err})
+    if (q) (void)hipFree(q);
   adj_release(p->adj);
   delete p;
   return APH_OK;
@@ -562,10 +603,36 @@ int aph_plan_set_amr(aph_context *ctx, aph_plan *p, const aph_amr_plan *amr,
                          f);
     }
   }
+  // region index per block face: a role-2 face's own region, a role-1 face's target
+  std::map<int64_t, int> region_index;
+  for (const auto &r : regions) region_index.emplace(r.first, (int)region_index.size());
+  p->face_region.assign((size_t)6 * bs->nblocks, -1);
+  for (int b = 0; b < bs->nblocks; ++b)
+    for (int f = 0; f < 6; ++f)
+      if (faces[b * 6 + f].role) p->face_region[b * 6 + f] = region_index.at(faces[b * 6 + f].base);
+  p->faces_h.assign(faces, faces + (size_t)6 * bs->nblocks);
+  p->nregions = (int)region_index.size();
   if (p->d_amr_face) (void)hipFree(p->d_amr_face);
   if (p->d_amr_buf) (void)hipFree(p->d_amr_buf);
+  for (void *q : {(void *)p->d_region, (void *)p->d_count, (void *)p->d_expect,
+                  (void *)p->d_epoch, (void *)p->d_err})
+    if (q) (void)hipFree(q);
   p->d_amr_face = nullptr;
   p->d_amr_buf = nullptr;
+  p->d_region = nullptr;
+  p->d_count = nullptr;
+  p->d_expect = nullptr;
+  p->d_epoch = nullptr;
+  p->d_err = nullptr;
+  const size_t nr = (size_t)std::max(p->nregions, 1);
+  if (hipMalloc(&p->d_region, sizeof(int32_t) * 6 * bs->nblocks) != hipSuccess ||
+      hipMalloc(&p->d_count, sizeof(unsigned long long) * nr) != hipSuccess ||
+      hipMalloc(&p->d_expect, sizeof(unsigned) * nr) != hipSuccess ||
+      hipMalloc(&p->d_epoch, sizeof(unsigned)) != hipSuccess ||
+      hipMalloc(&p->d_err, sizeof(int)) != hipSuccess)
+    return set_error(ctx, APH_E_NOMEM, "aph_plan_set_amr: device allocation failed");
+  APH_HIP_CHECK(ctx, hipMemcpy(p->d_region, p->face_region.data(),
+                               sizeof(int32_t) * 6 * bs->nblocks, hipMemcpyHostToDevice));
   if (hipMalloc(&p->d_amr_face, sizeof(aph_amr_face) * 6 * bs->nblocks) != hipSuccess ||
       hipMalloc(&p->d_amr_buf, sizeof(double) * std::max<int64_t>(buffer_doubles, 1)) !=
           hipSuccess)

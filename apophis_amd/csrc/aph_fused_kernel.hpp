@@ -41,6 +41,15 @@ struct FusedArgs {
   int last;        // last stage of the step: estimate the next dt
   const aph_amr_face *amr_face;  // multilevel mesh: [block][6] interface roles, or null
   double *amr_buf;               // interface flux buffer (fine side writes, coarse reads)
+  // one launch for every level: a block face's interface region (-1: none), per region
+  // the count of fine tiles that finished writing it (never reset; compared against
+  // epoch x expect), the fine tiles per region and launch, the launch epoch, and the flag
+  // a wait that ran out sets
+  const int *amr_region;
+  unsigned long long *amr_count;
+  const unsigned *amr_expect;
+  const unsigned *amr_epoch;
+  int *amr_err;
 };
 
 namespace {
@@ -94,6 +103,36 @@ __device__ __forceinline__ bool load_cell(const FusedArgs &a, const aph_block &b
   return flip;
 }
 
+// Bounded wait of a coarse tile for the fine tiles that write interface region r in this
+// launch (workgroup lane 0).  The items are ordered finest level first and every
+// workgroup of the persistent grid is resident, so the writers are running or done; the
+// bound only turns a broken plan into a flagged error instead of a hang.
+__device__ __forceinline__ void amr_wait(const FusedArgs &a, int r) {
+  constexpr int kSpinLimit = 1 << 20;
+  const unsigned long long target =
+      (unsigned long long)(*a.amr_epoch) * (unsigned long long)a.amr_expect[r];
+  int n = 0;
+  while (__hip_atomic_load(a.amr_count + r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
+         target) {
+    __builtin_amdgcn_s_sleep(4);
+    ++n;
+    // once any wait has run out the launch is lost: stop waiting at once
+    if ((n & 255) == 0 &&
+        __hip_atomic_load(a.amr_err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      break;
+    if (n > kSpinLimit) {
+      __hip_atomic_fetch_or(a.amr_err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      break;
+    }
+  }
+}
+
+// Faces of the tile (origin t0) on the block's low / high boundary of axis D.
+__device__ __forceinline__ bool tile_on_face(const TileCfg &cfg, const int *t0, int n_d,
+                                             int D, int side) {
+  return side ? (t0[D] + cfg.t[D] == n_d) : (t0[D] == 0);
+}
+
 template <int RECON, int RS, int NV, int NDIM, int D, bool AMR>
 __device__ __forceinline__ void sweep(const FusedArgs &a, const aph_block &bk, int bidx,
                                       const TileCfg &cfg, const int *t0, double *du_s,
@@ -122,6 +161,22 @@ __device__ __forceinline__ void sweep(const FusedArgs &a, const aph_block &bk, i
   const double area = bk.area[D];
   const double vol = bk.vol;
   const double beta_dt = a.beta * a.dt[0];
+  if constexpr (AMR) {
+    // a face to a finer region: wait until every fine tile has written its fluxes (the
+    // fences write back / invalidate the XCD's L2, so only interface tiles take them)
+    bool waits = false;
+    for (int side = 0; side < 2; ++side) {
+      const int fi = bidx * 6 + 2 * D + side;
+      if (tile_on_face(cfg, t0, n_d, D, side) && a.amr_face[fi].role == 2) {
+        waits = true;
+        if (threadIdx.x == 0) amr_wait(a, a.amr_region[fi]);
+      }
+    }
+    if (waits) {
+      __syncthreads();
+      __threadfence();  // acquire: the buffer is read after the counts were seen
+    }
+  }
   const int tx = cfg.t[0], ty = cfg.t[1];
   for (int unit = threadIdx.x; unit < units; unit += 64) {
     int seg, pa, pb;
@@ -256,6 +311,22 @@ __device__ __forceinline__ void sweep(const FusedArgs &a, const aph_block &bk, i
       for (int v = 0; v < NV; ++v) fprev[v] = fo[v];
     };
     march<RECON, RS, NV, 1 + D, NV>(f_lo, f_hi, fetch, convert, emit);
+  }
+  if constexpr (AMR) {
+    // a face to a coarser leaf: publish this tile's interface fluxes
+    bool pub[2];
+    for (int side = 0; side < 2; ++side)
+      pub[side] = tile_on_face(cfg, t0, n_d, D, side) &&
+                  a.amr_face[bidx * 6 + 2 * D + side].role == 1;
+    if (pub[0] || pub[1]) {
+      __threadfence();  // release: the buffer writes of every lane before the count
+      __syncthreads();
+      if (threadIdx.x == 0)
+        for (int side = 0; side < 2; ++side)
+          if (pub[side])
+            __hip_atomic_fetch_add(a.amr_count + a.amr_region[bidx * 6 + 2 * D + side], 1ull,
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
   }
 }
 

@@ -330,8 +330,11 @@ int aph_plan_estimate_timestep(aph_context *ctx, aph_plan *plan, const double *c
    ascending order, the first slowest) and child (the same order); the coarse side
    replaces each boundary face flux by the mean of its children there, in that order --
    the flux correction of aph_amr_flux_correct, bitwise.  A block between two levels can
-   hold faces of both roles.  A stage runs aph_amr_fill on its input, then the kernel
-   once per level, the finest first. */
+   hold faces of both roles.  A stage runs aph_amr_fill on its input, then one persistent
+   launch over every level whose items run the finest level first: a tile with a face to
+   a finer region waits (bounded) until the fine tiles that write it have published.  If
+   that wait runs out (the device could not keep the whole grid resident), the stage
+   writes NaN to the first element of u_out. */
 typedef struct aph_amr_face {
   int64_t base;  /* element offset of the coarse face region in the interface buffer */
   int32_t role;  /* 0 none, 1 fine side (writes), 2 coarse side (reads)               */

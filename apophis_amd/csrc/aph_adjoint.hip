@@ -23,11 +23,13 @@
 //     k_adj_axis<D>    one lane per (block, pencil, segment) re-marches the forward
 //                      (ConsToPrim, reconstruction, Riemann) and back-propagates mu_f
 //                      through the Riemann solver and both adjacent reconstructions.
-//   Both forms write the primitive-space cotangent of a segment's own cells to Pbar (x
-//   writes, y and z add) and that of the <= 3 cells it reads beyond the segment on each
-//   side to a 6-slot spill record (primitive space; at a reflecting-boundary image the
-//   normal velocity's cotangent is flipped back to the interior cell's sign).
-//   k_adj_gather     per interior cell: Pbar plus the spills of the segments / blocks /
+//   Both forms write the primitive-space cotangent of a segment's own cells to the axis'
+//   own Pbar array (so the axes are independent and the PLM axes run in one launch) and
+//   that of the <= 3 cells it reads beyond the segment on each side to a 6-slot spill
+//   record (primitive space; at a reflecting-boundary image the normal velocity's
+//   cotangent is flipped back to the interior cell's sign).
+//   k_adj_gather     per interior cell: the axes' Pbar summed x, y, z, plus the spills of
+//                    the segments / blocks /
 //                    boundary images that read this cell (fixed order), the gravity
 //                    adjoint, J_prim^T, and the gam terms.
 // The x sweeps of the split form keep P, L_out and QL_bar / QR_bar in the x-transposed
@@ -68,7 +70,8 @@ struct AdjArgs {
   const double *dt;
   double gam0, gam1, beta;
   int stage;
-  double *pbar;                // NF x ncells, pack layout
+  double *pbar;                // per axis NF x ncells, pack layout (axis d at d * pbar_axis)
+  int64_t pbar_axis;
   double *prim, *prim_t;       // split form: NF x ncells, P of the interior cells in the
                                // pack / x-transposed layout
   double *lam_t;               // split form: NV x ncells, L_out x-transposed
@@ -452,7 +455,7 @@ __global__ void __launch_bounds__(64, 2) k_adj_face(AdjArgs a) {
 // cell m-1's share plus cell m's -- once.  A face between two segments gets each
 // segment's share from that segment's lane (the face adjoint is linear in its cotangent).
 // The contributions to cells m-2 .. m+2 are summed in a window; the segment's own cells
-// go to Pbar (axis x writes, y and z add), the <= 3 cells on either side to the segment's
+// go to the axis' Pbar, the <= 3 cells on either side to the segment's
 // 6-slot spill record.
 template <int RECON, int NV, int NDIM, int D>
 __global__ void __launch_bounds__(64) k_adj_recon(AdjArgs a) {
@@ -488,7 +491,7 @@ __global__ void __launch_bounds__(64) k_adj_recon(AdjArgs a) {
   const double *prim = (XT ? a.prim_t : a.prim) + fld * nc + L.base;
   const double *qlb_ptr = a.qlb + (int64_t)NF * bk.cell_base + fld * nc + L.base;
   const double *qrb_ptr = a.qrb + (int64_t)NF * bk.cell_base + fld * nc + L.base;
-  double *pb_ptr = a.pbar + (int64_t)NF * bk.cell_base + fld * nc + Lp.base;
+  double *pb_ptr = a.pbar + D * a.pbar_axis + (int64_t)NF * bk.cell_base + fld * nc + Lp.base;
   double *sp = a.spill[D] + a.spill_base[D * a.nblocks + b] +
                ((int64_t)pencil * nseg + seg) * 6 * NF + fld;
   // every halo cell is emitted exactly once: clear the record, then write each slot once
@@ -511,16 +514,12 @@ __global__ void __launch_bounds__(64) k_adj_recon(AdjArgs a) {
     v = prim[(int64_t)NF * base + (int64_t)idx * L.sd];
     ng = flip && vel;
   };
-  // y and z add into Pbar.  Each cell's value has one owner lane here, so the march loads
-  // it a step before it adds (pre), keeping the load off the step's critical path.
+  // each axis writes its own Pbar (the gather sums the axes in order), one owner lane
+  // per cell
   auto own = [&](int c) { return c >= a0 && c < a0 + S; };
-  auto emit = [&](int c, double v, const double *pre) {
+  auto emit = [&](int c, double v) {
     if (own(c)) {
-      double *q = pb_ptr + (int64_t)c * Lp.sd;
-      if (D == 0)
-        *q = v;
-      else
-        *q = (pre ? *pre : *q) + v;
+      pb_ptr[(int64_t)c * Lp.sd] = v;
     } else {
       const bool below = c < a0;
       const int slot = below ? c - (a0 - 3) : 3 + (c - (a0 + S));
@@ -556,11 +555,8 @@ __global__ void __launch_bounds__(64) k_adj_recon(AdjArgs a) {
   fetch(m_lo + GR, nv, nng);
   nql = (m_lo <= n_d - 1) ? qlb_ptr[(int64_t)m_lo * L.sd] : 0.0;
   nqr = (m_lo >= 0) ? qrb_ptr[(int64_t)m_lo * L.sd] : 0.0;
-  double npb = 0.0;  // Pbar of the cell the next step emits (y, z)
   for (int mm = m_lo; mm <= m_hi; ++mm) {
-    const double pb_cur = npb;  // cell mm - GR
     // (the first step's cell, m_lo - GR, is never the segment's own: no load needed)
-    if (D > 0 && own(mm + 1 - GR)) npb = pb_ptr[(int64_t)(mm + 1 - GR) * Lp.sd];
 #pragma unroll
     for (int s = 0; s < W - 1; ++s) col[s] = col[s + 1];
     col[W - 1] = nng ? -1.0 * nv : nv;
@@ -582,20 +578,20 @@ __global__ void __launch_bounds__(64) k_adj_recon(AdjArgs a) {
     flo = fhi;
 #pragma unroll
     for (int s = 0; s < W; ++s) cw[s] += colb[s];
-    emit(mm - GR, cw[0], &pb_cur);  // no later cell or face reaches mm-GR
+    emit(mm - GR, cw[0]);  // no later cell or face reaches mm-GR
 #pragma unroll
     for (int s = 0; s < W - 1; ++s) cw[s] = cw[s + 1];
     cw[W - 1] = 0.0;
   }
   ppm_face_adj(flo.t, fb_lo, cw);  // face m_hi+1/2 -> cells m_hi-1 .. m_hi+2 = cw[0..3]
 #pragma unroll
-  for (int s = 0; s < W - 1; ++s) emit(m_hi - GR + 1 + s, cw[s], nullptr);
+  for (int s = 0; s < W - 1; ++s) emit(m_hi - GR + 1 + s, cw[s]);
 }
 
 // One-march form (PLM) -------------------------------------------------------------
 
 template <int RECON, int RS, int NV, int NDIM, int D>
-__global__ void __launch_bounds__(64) k_adj_axis(AdjArgs a) {
+__device__ __forceinline__ void adj_axis_body(const AdjArgs &a) {
   constexpr int GR = ReconTraits<RECON>::GR;
   constexpr int W = ReconTraits<RECON>::W;
   constexpr int NF = NV + 2;
@@ -640,7 +636,7 @@ __global__ void __launch_bounds__(64) k_adj_axis(AdjArgs a) {
   const int fend = (seg == nseg - 1) ? n_d + 1 : a0 + S;  // faces [a0, fend)
   const double coef = a.beta * a.dt[0] * bk.area[D] / bk.vol;
   const double *lo_ptr = a.lam_out + m.self + tbase;
-  double *pb_ptr = a.pbar + (int64_t)NF * bk.cell_base + tbase;
+  double *pb_ptr = a.pbar + D * a.pbar_axis + (int64_t)NF * bk.cell_base + tbase;
   double *sp = a.spill[D] + a.spill_base[D * a.nblocks + b] +
                ((int64_t)pencil * nseg + seg) * 6 * NF;
   // every halo cell is emitted exactly once: clear the record, then write each slot once
@@ -651,27 +647,13 @@ __global__ void __launch_bounds__(64) k_adj_axis(AdjArgs a) {
     map_load<NV>(a, m, D, n_d, c, tbase, st[D], nc, uu);
     cons_to_prim_pt<NV>(uu, a.eos, dst, dst + NV);
   };
-  // final primitive-space cotangent of cell c from this segment
-  // y and z add into Pbar; every cell has one owner lane, so the value of the next cell
-  // to emit is loaded right after the previous emit (pre), a step before it is needed
-  double npb[NF];
-  auto prefetch = [&](int c) {
-    if (D > 0 && c >= a0 && c < a0 + S) {
-      const double *q = pb_ptr + (int64_t)c * st[D];
-#pragma unroll
-      for (int f = 0; f < NF; ++f) npb[f] = q[f * nc];
-    }
-  };
-  auto emit = [&](int c, const double *pbv, bool pre) {
+  // final primitive-space cotangent of cell c from this segment: each axis writes its own
+  // Pbar (the gather sums the axes in order), one owner lane per cell
+  auto emit = [&](int c, const double *pbv) {
     if (c >= a0 && c < a0 + S) {
       double *q = pb_ptr + (int64_t)c * st[D];
 #pragma unroll
-      for (int f = 0; f < NF; ++f) {
-        if (D == 0)
-          q[f * nc] = pbv[f];
-        else
-          q[f * nc] = (pre ? npb[f] : q[f * nc]) + pbv[f];
-      }
+      for (int f = 0; f < NF; ++f) q[f * nc] = pbv[f];
     } else {
       const bool below = c < a0;
       const int slot = below ? c - (a0 - 3) : 3 + (c - (a0 + S));
@@ -783,9 +765,8 @@ __global__ void __launch_bounds__(64) k_adj_axis(AdjArgs a) {
         for (int s = 0; s < W; ++s) cw[s][fld] += colb[s];
       }
       // cell mm-1-GR is complete: emit it and shift the cotangent window
-      emit(emitted, cw[0], true);
+      emit(emitted, cw[0]);
       ++emitted;
-      prefetch(emitted);
 #pragma unroll
       for (int s = 0; s < W - 1; ++s)
 #pragma unroll
@@ -796,7 +777,6 @@ __global__ void __launch_bounds__(64) k_adj_axis(AdjArgs a) {
       // mm == a0-1: no face yet; cells up to mm-1+GR stay pending.  The window of
       // cotangents starts at cell mm-GR from the next step on.
       emitted = mm - GR;
-      prefetch(emitted);
     }
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
@@ -820,7 +800,23 @@ __global__ void __launch_bounds__(64) k_adj_axis(AdjArgs a) {
       for (int s = 0; s < W; ++s) cw[s][fld] += colb[s];
     }
 #pragma unroll
-    for (int s = 0; s < W; ++s) emit(emitted + s, cw[s], s == 0);
+    for (int s = 0; s < W; ++s) emit(emitted + s, cw[s]);
+  }
+}
+
+// Every axis of the one-march form in one launch (blockIdx.z = axis): the axes write
+// separate Pbar arrays and spill records, so they are independent, and a small block set
+// fills the GPU twice (2D) or three times (3D) as deep as one axis alone.
+template <int RECON, int RS, int NV, int NDIM>
+__global__ void __launch_bounds__(64) k_adj_axes(AdjArgs a) {
+  if (blockIdx.z == 0) {
+    adj_axis_body<RECON, RS, NV, NDIM, 0>(a);
+  } else if constexpr (NDIM >= 2) {
+    if (blockIdx.z == 1) {
+      adj_axis_body<RECON, RS, NV, NDIM, 1>(a);
+    } else if constexpr (NDIM >= 3) {
+      adj_axis_body<RECON, RS, NV, NDIM, 2>(a);
+    }
   }
 }
 
@@ -859,10 +855,15 @@ __global__ void __launch_bounds__(256) k_adj_gather(AdjArgs a) {
       u[v] = a.u_in[o + v * nc];
       acc[v] = gsum * lo[v] + (a.lam_acc ? a.lam_acc[o + v * nc] : 0.0);
     }
+    // the axes' Pbar, summed x, then y, then z
     double pbv[NF];
     const double *pp = a.pbar + (int64_t)NF * bk.cell_base + cell;
 #pragma unroll
     for (int f = 0; f < NF; ++f) pbv[f] = pp[f * nc];
+#pragma unroll
+    for (int d = 1; d < NDIM; ++d)
+#pragma unroll
+      for (int f = 0; f < NF; ++f) pbv[f] = pbv[f] + pp[d * a.pbar_axis + f * nc];
     // spills (primitive space) of the segments / blocks / boundary images that read
     // this cell
     for (int d = 0; d < NDIM; ++d) {
@@ -942,7 +943,7 @@ void set_axis(AxisKernels &k) {
     k.face = k_adj_face<RECON, RS, NV, ND, D>;
     k.recon = k_adj_recon<RECON, NV, ND, D>;
   } else {
-    k.axis = k_adj_axis<RECON, RS, NV, ND, D>;
+    k.axis = k_adj_axes<RECON, RS, NV, ND>;  // every axis in one launch
   }
 }
 
@@ -1043,7 +1044,7 @@ int adj_prepare(aph_context *ctx, const aph_blockset *bs, int recon, AdjScratch 
       base[(size_t)d * nb + b] = tot[d];
       tot[d] += (int64_t)k.nx[A] * k.nx[B] * nseg * 6 * nf;
     }
-  if (hipMalloc(&s.pbar, sizeof(double) * nf * bs->ncells) != hipSuccess ||
+  if (hipMalloc(&s.pbar, sizeof(double) * nf * bs->ncells * bs->ndim) != hipSuccess ||
       hipMalloc(&s.lam1, sizeof(double) * nv * bs->ncells) != hipSuccess ||
       hipMalloc(&s.lam0p, sizeof(double) * nv * bs->ncells) != hipSuccess ||
       hipMalloc(&s.spill_base, sizeof(int64_t) * base.size()) != hipSuccess)
@@ -1053,7 +1054,7 @@ int adj_prepare(aph_context *ctx, const aph_blockset *bs, int recon, AdjScratch 
       return set_error(ctx, APH_E_NOMEM, "adjoint spill allocation failed");
   APH_HIP_CHECK(ctx, hipMemcpy(s.spill_base, base.data(), sizeof(int64_t) * base.size(),
                                hipMemcpyHostToDevice));
-  APH_HIP_CHECK(ctx, hipMemset(s.pbar, 0, sizeof(double) * nf * bs->ncells));
+  APH_HIP_CHECK(ctx, hipMemset(s.pbar, 0, sizeof(double) * nf * bs->ncells * bs->ndim));
   APH_HIP_CHECK(ctx, hipMemset(s.lam1, 0, sizeof(double) * nv * bs->ncells));
   APH_HIP_CHECK(ctx, hipMemset(s.lam0p, 0, sizeof(double) * nv * bs->ncells));
   s.ncells = bs->ncells;
@@ -1102,6 +1103,7 @@ int adj_stage(aph_context *ctx, const aph_blockset *bs, int recon, int rs, const
   a.beta = stage == 1 ? 1.0 : 0.5;
   a.stage = stage;
   a.pbar = s.pbar;
+  a.pbar_axis = (int64_t)(bs->nvar + 2) * bs->ncells;
   a.prim = s.prim;
   a.prim_t = s.prim_t;
   a.lam_t = s.lam_t;
@@ -1114,6 +1116,8 @@ int adj_stage(aph_context *ctx, const aph_blockset *bs, int recon, int rs, const
   const dim3 gc((unsigned)std::max<int64_t>(1, std::min<int64_t>(64, (bs->max_interior + 255) / 256)),
                 (unsigned)bs->nblocks);
   if (ks[0].face) hipLaunchKernelGGL(pick_prim(bs->nvar, nd), gc, dim3(256), 0, ctx->stream, a);
+  const unsigned nbk = (unsigned)bs->nblocks;
+  int64_t maxu_all = 0;
   for (int d = 0; d < nd; ++d) {
     int64_t maxu = 0;
     for (const auto &k : bs->h) {
@@ -1121,17 +1125,17 @@ int adj_stage(aph_context *ctx, const aph_blockset *bs, int recon, int rs, const
       maxu = std::max<int64_t>(maxu, (int64_t)k.nx[A] * k.nx[B] *
                                          (k.nx[d] / adj_seg(k.nx[d], s.segcap[d])));
     }
-    const unsigned nbk = (unsigned)bs->nblocks;
+    maxu_all = std::max(maxu_all, maxu);
     if (ks[d].face) {
       hipLaunchKernelGGL(ks[d].face, dim3((unsigned)((maxu + 63) / 64), nbk), dim3(64), 0,
                          ctx->stream, a);
       hipLaunchKernelGGL(ks[d].recon, dim3((unsigned)((maxu * nf + 63) / 64), nbk), dim3(64),
                          0, ctx->stream, a);
-    } else {
-      hipLaunchKernelGGL(ks[d].axis, dim3((unsigned)((maxu + 63) / 64), nbk), dim3(64), 0,
-                         ctx->stream, a);
     }
   }
+  if (ks[0].axis)  // one-march form: all axes in one launch, z = axis
+    hipLaunchKernelGGL(ks[0].axis, dim3((unsigned)((maxu_all + 63) / 64), nbk, (unsigned)nd),
+                       dim3(64), 0, ctx->stream, a);
   hipLaunchKernelGGL(pick_gather(bs->nvar, nd), gc, dim3(256), 0, ctx->stream, a);
   APH_LAUNCH_CHECK(ctx);
   return APH_OK;

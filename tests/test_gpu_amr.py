@@ -294,3 +294,47 @@ def test_three_level_steps_with_a_passive_scalar(gpu_ctx, form):
     mask = np.tile(bs.interior_mask(), 6)
     assert np.array_equal(got[mask], ref.u0[mask])
     assert abs(tot(ref.u0) - s0) <= 1e-13 * abs(s0)
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_fused_multilevel_with_several_tiles_per_face(gpu_ctx, ndim):
+    """Three levels with blocks larger than the tile (8-cell tiles: 4 tiles per block face),
+    so every interface region is written by several fine tiles and read by several coarse
+    tiles inside the one launch: the fused steps equal the plain form's bitwise."""
+    print(f"[case] multilevel tiles {ndim}d", flush=True)
+    if ndim == 2:
+        nodes = [(0, (x, y)) for x in (1, 2) for y in (1, 2)] + [(1, (3, 3)), (1, (4, 3))]
+        m = amr.AmrMesh(2, (4, 4), (32, 32), 2, (0, 0, 0), (1, 1, 1), nodes, max_level=2)
+        tile = (32, 8, 8, 1)
+    else:
+        nodes = [(0, (x, y, z)) for x in (1, 2) for y in (1, 2) for z in (0, 1)] + \
+            [(1, (3, 3, 1))]
+        m = amr.AmrMesh(3, (4, 4, 2), (16, 16, 16), 2, (0, 0, 0), (1, 1, 1), nodes,
+                        max_level=2)
+        tile = (16, 8, 8, 8)
+    p = _problem(m)
+    ref = ao.AmrState(p, m) if ndim == 2 else None
+    pkg = hydro.initialize_hydro({
+        "parthenon/time": {"cfl": p.cfl, "integrator": "rk2"},
+        "parthenon/mesh": {"nghost": m.ng},
+        "hydro": {"reconstruction": "plm", "riemann": "hllc"},
+        "eos": {"gm1": float(p.eos["gm1"][0])}})
+    md = hydro.MeshData(gpu_ctx, p.bs, p.cons)
+    drv = hydro.ApophisDriver(pkg, md, amr=amr.AmrPlan(gpu_ctx, md.dbs, m, p.bs))
+    dt = drv.initialize()
+    fd = hydro.FusedDriver(gpu_ctx, p.bs, p.cons, "plm", "hllc", p.eos, p.cfl, amr_mesh=m)
+    fd.set_tile(*tile)
+    assert fd.tiles()[tuple(int(v) for v in p.bs.blocks["nx"][0])] == tile[1:]
+    fd.set_dt(dt)
+    for _ in range(2):
+        drv.step(dt)
+        fd.step(fixed_dt=True)
+        if ref is not None:
+            ref.step(dt)
+    torch.cuda.synchronize()
+    mask = np.tile(p.bs.interior_mask(), 5)
+    plain = md.cons.cpu().numpy()
+    assert np.array_equal(fd.u0.cpu().numpy()[mask], plain[mask])
+    assert np.isfinite(plain[mask]).all()
+    if ref is not None:
+        assert np.array_equal(plain, ref.u0)

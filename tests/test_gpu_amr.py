@@ -338,3 +338,30 @@ def test_fused_multilevel_with_several_tiles_per_face(gpu_ctx, ndim):
     assert np.isfinite(plain[mask]).all()
     if ref is not None:
         assert np.array_equal(plain, ref.u0)
+
+
+def test_autotuned_multilevel_plan_matches_plain(gpu_ctx):
+    """aph_plan_autotune on a multilevel plan (every candidate tile rebuilds the item list
+    and the interface counts) leaves a plan whose steps equal the plain form's bitwise."""
+    print("[case] multilevel autotune", flush=True)
+    nodes = [(0, (x, y)) for x in (1, 2) for y in (1, 2)] + [(1, (3, 3))]
+    m = amr.AmrMesh(2, (4, 4), (32, 32), 2, (0, 0, 0), (1, 1, 1), nodes, max_level=2)
+    p = _problem(m)
+    pkg = hydro.initialize_hydro({
+        "parthenon/time": {"cfl": p.cfl, "integrator": "rk2"},
+        "parthenon/mesh": {"nghost": m.ng},
+        "hydro": {"reconstruction": "plm", "riemann": "hllc"},
+        "eos": {"gm1": float(p.eos["gm1"][0])}})
+    md = hydro.MeshData(gpu_ctx, p.bs, p.cons)
+    drv = hydro.ApophisDriver(pkg, md, amr=amr.AmrPlan(gpu_ctx, md.dbs, m, p.bs))
+    dt = drv.initialize()
+    fd = hydro.FusedDriver(gpu_ctx, p.bs, p.cons, "plm", "hllc", p.eos, p.cfl, amr_mesh=m)
+    fd.set_dt(dt)
+    assert fd.autotune(reps=2)  # a non-empty timing report
+    fd.u0.copy_(torch.from_numpy(p.cons))  # the cross-level fill wrote u0's ghost cells
+    for _ in range(2):
+        drv.step(dt)
+        fd.step(fixed_dt=True)
+    torch.cuda.synchronize()
+    mask = np.tile(p.bs.interior_mask(), 5)
+    assert np.array_equal(fd.u0.cpu().numpy()[mask], md.cons.cpu().numpy()[mask])

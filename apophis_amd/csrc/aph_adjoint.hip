@@ -72,6 +72,7 @@ struct AdjArgs {
   int stage;
   double *pbar;                // per axis NF x ncells, pack layout (axis d at d * pbar_axis)
   int64_t pbar_axis;
+  int64_t qb_axis;             // per axis offset of qlb / qrb (the axes run concurrently)
   double *prim, *prim_t;       // split form: NF x ncells, P of the interior cells in the
                                // pack / x-transposed layout
   double *lam_t;               // split form: NV x ncells, L_out x-transposed
@@ -283,7 +284,7 @@ __global__ void __launch_bounds__(256) k_adj_prim(AdjArgs a) {
 // output of cell f), NF fields each; the block-local copies for the cells -1 and n go to
 // the ghost slots.
 template <int RECON, int RS, int NV, int NDIM, int D>
-__global__ void __launch_bounds__(64, 2) k_adj_face(AdjArgs a) {
+__device__ __forceinline__ void adj_face_body(const AdjArgs &a) {
   constexpr int GR = ReconTraits<RECON>::GR;
   constexpr int W = ReconTraits<RECON>::W;
   constexpr int NF = NV + 2;
@@ -310,8 +311,8 @@ __global__ void __launch_bounds__(64, 2) k_adj_face(AdjArgs a) {
   const double coef = a.beta * a.dt[0] * bk.area[D] / bk.vol;
   const double *prim = XT ? a.prim_t : a.prim;
   const double *lo_ptr = (XT ? a.lam_t : a.lam_out) + (int64_t)NV * bk.cell_base + L.base;
-  double *qlb_ptr = a.qlb + (int64_t)NF * bk.cell_base + L.base;
-  double *qrb_ptr = a.qrb + (int64_t)NF * bk.cell_base + L.base;
+  double *qlb_ptr = a.qlb + D * a.qb_axis + (int64_t)NF * bk.cell_base + L.base;
+  double *qrb_ptr = a.qrb + D * a.qb_axis + (int64_t)NF * bk.cell_base + L.base;
   auto load = [&](int c, double *dst) {
     int idx;
     bool flip;
@@ -458,7 +459,7 @@ __global__ void __launch_bounds__(64, 2) k_adj_face(AdjArgs a) {
 // go to the axis' Pbar, the <= 3 cells on either side to the segment's
 // 6-slot spill record.
 template <int RECON, int NV, int NDIM, int D>
-__global__ void __launch_bounds__(64) k_adj_recon(AdjArgs a) {
+__device__ __forceinline__ void adj_recon_body(const AdjArgs &a) {
   static_assert(RECON == RECON_PPM, "the split form is the PPM adjoint");
   constexpr int GR = ReconTraits<RECON>::GR;
   constexpr int W = ReconTraits<RECON>::W;
@@ -489,8 +490,8 @@ __global__ void __launch_bounds__(64) k_adj_recon(AdjArgs a) {
   const int m_lo = (seg == 0) ? -1 : a0;
   const int m_hi = (seg == nseg - 1) ? n_d : a0 + S - 1;
   const double *prim = (XT ? a.prim_t : a.prim) + fld * nc + L.base;
-  const double *qlb_ptr = a.qlb + (int64_t)NF * bk.cell_base + fld * nc + L.base;
-  const double *qrb_ptr = a.qrb + (int64_t)NF * bk.cell_base + fld * nc + L.base;
+  const double *qlb_ptr = a.qlb + D * a.qb_axis + (int64_t)NF * bk.cell_base + fld * nc + L.base;
+  const double *qrb_ptr = a.qrb + D * a.qb_axis + (int64_t)NF * bk.cell_base + fld * nc + L.base;
   double *pb_ptr = a.pbar + D * a.pbar_axis + (int64_t)NF * bk.cell_base + fld * nc + Lp.base;
   double *sp = a.spill[D] + a.spill_base[D * a.nblocks + b] +
                ((int64_t)pencil * nseg + seg) * 6 * NF + fld;
@@ -586,6 +587,34 @@ __global__ void __launch_bounds__(64) k_adj_recon(AdjArgs a) {
   ppm_face_adj(flo.t, fb_lo, cw);  // face m_hi+1/2 -> cells m_hi-1 .. m_hi+2 = cw[0..3]
 #pragma unroll
   for (int s = 0; s < W - 1; ++s) emit(m_hi - GR + 1 + s, cw[s]);
+}
+
+// The split form's axes in one launch each (blockIdx.z = axis): they read the same
+// primitives and write separate QL_bar / QR_bar, Pbar arrays and spill records.
+template <int RECON, int RS, int NV, int NDIM>
+__global__ void __launch_bounds__(64, 2) k_adj_face(AdjArgs a) {
+  if (blockIdx.z == 0) {
+    adj_face_body<RECON, RS, NV, NDIM, 0>(a);
+  } else if constexpr (NDIM >= 2) {
+    if (blockIdx.z == 1) {
+      adj_face_body<RECON, RS, NV, NDIM, 1>(a);
+    } else if constexpr (NDIM >= 3) {
+      adj_face_body<RECON, RS, NV, NDIM, 2>(a);
+    }
+  }
+}
+
+template <int RECON, int NV, int NDIM>
+__global__ void __launch_bounds__(64) k_adj_recon(AdjArgs a) {
+  if (blockIdx.z == 0) {
+    adj_recon_body<RECON, NV, NDIM, 0>(a);
+  } else if constexpr (NDIM >= 2) {
+    if (blockIdx.z == 1) {
+      adj_recon_body<RECON, NV, NDIM, 1>(a);
+    } else if constexpr (NDIM >= 3) {
+      adj_recon_body<RECON, NV, NDIM, 2>(a);
+    }
+  }
 }
 
 // One-march form (PLM) -------------------------------------------------------------
@@ -940,8 +969,8 @@ struct AxisKernels {
 template <int RECON, int RS, int NV, int ND, int D>
 void set_axis(AxisKernels &k) {
   if constexpr (RECON == RECON_PPM) {
-    k.face = k_adj_face<RECON, RS, NV, ND, D>;
-    k.recon = k_adj_recon<RECON, NV, ND, D>;
+    k.face = k_adj_face<RECON, RS, NV, ND>;  // every axis in one launch
+    k.recon = k_adj_recon<RECON, NV, ND>;
   } else {
     k.axis = k_adj_axes<RECON, RS, NV, ND>;  // every axis in one launch
   }
@@ -1005,8 +1034,8 @@ int adj_prepare(aph_context *ctx, const aph_blockset *bs, int recon, AdjScratch 
     if (hipMalloc(&s.prim, sizeof(double) * nf * bs->ncells) != hipSuccess ||
         hipMalloc(&s.prim_t, sizeof(double) * nf * bs->ncells) != hipSuccess ||
         hipMalloc(&s.lam_t, sizeof(double) * nv * bs->ncells) != hipSuccess ||
-        hipMalloc(&s.qlb, sizeof(double) * nf * bs->ncells) != hipSuccess ||
-        hipMalloc(&s.qrb, sizeof(double) * nf * bs->ncells) != hipSuccess)
+        hipMalloc(&s.qlb, sizeof(double) * nf * bs->ncells * nd) != hipSuccess ||
+        hipMalloc(&s.qrb, sizeof(double) * nf * bs->ncells * nd) != hipSuccess)
       return set_error(ctx, APH_E_NOMEM, "adjoint scratch allocation failed");
   }
   if (s.pbar) return APH_OK;
@@ -1104,6 +1133,7 @@ int adj_stage(aph_context *ctx, const aph_blockset *bs, int recon, int rs, const
   a.stage = stage;
   a.pbar = s.pbar;
   a.pbar_axis = (int64_t)(bs->nvar + 2) * bs->ncells;
+  a.qb_axis = (int64_t)(bs->nvar + 2) * bs->ncells;
   a.prim = s.prim;
   a.prim_t = s.prim_t;
   a.lam_t = s.lam_t;
@@ -1126,12 +1156,13 @@ int adj_stage(aph_context *ctx, const aph_blockset *bs, int recon, int rs, const
                                          (k.nx[d] / adj_seg(k.nx[d], s.segcap[d])));
     }
     maxu_all = std::max(maxu_all, maxu);
-    if (ks[d].face) {
-      hipLaunchKernelGGL(ks[d].face, dim3((unsigned)((maxu + 63) / 64), nbk), dim3(64), 0,
-                         ctx->stream, a);
-      hipLaunchKernelGGL(ks[d].recon, dim3((unsigned)((maxu * nf + 63) / 64), nbk), dim3(64),
-                         0, ctx->stream, a);
-    }
+  }
+  if (ks[0].face) {  // split form: the faces of every axis, then their reconstructions
+    hipLaunchKernelGGL(ks[0].face, dim3((unsigned)((maxu_all + 63) / 64), nbk, (unsigned)nd),
+                       dim3(64), 0, ctx->stream, a);
+    hipLaunchKernelGGL(ks[0].recon,
+                       dim3((unsigned)((maxu_all * nf + 63) / 64), nbk, (unsigned)nd), dim3(64),
+                       0, ctx->stream, a);
   }
   if (ks[0].axis)  // one-march form: all axes in one launch, z = axis
     hipLaunchKernelGGL(ks[0].axis, dim3((unsigned)((maxu_all + 63) / 64), nbk, (unsigned)nd),

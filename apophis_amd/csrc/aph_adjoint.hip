@@ -11,27 +11,28 @@
 //   per SIMD):
 //     k_adj_prim       P = ConsToPrim(U) of every interior cell, in the pack layout and in
 //                      the x-transposed layout, and L_out x-transposed;
-//     per axis D:
-//     k_adj_face<D>    one lane per (block, pencil, segment) re-marches the forward
-//                      reconstruction over its faces from P and back-propagates the face
-//                      cotangents mu_f = beta dt A/V (L_out(f) - L_out(f-1)) through the
-//                      Riemann solver: QL_bar / QR_bar, the cotangents of every cell's two
-//                      reconstructed face states;
-//     k_adj_recon<D>   one lane per (block, pencil, segment, field) applies the adjoint of
-//                      that field's reconstruction of each cell to them.
+//     k_adj_face       (every axis in one launch, blockIdx.z = axis) one lane per (block,
+//                      pencil, segment) re-marches the forward reconstruction over its
+//                      faces from P and back-propagates the face cotangents
+//                      mu_f = beta dt A/V (L_out(f) - L_out(f-1)) through the Riemann
+//                      solver: QL_bar / QR_bar (per axis), the cotangents of every cell's
+//                      two reconstructed face states;
+//     k_adj_recon      (every axis in one launch) one lane per (block, pencil, segment,
+//                      field) applies the adjoint of that field's reconstruction of each
+//                      cell to them.
 //   PLM, one march per axis:
-//     k_adj_axis<D>    one lane per (block, pencil, segment) re-marches the forward
-//                      (ConsToPrim, reconstruction, Riemann) and back-propagates mu_f
-//                      through the Riemann solver and both adjacent reconstructions.
+//     k_adj_axes       (every axis in one launch) one lane per (block, pencil, segment)
+//                      re-marches the forward (ConsToPrim, reconstruction, Riemann) and
+//                      back-propagates mu_f through the Riemann solver and both adjacent
+//                      reconstructions.
 //   Both forms write the primitive-space cotangent of a segment's own cells to the axis'
-//   own Pbar array (so the axes are independent and the PLM axes run in one launch) and
-//   that of the <= 3 cells it reads beyond the segment on each side to a 6-slot spill
-//   record (primitive space; at a reflecting-boundary image the normal velocity's
-//   cotangent is flipped back to the interior cell's sign).
+//   own Pbar array (the axes are independent) and that of the <= 3 cells it reads beyond
+//   the segment on each side to a 6-slot spill record (primitive space; at a
+//   reflecting-boundary image the normal velocity's cotangent is flipped back to the
+//   interior cell's sign).
 //   k_adj_gather     per interior cell: the axes' Pbar summed x, y, z, plus the spills of
-//                    the segments / blocks /
-//                    boundary images that read this cell (fixed order), the gravity
-//                    adjoint, J_prim^T, and the gam terms.
+//                    the segments / blocks / boundary images that read this cell (fixed
+//                    order), the gravity adjoint, J_prim^T, and the gam terms.
 // The x sweeps of the split form keep P, L_out and QL_bar / QR_bar in the x-transposed
 // layout (y fastest, then z, then x): the lanes of a wave take adjacent pencils, so every
 // load of the march reads adjacent addresses.

@@ -104,36 +104,7 @@ class AmrMesh:
         self.level = np.array([lv for lv, _ in leaves], dtype=np.int32)
         self.loc = np.array([lc for _, lc in leaves], dtype=np.int64).reshape(-1, 3)
         self.index = {(lv, lc): b for b, (lv, lc) in enumerate(leaves)}
-        self._check_balance()
-        span = [float(xmax[d]) - float(xmin[d]) for d in range(3)]
-        out = np.zeros(len(leaves), dtype=BLOCK_DTYPE)
-        for b, (lv, lc) in enumerate(leaves):
-            r = out[b]
-            f = 2 ** lv
-            dx = [span[d] / (self.nbase[d] * self.nx[d] * (f if d < ndim else 1))
-                  for d in range(3)]
-            r["nx"] = self.nx
-            r["ng"] = self.ng
-            r["dx"] = dx
-            r["area"] = (dx[1] * dx[2], dx[0] * dx[2], dx[0] * dx[1])
-            r["vol"] = dx[0] * dx[1] * dx[2]
-            r["xmin"] = [float(xmin[d]) + (lc[d] * self.nx[d] * dx[d] if d < ndim else 0.0)
-                         for d in range(3)]
-            r["mesh_id"] = mesh_id
-            nbr = np.full(27, -1, dtype=np.int32)
-            for o in self.offsets():
-                nbr[(o[2] + 1) * 9 + (o[1] + 1) * 3 + (o[0] + 1)] = self.index.get(
-                    (lv, self.shift(lv, lc, o)), -1)
-            r["nbr"] = nbr
-            bc = [1] * 6  # outflow placeholder on faces with a same-level neighbour
-            for d in range(ndim):
-                for side in (0, 1):
-                    o = [0, 0, 0]
-                    o[d] = 1 if side else -1
-                    if nbr[(o[2] + 1) * 9 + (o[1] + 1) * 3 + (o[0] + 1)] < 0:
-                        bc[2 * d + side] = BC_AMR
-            r["bc"] = bc
-        self.blocks = out
+        self.blocks = self._native_blocks()
 
     # -- tree geometry
     def nlat(self, level, d):
@@ -188,21 +159,27 @@ class AmrMesh:
                 out.append(c)
         return out
 
-    def _check_balance(self):
-        for lv, lc in self.index:
-            for o in self.offsets():
-                p = self.shift(lv, lc, o)
-                if (lv, p) in self.index:
-                    continue
-                if (lv, p) in self.refined:
-                    for c in self.touching_children(p, o):
-                        if (lv + 1, c) in self.refined:
-                            raise ValueError(f"leaf {(lv, lc)} touches level {lv + 2} "
-                                             "(2:1 balance)")
-                    continue
-                if (lv - 1, self.parent(p)) not in self.index:
-                    raise ValueError(f"leaf {(lv, lc)} touches a level below {lv - 1} "
-                                     "(2:1 balance)")
+    def _native_blocks(self):
+        """The block table from the library's tree (aph_amr_block_table), which checks 2:1
+        balance; byte for byte the table of ``block_table_py``."""
+        vp = ctypes.c_void_p
+        n = len(self.level)
+        out = np.zeros(n, dtype=BLOCK_DTYPE)
+        nbase = np.array(self.nbase, dtype=np.int32)
+        nx = np.array(self.nx, dtype=np.int32)
+        xmin = np.array(self.xmin, dtype=np.float64)
+        xmax = np.array(self.xmax, dtype=np.float64)
+        level = np.ascontiguousarray(self.level, dtype=np.int32)
+        loc = np.ascontiguousarray(self.loc, dtype=np.int64)
+        err = ctypes.create_string_buffer(512)
+        rc = _lib.lib().aph_amr_block_table(
+            self.ndim, nbase.ctypes.data_as(vp), nx.ctypes.data_as(vp), self.ng,
+            xmin.ctypes.data_as(vp), xmax.ctypes.data_as(vp), self.mesh_id, n,
+            level.ctypes.data_as(vp), loc.ctypes.data_as(vp), out.ctypes.data_as(vp), err,
+            len(err))
+        if rc != _lib.APH_OK:
+            raise ValueError(err.value.decode())
+        return out
 
     def block_set(self, nscalars=0):
         return BlockSet(self.ndim, [self.blocks], nscalars=nscalars)
@@ -215,6 +192,57 @@ class TwoLevelMesh(AmrMesh):
     def __init__(self, ndim, nbase, block_nx, ng, xmin, xmax, refined, mesh_id=0):
         super().__init__(ndim, nbase, block_nx, ng, xmin, xmax,
                          [(0, tuple(r)) for r in refined], mesh_id=mesh_id, max_level=1)
+
+
+def block_table_py(mesh: AmrMesh):
+    """The block table of a mesh's leaves, stated in Python (the library builds the same
+    bytes, aph_amr_block_table): 2:1 balance checked; dx, areas, volume and xmin per
+    leaf; nbr to same-level leaves with periodic wrap; BC_AMR on faces without one."""
+    ndim = mesh.ndim
+    for lv, lc in mesh.index:
+        for o in mesh.offsets():
+            p = mesh.shift(lv, lc, o)
+            if (lv, p) in mesh.index:
+                continue
+            if (lv, p) in mesh.refined:
+                for c in mesh.touching_children(p, o):
+                    if (lv + 1, c) in mesh.refined:
+                        raise ValueError(f"leaf {(lv, lc)} touches level {lv + 2} "
+                                         "(2:1 balance)")
+                continue
+            if (lv - 1, mesh.parent(p)) not in mesh.index:
+                raise ValueError(f"leaf {(lv, lc)} touches a level below {lv - 1} "
+                                 "(2:1 balance)")
+    span = [float(mesh.xmax[d]) - float(mesh.xmin[d]) for d in range(3)]
+    out = np.zeros(len(mesh.level), dtype=BLOCK_DTYPE)
+    for b in range(len(mesh.level)):
+        lv, lc = int(mesh.level[b]), tuple(int(v) for v in mesh.loc[b])
+        r = out[b]
+        f = 2 ** lv
+        dx = [span[d] / (mesh.nbase[d] * mesh.nx[d] * (f if d < ndim else 1))
+              for d in range(3)]
+        r["nx"] = mesh.nx
+        r["ng"] = mesh.ng
+        r["dx"] = dx
+        r["area"] = (dx[1] * dx[2], dx[0] * dx[2], dx[0] * dx[1])
+        r["vol"] = dx[0] * dx[1] * dx[2]
+        r["xmin"] = [float(mesh.xmin[d]) + (lc[d] * mesh.nx[d] * dx[d] if d < ndim else 0.0)
+                     for d in range(3)]
+        r["mesh_id"] = mesh.mesh_id
+        nbr = np.full(27, -1, dtype=np.int32)
+        for o in mesh.offsets():
+            nbr[(o[2] + 1) * 9 + (o[1] + 1) * 3 + (o[0] + 1)] = mesh.index.get(
+                (lv, mesh.shift(lv, lc, o)), -1)
+        r["nbr"] = nbr
+        bc = [1] * 6  # outflow placeholder on faces with a same-level neighbour
+        for d in range(ndim):
+            for side in (0, 1):
+                o = [0, 0, 0]
+                o[d] = 1 if side else -1
+                if nbr[(o[2] + 1) * 9 + (o[1] + 1) * 3 + (o[0] + 1)] < 0:
+                    bc[2 * d + side] = BC_AMR
+        r["bc"] = bc
+    return out
 
 
 class _Refs:
@@ -572,26 +600,13 @@ class _Sampler:
         return val
 
 
-def regrid(mesh: AmrMesh, bs: BlockSet, cons, tags):
-    """One regrid from per-block tags (-1 / 0 / 1, as aph_refinement_tag returns them),
-    Parthenon's rules on a periodic tree:
-    * a leaf tagged 1 below ``mesh.max_level`` is refined;
-    * then every leaf that touches a leaf two levels finer is refined (2:1 balance);
-    * a sibling group whose 2^ndim leaves were all tagged -1 (and none was refined) is
-      merged into its parent, unless the parent would touch a leaf two levels finer;
-      groups are taken finest level first, then by position, so a merge on a finer level
-      can clear the way for one on a coarser level in the same regrid (a block loses at
-      most one level per regrid).
-    Returns (new_mesh, new_bs, new_cons): a kept leaf keeps its data, a merged parent
-    takes the mean of its children (dz, dy, dx order), a new child the minmod prolongation
-    of its parent's level (the neighbours a cell, a mean of children or a prolongation
-    from the level above, as the ghost exchange fills them).  Ghost cells are left to the next ghost fill.  Totals are conserved to
-    round-off and refining then merging returns the data."""
-    nd = mesh.ndim
+def regrid_tree_py(mesh: AmrMesh, tags):
+    """The refined nodes after one regrid (the rules of ``regrid``), stated in Python; the
+    library's aph_amr_regrid_tree gives the same tree."""
     tags = np.asarray(tags)
     ref = set(mesh.refined)
     leaves = [(int(mesh.level[b]), tuple(int(v) for v in mesh.loc[b]))
-              for b in range(bs.nblocks)]
+              for b in range(len(mesh.level))]
     for b, (lv, lc) in enumerate(leaves):
         if tags[b] == 1 and lv < mesh.max_level:
             ref.add((lv, lc))
@@ -636,6 +651,65 @@ def regrid(mesh: AmrMesh, bs: BlockSet, cons, tags):
         ref.discard(P)
         if touches_finer(P[0], P[1], ref):
             ref.add(P)
+    return ref
+
+
+def _ancestors(levels, locs, nd):
+    ref = set()
+    for lv, lc in zip(levels, locs):
+        c = tuple(int(v) for v in lc)
+        for L in range(int(lv) - 1, -1, -1):
+            c = tuple(c[d] // 2 if d < nd else 0 for d in range(3))
+            if (L, c) in ref:
+                break
+            ref.add((L, c))
+    return ref
+
+
+def regrid_tree(mesh: AmrMesh, tags):
+    """The refined nodes after one regrid, from the library's tree (aph_amr_regrid_tree)."""
+    vp = ctypes.c_void_p
+    L = _lib.lib()
+    n = len(mesh.level)
+    nbase = np.array(mesh.nbase, dtype=np.int32)
+    level = np.ascontiguousarray(mesh.level, dtype=np.int32)
+    loc = np.ascontiguousarray(mesh.loc, dtype=np.int64)
+    t = np.ascontiguousarray(np.asarray(tags), dtype=np.int32)
+    err = ctypes.create_string_buffer(512)
+    m = ctypes.c_int32(0)
+    args = lambda ol, oc, cap: (mesh.ndim, nbase.ctypes.data_as(vp), mesh.max_level, n,
+                                level.ctypes.data_as(vp), loc.ctypes.data_as(vp),
+                                t.ctypes.data_as(vp), ol, oc, ctypes.byref(m), cap, err,
+                                len(err))
+    rc = L.aph_amr_regrid_tree(*args(None, None, 0))
+    if rc != _lib.APH_OK:
+        raise ValueError(err.value.decode())
+    out_level = np.zeros(m.value, dtype=np.int32)
+    out_loc = np.zeros((m.value, 3), dtype=np.int64)
+    rc = L.aph_amr_regrid_tree(*args(out_level.ctypes.data_as(vp), out_loc.ctypes.data_as(vp),
+                                     m.value))
+    if rc != _lib.APH_OK:
+        raise ValueError(err.value.decode())
+    return _ancestors(out_level, out_loc, mesh.ndim)
+
+
+def regrid(mesh: AmrMesh, bs: BlockSet, cons, tags):
+    """One regrid from per-block tags (-1 / 0 / 1, as aph_refinement_tag returns them),
+    Parthenon's rules on a periodic tree:
+    * a leaf tagged 1 below ``mesh.max_level`` is refined;
+    * then every leaf that touches a leaf two levels finer is refined (2:1 balance);
+    * a sibling group whose 2^ndim leaves were all tagged -1 (and none was refined) is
+      merged into its parent, unless the parent would touch a leaf two levels finer;
+      groups are taken finest level first, then by position, so a merge on a finer level
+      can clear the way for one on a coarser level in the same regrid (a block loses at
+      most one level per regrid).
+    Returns (new_mesh, new_bs, new_cons): a kept leaf keeps its data, a merged parent
+    takes the mean of its children (dz, dy, dx order), a new child the minmod prolongation
+    of its parent's level (the neighbours a cell, a mean of children or a prolongation
+    from the level above, as the ghost exchange fills them).  Ghost cells are left to the next ghost fill.  Totals are conserved to
+    round-off and refining then merging returns the data."""
+    nd = mesh.ndim
+    ref = regrid_tree(mesh, tags)
     new = AmrMesh(nd, mesh.nbase[:nd], mesh.nx[:nd], mesh.ng, mesh.xmin, mesh.xmax, ref,
                   mesh_id=mesh.mesh_id, max_level=mesh.max_level)
     nbs = new.block_set(nscalars=bs.nscalars)

@@ -15,6 +15,8 @@
 // axes ascending with the first slowest -- the order the Python statement of the same
 // builder (apophis_amd/amr.py amr_plan) uses, so both give identical lists.
 // Host code only; no device work.
+#include <algorithm>
+#include <array>
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
@@ -288,6 +290,294 @@ extern "C" int aph_amr_build(const aph_block *blocks, int32_t nblocks, int32_t n
     if (fixes && !f.empty()) std::memcpy(fixes, f.data(), sizeof(aph_flux_fix) * f.size());
     if (refs && !t.refs.empty())
       std::memcpy(refs, t.refs.data(), sizeof(aph_amr_ref) * t.refs.size());
+  } catch (const BuildError &e) {
+    return report(APH_E_INVALID, e.msg);
+  } catch (const std::bad_alloc &) {
+    return report(APH_E_NOMEM, "host allocation failed");
+  }
+  if (err && err_len) err[0] = 0;
+  return APH_OK;
+}
+
+// ------------------------------------------------------------------ the tree itself
+// Leaves are (level, logical position); the refined nodes are their ancestors.  Every axis
+// is periodic.  Children are ordered (cz, cy, cx) with x fastest, and the depth-first leaf
+// order walks the base lattice z, y, x -- the order of apophis_amd/amr.py AmrMesh.
+namespace {
+
+struct Forest {
+  int nd = 0;
+  int64_t nbase[3] = {1, 1, 1};
+  std::unordered_set<uint64_t> refined;
+
+  int64_t nlat(int lv, int d) const { return d < nd ? nbase[d] << lv : 1; }
+  uint64_t k(int lv, const int64_t *c) const { return key(lv, c[0], c[1], c[2]); }
+  bool is_ref(int lv, const int64_t *c) const { return refined.count(k(lv, c)) != 0; }
+  void shift(int lv, const int64_t *c, const int *o, int64_t *out) const {
+    for (int d = 0; d < 3; ++d) {
+      const int64_t n = nlat(lv, d);
+      out[d] = ((c[d] + o[d]) % n + n) % n;
+    }
+  }
+  int nchildren() const { return nd == 3 ? 8 : 4; }
+  void child(const int64_t *c, int i, int64_t *out) const {  // i in (cz, cy, cx) order
+    const int cx = i & 1, cy = (i >> 1) & 1, cz = (i >> 2) & 1;
+    out[0] = 2 * c[0] + cx;
+    out[1] = 2 * c[1] + cy;
+    out[2] = nd == 3 ? 2 * c[2] + cz : 0;
+  }
+  template <class F>
+  void offsets(F f) const {
+    for (int oz = (nd == 3 ? -1 : 0); oz <= (nd == 3 ? 1 : 0); ++oz)
+      for (int oy = -1; oy <= 1; ++oy)
+        for (int ox = -1; ox <= 1; ++ox) {
+          if (!ox && !oy && !oz) continue;
+          const int o[3] = {ox, oy, oz};
+          f(o);
+        }
+  }
+  // whether leaf (lv, c) touches a leaf at level lv + 2 or deeper
+  bool touches_finer(int lv, const int64_t *c) const {
+    bool hit = false;
+    offsets([&](const int *o) {
+      if (hit) return;
+      int64_t p[3];
+      shift(lv, c, o, p);
+      if (!is_ref(lv, p)) return;
+      for (int i = 0; i < nchildren() && !hit; ++i) {
+        int64_t ch[3];
+        child(p, i, ch);
+        bool touches = true;  // the child of p on the side facing (lv, c)
+        for (int d = 0; d < nd; ++d)
+          if (o[d] != 0 && (ch[d] & 1) != (o[d] < 0 ? 1 : 0)) touches = false;
+        if (touches && is_ref(lv + 1, ch)) hit = true;
+      }
+    });
+    return hit;
+  }
+  template <class F>
+  void leaves(F f) const {  // depth-first, base lattice z, y, x
+    std::vector<std::pair<int, std::array<int64_t, 3>>> stack;
+    for (int64_t z = nbase[2] - 1; z >= 0; --z)
+      for (int64_t y = nbase[1] - 1; y >= 0; --y)
+        for (int64_t x = nbase[0] - 1; x >= 0; --x) stack.push_back({0, {x, y, z}});
+    while (!stack.empty()) {
+      auto [lv, c] = stack.back();
+      stack.pop_back();
+      if (is_ref(lv, c.data())) {
+        for (int i = nchildren() - 1; i >= 0; --i) {
+          std::array<int64_t, 3> ch;
+          child(c.data(), i, ch.data());
+          stack.push_back({lv + 1, ch});
+        }
+      } else {
+        f(lv, c.data());
+      }
+    }
+  }
+};
+
+void forest_setup(Forest &t, int32_t ndim, const int32_t nbase[3], int32_t n, const int32_t *level,
+                  const int64_t *loc) {
+  if (ndim < 2 || ndim > 3 || !nbase || n < 1 || !level || !loc) fail("bad arguments");
+  t.nd = ndim;
+  for (int d = 0; d < 3; ++d) {
+    t.nbase[d] = d < ndim ? nbase[d] : 1;
+    if (t.nbase[d] < 1) fail("nbase[%d] = %d", d, nbase[d]);
+  }
+  std::unordered_set<uint64_t> leafset;
+  for (int b = 0; b < n; ++b) {
+    if (level[b] < 0 || level[b] > kMaxLevel) fail("leaf %d: level %d", b, level[b]);
+    int64_t c[3];
+    for (int d = 0; d < 3; ++d) {
+      c[d] = d < ndim ? loc[3 * b + d] : 0;
+      if (c[d] < 0 || c[d] >= t.nlat(level[b], d) || c[d] >= kMaxCoord)
+        fail("leaf %d: logical position outside level %d", b, level[b]);
+    }
+    if (!leafset.insert(t.k(level[b], c)).second) fail("leaf %d: two leaves at one node", b);
+  }
+  for (int b = 0; b < n; ++b) {
+    int64_t c[3] = {loc[3 * b], loc[3 * b + 1], ndim == 3 ? loc[3 * b + 2] : 0};
+    for (int lv = level[b] - 1; lv >= 0; --lv) {
+      for (int d = 0; d < ndim; ++d) c[d] >>= 1;
+      const uint64_t kk = t.k(lv, c);
+      if (leafset.count(kk)) fail("leaf %d lies inside another leaf", b);
+      if (!t.refined.insert(kk).second) break;
+    }
+  }
+  int64_t nroot = t.nbase[0] * t.nbase[1] * t.nbase[2];
+  if ((int64_t)n != nroot + (int64_t)(t.nchildren() - 1) * (int64_t)t.refined.size())
+    fail("the leaves do not cover the domain");
+}
+
+int emit_leaves(const Forest &t, int32_t *out_level, int64_t *out_loc, int32_t *out_n,
+                int32_t cap) {
+  int32_t m = 0;
+  t.leaves([&](int lv, const int64_t *c) {
+    if (out_level && m < cap) {
+      out_level[m] = lv;
+      for (int d = 0; d < 3; ++d) out_loc[3 * m + d] = c[d];
+    }
+    ++m;
+  });
+  *out_n = m;
+  return (out_level && m > cap) ? -1 : 0;
+}
+
+}  // namespace
+
+extern "C" int aph_amr_regrid_tree(int32_t ndim, const int32_t nbase[3], int32_t max_level,
+                                   int32_t n, const int32_t *level, const int64_t *loc,
+                                   const int32_t *tags, int32_t *out_level, int64_t *out_loc,
+                                   int32_t *out_n, int32_t cap, char *err, size_t err_len) {
+  auto report = [&](int code, const char *msg) {
+    if (err && err_len) snprintf(err, err_len, "aph_amr_regrid_tree: %s", msg);
+    return code;
+  };
+  if (!out_n || !tags || (out_level && !out_loc)) return report(APH_E_INVALID, "bad arguments");
+  try {
+    Forest t;
+    forest_setup(t, ndim, nbase, n, level, loc);
+    std::unordered_map<uint64_t, int> old_leaf;
+    for (int b = 0; b < n; ++b) {
+      const int64_t c[3] = {loc[3 * b], loc[3 * b + 1], ndim == 3 ? loc[3 * b + 2] : 0};
+      old_leaf.emplace(t.k(level[b], c), b);
+    }
+    // refine the tagged leaves below max_level
+    for (int b = 0; b < n; ++b)
+      if (tags[b] == 1 && level[b] < max_level) {
+        const int64_t c[3] = {loc[3 * b], loc[3 * b + 1], ndim == 3 ? loc[3 * b + 2] : 0};
+        t.refined.insert(t.k(level[b], c));
+      }
+    // 2:1 balance: refine every leaf that touches a leaf two levels finer (a monotone
+    // closure, so the order of the sweep does not change the result)
+    for (bool changed = true; changed;) {
+      changed = false;
+      std::vector<std::pair<int, std::array<int64_t, 3>>> add;
+      t.leaves([&](int lv, const int64_t *c) {
+        if (t.touches_finer(lv, c)) add.push_back({lv, {c[0], c[1], c[2]}});
+      });
+      for (auto &a : add) changed |= t.refined.insert(t.k(a.first, a.second.data())).second;
+    }
+    // merges: parents of leaves tagged -1, finest level first, then by position (x, y, z)
+    std::vector<std::pair<int, std::array<int64_t, 3>>> groups;
+    {
+      std::unordered_set<uint64_t> seen;
+      for (int b = 0; b < n; ++b)
+        if (level[b] > 0 && tags[b] == -1) {
+          std::array<int64_t, 3> p = {loc[3 * b] >> 1, loc[3 * b + 1] >> 1,
+                                      ndim == 3 ? loc[3 * b + 2] >> 1 : 0};
+          if (seen.insert(t.k(level[b] - 1, p.data())).second) groups.push_back({level[b] - 1, p});
+        }
+    }
+    std::sort(groups.begin(), groups.end(), [](const auto &a, const auto &b) {
+      if (a.first != b.first) return a.first > b.first;
+      return a.second < b.second;
+    });
+    for (const auto &P : groups) {
+      const int lv = P.first + 1;
+      bool ok = true;
+      for (int i = 0; i < t.nchildren() && ok; ++i) {
+        int64_t ch[3];
+        t.child(P.second.data(), i, ch);
+        auto it = old_leaf.find(t.k(lv, ch));
+        if (it == old_leaf.end() || tags[it->second] != -1 || t.is_ref(lv, ch)) ok = false;
+      }
+      if (!ok) continue;
+      const uint64_t pk = t.k(P.first, P.second.data());
+      t.refined.erase(pk);
+      if (t.touches_finer(P.first, P.second.data())) t.refined.insert(pk);
+    }
+    if (emit_leaves(t, out_level, out_loc, out_n, cap))
+      return report(APH_E_INVALID, "output arrays too short");
+  } catch (const BuildError &e) {
+    return report(APH_E_INVALID, e.msg);
+  } catch (const std::bad_alloc &) {
+    return report(APH_E_NOMEM, "host allocation failed");
+  }
+  if (err && err_len) err[0] = 0;
+  return APH_OK;
+}
+
+extern "C" int aph_amr_block_table(int32_t ndim, const int32_t nbase[3], const int32_t nx[3],
+                                   int32_t ng, const double xmin[3], const double xmax[3],
+                                   int32_t mesh_id, int32_t n, const int32_t *level,
+                                   const int64_t *loc, aph_block *out, char *err,
+                                   size_t err_len) {
+  auto report = [&](int code, const char *msg) {
+    if (err && err_len) snprintf(err, err_len, "aph_amr_block_table: %s", msg);
+    return code;
+  };
+  if (!nx || !xmin || !xmax || !out) return report(APH_E_INVALID, "bad arguments");
+  try {
+    Forest t;
+    forest_setup(t, ndim, nbase, n, level, loc);
+    std::unordered_map<uint64_t, int> leaf;
+    for (int b = 0; b < n; ++b) {
+      const int64_t c[3] = {loc[3 * b], loc[3 * b + 1], ndim == 3 ? loc[3 * b + 2] : 0};
+      leaf.emplace(t.k(level[b], c), b);
+    }
+    for (int d = 0; d < ndim; ++d)
+      if (nx[d] < 2 * ng || nx[d] % 2) fail("nx[%d] must be even and at least 2*ng", d);
+    for (int b = 0; b < n; ++b) {
+      const int lv = level[b];
+      const int64_t c[3] = {loc[3 * b], loc[3 * b + 1], ndim == 3 ? loc[3 * b + 2] : 0};
+      // 2:1 balance: every neighbour node is a leaf, a refined node whose children facing
+      // this leaf are leaves, or lies inside a leaf one level coarser
+      t.offsets([&](const int *o) {
+        int64_t p[3];
+        t.shift(lv, c, o, p);
+        if (leaf.count(t.k(lv, p))) return;
+        if (t.is_ref(lv, p)) {
+          for (int i = 0; i < t.nchildren(); ++i) {
+            int64_t ch[3];
+            t.child(p, i, ch);
+            bool touches = true;
+            for (int d = 0; d < ndim; ++d)
+              if (o[d] != 0 && (ch[d] & 1) != (o[d] < 0 ? 1 : 0)) touches = false;
+            if (touches && t.is_ref(lv + 1, ch))
+              fail("leaf %d touches level %d (2:1 balance)", b, lv + 2);
+          }
+          return;
+        }
+        int64_t q[3] = {p[0] >> 1, p[1] >> 1, ndim == 3 ? p[2] >> 1 : 0};
+        if (lv == 0 || !leaf.count(t.k(lv - 1, q)))
+          fail("leaf %d touches a level below %d (2:1 balance)", b, lv - 1);
+      });
+      aph_block &r = out[b];
+      std::memset(&r, 0, sizeof(r));
+      double dx[3];
+      for (int d = 0; d < 3; ++d) {
+        const int64_t nxd = d < ndim ? nx[d] : 1;
+        const int64_t cells = t.nbase[d] * nxd * (d < ndim ? (int64_t)1 << lv : 1);
+        dx[d] = (xmax[d] - xmin[d]) / (double)cells;
+        r.nx[d] = (int32_t)nxd;
+      }
+      r.ng = ng;
+      r.cell_base = 0;
+      for (int d = 0; d < 3; ++d) r.dx[d] = dx[d];
+      r.area[0] = dx[1] * dx[2];
+      r.area[1] = dx[0] * dx[2];
+      r.area[2] = dx[0] * dx[1];
+      r.vol = dx[0] * dx[1] * dx[2];
+      for (int d = 0; d < 3; ++d)
+        r.xmin[d] = xmin[d] + (d < ndim ? (double)(c[d] * (int64_t)nx[d]) * dx[d] : 0.0);
+      r.mesh_id = mesh_id;
+      for (int q = 0; q < 27; ++q) r.nbr[q] = -1;
+      t.offsets([&](const int *o) {
+        int64_t p[3];
+        t.shift(lv, c, o, p);
+        auto it = leaf.find(t.k(lv, p));
+        r.nbr[(o[2] + 1) * 9 + (o[1] + 1) * 3 + (o[0] + 1)] = it == leaf.end() ? -1 : it->second;
+      });
+      for (int f = 0; f < 6; ++f) r.bc[f] = APH_BC_OUTFLOW;  // faces with a same-level neighbour
+      for (int d = 0; d < ndim; ++d)
+        for (int side = 0; side < 2; ++side) {
+          int o[3] = {0, 0, 0};
+          o[d] = side ? 1 : -1;
+          if (r.nbr[(o[2] + 1) * 9 + (o[1] + 1) * 3 + (o[0] + 1)] < 0) r.bc[2 * d + side] = APH_BC_AMR;
+        }
+    }
   } catch (const BuildError &e) {
     return report(APH_E_INVALID, e.msg);
   } catch (const std::bad_alloc &) {

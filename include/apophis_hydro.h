@@ -183,8 +183,9 @@ int aph_refinement_tag(aph_context *ctx, const aph_blockset *bs, int32_t criteri
 /* ------------------------------------------- multilevel meshes (plain form)
  * Blocks of several refinement levels in one mesh, 2:1 balanced (leaves that touch differ
  * by at most one level).  nbr[] links blocks of the same level only; a face whose
- * neighbour region belongs to another level has nbr = -1 and bc = APH_BC_AMR.  The ghost cells there, and the coarse fluxes on faces shared
- * with finer blocks, come from a plan the host builds from the mesh tree.  This replaces
+ * neighbour region belongs to another level has nbr = -1 and bc = APH_BC_AMR.  The ghost
+ * cells there, and the coarse fluxes on faces shared with finer blocks, come from a plan
+ * the host builds from the mesh tree.  This replaces
  * Parthenon's multilevel boundary exchange (RestrictAverage / ProlongateSharedMinMod
  * for the cons field) and its flux correction (LoadAndSendFluxCorrections /
  * SetFluxCorrections), the tasks at apophis_driver.cpp:68,74-76 and :94.
@@ -245,6 +246,27 @@ int aph_amr_plan_destroy(aph_context *ctx, aph_amr_plan *plan);
    On input *nops, *nfixes, *nrefs are the capacities of the arrays (an array may be NULL:
    then only its length is returned); on output the lengths.  On failure the message goes
    to err (err_len bytes, may be NULL). */
+/* The mesh tree (host only, like aph_amr_build).  A periodic tree is given by its leaves:
+   level[b] and loc[3*b .. 3*b+2] in a base lattice of nbase[0..ndim-1] nodes; the leaves
+   must cover the domain exactly once.
+   aph_amr_regrid_tree: one regrid from per-leaf tags (-1 / 0 / 1), Parthenon's rules --
+   leaves tagged 1 below max_level split; every leaf touching a leaf two levels finer splits
+   (2:1 balance); sibling groups all tagged -1 (none split) merge, finest level first, then
+   by position (x, then y, then z), unless the parent would touch a leaf two levels finer.
+   Writes the new leaves in depth-first order (base lattice z, y, x; children z, y, x) to
+   out_level / out_loc (cap entries; NULL: only *out_n is set).
+   aph_amr_block_table: the aph_block table of the leaves (in their order) after checking
+   2:1 balance: extents, ghosts, dx = (xmax - xmin) / (nbase nx 2^level) per active axis,
+   areas, volume, xmin, nbr to same-level leaves (periodic), bc = APH_BC_AMR on faces
+   without one (APH_BC_OUTFLOW elsewhere), cell_base 0 (the block set fills it). */
+int aph_amr_regrid_tree(int32_t ndim, const int32_t nbase[3], int32_t max_level, int32_t n,
+                        const int32_t *level, const int64_t *loc, const int32_t *tags,
+                        int32_t *out_level, int64_t *out_loc, int32_t *out_n, int32_t cap,
+                        char *err, size_t err_len);
+int aph_amr_block_table(int32_t ndim, const int32_t nbase[3], const int32_t nx[3], int32_t ng,
+                        const double xmin[3], const double xmax[3], int32_t mesh_id, int32_t n,
+                        const int32_t *level, const int64_t *loc, aph_block *out, char *err,
+                        size_t err_len);
 int aph_amr_build(const aph_block *blocks, int32_t nblocks, int32_t ndim, int32_t nvar,
                   const int32_t nbase[3], const int32_t *level, const int64_t *loc,
                   aph_amr_op *ops, int64_t *nops, aph_flux_fix *fixes, int64_t *nfixes,

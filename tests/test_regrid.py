@@ -192,3 +192,34 @@ def test_three_level_refine_merge_round_trip_and_linear_exactness(nd):
     assert m3.refined == m.refined
     mask = np.tile(bs.interior_mask(), bs.nvar)
     np.testing.assert_allclose(c3[mask], noisy[mask], rtol=2e-15, atol=0)
+
+
+def _random_meshes():
+    out = [_three_level(2), _three_level(3)]
+    out.append(amr.TwoLevelMesh(3, (3, 3, 2), (6, 6, 6), 2, (0, 0, 0), (1, 1, 1), [(1, 1, 0)]))
+    out.append(amr.AmrMesh(2, (5, 3), (8, 8), 2, (-0.5, -1.0, 0.0), (0.5, 2.0, 1.0),
+                           [(0, (x, y)) for x in (1, 2) for y in (0, 1, 2)] + [(1, (3, 2))],
+                           max_level=3))
+    return out
+
+
+def test_native_block_table_equals_python_statement():
+    for m in _random_meshes():
+        assert m.blocks.tobytes() == amr.block_table_py(m).tobytes()
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_native_regrid_tree_equals_python_statement(seed):
+    """aph_amr_regrid_tree against regrid_tree_py on random tags, then on the result again
+    (refines, balance refinements and merges across levels)."""
+    rng = np.random.default_rng(seed)
+    for m in _random_meshes():
+        for _ in range(3):
+            tags = rng.choice([-1, 0, 1], size=len(m.level), p=[0.45, 0.3, 0.25])
+            tags = tags.astype(np.int32)
+            want = amr.regrid_tree_py(m, tags)
+            got = amr.regrid_tree(m, tags)
+            assert got == want
+            m = amr.AmrMesh(m.ndim, m.nbase[:m.ndim], m.nx[:m.ndim], m.ng, m.xmin, m.xmax, got,
+                            mesh_id=m.mesh_id, max_level=m.max_level)
+            assert m.blocks.tobytes() == amr.block_table_py(m).tobytes()

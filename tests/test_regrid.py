@@ -223,3 +223,25 @@ def test_native_regrid_tree_equals_python_statement(seed):
             m = amr.AmrMesh(m.ndim, m.nbase[:m.ndim], m.nx[:m.ndim], m.ng, m.xmin, m.xmax, got,
                             mesh_id=m.mesh_id, max_level=m.max_level)
             assert m.blocks.tobytes() == amr.block_table_py(m).tobytes()
+
+
+def test_native_tree_refuses_bad_leaf_sets():
+    """aph_amr_regrid_tree / aph_amr_block_table refuse leaf sets that leave a hole, nest a
+    leaf in another or duplicate one, and unbalanced trees (host only, no GPU)."""
+    m = _three_level()
+    tags = np.zeros(len(m.level), np.int32)
+    bad = amr.AmrMesh.__new__(amr.AmrMesh)
+    bad.__dict__.update(m.__dict__)
+    bad.level, bad.loc = m.level[1:].copy(), m.loc[1:].copy()      # a hole
+    with pytest.raises(ValueError, match="cover"):
+        amr.regrid_tree(bad, tags[1:])
+    with pytest.raises(ValueError, match="cover"):
+        bad._native_blocks()
+    bad.level, bad.loc = m.level.copy(), m.loc.copy()
+    i2 = int(np.flatnonzero(m.level == 2)[0])
+    bad.loc[i2 + 1] = bad.loc[i2]                                  # a duplicate
+    with pytest.raises(ValueError, match="two leaves"):
+        amr.regrid_tree(bad, tags)
+    bad.level, bad.loc = np.append(m.level, 0), np.vstack([m.loc, [[1, 1, 0]]])
+    with pytest.raises(ValueError, match="inside another"):          # nested in a leaf
+        amr.regrid_tree(bad, np.zeros(len(bad.level), np.int32))

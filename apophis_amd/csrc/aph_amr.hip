@@ -51,7 +51,9 @@ __device__ __forceinline__ double sgn1(double x) { return x < 0.0 ? -1.0 : 1.0; 
 template <int NV>
 __global__ void __launch_bounds__(kThreads)
     k_amr_fill(const aph_amr_op *__restrict__ ops, int64_t nops,
-               const aph_amr_ref *__restrict__ refs, int ndim, double *u) {
+               const aph_amr_ref *__restrict__ refs, int ndim, double *u, unsigned *epoch) {
+  // the fused form's launch epoch rides on this launch (one launch fewer per stage)
+  if (epoch && blockIdx.x == 0 && threadIdx.x == 0) *epoch = *epoch + 1u;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nops;
        t += (int64_t)gridDim.x * blockDim.x) {
     const aph_amr_op op = ops[t];
@@ -207,6 +209,11 @@ __global__ void __launch_bounds__(kThreads)
 }
 
 }  // namespace
+
+// The cross-level fill of cons; when epoch is non-null the same launch bumps the fused
+// form's launch epoch (it is launched even without ops then).
+int amr_fill_epoch(aph_context *ctx, const aph_amr_plan *p, double *cons, unsigned *epoch);
+
 }  // namespace aph
 
 using namespace aph;
@@ -340,12 +347,7 @@ int aph_amr_plan_destroy(aph_context *ctx, aph_amr_plan *p) {
 int aph_amr_fill(aph_context *ctx, const aph_amr_plan *p, double *cons) {
   if (!ctx) return APH_E_INVALID;
   if (!p || !cons) return set_error(ctx, APH_E_INVALID, "aph_amr_fill: null");
-  if (!p->nops) return APH_OK;
-  const unsigned grid = (unsigned)std::min<int64_t>((p->nops + kThreads - 1) / kThreads, 4096);
-  APH_AMR_NV(p->nvar, (k_amr_fill<NV><<<grid, kThreads, 0, ctx->stream>>>(
-                          p->ops, p->nops, p->refs, p->ndim, cons)));
-  APH_LAUNCH_CHECK(ctx);
-  return APH_OK;
+  return aph::amr_fill_epoch(ctx, p, cons, nullptr);
 }
 
 int aph_amr_flux_correct(aph_context *ctx, const aph_amr_plan *p, double *f1, double *f2,
@@ -362,3 +364,17 @@ int aph_amr_flux_correct(aph_context *ctx, const aph_amr_plan *p, double *f1, do
 }
 
 }  // extern "C"
+
+namespace aph {
+
+int amr_fill_epoch(aph_context *ctx, const aph_amr_plan *p, double *cons, unsigned *epoch) {
+  if (!p->nops && !epoch) return APH_OK;
+  const unsigned grid =
+      (unsigned)std::max<int64_t>(1, std::min<int64_t>((p->nops + kThreads - 1) / kThreads, 4096));
+  APH_AMR_NV(p->nvar, (k_amr_fill<NV><<<grid, kThreads, 0, ctx->stream>>>(
+                          p->ops, p->nops, p->refs, p->ndim, cons, epoch)));
+  APH_LAUNCH_CHECK(ctx);
+  return APH_OK;
+}
+
+}  // namespace aph

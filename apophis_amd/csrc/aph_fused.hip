@@ -29,12 +29,12 @@
 
 namespace aph {
 
+// aph_amr.hip: the cross-level fill, bumping the multilevel launch epoch in the same launch
+int amr_fill_epoch(aph_context *ctx, const aph_amr_plan *p, double *cons, unsigned *epoch);
+
 namespace {
 
 __global__ void k_fill(double *p, double v) { *p = v; }
-
-// launch epoch of a multilevel stage (the interface counts are compared against it)
-__global__ void k_bump(unsigned *epoch) { *epoch = *epoch + 1u; }
 
 // A multilevel stage whose cross-level wait ran out (its fine tiles were not all
 // resident): the result is wrong, so make it visibly so.
@@ -377,9 +377,8 @@ int launch_kernel(aph_context *ctx, aph_plan *p, int recon, int stage, int nst, 
   // multilevel mesh: cross-level ghosts of the stage input, then one launch whose items
   // run the finest level first (each tile records the fluxes on its faces to a coarser
   // leaf; a tile with a face to a finer region waits for them, then takes their mean)
-  int rc = aph_amr_fill(ctx, p->amr, u_in);
+  int rc = amr_fill_epoch(ctx, p->amr, u_in, p->d_epoch);  // also bumps the epoch
   if (rc) return rc;
-  k_bump<<<1, 1, 0, ctx->stream>>>(p->d_epoch);
   hipLaunchKernelGGL(fn, dim3(p->grid), dim3(64), p->lds, ctx->stream, a);
   APH_LAUNCH_CHECK(ctx);
   k_amr_check<<<1, 1, 0, ctx->stream>>>(p->d_err, u_out);

@@ -120,7 +120,8 @@ def adjoint_leg(ctx, hydro, problems, torch):
     """Kelvin-Helmholtz 512^2 (16x16 blocks of 32^2, PLM+HLLC, one passive scalar): time
     of a frozen-dt forward RK2 step vs one adjoint step (both stage adjoints, given the
     step's input and stage-1 states, as the rollout's backward pass runs it; also without
-    the stage-1 state, which is then recomputed), and of a whole 200-step rollout with
+    the stage-1 state, which is then recomputed), and (after an untimed one-segment
+    warm-up) of a whole 200-step rollout with
     checkpoints every 20 steps, forward and backward (segment recomputation included)."""
     from apophis_amd import adjoint
 
@@ -157,6 +158,11 @@ def adjoint_leg(ctx, hydro, problems, torch):
         st.adjoint_step_(states[k], lam)
     torch.cuda.synchronize()
     t_adj_rc = (time.perf_counter() - t0) / n
+    # warm-up: one checkpoint segment forward and back (the allocator's blocks for the
+    # segment's states), untimed
+    amps_w = torch.from_numpy(p.meta["amps"]).cuda().requires_grad_(True)
+    adjoint.scalar_variance(adjoint.rollout(adjoint.kh_state(p.bs, amps_w), st, 20, 20),
+                            p.bs).backward()
     amps = torch.from_numpy(p.meta["amps"]).cuda().requires_grad_(True)
     torch.cuda.synchronize()
     u0 = adjoint.kh_state(p.bs, amps)
@@ -182,8 +188,9 @@ def adjoint_leg(ctx, hydro, problems, torch):
 def rt_batch_leg(ctx, hydro, problems, torch, nsteps=300, every=20):
     """Batched Rayleigh-Taylor with adjoint: 64 independent instances (grids drawn from
     128x384 / 256x768 / 512x1536, blocks of 32^2), PPM+HLLC, gravity, reflecting walls,
-    all instances in one fused launch per stage.  A 300-step rollout at the frozen
-    initial CFL dt with checkpoints every 20 steps, then the gradient of the summed
+    all instances in one fused launch per stage.  After an untimed one-segment warm-up,
+    a 300-step rollout at the frozen initial CFL dt with checkpoints every 20 steps, then
+    the gradient of the summed
     mixing-layer width with respect to the 64x6 mode amplitudes."""
     from apophis_amd import adjoint
 
@@ -193,6 +200,12 @@ def rt_batch_leg(ctx, hydro, problems, torch, nsteps=300, every=20):
     dt = float(fd.initial_dt().item())
     fd.autotune(reps=2)
     st = adjoint.Stepper(fd, dt)
+    # warm-up: one checkpoint segment forward and back (the adjoint's scratch and the
+    # allocator's blocks for the segment's states), untimed
+    a_w = torch.from_numpy(p.meta["amps"]).cuda().requires_grad_(True)
+    adjoint.mixing_width(adjoint.rollout(adjoint.rt_state(p.bs, a_w), st, every, every),
+                         p.bs).backward()
+    del a_w
     a = torch.from_numpy(p.meta["amps"]).cuda().requires_grad_(True)
     u0 = adjoint.rt_state(p.bs, a)
     torch.cuda.synchronize()

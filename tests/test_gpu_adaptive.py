@@ -93,6 +93,19 @@ def test_plain_adaptive_cycles_match_oracle(gpu_ctx, reference_run):
     drv, got = _run(gpu_ctx, "plain", reference_run)
     ref = reference_run[-1]
     assert drv.stats["regrids"] >= 2
+    # mass and energy totals through refinements and merges (periodic: conserved)
+    def totals(bs, u):
+        t = np.zeros(5)
+        for b in range(bs.nblocks):
+            t += bs.interior(u, b, bs.nvar).reshape(bs.nvar, -1)[:5].sum(axis=1) * \
+                bs.blocks["vol"][b]
+        return t
+    pkg, m, bs, cons, _ = reference_run
+    t0 = totals(bs, cons)
+    for _, _, b_, u_ in got:
+        t = totals(b_, u_)
+        for v in (0, 4):
+            assert abs(t[v] - t0[v]) <= 1e-12 * abs(t0[v]), (v, t[v] - t0[v])
     levels = set()
     for c, ((dt, refd, bs, u), (rdt, rrefd, rbs, ru)) in enumerate(zip(got, ref)):
         assert dt == rdt, c

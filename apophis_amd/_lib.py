@@ -60,6 +60,8 @@ _SIGS = {
                                    ctypes.c_char_p, ctypes.c_size_t]),
     "aph_amr_block_table": (_i32, [_i32, _vp, _vp, _i32, _vp, _vp, _i32, _i32, _vp, _vp, _vp,
                                    ctypes.c_char_p, ctypes.c_size_t]),
+    "aph_amr_remap": (_i32, [_i32, _vp, _vp, _i32, _i32, _i32, _vp, _vp, _vp, _i32, _vp, _vp,
+                             _vp, ctypes.c_char_p, ctypes.c_size_t]),
     "aph_amr_fill": (_i32, [_vp, _vp, _vp]),
     "aph_amr_flux_correct": (_i32, [_vp, _vp, _vp, _vp, _vp]),
     "aph_plan_set_amr": (_i32, [_vp, _vp, _vp, _vp, _vp, _i64]),

@@ -713,6 +713,35 @@ def regrid(mesh: AmrMesh, bs: BlockSet, cons, tags):
     new = AmrMesh(nd, mesh.nbase[:nd], mesh.nx[:nd], mesh.ng, mesh.xmin, mesh.xmax, ref,
                   mesh_id=mesh.mesh_id, max_level=mesh.max_level)
     nbs = new.block_set(nscalars=bs.nscalars)
+    return new, nbs, remap(mesh, bs, cons, new, nbs)
+
+
+def remap(mesh: AmrMesh, bs: BlockSet, cons, new: AmrMesh, nbs: BlockSet):
+    """The interiors of ``new`` (block set ``nbs``) from the state ``cons`` of ``mesh``,
+    by the library (aph_amr_remap): the bits of ``remap_py``."""
+    vp = ctypes.c_void_p
+    out = np.zeros(nbs.field_size(bs.nvar))
+    u = np.ascontiguousarray(cons, dtype=np.float64)
+    args = []
+    for m_ in (mesh, new):
+        args.append((np.ascontiguousarray(m_.level, dtype=np.int32),
+                     np.ascontiguousarray(m_.loc, dtype=np.int64)))
+    nbase = np.array(mesh.nbase, dtype=np.int32)
+    nx = np.array(mesh.nx, dtype=np.int32)
+    err = ctypes.create_string_buffer(512)
+    rc = _lib.lib().aph_amr_remap(
+        mesh.ndim, nbase.ctypes.data_as(vp), nx.ctypes.data_as(vp), mesh.ng, bs.nvar,
+        len(mesh.level), args[0][0].ctypes.data_as(vp), args[0][1].ctypes.data_as(vp),
+        u.ctypes.data_as(vp), len(new.level), args[1][0].ctypes.data_as(vp),
+        args[1][1].ctypes.data_as(vp), out.ctypes.data_as(vp), err, len(err))
+    if rc != _lib.APH_OK:
+        raise ValueError(err.value.decode())
+    return out
+
+
+def remap_py(mesh: AmrMesh, bs: BlockSet, cons, new: AmrMesh, nbs: BlockSet):
+    """``remap`` stated in Python with the level sampler."""
+    nd = mesh.ndim
     nv, nx = bs.nvar, mesh.nx
     old = _Sampler(mesh, bs, cons)
     out = np.zeros(nbs.field_size(nv))
@@ -734,4 +763,4 @@ def regrid(mesh: AmrMesh, bs: BlockSet, cons, tags):
             sl = tuple(slice(lc[d] * nx[d] - g0[d], (lc[d] + 1) * nx[d] - g0[d])
                        for d in (2, 1, 0))
             nbs.interior(out, b, nv)[:] = v[(slice(None),) + sl]
-    return new, nbs, out
+    return out

@@ -17,6 +17,7 @@
 // Host code only; no device work.
 #include <algorithm>
 #include <array>
+#include <cmath>
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
@@ -577,6 +578,207 @@ extern "C" int aph_amr_block_table(int32_t ndim, const int32_t nbase[3], const i
           o[d] = side ? 1 : -1;
           if (r.nbr[(o[2] + 1) * 9 + (o[1] + 1) * 3 + (o[0] + 1)] < 0) r.bc[2 * d + side] = APH_BC_AMR;
         }
+    }
+  } catch (const BuildError &e) {
+    return report(APH_E_INVALID, e.msg);
+  } catch (const std::bad_alloc &) {
+    return report(APH_E_NOMEM, "host allocation failed");
+  }
+  if (err && err_len) err[0] = 0;
+  return APH_OK;
+}
+
+// ------------------------------------------------------------------ interior remap
+// The interiors of a regridded mesh from the old one: a kept leaf copies its cells, any
+// other new leaf takes its box of the old mesh's level values -- a leaf's cells, the mean
+// of a refined node's children ((dz, dy, dx) order, scaled by 2^-d), or where a coarser
+// leaf covers the box, c + sum_d (sign_d / 4) minmod(c - c_-d, c_+d - c) of the level above
+// (the ghost exchange's rule), axes in order -- the same operations in the same order as
+// apophis_amd/amr.py's sampler, so the bits agree.  Fields hold blocks of one size, in
+// the pack layout.
+namespace {
+
+struct Remap {
+  int nd = 0, nv = 0, ng = 0;
+  int64_t nbase[3] = {1, 1, 1}, nx[3] = {1, 1, 1}, E[3] = {1, 1, 1}, ncell = 0;
+  const double *u = nullptr;
+  std::unordered_map<uint64_t, int> leaf;
+  std::unordered_set<uint64_t> refined;
+
+  int64_t nlat(int L, int d) const { return d < nd ? nbase[d] << L : 1; }
+  static int64_t fdiv(int64_t a, int64_t b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
+  double cell(int b, int v, int64_t i, int64_t j, int64_t k) const {
+    const int64_t g0 = ng, g1 = nd > 1 ? ng : 0, g2 = nd > 2 ? ng : 0;
+    return u[(int64_t)nv * ncell * b + v * ncell + ((k + g2) * E[1] + (j + g1)) * E[0] + (i + g0)];
+  }
+  struct Seg { int64_t node, first, end, off; };
+  std::vector<Seg> segments(int L, int d, int64_t lo, int64_t n) const {
+    std::vector<Seg> out;
+    if (d >= nd) {
+      out.push_back({0, 0, 1, 0});
+      return out;
+    }
+    for (int64_t c = lo; c < lo + n;) {
+      const int64_t node = fdiv(c, nx[d]);
+      const int64_t end = std::min((node + 1) * nx[d], lo + n);
+      const int64_t nl = nlat(L, d);
+      out.push_back({((node % nl) + nl) % nl, c - node * nx[d], end - node * nx[d], c - lo});
+      c = end;
+    }
+    return out;
+  }
+  // level-L values over cells lo .. lo + n (x, y, z) into out[v][z][y][x]
+  void box(int L, const int64_t *lo, const int64_t *n, std::vector<double> &out) const {
+    out.assign((size_t)nv * n[0] * n[1] * n[2], 0.0);
+    const auto sx = segments(L, 0, lo[0], n[0]), sy = segments(L, 1, lo[1], n[1]),
+               sz = segments(L, 2, lo[2], n[2]);
+    auto at = [&](int v, int64_t z, int64_t y, int64_t x) -> double & {
+      return out[(((size_t)v * n[2] + z) * n[1] + y) * n[0] + x];
+    };
+    for (const Seg &a2 : sz)
+      for (const Seg &a1 : sy)
+        for (const Seg &a0 : sx) {
+          const Seg s[3] = {a0, a1, a2};
+          const int64_t node[3] = {a0.node, a1.node, a2.node};
+          int64_t cnt[3];
+          for (int d = 0; d < 3; ++d) cnt[d] = s[d].end - s[d].first;
+          auto it = leaf.find(key(L, node[0], node[1], node[2]));
+          if (it != leaf.end()) {
+            for (int v = 0; v < nv; ++v)
+              for (int64_t z = 0; z < cnt[2]; ++z)
+                for (int64_t y = 0; y < cnt[1]; ++y)
+                  for (int64_t x = 0; x < cnt[0]; ++x)
+                    at(v, s[2].off + z, s[1].off + y, s[0].off + x) =
+                        cell(it->second, v, s[0].first + x, s[1].first + y, s[2].first + z);
+            continue;
+          }
+          int64_t g0[3];
+          for (int d = 0; d < 3; ++d) g0[d] = d < nd ? node[d] * nx[d] + s[d].first : 0;
+          std::vector<double> sub;
+          if (refined.count(key(L, node[0], node[1], node[2]))) {
+            int64_t flo[3], fn[3];
+            for (int d = 0; d < 3; ++d) {
+              flo[d] = d < nd ? 2 * g0[d] : 0;
+              fn[d] = d < nd ? 2 * cnt[d] : 1;
+            }
+            std::vector<double> fine;
+            box(L + 1, flo, fn, fine);
+            const int np = nd == 3 ? 8 : 4;
+            auto f = [&](int v, int64_t z, int64_t y, int64_t x) {
+              return fine[(((size_t)v * fn[2] + z) * fn[1] + y) * fn[0] + x];
+            };
+            for (int v = 0; v < nv; ++v)
+              for (int64_t z = 0; z < cnt[2]; ++z)
+                for (int64_t y = 0; y < cnt[1]; ++y)
+                  for (int64_t x = 0; x < cnt[0]; ++x) {
+                    double acc = 0.0;
+                    int k = 0;
+                    for (int dz = 0; dz < (nd == 3 ? 2 : 1); ++dz)
+                      for (int dy = 0; dy < 2; ++dy)
+                        for (int dx = 0; dx < 2; ++dx, ++k) {
+                          const double q = f(v, nd == 3 ? 2 * z + dz : 0, 2 * y + dy, 2 * x + dx);
+                          acc = k == 0 ? q : acc + q;
+                        }
+                    at(v, s[2].off + z, s[1].off + y, s[0].off + x) = acc * (1.0 / np);
+                  }
+          } else {
+            prolong(L, g0, cnt, sub);
+            for (int v = 0; v < nv; ++v)
+              for (int64_t z = 0; z < cnt[2]; ++z)
+                for (int64_t y = 0; y < cnt[1]; ++y)
+                  for (int64_t x = 0; x < cnt[0]; ++x)
+                    at(v, s[2].off + z, s[1].off + y, s[0].off + x) =
+                        sub[(((size_t)v * cnt[2] + z) * cnt[1] + y) * cnt[0] + x];
+          }
+        }
+  }
+  // level-L cells g0 .. g0 + cnt under a coarser leaf, into out[v][z][y][x]
+  void prolong(int L, const int64_t *g0, const int64_t *cnt, std::vector<double> &out) const {
+    if (L == 0) fail("a level-0 region outside every level-0 node");
+    int64_t clo[3], cn[3];
+    for (int d = 0; d < 3; ++d) {
+      clo[d] = d < nd ? fdiv(g0[d], 2) - 1 : 0;
+      const int64_t chi = d < nd ? fdiv(g0[d] + cnt[d] - 1, 2) + 1 : 0;
+      cn[d] = chi - clo[d] + 1;
+    }
+    std::vector<double> C;
+    box(L - 1, clo, cn, C);
+    auto c_at = [&](int v, const int64_t *I) {
+      return C[(((size_t)v * cn[2] + I[2]) * cn[1] + I[1]) * cn[0] + I[0]];
+    };
+    auto sgn = [](double a) { return a < 0.0 ? -1.0 : 1.0; };
+    out.assign((size_t)nv * cnt[0] * cnt[1] * cnt[2], 0.0);
+    for (int v = 0; v < nv; ++v)
+      for (int64_t z = 0; z < cnt[2]; ++z)
+        for (int64_t y = 0; y < cnt[1]; ++y)
+          for (int64_t x = 0; x < cnt[0]; ++x) {
+            const int64_t G[3] = {g0[0] + x, g0[1] + y, g0[2] + z};
+            int64_t I[3];
+            for (int d = 0; d < 3; ++d) I[d] = d < nd ? fdiv(G[d], 2) - clo[d] : 0;
+            const double c = c_at(v, I);
+            double val = c;
+            for (int d = 0; d < nd; ++d) {
+              int64_t lo[3] = {I[0], I[1], I[2]}, hi[3] = {I[0], I[1], I[2]};
+              lo[d] -= 1;
+              hi[d] += 1;
+              const double a = c - c_at(v, lo);
+              const double b = c_at(v, hi) - c;
+              const double fa = std::fabs(a), fb = std::fabs(b);
+              const double g = (0.5 * (sgn(a) + sgn(b))) * (fb < fa ? fb : fa);
+              const double sign = (((G[d] % 2) + 2) % 2 == 0) ? -1.0 : 1.0;
+              val = val + (sign * 0.25) * g;
+            }
+            out[(((size_t)v * cnt[2] + z) * cnt[1] + y) * cnt[0] + x] = val;
+          }
+  }
+};
+
+}  // namespace
+
+extern "C" int aph_amr_remap(int32_t ndim, const int32_t nbase[3], const int32_t nx[3],
+                             int32_t ng, int32_t nvar, int32_t n_old, const int32_t *old_level,
+                             const int64_t *old_loc, const double *old_u, int32_t n_new,
+                             const int32_t *new_level, const int64_t *new_loc, double *new_u,
+                             char *err, size_t err_len) {
+  auto report = [&](int code, const char *msg) {
+    if (err && err_len) snprintf(err, err_len, "aph_amr_remap: %s", msg);
+    return code;
+  };
+  if (!nx || !old_u || !new_u || nvar < 1 || n_new < 1 || !new_level || !new_loc)
+    return report(APH_E_INVALID, "bad arguments");
+  try {
+    Forest t;
+    forest_setup(t, ndim, nbase, n_old, old_level, old_loc);
+    Remap r;
+    r.nd = ndim;
+    r.nv = nvar;
+    r.ng = ng;
+    for (int d = 0; d < 3; ++d) {
+      r.nbase[d] = t.nbase[d];
+      r.nx[d] = d < ndim ? nx[d] : 1;
+      r.E[d] = r.nx[d] + (d < ndim ? 2 * ng : 0);
+    }
+    r.ncell = r.E[0] * r.E[1] * r.E[2];
+    r.u = old_u;
+    r.refined = t.refined;
+    for (int b = 0; b < n_old; ++b)
+      r.leaf.emplace(key(old_level[b], old_loc[3 * b], old_loc[3 * b + 1],
+                         ndim == 3 ? old_loc[3 * b + 2] : 0), b);
+    std::vector<double> vals;
+    for (int b = 0; b < n_new; ++b) {
+      const int lv = new_level[b];
+      const int64_t lc[3] = {new_loc[3 * b], new_loc[3 * b + 1], ndim == 3 ? new_loc[3 * b + 2] : 0};
+      const int64_t lo[3] = {lc[0] * r.nx[0], lc[1] * r.nx[1], lc[2] * r.nx[2]};
+      const int64_t n[3] = {r.nx[0], r.nx[1], r.nx[2]};
+      r.box(lv, lo, n, vals);  // a kept leaf: its own cells
+      const int64_t g1 = ndim > 1 ? ng : 0, g2 = ndim > 2 ? ng : 0;
+      for (int v = 0; v < nvar; ++v)
+        for (int64_t z = 0; z < n[2]; ++z)
+          for (int64_t y = 0; y < n[1]; ++y)
+            for (int64_t x = 0; x < n[0]; ++x)
+              new_u[(int64_t)nvar * r.ncell * b + v * r.ncell +
+                    ((z + g2) * r.E[1] + (y + g1)) * r.E[0] + (x + ng)] =
+                  vals[(((size_t)v * n[2] + z) * n[1] + y) * n[0] + x];
     }
   } catch (const BuildError &e) {
     return report(APH_E_INVALID, e.msg);

@@ -267,6 +267,17 @@ int aph_amr_block_table(int32_t ndim, const int32_t nbase[3], const int32_t nx[3
                         const double xmin[3], const double xmax[3], int32_t mesh_id, int32_t n,
                         const int32_t *level, const int64_t *loc, aph_block *out, char *err,
                         size_t err_len);
+/* The interiors of a regridded mesh (host arrays in the pack layout, every block nx^ndim
+   cells with ng ghosts, nvar components, blocks in leaf order): a kept leaf copies its
+   cells; any other new leaf takes its box of the old mesh's level values -- a leaf's cells,
+   the mean of a refined node's children ((dz, dy, dx) order, scaled by 2^-ndim), or under
+   a coarser leaf c + sum_d (sign_d / 4) minmod(c - c_-d, c_+d - c) of the level above, with
+   the stencil's values taken the same way (Parthenon's refinement / derefinement data
+   transfer).  Ghost cells of new_u are left untouched. */
+int aph_amr_remap(int32_t ndim, const int32_t nbase[3], const int32_t nx[3], int32_t ng,
+                  int32_t nvar, int32_t n_old, const int32_t *old_level, const int64_t *old_loc,
+                  const double *old_u, int32_t n_new, const int32_t *new_level,
+                  const int64_t *new_loc, double *new_u, char *err, size_t err_len);
 int aph_amr_build(const aph_block *blocks, int32_t nblocks, int32_t ndim, int32_t nvar,
                   const int32_t nbase[3], const int32_t *level, const int64_t *loc,
                   aph_amr_op *ops, int64_t *nops, aph_flux_fix *fixes, int64_t *nfixes,

@@ -245,3 +245,23 @@ def test_native_tree_refuses_bad_leaf_sets():
     bad.level, bad.loc = np.append(m.level, 0), np.vstack([m.loc, [[1, 1, 0]]])
     with pytest.raises(ValueError, match="inside another"):          # nested in a leaf
         amr.regrid_tree(bad, np.zeros(len(bad.level), np.int32))
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_native_remap_equals_python_statement(seed):
+    """aph_amr_remap (copy / restriction / minmod prolongation, nested where a new leaf's
+    stencil reaches under a coarser leaf) against the Python sampler, bitwise."""
+    rng = np.random.default_rng(100 + seed)
+    for m in _random_meshes():
+        bs = m.block_set()
+        u = 1.0 + rng.random(bs.field_size(5))
+        for _ in range(2):
+            tags = rng.choice([-1, 0, 1], size=len(m.level), p=[0.4, 0.3, 0.3]).astype(np.int32)
+            ref = amr.regrid_tree(m, tags)
+            new = amr.AmrMesh(m.ndim, m.nbase[:m.ndim], m.nx[:m.ndim], m.ng, m.xmin, m.xmax,
+                              ref, mesh_id=m.mesh_id, max_level=m.max_level)
+            nbs = new.block_set()
+            got = amr.remap(m, bs, u, new, nbs)
+            want = amr.remap_py(m, bs, u, new, nbs)
+            assert got.tobytes() == want.tobytes()
+            m, bs, u = new, nbs, got + 0.01 * rng.random(got.shape)

@@ -63,23 +63,28 @@ class DerefineCounter:
 
     def __init__(self, count=10):
         self.count = int(count)
-        self.seen = {}
+        self._mesh = None
+        self._keys = []    # (level, loc) of the last mesh's leaves, in block order
+        self._c = np.zeros(0, dtype=np.int64)   # their counts
+
+    @property
+    def seen(self):
+        """(level, loc) -> count, for the leaves with a nonzero count."""
+        return {k: int(v) for k, v in zip(self._keys, self._c) if v}
 
     def flags(self, mesh, tags):
-        out = np.zeros(len(tags), dtype=np.int32)
-        seen = {}
-        for b, t in enumerate(np.asarray(tags)):
-            key = (int(mesh.level[b]), tuple(int(v) for v in mesh.loc[b]))
-            if t > 0:
-                out[b] = 1
-            elif t < 0:
-                c = self.seen.get(key, 0)
-                if c < self.count:
-                    seen[key] = c + 1
-                else:
-                    seen[key] = c
-                    out[b] = -1
-        self.seen = seen
+        tags = np.asarray(tags)
+        if mesh is not self._mesh:  # a new block order: carry the counts over by key
+            seen = self.seen
+            keys = [(int(mesh.level[b]), tuple(int(v) for v in mesh.loc[b]))
+                    for b in range(len(mesh.level))]
+            self._c = np.array([seen.get(k, 0) for k in keys], dtype=np.int64)
+            self._keys, self._mesh = keys, mesh
+        c = self._c
+        neg = tags < 0
+        out = np.where(tags > 0, 1, 0).astype(np.int32)
+        out[neg & (c >= self.count)] = -1
+        c[:] = np.where(neg, np.minimum(c + 1, self.count), 0)
         return out
 
 
@@ -158,9 +163,11 @@ class AdaptiveDriver:
         flags[(flags < 0) & (self.mesh.level == 0)] = 0
         if flags.any():
             t0 = time.perf_counter()
-            u = self.state().cpu().numpy()
-            new, nbs, ncons = amr.regrid(self.mesh, self.bs, u, flags)
-            if new.refined != self.mesh.refined:
+            leaves = amr.regrid_leaves(self.mesh, flags)  # the tree alone first
+            if not (np.array_equal(leaves[0], self.mesh.level) and
+                    np.array_equal(leaves[1], self.mesh.loc)):
+                u = self.state().cpu().numpy()
+                new, nbs, ncons = amr.regrid(self.mesh, self.bs, u, flags, leaves=leaves)
                 self.mesh, self.bs = new, nbs
                 self._drop()
                 self._build(ncons)

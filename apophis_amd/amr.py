@@ -668,6 +668,14 @@ def _ancestors(levels, locs, nd):
 
 def regrid_tree(mesh: AmrMesh, tags):
     """The refined nodes after one regrid, from the library's tree (aph_amr_regrid_tree)."""
+    level, loc = regrid_leaves(mesh, tags)
+    return _ancestors(level, loc, mesh.ndim)
+
+
+def regrid_leaves(mesh: AmrMesh, tags):
+    """The leaves (level [n], loc [n, 3]) after one regrid, in the mesh's leaf order, from
+    the library's tree (aph_amr_regrid_tree); equal to (mesh.level, mesh.loc) when the
+    tags change nothing."""
     vp = ctypes.c_void_p
     L = _lib.lib()
     n = len(mesh.level)
@@ -690,10 +698,10 @@ def regrid_tree(mesh: AmrMesh, tags):
                                      m.value))
     if rc != _lib.APH_OK:
         raise ValueError(err.value.decode())
-    return _ancestors(out_level, out_loc, mesh.ndim)
+    return out_level, out_loc
 
 
-def regrid(mesh: AmrMesh, bs: BlockSet, cons, tags):
+def regrid(mesh: AmrMesh, bs: BlockSet, cons, tags, leaves=None):
     """One regrid from per-block tags (-1 / 0 / 1, as aph_refinement_tag returns them),
     Parthenon's rules on a periodic tree:
     * a leaf tagged 1 below ``mesh.max_level`` is refined;
@@ -709,7 +717,9 @@ def regrid(mesh: AmrMesh, bs: BlockSet, cons, tags):
     from the level above, as the ghost exchange fills them).  Ghost cells are left to the next ghost fill.  Totals are conserved to
     round-off and refining then merging returns the data."""
     nd = mesh.ndim
-    ref = regrid_tree(mesh, tags)
+    if leaves is None:
+        leaves = regrid_leaves(mesh, tags)
+    ref = _ancestors(leaves[0], leaves[1], nd)
     new = AmrMesh(nd, mesh.nbase[:nd], mesh.nx[:nd], mesh.ng, mesh.xmin, mesh.xmax, ref,
                   mesh_id=mesh.mesh_id, max_level=mesh.max_level)
     nbs = new.block_set(nscalars=bs.nscalars)

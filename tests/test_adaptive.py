@@ -43,6 +43,22 @@ def test_derefine_counter_delays_and_resets():
     assert c.flags(m, tags)[1] == 1
 
 
+def test_derefine_counter_carries_counts_by_leaf_across_meshes():
+    m = amr.AmrMesh(2, (2, 2), (4, 4), 2, (0, 0, 0), (1, 1, 1), [(0, (0, 0))], max_level=1)
+    c = adaptive.DerefineCounter(2)
+    c.flags(m, np.full(len(m.level), -1, np.int32))
+    c.flags(m, np.full(len(m.level), -1, np.int32))      # every leaf counted twice
+    # a mesh with (0, (1, 1)) refined too: the old leaves keep their counts, new ones start at 0
+    m2 = amr.AmrMesh(2, (2, 2), (4, 4), 2, (0, 0, 0), (1, 1, 1),
+                     [(0, (0, 0)), (0, (1, 1))], max_level=1)
+    f = c.flags(m2, np.full(len(m2.level), -1, np.int32))
+    old = {(int(l), tuple(int(v) for v in x)) for l, x in zip(m.level, m.loc)}
+    for b in range(len(m2.level)):
+        key = (int(m2.level[b]), tuple(int(v) for v in m2.loc[b]))
+        assert f[b] == (-1 if key in old else 0), key
+    assert all(v == (2 if k in old else 1) for k, v in c.seen.items())
+
+
 def test_blast_deck_pgen_restates_blast_cpp():
     """blast.cpp:16-71: inside rin (= radius_outer - radius_inner) the (drat damb,
     prat pamb) state, outside radius_outer the ambient state, between them a logarithmic

@@ -414,24 +414,27 @@ def fused_faces(mesh: AmrMesh, bs: BlockSet):
     trans = lambda d: ((1 if d == 0 else 0), (1 if d == 2 else 2))
 
     def across(b, d, side):
+        b, d, side = int(b), int(d), int(side)
         o = [0, 0, 0]
         o[d] = 1 if side else -1
         return mesh.shift(int(mesh.level[b]), tuple(int(v) for v in mesh.loc[b]), o)
 
-    amr_faces = [(b, d, side) for b in range(bs.nblocks) for d in range(nd)
-                 for side in (0, 1) if bs.blocks["bc"][b][2 * d + side] == BC_AMR]
-    for b, d, side in amr_faces:
-        if (int(mesh.level[b]), across(b, d, side)) not in mesh.refined:
+    bc = bs.blocks["bc"]
+    # (block, axis, side, the other side is refined) of every face on a level change
+    amr_faces = [(b, d, side, (int(mesh.level[b]), across(b, d, side)) in mesh.refined)
+                 for b, d, side in zip(*np.nonzero(bc[:, :2 * nd].reshape(-1, nd, 2) == BC_AMR))]
+    for b, d, side, finer in amr_faces:
+        if not finer:
             continue
         A, B = trans(d)
         region[(b, d, side)] = total
         f = faces[b * 6 + 2 * d + side]
         f["base"], f["role"], f["na"] = total, 2, nx[A]
         total += nx[A] * (nx[B] if B < nd else 1) * nch * nv
-    for b, d, side in amr_faces:
-        lv = int(mesh.level[b])
-        if (lv, across(b, d, side)) in mesh.refined:
+    for b, d, side, finer in amr_faces:
+        if finer:
             continue
+        lv = int(mesh.level[b])
         A, B = trans(d)
         lf = mesh.loc[b]
         # the coarse cell across the face, and the coarse leaf holding it
@@ -465,17 +468,33 @@ def native_plan(mesh: AmrMesh, bs: BlockSet):
                             nbase.ctypes.data_as(vp), level.ctypes.data_as(vp),
                             loc.ctypes.data_as(vp), o, ctypes.byref(n[0]), f,
                             ctypes.byref(n[1]), r, ctypes.byref(n[2]), err, len(err))
-    rc = L.aph_amr_build(*args(None, None, None))
-    if rc != _lib.APH_OK:
-        raise _lib.HydroError(rc, err.value.decode())
-    ops = np.zeros(n[0].value, dtype=AMR_OP_DTYPE)
-    fixes = np.zeros(n[1].value, dtype=FLUX_FIX_DTYPE)
-    refs = np.zeros(n[2].value, dtype=AMR_REF_DTYPE)
-    rc = L.aph_amr_build(*args(ops.ctypes.data_as(vp), fixes.ctypes.data_as(vp),
-                               refs.ctypes.data_as(vp)))
-    if rc != _lib.APH_OK:
-        raise _lib.HydroError(rc, err.value.decode())
-    return ops, fixes, refs
+    # one build when the arrays are long enough: they are sized from the last build of a
+    # mesh of this shape (a regrid changes the lists a little); a build into arrays too
+    # short reports the exact lengths, and the second build uses them
+    shape = (mesh.ndim, bs.nvar, tuple(int(v) for v in mesh.nx), int(bs.blocks["ng"][0]))
+    hint = _PLAN_HINT.get(shape)
+    cap = None if hint is None else [int(1.25 * h) + 64 for h in hint]
+    for _ in range(3):
+        c = cap or [0, 0, 0]
+        ops = np.zeros(c[0], dtype=AMR_OP_DTYPE)
+        fixes = np.zeros(c[1], dtype=FLUX_FIX_DTYPE)
+        refs = np.zeros(c[2], dtype=AMR_REF_DTYPE)
+        for q in range(3):
+            n[q].value = c[q]
+        ptr = [None] * 3 if cap is None else [a.ctypes.data_as(vp) for a in (ops, fixes, refs)]
+        rc = L.aph_amr_build(*args(*ptr))
+        got = [n[q].value for q in range(3)]
+        short = any(g > k for g, k in zip(got, c))
+        if rc == _lib.APH_OK and cap is not None:
+            break
+        if rc != _lib.APH_OK and not (cap is not None and short):
+            raise _lib.HydroError(rc, err.value.decode())
+        cap = got
+    _PLAN_HINT[shape] = got
+    return ops[:got[0]], fixes[:got[1]], refs[:got[2]]
+
+
+_PLAN_HINT = {}  # (ndim, nvar, nx, ng) -> list lengths of the last native build
 
 
 class AmrPlan:

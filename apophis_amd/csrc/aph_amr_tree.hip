@@ -88,11 +88,11 @@ struct Tree {
     for (int d = 0; d < 3; ++d) l[d] = W[d] - node[d] * nx[d] + (d < nd ? ng : 0);
     return true;
   }
-  aph_amr_val add(const std::vector<int64_t> &offs) {
+  aph_amr_val add(const int64_t *offs, int n) {
     aph_amr_val v;
     v.first = (int32_t)refs.size();
-    v.count = (int32_t)offs.size();
-    for (int64_t o : offs) refs.push_back(aph_amr_ref{o, ncell});
+    v.count = n;
+    for (int q = 0; q < n; ++q) refs.push_back(aph_amr_ref{offs[q], ncell});
     if (refs.size() > (size_t)INT32_MAX) fail("too many cell references");
     return v;
   }
@@ -100,10 +100,14 @@ struct Tree {
   aph_amr_val value(int lv, const int64_t I[3]) {
     int b;
     int64_t l[3];
-    if (interior(lv, I, b, l)) return add({cell(b, l)});
+    if (interior(lv, I, b, l)) {
+      const int64_t c = cell(b, l);
+      return add(&c, 1);
+    }
     int64_t W[3];
     for (int d = 0; d < 3; ++d) W[d] = wrap(I[d], lv, d);
-    std::vector<int64_t> kids;
+    int64_t kids[8];
+    int nk = 0;
     for (int cz = 0; cz < (nd == 3 ? 2 : 1); ++cz)
       for (int cy = 0; cy < 2; ++cy)
         for (int cx = 0; cx < 2; ++cx) {
@@ -112,9 +116,9 @@ struct Tree {
             fail("cell (%lld, %lld, %lld) of level %d is neither a leaf cell nor the parent "
                  "of leaf cells (2:1 balance)", (long long)W[0], (long long)W[1],
                  (long long)W[2], lv);
-          kids.push_back(cell(b, l));
+          kids[nk++] = cell(b, l);
         }
-    return add(kids);
+    return add(kids, nk);
   }
 };
 
@@ -236,7 +240,8 @@ void build(Tree &t, std::vector<aph_amr_op> &ops, std::vector<aph_flux_fix> &fix
             int64_t l[3] = {0, 0, 0};
             l[d] = g + (side ? t.nx[d] : 0);
             for (int q = 0; q < ntr; ++q) l[tr[q]] = g + tc[q];
-            std::vector<int64_t> cells;
+            int64_t cells[4];
+            int nc = 0;
             for (int k0 = 0; k0 < 2; ++k0)
               for (int k1 = 0; k1 < (ntr > 1 ? 2 : 1); ++k1) {
                 const int kid[2] = {k0, k1};
@@ -249,14 +254,14 @@ void build(Tree &t, std::vector<aph_amr_op> &ops, std::vector<aph_flux_fix> &fix
                 if (!t.interior(lv + 1, F, fb, fl))
                   fail("block %d face %d: fine face outside the finer level", b, 2 * d + side);
                 if (!side) fl[d] += 1;  // upper face of the last fine cell below
-                cells.push_back(t.cell(fb, fl));
+                cells[nc++] = t.cell(fb, fl);
               }
             aph_flux_fix fx;
             std::memset(&fx, 0, sizeof(fx));
             fx.dst = t.cell(b, l);
             fx.dst_stride = t.ncell;
             fx.dir = d;
-            fx.val = t.add(cells);
+            fx.val = t.add(cells, nc);
             fixes.push_back(fx);
           }
       }

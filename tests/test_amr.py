@@ -324,10 +324,19 @@ def test_native_tree_builds_the_same_lists(name):
         m = _mesh(name)
         bs = m.block_set()
     want = amr.amr_plan(m, bs)
-    got = amr.native_plan(m, bs)
-    for w, g in zip(want, got):
-        assert w.dtype == g.dtype and len(w) == len(g)
-        assert w.tobytes() == g.tobytes()
+    # no length hint (a counting build first), a hint too short (a second build at the
+    # reported lengths) and the hint the last build left (one build)
+    shape = (m.ndim, bs.nvar, tuple(int(v) for v in m.nx), int(bs.blocks["ng"][0]))
+    for hint in (None, [0, 0, 0], "kept"):
+        if hint is None:
+            amr._PLAN_HINT.pop(shape, None)
+        elif hint != "kept":
+            amr._PLAN_HINT[shape] = hint
+        got = amr.native_plan(m, bs)
+        for w, g in zip(want, got):
+            assert w.dtype == g.dtype and len(w) == len(g)
+            assert w.tobytes() == g.tobytes()
+        assert amr._PLAN_HINT[shape] == [len(w) for w in want]
 
 
 def test_native_tree_refuses_bad_trees():

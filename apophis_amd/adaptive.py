@@ -88,6 +88,20 @@ class DerefineCounter:
         return out
 
 
+def _may_change(mesh, flags):
+    """False when no regrid rule can apply: no leaf is flagged to refine and no sibling
+    group has all of its 2^ndim leaves flagged to derefine (merges need the whole group;
+    balance only acts on refinements and merges)."""
+    if (flags > 0).any():
+        return True
+    neg = flags < 0
+    if not neg.any():
+        return False
+    par = np.concatenate([mesh.level[neg, None] - 1, mesh.loc[neg] >> 1], axis=1)
+    _, cnt = np.unique(par, axis=0, return_counts=True)
+    return bool((cnt == 2 ** mesh.ndim).any())
+
+
 class AdaptiveDriver:
     """Cycles of one step + tag + regrid on an adaptive multilevel mesh.
 
@@ -161,7 +175,7 @@ class AdaptiveDriver:
         flags = self.counter.flags(self.mesh, self._tags())
         flags[(flags > 0) & (self.mesh.level >= self.mesh.max_level)] = 0
         flags[(flags < 0) & (self.mesh.level == 0)] = 0
-        if flags.any():
+        if _may_change(self.mesh, flags):
             t0 = time.perf_counter()
             leaves = amr.regrid_leaves(self.mesh, flags)  # the tree alone first
             if not (np.array_equal(leaves[0], self.mesh.level) and

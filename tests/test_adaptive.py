@@ -106,3 +106,31 @@ def test_adaptive_start_refines_around_the_blast():
             assert m.level[b] == 2, (b, m.loc[b])
     # the state is generated again on the final mesh, not prolongated onto it
     np.testing.assert_array_equal(cons, problems.blast_pgen()(m, bs))
+
+
+@pytest.mark.parametrize("nd", [2, 3])
+def test_no_change_check_never_skips_a_regrid(nd):
+    """The driver skips the regrid when no leaf is flagged 1 and no sibling group is
+    flagged -1 whole; on random flags the library's regrid then returns the same leaves,
+    and the check passes every flag set that does change them."""
+    rng = np.random.default_rng(7 + nd)
+    base = (2, 2, 2)[:nd] + (1,) * (3 - nd)
+    refined = [(0, (0, 0, 0)[:nd] + (0,) * (3 - nd)), (0, (1, 1, 0) if nd == 2 else (1, 1, 1))]
+    m = amr.AmrMesh(nd, base[:nd], (4,) * nd, 1, (0, 0, 0), (1, 1, 1),
+                    [(l, tuple(c[:nd])) for l, c in refined], max_level=2)
+    skipped = changed = 0
+    for trial in range(300):
+        p = rng.choice([0.0, 0.02, 0.3])
+        flags = np.where(rng.random(len(m.level)) < 0.85, -1, 0).astype(np.int32)
+        flags[rng.random(len(m.level)) < p] = 1
+        if trial % 3 == 0:  # break up sibling groups
+            flags[rng.random(len(m.level)) < 0.5] = 0
+        flags[(flags < 0) & (m.level == 0)] = 0
+        flags[(flags > 0) & (m.level >= m.max_level)] = 0
+        lv, lc = amr.regrid_leaves(m, flags)
+        same = np.array_equal(lv, m.level) and np.array_equal(lc, m.loc)
+        if not adaptive._may_change(m, flags):
+            assert same
+            skipped += 1
+        changed += not same
+    assert skipped > 20 and changed > 20

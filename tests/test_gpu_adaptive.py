@@ -144,3 +144,42 @@ def test_evolve_ends_at_tlim(gpu_ctx):
     (r0, b0, u0), (r1, b1, u1) = ends
     mask = np.tile(b0.interior_mask(), b0.nvar)
     assert r0 == r1 and np.array_equal(u0[mask], u1[mask])
+
+
+def _totals(bs, u):
+    t = np.zeros(5)
+    for b in range(bs.nblocks):
+        t += bs.interior(u, b, bs.nvar).reshape(bs.nvar, -1)[:5].sum(axis=1) * bs.blocks["vol"][b]
+    return t
+
+
+def test_blast_deck_to_tlim_fused_equals_plain(gpu_ctx):
+    """The reference's adaptive deck (inputs/blast.in: 128^2 on 16^2 blocks, three levels)
+    run to its tlim in the fused and the plain form: the same cycles and regrids, the same
+    final tree and the same bits in every interior cell, with mass and energy totals kept
+    to round-off through the ~300 refines and merges."""
+    print("[case] blast deck to tlim, fused and plain", flush=True)
+    pkg = hydro.initialize_hydro(problems.blast_deck_pin())
+    m, bs, cons = adaptive.initial_mesh(
+        problems.blast_deck_base_mesh(), problems.blast_pgen(),
+        lambda mm, bb, cc: adaptive.device_tags(gpu_ctx, pkg, mm, bb, cc))
+    t0 = _totals(bs, cons)
+    ends = []
+    for form in ("fused", "plain"):
+        drv = adaptive.AdaptiveDriver(gpu_ctx, pkg, m, bs, cons, form=form,
+                                      tlim=problems.BLAST_DECK["tlim"])
+        n = drv.evolve()
+        u = drv.state().cpu().numpy()
+        ends.append((n, drv.stats["regrids"], drv.time, set(drv.mesh.refined), drv.bs, u))
+        print(f"[case] {form}: {n} cycles, {drv.stats['regrids']} regrids, "
+              f"{drv.bs.nblocks} blocks", flush=True)
+        t1 = _totals(drv.bs, u)
+        assert abs(t1[0] - t0[0]) <= 1e-13 * abs(t0[0])
+        assert abs(t1[4] - t0[4]) <= 1e-13 * abs(t0[4])
+        del drv
+    (n0, r0, T0, f0, b0, u0), (n1, r1, T1, f1, b1, u1) = ends
+    assert n0 == n1 and r0 == r1 and r0 > 100 and T0 == T1
+    assert T0 == pytest.approx(problems.BLAST_DECK["tlim"], rel=1e-15, abs=0)
+    assert f0 == f1
+    mask = np.tile(b0.interior_mask(), b0.nvar)
+    assert np.array_equal(u0[mask], u1[mask])

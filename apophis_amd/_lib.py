@@ -70,6 +70,8 @@ _SIGS = {
     "aph_plan_destroy": (_i32, [_vp, _vp]),
     "aph_plan_set_tile": (_i32, [_vp, _vp, ctypes.POINTER(Tile)]),
     "aph_plan_set_integrator": (_i32, [_vp, _vp, _i32]),
+    "aph_plan_status": (_i32, [_vp, _vp]),
+    "aph_plan_set_wait_limit": (_i32, [_vp, _vp, _i32]),
     "aph_plan_autotune": (_i32, [_vp, _vp, _vp, _vp, _vp, _i32, ctypes.c_char_p,
                                  ctypes.c_size_t]),
     "aph_plan_get_tile": (_i32, [_vp, _vp, _i32, ctypes.POINTER(Tile)]),

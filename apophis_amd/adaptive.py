@@ -57,8 +57,9 @@ def initial_mesh(mesh, pgen, tagger, max_rounds=32):
 
 class DerefineCounter:
     """Parthenon's derefinement delay (MeshRefinement, parthenon/mesh/derefine_count,
-    default 10): a leaf tagged -1 counts up and only becomes a derefine flag once it has
-    been tagged so `count` times in a row; any other tag resets it.  Leaves are keyed by
+    default 10): a leaf tagged -1 counts up and becomes a derefine flag on the `count`-th
+    -1 tag in a row (deref_count_ is incremented, then compared with the threshold); any
+    other tag resets it.  Leaves are keyed by
     (level, loc), so counters survive regrids of other blocks; new leaves start at 0."""
 
     def __init__(self, count=10):
@@ -83,8 +84,11 @@ class DerefineCounter:
         c = self._c
         neg = tags < 0
         out = np.where(tags > 0, 1, 0).astype(np.int32)
-        out[neg & (c >= self.count)] = -1
-        c[:] = np.where(neg, np.minimum(c + 1, self.count), 0)
+        # MeshRefinement's rule: the count goes up first, and the leaf is flagged once it
+        # reaches `count` (the count-th -1 tag in a row)
+        c_new = np.where(neg, c + 1, 0)
+        out[neg & (c_new >= self.count)] = -1
+        c[:] = np.minimum(c_new, self.count)
         return out
 
 
@@ -166,6 +170,7 @@ class AdaptiveDriver:
             self.fd.set_dt(dt)
             self.fd.step(fixed_dt=True)
             est = float(self.fd.initial_dt().item())
+            self.fd.status()  # synchronised already: raise if a multilevel wait ran out
         else:
             self._est = self.drv.step(dt)
             est = self._est

@@ -749,6 +749,17 @@ def remap(mesh: AmrMesh, bs: BlockSet, cons, new: AmrMesh, nbs: BlockSet):
     """The interiors of ``new`` (block set ``nbs``) from the state ``cons`` of ``mesh``,
     by the library (aph_amr_remap): the bits of ``remap_py``."""
     vp = ctypes.c_void_p
+    # the library takes every old block as nx^ndim interior cells with ng ghosts and reads
+    # nvar * ncell values per block: check that the arrays are laid out that way
+    for s_, m_ in ((bs, mesh), (nbs, new)):
+        if s_.nblocks != len(m_.level):
+            raise ValueError("remap: block set and mesh differ in block count")
+        if s_.nblocks and ((s_.blocks["ng"] != mesh.ng).any() or
+                           (s_.blocks["nx"] != np.asarray(mesh.nx)[None, :]).any()):
+            raise ValueError("remap: every block must have the mesh's nx and ng")
+    if np.size(cons) != bs.field_size(bs.nvar):
+        raise ValueError(f"remap: cons has {np.size(cons)} values, the block set needs "
+                         f"{bs.field_size(bs.nvar)}")
     out = np.zeros(nbs.field_size(bs.nvar))
     u = np.ascontiguousarray(cons, dtype=np.float64)
     args = []

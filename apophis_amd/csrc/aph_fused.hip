@@ -29,6 +29,8 @@
 
 namespace aph {
 
+constexpr int kDefaultWaitLimit = 1 << 20;
+
 // aph_amr.hip: the cross-level fill, bumping the multilevel launch epoch in the same launch
 int amr_fill_epoch(aph_context *ctx, const aph_amr_plan *p, double *cons, unsigned *epoch);
 
@@ -43,10 +45,16 @@ __global__ void k_amr_check(const int *err, double *u_out) {
 }
 
 // dt policy of the upstream EvolutionDriver: time += dt; dt <- min(2 dt, estimate),
-// clipped so that the step does not pass tlim.
+// clipped so that the step does not pass tlim.  A multilevel wait that ran out (err set)
+// makes dt and time NaN, so every later step and any read of them shows it.
 __global__ void k_dt_finalize(double *dt, double *time, const double *est, double tlim,
-                              int fixed) {
+                              int fixed, const int *err) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (err && *err) {
+    *dt = __builtin_nan("");
+    *time = __builtin_nan("");
+    return;
+  }
   const double d = *dt;
   const double t = *time + d;
   *time = t;
@@ -103,6 +111,7 @@ struct aph_plan {
   unsigned *d_expect = nullptr;
   unsigned *d_epoch = nullptr;
   int *d_err = nullptr;
+  int wait_limit = kDefaultWaitLimit;  // polls of a multilevel wait before it gives up
   bool amr_faces = false;  // the block set has faces to another level: needs set_amr
 };
 
@@ -293,13 +302,22 @@ int build_items(aph_context *ctx, aph_plan *p) {
     return set_error(ctx, APH_E_INVALID, "tile too large for LDS (%zu bytes)", p->lds);
   p->tiles.assign(tl.begin(), tl.end());
   // persistent grid: resident workgroups, a multiple of the XCD count
-  KernelFn fn = p->amr ? pick_kernel_amr(p->recon, p->rsolver, bs->nvar, ndim)
-                       : pick_kernel(p->recon, p->rsolver, bs->nvar, ndim);
-  if (!fn) return set_error(ctx, APH_E_INVALID, "no fused kernel for this plan");
-  int dev = ctx->device, ncu = 0, occ = 0;
+  // every kernel variant the plan launches (vl2: the first-order predictor too); the
+  // grid takes the smallest residency so that a multilevel launch keeps every
+  // workgroup resident whichever variant runs
+  std::vector<int> recons{p->recon};
+  if (p->integrator == APH_INTEGRATOR_VL2) recons.push_back(RECON_DC);
+  int dev = ctx->device, ncu = 0, occ = 1 << 30;
   APH_HIP_CHECK(ctx, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-  APH_HIP_CHECK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                         &occ, reinterpret_cast<const void *>(fn), 64, p->lds));
+  for (int r : recons) {
+    KernelFn fn = p->amr ? pick_kernel_amr(r, p->rsolver, bs->nvar, ndim)
+                         : pick_kernel(r, p->rsolver, bs->nvar, ndim);
+    if (!fn) return set_error(ctx, APH_E_INVALID, "no fused kernel for this plan");
+    int o = 0;
+    APH_HIP_CHECK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                           &o, reinterpret_cast<const void *>(fn), 64, p->lds));
+    occ = std::min(occ, o);
+  }
   occ = std::max(occ, 1);
   int grid = ncu * occ;
   grid = std::max(kXcd, (grid / kXcd) * kXcd);
@@ -350,6 +368,7 @@ FusedArgs make_args(aph_plan *p, int stage, int nstages, double gam0, double gam
   a.amr_expect = p->d_expect;
   a.amr_epoch = p->d_epoch;
   a.amr_err = p->d_err;
+  a.amr_wait_limit = p->wait_limit;
   return a;
 }
 
@@ -791,7 +810,31 @@ int aph_plan_set_integrator(aph_context *ctx, aph_plan *p, int32_t integrator) {
     }
   }
   p->integrator = integrator;
+  p->dirty = true;  // the grid is sized over the kernel variants the integrator launches
   drop_graph(p);
+  return APH_OK;
+}
+
+int aph_plan_status(aph_context *ctx, aph_plan *p) {
+  if (!ctx) return APH_E_INVALID;
+  if (!p) return set_error(ctx, APH_E_INVALID, "aph_plan_status: null plan");
+  if (!p->d_err) return APH_OK;
+  APH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  int err = 0;
+  APH_HIP_CHECK(ctx, hipMemcpy(&err, p->d_err, sizeof(int), hipMemcpyDeviceToHost));
+  if (!err) return APH_OK;
+  APH_HIP_CHECK(ctx, hipMemset(p->d_err, 0, sizeof(int)));  // reported: cleared
+  return set_error(ctx, APH_E_STATE,
+                   "multilevel stage: a coarse tile's wait for the finer level's interface "
+                   "fluxes ran out; the states, dt and time written since the last "
+                   "aph_plan_status are invalid (NaN)");
+}
+
+int aph_plan_set_wait_limit(aph_context *ctx, aph_plan *p, int32_t polls) {
+  if (!ctx) return APH_E_INVALID;
+  if (!p) return set_error(ctx, APH_E_INVALID, "aph_plan_set_wait_limit: null plan");
+  p->wait_limit = polls == 0 ? kDefaultWaitLimit : polls;
+  drop_graph(p);  // the limit is a launch argument
   return APH_OK;
 }
 
@@ -838,7 +881,8 @@ int aph_plan_step(aph_context *ctx, aph_plan *p, double *u0, double *u1, double 
       rc = launch_stage(ctx, p, 2, u1, u0, u0, dt_dev, cfl);
   }
   if (rc) return rc;
-  k_dt_finalize<<<1, 1, 0, ctx->stream>>>(dt_dev, time_dev, p->d_est, tlim, fixed_dt);
+  k_dt_finalize<<<1, 1, 0, ctx->stream>>>(dt_dev, time_dev, p->d_est, tlim, fixed_dt,
+                                          p->amr ? p->d_err : nullptr);
   APH_LAUNCH_CHECK(ctx);
   return APH_OK;
 }

@@ -50,6 +50,9 @@ struct FusedArgs {
   const unsigned *amr_expect;
   const unsigned *amr_epoch;
   int *amr_err;
+  // polls a multilevel wait makes before it gives up and sets amr_err; negative: every
+  // wait gives up at once (forces the error path from the host, for its tests)
+  int amr_wait_limit;
 };
 
 namespace {
@@ -108,7 +111,11 @@ __device__ __forceinline__ bool load_cell(const FusedArgs &a, const aph_block &b
 // workgroup of the persistent grid is resident, so the writers are running or done; the
 // bound only turns a broken plan into a flagged error instead of a hang.
 __device__ __forceinline__ void amr_wait(const FusedArgs &a, int r) {
-  constexpr int kSpinLimit = 1 << 20;
+  const int limit = a.amr_wait_limit;
+  if (limit < 0) {
+    __hip_atomic_fetch_or(a.amr_err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return;
+  }
   const unsigned long long target =
       (unsigned long long)(*a.amr_epoch) * (unsigned long long)a.amr_expect[r];
   int n = 0;
@@ -120,7 +127,7 @@ __device__ __forceinline__ void amr_wait(const FusedArgs &a, int r) {
     if ((n & 255) == 0 &&
         __hip_atomic_load(a.amr_err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
       break;
-    if (n > kSpinLimit) {
+    if (n > limit) {
       __hip_atomic_fetch_or(a.amr_err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       break;
     }

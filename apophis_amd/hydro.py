@@ -562,6 +562,17 @@ class FusedDriver:
     def replay(self):
         self.ctx.check(_lib.lib().aph_plan_graph_launch(self.ctx.h, self.plan))
 
+    def status(self):
+        """Synchronises and raises if a multilevel wait of any launch since the last call
+        ran out (aph_plan_status; the flag is cleared once reported)."""
+        self.ctx.check(_lib.lib().aph_plan_status(self.ctx.h, self.plan))
+
+    def set_wait_limit(self, polls):
+        """Polls of a multilevel wait before it gives up (0: default; negative: give up
+        at once, which forces the error path)."""
+        self.ctx.check(_lib.lib().aph_plan_set_wait_limit(self.ctx.h, self.plan, int(polls)))
+        self.graph_steps = 0
+
     def close(self):
         if self.plan:
             _lib.lib().aph_plan_destroy(self.ctx.h, self.plan)

@@ -367,7 +367,8 @@ int aph_plan_estimate_timestep(aph_context *ctx, aph_plan *plan, const double *c
    launch over every level whose items run the finest level first: a tile with a face to
    a finer region waits (bounded) until the fine tiles that write it have published.  If
    that wait runs out (the device could not keep the whole grid resident), the stage
-   writes NaN to the first element of u_out. */
+   writes NaN to the first element of u_out, aph_plan_step also sets dt and time to NaN,
+   and the plan's error flag stays set until aph_plan_status reports it. */
 typedef struct aph_amr_face {
   int64_t base;  /* element offset of the coarse face region in the interface buffer */
   int32_t role;  /* 0 none, 1 fine side (writes), 2 coarse side (reads)               */
@@ -384,6 +385,15 @@ typedef struct aph_amr_face {
 int aph_plan_set_amr(aph_context *ctx, aph_plan *plan, const aph_amr_plan *amr,
                      const aph_amr_face *faces, const int32_t *level,
                      int64_t buffer_doubles);
+/* Synchronous: waits for the context's stream, then returns APH_E_STATE (with a message)
+   if a multilevel wait of any launch since the last call ran out, and clears the flag;
+   APH_OK otherwise (always, for a single-level plan).  No reference counterpart: the
+   reference's multilevel exchange is host MPI, which fails by aborting. */
+int aph_plan_status(aph_context *ctx, aph_plan *plan);
+/* Polls a multilevel wait makes before it gives up (0: the default, 2^20).  A negative
+   value makes every wait give up at once, which forces the error path above from the
+   host (used by its tests).  Drops a captured graph. */
+int aph_plan_set_wait_limit(aph_context *ctx, aph_plan *plan, int32_t polls);
 
 /* ----------------------------------------------------- adjoint (frozen dt)
  * Reverse-mode counterparts of the fused stage (no reference counterpart: the

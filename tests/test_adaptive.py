@@ -31,9 +31,9 @@ def test_derefine_counter_delays_and_resets():
     c = adaptive.DerefineCounter(3)
     n = len(m.level)
     tags = np.full(n, -1, np.int32)
-    for _ in range(3):
-        assert not (c.flags(m, tags) == -1).any()  # counting 1, 2, 3
-    assert (c.flags(m, tags) == -1).all()           # the fourth in a row
+    for _ in range(2):
+        assert not (c.flags(m, tags) == -1).any()  # counting 1, 2
+    assert (c.flags(m, tags) == -1).all()           # the third in a row
     tags[0] = 0
     f = c.flags(m, tags)
     assert f[0] == 0 and (f[1:] == -1).all()

@@ -170,6 +170,41 @@ def test_fused_amr_steps_match_oracle(gpu_ctx, name, recon, riemann):
     assert np.array_equal(got[mask], ref.u0[mask])
 
 
+def test_fused_amr_wait_that_runs_out_is_reported(gpu_ctx):
+    """A multilevel wait that gives up poisons the stage output, dt and time with NaN, and
+    aph_plan_status reports it once, then clears it.  The failure is forced from the host
+    (a negative wait limit: every wait gives up without polling), so every kernel still
+    exits normally.  With the default limit restored the steps equal the oracle's again."""
+    print("[case] fused amr wait reported", flush=True)
+    m = _mesh("2d_L")
+    p = _problem(m, recon="plm")
+    p.riemann = "hllc"
+    ref = ao.AmrState(p, m)
+    dt = ref.estimate_dt()
+    fd = hydro.FusedDriver(gpu_ctx, p.bs, p.cons, "plm", "hllc", p.eos, p.cfl, amr_mesh=m)
+    fd.status()  # nothing to report yet
+    fd.set_dt(dt)
+    fd.set_wait_limit(-1)
+    fd.step(fixed_dt=False)
+    torch.cuda.synchronize()
+    assert np.isnan(fd.dt.item()) and np.isnan(fd.time.item())
+    assert np.isnan(fd.u0[0].item())
+    with pytest.raises(hydro._lib.HydroError, match="ran out"):
+        fd.status()
+    fd.status()  # reported once, then cleared
+    fd.set_wait_limit(0)
+    fd.u0.copy_(torch.from_numpy(p.cons))
+    fd.time.zero_()
+    fd.set_dt(dt)
+    for _ in range(2):
+        fd.step(fixed_dt=True)
+        ref.step(dt)
+    fd.status()
+    got = fd.u0.cpu().numpy()
+    mask = np.tile(p.bs.interior_mask(), 5)
+    assert np.array_equal(got[mask], ref.u0[mask])
+
+
 @pytest.mark.parametrize("ndim", [2, 3])
 def test_refinement_tags_match_oracle(gpu_ctx, ndim):
     """aph_refinement_tag on an evolved blast state (prim with ghosts from the plain form)

@@ -265,3 +265,20 @@ def test_native_remap_equals_python_statement(seed):
             want = amr.remap_py(m, bs, u, new, nbs)
             assert got.tobytes() == want.tobytes()
             m, bs, u = new, nbs, got + 0.01 * rng.random(got.shape)
+
+
+def test_native_remap_refuses_a_short_state():
+    """The library reads nvar * ncell values per old block: a state array of another
+    length is refused before the call."""
+    m = _random_meshes()[0]
+    bs = m.block_set()
+    nbs = m.block_set()
+    u = np.ones(bs.field_size(5))
+    with pytest.raises(ValueError, match="cons has"):
+        amr.remap(m, bs, u[:-1], m, nbs)
+    other = amr.AmrMesh(m.ndim, m.nbase[:m.ndim], tuple(2 * v for v in m.nx[:m.ndim]), m.ng,
+                        m.xmin, m.xmax, sorted(m.refined), mesh_id=m.mesh_id,
+                        max_level=m.max_level)
+    assert len(other.level) == len(m.level)
+    with pytest.raises(ValueError, match="nx and ng"):
+        amr.remap(m, other.block_set(), np.ones(other.block_set().field_size(5)), m, nbs)

@@ -651,6 +651,7 @@ int aph_plan_set_amr(aph_context *ctx, aph_plan *p, const aph_amr_plan *amr,
     return set_error(ctx, APH_E_NOMEM, "aph_plan_set_amr: device allocation failed");
   APH_HIP_CHECK(ctx, hipMemcpy(p->d_region, p->face_region.data(),
                                sizeof(int32_t) * 6 * bs->nblocks, hipMemcpyHostToDevice));
+  APH_HIP_CHECK(ctx, hipMemset(p->d_err, 0, sizeof(int)));  // read by aph_plan_status
   if (hipMalloc(&p->d_amr_face, sizeof(aph_amr_face) * 6 * bs->nblocks) != hipSuccess ||
       hipMalloc(&p->d_amr_buf, sizeof(double) * std::max<int64_t>(buffer_doubles, 1)) !=
           hipSuccess)

@@ -52,6 +52,7 @@ _SIGS = {
     "aph_fill_ghosts": (_i32, [_vp, _vp, _i32, _i32, _vp]),
     "aph_estimate_timestep": (_i32, [_vp, _vp, _i32, _i32, _vp, _vp, _vp, _d, _vp]),
     "aph_global_sum": (_i32, [_vp, _vp, _vp, _i32, _vp]),
+    "aph_global_sum_noalloc": (_i32, [_vp, _vp, _vp, _i32, _vp]),
     "aph_amr_plan_create": (_i32, [_vp, _vp, _vp, _i64, _vp, _i64, _vp, _i64, _pp]),
     "aph_amr_plan_destroy": (_i32, [_vp, _vp]),
     "aph_amr_build": (_i32, [_vp, _i32, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,

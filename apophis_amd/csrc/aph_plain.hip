@@ -9,13 +9,25 @@
 //   aph_estimate_timestep            <- EstimateTimestep<euler>   hydro.cpp:351-420
 //   aph_global_sum                   <- GlobalQuantHst            hydro.cpp:746-771
 #include <algorithm>
+#include <mutex>
 
 #include "aph_device.hpp"
 #include "aph_march.hpp"
 #include "aph_runtime.hpp"
 
+// Per-block partial sums of aph_global_sum_noalloc: a fixed device array instead of an
+// allocation, one copy per GPU, held until the process ends.  Host code never passes
+// `scratch` itself (in host code the name stands in for the symbol, not a device
+// address); it passes the address hipGetSymbolAddress returned for the current device.
+static __device__ __align__(256) double scratch[aph::kNoAllocMaxBlocks];
+
 namespace aph {
 namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};     // device address of `scratch`, per device
+hipError_t g_scratch_err[kMaxDevices] = {};  // a failed lookup, per device
 
 constexpr int kThreads = 256;
 
@@ -301,6 +313,13 @@ __global__ void k_sum_partials(const double *__restrict__ partial, int n, double
   }
 }
 
+// The device address of this device's `scratch`, or null when its lookup failed.
+double *noalloc_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch_ptr[device];
+}
+
 // ---------------------------------------------------------------- dispatch helpers
 inline dim3 grid_for(int64_t work, int threads, int nblk, int64_t cap = 1024) {
   int64_t x = (work + threads - 1) / threads;
@@ -504,6 +523,62 @@ int aph_global_sum(aph_context *ctx, const aph_blockset *bs, const double *cons,
   APH_NV_DISPATCH(bs->nvar, (k_block_sum<NV><<<bs->nblocks, kThreads, 0, ctx->stream>>>(
                                 bs->d, bs->ndim, cons, var, ctx->d_scalar)));
   k_sum_partials<<<1, 1, 0, ctx->stream>>>(ctx->d_scalar, bs->nblocks, out);
+  APH_LAUNCH_CHECK(ctx);
+  return APH_OK;
+}
+
+}  // extern "C"
+
+namespace aph {
+
+// Set-up of aph_global_sum_noalloc, called by aph_context_create with the context's
+// device current: looks up this device's address of `scratch` once.
+int noalloc_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) return APH_E_INVALID;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return APH_OK;
+  void *ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) {
+    g_scratch_err[device] = e != hipSuccess ? e : hipErrorInvalidSymbol;
+    return APH_E_HIP;
+  }
+  g_scratch_ptr[device] = static_cast<double *>(ptr);
+  g_scratch_err[device] = hipSuccess;
+  return APH_OK;
+}
+
+}  // namespace aph
+
+extern "C" {
+
+int aph_global_sum_noalloc(aph_context *ctx, const aph_blockset *bs, const double *cons,
+                           int32_t var, double *out) {
+  if (!ctx) return APH_E_INVALID;
+  if (!bs || !out || var < 0 || var >= bs->nvar || (!cons && bs->nblocks > 0))
+    return set_error(ctx, APH_E_INVALID, "aph_global_sum_noalloc: bad arguments");
+  if (bs->nblocks > kNoAllocMaxBlocks)
+    return set_error(ctx, APH_E_INVALID,
+                     "aph_global_sum_noalloc: %d blocks exceed its fixed scratch of %d "
+                     "partial sums (use aph_global_sum)", bs->nblocks, kNoAllocMaxBlocks);
+  if (bs->nblocks == 0) {
+    k_set<<<1, 1, 0, ctx->stream>>>(out, 0.0);
+    APH_LAUNCH_CHECK(ctx);
+    return APH_OK;
+  }
+  double *partial = noalloc_scratch(ctx->device);
+  if (!partial)
+    return set_error(ctx, APH_E_HIP,
+                     "aph_global_sum_noalloc: no scratch address for device %d (%s)",
+                     ctx->device,
+                     ctx->device >= 0 && ctx->device < kMaxDevices
+                         ? hipGetErrorString(g_scratch_err[ctx->device])
+                         : "device index out of range");
+  // every block writes its own partial before the final sum reads them, so nothing
+  // left from an earlier call is read
+  APH_NV_DISPATCH(bs->nvar, (k_block_sum<NV><<<bs->nblocks, kThreads, 0, ctx->stream>>>(
+                                bs->d, bs->ndim, cons, var, partial)));
+  k_sum_partials<<<1, 1, 0, ctx->stream>>>(partial, bs->nblocks, out);
   APH_LAUNCH_CHECK(ctx);
   return APH_OK;
 }

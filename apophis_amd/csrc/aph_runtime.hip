@@ -51,6 +51,9 @@ int aph_context_create(void *stream, aph_context **out) {
     delete c;
     return APH_E_HIP;
   }
+  // aph_global_sum_noalloc's scratch address on this device; a failed lookup is kept
+  // and reported by that entry point alone, so every other entry point still works
+  (void)aph::noalloc_scratch_init(c->device);
   *out = c;
   return APH_OK;
 }

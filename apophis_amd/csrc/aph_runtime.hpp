@@ -33,6 +33,11 @@ namespace aph {
 
 int set_error(aph_context *ctx, int code, const char *fmt, ...);
 
+// aph_global_sum_noalloc's fixed scratch: partial sums for at most this many blocks
+// (512 KiB per device), and its per-device set-up (aph_plain.hip).
+constexpr int kNoAllocMaxBlocks = 65536;
+int noalloc_scratch_init(int device);
+
 #define APH_HIP_CHECK(ctx, call)                                                      \
   do {                                                                                \
     hipError_t e_ = (call);                                                           \

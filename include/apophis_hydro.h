@@ -156,6 +156,16 @@ int aph_estimate_timestep(aph_context *ctx, const aph_blockset *bs, int32_t firs
 /* GlobalQuantHst<Sum, var> (hydro.cpp:746-771): *out (device) = sum(cons[var]*vol). */
 int aph_global_sum(aph_context *ctx, const aph_blockset *bs, const double *cons,
                    int32_t var, double *out);
+/* The same sum without any allocation.  No counterpart in the reference: it is for
+   callers that cannot allocate device memory (code under HIP graph capture, tight
+   serving loops).  aph_global_sum allocates its per-block partial sums on first use;
+   this variant keeps them in a fixed device array inside the library (one per GPU,
+   65536 blocks = 512 KiB, held until the process ends), so it fails with APH_E_INVALID
+   for block sets of more than 65536 blocks.  Every call on one GPU shares that array:
+   calls that overlap on one GPU (e.g. on two streams) race and must be ordered by the
+   caller.  Same kernels and summation order as aph_global_sum, so the same bits. */
+int aph_global_sum_noalloc(aph_context *ctx, const aph_blockset *bs, const double *cons,
+                           int32_t var, double *out);
 
 /* Refinement criteria of the hydro package (hydro.cpp:303-333, src/refinement/), one
  * batched launch over the blocks: per block, the reference's reduction over its own index

@@ -248,7 +248,11 @@ __device__ __forceinline__ void sweep(const FusedArgs &a, const aph_block &bk, i
         lc[D] = m - t0[D];
         lc[A] = pa;
         lc[B] = pb;
-        const int l = (lc[2] * ty + lc[1]) * tx + lc[0];
+        // the x position is permuted within its row by the row index (tx a power of two):
+        // the x sweep's lanes write one cell of each of 64 rows, and rows 8 doubles apart
+        // would otherwise fall on 4 of the 32 LDS banks
+        const int row = lc[2] * ty + lc[1];
+        const int l = row * tx + ((tx & (tx - 1)) ? lc[0] : (lc[0] ^ ((row >> 1) & (tx - 1))));
         const int tcells = tx * ty * cfg.t[2];
         if constexpr (!LAST) {
 #pragma unroll

@@ -33,6 +33,7 @@ __device__ __forceinline__ void march(int f_lo, int f_hi, Fetch &&fetch, Convert
   double win[W][NF];
   double qlp[NF];
   double fv[NF];  // PPM: limited value of the face below the cell being reconstructed
+  double d2c[NF];  // PPM: second difference (ppm_d2) of the cell being reconstructed
   double raw[NR];
   // PPM: bit s is set when every field of window cell s is +-0 or 2^-900 <= |q| < 2^1000
   // (frexp exponent in [-899, 1000]); a face whose four cells all qualify may take
@@ -57,15 +58,22 @@ __device__ __forceinline__ void march(int f_lo, int f_hi, Fetch &&fetch, Convert
   }
   fetch(f_lo - 1 + GR, raw);
   if constexpr (RECON == RECON_PPM) {
-    // lower face of the first reconstructed cell (m = f_lo - 1): cells f_lo-3 .. f_lo
+    // lower face of the first reconstructed cell (m = f_lo - 1): cells f_lo-3 .. f_lo,
+    // with the second differences of cells f_lo-2 and f_lo-1 (the latter is carried)
+    double d2lo[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      d2lo[f] = ppm_d2(win[1][f], win[2][f], win[3][f]);
+      d2c[f] = ppm_d2(win[2][f], win[3][f], win[4][f]);
+    }
     if (((plain >> 1) & 0xFu) == 0xFu) {
 #pragma unroll
       for (int f = 0; f < NF; ++f)
-        fv[f] = ppm_face<true>(win[1][f], win[2][f], win[3][f], win[4][f]);
+        fv[f] = ppm_face<true>(win[1][f], win[2][f], win[3][f], win[4][f], d2lo[f], d2c[f]);
     } else {
 #pragma unroll
       for (int f = 0; f < NF; ++f)
-        fv[f] = ppm_face<false>(win[1][f], win[2][f], win[3][f], win[4][f]);
+        fv[f] = ppm_face<false>(win[1][f], win[2][f], win[3][f], win[4][f], d2lo[f], d2c[f]);
     }
   }
   for (int m = f_lo - 1; m <= f_hi; ++m) {
@@ -80,21 +88,28 @@ __device__ __forceinline__ void march(int f_lo, int f_hi, Fetch &&fetch, Convert
       // face m+1/2 once (cells m-1 .. m+2 = win[1..4]); cell m's own face m-1/2 was the
       // previous step's upper face
       plain = (plain >> 1) | (plain_cell(win[W - 1]) << (W - 1));
+      // second differences: cell m+1 (new), cell m (carried), cell m-1 (recomputed from
+      // the window: three operations, cheaper in registers than carrying it)
+      double d2n[NF];
+#pragma unroll
+      for (int f = 0; f < NF; ++f) d2n[f] = ppm_d2(win[2][f], win[3][f], win[4][f]);
       double up[NF];
       if (((plain >> 1) & 0xFu) == 0xFu) {
 #pragma unroll
         for (int f = 0; f < NF; ++f)
-          up[f] = ppm_face<true>(win[1][f], win[2][f], win[3][f], win[4][f]);
+          up[f] = ppm_face<true>(win[1][f], win[2][f], win[3][f], win[4][f], d2c[f], d2n[f]);
       } else {
 #pragma unroll
         for (int f = 0; f < NF; ++f)
-          up[f] = ppm_face<false>(win[1][f], win[2][f], win[3][f], win[4][f]);
+          up[f] = ppm_face<false>(win[1][f], win[2][f], win[3][f], win[4][f], d2c[f], d2n[f]);
       }
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
-        ppm_cell(win[0][f], win[1][f], win[2][f], win[3][f], win[4][f], fv[f], up[f], ql[f],
+        ppm_cell(win[0][f], win[1][f], win[2][f], win[3][f], win[4][f],
+                 ppm_d2(win[0][f], win[1][f], win[2][f]), d2c[f], d2n[f], fv[f], up[f], ql[f],
                  qr[f]);
         fv[f] = up[f];
+        d2c[f] = d2n[f];
       }
     } else {
 #pragma unroll

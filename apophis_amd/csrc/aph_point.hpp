@@ -148,12 +148,22 @@ APH_HD double div6_fast(double x) {
 APH_HD bool neg(double x) { return x < 0.0; }
 APH_HD double with_sign_of(double s, double m) { return neg(s) ? -m : m; }
 
+// The second difference of cell i, q_{i-1} + q_{i+1} - 2 q_i, in the operand order of
+// every d2qc* in ppm_simple.hpp:55-60 (so one value serves each face and cell that uses it).
+APH_HD double ppm_d2(double q_im1, double q_i, double q_ip1) {
+  return q_im1 + q_ip1 - 2.0 * q_i;
+}
+
 //
 // FAST = true evaluates the two divisions by 6 with div6_fast.  It is exact when every
 // q is +-0 or 2^-900 <= |q| < 2^1000 (a "plain" window): then every intermediate is a
 // multiple of 2^-1008 below 2^1007, so both dividends are 0 or inside div6_fast's range.
+// d2qc_im1, d2qc: ppm_d2 of cells i-1 and i.  The limited value (dph_tmp) is formed only
+// where the limiter selects it; the reference forms it always and then selects, which
+// gives the same bits.
 template <bool FAST = false>
-APH_HD double ppm_face(double q_im2, double q_im1, double q_i, double q_ip1) {
+APH_HD double ppm_face(double q_im2, double q_im1, double q_i, double q_ip1, double d2qc_im1,
+                       double d2qc) {
   const double C2 = 1.25;
   const double qa0 = (q_i - q_im1);
   const double qb0 = (q_ip1 - q_i);
@@ -161,25 +171,31 @@ APH_HD double ppm_face(double q_im2, double q_im1, double q_i, double q_ip1) {
   const double dd = 0.5 * qb0 + 0.5 * qa0;
   const double avg = 0.5 * (q_im1 + q_i);
   double dph = avg + (FAST ? div6_fast(dd_im1 - dd) : (dd_im1 - dd) / 6.0);
-  const double d2qc_im1 = q_im2 + q_i - 2.0 * q_im1;
-  const double d2qc = q_im1 + q_ip1 - 2.0 * q_i;
   const double qa_tmp = dph - q_im1;
   const double qb_tmp = q_i - dph;
-  const double qa = 3.0 * (q_im1 + q_i - 2.0 * dph);
-  double qd = 0.0;
-  if (neg(qa) == neg(d2qc_im1) && neg(qa) == neg(d2qc))
-    qd = with_sign_of(qa, fmin(C2 * fabs(d2qc_im1), fmin(C2 * fabs(d2qc), fabs(qa))));
-  const double dph_tmp = avg - (FAST ? div6_fast(qd) : qd / 6.0);
-  if (qa_tmp * qb_tmp < 0.0) dph = dph_tmp;
+  if (qa_tmp * qb_tmp < 0.0) {
+    const double qa = 3.0 * (q_im1 + q_i - 2.0 * dph);
+    double qd = 0.0;
+    if (neg(qa) == neg(d2qc_im1) && neg(qa) == neg(d2qc))
+      qd = with_sign_of(qa, fmin(C2 * fabs(d2qc_im1), fmin(C2 * fabs(d2qc), fabs(qa))));
+    dph = avg - (FAST ? div6_fast(qd) : qd / 6.0);
+  }
   return dph;
 }
 
+template <bool FAST = false>
+APH_HD double ppm_face(double q_im2, double q_im1, double q_i, double q_ip1) {
+  return ppm_face<FAST>(q_im2, q_im1, q_i, q_ip1, ppm_d2(q_im2, q_im1, q_i),
+                        ppm_d2(q_im1, q_i, q_ip1));
+}
+
+// d2qc_im1, d2qc, d2qc_ip1: ppm_d2 of cells i-1, i, i+1.  The extremum branch's quantities
+// (qe, the roundoff threshold and the division rho = qe/d2qf) are formed only where that
+// branch is taken; elsewhere the reference's rho is computed but never used.
 APH_HD void ppm_cell(double q_im2, double q_im1, double q_i, double q_ip1, double q_ip2,
-                     double dph, double dph_ip1, double &ql_ip1, double &qr_i) {
+                     double d2qc_im1, double d2qc, double d2qc_ip1, double dph,
+                     double dph_ip1, double &ql_ip1, double &qr_i) {
   const double C2 = 1.25;
-  const double d2qc_im1 = q_im2 + q_i - 2.0 * q_im1;
-  const double d2qc = q_im1 + q_ip1 - 2.0 * q_i;
-  const double d2qc_ip1 = q_i + q_ip2 - 2.0 * q_ip1;
   const double d2qf = 6.0 * (dph + dph_ip1 - 2.0 * q_i);
   double qr = dph;
   double ql = dph_ip1;
@@ -187,30 +203,36 @@ APH_HD void ppm_cell(double q_im2, double q_im1, double q_i, double q_ip1, doubl
   const double dqf_plus = ql - q_i;
   const double qa_tmp = dqf_minus * dqf_plus;
   const double qb_tmp = (q_ip1 - q_i) * (q_i - q_im1);
-  double qe = 0.0;
-  if (neg(d2qc_im1) == neg(d2qc) && neg(d2qc_im1) == neg(d2qc_ip1) &&
-      neg(d2qc_im1) == neg(d2qf))
-    qe = with_sign_of(d2qf, fmin(fmin(C2 * fabs(d2qc_im1), C2 * fabs(d2qc)),
-                                 fmin(C2 * fabs(d2qc_ip1), fabs(d2qf))));
-  const double qa = fmax(fabs(q_im1), fabs(q_im2));
-  const double qb = fmax(fmax(fabs(q_i), fabs(q_ip1)), fabs(q_ip2));
-  double rho = 0.0;
-  if (fabs(d2qf) > (1.0e-12) * fmax(qa, qb)) rho = qe / d2qf;
-  const double tmp_m = q_i - rho * dqf_minus;
-  const double tmp_p = q_i + rho * dqf_plus;
-  const double tmp2_m = q_i - 2.0 * dqf_plus;
-  const double tmp2_p = q_i + 2.0 * dqf_minus;
   if ((qa_tmp <= 0.0 || qb_tmp <= 0.0)) {
-    if (rho <= (1.0 - (1.0e-12))) {
-      qr = tmp_m;
-      ql = tmp_p;
+    const double qa = fmax(fabs(q_im1), fabs(q_im2));
+    const double qb = fmax(fmax(fabs(q_i), fabs(q_ip1)), fabs(q_ip2));
+    if (fabs(d2qf) > (1.0e-12) * fmax(qa, qb)) {
+      double qe = 0.0;
+      if (neg(d2qc_im1) == neg(d2qc) && neg(d2qc_im1) == neg(d2qc_ip1) &&
+          neg(d2qc_im1) == neg(d2qf))
+        qe = with_sign_of(d2qf, fmin(fmin(C2 * fabs(d2qc_im1), C2 * fabs(d2qc)),
+                                     fmin(C2 * fabs(d2qc_ip1), fabs(d2qf))));
+      const double rho = qe / d2qf;
+      if (rho <= (1.0 - (1.0e-12))) {
+        qr = q_i - rho * dqf_minus;
+        ql = q_i + rho * dqf_plus;
+      }
+    } else {  // rho = 0: the limited values are q_i -/+ 0 * dqf
+      qr = q_i - 0.0 * dqf_minus;
+      ql = q_i + 0.0 * dqf_plus;
     }
   } else {
-    if (fabs(dqf_minus) >= 2.0 * fabs(dqf_plus)) qr = tmp2_m;
-    if (fabs(dqf_plus) >= 2.0 * fabs(dqf_minus)) ql = tmp2_p;
+    if (fabs(dqf_minus) >= 2.0 * fabs(dqf_plus)) qr = q_i - 2.0 * dqf_plus;
+    if (fabs(dqf_plus) >= 2.0 * fabs(dqf_minus)) ql = q_i + 2.0 * dqf_minus;
   }
   qr_i = qr;
   ql_ip1 = ql;
+}
+
+APH_HD void ppm_cell(double q_im2, double q_im1, double q_i, double q_ip1, double q_ip2,
+                     double dph, double dph_ip1, double &ql_ip1, double &qr_i) {
+  ppm_cell(q_im2, q_im1, q_i, q_ip1, q_ip2, ppm_d2(q_im2, q_im1, q_i), ppm_d2(q_im1, q_i, q_ip1),
+           ppm_d2(q_i, q_ip1, q_ip2), dph, dph_ip1, ql_ip1, qr_i);
 }
 
 // One reconstruction over a window w[0..W-1] (cells m-GR .. m+GR) of field values.

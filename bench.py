@@ -407,7 +407,8 @@ def profile_run(args, ctx, hydro, problems, torch):
         return
     if args.profile_run == "graph":
         # the timed configuration: the headline workload's RK2 steps replayed from a
-        # captured hipGraph (5 steps per graph, replayed twice: 20 stage launches), with
+        # captured hipGraph (5 steps per graph, replayed three times: 30 stage launches, the
+        # first 10 a warm-up), with
         # the tile the autotuner picks for 32^3 blocks set directly, so no tuning
         # launches enter the profile
         p = problems.blast3d(nblk=args.nblk, n=32)
@@ -415,7 +416,7 @@ def profile_run(args, ctx, hydro, problems, torch):
         fd.set_tile(32, 8, 8, 8)
         fd.initial_dt()
         fd.capture(5)
-        for _ in range(2):
+        for _ in range(3):  # the first replay warms clocks and caches (read apart)
             fd.replay()
         torch.cuda.synchronize()
         _calibration_copies(torch)

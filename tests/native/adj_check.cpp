@@ -27,6 +27,27 @@ void chk_riemann_adj(int kind, const double *x, const double *fb, double *xb) {
   else
     hlle_adj(x, x + 5, x + 10, x + 12, fb, xb, xb + 5, xb + 10, xb + 12);
 }
+// The PPM march of the device kernels on one field along a pencil q[0..n): the limited
+// value of each face formed once and shared by the two cells beside it, each cell's second
+// difference formed once and carried (ppm_d2), div6_fast when fast != 0.  Writes
+// (ql at face i+1, qr at face i) of cells i = 2 .. n-3 to out[2*(i-2) ..].
+void chk_ppm_march(const double *q, long n, int fast, double *out) {
+  if (n < 5) return;
+  auto face = [&](long i, double dm, double d0) {  // face i-1/2: cells i-2 .. i+1
+    return fast ? ppm_face<true>(q[i - 2], q[i - 1], q[i], q[i + 1], dm, d0)
+                : ppm_face<false>(q[i - 2], q[i - 1], q[i], q[i + 1], dm, d0);
+  };
+  double d2c = ppm_d2(q[1], q[2], q[3]);                        // cell 2
+  double fv = face(2, ppm_d2(q[0], q[1], q[2]), d2c);            // face 3/2
+  for (long m = 2; m <= n - 3; ++m) {
+    const double d2n = ppm_d2(q[m], q[m + 1], q[m + 2]);        // cell m+1
+    const double up = face(m + 1, d2c, d2n);                     // face m+1/2
+    ppm_cell(q[m - 2], q[m - 1], q[m], q[m + 1], q[m + 2], ppm_d2(q[m - 2], q[m - 1], q[m]),
+             d2c, d2n, fv, up, out[2 * (m - 2)], out[2 * (m - 2) + 1]);
+    fv = up;
+    d2c = d2n;
+  }
+}
 // u[6] -> (w[6], gamma[2]) ; eos = (gm1, cv, dfloor, pfloor)
 void chk_c2p(const double *u, const double *e, double *out) {
   aph_eos eos{e[0], e[1], e[2], e[3]};

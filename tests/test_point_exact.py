@@ -105,6 +105,41 @@ def test_face_shared_ppm_is_bitwise_ppm(lib):
         assert tuple(out) == orc.ppm(*q)
 
 
+def _pencils(rng):
+    """Pencils of the kinds the march meets: smooth, noisy, plateaus and exact extrema,
+    jumps, zeros and sign changes, and values a few ulps apart (the adiabatic-index
+    fields), at several scales."""
+    n = 40
+    x = np.arange(n) / n
+    kinds = [
+        np.sin(2 * np.pi * x * rng.uniform(0.5, 3)) * rng.uniform(0.1, 10),
+        1.0 + 0.01 * rng.uniform(-1, 1, n),
+        np.where(x < rng.uniform(0.2, 0.8), 1.0, 0.125),
+        np.repeat(rng.normal(size=n // 4), 4),
+        rng.normal(size=n) * (rng.random(n) < 0.5),
+        (5.0 / 3.0) * (1 + rng.integers(-4, 5, n) * 2.0 ** -52),
+        np.round(rng.normal(size=n) * 4) / 4,
+        rng.normal(size=n) * rng.choice([1e-8, 1.0, 1e8]),
+    ]
+    return kinds
+
+
+@pytest.mark.parametrize("fast", [0, 1])
+def test_ppm_march_sharing_matches_oracle_bitwise(lib, fast):
+    """The device march's form of PPM -- each face's limited value formed once for both
+    cells beside it, each cell's second difference formed once and carried to the next
+    step, the limiter-only arithmetic formed only where its branch is taken -- equals the
+    oracle's ppm (ppm_simple.hpp:31-161) cell by cell, bit for bit."""
+    rng = np.random.default_rng(29)
+    for _ in range(300):
+        for q in _pencils(rng):
+            q = np.ascontiguousarray(q, dtype=np.float64)
+            out = np.zeros(2 * (q.size - 4))
+            lib.chk_ppm_march(_p(q), ctypes.c_long(q.size), fast, _p(out))
+            want = np.array([orc.ppm(*q[i - 2:i + 3]) for i in range(2, q.size - 2)]).ravel()
+            assert _bits(out).tolist() == _bits(want).tolist()
+
+
 def _face_states(rng):
     """Left/right primitive states and adiabatic indices covering both signs of the
     contact speed, one- and two-sided shocks (pstar above one or both pressures),

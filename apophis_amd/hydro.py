@@ -375,9 +375,11 @@ class ApophisDriver:
     package's integrator (rk1 / rk2 / rk3 / vl2).
 
     pack_size: blocks per partition (None = all blocks in one pack, the default
-    Parthenon partitioning); each task is issued once per partition."""
+    Parthenon partitioning); each task is issued once per partition.  partitions: an
+    explicit list of (first_block, nblocks) ranges instead (e.g. one pack per block size,
+    the reference's packs needing blocks of one shape, hydro.cpp:537-549)."""
 
-    def __init__(self, pkg, md, g=(0.0, 0.0, 0.0), pack_size=None, amr=None):
+    def __init__(self, pkg, md, g=(0.0, 0.0, 0.0), pack_size=None, amr=None, partitions=None):
         self.pkg = pkg
         self.md = md
         self.amr = amr  # amr.AmrPlan of a multilevel mesh, or None
@@ -386,9 +388,21 @@ class ApophisDriver:
         self.g = tuple(float(v) for v in g)
         self.u1 = torch.zeros_like(md.cons)
         nb = md.bs.nblocks
-        ps = pack_size or max(nb, 1)
-        # an empty block set is one empty partition
-        self.parts = [md.pack(s, min(ps, nb - s)) for s in range(0, nb, ps)] or [md.pack(0, 0)]
+        if partitions is not None:
+            cover = sorted((int(f), int(c)) for f, c in partitions)
+            at = 0
+            for f, c in cover:
+                if f != at or c < 1:
+                    raise ValueError("partitions must cover the blocks once, in ranges")
+                at = f + c
+            if at != nb:
+                raise ValueError("partitions must cover the blocks once, in ranges")
+            self.parts = [md.pack(f, c) for f, c in partitions] or [md.pack(0, 0)]
+        else:
+            ps = pack_size or max(nb, 1)
+            # an empty block set is one empty partition
+            self.parts = ([md.pack(s, min(ps, nb - s)) for s in range(0, nb, ps)] or
+                          [md.pack(0, 0)])
         self.dt_dev = torch.empty(1, dtype=torch.float64, device=md.cons.device)
         self.rk = INTEGRATORS[pkg.param("integrator").name]
 

@@ -152,7 +152,7 @@ def _fourier_field(seed, ctr0, x, y, z, ndim):
 
 def uneven_blocks(nblocks=4096, sizes=(8, 16, 32), probs=(0.5, 0.35, 0.15), ng=4,
                   seed=7, recon="ppm", riemann="hllc", cfl=0.3, nsteps=50,
-                  jump_frac=0.3, ndim=3):
+                  jump_frac=0.3, ndim=3, by_size=False):
     """Independent cubic blocks of uneven edge (uneven AMR-like block set).
 
     Builder-defined rules (the reference has no uneven packs): every block spans the
@@ -160,23 +160,28 @@ def uneven_blocks(nblocks=4096, sizes=(8, 16, 32), probs=(0.5, 0.35, 0.15), ng=4
     global dt is the minimum over all blocks.  Each block holds a smooth field from 4
     seeded Fourier modes (rho in [0.5,2], p in [0.1,1], |u| <= 1) and, in jump_frac of
     the blocks, planar discontinuities at a random orientation (density x4, pressure x10
-    inside a slab bounded by two parallel planes, see below)."""
+    inside a slab bounded by two parallel planes, see below).
+
+    by_size: the same blocks (data keyed by the block's draw index, kept as mesh_id) in
+    the order of their edge, so that blocks of one size form one contiguous range, as the
+    reference's packs of equal-shape blocks need."""
     gm1 = 2.0 / 3.0
     r = uniform(seed, np.arange(nblocks, dtype=np.uint64))
     cum = np.cumsum(probs)
     which = np.searchsorted(cum, r * cum[-1], side="right")
     edges = np.asarray(sizes)[np.minimum(which, len(sizes) - 1)]
+    order = np.argsort(edges, kind="stable") if by_size else np.arange(nblocks)
     meshes = []
-    for b in range(nblocks):
+    for b in order:
         n = int(edges[b])
         nx = (n, n if ndim > 1 else 1, n if ndim > 2 else 1)
         meshes.append(lattice_mesh(ndim, (1, 1, 1), nx, ng, (0.0, 0.0, 0.0),
-                                   (1.0, 1.0, 1.0), ["periodic"] * 6, mesh_id=b))
+                                   (1.0, 1.0, 1.0), ["periodic"] * 6, mesh_id=int(b)))
     bs = BlockSet(ndim, meshes)
     cons = np.zeros(bs.field_size(bs.nvar))
-    for b in range(nblocks):
-        x, y, z = bs.cell_centers(b)
-        base = np.uint64(1_000_000 + 256 * b)
+    for slot, b in enumerate(order):
+        x, y, z = bs.cell_centers(slot)
+        base = np.uint64(1_000_000 + 256 * int(b))
         rho = 1.25 + 0.75 * _fourier_field(seed, base, x, y, z, ndim)
         prs = 0.55 + 0.45 * _fourier_field(seed, base + np.uint64(40), x, y, z, ndim)
         vel = [_fourier_field(seed, base + np.uint64(80 + 40 * d), x, y, z, ndim) / np.sqrt(3.0)
@@ -195,10 +200,10 @@ def uneven_blocks(nblocks=4096, sizes=(8, 16, 32), probs=(0.5, 0.35, 0.15), ng=4
             rho = np.where(side, rho * 4.0, rho)
             prs = np.where(side, prs * 10.0, prs)
         ek = 0.5 * rho * (vel[0] ** 2 + vel[1] ** 2 + vel[2] ** 2)
-        _set_interior(bs, cons, b, rho, rho * vel[0], rho * vel[1], rho * vel[2],
+        _set_interior(bs, cons, slot, rho, rho * vel[0], rho * vel[1], rho * vel[2],
                       prs / gm1 + ek)
     return Problem("uneven", bs, cons, make_eos(gm1), recon, riemann, cfl,
-                   nsteps=nsteps, meta={"edges": edges, "seed": seed})
+                   nsteps=nsteps, meta={"edges": edges[order], "seed": seed})
 
 
 # ---------------------------------------------------------------- KH

@@ -89,31 +89,65 @@ def event_time_stage(fd, torch, reps=10):
     return float(np.mean(times)), float(np.min(times))
 
 
-def cpu_baseline(threads=None):
-    """Oracle (C restatement, OpenMP over blocks) on a bounded sample of the workload:
-    a 128^3 periodic blast (64 blocks of 32^3, PPM+HLLC), RK2 steps until ~10 s of
-    CPU time have elapsed (at most 40 steps)."""
-    from apophis_amd import problems
+def _cpu_info():
+    """(model name, physical cores) of the host from lscpu, or (None, None)."""
+    import subprocess
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+    except (OSError, subprocess.SubprocessError):
+        return None, None
+    d = {}
+    for line in out.splitlines():
+        k, _, v = line.partition(":")
+        d[k.strip()] = v.strip()
+    try:
+        phys = int(d.get("Core(s) per socket", "0")) * int(d.get("Socket(s)", "0")) or None
+    except ValueError:
+        phys = None
+    return d.get("Model name"), phys
+
+
+def _time_oracle(p, threads, max_steps, budget_s):
+    """Oracle RK2 steps of problem p at its initial CFL dt on `threads` OpenMP threads:
+    one untimed warm-up step, then steps until max_steps or budget_s seconds."""
     from oracle import oracle as orc
 
-    ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    threads = threads or min(ncpu, 16)
     orc.set_threads(threads)
-    p = problems.blast3d(nblk=4, n=32)
     st = orc.State(p)
     dt = st.estimate_dt()
-    st.step(dt)  # warm-up
-    nsteps = 0
+    st.step(dt)
+    n = 0
     t0 = time.perf_counter()
-    while nsteps < 40 and time.perf_counter() - t0 < 10.0:
+    while n < max_steps and time.perf_counter() - t0 < budget_s:
         st.step(dt)
-        nsteps += 1
+        n += 1
     el = time.perf_counter() - t0
-    cells = p.bs.interior_cells()
-    return {"value": cells * nsteps / el, "unit": "zone-cycles/s", "cores": threads,
-            "kind": "port",
-            "sample": f"oracle rk2 step, 128^3 periodic blast (64 blocks of 32^3, PPM+HLLC), "
-                      f"{nsteps} steps after 1 warm-up, {el:.1f} s, OMP threads={threads}"}
+    return p.bs.interior_cells() * n / el, n, el
+
+
+def cpu_baseline():
+    """The oracle -- the C restatement of the reference's flux path, built with gcc -O3
+    -ffp-contract=off and OpenMP over blocks -- on the headline workload itself (256^3
+    blast, 512 blocks of 32^3, PPM+HLLC), with every thread of this GPU's CPU share (the
+    affinity set, capped by the OMP_NUM_THREADS the machine sets), and single-threaded on
+    a 128^3 sample of the same blocks (64 of them), the reference's default of Kokkos
+    Serial per rank."""
+    from apophis_amd import problems
+
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    threads = min(aff, int(env)) if env.isdigit() and int(env) > 0 else aff
+    model, phys = _cpu_info()
+    v_all, n_all, el_all = _time_oracle(problems.blast3d(nblk=8, n=32), threads, 6, 12.0)
+    v_one, n_one, el_one = _time_oracle(problems.blast3d(nblk=4, n=32), 1, 4, 12.0)
+    return {"value": v_all, "unit": "zone-cycles/s", "cores": threads, "kind": "port",
+            "sample": f"oracle rk2 steps of the 256^3 blast itself (512 blocks of 32^3, "
+                      f"PPM+HLLC), {n_all} steps after 1 warm-up, {el_all:.1f} s, "
+                      f"{threads} OpenMP threads",
+            "cpu_model": model, "host_physical_cores": phys, "affinity_cpus": aff,
+            "single_thread": {"value": v_one, "unit": "zone-cycles/s", "cores": 1,
+                              "sample": f"128^3 blast (64 blocks of 32^3), {n_one} steps "
+                                        f"after 1 warm-up, {el_one:.1f} s"}}
 
 
 def adjoint_leg(ctx, hydro, problems, torch):
@@ -601,12 +635,33 @@ def main():
             dt = dq.step(dt)
         torch.cuda.synchronize()
         vp = qc * npl / (time.perf_counter() - t0)
+        del dq, mq
+        # the plain form with one pack per block size (the reference's packs need blocks
+        # of one shape): the same blocks ordered by size, three partitions
+        qs = problems.uneven_blocks(nblocks=args.uneven_blocks, by_size=True)
+        ed = np.asarray(qs.meta["edges"])
+        cuts = [0] + [int(i) + 1 for i in np.flatnonzero(np.diff(ed))] + [len(ed)]
+        runs = [(a, b - a) for a, b in zip(cuts, cuts[1:])]
+        ms = hydro.MeshData(ctx, qs.bs, qs.cons)
+        ds = hydro.ApophisDriver(pkg, ms, partitions=runs)
+        dt = ds.initialize()
+        ds.step(dt)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(npl):
+            dt = ds.step(dt)
+        torch.cuda.synchronize()
+        vs = qc * npl / (time.perf_counter() - t0)
         extras["uneven_blocks"] = {"blocks": q.bs.nblocks, "cells": qc, "tiles": uneven_tiles,
                                    "batched_value": vb, "plain_per_block_value": vp,
-                                   "batched_over_plain": vb / vp}
-        print(f"# uneven: batched {vb:.4e}, plain per-block {vp:.4e}", file=sys.stderr,
-              flush=True)
-        del dq, mq
+                                   "batched_over_plain": vb / vp,
+                                   "plain_size_class_packs": {str(int(ed[a])): n
+                                                              for a, n in runs},
+                                   "plain_per_size_class_value": vs,
+                                   "batched_over_plain_per_size_class": vb / vs}
+        print(f"# uneven: batched {vb:.4e}, plain per-block {vp:.4e}, plain per size class "
+              f"{vs:.4e}", file=sys.stderr, flush=True)
+        del ds, ms
 
     if not args.no_extras:
         extras["two_level_blast2d"] = two_level_leg(ctx, hydro, problems, torch)

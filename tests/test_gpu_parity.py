@@ -219,6 +219,31 @@ def test_pack_size_one_equals_single_pack(gpu_ctx):
     assert torch.equal(d1.md.cons, d2.md.cons)
 
 
+def test_size_class_packs_match_oracle(gpu_ctx):
+    """The plain form with one pack per block size (blocks ordered by size; the reference's
+    packs hold blocks of one shape) equals the oracle's step bitwise."""
+    hydro = _hydro()
+    p = problems.uneven_blocks(nblocks=12, sizes=(8, 16), probs=(0.6, 0.4), by_size=True)
+    ed = np.asarray(p.meta["edges"])
+    cuts = [0] + [int(i) + 1 for i in np.flatnonzero(np.diff(ed))] + [len(ed)]
+    runs = [(a, b - a) for a, b in zip(cuts, cuts[1:])]
+    assert len(runs) == 2
+    print("[case] size-class packs", flush=True)
+    pkg = hydro.initialize_hydro(_pin(p))
+    md = hydro.MeshData(gpu_ctx, p.bs, p.cons)
+    drv = hydro.ApophisDriver(pkg, md, partitions=runs)
+    ref = orc.State(p)
+    dt = drv.initialize()
+    assert dt == ref.estimate_dt()
+    drv.step(dt)
+    ref.step(dt)
+    torch.cuda.synchronize()
+    mask = np.tile(p.bs.interior_mask(), p.bs.nvar)
+    assert np.array_equal(md.cons.cpu().numpy()[mask], ref.u0[mask])
+    with pytest.raises(ValueError, match="cover"):
+        hydro.ApophisDriver(pkg, md, partitions=runs[:1])
+
+
 def test_global_sum_matches_oracle(gpu_ctx):
     hydro = _hydro()
     p = problems.uneven_blocks(nblocks=20, seed=4)

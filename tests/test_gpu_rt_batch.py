@@ -148,5 +148,7 @@ def test_width_gradient_vs_central_fd(gpu_ctx, width):
                 fd[i, k] = (lp - lm) / (2 * h)
         errs[h] = np.abs(grad - fd).max() / np.abs(fd).max()
         print(f"  h={h:g}: loss {loss:.10e} max rel {errs[h]:.2e}", flush=True)
-    assert errs[1e-7] <= 1e-4
+    # sharp interface: 7e-7 at h = 1e-7; the smooth one (the limiter acts across its
+    # whole width): 2.7e-5 at h = 1e-7; both fall as h does
+    assert errs[1e-7] <= (1e-6 if width == 0.0 else 1e-4)
     assert errs[1e-7] < errs[1e-5] < errs[1e-4]

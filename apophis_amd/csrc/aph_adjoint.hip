@@ -70,7 +70,8 @@ struct AdjArgs {
   double *lam_n;
   const double *dt;
   double gam0, gam1, beta;
-  int stage;
+  int stage;                   // 1: u_n is u_in (a step's first stage); > 1: lam_n written
+  int lam_n_add;               // lam_n += gam1 L_out instead of lam_n = gam1 L_out
   double *pbar;                // per axis NF x ncells, pack layout (axis d at d * pbar_axis)
   int64_t pbar_axis;
   int64_t qb_axis;             // per axis offset of qlb / qrb (the axes run concurrently)
@@ -949,9 +950,11 @@ __global__ void __launch_bounds__(256) k_adj_gather(AdjArgs a) {
     cons_to_prim_adj<NV>(u, a.eos, pbv, pbv + NV, acc);
 #pragma unroll
     for (int v = 0; v < NV; ++v) a.lam_in[o + v * nc] = acc[v];
-    if (a.lam_n && a.stage == 2) {
+    if (a.lam_n && a.stage > 1) {
 #pragma unroll
-      for (int v = 0; v < NV; ++v) a.lam_n[o + v * nc] = a.gam1 * lo[v];
+      for (int v = 0; v < NV; ++v)
+        a.lam_n[o + v * nc] = a.lam_n_add ? a.gam1 * lo[v] + a.lam_n[o + v * nc]
+                                          : a.gam1 * lo[v];
     }
   }
 }
@@ -1002,8 +1005,13 @@ AxisKernels pick_adj_axis(int recon, int rs, int nv, int nd, int d) {
   }
   APH_AX(RECON_PPM, RS_HLLC)
   APH_AX(RECON_PPM, RS_HLLE)
+  APH_AX(RECON_PPM, RS_LHLLC)
   APH_AX(RECON_PLM, RS_HLLC)
   APH_AX(RECON_PLM, RS_HLLE)
+  APH_AX(RECON_PLM, RS_LHLLC)
+  APH_AX(RECON_DC, RS_HLLC)   // vl2's first-order predictor stage
+  APH_AX(RECON_DC, RS_HLLE)
+  APH_AX(RECON_DC, RS_LHLLC)
 #undef APH_AX
   return k;
 }
@@ -1092,7 +1100,7 @@ int adj_prepare(aph_context *ctx, const aph_blockset *bs, int recon, AdjScratch 
 }
 
 void adj_release(AdjScratch &s) {
-  for (double *p : {s.pbar, s.lam1, s.lam0p, s.prim, s.prim_t, s.lam_t, s.qlb, s.qrb})
+  for (double *p : {s.pbar, s.lam1, s.lam2, s.lam0p, s.prim, s.prim_t, s.lam_t, s.qlb, s.qrb})
     if (p) (void)hipFree(p);
   for (auto &p : s.spill)
     if (p) (void)hipFree(p);
@@ -1101,16 +1109,16 @@ void adj_release(AdjScratch &s) {
 }
 
 int adj_stage(aph_context *ctx, const aph_blockset *bs, int recon, int rs, const aph_eos &eos,
-              const double *g, int grav, AdjScratch &s, int stage, const double *u_in,
+              const double *g, int grav, AdjScratch &s, const AdjCoef &c, const double *u_in,
               const double *dt_dev, const double *lam_out, const double *lam_acc,
-              double *lam_in, double *lam_n) {
+              double *lam_in, double *lam_n, int lam_n_add) {
   if (bs->nblocks == 0) return APH_OK;
-  int rc = adj_prepare(ctx, bs, recon, s);
+  int rc = adj_prepare(ctx, bs, c.plan_recon, s);
   if (rc) return rc;
   const int nd = bs->ndim;
   AxisKernels ks[3];
   for (int d = 0; d < nd; ++d) {
-    ks[d] = pick_adj_axis(recon, rs, bs->nvar, nd, d);
+    ks[d] = pick_adj_axis(recon, rs, bs->nvar, nd, d);  // the stage's reconstruction
     if (!ks[d].face && !ks[d].axis)
       return set_error(ctx, APH_E_INVALID, "adjoint: unsupported combination");
   }
@@ -1128,10 +1136,11 @@ int adj_stage(aph_context *ctx, const aph_blockset *bs, int recon, int rs, const
   a.lam_in = lam_in;
   a.lam_n = lam_n;
   a.dt = dt_dev;
-  a.gam0 = stage == 1 ? 0.0 : 0.5;
-  a.gam1 = stage == 1 ? 1.0 : 0.5;
-  a.beta = stage == 1 ? 1.0 : 0.5;
-  a.stage = stage;
+  a.gam0 = c.gam0;
+  a.gam1 = c.gam1;
+  a.beta = c.beta;
+  a.stage = c.stage;
+  a.lam_n_add = lam_n_add;
   a.pbar = s.pbar;
   a.pbar_axis = (int64_t)(bs->nvar + 2) * bs->ncells;
   a.qb_axis = (int64_t)(bs->nvar + 2) * bs->ncells;

@@ -446,7 +446,12 @@ APH_HD void sound_speed_adj(double g, double ig, double rho, double p, double cb
   pb -= dlb * t / p;
 }
 
-// ------------------------------------------------------------------ HLLC
+// ------------------------------------------------------------------ HLLC / LHLLC
+// LOWMACH: the LHLLC solver (hydro_lhllc.hpp:70-79), whose PVRS term is scaled by
+// phi = chi (2 - chi), chi = max(|v_L|, |v_R|); its derivative goes to the side that
+// attains the max (the forward's std::max picks the left one on ties), and |v| = 0 has
+// none.
+template <bool LOWMACH = false>
 APH_HD void hllc_adj(const double *wl, const double *wr, const double *ifl, const double *ifr,
                      const double *flxb, double *wlb, double *wrb, double *iflb,
                      double *ifrb) {
@@ -463,7 +468,16 @@ APH_HD void hllc_adj(const double *wl, const double *wr, const double *ifl, cons
   const double rhoa = .5 * (wl[0] + wr[0]);
   const double ca = .5 * (cl + cr);
   const double h = .5 * (wr[1] - wl[1]);
-  const double pstar = .5 * (wl[4] + wr[4]) - h * rhoa * ca;
+  double pvrs = h * rhoa * ca;
+  double machl = 0.0, machr = 0.0, chi = 0.0, phi = 1.0;
+  if constexpr (LOWMACH) {
+    machl = sqrt(wl[1] * wl[1] + wl[2] * wl[2] + wl[3] * wl[3]);
+    machr = sqrt(wr[1] * wr[1] + wr[2] * wr[2] + wr[3] * wr[3]);
+    chi = mx(machl, machr);
+    phi = chi * (2.0 - chi);
+    pvrs = pvrs * phi;
+  }
+  const double pstar = .5 * (wl[4] + wr[4]) - pvrs;
   const double K = (gl + 1) / (2 * gr);
   const bool shl = !(pstar <= wl[4]), shr = !(pstar <= wr[4]);
   const double ql = shl ? sqrt(1.0 + K * (pstar / wl[4] - 1.0)) : 1.0;
@@ -599,12 +613,24 @@ APH_HD void hllc_adj(const double *wl, const double *wr, const double *ifl, cons
     wrb[4] -= Xb * K * pstar / (wr[4] * wr[4]);
   }
   double glb = Kb / (2 * gr), grb = -Kb * (gl + 1) / (2 * gr * gr);
-  // pstar = .5 (pl + pr) - h * rhoa * ca
+  // pstar = .5 (pl + pr) - h * rhoa * ca [* phi]
   wlb[4] += .5 * pstarb;
   wrb[4] += .5 * pstarb;
-  const double hb = -pstarb * rhoa * ca;
-  const double rhoab = -pstarb * h * ca;
-  const double cab = -pstarb * h * rhoa;
+  double rawb = -pstarb;  // cotangent of h * rhoa * ca
+  if constexpr (LOWMACH) {
+    const double phib = rawb * (h * rhoa * ca);
+    rawb = rawb * phi;
+    const double chib = phib * (2.0 - 2.0 * chi);
+    if (machl < machr) {
+      if (machr > 0.0)
+        for (int k = 1; k <= 3; ++k) wrb[k] += chib * wr[k] / machr;
+    } else if (machl > 0.0) {
+      for (int k = 1; k <= 3; ++k) wlb[k] += chib * wl[k] / machl;
+    }
+  }
+  const double hb = rawb * rhoa * ca;
+  const double rhoab = rawb * h * ca;
+  const double cab = rawb * h * rhoa;
   wrb[1] += .5 * hb;
   wlb[1] -= .5 * hb;
   wlb[0] += .5 * rhoab;
@@ -762,7 +788,9 @@ APH_HD void riemann_adj(const double *wl, const double *wr, const double *ifl,
                         const double *ifr, const double *flxb, double *wlb, double *wrb,
                         double *iflb, double *ifrb) {
   if constexpr (RS == RS_HLLC)
-    hllc_adj(wl, wr, ifl, ifr, flxb, wlb, wrb, iflb, ifrb);
+    hllc_adj<false>(wl, wr, ifl, ifr, flxb, wlb, wrb, iflb, ifrb);
+  else if constexpr (RS == RS_LHLLC)
+    hllc_adj<true>(wl, wr, ifl, ifr, flxb, wlb, wrb, iflb, ifrb);
   else
     hlle_adj(wl, wr, ifl, ifr, flxb, wlb, wrb, iflb, ifrb);
 }

@@ -18,14 +18,23 @@ struct AdjScratch {
   double *spill[3] = {nullptr, nullptr, nullptr};
   int64_t *spill_base = nullptr;
   double *lam1 = nullptr, *lam0p = nullptr;
+  double *lam2 = nullptr;  // third cotangent register of a three-stage step (rk3)
   int64_t ncells = 0;
   int segcap[3] = {8, 8, 8};  // longest segment per axis (shorter when blocks are few)
 };
 
+// One stage of the plan's integrator: u_out = gam0 u_in + gam1 u_n + beta dt L(u_in), with
+// stage == 1 when u_n is u_in (the step's first stage); plan_recon sizes the scratch (the
+// split form's buffers when it is PPM), the recon argument of adj_stage is the stage's.
+struct AdjCoef {
+  double gam0, gam1, beta;
+  int stage;
+  int plan_recon;
+};
 int adj_stage(aph_context *ctx, const aph_blockset *bs, int recon, int rs, const aph_eos &eos,
-              const double *g, int grav, AdjScratch &s, int stage, const double *u_in,
+              const double *g, int grav, AdjScratch &s, const AdjCoef &c, const double *u_in,
               const double *dt_dev, const double *lam_out, const double *lam_acc,
-              double *lam_in, double *lam_n);
+              double *lam_in, double *lam_n, int lam_n_add);
 int adj_prepare(aph_context *ctx, const aph_blockset *bs, int recon, AdjScratch &s);
 void adj_release(AdjScratch &s);
 bool adj_supported(int recon, int rs, int nv, int nd);

@@ -149,6 +149,7 @@ KernelFn pick_kernel(int recon, int rs, int nv, int ndim) {
   APH_PICK(RECON_DC, RS_HLLC)
   APH_PICK(RECON_DC, RS_HLLE)
   APH_PICK(RECON_DC, RS_LHLLC)
+  APH_PICK(RECON_DC, RS_NONE)  // hydro switched off (hydro_none.hpp:17-35)
 #undef APH_PICK
   return nullptr;
 }
@@ -425,6 +426,29 @@ int plan_nstages(const aph_plan *p) {
   double g0, g1, b;
   int r;
   return stage_coef(p->integrator, 1, p->recon, g0, g1, b, r);
+}
+
+// The adjoint's view of stage s of the plan's integrator (its coefficients and
+// reconstruction); 0 when s is out of range.
+int adj_coef(const aph_plan *p, int s, AdjCoef &c, int &srecon) {
+  double g0, g1, b;
+  const int n = stage_coef(p->integrator, s, p->recon, g0, g1, b, srecon);
+  if (!n) return 0;
+  c = AdjCoef{g0, g1, b, s, p->recon};
+  return n;
+}
+
+// The adjoint entry points' checks: a single-level plan whose reconstruction / Riemann
+// pair (and vl2's first-order predictor) the adjoint kernels cover.
+int adj_plan_ok(aph_context *ctx, const aph_plan *p) {
+  if (p->amr) return set_error(ctx, APH_E_INVALID, "adjoint: multilevel meshes are not supported");
+  if (!adj_supported(p->recon, p->rsolver, p->bs->nvar, p->bs->ndim) ||
+      (p->integrator == APH_INTEGRATOR_VL2 &&
+       !adj_supported(RECON_DC, p->rsolver, p->bs->nvar, p->bs->ndim)))
+    return set_error(ctx, APH_E_INVALID, "adjoint: unsupported (recon, riemann, nvar)");
+  if (p->integrator == APH_INTEGRATOR_RK3 && !p->d_u2)
+    return set_error(ctx, APH_E_STATE, "adjoint: rk3 plan without its third register");
+  return APH_OK;
 }
 
 // Candidate tiles for one block shape: divisors of the block edges, x extent at least
@@ -970,58 +994,81 @@ int aph_adjoint_stage(aph_context *ctx, aph_plan *p, int32_t stage, const double
                       const double *dt_dev, const double *lam_out, const double *lam_acc,
                       double *lam_in, double *lam_n) {
   if (!ctx) return APH_E_INVALID;
-  if (p && p->integrator != APH_INTEGRATOR_RK2)
-    return set_error(ctx, APH_E_INVALID, "adjoint: the plan's integrator must be rk2");
-  if (!p || !u_in || !dt_dev || !lam_out || !lam_in || (stage != 1 && stage != 2))
+  if (!p || !u_in || !dt_dev || !lam_out || !lam_in)
     return set_error(ctx, APH_E_INVALID, "aph_adjoint_stage: bad arguments");
-  if (p->amr)
-    return set_error(ctx, APH_E_INVALID, "adjoint: multilevel meshes are not supported");
+  int rc = adj_plan_ok(ctx, p);
+  if (rc) return rc;
   if (lam_in == lam_out || (lam_acc && lam_acc == lam_in))
     return set_error(ctx, APH_E_INVALID, "aph_adjoint_stage: lam_in must not alias inputs");
-  if (!adj_supported(p->recon, p->rsolver, p->bs->nvar, p->bs->ndim))
-    return set_error(ctx, APH_E_INVALID, "adjoint: unsupported (recon, riemann, nvar)");
-  return adj_stage(ctx, p->bs, p->recon, p->rsolver, p->eos, p->g, p->grav, p->adj, stage,
-                   u_in, dt_dev, lam_out, lam_acc, lam_in, lam_n);
+  AdjCoef c;
+  int sr;
+  if (!adj_coef(p, stage, c, sr))
+    return set_error(ctx, APH_E_INVALID, "aph_adjoint_stage: stage %d out of range", stage);
+  return adj_stage(ctx, p->bs, sr, p->rsolver, p->eos, p->g, p->grav, p->adj, c, u_in,
+                   dt_dev, lam_out, lam_acc, lam_in, lam_n, 0);
 }
 
 int aph_adjoint_step(aph_context *ctx, aph_plan *p, double *u0, double *u1,
                      const double *dt_dev, double *lam) {
   if (!ctx) return APH_E_INVALID;
-  if (p && p->integrator != APH_INTEGRATOR_RK2)
-    return set_error(ctx, APH_E_INVALID, "adjoint: the plan's integrator must be rk2");
   if (!p || !u0 || !u1 || !dt_dev || !lam || u0 == u1)
     return set_error(ctx, APH_E_INVALID, "aph_adjoint_step: bad arguments");
-  if (p->amr)
-    return set_error(ctx, APH_E_INVALID, "adjoint: multilevel meshes are not supported");
-  if (!adj_supported(p->recon, p->rsolver, p->bs->nvar, p->bs->ndim))
-    return set_error(ctx, APH_E_INVALID, "adjoint: unsupported (recon, riemann, nvar)");
-  int rc = adj_prepare(ctx, p->bs, p->recon, p->adj);
+  int rc = adj_plan_ok(ctx, p);
   if (rc) return rc;
-  // recompute the stage-1 state from the step's checkpoint, then stage 2 and stage 1
-  rc = launch_stage(ctx, p, 1, u0, nullptr, u1, dt_dev, 0.0);
+  rc = adj_prepare(ctx, p->bs, p->recon, p->adj);
   if (rc) return rc;
+  // recompute the stage-1 state from the step's checkpoint, then reverse the stages
+  if (plan_nstages(p) > 1) {
+    rc = launch_stage(ctx, p, 1, u0, nullptr, u1, dt_dev, 0.0);
+    if (rc) return rc;
+  }
   return aph_adjoint_step_stored(ctx, p, u0, u1, dt_dev, lam);
 }
 
 int aph_adjoint_step_stored(aph_context *ctx, aph_plan *p, const double *u0, const double *u1,
                             const double *dt_dev, double *lam) {
   if (!ctx) return APH_E_INVALID;
-  if (p && p->integrator != APH_INTEGRATOR_RK2)
-    return set_error(ctx, APH_E_INVALID, "adjoint: the plan's integrator must be rk2");
   if (!p || !u0 || !u1 || !dt_dev || !lam || u0 == u1 || lam == u0 || lam == u1)
     return set_error(ctx, APH_E_INVALID, "aph_adjoint_step_stored: bad arguments");
-  if (p->amr)
-    return set_error(ctx, APH_E_INVALID, "adjoint: multilevel meshes are not supported");
-  if (!adj_supported(p->recon, p->rsolver, p->bs->nvar, p->bs->ndim))
-    return set_error(ctx, APH_E_INVALID, "adjoint: unsupported (recon, riemann, nvar)");
-  int rc = adj_prepare(ctx, p->bs, p->recon, p->adj);
+  int rc = adj_plan_ok(ctx, p);
   if (rc) return rc;
-  // stage 2 (input u1) then stage 1 (input u0)
-  rc = adj_stage(ctx, p->bs, p->recon, p->rsolver, p->eos, p->g, p->grav, p->adj, 2, u1,
-                 dt_dev, lam, nullptr, p->adj.lam1, p->adj.lam0p);
+  rc = adj_prepare(ctx, p->bs, p->recon, p->adj);
   if (rc) return rc;
-  return adj_stage(ctx, p->bs, p->recon, p->rsolver, p->eos, p->g, p->grav, p->adj, 1, u0,
-                   dt_dev, p->adj.lam1, p->adj.lam0p, lam, nullptr);
+  const aph_blockset *bs = p->bs;
+  AdjScratch &A = p->adj;
+  const int n = plan_nstages(p);
+  AdjCoef c[3];
+  int sr[3];
+  for (int s = 1; s <= n; ++s) adj_coef(p, s, c[s - 1], sr[s - 1]);
+  auto stage = [&](int s, const double *x, const double *lo, const double *acc, double *li,
+                   double *ln, int add) {
+    return adj_stage(ctx, bs, sr[s - 1], p->rsolver, p->eos, p->g, p->grav, A, c[s - 1], x,
+                     dt_dev, lo, acc, li, ln, add);
+  };
+  if (n == 1) {  // rk1: U0' = U0 + dt L(U0); lam must not be both input and output
+    APH_HIP_CHECK(ctx, hipMemcpyAsync(A.lam1, lam, sizeof(double) * bs->nvar * bs->ncells,
+                                      hipMemcpyDeviceToDevice, ctx->stream));
+    return stage(1, u0, A.lam1, nullptr, lam, nullptr, 0);
+  }
+  if (n == 2) {  // rk2, vl2: stage 2 (input u1), then stage 1 (input u0)
+    rc = stage(2, u1, lam, nullptr, A.lam1, A.lam0p, 0);
+    if (rc) return rc;
+    return stage(1, u0, A.lam1, A.lam0p, lam, nullptr, 0);
+  }
+  // rk3: the stage-2 state X3 = stage2(u1, u0) is recomputed into the plan's third
+  // register; then stage 3 (input X3), stage 2 (input u1), stage 1 (input u0), the
+  // gam1 parts of stages 3 and 2 accumulated into the cotangent of u0
+  if (!A.lam2 && hipMalloc(&A.lam2, sizeof(double) * bs->nvar * bs->ncells) != hipSuccess) {
+    A.lam2 = nullptr;
+    return set_error(ctx, APH_E_NOMEM, "adjoint: rk3 cotangent register allocation failed");
+  }
+  rc = launch_stage(ctx, p, 2, const_cast<double *>(u1), u0, p->d_u2, dt_dev, 0.0);
+  if (rc) return rc;
+  rc = stage(3, p->d_u2, lam, nullptr, A.lam1, A.lam0p, 0);
+  if (rc) return rc;
+  rc = stage(2, u1, A.lam1, nullptr, A.lam2, A.lam0p, 1);
+  if (rc) return rc;
+  return stage(1, u0, A.lam2, A.lam0p, lam, nullptr, 0);
 }
 
 }  // extern "C"

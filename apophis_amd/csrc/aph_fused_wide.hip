@@ -48,6 +48,7 @@ KernelFn pick_kernel_wide(int recon, int rs, int nv, int ndim) {
   APH_PICK(RECON_DC, RS_HLLC)
   APH_PICK(RECON_DC, RS_HLLE)
   APH_PICK(RECON_DC, RS_LHLLC)
+  APH_PICK(RECON_DC, RS_NONE)  // hydro switched off (hydro_none.hpp:17-35)
 #undef APH_PICK
   return nullptr;
 }

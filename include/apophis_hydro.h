@@ -339,7 +339,8 @@ int aph_plan_get_tile(aph_context *ctx, aph_plan *plan, int32_t block, aph_tile 
 /* Time integrator of the plan (default rk2): the upstream Parthenon low-storage
  * schemes rk1 / rk2 / rk3 / vl2 that ApophisDriver uses (apophis_driver.cpp:78-85,
  * hydro.cpp:114-139; vl2 predicts with DC reconstruction).  rk3 allocates one more
- * state register inside the plan.  Synchronous.  The adjoint entry points need rk2. */
+ * state register inside the plan.  Synchronous.  The adjoint entry points follow the
+ * plan's integrator. */
 int aph_plan_set_integrator(aph_context *ctx, aph_plan *plan, int32_t integrator);
 /* Stage `stage` (1..number of stages of the plan's integrator):
  * u_out = gam0*u_in + gam1*u_n + beta*dt*(-div F(u_in)) [+ gravity]; u_n (the step's
@@ -407,24 +408,28 @@ int aph_plan_set_wait_limit(aph_context *ctx, aph_plan *plan, int32_t polls);
 
 /* ----------------------------------------------------- adjoint (frozen dt)
  * Reverse-mode counterparts of the fused stage (no reference counterpart: the
- * reference has no adjoint).  Given the cotangent lam_out of a stage's output:
- *   stage 1:  lam_in = (0 + 1) * lam_out + J^T lam_out [+ lam_acc]
- *   stage 2:  lam_in = 0.5 * lam_out + J^T lam_out [+ lam_acc],  lam_n = 0.5 * lam_out
+ * reference has no adjoint).  Stage s of the plan's integrator computes
+ * u_out = gam0 u_in + gam1 u_n + beta dt L(u_in) (u_n = u_in in stage 1; vl2's stage 1
+ * reconstructs with DC).  Given the cotangent lam_out of a stage's output:
+ *   stage 1:  lam_in = (gam0 + gam1) * lam_out + beta dt J^T lam_out [+ lam_acc]
+ *   stage s > 1:  lam_in = gam0 * lam_out + beta dt J^T lam_out [+ lam_acc],
+ *                 lam_n = gam1 * lam_out
  * where J is the Jacobian of the flux-divergence (+ gravity) term with respect to the
  * stage input u_in (ConsToPrim, reconstruction through the limiter's active branch, the
- * Riemann solver's active wave branches).  Only interior cells of the cotangent arrays
- * are read or written.  lam_acc and lam_n may be NULL. */
+ * Riemann solver's active wave branches; HLLE, HLLC and LHLLC).  Only interior cells of
+ * the cotangent arrays are read or written.  lam_acc and lam_n may be NULL. */
 int aph_adjoint_stage(aph_context *ctx, aph_plan *plan, int32_t stage, const double *u_in,
                       const double *dt_dev, const double *lam_out, const double *lam_acc,
                       double *lam_in, double *lam_n);
-/* One RK2 step backwards: u0 is the step's input state (checkpoint), u1 scratch of the
- * same size (receives the recomputed stage-1 state), lam holds the cotangent of the
- * step's output on entry and that of u0 on return. */
+/* One step of the plan's integrator backwards (rk1, rk2, rk3, vl2): u0 is the step's
+ * input state (checkpoint), u1 scratch of the same size (receives the recomputed
+ * stage-1 state), lam holds the cotangent of the step's output on entry and that of u0
+ * on return.  rk3 recomputes its stage-2 state into the plan's third register. */
 int aph_adjoint_step(aph_context *ctx, aph_plan *plan, double *u0, double *u1,
                      const double *dt_dev, double *lam);
 /* The same with the step's stage-1 state u1 = stage1(u0) supplied by the caller (kept
- * when the forward step was taken or recomputed), so no stage is recomputed.  u0 and u1
- * are only read. */
+ * when the forward step was taken or recomputed), so the first stage is not recomputed
+ * (rk1 ignores u1; rk3 still recomputes its stage-2 state).  u0 and u1 are only read. */
 int aph_adjoint_step_stored(aph_context *ctx, aph_plan *plan, const double *u0,
                             const double *u1, const double *dt_dev, double *lam);
 

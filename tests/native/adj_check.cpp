@@ -23,7 +23,9 @@ void chk_riemann(int kind, const double *x, double *f) {
 }
 void chk_riemann_adj(int kind, const double *x, const double *fb, double *xb) {
   if (kind == RS_HLLC)
-    hllc_adj(x, x + 5, x + 10, x + 12, fb, xb, xb + 5, xb + 10, xb + 12);
+    hllc_adj<false>(x, x + 5, x + 10, x + 12, fb, xb, xb + 5, xb + 10, xb + 12);
+  else if (kind == RS_LHLLC)
+    hllc_adj<true>(x, x + 5, x + 10, x + 12, fb, xb, xb + 5, xb + 10, xb + 12);
   else
     hlle_adj(x, x + 5, x + 10, x + 12, fb, xb, xb + 5, xb + 10, xb + 12);
 }

@@ -92,7 +92,7 @@ def _states(rng):
     return np.concatenate([wl, wr, gam[:2], gam[2:]])
 
 
-@pytest.mark.parametrize("kind,code", [("hllc", 3), ("hlle", 2)])
+@pytest.mark.parametrize("kind,code", [("hllc", 3), ("hlle", 2), ("lhllc", 4)])
 def test_riemann_adjoint_matches_fd(lib, kind, code):
     rng = np.random.default_rng(2)
     f = lambda x: _call(lib.chk_riemann, 5, code, x)

@@ -30,7 +30,8 @@ def _mods():
 
 def make_stepper(ctx, p, dt):
     hydro, adjoint = _mods()
-    fd = hydro.FusedDriver(ctx, p.bs, p.cons, p.recon, p.riemann, p.eos, p.cfl, g=p.g)
+    fd = hydro.FusedDriver(ctx, p.bs, p.cons, p.recon, p.riemann, p.eos, p.cfl, g=p.g,
+                           integrator=p.integrator)
     return adjoint.Stepper(fd, dt)
 
 
@@ -44,7 +45,22 @@ def interior_t(p, u):
     return torch.from_numpy(np.ascontiguousarray(p.bs.interior(u, 0, p.bs.nvar)))
 
 
+def _with(p, **kw):
+    import dataclasses
+    return dataclasses.replace(p, **kw)
+
+
 REF_CASES = {
+    # LHLLC, and the integrators rk1 / rk3 / vl2 (vl2 predicts with DC)
+    "kh_ppm_lhllc": lambda: problems.kh2d(nblk=1, n=16, recon="ppm", riemann="lhllc"),
+    "kh_plm_lhllc_vl2": lambda: _with(problems.kh2d(nblk=1, n=16, riemann="lhllc"),
+                                      integrator="vl2"),
+    "kh_plm_hllc_rk1": lambda: _with(problems.kh2d(nblk=1, n=16), integrator="rk1"),
+    "kh_plm_hllc_rk3": lambda: _with(problems.kh2d(nblk=1, n=16), integrator="rk3"),
+    "kh_ppm_hlle_rk3": lambda: _with(problems.kh2d(nblk=1, n=16, recon="ppm", riemann="hlle"),
+                                     integrator="rk3"),
+    "kh_ppm_hllc_vl2": lambda: _with(problems.kh2d(nblk=1, n=16, recon="ppm"),
+                                     integrator="vl2"),
     "kh_plm_hllc": lambda: problems.kh2d(nblk=1, n=16),
     "kh_ppm_hlle": lambda: problems.kh2d(nblk=1, n=16, recon="ppm", riemann="hlle"),
     "uneven8_ppm_hllc_3d": lambda: problems.uneven_blocks(nblocks=1, sizes=(8,), probs=(1.0,),
@@ -69,7 +85,8 @@ def test_adjoint_matches_torch_autograd(gpu_ctx, name):
     u0 = interior_t(p, p.cons).clone().requires_grad_(True)
     u = u0
     for _ in range(nsteps):
-        u = torch_ref.rk2_step(u, dt, p.recon, p.riemann, eos, p.bs.blocks["dx"][0], p.bs.ndim)
+        u = torch_ref.step(u, dt, p.integrator, p.recon, p.riemann, eos, p.bs.blocks["dx"][0],
+                           p.bs.ndim)
     (u * torch.from_numpy(lam_int)).sum().backward()
     want = u0.grad.numpy()
     # GPU: forward rollout + adjoint steps through the C ABI
@@ -83,9 +100,16 @@ def test_adjoint_matches_torch_autograd(gpu_ctx, name):
     lv = np.zeros_like(p.cons)
     p.bs.interior(lv, 0, p.bs.nvar)[...] = lam_int
     lam.copy_(torch.from_numpy(lv))
+    lam2 = lam.clone()
+    stage1 = []
+    for n in range(nsteps):  # the stage-1 states, for the stored form
+        st.stage1_(states[n])
+        stage1.append(st.stage1().clone())
     for n in range(nsteps - 1, -1, -1):
         st.adjoint_step_(states[n], lam)
+        st.adjoint_step_(states[n], lam2, stage1=stage1[n])
     torch.cuda.synchronize()
+    assert torch.equal(lam, lam2)  # recomputed and stored stage-1 states: the same bits
     got = p.bs.interior(lam.cpu().numpy(), 0, p.bs.nvar)
     scale = np.abs(want).max(axis=(1, 2, 3), keepdims=True)
     err = np.max(np.abs(got - want) / np.maximum(scale, 1e-300))
@@ -290,8 +314,9 @@ def test_kh_c4_gradient_matches_central_fd_to_1e6(gpu_ctx):
                 out[k] = dl / (2 * h)
         return out
 
-    fds = {f"h={h:g}{' cellwise' if pr else ''}": central(h, pr)
-           for h in (1e-5, 1e-6, 1e-7) for pr in (False, True)}
+    fds = {f"h={hs}{' cellwise' if pr else ''}": central(h, pr)
+           for hs, h in (("1e-5", 1e-5), ("1e-6", 1e-6), ("1e-7", 1e-7))
+           for pr in (False, True)}
     errs = {k: float(np.abs(grad - v).max() / np.abs(v).max()) for k, v in fds.items()}
     print(f"  grad {grad}", flush=True)
     for k, v in fds.items():

@@ -139,7 +139,14 @@ def sound(g, ig, rho, p):
     return C_LIGHT * csqrt(g / (1 + (ig + rho * C2L / p)))
 
 
-def hllc(wl, wr, gl_, gr_):
+def _vmag(w):
+    """|v| with a zero derivative at v = 0 (as the hand-written LHLLC adjoint takes it)."""
+    s = w[1] * w[1] + w[2] * w[2] + w[3] * w[3]
+    pos = s > 0.0
+    return torch.where(pos, csqrt(torch.where(pos, s, torch.ones_like(s))), torch.zeros_like(s))
+
+
+def hllc(wl, wr, gl_, gr_, lowmach=False):
     gammal, gammar = gl_[0], gr_[0]
     igl = 1.0 / (gl_[1] - 1)
     igr = 1.0 / (gr_[1] - 1)
@@ -149,7 +156,11 @@ def hllc(wl, wr, gl_, gr_):
     er = wr[4] * igr + 0.5 * wr[0] * (wr[1] * wr[1] + wr[2] * wr[2] + wr[3] * wr[3])
     rhoa = .5 * (wl[0] + wr[0])
     ca = .5 * (cl + cr)
-    pstar = .5 * (wl[4] + wr[4]) - .5 * (wr[1] - wl[1]) * rhoa * ca
+    pvrs = .5 * (wr[1] - wl[1]) * rhoa * ca
+    if lowmach:  # hydro_lhllc.hpp:70-79: phi = chi (2 - chi), chi = max(|v_L|, |v_R|)
+        chi = mx(_vmag(wl), _vmag(wr))
+        pvrs = pvrs * (chi * (2.0 - chi))
+    pstar = .5 * (wl[4] + wr[4]) - pvrs
     K = (gammal + 1) / (2 * gammar)
     one = torch.ones_like(pstar)
     ql = torch.where(pstar <= wl[4], one,
@@ -218,6 +229,31 @@ def hlle(wl, wr, gl_, gr_):
     return [0.5 * (fl[n] + fr[n]) + (fl[n] - fr[n]) * tmp for n in range(5)]
 
 
+def riemann_fn(riemann):
+    if riemann == "hllc":
+        return hllc
+    if riemann == "lhllc":
+        return lambda wl, wr, gl_, gr_: hllc(wl, wr, gl_, gr_, lowmach=True)
+    return hlle
+
+
+def recon_fn(recon, tk):
+    """(ql, qr) of the cells at offset 0 from the stencil accessor tk(k)."""
+    if recon == "ppm":
+        return ppm(tk(-2), tk(-1), tk(0), tk(1), tk(2))
+    if recon == "plm":
+        return plm(tk(-1), tk(0), tk(1))
+    q = tk(0)
+    return q, q
+
+
+# the upstream low-storage stage tables (gam0, gam1, beta) and vl2's first-order predictor
+STAGES = {"rk1": [(0.0, 1.0, 1.0)],
+          "rk2": [(0.0, 1.0, 1.0), (0.5, 0.5, 0.5)],
+          "rk3": [(0.0, 1.0, 1.0), (0.25, 0.75, 0.25), (2.0 / 3.0, 1.0 / 3.0, 2.0 / 3.0)],
+          "vl2": [(0.0, 1.0, 0.5), (0.0, 1.0, 1.0)]}
+
+
 def stage(u_in, u_n, gam0, gam1, beta_dt, recon, riemann, eos, dx, ndim):
     """One fused stage on a periodic block: u (NV, nz, ny, nx) tensors."""
     gm1e, dfloor, pfloor = eos
@@ -230,17 +266,14 @@ def stage(u_in, u_n, gam0, gam1, beta_dt, recon, riemann, eos, dx, ndim):
     for d in range(ndim):
         ax = 3 - d  # tensor axis of spatial axis d
         sh = lambda k: torch.roll(q, shifts=-k, dims=ax)  # value at cell i+k
-        if recon == "ppm":
-            ql, qr = ppm(sh(-2), sh(-1), q, sh(1), sh(2))
-        else:
-            ql, qr = plm(sh(-1), q, sh(1))
+        ql, qr = recon_fn(recon, sh)
         # face i+1/2 of cell i: left state ql[i], right state qr[i+1]
         WL = ql
         WR = torch.roll(qr, shifts=-1, dims=ax)
         ivx, ivy, ivz = 1 + d, 1 + (1 + d) % 3, 1 + (2 + d) % 3
         wl = [WL[0], WL[ivx], WL[ivy], WL[ivz], WL[4]]
         wr = [WR[0], WR[ivx], WR[ivy], WR[ivz], WR[4]]
-        f = (hllc if riemann == "hllc" else hlle)(wl, wr, WL[nv:], WR[nv:])
+        f = riemann_fn(riemann)(wl, wr, WL[nv:], WR[nv:])
         F = [None] * nv
         F[0], F[ivx], F[ivy], F[ivz], F[4] = f
         for n in range(5, nv):
@@ -257,6 +290,15 @@ def stage(u_in, u_n, gam0, gam1, beta_dt, recon, riemann, eos, dx, ndim):
 def rk2_step(u, dt, recon, riemann, eos, dx, ndim):
     u1 = stage(u, u, 0.0, 1.0, 1.0 * dt, recon, riemann, eos, dx, ndim)
     return stage(u1, u, 0.5, 0.5, 0.5 * dt, recon, riemann, eos, dx, ndim)
+
+
+def step(u, dt, integrator, recon, riemann, eos, dx, ndim):
+    """One step of an integrator: stage s maps x -> gam0 x + gam1 u + beta dt L(x)."""
+    x = u
+    for s, (g0, g1, b) in enumerate(STAGES[integrator]):
+        r = "dc" if (integrator == "vl2" and s == 0) else recon
+        x = stage(x, u, g0, g1, b * dt, r, riemann, eos, dx, ndim)
+    return x
 
 
 def _pad(u, ax, G, kind, d):
@@ -294,16 +336,13 @@ def stage_bc(u_in, u_n, gam0, gam1, beta_dt, recon, riemann, eos, dx, ndim, bc, 
         w, gm = cons_to_prim(U, gm1e, dfloor, pfloor)
         q = torch.cat([w, gm])
         tk = lambda k: q.narrow(ax, G - 1 + k, n + 2)  # stencil offset k of cells -1..n
-        if recon == "ppm":
-            ql, qr = ppm(tk(-2), tk(-1), tk(0), tk(1), tk(2))
-        else:
-            ql, qr = plm(tk(-1), tk(0), tk(1))
+        ql, qr = recon_fn(recon, tk)
         WL = ql.narrow(ax, 0, n + 1)   # ql output of cell f-1, faces f = 0..n
         WR = qr.narrow(ax, 1, n + 1)   # qr output of cell f
         ivx, ivy, ivz = 1 + d, 1 + (1 + d) % 3, 1 + (2 + d) % 3
         wl = [WL[0], WL[ivx], WL[ivy], WL[ivz], WL[4]]
         wr = [WR[0], WR[ivx], WR[ivy], WR[ivz], WR[4]]
-        f = (hllc if riemann == "hllc" else hlle)(wl, wr, WL[nv:], WR[nv:])
+        f = riemann_fn(riemann)(wl, wr, WL[nv:], WR[nv:])
         F = [None] * nv
         F[0], F[ivx], F[ivy], F[ivz], F[4] = f
         for m in range(5, nv):

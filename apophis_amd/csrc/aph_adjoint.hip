@@ -80,7 +80,8 @@ struct AdjArgs {
   double *lam_t;               // split form: NV x ncells, L_out x-transposed
   double *qlb, *qrb;           // split form: NF x ncells, cotangents of each cell's ql / qr
                                // output, in the layout of the axis being swept
-  double *spill[3];            // per axis, 6 slots x NF per (pencil, segment)
+  int pbar_xt;                 // the x axis' Pbar is in the x-transposed layout (split form)
+  double *spill[3];            // per axis, [segment][slot][field][pencil] per block
   const int64_t *spill_base;   // [3][nblocks] element offsets
   int segcap[3];               // adj_seg cap per axis
 };
@@ -188,6 +189,29 @@ __device__ __forceinline__ void split_geom(int u, int nA, int nseg, int &seg, in
   const int r = u / nA;
   seg = r % nseg;
   pb = r / nseg;
+}
+
+// Spill record of axis d, block blk: slot `slot` (0-2 below, 3-5 above the segment) of
+// (pencil, segment), field 0; field f lies f * fstride further.  The fastest index is the
+// one the writing lanes vary along, so a wave writes adjacent addresses: pencils for the
+// split form and the one-march form's y and z axes ([segment][slot][field][pencil]),
+// segments for the one-march form's x axis, whose adjacent lanes take the segments of one
+// row ([slot][field][pencil][segment]).
+template <int NF>
+__device__ __forceinline__ double *spill_at(const AdjArgs &a, int d, int blk, int pencil,
+                                            int seg, int slot, bool segfast,
+                                            int64_t &fstride) {
+  const aph_block &k = a.blocks[blk];
+  const int A = (d == 0) ? 1 : 0, B = (d == 2) ? 1 : 2;
+  const int64_t npen = (int64_t)k.nx[A] * k.nx[B];
+  const int64_t base = a.spill_base[d * a.nblocks + blk];
+  if (segfast) {
+    const int64_t nseg = k.nx[d] / adj_seg(k.nx[d], a.segcap[d]);
+    fstride = npen * nseg;
+    return a.spill[d] + base + (int64_t)slot * NF * fstride + (int64_t)pencil * nseg + seg;
+  }
+  fstride = npen;
+  return a.spill[d] + base + ((int64_t)(seg * 6 + slot) * NF) * npen + pencil;
 }
 
 // Offset of pencil (pa, pb) at axis coordinate 0 and the stride along axis D, in the pack
@@ -486,8 +510,9 @@ __device__ __forceinline__ void adj_recon_body(const AdjArgs &a) {
   const int64_t nc = (int64_t)N0 * N1 * N2;
   const int g[3] = {gst(bk, 0, NDIM), gst(bk, 1, NDIM), gst(bk, 2, NDIM)};
   const CellMap m = cell_map<D>(a, bk);
-  const Lay L = axis_lay<D, XT>(pa, pb, N0, N1, N2, g);      // P, QL_bar, QR_bar
-  const Lay Lp = axis_lay<D, false>(pa, pb, N0, N1, N2, g);  // Pbar
+  const Lay L = axis_lay<D, XT>(pa, pb, N0, N1, N2, g);  // P, QL_bar, QR_bar and Pbar
+  const Lay Lp = L;  // the x axis writes its Pbar x-transposed too: adjacent lanes, adjacent
+                     // addresses (the gather transposes it back through LDS)
   const int a0 = seg * S;
   const int m_lo = (seg == 0) ? -1 : a0;
   const int m_hi = (seg == nseg - 1) ? n_d : a0 + S - 1;
@@ -495,11 +520,12 @@ __device__ __forceinline__ void adj_recon_body(const AdjArgs &a) {
   const double *qlb_ptr = a.qlb + D * a.qb_axis + (int64_t)NF * bk.cell_base + fld * nc + L.base;
   const double *qrb_ptr = a.qrb + D * a.qb_axis + (int64_t)NF * bk.cell_base + fld * nc + L.base;
   double *pb_ptr = a.pbar + D * a.pbar_axis + (int64_t)NF * bk.cell_base + fld * nc + Lp.base;
-  double *sp = a.spill[D] + a.spill_base[D * a.nblocks + b] +
-               ((int64_t)pencil * nseg + seg) * 6 * NF + fld;
+  int64_t fstr;
+  double *sp = spill_at<NF>(a, D, b, pencil, seg, 0, false, fstr) + fld * fstr;
+  const int64_t sstride = NF * fstr;  // one slot to the next
   // every halo cell is emitted exactly once: clear the record, then write each slot once
 #pragma unroll
-  for (int s = 0; s < 6; ++s) sp[s * NF] = 0.0;
+  for (int s = 0; s < 6; ++s) sp[s * sstride] = 0.0;
   const bool vel = (fld == IVX);
   auto load = [&](int c) -> double {
     int idx;
@@ -528,7 +554,7 @@ __device__ __forceinline__ void adj_recon_body(const AdjArgs &a) {
       const int slot = below ? c - (a0 - 3) : 3 + (c - (a0 + S));
       const bool image = below ? (c < 0 && m.lo < 0 && m.bclo == APH_BC_REFLECT)
                                : (c >= n_d && m.hi < 0 && m.bchi == APH_BC_REFLECT);
-      sp[slot * NF] = (image && vel) ? -1.0 * v : v;
+      sp[slot * sstride] = (image && vel) ? -1.0 * v : v;
     }
   };
   // bit k of `plain`: col[k] is +-0 or 2^-900 <= |.| < 2^1000, so a face over four such
@@ -594,7 +620,7 @@ __device__ __forceinline__ void adj_recon_body(const AdjArgs &a) {
 // The split form's axes in one launch each (blockIdx.z = axis): they read the same
 // primitives and write separate QL_bar / QR_bar, Pbar arrays and spill records.
 template <int RECON, int RS, int NV, int NDIM>
-__global__ void __launch_bounds__(64, 2) k_adj_face(AdjArgs a) {
+__global__ void __launch_bounds__(64, 1) k_adj_face(AdjArgs a) {
   if (blockIdx.z == 0) {
     adj_face_body<RECON, RS, NV, NDIM, 0>(a);
   } else if constexpr (NDIM >= 2) {
@@ -668,10 +694,10 @@ __device__ __forceinline__ void adj_axis_body(const AdjArgs &a) {
   const double coef = a.beta * a.dt[0] * bk.area[D] / bk.vol;
   const double *lo_ptr = a.lam_out + m.self + tbase;
   double *pb_ptr = a.pbar + D * a.pbar_axis + (int64_t)NF * bk.cell_base + tbase;
-  double *sp = a.spill[D] + a.spill_base[D * a.nblocks + b] +
-               ((int64_t)pencil * nseg + seg) * 6 * NF;
+  int64_t fstr;  // the x axis' lanes take the segments of one row: segment-fastest records
+  double *sp = spill_at<NF>(a, D, b, pencil, seg, 0, D == 0, fstr);
   // every halo cell is emitted exactly once: clear the record, then write each slot once
-  for (int s = 0; s < 6 * NF; ++s) sp[s] = 0.0;
+  for (int s = 0; s < 6 * NF; ++s) sp[s * fstr] = 0.0;
 
   auto load = [&](int c, double *dst) {
     double uu[NV];
@@ -692,7 +718,7 @@ __device__ __forceinline__ void adj_axis_body(const AdjArgs &a) {
                                : (c >= n_d && m.hi < 0 && m.bchi == APH_BC_REFLECT);
 #pragma unroll
       for (int f = 0; f < NF; ++f)
-        sp[slot * NF + f] = (image && f == IVX) ? -1.0 * pbv[f] : pbv[f];
+        sp[(slot * NF + f) * fstr] = (image && f == IVX) ? -1.0 * pbv[f] : pbv[f];
     }
   };
 
@@ -854,111 +880,135 @@ __global__ void __launch_bounds__(64) k_adj_axes(AdjArgs a) {
 template <int NF>
 __device__ __forceinline__ void add_slot(const AdjArgs &a, int d, int blk, int pencil,
                                          int seg, int nseg, int slot, double *acc) {
-  const double *p = a.spill[d] + a.spill_base[d * a.nblocks + blk] +
-                    (((int64_t)pencil * nseg + seg) * 6 + slot) * NF;
+  (void)nseg;
+  int64_t fstr;
+  const double *p = spill_at<NF>(a, d, blk, pencil, seg, slot, !a.pbar_xt && d == 0, fstr);
 #pragma unroll
-  for (int v = 0; v < NF; ++v) acc[v] += p[v];
+  for (int v = 0; v < NF; ++v) acc[v] += p[v * fstr];
 }
 
 template <int NV, int NDIM>
 __global__ void __launch_bounds__(256) k_adj_gather(AdjArgs a) {
   constexpr int NF = NV + 2;
+  constexpr int T = 16;
+  // the x axis' Pbar of one 16 x 16 (x, y) tile, read along y from the x-transposed
+  // layout and used along x (split form)
+  __shared__ double xt[NF][T][T + 1];
   const int b = blockIdx.y;
   const aph_block &bk = a.blocks[b];
   const int N0 = ext(bk, 0, NDIM), N1 = ext(bk, 1, NDIM), N2 = ext(bk, 2, NDIM);
   const int64_t nc = (int64_t)N0 * N1 * N2;
   const int g[3] = {gst(bk, 0, NDIM), gst(bk, 1, NDIM), gst(bk, 2, NDIM)};
-  const int64_t nint = (int64_t)bk.nx[0] * bk.nx[1] * bk.nx[2];
+  const int nx0 = bk.nx[0], nx1 = bk.nx[1], nx2 = bk.nx[2];
+  const int nti = (nx0 + T - 1) / T, ntj = (nx1 + T - 1) / T;
+  const int ntiles = nti * ntj * nx2;
   const double beta_dt = a.beta * a.dt[0];
   const double gsum = (a.stage == 1) ? a.gam0 + a.gam1 : a.gam0;
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nint;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    int cc[3];
-    cc[0] = (int)(t % bk.nx[0]);
-    cc[1] = (int)((t / bk.nx[0]) % bk.nx[1]);
-    cc[2] = (int)(t / ((int64_t)bk.nx[0] * bk.nx[1]));
-    const int64_t cell = ((int64_t)(cc[2] + g[2]) * N1 + (cc[1] + g[1])) * N0 + (cc[0] + g[0]);
-    const int64_t o = (int64_t)NV * bk.cell_base + cell;
-    double lo[NV], u[NV], acc[NV];
+  const int tx = threadIdx.x % T, ty = threadIdx.x / T;
+  const double *pbx = a.pbar + (int64_t)NF * bk.cell_base;
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const int kz = t / (nti * ntj), r = t % (nti * ntj);
+    const int i0 = (r % nti) * T, j0 = (r / nti) * T;
+    if (a.pbar_xt) {  // thread (tx, ty) reads cell (i0 + ty, j0 + tx): adjacent y
+      const int i = i0 + ty, j = j0 + tx;
+      if (i < nx0 && j < nx1) {
+        const int64_t cellt = ((int64_t)(i + g[0]) * N2 + (kz + g[2])) * N1 + (j + g[1]);
 #pragma unroll
-    for (int v = 0; v < NV; ++v) {
-      lo[v] = a.lam_out[o + v * nc];
-      u[v] = a.u_in[o + v * nc];
-      acc[v] = gsum * lo[v] + (a.lam_acc ? a.lam_acc[o + v * nc] : 0.0);
-    }
-    // the axes' Pbar, summed x, then y, then z
-    double pbv[NF];
-    const double *pp = a.pbar + (int64_t)NF * bk.cell_base + cell;
-#pragma unroll
-    for (int f = 0; f < NF; ++f) pbv[f] = pp[f * nc];
-#pragma unroll
-    for (int d = 1; d < NDIM; ++d)
-#pragma unroll
-      for (int f = 0; f < NF; ++f) pbv[f] = pbv[f] + pp[d * a.pbar_axis + f * nc];
-    // spills (primitive space) of the segments / blocks / boundary images that read
-    // this cell
-    for (int d = 0; d < NDIM; ++d) {
-      const int Ad = (d == 0) ? 1 : 0, Bd = (d == 2) ? 1 : 2;
-      const int n_d = bk.nx[d];
-      const int S = adj_seg(n_d, a.segcap[d]), nseg = n_d / S;
-      const int pencil = cc[Ad] + cc[Bd] * bk.nx[Ad];
-      const int s = cc[d] / S, off = cc[d] % S;
-      const int sdn = (d == 0) ? 1 : (d == 1 ? 3 : 9);
-      if (off < 3) {  // from below
-        if (s > 0) {
-          add_slot<NF>(a, d, b, pencil, s - 1, nseg, 3 + off, pbv);
-        } else {
-          const int nb = bk.nbr[13 - sdn];
-          if (nb >= 0) {
-            add_slot<NF>(a, d, nb, pencil, nseg - 1, nseg, 3 + off, pbv);
-          } else if (bk.bc[2 * d] == APH_BC_REFLECT) {
-            add_slot<NF>(a, d, b, pencil, 0, nseg, 2 - off, pbv);
-          } else if (cc[d] == 0) {
-            for (int sl = 0; sl < 3; ++sl) add_slot<NF>(a, d, b, pencil, 0, nseg, sl, pbv);
-          }
-        }
+        for (int f = 0; f < NF; ++f) xt[f][ty][tx] = pbx[f * nc + cellt];
       }
-      if (off >= S - 3) {  // from above
-        if (s < nseg - 1) {
-          add_slot<NF>(a, d, b, pencil, s + 1, nseg, off - (S - 3), pbv);
-        } else {
-          const int nb = bk.nbr[13 + sdn];
-          if (nb >= 0) {
-            add_slot<NF>(a, d, nb, pencil, 0, nseg, off - (S - 3), pbv);
-          } else if (bk.bc[2 * d + 1] == APH_BC_REFLECT) {
-            add_slot<NF>(a, d, b, pencil, nseg - 1, nseg, 3 + (n_d - 1 - cc[d]), pbv);
-          } else if (cc[d] == n_d - 1) {
-            for (int sl = 3; sl < 6; ++sl)
-              add_slot<NF>(a, d, b, pencil, nseg - 1, nseg, sl, pbv);
-          }
-        }
-      }
+      __syncthreads();
     }
-    if (a.grav) {  // out[1+d] += g_d w0 bdt ; out[4] += (sum g_d w_{1+d}) w0 bdt
-      double uu[NV], w[NV], gm[2];
+    const int cc[3] = {i0 + tx, j0 + ty, kz};
+    if (cc[0] < nx0 && cc[1] < nx1) {
+      const int64_t cell =
+          ((int64_t)(cc[2] + g[2]) * N1 + (cc[1] + g[1])) * N0 + (cc[0] + g[0]);
+      const int64_t o = (int64_t)NV * bk.cell_base + cell;
+      double lo[NV], u[NV], acc[NV];
 #pragma unroll
-      for (int v = 0; v < NV; ++v) uu[v] = u[v];
-      cons_to_prim_pt<NV>(uu, a.eos, w, gm);
-      double de = 0.0;
+      for (int v = 0; v < NV; ++v) {
+        lo[v] = a.lam_out[o + v * nc];
+        u[v] = a.u_in[o + v * nc];
+        acc[v] = gsum * lo[v] + (a.lam_acc ? a.lam_acc[o + v * nc] : 0.0);
+      }
+      // the axes' Pbar, summed x, then y, then z
+      double pbv[NF];
+      const double *pp = a.pbar + (int64_t)NF * bk.cell_base + cell;
+      if (a.pbar_xt) {
+#pragma unroll
+        for (int f = 0; f < NF; ++f) pbv[f] = xt[f][tx][ty];
+      } else {
+#pragma unroll
+        for (int f = 0; f < NF; ++f) pbv[f] = pp[f * nc];
+      }
+#pragma unroll
+      for (int d = 1; d < NDIM; ++d)
+#pragma unroll
+        for (int f = 0; f < NF; ++f) pbv[f] = pbv[f] + pp[d * a.pbar_axis + f * nc];
+      // spills (primitive space) of the segments / blocks / boundary images that read
+      // this cell
       for (int d = 0; d < NDIM; ++d) {
-        de += a.g[d] * w[1 + d];
-        pbv[0] += a.g[d] * beta_dt * lo[1 + d];
-        pbv[1 + d] += a.g[d] * w[0] * beta_dt * lo[4];
+        const int Ad = (d == 0) ? 1 : 0, Bd = (d == 2) ? 1 : 2;
+        const int n_d = bk.nx[d];
+        const int S = adj_seg(n_d, a.segcap[d]), nseg = n_d / S;
+        const int pencil = cc[Ad] + cc[Bd] * bk.nx[Ad];
+        const int s = cc[d] / S, off = cc[d] % S;
+        const int sdn = (d == 0) ? 1 : (d == 1 ? 3 : 9);
+        if (off < 3) {  // from below
+          if (s > 0) {
+            add_slot<NF>(a, d, b, pencil, s - 1, nseg, 3 + off, pbv);
+          } else {
+            const int nb = bk.nbr[13 - sdn];
+            if (nb >= 0) {
+              add_slot<NF>(a, d, nb, pencil, nseg - 1, nseg, 3 + off, pbv);
+            } else if (bk.bc[2 * d] == APH_BC_REFLECT) {
+              add_slot<NF>(a, d, b, pencil, 0, nseg, 2 - off, pbv);
+            } else if (cc[d] == 0) {
+              for (int sl = 0; sl < 3; ++sl) add_slot<NF>(a, d, b, pencil, 0, nseg, sl, pbv);
+            }
+          }
+        }
+        if (off >= S - 3) {  // from above
+          if (s < nseg - 1) {
+            add_slot<NF>(a, d, b, pencil, s + 1, nseg, off - (S - 3), pbv);
+          } else {
+            const int nb = bk.nbr[13 + sdn];
+            if (nb >= 0) {
+              add_slot<NF>(a, d, nb, pencil, 0, nseg, off - (S - 3), pbv);
+            } else if (bk.bc[2 * d + 1] == APH_BC_REFLECT) {
+              add_slot<NF>(a, d, b, pencil, nseg - 1, nseg, 3 + (n_d - 1 - cc[d]), pbv);
+            } else if (cc[d] == n_d - 1) {
+              for (int sl = 3; sl < 6; ++sl)
+                add_slot<NF>(a, d, b, pencil, nseg - 1, nseg, sl, pbv);
+            }
+          }
+        }
       }
-      pbv[0] += de * beta_dt * lo[4];
+      if (a.grav) {  // out[1+d] += g_d w0 bdt ; out[4] += (sum g_d w_{1+d}) w0 bdt
+        double uu[NV], w[NV], gm[2];
+  #pragma unroll
+        for (int v = 0; v < NV; ++v) uu[v] = u[v];
+        cons_to_prim_pt<NV>(uu, a.eos, w, gm);
+        double de = 0.0;
+        for (int d = 0; d < NDIM; ++d) {
+          de += a.g[d] * w[1 + d];
+          pbv[0] += a.g[d] * beta_dt * lo[1 + d];
+          pbv[1 + d] += a.g[d] * w[0] * beta_dt * lo[4];
+        }
+        pbv[0] += de * beta_dt * lo[4];
+      }
+      cons_to_prim_adj<NV>(u, a.eos, pbv, pbv + NV, acc);
+  #pragma unroll
+      for (int v = 0; v < NV; ++v) a.lam_in[o + v * nc] = acc[v];
+      if (a.lam_n && a.stage > 1) {
+  #pragma unroll
+        for (int v = 0; v < NV; ++v)
+          a.lam_n[o + v * nc] = a.lam_n_add ? a.gam1 * lo[v] + a.lam_n[o + v * nc]
+                                            : a.gam1 * lo[v];
+      }
     }
-    cons_to_prim_adj<NV>(u, a.eos, pbv, pbv + NV, acc);
-#pragma unroll
-    for (int v = 0; v < NV; ++v) a.lam_in[o + v * nc] = acc[v];
-    if (a.lam_n && a.stage > 1) {
-#pragma unroll
-      for (int v = 0; v < NV; ++v)
-        a.lam_n[o + v * nc] = a.lam_n_add ? a.gam1 * lo[v] + a.lam_n[o + v * nc]
-                                          : a.gam1 * lo[v];
-    }
+    if (a.pbar_xt) __syncthreads();  // the tile is used before the next one is loaded
   }
 }
-
 }  // namespace
 
 // host side ------------------------------------------------------------------------
@@ -1151,6 +1201,7 @@ int adj_stage(aph_context *ctx, const aph_blockset *bs, int recon, int rs, const
   a.qrb = s.qrb;
   for (int d = 0; d < 3; ++d) a.spill[d] = s.spill[d];
   a.spill_base = s.spill_base;
+  a.pbar_xt = ks[0].face != nullptr;  // the split form writes the x axis' Pbar transposed
   for (int d = 0; d < 3; ++d) a.segcap[d] = s.segcap[d];
   const int nf = bs->nvar + 2;
   const dim3 gc((unsigned)std::max<int64_t>(1, std::min<int64_t>(64, (bs->max_interior + 255) / 256)),

@@ -406,8 +406,8 @@ def profile_run(args, ctx, hydro, problems, torch):
 
         if args.profile_run == "adjoint":
             p = problems.kh2d(nblk=16, n=32)
-        else:  # 8 RT instances, PPM+HLLC + gravity
-            p = problems.rt_batch(ninst=8)
+        else:  # batched RT instances (8 by default), PPM+HLLC + gravity
+            p = problems.rt_batch(ninst=args.rt_instances)
         fd = hydro.FusedDriver(ctx, p.bs, p.cons, p.recon, p.riemann, p.eos, p.cfl, g=p.g)
         st = adjoint.Stepper(fd, 1e-4)
         u = torch.from_numpy(p.cons.copy()).cuda()
@@ -501,6 +501,8 @@ def main():
     ap.add_argument("--uneven-blocks", type=int, default=4096)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the KH adjoint leg")
     ap.add_argument("--no-rt", action="store_true", help="skip the batched RT adjoint leg")
+    ap.add_argument("--rt-instances", type=int, default=8,
+                    help="RT instances of --profile-run rt_adjoint")
     ap.add_argument("--profile-run", choices=["fused", "graph", "plain", "uneven", "adjoint",
                                               "rt_adjoint", "amr"],
                     default=None,

@@ -76,6 +76,8 @@ _SIGS = {
     "aph_plan_autotune": (_i32, [_vp, _vp, _vp, _vp, _vp, _i32, ctypes.c_char_p,
                                  ctypes.c_size_t]),
     "aph_plan_get_tile": (_i32, [_vp, _vp, _i32, ctypes.POINTER(Tile)]),
+    "aph_plan_get_launch": (_i32, [_vp, _vp, _i32, ctypes.POINTER(ctypes.c_int32)]),
+    "aph_plan_set_workgroup": (_i32, [_vp, _vp, _i32]),
     "aph_fused_stage": (_i32, [_vp, _vp, _i32, _vp, _vp, _vp, _vp]),
     "aph_plan_step": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _i32]),
     "aph_plan_graph_capture": (_i32, [_vp, _vp, _i32, _vp, _vp, _vp, _vp, _d, _d, _i32]),

@@ -80,9 +80,18 @@ struct aph_plan {
   double g[3] = {0, 0, 0};
   int grav = 0;
   std::vector<aph_tile> user_tiles;  // aph_plan_set_tile overrides, keyed by nx[0]
-  std::vector<std::pair<int4, aph_tile>> tuned;  // autotuner choices, keyed by block shape
+  // autotuner choices, keyed by block shape: the tile and the segment scale (1: the
+  // shortest segments that keep every lane busy, 2: twice as long where they divide)
+  struct Tuned {
+    int4 shape;
+    aph_tile tile;
+    int seg_scale;
+  };
+  std::vector<Tuned> tuned;
   int only_class = -1;               // autotuner: item list restricted to one class
+  int wg = 64;                       // lanes per workgroup (autotuner: 64, 128 or 256)
   std::vector<aph_tile> tiles;       // resolved tile per class
+  std::vector<int> seg_scale;        // resolved segment scale per class
   std::vector<int4> class_shape;  // class index -> block shape (nx0, nx1, nx2)
   int4 *d_items = nullptr;
   int nitems = 0;
@@ -191,12 +200,15 @@ aph_tile default_tile(const aph_block &k, int ndim) {
   return t;
 }
 
-// Segment length along an axis: the largest divisor of the tile extent that keeps
-// at least 64 (pencil, segment) units per sweep, so every lane of the wave works.
-int seg_len(int t_d, int pencils) {
+// Segment length along an axis: the largest divisor of the tile extent that keeps at
+// least `lanes` (pencil, segment) units per sweep, so every lane of the workgroup works;
+// scale 2 doubles it where that still divides the extent (fewer recomputed halo cells,
+// some lanes idle).
+int seg_len(int t_d, int pencils, int lanes, int scale) {
   int best = 1;
   for (int s = 1; s <= t_d; ++s)
-    if (t_d % s == 0 && (int64_t)pencils * (t_d / s) >= 64) best = s;
+    if (t_d % s == 0 && (int64_t)pencils * (t_d / s) >= lanes) best = s;
+  if (scale == 2 && t_d % (2 * best) == 0) best *= 2;
   return best;
 }
 
@@ -223,13 +235,20 @@ int build_items(aph_context *ctx, aph_plan *p) {
   // tile per class: a user tile for edge n = nx0 (aph_plan_set_tile) if one is set,
   // else the autotuner's choice for the shape, else the default
   std::vector<aph_tile> tl(p->class_shape.size());
+  std::vector<int> sc(p->class_shape.size(), 1);
   for (size_t c = 0; c < p->class_shape.size(); ++c) {
     const aph_block &k = bs->h[p->class_shape[c].w];
     aph_tile t = default_tile(k, ndim);
     for (const auto &u : p->tuned)
-      if (u.first.x == k.nx[0] && u.first.y == k.nx[1] && u.first.z == k.nx[2]) t = u.second;
+      if (u.shape.x == k.nx[0] && u.shape.y == k.nx[1] && u.shape.z == k.nx[2]) {
+        t = u.tile;
+        sc[c] = u.seg_scale;
+      }
     for (const auto &u : p->user_tiles)
-      if (u.n == k.nx[0]) t = u;
+      if (u.n == k.nx[0]) {
+        t = u;
+        sc[c] = 1;
+      }
     tl[c] = t;
   }
   // per XCD lists, greedy by cost (cells) so every L2 sees whole blocks
@@ -302,6 +321,7 @@ int build_items(aph_context *ctx, aph_plan *p) {
   if (p->lds > 64 * 1024)
     return set_error(ctx, APH_E_INVALID, "tile too large for LDS (%zu bytes)", p->lds);
   p->tiles.assign(tl.begin(), tl.end());
+  p->seg_scale.assign(sc.begin(), sc.end());
   // persistent grid: resident workgroups, a multiple of the XCD count
   // every kernel variant the plan launches (vl2: the first-order predictor too); the
   // grid takes the smallest residency so that a multilevel launch keeps every
@@ -316,7 +336,7 @@ int build_items(aph_context *ctx, aph_plan *p) {
     if (!fn) return set_error(ctx, APH_E_INVALID, "no fused kernel for this plan");
     int o = 0;
     APH_HIP_CHECK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                           &o, reinterpret_cast<const void *>(fn), 64, p->lds));
+                           &o, reinterpret_cast<const void *>(fn), p->wg, p->lds));
     occ = std::min(occ, o);
   }
   occ = std::max(occ, 1);
@@ -343,7 +363,7 @@ FusedArgs make_args(aph_plan *p, int stage, int nstages, double gam0, double gam
     k.t[2] = t.tz;
     for (int d = 0; d < 3; ++d) {
       const int A = (d == 0) ? 1 : 0, B = (d == 2) ? 1 : 2;
-      k.s[d] = (d < a.ndim) ? seg_len(k.t[d], k.t[A] * k.t[B]) : 1;
+      k.s[d] = (d < a.ndim) ? seg_len(k.t[d], k.t[A] * k.t[B], p->wg, p->seg_scale[c]) : 1;
     }
   }
   a.eos = p->eos;
@@ -390,7 +410,7 @@ int launch_kernel(aph_context *ctx, aph_plan *p, int recon, int stage, int nst, 
   if (!fn) return set_error(ctx, APH_E_INVALID, "no fused kernel for this stage");
   FusedArgs a = make_args(p, stage, nst, gam0, gam1, beta, u_in, u_n, u_out, dt_dev, cfl);
   if (!p->amr) {
-    hipLaunchKernelGGL(fn, dim3(p->grid), dim3(64), p->lds, ctx->stream, a);
+    hipLaunchKernelGGL(fn, dim3(p->grid), dim3(p->wg), p->lds, ctx->stream, a);
     APH_LAUNCH_CHECK(ctx);
     return APH_OK;
   }
@@ -399,7 +419,7 @@ int launch_kernel(aph_context *ctx, aph_plan *p, int recon, int stage, int nst, 
   // leaf; a tile with a face to a finer region waits for them, then takes their mean)
   int rc = amr_fill_epoch(ctx, p->amr, u_in, p->d_epoch);  // also bumps the epoch
   if (rc) return rc;
-  hipLaunchKernelGGL(fn, dim3(p->grid), dim3(64), p->lds, ctx->stream, a);
+  hipLaunchKernelGGL(fn, dim3(p->grid), dim3(p->wg), p->lds, ctx->stream, a);
   APH_LAUNCH_CHECK(ctx);
   k_amr_check<<<1, 1, 0, ctx->stream>>>(p->d_err, u_out);
   APH_LAUNCH_CHECK(ctx);
@@ -451,10 +471,10 @@ int adj_plan_ok(aph_context *ctx, const aph_plan *p) {
   return APH_OK;
 }
 
-// Candidate tiles for one block shape: divisors of the block edges, x extent at least
-// 4 where the block allows (coalesced rows), 64..1024 cells (a whole small block is a
-// candidate too), du tile within the LDS limit.
-std::vector<aph_tile> tile_candidates(const int4 &sh, int ndim, int nvar) {
+// Candidate tiles for one block shape and workgroup size: divisors of the block edges,
+// x extent at least 4 where the block allows (coalesced rows), one to sixteen cells per
+// lane (a whole small block is a candidate too), du tile within the LDS limit.
+std::vector<aph_tile> tile_candidates(const int4 &sh, int ndim, int nvar, int wg) {
   auto divs = [](int n) {
     std::vector<int> d;
     for (int i = 1; i <= n; ++i)
@@ -472,8 +492,8 @@ std::vector<aph_tile> tile_candidates(const int4 &sh, int ndim, int nvar) {
       for (int tz : dz) {
         const int64_t cells = (int64_t)tx * ty * tz;
         if (ndim >= 2 && cells > lds_cells) continue;
-        if (cells > 1024 && cells != whole) continue;
-        if (cells < 64 && cells != whole) continue;
+        if (cells > 16 * wg && cells != whole) continue;
+        if (cells < wg && cells != whole) continue;
         out.push_back(aph_tile{sh.x, tx, ty, tz});
       }
   }
@@ -713,6 +733,24 @@ int aph_plan_set_tile(aph_context *ctx, aph_plan *p, const aph_tile *t) {
   return APH_OK;
 }
 
+int aph_plan_set_workgroup(aph_context *ctx, aph_plan *p, int32_t lanes) {
+  if (!ctx) return APH_E_INVALID;
+  if (!p || (lanes != 64 && lanes != 128 && lanes != 256))
+    return set_error(ctx, APH_E_INVALID, "aph_plan_set_workgroup: lanes must be 64, 128 or 256");
+  const int keep = p->wg;
+  p->wg = lanes;
+  p->dirty = true;
+  int rc = build_items(ctx, p);
+  if (rc) {
+    p->wg = keep;
+    p->dirty = true;
+    build_items(ctx, p);
+    return rc;
+  }
+  drop_graph(p);
+  return APH_OK;
+}
+
 int aph_plan_autotune(aph_context *ctx, aph_plan *p, const double *u_in, double *u_scratch,
                       const double *dt_dev, int32_t reps, char *report, size_t report_len) {
   if (!ctx) return APH_E_INVALID;
@@ -736,58 +774,130 @@ int aph_plan_autotune(aph_context *ctx, aph_plan *p, const double *u_in, double 
     return set_error(ctx, APH_E_HIP, "hipEventCreate failed");
   }
   int rc = APH_OK;
-  char line[160];
-  for (size_t c = 0; c < shapes.size() && rc == APH_OK; ++c) {
-    const int4 sh = shapes[c];
-    bool user = false;
-    for (const auto &u : p->user_tiles) user = user || (u.n == sh.x);
-    snprintf(line, sizeof line, "block %dx%dx%d:%s\n", sh.x, sh.y, sh.z,
-             user ? " tile set by aph_plan_set_tile, not tuned" : "");
-    rep += line;
-    if (user) continue;
-    // drop an earlier choice for this shape, then time every candidate on this class only
-    auto &tv = p->tuned;
-    tv.erase(std::remove_if(tv.begin(), tv.end(),
-                            [&](const std::pair<int4, aph_tile> &e) {
-                              return e.first.x == sh.x && e.first.y == sh.y && e.first.z == sh.z;
-                            }),
-             tv.end());
-    tv.push_back({sh, aph_tile{}});
-    double best = 1e300;
-    aph_tile bt = default_tile(p->bs->h[sh.w], ndim);
-    for (const aph_tile &t : tile_candidates(sh, ndim, p->bs->nvar)) {
-      tv.back().second = t;
-      p->only_class = (int)c;
-      if (build_items(ctx, p) != APH_OK) continue;  // e.g. LDS too small for this tile
+  char line[200];
+  // average stage-1 launch time (ms) of class c with the plan's current settings; false
+  // when that configuration cannot be built (e.g. the tile exceeds LDS)
+  auto time_class = [&](int c, double &avg) -> bool {
+    p->only_class = c;
+    if (build_items(ctx, p) != APH_OK) return false;
+    rc = launch_kernel(ctx, p, p->recon, 1, 2, 0.0, 1.0, 1.0, const_cast<double *>(u_in),
+                       nullptr, u_scratch, dt_dev, 0.0);
+    if (rc) return false;
+    if (hipEventRecord(e0, ctx->stream) != hipSuccess) {
+      rc = set_error(ctx, APH_E_HIP, "autotune: hipEventRecord failed");
+      return false;
+    }
+    for (int r = 0; r < reps && rc == APH_OK; ++r)
       rc = launch_kernel(ctx, p, p->recon, 1, 2, 0.0, 1.0, 1.0, const_cast<double *>(u_in),
                          nullptr, u_scratch, dt_dev, 0.0);
-      if (rc) break;
-      if (hipEventRecord(e0, ctx->stream) != hipSuccess) { rc = APH_E_HIP; break; }
-      for (int r = 0; r < reps && rc == APH_OK; ++r)
-        rc = launch_kernel(ctx, p, p->recon, 1, 2, 0.0, 1.0, 1.0, const_cast<double *>(u_in),
-                         nullptr, u_scratch, dt_dev, 0.0);
-      if (rc) break;
-      float ms = 0.f;
-      if (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-          hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
-        rc = set_error(ctx, APH_E_HIP, "autotune: event timing failed");
-        break;
-      }
-      const double avg = ms / reps;
-      snprintf(line, sizeof line, "  %3dx%3dx%3d  %.4f ms\n", t.tx, t.ty, t.tz, avg);
-      rep += line;
-      if (avg < best) {
-        best = avg;
-        bt = t;
-      }
+    if (rc) return false;
+    float ms = 0.f;
+    if (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+      rc = set_error(ctx, APH_E_HIP, "autotune: event timing failed");
+      return false;
     }
-    tv.back().second = bt;
-    snprintf(line, sizeof line, "  chosen %dx%dx%d\n", bt.tx, bt.ty, bt.tz);
-    rep += line;
+    avg = ms / reps;
+    return true;
+  };
+  auto user_tile = [&](const int4 &sh) {
+    for (const auto &u : p->user_tiles)
+      if (u.n == sh.x) return true;
+    return false;
+  };
+  auto set_tuned = [&](const int4 &sh, const aph_tile &t, int scale) {
+    auto &tv = p->tuned;
+    tv.erase(std::remove_if(tv.begin(), tv.end(),
+                            [&](const aph_plan::Tuned &e) {
+                              return e.shape.x == sh.x && e.shape.y == sh.y && e.shape.z == sh.z;
+                            }),
+             tv.end());
+    tv.push_back({sh, t, scale});
+  };
+  auto segs = [&](const aph_tile &t, int wg, int scale, int *sg) {
+    const int tt[3] = {t.tx, t.ty, t.tz};
+    for (int d = 0; d < 3; ++d) {
+      const int A = (d == 0) ? 1 : 0, B = (d == 2) ? 1 : 2;
+      sg[d] = d < ndim ? seg_len(tt[d], tt[A] * tt[B], wg, scale) : 1;
+    }
+  };
+  // the workgroup size is one per launch, so it is tuned for the whole plan: for each
+  // size, the best (tile, segment scale) of every block shape, then the size whose summed
+  // best times are smallest
+  struct Best {
+    aph_tile tile;
+    int scale;
+    double ms;
+  };
+  const int wgs[3] = {64, 128, 256};
+  double best_total = 1e300;
+  int best_wg = p->wg;
+  std::vector<Best> best_cfg;
+  for (int wg : wgs) {
+    if (rc) break;
+    p->wg = wg;
+    std::vector<Best> cfg(shapes.size());
+    double total = 0.0;
+    bool ok = true;
+    for (size_t c = 0; c < shapes.size() && rc == APH_OK && ok; ++c) {
+      const int4 sh = shapes[c];
+      snprintf(line, sizeof line, "workgroup %d, block %dx%dx%d:%s\n", wg, sh.x, sh.y, sh.z,
+               user_tile(sh) ? " tile set by aph_plan_set_tile, not tuned" : "");
+      rep += line;
+      std::vector<aph_tile> cand;
+      if (user_tile(sh)) {
+        for (const auto &u : p->user_tiles)
+          if (u.n == sh.x) cand.push_back(u);
+      } else {
+        cand = tile_candidates(sh, ndim, p->bs->nvar, wg);
+      }
+      Best b{default_tile(p->bs->h[sh.w], ndim), 1, 1e300};
+      for (const aph_tile &t : cand) {
+        for (int scale = 1; scale <= (user_tile(sh) ? 1 : 2); ++scale) {
+          int s1[3], s2[3];
+          segs(t, wg, 1, s1);
+          segs(t, wg, scale, s2);
+          if (scale == 2 && s1[0] == s2[0] && s1[1] == s2[1] && s1[2] == s2[2]) continue;
+          if (!user_tile(sh)) set_tuned(sh, t, scale);
+          double avg = 0.0;
+          if (!time_class((int)c, avg)) {
+            if (rc) break;
+            continue;
+          }
+          snprintf(line, sizeof line, "  %3dx%3dx%3d  segments %d/%d/%d  %.4f ms\n", t.tx, t.ty,
+                   t.tz, s2[0], s2[1], s2[2], avg);
+          rep += line;
+          if (avg < b.ms) b = Best{t, scale, avg};
+        }
+        if (rc) break;
+      }
+      if (b.ms >= 1e300) ok = false;
+      cfg[c] = b;
+      total += b.ms;
+    }
+    if (rc) break;
+    if (ok && total < best_total) {
+      best_total = total;
+      best_wg = wg;
+      best_cfg = cfg;
+    }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   p->only_class = -1;
+  p->wg = best_wg;
+  for (size_t c = 0; c < best_cfg.size(); ++c)
+    if (!user_tile(shapes[c])) set_tuned(shapes[c], best_cfg[c].tile, best_cfg[c].scale);
+  snprintf(line, sizeof line, "chosen: workgroup %d\n", best_wg);
+  rep += line;
+  for (size_t c = 0; c < best_cfg.size(); ++c) {
+    int sg[3];
+    segs(best_cfg[c].tile, best_wg, best_cfg[c].scale, sg);
+    snprintf(line, sizeof line, "  block %dx%dx%d: tile %dx%dx%d, segments %d/%d/%d\n",
+             shapes[c].x, shapes[c].y, shapes[c].z, best_cfg[c].tile.tx, best_cfg[c].tile.ty,
+             best_cfg[c].tile.tz, sg[0], sg[1], sg[2]);
+    rep += line;
+  }
   p->dirty = true;
   const int rb = build_items(ctx, p);
   drop_graph(p);
@@ -798,6 +908,31 @@ int aph_plan_autotune(aph_context *ctx, aph_plan *p, const double *u_in, double 
   }
   if (rc) return rc;
   return rb;
+}
+
+int aph_plan_get_launch(aph_context *ctx, aph_plan *p, int32_t block, int32_t out[4]) {
+  if (!ctx) return APH_E_INVALID;
+  if (!p || !out || block < 0 || block >= p->bs->nblocks)
+    return set_error(ctx, APH_E_INVALID, "aph_plan_get_launch: bad arguments");
+  if (p->dirty) {
+    int rc = build_items(ctx, p);
+    if (rc) return rc;
+  }
+  const aph_block &k = p->bs->h[block];
+  for (size_t c = 0; c < p->class_shape.size(); ++c) {
+    const int4 &sh = p->class_shape[c];
+    if (sh.x == k.nx[0] && sh.y == k.nx[1] && sh.z == k.nx[2]) {
+      const aph_tile &t = p->tiles[c];
+      const int tt[3] = {t.tx, t.ty, t.tz};
+      out[0] = p->wg;
+      for (int d = 0; d < 3; ++d) {
+        const int A = (d == 0) ? 1 : 0, B = (d == 2) ? 1 : 2;
+        out[1 + d] = d < p->bs->ndim ? seg_len(tt[d], tt[A] * tt[B], p->wg, p->seg_scale[c]) : 1;
+      }
+      return APH_OK;
+    }
+  }
+  return set_error(ctx, APH_E_STATE, "aph_plan_get_launch: block shape not in plan");
 }
 
 int aph_plan_get_tile(aph_context *ctx, aph_plan *p, int32_t block, aph_tile *out) {

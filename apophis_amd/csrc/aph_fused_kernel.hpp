@@ -185,7 +185,7 @@ __device__ __forceinline__ void sweep(const FusedArgs &a, const aph_block &bk, i
     }
   }
   const int tx = cfg.t[0], ty = cfg.t[1];
-  for (int unit = threadIdx.x; unit < units; unit += 64) {
+  for (int unit = threadIdx.x; unit < units; unit += blockDim.x) {
     int seg, pa, pb;
     if (D == 0) {  // segments of one row on adjacent lanes
       seg = unit % nseg;
@@ -343,8 +343,10 @@ __device__ __forceinline__ void sweep(const FusedArgs &a, const aph_block &bk, i
 
 // AMR: the multilevel-mesh variant (interface fluxes recorded / corrected at block faces
 // to the other level); the single-level kernel carries none of that code.
+// One kernel for workgroups of 64, 128 or 256 lanes (the autotuner's choice): the register
+// budget does not depend on it (217 VGPRs for PPM/HLLC in 3D, two waves per SIMD).
 template <int RECON, int RS, int NV, int NDIM, bool AMR = false>
-__global__ void __launch_bounds__(64) k_fused(FusedArgs a) {
+__global__ void __launch_bounds__(256) k_fused(FusedArgs a) {
   extern __shared__ double du_s[];
   double dtmin = 1.79769313486231570815e308;
   for (int it = blockIdx.x; it < a.nitems; it += gridDim.x) {
@@ -366,9 +368,9 @@ __global__ void __launch_bounds__(64) k_fused(FusedArgs a) {
     }
     __syncthreads();
   }
-  if (a.last && a.dt_min) {
+  if (a.last && a.dt_min) {  // one fold per wave
     for (int o = 32; o > 0; o >>= 1) dtmin = fmin(dtmin, __shfl_xor(dtmin, o));
-    if (threadIdx.x == 0) fold_min_pos(a.dt_min, a.cfl * dtmin);
+    if ((threadIdx.x & 63) == 0) fold_min_pos(a.dt_min, a.cfl * dtmin);
   }
 }
 

@@ -527,7 +527,7 @@ class FusedDriver:
         """Time the candidate tiles of every block shape on the current state and keep
         the fastest (aph_plan_autotune; u1 is used as scratch).  Returns the timing
         report."""
-        buf = ctypes.create_string_buffer(1 << 16)
+        buf = ctypes.create_string_buffer(1 << 18)
         self.ctx.check(_lib.lib().aph_plan_autotune(self.ctx.h, self.plan, _ptr(self.u0),
                                                     _ptr(self.u1), _ptr(self.dt), int(reps),
                                                     buf, len(buf)))
@@ -544,6 +544,22 @@ class FusedDriver:
                 self.ctx.check(_lib.lib().aph_plan_get_tile(self.ctx.h, self.plan, b,
                                                             ctypes.byref(t)))
                 out[shape] = (t.tx, t.ty, t.tz)
+        return out
+
+    def set_workgroup(self, lanes):
+        """Lanes per workgroup of the stage launches (64, 128 or 256)."""
+        self.ctx.check(_lib.lib().aph_plan_set_workgroup(self.ctx.h, self.plan, int(lanes)))
+        self.graph_steps = 0
+
+    def launch(self):
+        """{block shape (nx, ny, nz): (lanes per workgroup, segment x, y, z)} in use."""
+        out = {}
+        for b in range(self.bs.nblocks):
+            shape = tuple(int(v) for v in self.bs.blocks["nx"][b])
+            if shape not in out:
+                v = (ctypes.c_int32 * 4)()
+                self.ctx.check(_lib.lib().aph_plan_get_launch(self.ctx.h, self.plan, b, v))
+                out[shape] = tuple(int(x) for x in v)
         return out
 
     def initial_dt(self):

@@ -456,6 +456,10 @@ def profile_run(args, ctx, hydro, problems, torch):
         _calibration_copies(torch)
         print(f"profile-run {args.profile_run}: done", flush=True)
         return
+    if args.profile_run == "autotune":
+        autotune_tables(ctx, hydro, problems, torch)
+        print(f"profile-run {args.profile_run}: done", flush=True)
+        return
     if args.profile_run == "uneven":
         p = problems.uneven_blocks(nblocks=args.uneven_blocks)
     else:
@@ -478,6 +482,46 @@ def profile_run(args, ctx, hydro, problems, torch):
     torch.cuda.synchronize()
     _calibration_copies(torch)
     print(f"profile-run {args.profile_run}: done", flush=True)
+
+
+def _launch_desc(fd):
+    """{block shape: "wg<lanes> seg <x>/<y>/<z>"} of a fused driver's plan."""
+    return {"x".join(map(str, k)): f"wg{v[0]} seg {v[1]}/{v[2]}/{v[3]}"
+            for k, v in fd.launch().items()}
+
+
+def autotune_tables(ctx, hydro, problems, torch):
+    """The autotuner's timing tables for the 256^3 blast cut into blocks of 8^3, 16^3 and
+    32^3 (32768, 4096 and 512 blocks) and for the uneven 8^3/16^3/32^3 pack, each followed
+    by the chosen launch's zone-cycles/s over 10 graph-replayed RK2 steps."""
+    cases = [(f"blast3d 256^3, blocks of {n}^3", lambda n=n: problems.blast3d(nblk=256 // n,
+                                                                               n=n))
+             for n in (8, 16, 32)]
+    cases.append(("uneven 8^3/16^3/32^3 pack, 4096 blocks",
+                  lambda: problems.uneven_blocks(nblocks=4096)))
+    for name, make in cases:
+        p = make()
+        cells = p.bs.interior_cells()
+        fd = hydro.FusedDriver(ctx, p.bs, p.cons, p.recon, p.riemann, p.eos, p.cfl)
+        fd.initial_dt()
+        t0 = time.perf_counter()
+        report = fd.autotune(reps=5)
+        t_tune = time.perf_counter() - t0
+        for _ in range(2):
+            fd.step()
+        fd.capture(10)
+        torch.cuda.synchronize()
+        fd.replay()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fd.replay()
+        torch.cuda.synchronize()
+        zc = cells * 10 / (time.perf_counter() - t0)
+        print(f"## {name}: {p.bs.nblocks} blocks, {cells} cells; tuning {t_tune:.1f} s; "
+              f"chosen launch {_launch_desc(fd)}: {zc:.4e} zone-cycles/s", flush=True)
+        print(report, flush=True)
+        del fd, p
+        torch.cuda.empty_cache()
 
 
 def _calibration_copies(torch):
@@ -503,7 +547,8 @@ def main():
     ap.add_argument("--no-rt", action="store_true", help="skip the batched RT adjoint leg")
     ap.add_argument("--rt-instances", type=int, default=8,
                     help="RT instances of --profile-run rt_adjoint")
-    ap.add_argument("--profile-run", choices=["fused", "graph", "plain", "uneven", "adjoint",
+    ap.add_argument("--profile-run", choices=["fused", "graph", "plain", "uneven", "autotune",
+                                              "adjoint",
                                               "rt_adjoint", "amr"],
                     default=None,
                     help="short eager run for rocprofv3 (no timing, no JSON)")
@@ -534,7 +579,8 @@ def main():
     t_tune = time.perf_counter()
     report = fd.autotune(reps=3)
     tiles = {"x".join(map(str, k)): "x".join(map(str, v)) for k, v in fd.tiles().items()}
-    print(f"# autotune {time.perf_counter() - t_tune:.1f}s: {tiles}", file=sys.stderr,
+    launch = _launch_desc(fd)
+    print(f"# autotune {time.perf_counter() - t_tune:.1f}s: {tiles} {launch}", file=sys.stderr,
           flush=True)
     for _ in range(args.warmup):
         fd.step()
@@ -611,6 +657,7 @@ def main():
         fq.autotune(reps=3)
         uneven_tiles = {"x".join(map(str, k)): "x".join(map(str, v))
                         for k, v in fq.tiles().items()}
+        uneven_launch = _launch_desc(fq)
         for _ in range(2):
             fq.step()
         nq = 20
@@ -655,6 +702,7 @@ def main():
         torch.cuda.synchronize()
         vs = qc * npl / (time.perf_counter() - t0)
         extras["uneven_blocks"] = {"blocks": q.bs.nblocks, "cells": qc, "tiles": uneven_tiles,
+                                   "launch": uneven_launch,
                                    "batched_value": vb, "plain_per_block_value": vp,
                                    "batched_over_plain": vb / vp,
                                    "plain_size_class_packs": {str(int(ed[a])): n
@@ -690,7 +738,7 @@ def main():
         "config": {"workload": f"blast3d {args.nblk * 32}^3 periodic, {p.bs.nblocks} blocks "
                                "of 32^3, PPM+HLLC, RK2, CFL 0.3, gamma 5/3",
                    "form": "batched fused, hipGraph-replayed, device dt",
-                   "tiles": tiles, "finite": finite},
+                   "tiles": tiles, "launch": launch, "finite": finite},
         "roofline": roof,
         "valu_roofline": valu_roof,
         "cpu_baseline": cpu,

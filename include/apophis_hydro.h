@@ -324,10 +324,13 @@ int aph_plan_destroy(aph_context *ctx, aph_plan *plan);
 /* Override the tile shape for blocks of edge t->n (autotuner hook).  Synchronous. */
 int aph_plan_set_tile(aph_context *ctx, aph_plan *plan, const aph_tile *t);
 /* Autotuner (no counterpart in the reference, whose flux kernels take one fixed team
- * shape, hydro.cpp:580-583).  For every block shape of the plan it times `reps` stage-1
- * launches (u_in -> u_scratch, *dt_dev on the device) over the blocks of that shape for
- * each candidate tile (divisors of the block edges, 64..1024 cells, within LDS), and keeps
- * the fastest; tiles set with aph_plan_set_tile are kept as they are.  Synchronous; u_in
+ * shape, hydro.cpp:580-583).  For each workgroup size (64, 128, 256 lanes) and every block
+ * shape of the plan it times `reps` stage-1 launches (u_in -> u_scratch, *dt_dev on the
+ * device) over the blocks of that shape for each candidate tile (divisors of the block
+ * edges, one to sixteen cells per lane, within LDS) and segment length (the shortest that
+ * keeps every lane busy, or twice that), keeps each shape's fastest, and takes the
+ * workgroup size whose shapes sum to the least time; tiles set with aph_plan_set_tile are
+ * kept as they are (their segments are not tuned).  Synchronous; u_in
  * is only read, u_scratch is overwritten.  If `report` is non-null, a text table of the
  * candidates and their average launch times (ms) is written to it (at most report_len
  * bytes, NUL-terminated).  Fails with APH_E_STATE while the context's stream is
@@ -336,6 +339,14 @@ int aph_plan_autotune(aph_context *ctx, aph_plan *plan, const double *u_in, doub
                       const double *dt_dev, int32_t reps, char *report, size_t report_len);
 /* Tile currently used for block `block` of the plan's block set. */
 int aph_plan_get_tile(aph_context *ctx, aph_plan *plan, int32_t block, aph_tile *out);
+/* Lanes per workgroup of the plan's stage launches: 64, 128 or 256 (default 64; the
+ * autotuner sets it).  Results do not depend on it bit for bit.  Synchronous.  No
+ * reference counterpart. */
+int aph_plan_set_workgroup(aph_context *ctx, aph_plan *plan, int32_t lanes);
+/* Launch shape for block `block`: out[0] = lanes per workgroup (one per plan: 64, 128 or
+ * 256), out[1..3] = the segment length one lane marches along x, y, z.  No reference
+ * counterpart (the reference's flux kernels take one fixed team shape). */
+int aph_plan_get_launch(aph_context *ctx, aph_plan *plan, int32_t block, int32_t out[4]);
 /* Time integrator of the plan (default rk2): the upstream Parthenon low-storage
  * schemes rk1 / rk2 / rk3 / vl2 that ApophisDriver uses (apophis_driver.cpp:78-85,
  * hydro.cpp:114-139; vl2 predicts with DC reconstruction).  rk3 allocates one more

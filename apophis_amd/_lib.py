@@ -78,6 +78,15 @@ _SIGS = {
     "aph_plan_get_tile": (_i32, [_vp, _vp, _i32, ctypes.POINTER(Tile)]),
     "aph_plan_get_launch": (_i32, [_vp, _vp, _i32, ctypes.POINTER(ctypes.c_int32)]),
     "aph_plan_set_workgroup": (_i32, [_vp, _vp, _i32]),
+    "aph_apply_gravity_field": (_i32, [_vp, _vp, _i32, _i32, _vp, _vp, _vp, _d]),
+    "aph_monopole_create": (_i32, [_vp, _i32, _i32, _d, _d, ctypes.POINTER(_vp)]),
+    "aph_monopole_destroy": (_i32, [_vp, _vp]),
+    "aph_monopole_zero_radial_bins": (_i32, [_vp, _vp, _vp, _vp]),
+    "aph_monopole_accumulate_mass": (_i32, [_vp, _vp, _vp, _i32, _i32, _vp]),
+    "aph_monopole_build_radial_tables": (_i32, [_vp, _vp]),
+    "aph_monopole_gravity_field": (_i32, [_vp, _vp, _vp, _i32, _i32, _vp, _vp]),
+    "aph_monopole_get_tables": (_i32, [_vp, _vp, _vp, _vp]),
+    "aph_plan_set_monopole": (_i32, [_vp, _vp, _vp]),
     "aph_fused_stage": (_i32, [_vp, _vp, _i32, _vp, _vp, _vp, _vp]),
     "aph_plan_step": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _i32]),
     "aph_plan_graph_capture": (_i32, [_vp, _vp, _i32, _vp, _vp, _vp, _vp, _d, _d, _i32]),

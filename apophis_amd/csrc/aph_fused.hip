@@ -79,6 +79,8 @@ struct aph_plan {
   aph_eos eos{};
   double g[3] = {0, 0, 0};
   int grav = 0;
+  aph_monopole *mono = nullptr;  // monopole gravity before every stage (aph_plan_set_monopole)
+  double *d_gfield = nullptr;    // its field: 3 components, pack layout
   std::vector<aph_tile> user_tiles;  // aph_plan_set_tile overrides, keyed by nx[0]
   // autotuner choices, keyed by block shape: the tile and the segment scale (1: the
   // shortest segments that keep every lane busy, 2: twice as long where they divide)
@@ -370,7 +372,8 @@ FusedArgs make_args(aph_plan *p, int stage, int nstages, double gam0, double gam
   a.g[0] = p->g[0];
   a.g[1] = p->g[1];
   a.g[2] = p->g[2];
-  a.grav = p->grav;
+  a.grav = p->grav || p->mono;
+  a.gfield = p->mono ? p->d_gfield : nullptr;
   a.u_in = u_in;
   a.u_n = u_n;
   a.u_out = u_out;
@@ -408,6 +411,10 @@ int launch_kernel(aph_context *ctx, aph_plan *p, int recon, int stage, int nst, 
   KernelFn fn = p->amr ? pick_kernel_amr(recon, p->rsolver, p->bs->nvar, p->bs->ndim)
                        : pick_kernel(recon, p->rsolver, p->bs->nvar, p->bs->ndim);
   if (!fn) return set_error(ctx, APH_E_INVALID, "no fused kernel for this stage");
+  if (p->mono) {  // the solver's tasks on the stage input, then the stage reads the field
+    int rc = monopole_pass(ctx, p->mono, p->bs, u_in, p->eos.dfloor, p->d_gfield);
+    if (rc) return rc;
+  }
   FusedArgs a = make_args(p, stage, nst, gam0, gam1, beta, u_in, u_n, u_out, dt_dev, cfl);
   if (!p->amr) {
     hipLaunchKernelGGL(fn, dim3(p->grid), dim3(p->wg), p->lds, ctx->stream, a);
@@ -462,6 +469,7 @@ int adj_coef(const aph_plan *p, int s, AdjCoef &c, int &srecon) {
 // pair (and vl2's first-order predictor) the adjoint kernels cover.
 int adj_plan_ok(aph_context *ctx, const aph_plan *p) {
   if (p->amr) return set_error(ctx, APH_E_INVALID, "adjoint: multilevel meshes are not supported");
+  if (p->mono) return set_error(ctx, APH_E_INVALID, "adjoint: monopole gravity is not supported");
   if (!adj_supported(p->recon, p->rsolver, p->bs->nvar, p->bs->ndim) ||
       (p->integrator == APH_INTEGRATOR_VL2 &&
        !adj_supported(RECON_DC, p->rsolver, p->bs->nvar, p->bs->ndim)))
@@ -568,6 +576,7 @@ int aph_plan_destroy(aph_context *ctx, aph_plan *p) {
   if (p->d_items) (void)hipFree(p->d_items);
   if (p->d_est) (void)hipFree(p->d_est);
   if (p->d_u2) (void)hipFree(p->d_u2);
+  if (p->d_gfield) (void)hipFree(p->d_gfield);
   if (p->d_amr_face) (void)hipFree(p->d_amr_face);
   if (p->d_amr_buf) (void)hipFree(p->d_amr_buf);
   for (void *q : {(void *)p->d_region, (void *)p->d_count, (void *)p->d_expect,
@@ -729,6 +738,18 @@ int aph_plan_set_tile(aph_context *ctx, aph_plan *p, const aph_tile *t) {
     build_items(ctx, p);
     return rc;
   }
+  drop_graph(p);
+  return APH_OK;
+}
+
+int aph_plan_set_monopole(aph_context *ctx, aph_plan *p, aph_monopole *m) {
+  if (!ctx) return APH_E_INVALID;
+  if (!p) return set_error(ctx, APH_E_INVALID, "aph_plan_set_monopole: null plan");
+  if (m && !p->d_gfield) {
+    APH_HIP_CHECK(ctx, hipMalloc(&p->d_gfield, sizeof(double) * 3 *
+                                                   std::max<int64_t>(p->bs->ncells, 1)));
+  }
+  p->mono = m;
   drop_graph(p);
   return APH_OK;
 }

@@ -30,6 +30,7 @@ struct FusedArgs {
   aph_eos eos;
   double g[3];
   int grav;
+  const double *gfield;  // per-cell acceleration (3 components, pack layout) or null: g
   const double *u_in;
   const double *u_n;
   double *u_out;
@@ -296,15 +297,22 @@ __device__ __forceinline__ void sweep(const FusedArgs &a, const aph_block &bk, i
           }
           if (a.grav) {  // ApplyGravity with the stage-start primitives
             const double gdt = beta_dt;
-            out[1] += a.g[0] * w[0] * gdt;
-            double de = a.g[0] * w[1];
+            double g0 = a.g[0], g1 = a.g[1], g2 = a.g[2];
+            if (a.gfield) {  // a per-cell field (monopole gravity)
+              const double *gp = a.gfield + 3 * bk.cell_base + cidx;
+              g0 = gp[0];
+              g1 = gp[nc];
+              g2 = gp[2 * nc];
+            }
+            out[1] += g0 * w[0] * gdt;
+            double de = g0 * w[1];
             if (NDIM >= 2) {
-              out[2] += a.g[1] * w[0] * gdt;
-              de += a.g[1] * w[2];
+              out[2] += g1 * w[0] * gdt;
+              de += g1 * w[2];
             }
             if (NDIM >= 3) {
-              out[3] += a.g[2] * w[0] * gdt;
-              de += a.g[2] * w[3];
+              out[3] += g2 * w[0] * gdt;
+              de += g2 * w[3];
             }
             out[4] += de * w[0] * gdt;
           }

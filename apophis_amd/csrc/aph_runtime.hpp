@@ -54,6 +54,11 @@ int noalloc_scratch_init(int device);
                             hipGetErrorString(e_));                                   \
   } while (0)
 
+// Monopole gravity's whole pass over every block of bs (aph_gravity.hip): bins from the
+// density of `field` floored at dfloor, tables, and the field (3 components) into grav.
+int monopole_pass(aph_context *ctx, aph_monopole *m, const aph_blockset *bs,
+                  const double *field, double dfloor, double *grav);
+
 // Resolve the (first_block, nblocks) sub-range convention of the plain entry points.
 int block_range(aph_context *ctx, const aph_blockset *bs, int32_t first, int32_t n,
                 int32_t *lo, int32_t *cnt);

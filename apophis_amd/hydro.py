@@ -369,6 +369,133 @@ def global_quant_hst(md, var, out=None):
     return out
 
 
+# ------------------------------------------------------------------ gravity
+GRAVITY_G = 6.673e-8  # constants.hpp:5 (cm^3 g^-1 s^-2)
+
+
+class Gravity(enum.IntEnum):  # main.hpp:55-61
+    undefined = 0
+    none = 1
+    constant = 2
+    monopole = 3
+    poisson = 4
+
+
+def initialize_gravity(pin):
+    """InitializeGravity (gravity.cpp:220-370) for the solvers this library runs: "none",
+    "constant" (the field is the driver's g) and "monopole" (nrbin: gravity/nrbin, else
+    the mesh's nx1, monopole_gravity.hpp:83-88; rmax: gravity/rmax, else the distance to
+    the farthest mesh corner, gravity.cpp:262-276).  "poisson" (multigrid / BiCGSTAB) is
+    refused."""
+    pkg = StateDescriptor("Gravity")
+    gtype = _get(pin, "gravity", "type", "none")
+    pkg.add_param("gravity_constant", float(_get(pin, "gravity", "gravity_constant",
+                                                 GRAVITY_G)))
+    if gtype not in ("none", "constant", "monopole", "poisson"):
+        raise ApophisError("[Apophis]: Gravity not recognized. Exiting.")
+    if gtype == "poisson":
+        raise ApophisError("[Apophis]: gravity/type = poisson (the multigrid Poisson solver) "
+                           "is not available in this library")
+    pkg.add_param("gravity", Gravity[gtype])
+    if gtype == "monopole":
+        nrbin = int(_get(pin, "gravity", "nrbin", -1))
+        if nrbin <= 0:
+            nrbin = int(_get(pin, "parthenon/mesh", "nx1", required=True))
+        edge = []
+        for a in "123":
+            lo = float(_get(pin, "parthenon/mesh", f"x{a}min", 0.0, required=(a == "1")))
+            hi = float(_get(pin, "parthenon/mesh", f"x{a}max", 0.0, required=(a == "1")))
+            edge.append(max(abs(lo), abs(hi)))
+        default_rmax = float(np.sqrt(edge[0] * edge[0] + edge[1] * edge[1] + edge[2] * edge[2]))
+        rmax = float(_get(pin, "gravity", "rmax", default_rmax))
+        if rmax <= 0.0:
+            raise ApophisError("[Apophis]: gravity/rmax must be positive for monopole gravity.")
+        pkg.add_param("nrbin", nrbin)
+        pkg.add_param("rmax", rmax)
+    return pkg
+
+
+class MonopoleGravitySolver:
+    """MonopoleGravitySolver (monopole_gravity.hpp:15-305) on the device: AddTasks'
+    sequence (:47-80) over the partitions of a mesh, writing the "gravity" field (3
+    components) and, if given, "gravity.phi" (1 component), both in the pack layout."""
+
+    def __init__(self, ctx, ndim, nrbin, rmax, gravity_constant):
+        self.ctx = ctx
+        self.ndim, self.nrbin = int(ndim), int(nrbin)
+        self.rmax, self.G = float(rmax), float(gravity_constant)
+        self.h = ctypes.c_void_p()
+        ctx.check(_lib.lib().aph_monopole_create(ctx.h, self.ndim, self.nrbin, self.rmax,
+                                                 self.G, ctypes.byref(self.h)))
+
+    @classmethod
+    def from_package(cls, ctx, pkg, ndim):
+        return cls(ctx, ndim, pkg.param("nrbin"), pkg.param("rmax"),
+                   pkg.param("gravity_constant"))
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        try:
+            if h and h.value:
+                _lib.lib().aph_monopole_destroy(self.ctx.h, h)
+                self.h = ctypes.c_void_p()
+        except Exception:  # interpreter shutdown: the module may be gone already
+            pass
+
+    def zero_radial_bins(self, md):
+        """ZeroRadialBins (:107-112); also fixes the scale of the sums from every block's
+        primitive density (md.prim)."""
+        self.ctx.check(_lib.lib().aph_monopole_zero_radial_bins(self.ctx.h, self.h, md.dbs,
+                                                                _ptr(md.prim)))
+        return TaskStatus.complete
+
+    def accumulate_mass(self, md):
+        """AccumulateMass (:114-153) over md's blocks."""
+        self.ctx.check(_lib.lib().aph_monopole_accumulate_mass(
+            self.ctx.h, self.h, md.dbs, md.first, md.count, _ptr(md.prim)))
+        return TaskStatus.complete
+
+    def build_radial_tables(self):
+        """BuildRadialTables (:155-213)."""
+        self.ctx.check(_lib.lib().aph_monopole_build_radial_tables(self.ctx.h, self.h))
+        return TaskStatus.complete
+
+    def compute_gravity_vector_direct(self, md, grav, phi=None):
+        """ComputeGravityVectorDirect (:215-302) over md's blocks."""
+        self.ctx.check(_lib.lib().aph_monopole_gravity_field(
+            self.ctx.h, self.h, md.dbs, md.first, md.count, _ptr(grav),
+            _ptr(phi) if phi is not None else None))
+        return TaskStatus.complete
+
+    def add_tasks(self, parts, grav, phi=None):
+        """The task list of one stage: bins zeroed once, masses per partition, the
+        tables once (after the all-reduce, here a single device's sums), the field per
+        partition."""
+        self.zero_radial_bins(parts[0])
+        for p in parts:
+            self.accumulate_mass(p)
+        self.build_radial_tables()
+        for p in parts:
+            self.compute_gravity_vector_direct(p, grav, phi)
+        return TaskStatus.complete
+
+    def tables(self):
+        """(m_enc, phi_rad) of the last pass, each nrbin + 1 values (synchronous)."""
+        m_enc, phi_rad = np.zeros(self.nrbin + 1), np.zeros(self.nrbin + 1)
+        self.ctx.check(_lib.lib().aph_monopole_get_tables(
+            self.ctx.h, self.h, m_enc.ctypes.data_as(ctypes.c_void_p),
+            phi_rad.ctypes.data_as(ctypes.c_void_p)))
+        return m_enc, phi_rad
+
+
+def apply_gravity_field(md, grav, dt):
+    """ApplyGravity (gravity.cpp:374-410) with a per-cell field (3 components)."""
+    md.ctx.check(_lib.lib().aph_apply_gravity_field(md.ctx.h, md.dbs, md.first, md.count,
+                                                    _ptr(md.cons), _ptr(md.prim),
+                                                    _ptr(grav), dt))
+    return TaskStatus.complete
+
+
 # ------------------------------------------------------------------ drivers
 class ApophisDriver:
     """Stage sequence of ApophisDriver::MakeTaskCollection, plain form, for the
@@ -379,7 +506,8 @@ class ApophisDriver:
     explicit list of (first_block, nblocks) ranges instead (e.g. one pack per block size,
     the reference's packs needing blocks of one shape, hydro.cpp:537-549)."""
 
-    def __init__(self, pkg, md, g=(0.0, 0.0, 0.0), pack_size=None, amr=None, partitions=None):
+    def __init__(self, pkg, md, g=(0.0, 0.0, 0.0), pack_size=None, amr=None, partitions=None,
+                 gravity=None):
         self.pkg = pkg
         self.md = md
         self.amr = amr  # amr.AmrPlan of a multilevel mesh, or None
@@ -387,6 +515,15 @@ class ApophisDriver:
         self.cfl = pkg.param("cfl")
         self.g = tuple(float(v) for v in g)
         self.u1 = torch.zeros_like(md.cons)
+        # gravity: a package from initialize_gravity; "monopole" runs its solver each stage
+        # and keeps the "gravity" and "gravity.phi" fields on md (md.grav, md.phi)
+        self.mono = None
+        if gravity is not None and gravity.param("gravity") == Gravity.monopole:
+            self.mono = MonopoleGravitySolver.from_package(md.ctx, gravity, md.bs.ndim)
+            md.grav = torch.zeros(md.bs.field_size(3), dtype=torch.float64,
+                                  device=md.cons.device)
+            md.phi = torch.zeros(md.bs.field_size(1), dtype=torch.float64,
+                                 device=md.cons.device)
         nb = md.bs.nblocks
         if partitions is not None:
             cover = sorted((int(f), int(c)) for f, c in partitions)
@@ -430,6 +567,8 @@ class ApophisDriver:
         flux_fun = self.pkg.param("flux_first_stage" if stage == 1 else "flux_other_stage")
         grav = any(v != 0.0 for v in self.g)
         rk = self.rk
+        if self.mono is not None:
+            self.mono.add_tasks(self.parts, self.md.grav, self.md.phi)      # :64-66
         for p in self.parts:
             flux_fun(p)                                                      # :70-72
         if self.amr is not None:
@@ -437,7 +576,9 @@ class ApophisDriver:
         for p in self.parts:
             update_with_flux_divergence(p, self.u1, rk["gam0"][s], rk["gam1"][s],
                                         rk["beta"][s] * dt)                  # :79-82
-            if grav:
+            if self.mono is not None:
+                apply_gravity_field(p, self.md.grav, rk["beta"][s] * dt)     # :84-85
+            elif grav:
                 apply_gravity(p, self.g, rk["beta"][s] * dt)                 # :84-85
         for p in self.parts:
             fill_ghosts(p)                                                   # :94, :103
@@ -480,7 +621,8 @@ class FusedDriver:
     stage); u1 holds the stage-1 result.  Ghost cells are never read."""
 
     def __init__(self, ctx, bs: BlockSet, cons, recon, riemann, eos, cfl, g=(0.0, 0.0, 0.0),
-                 tlim=np.inf, device="cuda", dbs=None, integrator="rk2", amr_mesh=None):
+                 tlim=np.inf, device="cuda", dbs=None, integrator="rk2", amr_mesh=None,
+                 monopole=None):
         self.ctx = ctx
         self.bs = bs
         self.device_bs = dbs or DeviceBlockSet(ctx, bs)
@@ -501,6 +643,10 @@ class FusedDriver:
         self.graph_steps = 0
         if integrator != "rk2":
             self.set_integrator(integrator)
+        self.mono = None
+        if monopole is not None:  # (nrbin, rmax, gravity_constant): monopole gravity
+            self.mono = MonopoleGravitySolver(ctx, bs.ndim, *monopole)
+            ctx.check(_lib.lib().aph_plan_set_monopole(ctx.h, self.plan, self.mono.h))
         self.amr = None
         if amr_mesh is not None:  # a multilevel mesh (amr.AmrMesh) of this block set
             from . import amr as _amr

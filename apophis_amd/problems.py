@@ -38,6 +38,7 @@ class Problem:
     g: tuple = (0.0, 0.0, 0.0)
     meta: dict = dataclasses.field(default_factory=dict)
     integrator: str = "rk2"  # rk1 / rk2 / rk3 / vl2 (main.hpp Integrator)
+    monopole: tuple = None  # monopole gravity (nrbin, rmax, gravity_constant), or None
 
 
 # ---------------------------------------------------------------- RNG
@@ -129,6 +130,63 @@ def blast3d(nblk=8, n=32, seed=20240601, recon="ppm", riemann="hllc", cfl=0.3,
         _set_interior(bs, cons, b, rho, 0.0, 0.0, 0.0, pres / gm1)
     return Problem("blast3d", bs, cons, make_eos(gm1), recon, riemann, cfl,
                    nsteps=nsteps, meta={"seed": seed, "N": N})
+
+
+# ---------------------------------------------------------------- monopole gravity
+def _default_rmax(xmin, xmax):
+    """gravity/rmax's default (gravity.cpp:262-272): the distance from the origin to the
+    farthest corner of the mesh box."""
+    edge = [max(abs(a), abs(b)) for a, b in zip(xmin, xmax)]
+    return float(np.sqrt(edge[0] * edge[0] + edge[1] * edge[1] + edge[2] * edge[2]))
+
+
+def grav_sphere(res=32, block=None, ndim=3, rho0=1.0, r0=0.25, nrbin=None, G=1.0,
+                gm1=0.666666667, recon="dc", riemann="hlle", cfl=0.4, center=(0.0, 0.0, 0.0)):
+    """The gravity accuracy test (grav_test.cpp:15-55, pid 1; inputs/grav_test.in with the
+    monopole solver as tst/.../gravity/gravity.py runs it): rho = rho0 (1 - r^2/r0^2)^2
+    inside r0 and 0 outside (ConsToPrim's floor then applies), at rest, E = 1, on
+    [-1/2, 1/2]^ndim with outflow boundaries; monopole gravity with nrbin = 4 res."""
+    block = block or res
+    nb = res // block
+    nblocks = tuple(nb if d < ndim else 1 for d in range(3))
+    bnx = tuple(block if d < ndim else 1 for d in range(3))
+    xmin, xmax = (-0.5,) * 3, (0.5,) * 3
+    m = lattice_mesh(ndim, nblocks, bnx, NEED_GHOST[recon], xmin, xmax, ["outflow"] * 6)
+    bs = BlockSet(ndim, [m])
+    cons = np.zeros(bs.field_size(bs.nvar))
+    for b in range(bs.nblocks):
+        x, y, z = bs.cell_centers(b)
+        r = np.sqrt((x - center[0]) ** 2 + (y - center[1]) ** 2 + (z - center[2]) ** 2)
+        rho = np.where(r <= r0, rho0 * (1.0 - r * r * (1.0 / (r0 * r0))) ** 2, 0.0)
+        _set_interior(bs, cons, b, rho, 0.0, 0.0, 0.0, 1.0)
+    # the mesh box of the deck on every axis (gravity.cpp reads x2/x3 extents too)
+    rmax = _default_rmax(xmin, xmax)
+    return Problem("grav_sphere", bs, cons, make_eos(gm1), recon, riemann, cfl,
+                   integrator="rk1", monopole=(nrbin or 4 * res, rmax, G),
+                   meta={"rho0": rho0, "r0": r0, "res": res, "box": (xmin, xmax)})
+
+
+def evrard(res=32, block=16, mass=1.0, radius=1.0, u_therm=0.05, nrbin=None, G=1.0,
+           gm1=0.666666667, recon="plm", riemann="hllc", cfl=0.4, tlim=0.8):
+    """Evrard's collapse (evrards_collapse.cpp:15-49, inputs/evrards_collapse.in) with the
+    monopole solver as tst/.../evrards_collapse/evrards_collapse.py runs it: rho =
+    M / (2 pi R^2 r) inside R and 1e-5 outside, at rest, E = u_therm rho, on [-1, 1]^3
+    (res^3 cells in blocks of block^3), outflow, PLM + HLLC, RK2, CFL 0.4, to t = 0.8,
+    nrbin = 4 res."""
+    nb = res // block
+    xmin, xmax = (-1.0,) * 3, (1.0,) * 3
+    m = lattice_mesh(3, (nb,) * 3, (block,) * 3, NEED_GHOST[recon], xmin, xmax,
+                     ["outflow"] * 6)
+    bs = BlockSet(3, [m])
+    cons = np.zeros(bs.field_size(bs.nvar))
+    for b in range(bs.nblocks):
+        x, y, z = bs.cell_centers(b)
+        r = np.sqrt(x * x + y * y + z * z)
+        rho = np.where(r < radius, mass / (2.0 * np.pi * (radius * radius) * r), 1e-5)
+        _set_interior(bs, cons, b, rho, 0.0, 0.0, 0.0, u_therm * rho)
+    return Problem("evrard", bs, cons, make_eos(gm1), recon, riemann, cfl, tlim=tlim,
+                   monopole=(nrbin or 4 * res, _default_rmax(xmin, xmax), G),
+                   meta={"res": res, "radius": radius, "box": (xmin, xmax)})
 
 
 # ---------------------------------------------------------------- uneven blocks

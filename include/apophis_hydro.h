@@ -87,6 +87,7 @@ typedef struct aph_eos {
 typedef struct aph_context aph_context;
 typedef struct aph_blockset aph_blockset;
 typedef struct aph_plan aph_plan;
+typedef struct aph_monopole aph_monopole;
 
 /* ---------------------------------------------------------------- context */
 int aph_abi_version(void);
@@ -141,6 +142,41 @@ int aph_update_with_flux_divergence(aph_context *ctx, const aph_blockset *bs,
 int aph_apply_gravity(aph_context *ctx, const aph_blockset *bs, int32_t first_block,
                       int32_t nblocks, double *cons, const double *prim,
                       const double g[3], double dt);
+
+/* ApplyGravity (gravity.cpp:374-410) with a per-cell acceleration field: grav holds 3
+   components in the pack layout (3 * cell_base + v * ncell(b) + cell), e.g. from
+   aph_monopole_gravity_field; prim is the stage-start primitive state. */
+int aph_apply_gravity_field(aph_context *ctx, const aph_blockset *bs, int32_t first_block,
+                            int32_t nblocks, double *cons, const double *prim,
+                            const double *grav, double dt);
+
+/* Monopole self-gravity: MonopoleGravitySolver (src/gravity/gsolvers/monopole_gravity.hpp,
+   gravity type "monopole", gravity.cpp:236-238,262-272).  nrbin radial bins of width
+   rmax / nrbin (the reference's defaults: nrbin = mesh nx1, rmax = distance to the
+   farthest mesh corner); gravity_constant is G.  The per-stage tasks, in order:
+     aph_monopole_zero_radial_bins    ZeroRadialBins (:107-112) over the whole block set
+                                      (it also fixes the scale of the sums from prim)
+     aph_monopole_accumulate_mass     AccumulateMass (:114-153), per partition
+     aph_monopole_build_radial_tables BuildRadialTables (:155-213), once
+     aph_monopole_gravity_field       ComputeGravityVectorDirect (:215-302), per partition
+   Bin masses are summed exactly (each cell mass rho*V rounded to a multiple of 2^-s, s
+   about 100 bits below the largest, summed as 128-bit integers), so they do not depend on
+   partitions or order; the reference's floating-point atomics do, in the last bits. */
+int aph_monopole_create(aph_context *ctx, int32_t ndim, int32_t nrbin, double rmax,
+                        double gravity_constant, aph_monopole **out);
+int aph_monopole_destroy(aph_context *ctx, aph_monopole *m);
+int aph_monopole_zero_radial_bins(aph_context *ctx, aph_monopole *m, const aph_blockset *bs,
+                                  const double *prim);
+int aph_monopole_accumulate_mass(aph_context *ctx, aph_monopole *m, const aph_blockset *bs,
+                                 int32_t first_block, int32_t nblocks, const double *prim);
+int aph_monopole_build_radial_tables(aph_context *ctx, aph_monopole *m);
+/* grav: 3 components (pack layout); phi ("gravity.phi", 1 component) may be NULL. */
+int aph_monopole_gravity_field(aph_context *ctx, aph_monopole *m, const aph_blockset *bs,
+                               int32_t first_block, int32_t nblocks, double *grav,
+                               double *phi);
+/* Copy m_enc[0..nrbin] and phi_rad[0..nrbin] to host arrays (synchronous). */
+int aph_monopole_get_tables(aph_context *ctx, aph_monopole *m, double *m_enc,
+                            double *phi_rad);
 
 /* Ghost exchange between neighbouring blocks followed by the physical boundary
    conditions (outflow / reflect, apophis_boundaries.cpp:17-92), axis by axis. */
@@ -353,6 +389,12 @@ int aph_plan_get_launch(aph_context *ctx, aph_plan *plan, int32_t block, int32_t
  * state register inside the plan.  Synchronous.  The adjoint entry points follow the
  * plan's integrator. */
 int aph_plan_set_integrator(aph_context *ctx, aph_plan *plan, int32_t integrator);
+/* Monopole gravity for every stage of the plan (replaces the constant g): before each
+   stage the solver bins the stage input's density (ConsToPrim's floor), builds its tables
+   and the field, and the stage adds the source with that field.  m must outlive the plan
+   (or a later call with m = NULL, which returns to the plan's constant g).  The adjoint
+   refuses such plans.  Synchronous (allocates the field). */
+int aph_plan_set_monopole(aph_context *ctx, aph_plan *plan, aph_monopole *m);
 /* Stage `stage` (1..number of stages of the plan's integrator):
  * u_out = gam0*u_in + gam1*u_n + beta*dt*(-div F(u_in)) [+ gravity]; u_n (the step's
  * start state) is unused in stage 1. */

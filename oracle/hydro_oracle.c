@@ -14,7 +14,9 @@
 #include "hydro_oracle.h"
 
 #include <float.h>
+#include <limits.h>
 #include <math.h>
+#include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -753,4 +755,332 @@ double orc_rk2_step(const aph_block *blocks, int nb, int ndim, int nvar, int rec
                     double *fx1, double *fx2, double *fx3) {
   return orc_step(blocks, nb, ndim, nvar, 2, recon, rsolver, eos, g, cfl, dt, u0, u1, prim,
                   gamma, fx1, fx2, fx3);
+}
+
+/* ------------------------------------------------ monopole gravity (f4)
+ * src/gravity/gsolvers/monopole_gravity.hpp.  AccumulateMass (:114-153) bins the
+ * interior cells' masses rho*V by the cell-centre radius into nrbin shells of width
+ * rmax/nrbin; BuildRadialTables (:155-213) forms the enclosed mass m_enc and the radial
+ * potential phi_rad; ComputeGravityVectorDirect (:215-302) gives each cell g and phi;
+ * ApplyGravity (gravity.cpp:374-410) adds g*rho*dt to the momenta and (g.v)*rho*dt to
+ * the energy.
+ *
+ * The reference sums a bin's masses with floating-point atomics, in no fixed order.
+ * Here (and on the GPU) each mass is rounded to an integer multiple of 2^-s and summed
+ * as a 128-bit integer, which any order adds up to the same value; s is chosen per
+ * call from the largest mass so that no sum can overflow (2^125 bound), which leaves
+ * about 100 bits below the largest mass.  A bin's mass is that sum rounded once to a
+ * double: for masses that are multiples of 2^-s it is the correctly rounded exact sum
+ * (math.fsum), and otherwise within (cells/2)*2^-s of it. */
+typedef unsigned __int128 orc_u128;
+
+/* rho*V*2^s rounded to nearest (ties to even) as a two's complement 128-bit integer. */
+static orc_u128 mono_quantize(double dm, int s) {
+  if (dm == 0.0) return 0;
+  uint64_t bits;
+  memcpy(&bits, &dm, sizeof bits);
+  const int neg = (int)(bits >> 63);
+  int ex = (int)((bits >> 52) & 0x7ff);
+  uint64_t man = bits & ((1ull << 52) - 1);
+  if (ex == 0)
+    ex = 1; /* subnormal */
+  else
+    man |= 1ull << 52;
+  const int k = ex - 1075 + s; /* |dm| * 2^s = man * 2^k */
+  orc_u128 q;
+  if (k >= 0) {
+    q = (orc_u128)man << k;
+  } else if (-k >= 64) {
+    q = 0; /* man < 2^53 <= 2^(-k-1): below one half */
+  } else {
+    const int sh = -k;
+    uint64_t m = man >> sh;
+    const uint64_t rem = man & ((1ull << sh) - 1), half = 1ull << (sh - 1);
+    if (rem > half || (rem == half && (m & 1))) ++m;
+    q = m;
+  }
+  return neg ? (orc_u128)0 - q : q;
+}
+
+/* The 128-bit sum (two's complement) times 2^-s, rounded once to nearest even. */
+static double mono_to_double(orc_u128 q, int s) {
+  const int neg = (int)(q >> 127);
+  if (neg) q = (orc_u128)0 - q;
+  if (q == 0) return 0.0;
+  int t = 127;
+  while (!((q >> t) & 1)) --t;
+  uint64_t m;
+  int e = 0;
+  if (t <= 52) {
+    m = (uint64_t)q;
+  } else {
+    const int sh = t - 52;
+    m = (uint64_t)(q >> sh);
+    const orc_u128 rem = q & (((orc_u128)1 << sh) - 1), half = (orc_u128)1 << (sh - 1);
+    if (rem > half || (rem == half && (m & 1))) ++m;
+    if (m == (1ull << 53)) {
+      m >>= 1;
+      ++e;
+    }
+    e += sh;
+  }
+  const double r = ldexp((double)m, e - s);
+  return neg ? -r : r;
+}
+
+/* Binary exponent bound of a mass: |dm| < 2^(e+1). */
+static int mono_exponent(double dm) {
+  uint64_t bits;
+  memcpy(&bits, &dm, sizeof bits);
+  const int ex = (int)((bits >> 52) & 0x7ff);
+  return (ex == 0 ? 1 : ex) - 1023;
+}
+
+/* The scale: the sum of `ncells` masses below 2^(emax+1) stays below 2^125. */
+int orc_mono_scale(int emax, long ncells) {
+  if (emax == INT_MIN) return 0;
+  int L = 0;
+  while ((1L << L) < ncells) ++L;
+  return 125 - L - (emax + 1);
+}
+
+/* Cell centre of interior cell (i, j, k) of a block, on every axis (the inactive axes'
+ * single cell is centred on the domain). */
+static void mono_centre(const aph_block *bk, int i, int j, int k, double *x, double *y,
+                        double *z) {
+  *x = bk->xmin[0] + (i + 0.5) * bk->dx[0];
+  *y = bk->xmin[1] + (j + 0.5) * bk->dx[1];
+  *z = bk->xmin[2] + (k + 0.5) * bk->dx[2];
+}
+
+/* AccumulateMass: bin masses of every interior cell with r < rmax (density from the
+ * primitive field, i.e. after ConsToPrim's floor), summed as above.  Returns s. */
+int orc_mono_bin_masses(const aph_block *blocks, int nb, int ndim, int nvar,
+                        const double *prim, int nrbin, double rmax, double *mbin) {
+  const double dr = rmax / nrbin;
+  long ncells = 0;
+  int emax = INT_MIN;
+  for (int pass = 0; pass < 2; ++pass) {
+    int s = 0;
+    orc_u128 *acc = NULL;
+    if (pass == 1) {
+      s = orc_mono_scale(emax, ncells);
+      acc = (orc_u128 *)calloc((size_t)nrbin, sizeof(orc_u128));
+    }
+    for (int b = 0; b < nb; ++b) {
+      const aph_block *bk = &blocks[b];
+      const int Nx = ext(bk, 0, ndim), Ny = ext(bk, 1, ndim);
+      const int gx = gst(bk, 0, ndim), gy = gst(bk, 1, ndim), gz = gst(bk, 2, ndim);
+      const double *w = cfld(prim, bk, ndim, nvar, 0);
+      if (pass == 0) ncells += (long)bk->nx[0] * bk->nx[1] * bk->nx[2];
+      for (int k = 0; k < bk->nx[2]; ++k)
+        for (int j = 0; j < bk->nx[1]; ++j)
+          for (int i = 0; i < bk->nx[0]; ++i) {
+            double x, y, z;
+            mono_centre(bk, i, j, k, &x, &y, &z);
+            const double r = sqrt(x * x + y * y + z * z);
+            if (r >= rmax) continue;
+            const long c = ((long)(k + gz) * Ny + (j + gy)) * Nx + (i + gx);
+            const double dm = w[c] * bk->vol;
+            if (pass == 0) {
+              if (dm != 0.0 && mono_exponent(dm) > emax) emax = mono_exponent(dm);
+              continue;
+            }
+            int bin = (int)(r / dr);
+            if (bin < 0) bin = 0;
+            if (bin >= nrbin) bin = nrbin - 1;
+            acc[bin] += mono_quantize(dm, s);
+          }
+    }
+    if (pass == 1) {
+      for (int i = 0; i < nrbin; ++i) mbin[i] = mono_to_double(acc[i], s);
+      free(acc);
+      return s;
+    }
+  }
+  return 0;
+}
+
+/* BuildRadialTables (monopole_gravity.hpp:155-213): m_enc[0..nrbin], phi_rad[0..nrbin]
+ * from the bin masses, both recurrences in the reference's order. */
+void orc_mono_tables(int nrbin, int ndim, double rmax, double G, const double *mbin,
+                     double *m_enc, double *phi_rad) {
+  const double dr = rmax / nrbin;
+  m_enc[0] = 0.0;
+  for (int i = 0; i < nrbin; ++i) m_enc[i + 1] = m_enc[i] + mbin[i];
+  const double mtot = m_enc[nrbin];
+  phi_rad[nrbin] = (rmax > 0.0 && ndim == 3) ? -G * mtot / rmax : 0.0;
+  for (int i = nrbin - 1; i >= 0; --i) {
+    const double r_inner = i * dr, r_outer = (i + 1) * dr;
+    double dphi = 0.0;
+    if (ndim == 3) {
+      const double m_inner = m_enc[i];
+      const double shell_mass = m_enc[i + 1] - m_inner;
+      if (r_inner == 0.0) {
+        dphi = -G * shell_mass / (2.0 * r_outer);
+      } else {
+        const double denom = r_outer * r_outer * r_outer - r_inner * r_inner * r_inner;
+        dphi = -G * (m_inner * (1.0 / r_inner - 1.0 / r_outer) +
+                     shell_mass / denom *
+                         (0.5 * (r_outer * r_outer - r_inner * r_inner) +
+                          r_inner * r_inner * r_inner * (1.0 / r_outer - 1.0 / r_inner)));
+      }
+    } else {
+      const double r_mid = 0.5 * (r_inner + r_outer);
+      const double m_mid = 0.5 * (m_enc[i] + m_enc[i + 1]);
+      double gr = 0.0;
+      if (r_mid > 0.0) gr = (ndim == 1) ? -G * m_mid : -2.0 * G * m_mid / r_mid;
+      dphi = gr * dr;
+    }
+    phi_rad[i] = phi_rad[i + 1] + dphi;
+  }
+}
+
+/* ComputeGravityVectorDirect for one point (x, y, z: the cell centre, y and z zero on
+ * inactive axes): g[3] and phi. */
+void orc_mono_point(int nrbin, int ndim, double rmax, double G, const double *m_enc,
+                    const double *phi_rad, double x, double y, double z, double *g,
+                    double *phi) {
+  const double dr = rmax / nrbin;
+  const double r2 = x * x + y * y + z * z;
+  const double r = sqrt(r2);
+  if (r < 1e-10) {
+    g[0] = g[1] = g[2] = 0.0;
+    *phi = phi_rad[0];
+    return;
+  }
+  double m_enclosed, phi_val;
+  if (r >= rmax) {
+    m_enclosed = m_enc[nrbin];
+    phi_val = (ndim == 3) ? -G * m_enclosed / r : phi_rad[nrbin];
+  } else {
+    const double rf = r / dr;
+    int bin = (int)rf;
+    if (bin < 0) bin = 0;
+    if (bin >= nrbin) bin = nrbin - 1;
+    const double r_inner = bin * dr, r_outer = (bin + 1) * dr;
+    const double shell_mass = m_enc[bin + 1] - m_enc[bin];
+    double mass_frac;
+    if (ndim == 1) {
+      mass_frac = (r - r_inner) / dr;
+    } else if (ndim == 2) {
+      const double denom = r_outer * r_outer - r_inner * r_inner;
+      mass_frac = (r * r - r_inner * r_inner) / denom;
+    } else {
+      const double r_inner3 = r_inner * r_inner * r_inner;
+      const double r_outer3 = r_outer * r_outer * r_outer;
+      mass_frac = (r * r * r - r_inner3) / (r_outer3 - r_inner3);
+    }
+    if (mass_frac < 0.0) mass_frac = 0.0;
+    if (mass_frac > 1.0) mass_frac = 1.0;
+    m_enclosed = m_enc[bin] + shell_mass * mass_frac;
+    const double frac = rf - bin;
+    phi_val = (1.0 - frac) * phi_rad[bin] + frac * phi_rad[bin + 1];
+  }
+  double g_r;
+  if (ndim == 1)
+    g_r = -G * m_enclosed;
+  else if (ndim == 2)
+    g_r = -2.0 * G * m_enclosed / r;
+  else
+    g_r = -G * m_enclosed / r2;
+  const double rinv = 1.0 / r;
+  g[0] = g_r * x * rinv;
+  g[1] = (ndim >= 2) ? g_r * y * rinv : 0.0;
+  g[2] = (ndim >= 3) ? g_r * z * rinv : 0.0;
+  *phi = phi_val;
+}
+
+/* The gravity field (3 components) and phi (1 component, may be NULL) of every
+ * interior cell, in the pack layout. */
+void orc_mono_field(const aph_block *blocks, int nb, int ndim, int nrbin, double rmax,
+                    double G, const double *m_enc, const double *phi_rad, double *grav,
+                    double *phi) {
+#pragma omp parallel for schedule(dynamic)
+  for (int b = 0; b < nb; ++b) {
+    const aph_block *bk = &blocks[b];
+    const int Nx = ext(bk, 0, ndim), Ny = ext(bk, 1, ndim);
+    const long nc = ncell(bk, ndim);
+    const int gx = gst(bk, 0, ndim), gy = gst(bk, 1, ndim), gz = gst(bk, 2, ndim);
+    double *gf = fld(grav, bk, ndim, 3, 0);
+    double *pf = phi ? fld(phi, bk, ndim, 1, 0) : NULL;
+    for (int k = 0; k < bk->nx[2]; ++k)
+      for (int j = 0; j < bk->nx[1]; ++j)
+        for (int i = 0; i < bk->nx[0]; ++i) {
+          double x, y, z, g[3], p;
+          mono_centre(bk, i, j, k, &x, &y, &z);
+          if (ndim < 2) y = 0.0;
+          if (ndim < 3) z = 0.0;
+          orc_mono_point(nrbin, ndim, rmax, G, m_enc, phi_rad, x, y, z, g, &p);
+          const long c = ((long)(k + gz) * Ny + (j + gy)) * Nx + (i + gx);
+          gf[c] = g[0];
+          gf[nc + c] = g[1];
+          gf[2 * nc + c] = g[2];
+          if (pf) pf[c] = p;
+        }
+  }
+}
+
+/* ApplyGravity (gravity.cpp:374-410) with a per-cell field (3 components, pack layout). */
+void orc_apply_gravity_field(const aph_block *blocks, int nb, int ndim, int nvar,
+                             double *cons, const double *prim, const double *grav,
+                             double dt) {
+#pragma omp parallel for schedule(dynamic)
+  for (int b = 0; b < nb; ++b) {
+    const aph_block *bk = &blocks[b];
+    const int Nx = ext(bk, 0, ndim), Ny = ext(bk, 1, ndim);
+    const long nc = ncell(bk, ndim);
+    const int gx = gst(bk, 0, ndim), gy = gst(bk, 1, ndim), gz = gst(bk, 2, ndim);
+    double *u = fld(cons, bk, ndim, nvar, 0);
+    const double *w = cfld(prim, bk, ndim, nvar, 0);
+    const double *gf = cfld(grav, bk, ndim, 3, 0);
+    for (int k = gz; k < gz + bk->nx[2]; ++k)
+      for (int j = gy; j < gy + bk->nx[1]; ++j)
+        for (int i = gx; i < gx + bk->nx[0]; ++i) {
+          const long c = ((long)k * Ny + j) * Nx + i;
+          const double g0 = gf[c], g1 = gf[nc + c], g2 = gf[2 * nc + c];
+          u[nc + c] += g0 * w[c] * dt;
+          double de = g0 * w[nc + c];
+          if (ndim >= 2) {
+            u[2 * nc + c] += g1 * w[c] * dt;
+            de += g1 * w[2 * nc + c];
+          }
+          if (ndim >= 3) {
+            u[3 * nc + c] += g2 * w[c] * dt;
+            de += g2 * w[3 * nc + c];
+          }
+          u[4 * nc + c] += de * w[c] * dt;
+        }
+  }
+}
+
+/* One step of `integrator` with monopole gravity (apophis_driver.cpp:64-85: each stage
+ * bins the stage-start primitive density, builds the tables and the field, then the
+ * update and ApplyGravity with beta*dt).  grav: scratch of 3 components (pack layout);
+ * m_enc / phi_rad (nrbin + 1 each) hold the last stage's tables on return. */
+double orc_step_monopole(const aph_block *blocks, int nb, int ndim, int nvar, int integrator,
+                         int recon, int rsolver, const aph_eos *eos, int nrbin, double rmax,
+                         double G, double cfl, double dt, double *u0, double *u1,
+                         double *prim, double *gamma, double *fx1, double *fx2, double *fx3,
+                         double *grav, double *m_enc, double *phi_rad) {
+  double gam0[3], gam1[3], beta[3];
+  const int nstages = orc_integrator_stages(integrator, gam0, gam1, beta);
+  long ntot = 0;
+  for (int b = 0; b < nb; ++b) ntot += ncell(&blocks[b], ndim);
+  memcpy(u1, u0, sizeof(double) * (size_t)nvar * ntot);
+  double *mbin = (double *)malloc(sizeof(double) * (size_t)nrbin);
+  for (int s = 0; s < nstages; ++s) {
+    const int rec = (s == 0 && integrator == 4) ? APH_RECON_DC : recon;
+    orc_mono_bin_masses(blocks, nb, ndim, nvar, prim, nrbin, rmax, mbin);
+    orc_mono_tables(nrbin, ndim, rmax, G, mbin, m_enc, phi_rad);
+    orc_mono_field(blocks, nb, ndim, nrbin, rmax, G, m_enc, phi_rad, grav, NULL);
+    orc_calculate_fluxes(blocks, nb, ndim, nvar, rec, rsolver, prim, gamma, fx1, fx2, fx3);
+    orc_update(blocks, nb, ndim, nvar, u0, u1, fx1, fx2, fx3, gam0[s], gam1[s],
+               beta[s] * dt);
+    orc_apply_gravity_field(blocks, nb, ndim, nvar, u0, prim, grav, beta[s] * dt);
+    orc_fill_ghosts(blocks, nb, ndim, nvar, u0);
+    orc_cons_to_prim(blocks, nb, ndim, nvar, eos, u0, prim, gamma);
+  }
+  free(mbin);
+  return orc_estimate_timestep(blocks, nb, ndim, nvar, eos, prim, u0, cfl);
 }

@@ -67,6 +67,30 @@ double orc_step(const aph_block *blocks, int nb, int ndim, int nvar, int integra
                 double *fx2, double *fx3);
 int orc_integrator_stages(int integrator, double *gam0, double *gam1, double *beta);
 
+/* Monopole gravity (monopole_gravity.hpp): the per-call fixed-point scale, the bin
+ * masses (returns the scale s), the radial tables, one point's g/phi, the field of every
+ * interior cell (grav: 3 components, phi: 1 or NULL, pack layout), ApplyGravity with
+ * that field, and one integrator step with it. */
+int orc_mono_scale(int emax, long ncells);
+int orc_mono_bin_masses(const aph_block *blocks, int nb, int ndim, int nvar,
+                        const double *prim, int nrbin, double rmax, double *mbin);
+void orc_mono_tables(int nrbin, int ndim, double rmax, double G, const double *mbin,
+                     double *m_enc, double *phi_rad);
+void orc_mono_point(int nrbin, int ndim, double rmax, double G, const double *m_enc,
+                    const double *phi_rad, double x, double y, double z, double *g,
+                    double *phi);
+void orc_mono_field(const aph_block *blocks, int nb, int ndim, int nrbin, double rmax,
+                    double G, const double *m_enc, const double *phi_rad, double *grav,
+                    double *phi);
+void orc_apply_gravity_field(const aph_block *blocks, int nb, int ndim, int nvar,
+                             double *cons, const double *prim, const double *grav,
+                             double dt);
+double orc_step_monopole(const aph_block *blocks, int nb, int ndim, int nvar, int integrator,
+                         int recon, int rsolver, const aph_eos *eos, int nrbin, double rmax,
+                         double G, double cfl, double dt, double *u0, double *u1,
+                         double *prim, double *gamma, double *fx1, double *fx2, double *fx3,
+                         double *grav, double *m_enc, double *phi_rad);
+
 #ifdef __cplusplus
 }
 #endif

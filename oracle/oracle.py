@@ -60,6 +60,22 @@ def lib():
         L.orc_step.restype = ctypes.c_double
         L.orc_integrator_stages.argtypes = [ctypes.c_int, _dp, _dp, _dp]
         L.orc_integrator_stages.restype = ctypes.c_int
+        L.orc_mono_scale.argtypes = [ctypes.c_int, ctypes.c_long]
+        L.orc_mono_scale.restype = ctypes.c_int
+        L.orc_mono_bin_masses.argtypes = ([_vp] + [ctypes.c_int] * 3 +
+                                          [_dp, ctypes.c_int, ctypes.c_double, _dp])
+        L.orc_mono_bin_masses.restype = ctypes.c_int
+        L.orc_mono_tables.argtypes = ([ctypes.c_int] * 2 + [ctypes.c_double] * 2 +
+                                      [_dp] * 3)
+        L.orc_mono_point.argtypes = ([ctypes.c_int] * 2 + [ctypes.c_double] * 2 + [_dp] * 2 +
+                                     [ctypes.c_double] * 3 + [_dp] * 2)
+        L.orc_mono_field.argtypes = ([_vp] + [ctypes.c_int] * 3 + [ctypes.c_double] * 2 +
+                                     [_dp] * 4)
+        L.orc_apply_gravity_field.argtypes = ([_vp] + [ctypes.c_int] * 3 + [_dp] * 3 +
+                                              [ctypes.c_double])
+        L.orc_step_monopole.argtypes = ([_vp] + [ctypes.c_int] * 6 + [_vp, ctypes.c_int] +
+                                        [ctypes.c_double] * 4 + [_dp] * 10)
+        L.orc_step_monopole.restype = ctypes.c_double
         _lib = L
     return _lib
 
@@ -128,6 +144,11 @@ class State:
         self.gamma = np.zeros(bs.field_size(2))
         self.fx = [np.zeros_like(self.u0) for _ in range(3)]
         self.g = np.ascontiguousarray(prob.g, dtype=np.float64)
+        self.mono = getattr(prob, "monopole", None)  # (nrbin, rmax, G) or None
+        if self.mono is not None:
+            self.grav = np.zeros(bs.field_size(3))
+            self.m_enc = np.zeros(int(self.mono[0]) + 1)
+            self.phi_rad = np.zeros(int(self.mono[0]) + 1)
         fill_ghosts(bs, self.u0)
         cons_to_prim(bs, prob.eos, self.u0, self.prim, self.gamma)
 
@@ -139,6 +160,15 @@ class State:
         need = {"dc": 1, "plm": 2, "ppm": 3}[p.recon]
         if int(self.bs.blocks["ng"].min()) < need:
             raise ValueError("Not enough ghost zones for reconstruction")
+        if self.mono is not None:
+            nrbin, rmax, G = self.mono
+            return lib().orc_step_monopole(
+                _bp(self.bs), self.bs.nblocks, self.bs.ndim, self.bs.nvar,
+                INTEGRATOR[p.integrator], RECON[p.recon], RIEMANN[p.riemann],
+                p.eos.ctypes.data_as(_vp), int(nrbin), float(rmax), float(G), p.cfl, dt,
+                _p(self.u0), _p(self.u1), _p(self.prim), _p(self.gamma), _p(self.fx[0]),
+                _p(self.fx[1]), _p(self.fx[2]), _p(self.grav), _p(self.m_enc),
+                _p(self.phi_rad))
         return lib().orc_step(_bp(self.bs), self.bs.nblocks, self.bs.ndim, self.bs.nvar,
                               INTEGRATOR[p.integrator], RECON[p.recon], RIEMANN[p.riemann],
                               p.eos.ctypes.data_as(_vp), _p(self.g), p.cfl, dt,
@@ -205,3 +235,44 @@ def estimate_timestep(bs, eos, prim, cons, cfl):
 
 def global_sum(bs, cons, var):
     return lib().orc_global_sum(_bp(bs), bs.nblocks, bs.ndim, bs.nvar, _p(cons), var)
+
+
+# ------------------------------------------------------------ monopole gravity
+def mono_scale(emax, ncells):
+    return lib().orc_mono_scale(int(emax), int(ncells))
+
+
+def mono_bin_masses(bs, prim, nrbin, rmax):
+    """(bin masses, fixed-point scale s) of AccumulateMass over every block."""
+    mbin = np.zeros(int(nrbin))
+    s = lib().orc_mono_bin_masses(_bp(bs), bs.nblocks, bs.ndim, bs.nvar, _p(prim),
+                                  int(nrbin), float(rmax), _p(mbin))
+    return mbin, s
+
+
+def mono_tables(nrbin, ndim, rmax, G, mbin):
+    m_enc, phi_rad = np.zeros(int(nrbin) + 1), np.zeros(int(nrbin) + 1)
+    lib().orc_mono_tables(int(nrbin), int(ndim), float(rmax), float(G),
+                          _p(np.ascontiguousarray(mbin, dtype=np.float64)), _p(m_enc),
+                          _p(phi_rad))
+    return m_enc, phi_rad
+
+
+def mono_point(nrbin, ndim, rmax, G, m_enc, phi_rad, x, y, z):
+    g, phi = np.zeros(3), np.zeros(1)
+    lib().orc_mono_point(int(nrbin), int(ndim), float(rmax), float(G), _p(m_enc),
+                         _p(phi_rad), float(x), float(y), float(z), _p(g), _p(phi))
+    return g, float(phi[0])
+
+
+def mono_field(bs, nrbin, rmax, G, m_enc, phi_rad):
+    """(gravity field, 3 components; phi, 1 component), interior cells, pack layout."""
+    grav, phi = np.zeros(bs.field_size(3)), np.zeros(bs.field_size(1))
+    lib().orc_mono_field(_bp(bs), bs.nblocks, bs.ndim, int(nrbin), float(rmax), float(G),
+                         _p(m_enc), _p(phi_rad), _p(grav), _p(phi))
+    return grav, phi
+
+
+def apply_gravity_field(bs, cons, prim, grav, dt):
+    lib().orc_apply_gravity_field(_bp(bs), bs.nblocks, bs.ndim, bs.nvar, _p(cons),
+                                  _p(prim), _p(grav), dt)

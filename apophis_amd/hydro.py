@@ -596,11 +596,22 @@ class ApophisDriver:
             est = self.make_task_collection(stage, dt)
         return est
 
-    def evolve(self, tlim=None, nlim=None, fixed_dt=None):
-        """EvolutionDriver loop with the upstream dt policy (<= 2x growth, clip to tlim)."""
+    def history_sums(self):
+        """The hydro package's history variables (hydro.cpp:336-346): total_mass and
+        total_energy, GlobalQuantHst sums of rho V and E V over every block."""
+        return [float(global_quant_hst(self.md, v).item()) for v in (0, 4)]
+
+    def evolve(self, tlim=None, nlim=None, fixed_dt=None, hst=None):
+        """EvolutionDriver loop with the upstream dt policy (<= 2x growth, clip to tlim).
+        hst: an output.HstOutput, written at the initial time and then whenever the time
+        reaches its next output time (its dt column: the step's dt, the initial estimate at
+        t = 0)."""
         t, n = 0.0, 0
         est = self.initialize()
         dt = np.inf
+        nb = self.md.bs.nblocks
+        if hst is not None:
+            hst.maybe_write(t, est, n, nb, self.history_sums)
         while (nlim is None or n < nlim) and (tlim is None or t < tlim * (1 - 1e-14)):
             if fixed_dt is not None:
                 dt = fixed_dt
@@ -611,6 +622,8 @@ class ApophisDriver:
             est = self.step(dt)
             t += dt
             n += 1
+            if hst is not None:
+                hst.maybe_write(t, dt, n, nb, self.history_sums)
         return t, n
 
 

@@ -4,9 +4,15 @@
   linear_wave.cpp's UserWorkAfterLoop computes (linear_wave.cpp:164-301) and the exact
   text layout of its ``linearwave-errors.dat``, which the reference's convergence test
   reads (convergence.py:120-132).
-* HistoryFile: the history sums the hydro package registers (GlobalQuantHst total_mass,
-  total_energy, hydro.cpp:336-346) written per step as a whitespace table.  The column
-  layout of Parthenon's own .hst writer is upstream and not restated bit for bit.
+* HstOutput: the history output (<parthenon/outputN> file_type = hst, e.g.
+  inputs/evrards_collapse.in:94-97) of the sums the hydro package registers
+  (GlobalQuantHst total_mass, total_energy, hydro.cpp:336-346).  The file layout is
+  Parthenon's history writer (upstream outputs/history.cpp, not vendored in the reference):
+  "<basename>.out<N>.hst", a "#  History data" line, one header line of "[i]=label"
+  columns (time, dt, cycle, nbtotal, then the registered labels), and one row per output
+  time with each real printed by the output's data_format (default "%14.6e") after a
+  blank.  Restated from the upstream writer's published source; parity unpinned (the
+  reference holds no .hst file or reader to check it against).
 """
 from __future__ import annotations
 
@@ -54,17 +60,48 @@ def write_linear_wave_errors(path, nx, ncycle, err):
         f.write("\n")
 
 
-class HistoryFile:
-    """time, dt and the registered history sums, one row per call of write()."""
+class HstOutput:
+    """One history output block: writes a row whenever the simulation time reaches the
+    next output time (the first at the initial time), then advances it by dt.
 
-    def __init__(self, path, names=("total_mass", "total_energy")):
-        self.path = path
-        self.names = tuple(names)
-        with open(path, "w") as f:
-            cols = ["time", "dt"] + list(self.names)
-            f.write("# History data\n")
-            f.write("# " + "  ".join(f"[{i + 1}]={c}" for i, c in enumerate(cols)) + "\n")
+    basename / block_number / dt / data_format are the output block's file_basename
+    ("parthenon"), its number N (outputN), dt and data_format."""
 
-    def write(self, time, dt, values):
+    def __init__(self, directory, dt, block_number=0, basename="parthenon",
+                 labels=("total_mass", "total_energy"), data_format="%14.6e"):
+        self.path = os.path.join(directory, f"{basename}.out{int(block_number)}.hst")
+        self.dt = float(dt)
+        self.labels = tuple(labels)
+        self.fmt = " " + data_format  # a blank separates the columns
+        self.next_time = None
+        self.file_number = 0
+
+    def due(self, time):
+        return self.next_time is None or time >= self.next_time
+
+    def write(self, time, dt, ncycle, nbtotal, values):
+        """Append one row (and the header before the first)."""
         with open(self.path, "a") as f:
-            f.write("  ".join("%.14e" % v for v in [time, dt] + list(values)) + "\n")
+            if self.file_number == 0:
+                f.write("#  History data\n")
+                cols = ["time", "dt", "cycle", "nbtotal"]
+                f.write("# [1]=time     [2]=dt       [3]=cycle    [4]=nbtotal  ")
+                for i, lbl in enumerate(self.labels):
+                    f.write("[%d]=%-8s" % (len(cols) + 1 + i, lbl))
+                f.write("\n")
+            f.write(self.fmt % time)
+            f.write(self.fmt % dt)
+            f.write("  %d " % ncycle)
+            f.write("  %d " % nbtotal)
+            for v in values:
+                f.write(self.fmt % v)
+            f.write("\n")
+        self.file_number += 1
+        self.next_time = (time if self.next_time is None else self.next_time) + self.dt
+
+    def maybe_write(self, time, dt, ncycle, nbtotal, values_fn):
+        """write() if an output is due at `time`; values_fn() gives the sums."""
+        if self.due(time):
+            self.write(time, dt, ncycle, nbtotal, values_fn())
+            return True
+        return False

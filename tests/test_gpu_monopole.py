@@ -210,3 +210,40 @@ def test_adjoint_refuses_monopole(gpu_ctx):
     lam = torch.zeros_like(fd.u0)
     with pytest.raises(hydro._lib.HydroError, match="monopole"):
         st.adjoint_step_(fd.u0.clone(), lam)
+
+
+def test_evrard_history_output_matches_oracle_sums(gpu_ctx, tmp_path):
+    """The deck's history output (inputs/evrards_collapse.in:94-97: hst every 0.05) from
+    the plain driver: a row at t = 0 and whenever t passes a multiple of 0.05, and its
+    total_mass / total_energy equal to the oracle's sums of the same states."""
+    from apophis_amd import output
+
+    p = problems.evrard()
+    print("[case] monopole evrard hst", flush=True)
+    hst = output.HstOutput(str(tmp_path), dt=0.05, block_number=1)
+    drv = _plain(gpu_ctx, p)
+    t, n = drv.evolve(tlim=p.tlim, hst=hst)
+    torch.cuda.synchronize()
+    rows = np.genfromtxt(tmp_path / "parthenon.out1.hst")
+    ref = orc.State(p)
+    want, tt, k = [], 0.0, 0
+    want.append((0.0, 0, orc.global_sum(p.bs, ref.u0, 0), orc.global_sum(p.bs, ref.u0, 4)))
+    nxt = 0.05
+    dt_est, dt = ref.estimate_dt(), np.inf
+    while tt < p.tlim * (1 - 1e-14):
+        dt = min(2.0 * dt, dt_est)
+        if p.tlim - tt < dt:
+            dt = p.tlim - tt
+        dt_est = ref.step(dt)
+        tt += dt
+        k += 1
+        if tt >= nxt:
+            want.append((tt, k, orc.global_sum(p.bs, ref.u0, 0), orc.global_sum(p.bs, ref.u0, 4)))
+            nxt += 0.05
+    want = np.array(want)
+    assert rows.shape == (len(want), 6)
+    np.testing.assert_array_equal(rows[:, 2], want[:, 1])
+    np.testing.assert_array_equal(rows[:, 3], p.bs.nblocks)
+    # the file keeps 7 significant digits
+    np.testing.assert_allclose(rows[:, 0], want[:, 0], rtol=1e-6, atol=1e-12)
+    np.testing.assert_allclose(rows[:, 4:], want[:, 2:], rtol=1e-6)

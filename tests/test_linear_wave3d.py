@@ -51,11 +51,24 @@ def test_oracle_converges_and_error_file_layout(tmp_path):
     np.testing.assert_allclose(data[:, 4], errs, rtol=1e-6)  # %e keeps 7 digits
 
 
-def test_history_file(tmp_path):
-    h = output.HistoryFile(str(tmp_path / "run.hst"))
-    h.write(0.0, 0.1, [1.0, 2.5])
-    h.write(0.1, 0.1, [1.0, 2.4999999999999])
-    data = np.genfromtxt(tmp_path / "run.hst")
-    assert data.shape == (2, 4)
-    assert data[1, 3] == 2.4999999999999
-    assert open(tmp_path / "run.hst").readline().startswith("# History data")
+def test_hst_output_layout(tmp_path):
+    """The history writer's layout (upstream Parthenon's, restated; parity unpinned):
+    file name, the two header lines, blank-separated %14.6e reals, integer cycle and
+    block count, and the output times every dt from the first call."""
+    h = output.HstOutput(str(tmp_path), dt=0.05, block_number=1)
+    t, n = 0.0, 0
+    for dt in (0.02, 0.02, 0.02, 0.03, 0.03, 0.01):
+        h.maybe_write(t, dt, n, 8, lambda: [1.0 + t, 2.5 - t])
+        t += dt
+        n += 1
+    path = tmp_path / "parthenon.out1.hst"
+    lines = open(path).read().splitlines()
+    assert lines[0] == "#  History data"
+    assert lines[1] == ("# [1]=time     [2]=dt       [3]=cycle    [4]=nbtotal  "
+                        "[5]=total_mass[6]=total_energy")
+    assert lines[2] == ("   0.000000e+00   2.000000e-02  0   8    1.000000e+00"
+                        "   2.500000e+00")
+    data = np.genfromtxt(path)
+    assert data.shape == (3, 6)  # t = 0, 0.06 (>= 0.05), 0.12 (>= 0.10)
+    np.testing.assert_allclose(data[:, 0], [0.0, 0.06, 0.12])
+    np.testing.assert_array_equal(data[:, 2], [0, 3, 5])

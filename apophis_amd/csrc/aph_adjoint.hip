@@ -51,6 +51,8 @@ namespace aph {
 // Segment length of the adjoint sweeps: the largest divisor of n not above cap; at least
 // 3 (a spill reaches 3 cells), else the whole pencil.  cap is 8 unless the block set is
 // too small to give every SIMD two waves with it (adj_prepare).
+constexpr int kSplitSegMax = 16;  // longest segment of the split (PPM) form
+
 __host__ __device__ inline int adj_seg(int n, int cap) {
   for (int d = (n < cap ? n : cap); d >= 3; --d)
     if (n % d == 0) return d;
@@ -245,58 +247,73 @@ __device__ __forceinline__ Lay axis_lay(int pa, int pb, int N0, int N1, int N2,
 // and L_out in the x-transposed layout.  A workgroup takes 16 x 16 (x, y) tiles of one
 // z-plane: it reads U and L_out and writes P along x, then writes the transposed copies
 // along y through LDS, so every access runs over 16 consecutive doubles.
+// prim / gather tiles: 32 cells along x by kPrimTileY / kGatherTileY along y (one row in
+// 1D).  Measured on 64 RT instances: the transposing prim pass is fastest with whole
+// 32 x 32 tiles (2.3 ms against 3.1 ms at 32 x 16 and 3.5 ms at 16 x 16), the gather with
+// 32 x 16 (2.4 ms against 3.0 and 2.55): it holds 104 registers a lane, so 1024-lane
+// workgroups leave one per CU with nothing to overlap its barrier.
+constexpr int kPrimTileY = 32, kGatherTileY = 16;
+
 template <int NV, int NDIM>
-__global__ void __launch_bounds__(256) k_adj_prim(AdjArgs a) {
+__global__ void __launch_bounds__(1024) k_adj_prim(AdjArgs a) {
   constexpr int NF = NV + 2;
-  constexpr int T = 16;
-  __shared__ double tile[NF + NV][T][T + 1];
+  constexpr int TX = 32, TY = (NDIM == 1) ? 1 : kPrimTileY;
+  // one 32 x 32 (x, y) tile per pass: rows of 32 doubles read and written along x, then
+  // the transposed copies written along y
+  __shared__ double tile[NF][TY][TX + 1];
   const int b = blockIdx.y;
   const aph_block &bk = a.blocks[b];
   const int N0 = ext(bk, 0, NDIM), N1 = ext(bk, 1, NDIM), N2 = ext(bk, 2, NDIM);
   const int64_t nc = (int64_t)N0 * N1 * N2;
   const int g[3] = {gst(bk, 0, NDIM), gst(bk, 1, NDIM), gst(bk, 2, NDIM)};
   const int nx0 = bk.nx[0], nx1 = bk.nx[1], nx2 = bk.nx[2];
-  const int nti = (nx0 + T - 1) / T, ntj = (nx1 + T - 1) / T;
+  const int nti = (nx0 + TX - 1) / TX, ntj = (nx1 + TY - 1) / TY;
   const int ntiles = nti * ntj * nx2;
   const int64_t o = (int64_t)NV * bk.cell_base;
   double *pp = a.prim + (int64_t)NF * bk.cell_base;
   double *pt = a.prim_t + (int64_t)NF * bk.cell_base;
-  const int tx = threadIdx.x % T, ty = threadIdx.x / T;
+  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const int k = t / (nti * ntj), r = t % (nti * ntj);
-    const int i0 = (r % nti) * T, j0 = (r / nti) * T;
-    {  // along x: thread (tx, ty) = cell (i0 + tx, j0 + ty)
-      const int i = i0 + tx, j = j0 + ty;
-      if (i < nx0 && j < nx1) {
-        const int64_t cell = ((int64_t)(k + g[2]) * N1 + (j + g[1])) * N0 + (i + g[0]);
-        double u[NV], w[NV], gm[2];
+    const int i0 = (r % nti) * TX, j0 = (r / nti) * TY;
+    const int i = i0 + tx, j = j0 + ty;  // along x
+    const bool in = i < nx0 && j < nx1;
+    const int64_t cell = ((int64_t)(k + g[2]) * N1 + (j + g[1])) * N0 + (i + g[0]);
+    // the transposed copies: thread q takes cell (i0 + q / TY, j0 + q % TY), adjacent y
+    const int jl = threadIdx.x % TY, il = threadIdx.x / TY;
+    const bool int_ = i0 + il < nx0 && j0 + jl < nx1;
+    const int64_t cellt =
+        ((int64_t)(i0 + il + g[0]) * N2 + (k + g[2])) * N1 + (j0 + jl + g[1]);
+    if (in) {
+      double u[NV], w[NV], gm[2];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) {
-          u[v] = a.u_in[o + v * nc + cell];
-          tile[NF + v][ty][tx] = a.lam_out[o + v * nc + cell];
-        }
-        cons_to_prim_pt<NV>(u, a.eos, w, gm);
+      for (int v = 0; v < NV; ++v) u[v] = a.u_in[o + v * nc + cell];
+      cons_to_prim_pt<NV>(u, a.eos, w, gm);
 #pragma unroll
-        for (int v = 0; v < NV; ++v) {
-          pp[v * nc + cell] = w[v];
-          tile[v][ty][tx] = w[v];
-        }
-        pp[NV * nc + cell] = gm[0];
-        pp[(NV + 1) * nc + cell] = gm[1];
-        tile[NV][ty][tx] = gm[0];
-        tile[NV + 1][ty][tx] = gm[1];
+      for (int v = 0; v < NV; ++v) {
+        pp[v * nc + cell] = w[v];
+        tile[v][ty][tx] = w[v];
       }
+      pp[NV * nc + cell] = gm[0];
+      pp[(NV + 1) * nc + cell] = gm[1];
+      tile[NV][ty][tx] = gm[0];
+      tile[NV + 1][ty][tx] = gm[1];
     }
     __syncthreads();
-    {  // along y: thread (tx, ty) = cell (i0 + ty, j0 + tx)
-      const int i = i0 + ty, j = j0 + tx;
-      if (i < nx0 && j < nx1) {
-        const int64_t cellt = ((int64_t)(i + g[0]) * N2 + (k + g[2])) * N1 + (j + g[1]);
+    if (int_) {
 #pragma unroll
-        for (int f = 0; f < NF; ++f) pt[f * nc + cellt] = tile[f][tx][ty];
+      for (int f = 0; f < NF; ++f) pt[f * nc + cellt] = tile[f][jl][il];
+    }
+    __syncthreads();
+    // L_out, the same way (the tile reused)
+    if (in) {
 #pragma unroll
-        for (int v = 0; v < NV; ++v) a.lam_t[o + v * nc + cellt] = tile[NF + v][tx][ty];
-      }
+      for (int v = 0; v < NV; ++v) tile[v][ty][tx] = a.lam_out[o + v * nc + cell];
+    }
+    __syncthreads();
+    if (int_) {
+#pragma unroll
+      for (int v = 0; v < NV; ++v) a.lam_t[o + v * nc + cellt] = tile[v][jl][il];
     }
     __syncthreads();
   }
@@ -888,33 +905,34 @@ __device__ __forceinline__ void add_slot(const AdjArgs &a, int d, int blk, int p
 }
 
 template <int NV, int NDIM>
-__global__ void __launch_bounds__(256) k_adj_gather(AdjArgs a) {
+__global__ void __launch_bounds__(512) k_adj_gather(AdjArgs a) {
   constexpr int NF = NV + 2;
-  constexpr int T = 16;
-  // the x axis' Pbar of one 16 x 16 (x, y) tile, read along y from the x-transposed
-  // layout and used along x (split form)
-  __shared__ double xt[NF][T][T + 1];
+  constexpr int TX = 32, TY = (NDIM == 1) ? 1 : kGatherTileY;
+  // the x axis' Pbar of one 32 x 16 (x, y) tile, read along y from the x-transposed
+  // layout and used along x (split form); every other array is read along x
+  __shared__ double xt[NF][TY][TX + 1];
   const int b = blockIdx.y;
   const aph_block &bk = a.blocks[b];
   const int N0 = ext(bk, 0, NDIM), N1 = ext(bk, 1, NDIM), N2 = ext(bk, 2, NDIM);
   const int64_t nc = (int64_t)N0 * N1 * N2;
   const int g[3] = {gst(bk, 0, NDIM), gst(bk, 1, NDIM), gst(bk, 2, NDIM)};
   const int nx0 = bk.nx[0], nx1 = bk.nx[1], nx2 = bk.nx[2];
-  const int nti = (nx0 + T - 1) / T, ntj = (nx1 + T - 1) / T;
+  const int nti = (nx0 + TX - 1) / TX, ntj = (nx1 + TY - 1) / TY;
   const int ntiles = nti * ntj * nx2;
   const double beta_dt = a.beta * a.dt[0];
   const double gsum = (a.stage == 1) ? a.gam0 + a.gam1 : a.gam0;
-  const int tx = threadIdx.x % T, ty = threadIdx.x / T;
+  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
   const double *pbx = a.pbar + (int64_t)NF * bk.cell_base;
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const int kz = t / (nti * ntj), r = t % (nti * ntj);
-    const int i0 = (r % nti) * T, j0 = (r / nti) * T;
-    if (a.pbar_xt) {  // thread (tx, ty) reads cell (i0 + ty, j0 + tx): adjacent y
-      const int i = i0 + ty, j = j0 + tx;
+    const int i0 = (r % nti) * TX, j0 = (r / nti) * TY;
+    if (a.pbar_xt) {  // thread q reads cell (i0 + q / TY, j0 + q % TY): adjacent y
+      const int jl = threadIdx.x % TY, il = threadIdx.x / TY;
+      const int i = i0 + il, j = j0 + jl;
       if (i < nx0 && j < nx1) {
         const int64_t cellt = ((int64_t)(i + g[0]) * N2 + (kz + g[2])) * N1 + (j + g[1]);
 #pragma unroll
-        for (int f = 0; f < NF; ++f) xt[f][ty][tx] = pbx[f * nc + cellt];
+        for (int f = 0; f < NF; ++f) xt[f][jl][il] = pbx[f * nc + cellt];
       }
       __syncthreads();
     }
@@ -935,7 +953,7 @@ __global__ void __launch_bounds__(256) k_adj_gather(AdjArgs a) {
       const double *pp = a.pbar + (int64_t)NF * bk.cell_base + cell;
       if (a.pbar_xt) {
 #pragma unroll
-        for (int f = 0; f < NF; ++f) pbv[f] = xt[f][tx][ty];
+        for (int f = 0; f < NF; ++f) pbv[f] = xt[f][ty][tx];
       } else {
 #pragma unroll
         for (int f = 0; f < NF; ++f) pbv[f] = pp[f * nc];
@@ -1099,13 +1117,16 @@ int adj_prepare(aph_context *ctx, const aph_blockset *bs, int recon, AdjScratch 
   }
   if (s.pbar) return APH_OK;
   // segment cap per axis: the longest segments (fewest recomputed halo cells) that still
-  // give >= 2 waves of 64 lanes per SIMD (MI355X: 256 CUs x 4 SIMDs)
+  // give >= 2 waves of 64 lanes per SIMD (MI355X: 256 CUs x 4 SIMDs); up to 16 cells for
+  // the split (PPM) form, whose passes move each segment's halo through HBM, 8 for the
+  // one-march (PLM) form
   constexpr int64_t kTargetLanes = 2LL * 64 * 1024;
+  const int cap0 = (recon == RECON_PPM) ? kSplitSegMax : 8;
   for (int d = 0; d < nd; ++d) {
     const int A = (d == 0) ? 1 : 0, B = (d == 2) ? 1 : 2;
-    int best = 8;
+    int best = cap0;
     int64_t best_lanes = -1;
-    for (int cap = 8; cap >= 3; --cap) {
+    for (int cap = cap0; cap >= 3; --cap) {
       int64_t lanes = 0;
       for (int b = 0; b < nb; ++b) {
         const aph_block &k = bs->h[b];
@@ -1204,9 +1225,20 @@ int adj_stage(aph_context *ctx, const aph_blockset *bs, int recon, int rs, const
   a.pbar_xt = ks[0].face != nullptr;  // the split form writes the x axis' Pbar transposed
   for (int d = 0; d < 3; ++d) a.segcap[d] = s.segcap[d];
   const int nf = bs->nvar + 2;
-  const dim3 gc((unsigned)std::max<int64_t>(1, std::min<int64_t>(64, (bs->max_interior + 255) / 256)),
-                (unsigned)bs->nblocks);
-  if (ks[0].face) hipLaunchKernelGGL(pick_prim(bs->nvar, nd), gc, dim3(256), 0, ctx->stream, a);
+  // prim / gather: one 32 x TY (x, y) tile (32 x 1 in 1D) per workgroup pass
+  auto tile_launch = [&](int ty, dim3 &grid, dim3 &block) {
+    const int tyd = nd == 1 ? 1 : ty;
+    int64_t tiles = 1;
+    for (const auto &k : bs->h)
+      tiles = std::max<int64_t>(tiles, (int64_t)((k.nx[0] + 31) / 32) *
+                                           ((k.nx[1] + tyd - 1) / tyd) * k.nx[2]);
+    grid = dim3((unsigned)std::min<int64_t>(64, tiles), (unsigned)bs->nblocks);
+    block = dim3((unsigned)(32 * tyd));
+  };
+  dim3 gp, bp, gg, bg;
+  tile_launch(kPrimTileY, gp, bp);
+  tile_launch(kGatherTileY, gg, bg);
+  if (ks[0].face) hipLaunchKernelGGL(pick_prim(bs->nvar, nd), gp, bp, 0, ctx->stream, a);
   const unsigned nbk = (unsigned)bs->nblocks;
   int64_t maxu_all = 0;
   for (int d = 0; d < nd; ++d) {
@@ -1228,7 +1260,7 @@ int adj_stage(aph_context *ctx, const aph_blockset *bs, int recon, int rs, const
   if (ks[0].axis)  // one-march form: all axes in one launch, z = axis
     hipLaunchKernelGGL(ks[0].axis, dim3((unsigned)((maxu_all + 63) / 64), nbk, (unsigned)nd),
                        dim3(64), 0, ctx->stream, a);
-  hipLaunchKernelGGL(pick_gather(bs->nvar, nd), gc, dim3(256), 0, ctx->stream, a);
+  hipLaunchKernelGGL(pick_gather(bs->nvar, nd), gg, bg, 0, ctx->stream, a);
   APH_LAUNCH_CHECK(ctx);
   return APH_OK;
 }

@@ -78,6 +78,7 @@ _SIGS = {
     "aph_plan_get_tile": (_i32, [_vp, _vp, _i32, ctypes.POINTER(Tile)]),
     "aph_plan_get_launch": (_i32, [_vp, _vp, _i32, ctypes.POINTER(ctypes.c_int32)]),
     "aph_plan_set_workgroup": (_i32, [_vp, _vp, _i32]),
+    "aph_plan_set_lds_reserve": (_i32, [_vp, _vp, _i64]),
     "aph_apply_gravity_field": (_i32, [_vp, _vp, _i32, _i32, _vp, _vp, _vp, _d]),
     "aph_monopole_create": (_i32, [_vp, _i32, _i32, _d, _d, ctypes.POINTER(_vp)]),
     "aph_monopole_destroy": (_i32, [_vp, _vp]),

@@ -99,6 +99,7 @@ struct aph_plan {
   int nitems = 0;
   int grid = 0;
   size_t lds = 0;
+  size_t lds_reserve = 0;  // aph_plan_set_lds_reserve: LDS per workgroup at least this
   double *d_est = nullptr;  // device dt estimate accumulator
   int integrator = APH_INTEGRATOR_RK2;
   double *d_u2 = nullptr;   // third register (rk3)
@@ -322,6 +323,7 @@ int build_items(aph_context *ctx, aph_plan *p) {
   p->lds = (ndim >= 2) ? maxcells * bs->nvar * sizeof(double) : 0;
   if (p->lds > 64 * 1024)
     return set_error(ctx, APH_E_INVALID, "tile too large for LDS (%zu bytes)", p->lds);
+  p->lds = std::max(p->lds, p->lds_reserve);
   p->tiles.assign(tl.begin(), tl.end());
   p->seg_scale.assign(sc.begin(), sc.end());
   // persistent grid: resident workgroups, a multiple of the XCD count
@@ -336,6 +338,10 @@ int build_items(aph_context *ctx, aph_plan *p) {
     KernelFn fn = p->amr ? pick_kernel_amr(r, p->rsolver, bs->nvar, ndim)
                          : pick_kernel(r, p->rsolver, bs->nvar, ndim);
     if (!fn) return set_error(ctx, APH_E_INVALID, "no fused kernel for this plan");
+    if (p->lds > 64 * 1024)  // beyond the default dynamic LDS limit (a reservation)
+      APH_HIP_CHECK(ctx, hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             (int)p->lds));
     int o = 0;
     APH_HIP_CHECK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(
                            &o, reinterpret_cast<const void *>(fn), p->wg, p->lds));
@@ -735,6 +741,24 @@ int aph_plan_set_tile(aph_context *ctx, aph_plan *p, const aph_tile *t) {
   int rc = build_items(ctx, p);
   if (rc) {
     p->user_tiles = keep;
+    build_items(ctx, p);
+    return rc;
+  }
+  drop_graph(p);
+  return APH_OK;
+}
+
+int aph_plan_set_lds_reserve(aph_context *ctx, aph_plan *p, int64_t bytes) {
+  if (!ctx) return APH_E_INVALID;
+  if (!p || bytes < 0 || bytes > 160 * 1024)
+    return set_error(ctx, APH_E_INVALID, "aph_plan_set_lds_reserve: 0..160 KiB");
+  const size_t keep = p->lds_reserve;
+  p->lds_reserve = (size_t)bytes;
+  p->dirty = true;
+  int rc = build_items(ctx, p);
+  if (rc) {
+    p->lds_reserve = keep;
+    p->dirty = true;
     build_items(ctx, p);
     return rc;
   }

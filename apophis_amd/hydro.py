@@ -710,6 +710,11 @@ class FusedDriver:
         self.ctx.check(_lib.lib().aph_plan_set_workgroup(self.ctx.h, self.plan, int(lanes)))
         self.graph_steps = 0
 
+    def set_lds_reserve(self, nbytes):
+        """At least nbytes of LDS per workgroup (occupancy studies; same results)."""
+        self.ctx.check(_lib.lib().aph_plan_set_lds_reserve(self.ctx.h, self.plan, int(nbytes)))
+        self.graph_steps = 0
+
     def launch(self):
         """{block shape (nx, ny, nz): (lanes per workgroup, segment x, y, z)} in use."""
         out = {}

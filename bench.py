@@ -456,6 +456,40 @@ def profile_run(args, ctx, hydro, problems, torch):
         _calibration_copies(torch)
         print(f"profile-run {args.profile_run}: done", flush=True)
         return
+    if args.profile_run == "occupancy":
+        # the headline workload (256^3 blast, 8x8x8 tiles, 64 lanes) with more LDS reserved
+        # per workgroup, so fewer workgroups are resident per CU: the time per stage launch
+        # at each occupancy, from 10 graph-replayed RK2 steps (after one warm-up replay)
+        p = problems.blast3d(nblk=args.nblk, n=32)
+        cells = p.bs.interior_cells()
+        fd = hydro.FusedDriver(ctx, p.bs, p.cons, p.recon, p.riemann, p.eos, p.cfl)
+        fd.set_tile(32, 8, 8, 8)
+        fd.set_workgroup(64)
+        fd.initial_dt()
+        u_init = fd.u0.clone()
+        for reserve in (0, 32 << 10, 40 << 10, 64 << 10, 96 << 10):
+            fd.u0.copy_(u_init)
+            fd.time.zero_()
+            fd.initial_dt()
+            fd.set_lds_reserve(reserve)
+            fd.capture(10)
+            fd.replay()
+            torch.cuda.synchronize()
+            fd.u0.copy_(u_init)
+            fd.time.zero_()
+            fd.initial_dt()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            fd.replay()
+            torch.cuda.synchronize()
+            el = time.perf_counter() - t0
+            wg_per_cu = min(8, (160 << 10) // max(reserve, 20 << 10))
+            print(f"occupancy: LDS {max(reserve, 20 << 10) >> 10} KiB per workgroup, "
+                  f"{wg_per_cu} workgroups per CU ({wg_per_cu / 4:g} waves per SIMD): "
+                  f"{el / 20 * 1e3:.3f} ms per stage, {cells * 10 / el:.4e} zone-cycles/s",
+                  flush=True)
+        print(f"profile-run {args.profile_run}: done", flush=True)
+        return
     if args.profile_run == "autotune":
         autotune_tables(ctx, hydro, problems, torch)
         print(f"profile-run {args.profile_run}: done", flush=True)
@@ -548,7 +582,7 @@ def main():
     ap.add_argument("--rt-instances", type=int, default=8,
                     help="RT instances of --profile-run rt_adjoint")
     ap.add_argument("--profile-run", choices=["fused", "graph", "plain", "uneven", "autotune",
-                                              "adjoint",
+                                              "occupancy", "adjoint",
                                               "rt_adjoint", "amr"],
                     default=None,
                     help="short eager run for rocprofv3 (no timing, no JSON)")

@@ -379,6 +379,10 @@ int aph_plan_get_tile(aph_context *ctx, aph_plan *plan, int32_t block, aph_tile 
  * autotuner sets it).  Results do not depend on it bit for bit.  Synchronous.  No
  * reference counterpart. */
 int aph_plan_set_workgroup(aph_context *ctx, aph_plan *plan, int32_t lanes);
+/* Reserve at least `bytes` (0..160 KiB) of LDS per workgroup of the plan's stage launches.
+ * Results are unchanged; fewer workgroups fit on a CU.  For occupancy studies (what a
+ * variant needing that much LDS would pay).  Synchronous.  No reference counterpart. */
+int aph_plan_set_lds_reserve(aph_context *ctx, aph_plan *plan, int64_t bytes);
 /* Launch shape for block `block`: out[0] = lanes per workgroup (one per plan: 64, 128 or
  * 256), out[1..3] = the segment length one lane marches along x, y, z.  No reference
  * counterpart (the reference's flux kernels take one fixed team shape). */

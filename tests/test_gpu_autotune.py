@@ -99,3 +99,14 @@ def test_set_workgroup_refuses_other_sizes(gpu_ctx):
     fd = _driver(gpu_ctx, problems.sod())
     with pytest.raises(hydro._lib.HydroError):
         fd.set_workgroup(96)
+
+
+def test_lds_reserve_changes_no_bit(gpu_ctx):
+    """A larger LDS reservation only lowers the resident workgroups per CU."""
+    p = problems.uneven_blocks(nblocks=24, seed=5)
+    print("[case] lds reserve", flush=True)
+    ref = _steps(_driver(gpu_ctx, p), 3)
+    fd = _driver(gpu_ctx, p)
+    fd.set_lds_reserve(96 << 10)
+    got = _steps(fd, 3)
+    assert np.array_equal(got.view(np.uint64), ref.view(np.uint64))

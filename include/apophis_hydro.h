@@ -9,6 +9,7 @@
  *   time step           Apophis::EstimateTimestep<euler>     src/hydro/hydro.cpp:351-420
  *   history sums        Apophis::GlobalQuantHst<T,idx>       src/hydro/hydro.cpp:746-771
  *   gravity source      Apophis::ApplyGravity                src/gravity/gravity.cpp:374-410
+ *   monopole gravity    MonopoleGravitySolver     src/gravity/gsolvers/monopole_gravity.hpp
  *   stage sequence      ApophisDriver::MakeTaskCollection    src/apophis_driver.cpp:27-130
  * and the upstream Parthenon pieces that task graph calls (flux-divergence update,
  * RK2 register copy, ghost exchange, outflow/periodic boundaries).
@@ -365,7 +366,8 @@ int aph_plan_set_tile(aph_context *ctx, aph_plan *plan, const aph_tile *t);
  * device) over the blocks of that shape for each candidate tile (divisors of the block
  * edges, one to sixteen cells per lane, within LDS) and segment length (the shortest that
  * keeps every lane busy, or twice that), keeps each shape's fastest, and takes the
- * workgroup size whose shapes sum to the least time; tiles set with aph_plan_set_tile are
+ * workgroup size whose shapes sum to the least time (a larger size only when it is more
+ * than 2% faster than the best smaller one); tiles set with aph_plan_set_tile are
  * kept as they are (their segments are not tuned).  Synchronous; u_in
  * is only read, u_scratch is overwritten.  If `report` is non-null, a text table of the
  * candidates and their average launch times (ms) is written to it (at most report_len

@@ -216,8 +216,45 @@ __device__ __forceinline__ void sweep(const FusedArgs &a, const aph_block &bk, i
       if (flip) u[1 + D] = -1.0 * u[1 + D];
       cons_to_prim_pt<NV>(u, a.eos, dst, dst + NV);
     };
+    // last sweep of PLM / DC: the next cell's u_in, u_n and acceleration field, loaded one
+    // step ahead (their latency overlaps the next face's reconstruction and Riemann
+    // solve).  It takes 26 more registers: free for PLM, whose LDS use already caps it at
+    // two waves per SIMD, but it would leave PPM (217 registers) one wave per SIMD.
+    constexpr bool PREF = LAST && RECON != RECON_PPM;
+    double pu[NV], pun[NV], pg[3];
+    auto prefetch = [&](int m) {
+      const int64_t cidx = tbase + (int64_t)m * st[D];
+      const double *pin = a.u_in + nb.self + cidx;
+#pragma unroll
+      for (int v = 0; v < NV; ++v) pu[v] = pin[v * nc];
+      if (a.has_un) {
+        const double *pn = a.u_n + nb.self + cidx;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) pun[v] = pn[v * nc];
+      }
+      if (a.gfield) {  // a per-cell field (monopole gravity)
+        const double *gp = a.gfield + 3 * bk.cell_base + cidx;
+        pg[0] = gp[0];
+        pg[1] = gp[nc];
+        pg[2] = gp[2 * nc];
+      }
+    };
     auto emit = [&](int f, const double *fo_in, const double *, const double *) {
       const double *fo = fo_in;
+      double cu[NV], cun[NV], cg[3];
+      if constexpr (PREF) {
+        if (f > f_lo) {
+#pragma unroll
+          for (int v = 0; v < NV; ++v) {
+            cu[v] = pu[v];
+            cun[v] = pun[v];
+          }
+          cg[0] = pg[0];
+          cg[1] = pg[1];
+          cg[2] = pg[2];
+        }
+        if (f < f_hi) prefetch(f);
+      }
       double fc[NV];
       if (AMR && (f == 0 || f == n_d)) {
         // a block face to the other level: the fine side records its flux, the coarse
@@ -267,18 +304,34 @@ __device__ __forceinline__ void sweep(const FusedArgs &a, const aph_block &bk, i
         } else {
           const int64_t cidx = tbase + (int64_t)m * st[D];
           double u[NV], w[NV], gm[2];
-          const double *pin = a.u_in + nb.self + cidx;
+          double g0 = a.g[0], g1 = a.g[1], g2 = a.g[2];
+          if constexpr (PREF) {
 #pragma unroll
-          for (int v = 0; v < NV; ++v) u[v] = pin[v * nc];
+            for (int v = 0; v < NV; ++v) u[v] = cu[v];
+            if (a.gfield) {
+              g0 = cg[0];
+              g1 = cg[1];
+              g2 = cg[2];
+            }
+          } else {  // PPM: each load where it is used (registers)
+            const double *pin = a.u_in + nb.self + cidx;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) u[v] = pin[v * nc];
+          }
           if (a.grav)  // the gravity source needs the stage-start primitives
             cons_to_prim_pt<NV>(u, a.eos, w, gm);
           else
             floor_cons_pt<NV>(u, a.eos);
           double un[NV];
           if (a.has_un) {
-            const double *pn = a.u_n + nb.self + cidx;
+            if constexpr (PREF) {
 #pragma unroll
-            for (int v = 0; v < NV; ++v) un[v] = pn[v * nc];
+              for (int v = 0; v < NV; ++v) un[v] = cun[v];
+            } else {
+              const double *pn = a.u_n + nb.self + cidx;
+#pragma unroll
+              for (int v = 0; v < NV; ++v) un[v] = pn[v * nc];
+            }
             floor_cons_pt<NV>(un, a.eos);
           } else {
 #pragma unroll
@@ -297,8 +350,7 @@ __device__ __forceinline__ void sweep(const FusedArgs &a, const aph_block &bk, i
           }
           if (a.grav) {  // ApplyGravity with the stage-start primitives
             const double gdt = beta_dt;
-            double g0 = a.g[0], g1 = a.g[1], g2 = a.g[2];
-            if (a.gfield) {  // a per-cell field (monopole gravity)
+            if (!PREF && a.gfield) {  // a per-cell field (monopole gravity)
               const double *gp = a.gfield + 3 * bk.cell_base + cidx;
               g0 = gp[0];
               g1 = gp[nc];

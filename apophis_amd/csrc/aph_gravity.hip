@@ -35,7 +35,7 @@ namespace aph {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kLdsMaxBins = 4096;  // LDS partial sums up to 64 KiB per workgroup
+constexpr int kLdsMaxBins = 4080;  // LDS partial sums within 64 KiB per workgroup
 using u128 = unsigned __int128;
 
 __device__ __forceinline__ int mono_exponent(double dm) {
@@ -146,21 +146,34 @@ __global__ void k_mono_zero(unsigned long long *acc, int n, int *emax) {
   if (blockIdx.x == 0 && threadIdx.x == 0) *emax = INT_MIN;
 }
 
+// One workgroup takes up to kChunk consecutive interior cells of one block (a whole 32^3
+// block), so that it touches few radial bins and adds each of them to HBM once.
+constexpr int kChunk = 32768;
+
 __global__ void __launch_bounds__(kThreads)
     k_mono_bound(const aph_block *__restrict__ blocks, int first, int ndim, int nvar,
                  const double *__restrict__ field, double dfloor, double rmax, int *emax) {
+  __shared__ int wave_max[kThreads / 64];
   const aph_block &bk = blocks[first + blockIdx.y];
   const int64_t nint = (int64_t)bk.nx[0] * bk.nx[1] * bk.nx[2];
+  const int64_t c0 = (int64_t)blockIdx.x * kChunk;
+  const int64_t c1 = c0 + kChunk < nint ? c0 + kChunk : nint;
   int e = INT_MIN;
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nint;
-       t += (int64_t)gridDim.x * blockDim.x) {
+  for (int64_t t = c0 + threadIdx.x; t < c1; t += blockDim.x) {
     int i, j, k;
     decompose(t, bk.nx[0], bk.nx[1], i, j, k);
     const CellMass c = cell_mass(bk, ndim, nvar, field, dfloor, i, j, k);
     if (c.r < rmax && c.dm != 0.0) e = max(e, mono_exponent(c.dm));
   }
   for (int o = 32; o > 0; o >>= 1) e = max(e, __shfl_xor(e, o));
-  if ((threadIdx.x & 63) == 0 && e != INT_MIN) atomicMax(emax, e);
+  if ((threadIdx.x & 63) == 0) wave_max[threadIdx.x >> 6] = e;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kThreads / 64; ++w) e = max(e, wave_max[w]);
+    // one atomic per workgroup, none when the bound is already as large
+    if (e != INT_MIN && e > __hip_atomic_load(emax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      atomicMax(emax, e);
+  }
 }
 
 template <bool LDS>
@@ -169,28 +182,41 @@ __global__ void __launch_bounds__(kThreads)
                  const double *__restrict__ field, double dfloor, double rmax, int nrbin,
                  int log2_cells, const int *__restrict__ emax, unsigned long long *acc) {
   extern __shared__ unsigned long long part[];  // [nrbin][2] when LDS
+  __shared__ int range[2];                      // bins this workgroup touched
   const aph_block &bk = blocks[first + blockIdx.y];
   const int64_t nint = (int64_t)bk.nx[0] * bk.nx[1] * bk.nx[2];
+  const int64_t c0 = (int64_t)blockIdx.x * kChunk;
+  const int64_t c1 = c0 + kChunk < nint ? c0 + kChunk : nint;
   const double dr = rmax / nrbin;
   const int s = mono_scale(*emax, log2_cells);
   if constexpr (LDS) {
     for (int i = threadIdx.x; i < 2 * nrbin; i += blockDim.x) part[i] = 0ull;
+    if (threadIdx.x == 0) {
+      range[0] = nrbin;
+      range[1] = -1;
+    }
     __syncthreads();
   }
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nint;
-       t += (int64_t)gridDim.x * blockDim.x) {
+  int blo = nrbin, bhi = -1;
+  for (int64_t t = c0 + threadIdx.x; t < c1; t += blockDim.x) {
     int i, j, k;
     decompose(t, bk.nx[0], bk.nx[1], i, j, k);
     const CellMass c = cell_mass(bk, ndim, nvar, field, dfloor, i, j, k);
     if (c.r >= rmax) continue;
     int bin = (int)(c.r / dr);
     bin = bin < 0 ? 0 : (bin >= nrbin ? nrbin - 1 : bin);
+    blo = min(blo, bin);
+    bhi = max(bhi, bin);
     const u128 q = mono_quantize(c.dm, s);
     add128(LDS ? part : acc, bin, (unsigned long long)q, (unsigned long long)(q >> 64));
   }
   if constexpr (LDS) {
+    if (bhi >= 0) {
+      atomicMin(&range[0], blo);
+      atomicMax(&range[1], bhi);
+    }
     __syncthreads();
-    for (int b = threadIdx.x; b < nrbin; b += blockDim.x) {
+    for (int b = range[0] + (int)threadIdx.x; b <= range[1]; b += blockDim.x) {
       const unsigned long long lo = part[2 * b], hi = part[2 * b + 1];
       if (lo | hi) add128(acc, b, lo, hi);
     }
@@ -199,27 +225,37 @@ __global__ void __launch_bounds__(kThreads)
 
 // BuildRadialTables (monopole_gravity.hpp:155-213).  One workgroup: the bin masses and
 // the potential increments in parallel, the two recurrences by one lane in the
-// reference's order.
+// reference's order, reading their operands from LDS (nrbin <= kLdsTableBins).
+constexpr int kLdsTableBins = 2048;
+
+template <bool LDS>
 __global__ void __launch_bounds__(kThreads)
     k_mono_tables(int nrbin, int ndim, double rmax, double G, int log2_cells,
                   const int *__restrict__ emax, const unsigned long long *__restrict__ acc,
                   double *mbin, double *m_enc, double *phi_rad) {
+  extern __shared__ double tab[];  // LDS: [nrbin] bin masses / increments, [nrbin+1] m_enc
+  double *a = LDS ? tab : mbin;
+  double *e = LDS ? tab + nrbin : m_enc;
   const int s = mono_scale(*emax, log2_cells);
   const double dr = rmax / nrbin;
   for (int i = threadIdx.x; i < nrbin; i += blockDim.x)
-    mbin[i] = mono_to_double(((u128)acc[2 * i + 1] << 64) | acc[2 * i], s);
+    a[i] = mono_to_double(((u128)acc[2 * i + 1] << 64) | acc[2 * i], s);
   __syncthreads();
   if (threadIdx.x == 0) {
-    m_enc[0] = 0.0;
-    for (int i = 0; i < nrbin; ++i) m_enc[i + 1] = m_enc[i] + mbin[i];
+    double run = 0.0;
+    e[0] = 0.0;
+    for (int i = 0; i < nrbin; ++i) {
+      run = run + a[i];
+      e[i + 1] = run;
+    }
   }
   __syncthreads();
   for (int i = threadIdx.x; i < nrbin; i += blockDim.x) {
     const double r_inner = i * dr, r_outer = (i + 1) * dr;
     double dphi;
     if (ndim == 3) {
-      const double m_inner = m_enc[i];
-      const double shell_mass = m_enc[i + 1] - m_inner;
+      const double m_inner = e[i];
+      const double shell_mass = e[i + 1] - m_inner;
       if (r_inner == 0.0) {
         dphi = -G * shell_mass / (2.0 * r_outer);
       } else {
@@ -231,17 +267,23 @@ __global__ void __launch_bounds__(kThreads)
       }
     } else {
       const double r_mid = 0.5 * (r_inner + r_outer);
-      const double m_mid = 0.5 * (m_enc[i] + m_enc[i + 1]);
+      const double m_mid = 0.5 * (e[i] + e[i + 1]);
       double gr = 0.0;
       if (r_mid > 0.0) gr = (ndim == 1) ? -G * m_mid : -2.0 * G * m_mid / r_mid;
       dphi = gr * dr;
     }
-    mbin[i] = dphi;
+    a[i] = dphi;
+    if (LDS) m_enc[i] = e[i];
   }
+  if (LDS && threadIdx.x == 0) m_enc[nrbin] = e[nrbin];
   __syncthreads();
   if (threadIdx.x == 0) {
-    phi_rad[nrbin] = (rmax > 0.0 && ndim == 3) ? -G * m_enc[nrbin] / rmax : 0.0;
-    for (int i = nrbin - 1; i >= 0; --i) phi_rad[i] = phi_rad[i + 1] + mbin[i];
+    double run = (rmax > 0.0 && ndim == 3) ? -G * e[nrbin] / rmax : 0.0;
+    phi_rad[nrbin] = run;
+    for (int i = nrbin - 1; i >= 0; --i) {
+      run = run + a[i];
+      phi_rad[i] = run;
+    }
   }
 }
 
@@ -354,6 +396,11 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
+dim3 chunk_grid(const aph_blockset *bs, int cnt) {
+  return dim3((unsigned)std::max<int64_t>(1, (bs->max_interior + kChunk - 1) / kChunk),
+              (unsigned)cnt);
+}
+
 dim3 cell_grid(const aph_blockset *bs, int cnt) {
   const int64_t per = (bs->max_interior + kThreads - 1) / kThreads;
   return dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(per, 64)), (unsigned)cnt);
@@ -387,7 +434,7 @@ int mono_begin(aph_context *ctx, aph_monopole *m, const aph_blockset *bs, const 
                      ctx->stream, m->acc, n, m->emax);
   APH_LAUNCH_CHECK(ctx);
   if (bs->nblocks > 0) {
-    hipLaunchKernelGGL(k_mono_bound, cell_grid(bs, bs->nblocks), dim3(kThreads), 0,
+    hipLaunchKernelGGL(k_mono_bound, chunk_grid(bs, bs->nblocks), dim3(kThreads), 0,
                        ctx->stream, bs->d, 0, bs->ndim, bs->nvar, field, dfloor, m->rmax,
                        m->emax);
     APH_LAUNCH_CHECK(ctx);
@@ -398,12 +445,12 @@ int mono_begin(aph_context *ctx, aph_monopole *m, const aph_blockset *bs, const 
 int mono_accumulate(aph_context *ctx, aph_monopole *m, const aph_blockset *bs, int lo, int cnt,
                     const double *field, double dfloor) {
   if (m->nrbin <= kLdsMaxBins) {
-    hipLaunchKernelGGL(k_mono_accum<true>, cell_grid(bs, cnt), dim3(kThreads),
+    hipLaunchKernelGGL(k_mono_accum<true>, chunk_grid(bs, cnt), dim3(kThreads),
                        sizeof(unsigned long long) * 2 * m->nrbin, ctx->stream, bs->d, lo,
                        bs->ndim, bs->nvar, field, dfloor, m->rmax, m->nrbin, m->log2_cells,
                        m->emax, m->acc);
   } else {
-    hipLaunchKernelGGL(k_mono_accum<false>, cell_grid(bs, cnt), dim3(kThreads), 0,
+    hipLaunchKernelGGL(k_mono_accum<false>, chunk_grid(bs, cnt), dim3(kThreads), 0,
                        ctx->stream, bs->d, lo, bs->ndim, bs->nvar, field, dfloor, m->rmax,
                        m->nrbin, m->log2_cells, m->emax, m->acc);
   }
@@ -499,9 +546,15 @@ int aph_monopole_accumulate_mass(aph_context *ctx, aph_monopole *m, const aph_bl
 int aph_monopole_build_radial_tables(aph_context *ctx, aph_monopole *m) {
   if (!ctx) return APH_E_INVALID;
   if (!m) return set_error(ctx, APH_E_INVALID, "aph_monopole_build_radial_tables: null");
-  hipLaunchKernelGGL(k_mono_tables, dim3(1), dim3(kThreads), 0, ctx->stream, m->nrbin,
-                     m->ndim, m->rmax, m->G, m->log2_cells, m->emax, m->acc, m->mbin,
-                     m->m_enc, m->phi_rad);
+  if (m->nrbin <= kLdsTableBins)
+    hipLaunchKernelGGL(k_mono_tables<true>, dim3(1), dim3(kThreads),
+                       sizeof(double) * (2 * (size_t)m->nrbin + 1), ctx->stream, m->nrbin,
+                       m->ndim, m->rmax, m->G, m->log2_cells, m->emax, m->acc, m->mbin,
+                       m->m_enc, m->phi_rad);
+  else
+    hipLaunchKernelGGL(k_mono_tables<false>, dim3(1), dim3(kThreads), 0, ctx->stream,
+                       m->nrbin, m->ndim, m->rmax, m->G, m->log2_cells, m->emax, m->acc,
+                       m->mbin, m->m_enc, m->phi_rad);
   APH_LAUNCH_CHECK(ctx);
   return APH_OK;
 }

@@ -290,6 +290,38 @@ def composition_leg(ctx, hydro, problems, torch, nsteps=20):
     return out
 
 
+def monopole_leg(ctx, hydro, problems, torch, nsteps=20, res=256):
+    """Evrard's collapse with monopole self-gravity (inputs/evrards_collapse.in's physics:
+    PLM + HLLC, RK2, outflow, G = 1, nrbin = 4 res) at 256^3 in blocks of 32^3, batched fused
+    form, graph-replayed: zone-cycles/s with the solver before every stage, and the same
+    steps with gravity off, whose difference is the solver's cost per stage."""
+    import dataclasses
+
+    p = problems.evrard(res=res, block=32)
+    cells = p.bs.interior_cells()
+    out = {"cells": cells, "blocks": p.bs.nblocks, "nrbin": p.monopole[0]}
+    for name, mono in (("monopole", p.monopole), ("no_gravity", None)):
+        fd = hydro.FusedDriver(ctx, p.bs, p.cons, p.recon, p.riemann, p.eos, p.cfl,
+                               tlim=p.tlim, monopole=mono)
+        fd.initial_dt()
+        for _ in range(2):
+            fd.step()
+        fd.capture(nsteps)
+        fd.replay()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fd.replay()
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        out[name] = {"zone_cycles_per_s": cells * nsteps / el,
+                     "ms_per_step": el / nsteps * 1e3}
+        del fd
+    out["solver_ms_per_stage"] = (out["monopole"]["ms_per_step"] -
+                                  out["no_gravity"]["ms_per_step"]) / 2
+    torch.cuda.empty_cache()
+    return out
+
+
 def two_level_leg(ctx, hydro, problems, torch, nsteps=20):
     """A two-level mesh (2D blast, 48^2 base blocks of 32^2 with the central 16^2 refined:
     3.1 M cells, 3072 blocks): zone-cycles/s of the batched form (cross-level ghosts, fine
@@ -456,6 +488,19 @@ def profile_run(args, ctx, hydro, problems, torch):
         _calibration_copies(torch)
         print(f"profile-run {args.profile_run}: done", flush=True)
         return
+    if args.profile_run == "monopole":
+        # Evrard 256^3 (512 blocks of 32^3) with monopole gravity: 3 RK2 steps, each stage
+        # preceded by the solver's zero / bound / accumulate / tables / field kernels
+        p = problems.evrard(res=256, block=32)
+        fd = hydro.FusedDriver(ctx, p.bs, p.cons, p.recon, p.riemann, p.eos, p.cfl,
+                               tlim=p.tlim, monopole=p.monopole)
+        fd.initial_dt()
+        for _ in range(3):
+            fd.step()
+        torch.cuda.synchronize()
+        _calibration_copies(torch)
+        print(f"profile-run {args.profile_run}: done", flush=True)
+        return
     if args.profile_run == "occupancy":
         # the headline workload (256^3 blast, 8x8x8 tiles, 64 lanes) with more LDS reserved
         # per workgroup, so fewer workgroups are resident per CU: the time per stage launch
@@ -582,7 +627,7 @@ def main():
     ap.add_argument("--rt-instances", type=int, default=8,
                     help="RT instances of --profile-run rt_adjoint")
     ap.add_argument("--profile-run", choices=["fused", "graph", "plain", "uneven", "autotune",
-                                              "occupancy", "adjoint",
+                                              "occupancy", "monopole", "adjoint",
                                               "rt_adjoint", "amr"],
                     default=None,
                     help="short eager run for rocprofv3 (no timing, no JSON)")
@@ -748,6 +793,8 @@ def main():
         del ds, ms
 
     if not args.no_extras:
+        extras["evrard_monopole"] = monopole_leg(ctx, hydro, problems, torch)
+        print(f"# monopole: {extras['evrard_monopole']}", file=sys.stderr, flush=True)
         extras["two_level_blast2d"] = two_level_leg(ctx, hydro, problems, torch)
         print(f"# two-level: {extras['two_level_blast2d']}", file=sys.stderr, flush=True)
         extras["kh_comp_ONe_lset"] = composition_leg(ctx, hydro, problems, torch)

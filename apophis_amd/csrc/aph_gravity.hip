@@ -149,11 +149,12 @@ __global__ void k_mono_zero(unsigned long long *acc, int n, int *emax) {
 // One workgroup takes up to kChunk consecutive interior cells of one block (a whole 32^3
 // block), so that it touches few radial bins and adds each of them to HBM once.
 constexpr int kChunk = 32768;
+constexpr int kPassThreads = 1024;  // bound / accumulate: 16 waves per workgroup
 
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kPassThreads)
     k_mono_bound(const aph_block *__restrict__ blocks, int first, int ndim, int nvar,
                  const double *__restrict__ field, double dfloor, double rmax, int *emax) {
-  __shared__ int wave_max[kThreads / 64];
+  __shared__ int wave_max[kPassThreads / 64];
   const aph_block &bk = blocks[first + blockIdx.y];
   const int64_t nint = (int64_t)bk.nx[0] * bk.nx[1] * bk.nx[2];
   const int64_t c0 = (int64_t)blockIdx.x * kChunk;
@@ -169,7 +170,7 @@ __global__ void __launch_bounds__(kThreads)
   if ((threadIdx.x & 63) == 0) wave_max[threadIdx.x >> 6] = e;
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int w = 1; w < kThreads / 64; ++w) e = max(e, wave_max[w]);
+    for (int w = 1; w < kPassThreads / 64; ++w) e = max(e, wave_max[w]);
     // one atomic per workgroup, none when the bound is already as large
     if (e != INT_MIN && e > __hip_atomic_load(emax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
       atomicMax(emax, e);
@@ -177,7 +178,7 @@ __global__ void __launch_bounds__(kThreads)
 }
 
 template <bool LDS>
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kPassThreads)
     k_mono_accum(const aph_block *__restrict__ blocks, int first, int ndim, int nvar,
                  const double *__restrict__ field, double dfloor, double rmax, int nrbin,
                  int log2_cells, const int *__restrict__ emax, unsigned long long *acc) {
@@ -434,7 +435,7 @@ int mono_begin(aph_context *ctx, aph_monopole *m, const aph_blockset *bs, const 
                      ctx->stream, m->acc, n, m->emax);
   APH_LAUNCH_CHECK(ctx);
   if (bs->nblocks > 0) {
-    hipLaunchKernelGGL(k_mono_bound, chunk_grid(bs, bs->nblocks), dim3(kThreads), 0,
+    hipLaunchKernelGGL(k_mono_bound, chunk_grid(bs, bs->nblocks), dim3(kPassThreads), 0,
                        ctx->stream, bs->d, 0, bs->ndim, bs->nvar, field, dfloor, m->rmax,
                        m->emax);
     APH_LAUNCH_CHECK(ctx);
@@ -445,12 +446,12 @@ int mono_begin(aph_context *ctx, aph_monopole *m, const aph_blockset *bs, const 
 int mono_accumulate(aph_context *ctx, aph_monopole *m, const aph_blockset *bs, int lo, int cnt,
                     const double *field, double dfloor) {
   if (m->nrbin <= kLdsMaxBins) {
-    hipLaunchKernelGGL(k_mono_accum<true>, chunk_grid(bs, cnt), dim3(kThreads),
+    hipLaunchKernelGGL(k_mono_accum<true>, chunk_grid(bs, cnt), dim3(kPassThreads),
                        sizeof(unsigned long long) * 2 * m->nrbin, ctx->stream, bs->d, lo,
                        bs->ndim, bs->nvar, field, dfloor, m->rmax, m->nrbin, m->log2_cells,
                        m->emax, m->acc);
   } else {
-    hipLaunchKernelGGL(k_mono_accum<false>, chunk_grid(bs, cnt), dim3(kThreads), 0,
+    hipLaunchKernelGGL(k_mono_accum<false>, chunk_grid(bs, cnt), dim3(kPassThreads), 0,
                        ctx->stream, bs->d, lo, bs->ndim, bs->nvar, field, dfloor, m->rmax,
                        m->nrbin, m->log2_cells, m->emax, m->acc);
   }

@@ -88,6 +88,12 @@ _SIGS = {
     "aph_monopole_gravity_field": (_i32, [_vp, _vp, _vp, _i32, _i32, _vp, _vp]),
     "aph_monopole_get_tables": (_i32, [_vp, _vp, _vp, _vp]),
     "aph_plan_set_monopole": (_i32, [_vp, _vp, _vp]),
+    "aph_poisson_create": (_i32, [_vp, _vp, _d, _d, _d, _i32, ctypes.POINTER(_vp)]),
+    "aph_poisson_destroy": (_i32, [_vp, _vp]),
+    "aph_poisson_solve": (_i32, [_vp, _vp, _vp, _vp, _vp]),
+    "aph_poisson_get_info": (_i32, [_vp, _vp, _vp, ctypes.POINTER(_i32),
+                                    ctypes.POINTER(_d)]),
+    "aph_plan_set_poisson": (_i32, [_vp, _vp, _vp]),
     "aph_fused_stage": (_i32, [_vp, _vp, _i32, _vp, _vp, _vp, _vp]),
     "aph_plan_step": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _i32]),
     "aph_plan_graph_capture": (_i32, [_vp, _vp, _i32, _vp, _vp, _vp, _vp, _d, _d, _i32]),

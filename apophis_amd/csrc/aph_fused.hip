@@ -80,7 +80,8 @@ struct aph_plan {
   double g[3] = {0, 0, 0};
   int grav = 0;
   aph_monopole *mono = nullptr;  // monopole gravity before every stage (aph_plan_set_monopole)
-  double *d_gfield = nullptr;    // its field: 3 components, pack layout
+  aph_poisson *poisson = nullptr;  // Poisson gravity before every stage (aph_plan_set_poisson)
+  double *d_gfield = nullptr;    // their field: 3 components, pack layout
   std::vector<aph_tile> user_tiles;  // aph_plan_set_tile overrides, keyed by nx[0]
   // autotuner choices, keyed by block shape: the tile and the segment scale (1: the
   // shortest segments that keep every lane busy, 2: twice as long where they divide)
@@ -378,8 +379,8 @@ FusedArgs make_args(aph_plan *p, int stage, int nstages, double gam0, double gam
   a.g[0] = p->g[0];
   a.g[1] = p->g[1];
   a.g[2] = p->g[2];
-  a.grav = p->grav || p->mono;
-  a.gfield = p->mono ? p->d_gfield : nullptr;
+  a.grav = p->grav || p->mono || p->poisson;
+  a.gfield = (p->mono || p->poisson) ? p->d_gfield : nullptr;
   a.u_in = u_in;
   a.u_n = u_n;
   a.u_out = u_out;
@@ -419,6 +420,10 @@ int launch_kernel(aph_context *ctx, aph_plan *p, int recon, int stage, int nst, 
   if (!fn) return set_error(ctx, APH_E_INVALID, "no fused kernel for this stage");
   if (p->mono) {  // the solver's tasks on the stage input, then the stage reads the field
     int rc = monopole_pass(ctx, p->mono, p->bs, u_in, p->eos.dfloor, p->d_gfield);
+    if (rc) return rc;
+  }
+  if (p->poisson) {
+    int rc = poisson_pass(ctx, p->poisson, u_in, p->eos.dfloor, p->d_gfield, nullptr);
     if (rc) return rc;
   }
   FusedArgs a = make_args(p, stage, nst, gam0, gam1, beta, u_in, u_n, u_out, dt_dev, cfl);
@@ -475,7 +480,8 @@ int adj_coef(const aph_plan *p, int s, AdjCoef &c, int &srecon) {
 // pair (and vl2's first-order predictor) the adjoint kernels cover.
 int adj_plan_ok(aph_context *ctx, const aph_plan *p) {
   if (p->amr) return set_error(ctx, APH_E_INVALID, "adjoint: multilevel meshes are not supported");
-  if (p->mono) return set_error(ctx, APH_E_INVALID, "adjoint: monopole gravity is not supported");
+  if (p->mono || p->poisson)
+    return set_error(ctx, APH_E_INVALID, "adjoint: monopole / Poisson gravity is not supported");
   if (!adj_supported(p->recon, p->rsolver, p->bs->nvar, p->bs->ndim) ||
       (p->integrator == APH_INTEGRATOR_VL2 &&
        !adj_supported(RECON_DC, p->rsolver, p->bs->nvar, p->bs->ndim)))
@@ -744,6 +750,20 @@ int aph_plan_set_tile(aph_context *ctx, aph_plan *p, const aph_tile *t) {
     build_items(ctx, p);
     return rc;
   }
+  drop_graph(p);
+  return APH_OK;
+}
+
+int aph_plan_set_poisson(aph_context *ctx, aph_plan *p, aph_poisson *ps) {
+  if (!ctx) return APH_E_INVALID;
+  if (!p) return set_error(ctx, APH_E_INVALID, "aph_plan_set_poisson: null plan");
+  if (ps && poisson_blockset(ps) != p->bs)
+    return set_error(ctx, APH_E_INVALID, "aph_plan_set_poisson: solver of another block set");
+  if (ps && !p->d_gfield) {
+    APH_HIP_CHECK(ctx, hipMalloc(&p->d_gfield, sizeof(double) * 3 *
+                                                   std::max<int64_t>(p->bs->ncells, 1)));
+  }
+  p->poisson = ps;
   drop_graph(p);
   return APH_OK;
 }
@@ -1102,6 +1122,9 @@ int aph_plan_graph_capture(aph_context *ctx, aph_plan *p, int32_t nsteps, double
   if (!p || nsteps < 1) return set_error(ctx, APH_E_INVALID, "aph_plan_graph_capture: bad args");
   if (!ctx->stream)
     return set_error(ctx, APH_E_STATE, "graph capture needs a non-default stream");
+  if (p->poisson)
+    return set_error(ctx, APH_E_STATE,
+                     "graph capture: the Poisson solve tests its convergence on the host");
   if (p->dirty) {
     int rc = build_items(ctx, p);
     if (rc) return rc;

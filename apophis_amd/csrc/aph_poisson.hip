@@ -1,0 +1,648 @@
+// aph_poisson.hip -- Poisson self-gravity (src/gravity/gsolvers/poisson_gravity.hpp) on a
+// uniform 3D lattice mesh whose outer faces are all physical boundaries.
+//
+// Per call of the solver's tasks:
+//   k_pmom_block, k_pmom_final  ComputeMPCoeff + ScaleMPCoeff (:430-500): the nine
+//                               multipole moments of rho V, per block then over blocks in
+//                               a fixed order (the same sums every run)
+//   k_prhs                      ComputeRhs (:502-589): 4 pi G rho minus 2 phi_b / dx^2 on
+//                               every domain face, gathered into one lattice array
+//   the solve                   the discrete problem of PoissonEquation (:31-388) with
+//                               D = 1: conjugate gradients on -A (symmetric positive
+//                               definite: the 7-point Laplacian with phi = 0 on the domain
+//                               faces, plus alpha), preconditioned by one geometric
+//                               multigrid V-cycle (cell averages down, constant values up,
+//                               two weighted-Jacobi sweeps each way), to the relative
+//                               residual tolerance.  The reference reaches the same
+//                               discrete solution with Parthenon's MG / BiCGSTAB.
+//   k_pfield                    ComputeGravityVector (:591-625) with MultipoleGravityBC's
+//                               ghost values (gravity.cpp:37-104): g = -grad phi.
+// Scalars of the iteration live on the device; once converged every later kernel of the
+// iteration returns at once, and the host reads the flag every few iterations.
+#include <climits>
+
+#include "aph_device.hpp"
+#include "aph_runtime.hpp"
+
+namespace {
+constexpr int kMaxLevels = 10;
+constexpr int kRedBlocks = 1024;  // workgroups of a reduction's first pass
+}  // namespace
+
+struct aph_poisson {
+  const aph_blockset *bs = nullptr;
+  int n[3] = {0, 0, 0};  // lattice cells along x, y, z
+  double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0}, dx[3] = {0, 0, 0};
+  double G = 0.0, alpha = 0.0, tol = 1e-8;
+  int max_it = 0;
+  int nlev = 0;
+  int ln[kMaxLevels][3] = {};
+  int32_t *d_boff = nullptr;  // per block: its lattice offset (x, y, z)
+  double *x = nullptr, *b = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
+  double *e[kMaxLevels] = {}, *rl[kMaxLevels] = {}, *t[kMaxLevels] = {};
+  double *part = nullptr;  // reduction partials
+  double *sc = nullptr;    // device scalars (see Sc)
+  double *coef = nullptr;  // nine multipole coefficients
+  double *bmom = nullptr;  // per-block moments
+  double *h_sc = nullptr;  // pinned host copy of the scalars
+};
+
+namespace aph {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr double kOmega = 0.8;  // weighted Jacobi (two sweeps before and after)
+constexpr int kCoarseSweeps = 40;
+
+// device scalars
+enum Sc { RZ = 0, PQ, RR, BB, ALPHA, BETA, RZN, DONE, ITERS, NSC };
+
+struct Grid {
+  int nx, ny, nz;
+  double ix2, iy2, iz2;  // 1/dx^2 per axis
+  double alpha;
+};
+
+__device__ __forceinline__ bool finished(const double *sc) { return sc[DONE] != 0.0; }
+
+__device__ __forceinline__ double multipole(const double *c, double x, double y, double z) {
+  const double r2 = x * x + y * y + z * z;
+  if (r2 == 0.0) return 0.0;
+  const double r = sqrt(r2);
+  const double r3 = r * r2;
+  const double r5 = r3 * r2;
+  return c[0] / r + (c[1] * y + c[2] * z + c[3] * x) / r3 +
+         (c[4] * x * y + c[5] * y * z + c[6] * (3.0 * z * z - r2) + c[7] * z * x +
+          c[8] * 0.5 * (x * x - y * y)) /
+             r5;
+}
+
+// L e = -(A e): sum over axes of (2 e - e_- - e_+) / dx^2 + alpha e, e_ghost = -e.
+__device__ __forceinline__ double apply_at(const Grid &g, const double *e, int i, int j, int k,
+                                           int64_t c) {
+  const double ec = e[c];
+  const int64_t sy = g.nx, sz = (int64_t)g.nx * g.ny;
+  const double xm = i > 0 ? e[c - 1] : -ec, xp = i < g.nx - 1 ? e[c + 1] : -ec;
+  const double ym = j > 0 ? e[c - sy] : -ec, yp = j < g.ny - 1 ? e[c + sy] : -ec;
+  const double zm = k > 0 ? e[c - sz] : -ec, zp = k < g.nz - 1 ? e[c + sz] : -ec;
+  return (2.0 * ec - xm - xp) * g.ix2 + (2.0 * ec - ym - yp) * g.iy2 +
+         (2.0 * ec - zm - zp) * g.iz2 + g.alpha * ec;
+}
+
+__device__ __forceinline__ double diag_at(const Grid &g, int i, int j, int k) {
+  return (2.0 + (i == 0) + (i == g.nx - 1)) * g.ix2 + (2.0 + (j == 0) + (j == g.ny - 1)) * g.iy2 +
+         (2.0 + (k == 0) + (k == g.nz - 1)) * g.iz2 + g.alpha;
+}
+
+__device__ __forceinline__ void split3(const Grid &g, int64_t c, int &i, int &j, int &k) {
+  i = (int)(c % g.nx);
+  const int64_t r = c / g.nx;
+  j = (int)(r % g.ny);
+  k = (int)(r / g.ny);
+}
+
+#define APH_GRID_LOOP(g, c)                                                              \
+  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x,                      \
+               n_ = (int64_t)(g).nx * (g).ny * (g).nz;                                  \
+       c < n_; c += (int64_t)gridDim.x * blockDim.x)
+
+__global__ void __launch_bounds__(kThreads) k_apply(Grid g, const double *sc, const double *e,
+                                                    double *out) {
+  if (finished(sc)) return;
+  APH_GRID_LOOP(g, c) {
+    int i, j, k;
+    split3(g, c, i, j, k);
+    out[c] = apply_at(g, e, i, j, k, c);
+  }
+}
+
+// first smoothing sweep from e = 0: e = omega r / diag
+__global__ void __launch_bounds__(kThreads) k_jacobi0(Grid g, const double *sc, const double *r,
+                                                      double *e) {
+  if (finished(sc)) return;
+  APH_GRID_LOOP(g, c) {
+    int i, j, k;
+    split3(g, c, i, j, k);
+    e[c] = kOmega * r[c] / diag_at(g, i, j, k);
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) k_jacobi(Grid g, const double *sc, const double *r,
+                                                     const double *e, double *out) {
+  if (finished(sc)) return;
+  APH_GRID_LOOP(g, c) {
+    int i, j, k;
+    split3(g, c, i, j, k);
+    out[c] = e[c] + kOmega * (r[c] - apply_at(g, e, i, j, k, c)) / diag_at(g, i, j, k);
+  }
+}
+
+// coarse residual: the mean of r - L e over each 2x2x2 family
+__global__ void __launch_bounds__(kThreads) k_restrict(Grid fine, Grid coarse, const double *sc,
+                                                       const double *r, const double *e,
+                                                       double *rc) {
+  if (finished(sc)) return;
+  APH_GRID_LOOP(coarse, c) {
+    int i, j, k;
+    split3(coarse, c, i, j, k);
+    double s = 0.0;
+    for (int dk = 0; dk < 2; ++dk)
+      for (int dj = 0; dj < 2; ++dj)
+        for (int di = 0; di < 2; ++di) {
+          const int fi = 2 * i + di, fj = 2 * j + dj, fk = 2 * k + dk;
+          const int64_t f = ((int64_t)fk * fine.ny + fj) * fine.nx + fi;
+          s += r[f] - apply_at(fine, e, fi, fj, fk, f);
+        }
+    rc[c] = 0.125 * s;
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) k_prolong_add(Grid fine, Grid coarse,
+                                                          const double *sc, const double *ec,
+                                                          double *e) {
+  if (finished(sc)) return;
+  APH_GRID_LOOP(fine, c) {
+    int i, j, k;
+    split3(fine, c, i, j, k);
+    e[c] += ec[((int64_t)(k >> 1) * coarse.ny + (j >> 1)) * coarse.nx + (i >> 1)];
+  }
+}
+
+// first pass of a dot product: one partial per workgroup, each over a fixed index set
+__global__ void __launch_bounds__(kThreads) k_dot(int64_t n, const double *sc, const double *a,
+                                                  const double *b, double *part) {
+  __shared__ double s[kThreads];
+  if (finished(sc)) return;
+  double v = 0.0;
+  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+       c += (int64_t)gridDim.x * blockDim.x)
+    v += a[c] * b[c];
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+// second pass: the partials in a fixed order into scalar `slot`
+__global__ void __launch_bounds__(kThreads) k_dot_final(int nparts, double *sc, int slot,
+                                                        const double *part) {
+  __shared__ double s[kThreads];
+  if (finished(sc)) return;
+  double v = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x) v += part[i];
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) sc[slot] = s[0];
+}
+
+__global__ void k_cg_init(double *sc) {
+  for (int i = 0; i < NSC; ++i) sc[i] = 0.0;
+}
+
+__global__ void k_cg_start(double *sc) {  // after bb: a zero right-hand side is solved
+  if (sc[BB] == 0.0) sc[DONE] = 1.0;
+}
+
+__global__ void k_cg_alpha(double *sc) {
+  if (finished(sc)) return;
+  sc[ALPHA] = sc[RZ] / sc[PQ];
+}
+
+__global__ void __launch_bounds__(kThreads) k_cg_xr(int64_t n, const double *sc, double *x,
+                                                    double *r, const double *p,
+                                                    const double *q) {
+  if (finished(sc)) return;
+  const double a = sc[ALPHA];
+  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+       c += (int64_t)gridDim.x * blockDim.x) {
+    x[c] += a * p[c];
+    r[c] -= a * q[c];
+  }
+}
+
+__global__ void k_cg_check(double *sc, double tol2) {
+  if (finished(sc)) return;
+  sc[ITERS] += 1.0;
+  if (sc[RR] <= tol2 * sc[BB]) sc[DONE] = 1.0;
+}
+
+__global__ void k_cg_beta(double *sc) {
+  if (finished(sc)) return;
+  sc[BETA] = sc[RZN] / sc[RZ];
+  sc[RZ] = sc[RZN];
+}
+
+__global__ void __launch_bounds__(kThreads) k_cg_p(int64_t n, const double *sc, const double *z,
+                                                   double *p, int first) {
+  if (finished(sc)) return;
+  const double bta = first ? 0.0 : sc[BETA];
+  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+       c += (int64_t)gridDim.x * blockDim.x)
+    p[c] = first ? z[c] : z[c] + bta * p[c];
+}
+
+// r = b, x = 0
+__global__ void __launch_bounds__(kThreads) k_cg_reset(int64_t n, const double *a, double *out,
+                                                       double *zero) {
+  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+       c += (int64_t)gridDim.x * blockDim.x) {
+    out[c] = a[c];
+    zero[c] = 0.0;
+  }
+}
+
+// ComputeMPCoeff for one block: the nine moments of its interior (fixed order)
+__global__ void __launch_bounds__(kThreads)
+    k_pmom_block(const aph_block *__restrict__ blocks, int nvar, const double *field,
+                 double dfloor, double *bmom) {
+  __shared__ double s[9][kThreads];
+  const aph_block &bk = blocks[blockIdx.x];
+  const int N0 = bk.nx[0] + 2 * bk.ng, N1 = bk.nx[1] + 2 * bk.ng;
+  const int64_t nint = (int64_t)bk.nx[0] * bk.nx[1] * bk.nx[2];
+  double m[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int64_t t = threadIdx.x; t < nint; t += blockDim.x) {
+    const int i = (int)(t % bk.nx[0]), j = (int)((t / bk.nx[0]) % bk.nx[1]),
+              k = (int)(t / ((int64_t)bk.nx[0] * bk.nx[1]));
+    const int64_t cell = ((int64_t)(k + bk.ng) * N1 + (j + bk.ng)) * N0 + (i + bk.ng);
+    double rho = field[(int64_t)nvar * bk.cell_base + cell];
+    rho = (rho > dfloor) ? rho : dfloor;
+    const double x = bk.xmin[0] + (i + 0.5) * bk.dx[0];
+    const double y = bk.xmin[1] + (j + 0.5) * bk.dx[1];
+    const double z = bk.xmin[2] + (k + 0.5) * bk.dx[2];
+    const double r2 = x * x + y * y + z * z;
+    const double sm = rho * bk.vol;
+    m[0] += sm;
+    m[1] += sm * y;
+    m[2] += sm * z;
+    m[3] += sm * x;
+    m[4] += sm * x * y;
+    m[5] += sm * y * z;
+    m[6] += sm * (3.0 * z * z - r2);
+    m[7] += sm * z * x;
+    m[8] += sm * 0.5 * (x * x - y * y);
+  }
+  for (int n = 0; n < 9; ++n) s[n][threadIdx.x] = m[n];
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int n = 0; n < 9; ++n) s[n][threadIdx.x] += s[n][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 9) bmom[(int64_t)blockIdx.x * 9 + threadIdx.x] = s[threadIdx.x][0];
+}
+
+// the blocks' moments in block order, scaled (ScaleMPCoeff, :479-495)
+__global__ void k_pmom_final(int nblocks, const double *bmom, double G, double *coef) {
+  const int n = threadIdx.x;
+  if (n >= 9) return;
+  const double scale[9] = {-1.0, -1.0, -1.0, -1.0, -3.0, -3.0, -0.25, -3.0, -3.0};
+  double s = 0.0;
+  for (int b = 0; b < nblocks; ++b) s += bmom[(int64_t)b * 9 + n];
+  coef[n] = s * scale[n] * G;
+}
+
+// ComputeRhs into the lattice, negated for the positive definite form -A
+__global__ void __launch_bounds__(kThreads)
+    k_prhs(const aph_block *__restrict__ blocks, int nvar, const double *field, double dfloor,
+           const int32_t *__restrict__ boff, Grid g, double lox, double loy, double loz,
+           double hix, double hiy, double hiz, double four_pi_g, const double *coef,
+           double *b) {
+  const aph_block &bk = blocks[blockIdx.y];
+  const int N0 = bk.nx[0] + 2 * bk.ng, N1 = bk.nx[1] + 2 * bk.ng;
+  const int64_t nint = (int64_t)bk.nx[0] * bk.nx[1] * bk.nx[2];
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nint;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(t % bk.nx[0]), j = (int)((t / bk.nx[0]) % bk.nx[1]),
+              k = (int)(t / ((int64_t)bk.nx[0] * bk.nx[1]));
+    const int64_t cell = ((int64_t)(k + bk.ng) * N1 + (j + bk.ng)) * N0 + (i + bk.ng);
+    double rho = field[(int64_t)nvar * bk.cell_base + cell];
+    rho = (rho > dfloor) ? rho : dfloor;
+    const int gi = boff[3 * blockIdx.y] + i, gj = boff[3 * blockIdx.y + 1] + j,
+              gk = boff[3 * blockIdx.y + 2] + k;
+    const double x = bk.xmin[0] + (i + 0.5) * bk.dx[0];
+    const double y = bk.xmin[1] + (j + 0.5) * bk.dx[1];
+    const double z = bk.xmin[2] + (k + 0.5) * bk.dx[2];
+    double v = four_pi_g * rho;
+    if (gi == 0) v -= 2.0 * multipole(coef, lox, y, z) * g.ix2;
+    if (gi == g.nx - 1) v -= 2.0 * multipole(coef, hix, y, z) * g.ix2;
+    if (gj == 0) v -= 2.0 * multipole(coef, x, loy, z) * g.iy2;
+    if (gj == g.ny - 1) v -= 2.0 * multipole(coef, x, hiy, z) * g.iy2;
+    if (gk == 0) v -= 2.0 * multipole(coef, x, y, loz) * g.iz2;
+    if (gk == g.nz - 1) v -= 2.0 * multipole(coef, x, y, hiz) * g.iz2;
+    b[((int64_t)gk * g.ny + gj) * g.nx + gi] = -v;
+  }
+}
+
+// g = -grad phi, centred, with the ghost value 2 phi_0(face) - phi(mirror) at the domain
+// faces; phi and g back into the pack layout
+__global__ void __launch_bounds__(kThreads)
+    k_pfield(const aph_block *__restrict__ blocks, const int32_t *__restrict__ boff, Grid g,
+             double lox, double loy, double loz, double hix, double hiy, double hiz,
+             const double *coef, const double *phi_l, double *grav, double *phi) {
+  const aph_block &bk = blocks[blockIdx.y];
+  const int N0 = bk.nx[0] + 2 * bk.ng, N1 = bk.nx[1] + 2 * bk.ng, N2 = bk.nx[2] + 2 * bk.ng;
+  const int64_t nc = (int64_t)N0 * N1 * N2;
+  const int64_t nint = (int64_t)bk.nx[0] * bk.nx[1] * bk.nx[2];
+  const int64_t sy = g.nx, sz = (int64_t)g.nx * g.ny;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nint;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(t % bk.nx[0]), j = (int)((t / bk.nx[0]) % bk.nx[1]),
+              k = (int)(t / ((int64_t)bk.nx[0] * bk.nx[1]));
+    const int gi = boff[3 * blockIdx.y] + i, gj = boff[3 * blockIdx.y + 1] + j,
+              gk = boff[3 * blockIdx.y + 2] + k;
+    const double x = bk.xmin[0] + (i + 0.5) * bk.dx[0];
+    const double y = bk.xmin[1] + (j + 0.5) * bk.dx[1];
+    const double z = bk.xmin[2] + (k + 0.5) * bk.dx[2];
+    const int64_t c = ((int64_t)gk * g.ny + gj) * g.nx + gi;
+    const double pc = phi_l[c];
+    const double xm = gi > 0 ? phi_l[c - 1] : 2.0 * multipole(coef, lox, y, z) - pc;
+    const double xp = gi < g.nx - 1 ? phi_l[c + 1] : 2.0 * multipole(coef, hix, y, z) - pc;
+    const double ym = gj > 0 ? phi_l[c - sy] : 2.0 * multipole(coef, x, loy, z) - pc;
+    const double yp = gj < g.ny - 1 ? phi_l[c + sy] : 2.0 * multipole(coef, x, hiy, z) - pc;
+    const double zm = gk > 0 ? phi_l[c - sz] : 2.0 * multipole(coef, x, y, loz) - pc;
+    const double zp = gk < g.nz - 1 ? phi_l[c + sz] : 2.0 * multipole(coef, x, y, hiz) - pc;
+    const int64_t cell = ((int64_t)(k + bk.ng) * N1 + (j + bk.ng)) * N0 + (i + bk.ng);
+    double *gp = grav + 3 * bk.cell_base + cell;
+    gp[0] = -((xp - xm) / (2.0 * bk.dx[0]));
+    gp[nc] = -((yp - ym) / (2.0 * bk.dx[1]));
+    gp[2 * nc] = -((zp - zm) / (2.0 * bk.dx[2]));
+    if (phi) phi[bk.cell_base + cell] = pc;
+  }
+}
+
+Grid level_grid(const aph_poisson *ps, int l) {
+  Grid g;
+  g.nx = ps->ln[l][0];
+  g.ny = ps->ln[l][1];
+  g.nz = ps->ln[l][2];
+  const double f = (double)(1 << l);
+  g.ix2 = 1.0 / ((ps->dx[0] * f) * (ps->dx[0] * f));
+  g.iy2 = 1.0 / ((ps->dx[1] * f) * (ps->dx[1] * f));
+  g.iz2 = 1.0 / ((ps->dx[2] * f) * (ps->dx[2] * f));
+  g.alpha = ps->alpha;
+  return g;
+}
+
+int64_t cells(const Grid &g) { return (int64_t)g.nx * g.ny * g.nz; }
+
+dim3 vec_grid(int64_t n) {
+  return dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>((n + kThreads - 1) / kThreads,
+                                                               4096)));
+}
+
+void dot(aph_context *ctx, aph_poisson *ps, int64_t n, const double *a, const double *b,
+         int slot) {
+  const unsigned parts =
+      (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + kThreads - 1) / kThreads, kRedBlocks));
+  hipLaunchKernelGGL(k_dot, dim3(parts), dim3(kThreads), 0, ctx->stream, n, ps->sc, a, b,
+                     ps->part);
+  hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kThreads), 0, ctx->stream, (int)parts, ps->sc,
+                     slot, ps->part);
+}
+
+// e_l = V-cycle(r_l) on level l
+void vcycle(aph_context *ctx, aph_poisson *ps, int l, const double *r, double *e) {
+  const Grid g = level_grid(ps, l);
+  const int64_t n = cells(g);
+  const dim3 gr = vec_grid(n);
+  double *t = ps->t[l];
+  if (l == ps->nlev - 1) {  // coarsest: Jacobi sweeps
+    hipLaunchKernelGGL(k_jacobi0, gr, dim3(kThreads), 0, ctx->stream, g, ps->sc, r, e);
+    for (int s = 1; s < kCoarseSweeps; s += 2) {
+      hipLaunchKernelGGL(k_jacobi, gr, dim3(kThreads), 0, ctx->stream, g, ps->sc, r, e, t);
+      hipLaunchKernelGGL(k_jacobi, gr, dim3(kThreads), 0, ctx->stream, g, ps->sc, r, t, e);
+    }
+    return;
+  }
+  // pre-smoothing (kSweeps = 2: e -> t -> e)
+  hipLaunchKernelGGL(k_jacobi0, gr, dim3(kThreads), 0, ctx->stream, g, ps->sc, r, t);
+  hipLaunchKernelGGL(k_jacobi, gr, dim3(kThreads), 0, ctx->stream, g, ps->sc, r, t, e);
+  const Grid gc = level_grid(ps, l + 1);
+  hipLaunchKernelGGL(k_restrict, vec_grid(cells(gc)), dim3(kThreads), 0, ctx->stream, g, gc,
+                     ps->sc, r, e, ps->rl[l + 1]);
+  vcycle(ctx, ps, l + 1, ps->rl[l + 1], ps->e[l + 1]);
+  hipLaunchKernelGGL(k_prolong_add, gr, dim3(kThreads), 0, ctx->stream, g, gc, ps->sc,
+                     ps->e[l + 1], e);
+  // post-smoothing (the adjoint order of the pre-smoothing: e -> t -> e)
+  hipLaunchKernelGGL(k_jacobi, gr, dim3(kThreads), 0, ctx->stream, g, ps->sc, r, e, t);
+  hipLaunchKernelGGL(k_jacobi, gr, dim3(kThreads), 0, ctx->stream, g, ps->sc, r, t, e);
+}
+
+int check_lattice(aph_context *ctx, const aph_blockset *bs, aph_poisson *ps,
+                  std::vector<int32_t> &boff) {
+  if (bs->ndim != 3)
+    return set_error(ctx, APH_E_INVALID, "aph_poisson_create: 3D block sets only");
+  if (bs->nblocks < 1) return set_error(ctx, APH_E_INVALID, "aph_poisson_create: no blocks");
+  const aph_block &b0 = bs->h[0];
+  for (int d = 0; d < 3; ++d) {
+    ps->dx[d] = b0.dx[d];
+    ps->lo[d] = b0.xmin[d];
+    ps->hi[d] = b0.xmin[d] + b0.nx[d] * b0.dx[d];
+  }
+  for (const aph_block &b : bs->h)
+    for (int d = 0; d < 3; ++d) {
+      if (b.nx[d] != b0.nx[d] || b.dx[d] != b0.dx[d])
+        return set_error(ctx, APH_E_INVALID,
+                         "aph_poisson_create: blocks of one size and spacing required");
+      ps->lo[d] = std::min(ps->lo[d], b.xmin[d]);
+      ps->hi[d] = std::max(ps->hi[d], b.xmin[d] + b.nx[d] * b.dx[d]);
+    }
+  int64_t tot = 1;
+  for (int d = 0; d < 3; ++d) {
+    ps->n[d] = (int)llround((ps->hi[d] - ps->lo[d]) / ps->dx[d]);
+    tot *= ps->n[d];
+  }
+  if (tot != (int64_t)bs->nblocks * b0.nx[0] * b0.nx[1] * b0.nx[2])
+    return set_error(ctx, APH_E_INVALID, "aph_poisson_create: the blocks must tile a box");
+  boff.assign(3 * (size_t)bs->nblocks, 0);
+  for (int b = 0; b < bs->nblocks; ++b) {
+    const aph_block &k = bs->h[b];
+    for (int d = 0; d < 3; ++d) {
+      const int o = (int)llround((k.xmin[d] - ps->lo[d]) / ps->dx[d]);
+      boff[3 * (size_t)b + d] = o;
+      const int sdn = (d == 0) ? 1 : (d == 1 ? 3 : 9);
+      // a block on a face of the box must have no neighbour beyond it (not periodic)
+      if ((o == 0 && k.nbr[13 - sdn] >= 0) || (o + k.nx[d] == ps->n[d] && k.nbr[13 + sdn] >= 0))
+        return set_error(ctx, APH_E_INVALID,
+                         "aph_poisson_create: every face of the domain must be a physical "
+                         "boundary (the periodic problem needs a mean density subtracted)");
+    }
+  }
+  return APH_OK;
+}
+
+}  // namespace
+
+// The solver's whole pass (PreComputeTasks then AddTasks) for a density field (conserved
+// or primitive, floored at dfloor): grav (3 components) and, if given, phi.
+const aph_blockset *poisson_blockset(const aph_poisson *ps) { return ps->bs; }
+
+int poisson_pass(aph_context *ctx, aph_poisson *ps, const double *field, double dfloor,
+                 double *grav, double *phi) {
+  const aph_blockset *bs = ps->bs;
+  hipLaunchKernelGGL(k_pmom_block, dim3((unsigned)bs->nblocks), dim3(kThreads), 0,
+                     ctx->stream, bs->d, bs->nvar, field, dfloor, ps->bmom);
+  hipLaunchKernelGGL(k_pmom_final, dim3(1), dim3(64), 0, ctx->stream, bs->nblocks, ps->bmom,
+                     ps->G, ps->coef);
+  const Grid g = level_grid(ps, 0);
+  const int64_t n = cells(g);
+  const dim3 cg((unsigned)std::max<int64_t>(1, std::min<int64_t>(64, (bs->max_interior + kThreads - 1) / kThreads)),
+                (unsigned)bs->nblocks);
+  hipLaunchKernelGGL(k_prhs, cg, dim3(kThreads), 0, ctx->stream, bs->d, bs->nvar, field,
+                     dfloor, ps->d_boff, g, ps->lo[0], ps->lo[1], ps->lo[2], ps->hi[0],
+                     ps->hi[1], ps->hi[2], 4.0 * M_PI * ps->G, ps->coef, ps->b);
+  const dim3 gr = vec_grid(n);
+  hipLaunchKernelGGL(k_cg_init, dim3(1), dim3(1), 0, ctx->stream, ps->sc);
+  hipLaunchKernelGGL(k_cg_reset, gr, dim3(kThreads), 0, ctx->stream, n, ps->b, ps->r, ps->x);
+  dot(ctx, ps, n, ps->b, ps->b, BB);
+  hipLaunchKernelGGL(k_cg_start, dim3(1), dim3(1), 0, ctx->stream, ps->sc);
+  vcycle(ctx, ps, 0, ps->r, ps->z);
+  dot(ctx, ps, n, ps->r, ps->z, RZ);
+  hipLaunchKernelGGL(k_cg_p, gr, dim3(kThreads), 0, ctx->stream, n, ps->sc, ps->z, ps->p, 1);
+  const double tol2 = ps->tol * ps->tol;
+  bool converged = false;
+  for (int it = 0; it < ps->max_it; ++it) {
+    hipLaunchKernelGGL(k_apply, gr, dim3(kThreads), 0, ctx->stream, g, ps->sc, ps->p, ps->q);
+    dot(ctx, ps, n, ps->p, ps->q, PQ);
+    hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(1), 0, ctx->stream, ps->sc);
+    hipLaunchKernelGGL(k_cg_xr, gr, dim3(kThreads), 0, ctx->stream, n, ps->sc, ps->x, ps->r,
+                       ps->p, ps->q);
+    dot(ctx, ps, n, ps->r, ps->r, RR);
+    hipLaunchKernelGGL(k_cg_check, dim3(1), dim3(1), 0, ctx->stream, ps->sc, tol2);
+    if (it % 4 == 3 || it == ps->max_it - 1) {  // read the flag now and then; later
+                                                // kernels are no-ops once it is set
+      APH_HIP_CHECK(ctx, hipMemcpyAsync(ps->h_sc, ps->sc, sizeof(double) * NSC,
+                                        hipMemcpyDeviceToHost, ctx->stream));
+      APH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+      if (ps->h_sc[DONE] != 0.0) {
+        converged = true;
+        break;
+      }
+    }
+    vcycle(ctx, ps, 0, ps->r, ps->z);
+    dot(ctx, ps, n, ps->r, ps->z, RZN);
+    hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(1), 0, ctx->stream, ps->sc);
+    hipLaunchKernelGGL(k_cg_p, gr, dim3(kThreads), 0, ctx->stream, n, ps->sc, ps->z, ps->p, 0);
+  }
+  if (!converged)
+    return set_error(ctx, APH_E_STATE,
+                     "poisson gravity: no convergence to %g in %d iterations (relative "
+                     "residual %g)", ps->tol, ps->max_it,
+                     ps->h_sc[BB] > 0.0 ? sqrt(ps->h_sc[RR] / ps->h_sc[BB]) : 0.0);
+  hipLaunchKernelGGL(k_pfield, cg, dim3(kThreads), 0, ctx->stream, bs->d, ps->d_boff, g,
+                     ps->lo[0], ps->lo[1], ps->lo[2], ps->hi[0], ps->hi[1], ps->hi[2],
+                     ps->coef, ps->x, grav, phi);
+  APH_LAUNCH_CHECK(ctx);
+  return APH_OK;
+}
+
+}  // namespace aph
+
+using namespace aph;
+
+extern "C" {
+
+int aph_poisson_create(aph_context *ctx, const aph_blockset *bs, double gravity_constant,
+                       double diagonal_alpha, double tolerance, int32_t max_iterations,
+                       aph_poisson **out) {
+  if (!ctx) return APH_E_INVALID;
+  if (!bs || !out || !(tolerance > 0.0) || max_iterations < 1 || diagonal_alpha < 0.0)
+    return set_error(ctx, APH_E_INVALID,
+                     "aph_poisson_create: tolerance > 0, max_iterations >= 1, alpha >= 0");
+  aph_poisson *ps = new aph_poisson;
+  std::vector<int32_t> boff;
+  int rc = check_lattice(ctx, bs, ps, boff);
+  if (rc) {
+    delete ps;
+    return rc;
+  }
+  ps->bs = bs;
+  ps->G = gravity_constant;
+  ps->alpha = diagonal_alpha;
+  ps->tol = tolerance;
+  ps->max_it = max_iterations;
+  // levels: halve while every extent is even and at least 4
+  int m[3] = {ps->n[0], ps->n[1], ps->n[2]};
+  ps->nlev = 0;
+  while (ps->nlev < kMaxLevels) {
+    for (int d = 0; d < 3; ++d) ps->ln[ps->nlev][d] = m[d];
+    ++ps->nlev;
+    if (m[0] % 2 || m[1] % 2 || m[2] % 2 || m[0] < 4 || m[1] < 4 || m[2] < 4) break;
+    for (int d = 0; d < 3; ++d) m[d] /= 2;
+  }
+  const int64_t n = (int64_t)ps->n[0] * ps->n[1] * ps->n[2];
+  hipError_t e = hipMalloc(&ps->d_boff, sizeof(int32_t) * boff.size());
+  for (double **v : {&ps->x, &ps->b, &ps->r, &ps->z, &ps->p, &ps->q})
+    if (e == hipSuccess) e = hipMalloc(v, sizeof(double) * n);
+  for (int l = 0; l < ps->nlev && e == hipSuccess; ++l) {
+    const int64_t nl = (int64_t)ps->ln[l][0] * ps->ln[l][1] * ps->ln[l][2];
+    if (l > 0) e = hipMalloc(&ps->e[l], sizeof(double) * nl);
+    if (l > 0 && e == hipSuccess) e = hipMalloc(&ps->rl[l], sizeof(double) * nl);
+    if (e == hipSuccess) e = hipMalloc(&ps->t[l], sizeof(double) * nl);
+  }
+  if (e == hipSuccess) e = hipMalloc(&ps->part, sizeof(double) * kRedBlocks);
+  if (e == hipSuccess) e = hipMalloc(&ps->sc, sizeof(double) * NSC);
+  if (e == hipSuccess) e = hipMalloc(&ps->coef, sizeof(double) * 9);
+  if (e == hipSuccess) e = hipMalloc(&ps->bmom, sizeof(double) * 9 * bs->nblocks);
+  if (e == hipSuccess) e = hipHostMalloc(&ps->h_sc, sizeof(double) * NSC);
+  if (e == hipSuccess)
+    e = hipMemcpy(ps->d_boff, boff.data(), sizeof(int32_t) * boff.size(),
+                  hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(ps->sc, 0, sizeof(double) * NSC);
+  if (e == hipSuccess) e = hipMemset(ps->coef, 0, sizeof(double) * 9);
+  if (e != hipSuccess) {
+    aph_poisson_destroy(ctx, ps);
+    return set_error(ctx, APH_E_HIP, "aph_poisson_create: %s", hipGetErrorString(e));
+  }
+  *out = ps;
+  return APH_OK;
+}
+
+int aph_poisson_destroy(aph_context *ctx, aph_poisson *ps) {
+  (void)ctx;
+  if (!ps) return APH_OK;
+  for (void *v : {(void *)ps->d_boff, (void *)ps->x, (void *)ps->b, (void *)ps->r,
+                  (void *)ps->z, (void *)ps->p, (void *)ps->q, (void *)ps->part,
+                  (void *)ps->sc, (void *)ps->coef, (void *)ps->bmom})
+    if (v) (void)hipFree(v);
+  for (int l = 0; l < kMaxLevels; ++l)
+    for (void *v : {(void *)ps->e[l], (void *)ps->rl[l], (void *)ps->t[l]})
+      if (v) (void)hipFree(v);
+  if (ps->h_sc) (void)hipHostFree(ps->h_sc);
+  delete ps;
+  return APH_OK;
+}
+
+int aph_poisson_solve(aph_context *ctx, aph_poisson *ps, const double *prim, double *grav,
+                      double *phi) {
+  if (!ctx) return APH_E_INVALID;
+  if (!ps || !prim || !grav) return set_error(ctx, APH_E_INVALID, "aph_poisson_solve: null");
+  // primitive densities are floored already: a floor of -inf leaves them as they are
+  return poisson_pass(ctx, ps, prim, -HUGE_VAL, grav, phi);
+}
+
+int aph_poisson_get_info(aph_context *ctx, aph_poisson *ps, double coeffs[9],
+                         int32_t *iterations, double *rel_residual) {
+  if (!ctx) return APH_E_INVALID;
+  if (!ps) return set_error(ctx, APH_E_INVALID, "aph_poisson_get_info: null");
+  double c[9];
+  APH_HIP_CHECK(ctx, hipMemcpyAsync(c, ps->coef, sizeof c, hipMemcpyDeviceToHost, ctx->stream));
+  APH_HIP_CHECK(ctx, hipMemcpyAsync(ps->h_sc, ps->sc, sizeof(double) * NSC,
+                                    hipMemcpyDeviceToHost, ctx->stream));
+  APH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  if (coeffs)
+    for (int i = 0; i < 9; ++i) coeffs[i] = c[i];
+  if (iterations) *iterations = (int32_t)ps->h_sc[ITERS];
+  if (rel_residual)
+    *rel_residual = ps->h_sc[BB] > 0.0 ? sqrt(ps->h_sc[RR] / ps->h_sc[BB]) : 0.0;
+  return APH_OK;
+}
+
+}  // extern "C"

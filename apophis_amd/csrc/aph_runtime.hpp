@@ -59,6 +59,12 @@ int noalloc_scratch_init(int device);
 int monopole_pass(aph_context *ctx, aph_monopole *m, const aph_blockset *bs,
                   const double *field, double dfloor, double *grav);
 
+// Poisson gravity's whole pass (aph_poisson.hip) on the density of `field` (floored at
+// dfloor): grav (3 components) and, if non-null, phi.
+int poisson_pass(aph_context *ctx, aph_poisson *ps, const double *field, double dfloor,
+                 double *grav, double *phi);
+const aph_blockset *poisson_blockset(const aph_poisson *ps);
+
 // Resolve the (first_block, nblocks) sub-range convention of the plain entry points.
 int block_range(aph_context *ctx, const aph_blockset *bs, int32_t first, int32_t n,
                 int32_t *lo, int32_t *cnt);

@@ -393,10 +393,21 @@ def initialize_gravity(pin):
                                                  GRAVITY_G)))
     if gtype not in ("none", "constant", "monopole", "poisson"):
         raise ApophisError("[Apophis]: Gravity not recognized. Exiting.")
-    if gtype == "poisson":
-        raise ApophisError("[Apophis]: gravity/type = poisson (the multigrid Poisson solver) "
-                           "is not available in this library")
     pkg.add_param("gravity", Gravity[gtype])
+    if gtype == "poisson":  # gravity.cpp:297-330 and the solver block's tolerance
+        pkg.add_param("max_iterations", int(_get(pin, "gravity", "max_iterations", 10000)))
+        pkg.add_param("diagonal_alpha", float(_get(pin, "gravity", "diagonal_alpha", 0.0)))
+        solver = _get(pin, "gravity", "solver", "MG")
+        if solver not in ("MG", "MGBiCGSTAB"):
+            raise ApophisError(f"Unknown solver type {solver}.")
+        pkg.add_param("solver", solver)
+        prolong = _get(pin, "gravity", "boundary_prolongation", "Linear")
+        if prolong not in ("Linear", "Quadratic", "Constant"):
+            raise ApophisError("Unknown prolongation method for gravity boundaries.")
+        err_tol = float(_get(pin, "gravity", "err_tol", 1.0e-8))
+        pkg.add_param("err_tol", err_tol)
+        pkg.add_param("residual_tolerance",
+                      float(_get(pin, "gravity/solver_params", "residual_tolerance", err_tol)))
     if gtype == "monopole":
         nrbin = int(_get(pin, "gravity", "nrbin", -1))
         if nrbin <= 0:
@@ -488,6 +499,51 @@ class MonopoleGravitySolver:
         return m_enc, phi_rad
 
 
+class PoissonGravitySolver:
+    """PoissonGravitySolver (poisson_gravity.hpp:390-625) on the device, for a 3D block set
+    tiling a box with physical boundaries on every face: the multipole coefficients, the
+    right-hand side, the solve (to the relative residual tolerance) and the "gravity" /
+    "gravity.phi" fields."""
+
+    def __init__(self, ctx, dbs, gravity_constant, diagonal_alpha=0.0, tolerance=1e-8,
+                 max_iterations=10000):
+        self.ctx = ctx
+        self.h = ctypes.c_void_p()
+        ctx.check(_lib.lib().aph_poisson_create(ctx.h, dbs, float(gravity_constant),
+                                                float(diagonal_alpha), float(tolerance),
+                                                int(max_iterations), ctypes.byref(self.h)))
+
+    @classmethod
+    def from_package(cls, ctx, pkg, dbs):
+        return cls(ctx, dbs, pkg.param("gravity_constant"), pkg.param("diagonal_alpha"),
+                   pkg.param("residual_tolerance"), pkg.param("max_iterations"))
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        try:
+            if h and h.value:
+                _lib.lib().aph_poisson_destroy(self.ctx.h, h)
+                self.h = ctypes.c_void_p()
+        except Exception:  # interpreter shutdown: the module may be gone already
+            pass
+
+    def solve(self, md, grav, phi=None):
+        """The stage's tasks on md.prim (every block): grav and phi filled."""
+        self.ctx.check(_lib.lib().aph_poisson_solve(self.ctx.h, self.h, _ptr(md.prim),
+                                                    _ptr(grav),
+                                                    _ptr(phi) if phi is not None else None))
+        return TaskStatus.complete
+
+    def info(self):
+        """(nine multipole coefficients, iterations, relative residual) of the last solve."""
+        c = (ctypes.c_double * 9)()
+        it = ctypes.c_int32()
+        res = ctypes.c_double()
+        self.ctx.check(_lib.lib().aph_poisson_get_info(self.ctx.h, self.h, c,
+                                                       ctypes.byref(it), ctypes.byref(res)))
+        return np.array(c[:]), int(it.value), float(res.value)
+
+
 def apply_gravity_field(md, grav, dt):
     """ApplyGravity (gravity.cpp:374-410) with a per-cell field (3 components)."""
     md.ctx.check(_lib.lib().aph_apply_gravity_field(md.ctx.h, md.dbs, md.first, md.count,
@@ -518,8 +574,13 @@ class ApophisDriver:
         # gravity: a package from initialize_gravity; "monopole" runs its solver each stage
         # and keeps the "gravity" and "gravity.phi" fields on md (md.grav, md.phi)
         self.mono = None
-        if gravity is not None and gravity.param("gravity") == Gravity.monopole:
-            self.mono = MonopoleGravitySolver.from_package(md.ctx, gravity, md.bs.ndim)
+        self.poisson = None
+        kind = gravity.param("gravity") if gravity is not None else Gravity.none
+        if kind in (Gravity.monopole, Gravity.poisson):
+            if kind == Gravity.monopole:
+                self.mono = MonopoleGravitySolver.from_package(md.ctx, gravity, md.bs.ndim)
+            else:
+                self.poisson = PoissonGravitySolver.from_package(md.ctx, gravity, md.dbs)
             md.grav = torch.zeros(md.bs.field_size(3), dtype=torch.float64,
                                   device=md.cons.device)
             md.phi = torch.zeros(md.bs.field_size(1), dtype=torch.float64,
@@ -569,6 +630,8 @@ class ApophisDriver:
         rk = self.rk
         if self.mono is not None:
             self.mono.add_tasks(self.parts, self.md.grav, self.md.phi)      # :64-66
+        if self.poisson is not None:  # one solve over the whole mesh       # :64-66
+            self.poisson.solve(self.md, self.md.grav, self.md.phi)
         for p in self.parts:
             flux_fun(p)                                                      # :70-72
         if self.amr is not None:
@@ -576,7 +639,7 @@ class ApophisDriver:
         for p in self.parts:
             update_with_flux_divergence(p, self.u1, rk["gam0"][s], rk["gam1"][s],
                                         rk["beta"][s] * dt)                  # :79-82
-            if self.mono is not None:
+            if self.mono is not None or self.poisson is not None:
                 apply_gravity_field(p, self.md.grav, rk["beta"][s] * dt)     # :84-85
             elif grav:
                 apply_gravity(p, self.g, rk["beta"][s] * dt)                 # :84-85
@@ -635,7 +698,7 @@ class FusedDriver:
 
     def __init__(self, ctx, bs: BlockSet, cons, recon, riemann, eos, cfl, g=(0.0, 0.0, 0.0),
                  tlim=np.inf, device="cuda", dbs=None, integrator="rk2", amr_mesh=None,
-                 monopole=None):
+                 monopole=None, poisson=None):
         self.ctx = ctx
         self.bs = bs
         self.device_bs = dbs or DeviceBlockSet(ctx, bs)
@@ -660,6 +723,10 @@ class FusedDriver:
         if monopole is not None:  # (nrbin, rmax, gravity_constant): monopole gravity
             self.mono = MonopoleGravitySolver(ctx, bs.ndim, *monopole)
             ctx.check(_lib.lib().aph_plan_set_monopole(ctx.h, self.plan, self.mono.h))
+        self.poisson = None
+        if poisson is not None:  # (gravity_constant, diagonal_alpha, tolerance, max_it)
+            self.poisson = PoissonGravitySolver(ctx, self.device_bs.h, *poisson)
+            ctx.check(_lib.lib().aph_plan_set_poisson(ctx.h, self.plan, self.poisson.h))
         self.amr = None
         if amr_mesh is not None:  # a multilevel mesh (amr.AmrMesh) of this block set
             from . import amr as _amr

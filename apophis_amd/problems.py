@@ -39,6 +39,8 @@ class Problem:
     meta: dict = dataclasses.field(default_factory=dict)
     integrator: str = "rk2"  # rk1 / rk2 / rk3 / vl2 (main.hpp Integrator)
     monopole: tuple = None  # monopole gravity (nrbin, rmax, gravity_constant), or None
+    # Poisson gravity (gravity_constant, diagonal_alpha, tolerance, max_iterations), or None
+    poisson: tuple = None
 
 
 # ---------------------------------------------------------------- RNG
@@ -141,11 +143,15 @@ def _default_rmax(xmin, xmax):
 
 
 def grav_sphere(res=32, block=None, ndim=3, rho0=1.0, r0=0.25, nrbin=None, G=1.0,
-                gm1=0.666666667, recon="dc", riemann="hlle", cfl=0.4, center=(0.0, 0.0, 0.0)):
-    """The gravity accuracy test (grav_test.cpp:15-55, pid 1; inputs/grav_test.in with the
-    monopole solver as tst/.../gravity/gravity.py runs it): rho = rho0 (1 - r^2/r0^2)^2
-    inside r0 and 0 outside (ConsToPrim's floor then applies), at rest, E = 1, on
-    [-1/2, 1/2]^ndim with outflow boundaries; monopole gravity with nrbin = 4 res."""
+                gm1=0.666666667, recon="dc", riemann="hlle", cfl=0.4, center=None,
+                solver="monopole", tol=1e-8):
+    """The gravity accuracy test (grav_test.cpp:15-55, pid 1; inputs/grav_test.in as
+    tst/.../gravity/gravity.py runs it): rho = rho0 (1 - r^2/r0^2)^2 inside r0 and 0
+    outside (ConsToPrim's floor then applies), at rest, E = 1, on [-1/2, 1/2]^ndim with
+    outflow boundaries.  solver "monopole": centred at the origin, nrbin = 4 res;
+    "poisson": centred at (0.01, 0, 0), the residual tolerance `tol`."""
+    if center is None:
+        center = (0.01, 0.0, 0.0) if solver == "poisson" else (0.0, 0.0, 0.0)
     block = block or res
     nb = res // block
     nblocks = tuple(nb if d < ndim else 1 for d in range(3))
@@ -161,18 +167,23 @@ def grav_sphere(res=32, block=None, ndim=3, rho0=1.0, r0=0.25, nrbin=None, G=1.0
         _set_interior(bs, cons, b, rho, 0.0, 0.0, 0.0, 1.0)
     # the mesh box of the deck on every axis (gravity.cpp reads x2/x3 extents too)
     rmax = _default_rmax(xmin, xmax)
+    grav = {"monopole": (nrbin or 4 * res, rmax, G)} if solver == "monopole" else \
+        {"poisson": (G, 0.0, tol, 10000)}
     return Problem("grav_sphere", bs, cons, make_eos(gm1), recon, riemann, cfl,
-                   integrator="rk1", monopole=(nrbin or 4 * res, rmax, G),
-                   meta={"rho0": rho0, "r0": r0, "res": res, "box": (xmin, xmax)})
+                   integrator="rk1", **grav,
+                   meta={"rho0": rho0, "r0": r0, "res": res, "box": (xmin, xmax),
+                         "center": center})
 
 
 def evrard(res=32, block=16, mass=1.0, radius=1.0, u_therm=0.05, nrbin=None, G=1.0,
-           gm1=0.666666667, recon="plm", riemann="hllc", cfl=0.4, tlim=0.8):
+           gm1=0.666666667, recon="plm", riemann="hllc", cfl=0.4, tlim=0.8,
+           solver="monopole", tol=1e-8):
     """Evrard's collapse (evrards_collapse.cpp:15-49, inputs/evrards_collapse.in) with the
     monopole solver as tst/.../evrards_collapse/evrards_collapse.py runs it: rho =
     M / (2 pi R^2 r) inside R and 1e-5 outside, at rest, E = u_therm rho, on [-1, 1]^3
-    (res^3 cells in blocks of block^3), outflow, PLM + HLLC, RK2, CFL 0.4, to t = 0.8,
-    nrbin = 4 res."""
+    (res^3 cells in blocks of block^3), outflow, PLM + HLLC, RK2, CFL 0.4, to t = 0.8;
+    solver "monopole" (nrbin = 4 res) or "poisson" (the deck's own, residual tolerance
+    `tol`)."""
     nb = res // block
     xmin, xmax = (-1.0,) * 3, (1.0,) * 3
     m = lattice_mesh(3, (nb,) * 3, (block,) * 3, NEED_GHOST[recon], xmin, xmax,
@@ -184,9 +195,10 @@ def evrard(res=32, block=16, mass=1.0, radius=1.0, u_therm=0.05, nrbin=None, G=1
         r = np.sqrt(x * x + y * y + z * z)
         rho = np.where(r < radius, mass / (2.0 * np.pi * (radius * radius) * r), 1e-5)
         _set_interior(bs, cons, b, rho, 0.0, 0.0, 0.0, u_therm * rho)
+    grav = {"monopole": (nrbin or 4 * res, _default_rmax(xmin, xmax), G)} \
+        if solver == "monopole" else {"poisson": (G, 0.0, tol, 10000)}
     return Problem("evrard", bs, cons, make_eos(gm1), recon, riemann, cfl, tlim=tlim,
-                   monopole=(nrbin or 4 * res, _default_rmax(xmin, xmax), G),
-                   meta={"res": res, "radius": radius, "box": (xmin, xmax)})
+                   **grav, meta={"res": res, "radius": radius, "box": (xmin, xmax)})
 
 
 # ---------------------------------------------------------------- uneven blocks

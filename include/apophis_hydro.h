@@ -89,6 +89,7 @@ typedef struct aph_context aph_context;
 typedef struct aph_blockset aph_blockset;
 typedef struct aph_plan aph_plan;
 typedef struct aph_monopole aph_monopole;
+typedef struct aph_poisson aph_poisson;
 
 /* ---------------------------------------------------------------- context */
 int aph_abi_version(void);
@@ -178,6 +179,33 @@ int aph_monopole_gravity_field(aph_context *ctx, aph_monopole *m, const aph_bloc
 /* Copy m_enc[0..nrbin] and phi_rad[0..nrbin] to host arrays (synchronous). */
 int aph_monopole_get_tables(aph_context *ctx, aph_monopole *m, double *m_enc,
                             double *phi_rad);
+
+/* Poisson self-gravity: PoissonGravitySolver (src/gravity/gsolvers/poisson_gravity.hpp,
+   gravity type "poisson", gravity.cpp:239-241,279-366) on a 3D block set whose blocks tile
+   a box in one lattice, every face of the box a physical boundary (the reference's
+   gravity/solver_params apply: tolerance = residual_tolerance, relative to the right-hand
+   side's norm; diagonal_alpha).  The discrete problem is PoissonEquation's with D = 1 (the
+   7-point Laplacian over cell centres minus alpha, phi = 0 on the domain faces, the face
+   values of the multipole expansion moved to the right-hand side, ComputeRhs :502-589);
+   it is solved by conjugate gradients preconditioned with a geometric multigrid V-cycle.
+   The reference solves the same discrete problem with Parthenon's MG or BiCGSTAB, so the
+   two agree to the tolerance.  Refinement, periodic faces, 1D and 2D: refused. */
+int aph_poisson_create(aph_context *ctx, const aph_blockset *bs, double gravity_constant,
+                       double diagonal_alpha, double tolerance, int32_t max_iterations,
+                       aph_poisson **out);
+int aph_poisson_destroy(aph_context *ctx, aph_poisson *ps);
+/* One stage's solver tasks on the primitive state: PreComputeTasks (ComputeMPCoeff,
+   ScaleMPCoeff, :394-500) and AddTasks (ComputeRhs, the solve, the boundary values of phi,
+   ComputeGravityVector, :415-625).  grav: 3 components (pack layout); phi ("gravity.phi",
+   1 component) may be NULL.  Synchronous (the iteration's convergence test reads the
+   device every few iterations); fails with APH_E_STATE when max_iterations pass without
+   convergence. */
+int aph_poisson_solve(aph_context *ctx, aph_poisson *ps, const double *prim, double *grav,
+                      double *phi);
+/* The last solve's nine multipole coefficients (scaled), iterations and relative residual.
+   Synchronous; any output may be NULL. */
+int aph_poisson_get_info(aph_context *ctx, aph_poisson *ps, double coeffs[9],
+                         int32_t *iterations, double *rel_residual);
 
 /* Ghost exchange between neighbouring blocks followed by the physical boundary
    conditions (outflow / reflect, apophis_boundaries.cpp:17-92), axis by axis. */
@@ -401,6 +429,10 @@ int aph_plan_set_integrator(aph_context *ctx, aph_plan *plan, int32_t integrator
    (or a later call with m = NULL, which returns to the plan's constant g).  The adjoint
    refuses such plans.  Synchronous (allocates the field). */
 int aph_plan_set_monopole(aph_context *ctx, aph_plan *plan, aph_monopole *m);
+/* Poisson gravity for every stage of the plan (as aph_plan_set_monopole; NULL returns to
+   the constant g).  A plan with it cannot be captured in a graph (the solve reads its
+   convergence on the host). */
+int aph_plan_set_poisson(aph_context *ctx, aph_plan *plan, aph_poisson *ps);
 /* Stage `stage` (1..number of stages of the plan's integrator):
  * u_out = gam0*u_in + gam1*u_n + beta*dt*(-div F(u_in)) [+ gravity]; u_n (the step's
  * start state) is unused in stage 1. */

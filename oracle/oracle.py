@@ -145,6 +145,9 @@ class State:
         self.fx = [np.zeros_like(self.u0) for _ in range(3)]
         self.g = np.ascontiguousarray(prob.g, dtype=np.float64)
         self.mono = getattr(prob, "monopole", None)  # (nrbin, rmax, G) or None
+        self.pois = getattr(prob, "poisson", None)  # (G, alpha, tol, max_it) or None
+        if self.pois is not None:
+            self.grav = np.zeros(bs.field_size(3))
         if self.mono is not None:
             self.grav = np.zeros(bs.field_size(3))
             self.m_enc = np.zeros(int(self.mono[0]) + 1)
@@ -160,6 +163,8 @@ class State:
         need = {"dc": 1, "plm": 2, "ppm": 3}[p.recon]
         if int(self.bs.blocks["ng"].min()) < need:
             raise ValueError("Not enough ghost zones for reconstruction")
+        if self.pois is not None:
+            return self._step_poisson(dt)
         if self.mono is not None:
             nrbin, rmax, G = self.mono
             return lib().orc_step_monopole(
@@ -174,6 +179,28 @@ class State:
                               p.eos.ctypes.data_as(_vp), _p(self.g), p.cfl, dt,
                               _p(self.u0), _p(self.u1), _p(self.prim), _p(self.gamma),
                               _p(self.fx[0]), _p(self.fx[1]), _p(self.fx[2]))
+
+    def _step_poisson(self, dt):
+        """One step with Poisson gravity (the exact discrete solution of each stage's
+        problem, oracle/poisson_oracle.py) between the flux update and the ghost fill, as
+        orc_step_monopole orders the tasks."""
+        from . import poisson_oracle as P
+
+        p, bs = self.prob, self.bs
+        G, alpha = self.pois[0], self.pois[1]
+        g0, g1, be = integrator_stages(p.integrator)
+        self.u1[:] = self.u0
+        for s in range(len(g0)):
+            rec = "dc" if (s == 0 and p.integrator == "vl2") else p.recon
+            _, _, g = P.gravity(bs, self.prim, G, alpha)
+            for d in range(3):
+                P.scatter(bs, g[d], self.grav, 3, d)
+            calculate_fluxes(bs, rec, p.riemann, self.prim, self.gamma, *self.fx)
+            update(bs, self.u0, self.u1, *self.fx, g0[s], g1[s], be[s] * dt)
+            apply_gravity_field(bs, self.u0, self.prim, self.grav, be[s] * dt)
+            fill_ghosts(bs, self.u0)
+            cons_to_prim(bs, p.eos, self.u0, self.prim, self.gamma)
+        return self.estimate_dt()
 
     def run(self, nsteps=None, tlim=None, fixed_dt=None):
         """Evolve with the upstream dt policy (<= 2x growth, clip to tlim)."""

@@ -1,6 +1,7 @@
 """The host side of the gravity package (hydro.initialize_gravity, restating
 InitializeGravity, gravity.cpp:220-370, and MonopoleGravitySolver's constructor,
-monopole_gravity.hpp:83-99): types, defaults and errors.  CPU only."""
+monopole_gravity.hpp:83-99): types, defaults and errors, for the monopole and Poisson
+solvers.  CPU only."""
 import math
 
 import pytest
@@ -43,5 +44,21 @@ def test_errors():
     with pytest.raises(hydro.ApophisError, match="rmax must be positive"):
         hydro.initialize_gravity({"gravity": {"type": "monopole", "rmax": 0.0},
                                   "parthenon/mesh": MESH})
-    with pytest.raises(hydro.ApophisError, match="poisson"):
-        hydro.initialize_gravity({"gravity": {"type": "poisson"}})
+    with pytest.raises(hydro.ApophisError, match="Unknown solver type CG."):
+        hydro.initialize_gravity({"gravity": {"type": "poisson", "solver": "CG"}})
+    with pytest.raises(hydro.ApophisError, match="Unknown prolongation method"):
+        hydro.initialize_gravity({"gravity": {"type": "poisson",
+                                              "boundary_prolongation": "Cubic"}})
+
+
+def test_poisson_defaults():
+    """gravity.cpp:297-330 and the solver block: max_iterations 10000, diagonal_alpha 0,
+    solver MG, err_tol 1e-8, the residual tolerance from gravity/solver_params."""
+    pkg = hydro.initialize_gravity({"gravity": {"type": "poisson"}})
+    assert pkg.param("gravity") == hydro.Gravity.poisson
+    assert (pkg.param("max_iterations"), pkg.param("diagonal_alpha"), pkg.param("solver"),
+            pkg.param("err_tol"), pkg.param("residual_tolerance")) == (10000, 0.0, "MG", 1e-8,
+                                                                       1e-8)
+    pkg = hydro.initialize_gravity({"gravity": {"type": "poisson", "solver": "MGBiCGSTAB"},
+                                    "gravity/solver_params": {"residual_tolerance": 1e-10}})
+    assert pkg.param("residual_tolerance") == 1e-10

@@ -56,11 +56,11 @@ _SIGS = {
     "aph_amr_plan_create": (_i32, [_vp, _vp, _vp, _i64, _vp, _i64, _vp, _i64, _pp]),
     "aph_amr_plan_destroy": (_i32, [_vp, _vp]),
     "aph_amr_build": (_i32, [_vp, _i32, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
-                             _vp, ctypes.c_char_p, ctypes.c_size_t]),
+                             _vp, _vp, ctypes.c_char_p, ctypes.c_size_t]),
     "aph_amr_regrid_tree": (_i32, [_i32, _vp, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _i32,
                                    ctypes.c_char_p, ctypes.c_size_t]),
-    "aph_amr_block_table": (_i32, [_i32, _vp, _vp, _i32, _vp, _vp, _i32, _i32, _vp, _vp, _vp,
-                                   ctypes.c_char_p, ctypes.c_size_t]),
+    "aph_amr_block_table": (_i32, [_i32, _vp, _vp, _i32, _vp, _vp, _vp, _i32, _i32, _vp, _vp,
+                                   _vp, ctypes.c_char_p, ctypes.c_size_t]),
     "aph_amr_remap": (_i32, [_i32, _vp, _vp, _i32, _i32, _i32, _vp, _vp, _vp, _i32, _vp, _vp,
                              _vp, ctypes.c_char_p, ctypes.c_size_t]),
     "aph_amr_fill": (_i32, [_vp, _vp, _vp]),

@@ -1,5 +1,6 @@
-"""Multilevel meshes for the plain and fused forms: a periodic block tree with 2:1 jumps,
-as Parthenon's AMR produces.
+"""Multilevel meshes for the plain and fused forms: a block tree with 2:1 jumps, as
+Parthenon's AMR and static refinement produce, whose axes are periodic or end in physical
+(outflow / reflecting) faces.
 
 The reference leaves multilevel meshes to Parthenon (apophis_driver.cpp:68,74-76,94):
 * the boundary exchange of `cons` fills a block's ghost cells over a finer region with the
@@ -14,9 +15,18 @@ library validates and runs as two gather kernels (``aph_amr_fill``,
 ``aph_amr_flux_correct``).  Same-level neighbours keep using the block table's ``nbr``
 (``aph_fill_ghosts``); a face whose neighbour region is on another level has nbr = -1 and
 bc = APH_BC_AMR.  Leaves that touch differ by at most one level (2:1 balance across
-faces, edges and corners), so every cross-level value is one level away.  Meshes are
-periodic; each block keeps the same cells per block at every level (spacing halves), so a
-coarse cell's children always lie in one fine block.
+faces, edges and corners), so every cross-level value is one level away.  Each block
+keeps the same cells per block at every level (spacing halves), so a coarse cell's
+children always lie in one fine block.
+
+On a physical domain face the ghost cells beyond the domain are the boundary conditions'
+(``aph_amr_fill`` runs them again after the cross-level values, so edge and corner ghosts
+copy the cross-level ghosts beside them, Parthenon's order); a prolongation's slope
+neighbour beyond the domain is the coarse level's boundary image, as Parthenon applies the
+boundary conditions to the coarse buffer before it prolongates: the coarse cell itself
+(outflow) or its mirror image with the normal momentum negated (reflect).  Adaptive
+regrids (``regrid``) take periodic meshes; ``static_refinement`` builds the tree of a
+deck's ``<parthenon/static_refinementN>`` blocks.
 
 Prolongation of coarse cell c to the child on the lower / upper side (sign -1 / +1) of axis
 d uses g_d = minmod(c - c_{-d}, c_{+d} - c) and gives c + sum_d sign_d g_d / 4, the
@@ -31,10 +41,11 @@ import itertools
 import numpy as np
 
 from . import _lib
-from .mesh import BLOCK_DTYPE, BlockSet
+from .mesh import BC_OUTFLOW, BC_PERIODIC, BC_REFLECT, BLOCK_DTYPE, BlockSet, _bc_code
 
 BC_AMR = 3  # APH_BC_AMR
 AMR_MEAN, AMR_PROLONG = 0, 1
+AMR_MIRROR_BIT = 8  # APH_AMR_MIRROR_BIT: value slot k mirrored at bit 8 + k - 1
 
 AMR_REF_DTYPE = np.dtype([("offset", "<i8"), ("stride", "<i8")])
 AMR_VAL_DTYPE = np.dtype([("first", "<i4"), ("count", "<i4")])
@@ -58,13 +69,19 @@ class AmrMesh:
     ``blocks`` is a BLOCK_DTYPE table, leaves in depth-first order (base lattice z, y, x;
     children z, y, x): nbr links same-level leaves, faces with no same-level neighbour
     carry BC_AMR.  ``level[b]``, ``loc[b]`` give each leaf's level and logical position;
-    ``index[(level, loc)]`` is the leaf's block number."""
+    ``index[(level, loc)]`` is the leaf's block number.
+
+    ``bc``: six domain face codes or names (-x1, +x1, -x2, +x2, -x3, +x3), "periodic" on
+    both faces of a periodic axis, else "outflow" / "reflect"; None: every axis periodic.
+    Across a physical face there is no neighbour (and no wrap)."""
 
     def __init__(self, ndim, nbase, block_nx, ng, xmin, xmax, refined, mesh_id=0,
-                 max_level=None):
+                 max_level=None, bc=None):
         if ndim not in (2, 3):
             raise ValueError("multilevel meshes are 2D or 3D")
         self.ndim = ndim
+        self.bc = _face_codes(bc, ndim)
+        self.periodic = tuple(self.bc[2 * d] == BC_PERIODIC for d in range(3))
         self.xmin = tuple(float(v) for v in xmin)
         self.xmax = tuple(float(v) for v in xmax)
         self.mesh_id = int(mesh_id)
@@ -133,8 +150,18 @@ class AmrMesh:
         return self._offsets
 
     def shift(self, level, lc, o):
+        """The node at offset o from lc (wrapped on periodic axes), or None beyond a
+        physical domain face."""
         n = self._lattice(level)
-        return ((lc[0] + o[0]) % n[0], (lc[1] + o[1]) % n[1], (lc[2] + o[2]) % n[2])
+        out = []
+        for d in range(3):
+            c = lc[d] + o[d]
+            if self.periodic[d]:
+                c %= n[d]
+            elif not 0 <= c < n[d]:
+                return None
+            out.append(c)
+        return tuple(out)
 
     def parent(self, lc):
         return tuple(lc[d] // 2 if d < self.ndim else 0 for d in range(3))
@@ -171,12 +198,13 @@ class AmrMesh:
         xmax = np.array(self.xmax, dtype=np.float64)
         level = np.ascontiguousarray(self.level, dtype=np.int32)
         loc = np.ascontiguousarray(self.loc, dtype=np.int64)
+        bc = np.array(self.bc, dtype=np.int32)
         err = ctypes.create_string_buffer(512)
         rc = _lib.lib().aph_amr_block_table(
             self.ndim, nbase.ctypes.data_as(vp), nx.ctypes.data_as(vp), self.ng,
-            xmin.ctypes.data_as(vp), xmax.ctypes.data_as(vp), self.mesh_id, n,
-            level.ctypes.data_as(vp), loc.ctypes.data_as(vp), out.ctypes.data_as(vp), err,
-            len(err))
+            xmin.ctypes.data_as(vp), xmax.ctypes.data_as(vp), bc.ctypes.data_as(vp),
+            self.mesh_id, n, level.ctypes.data_as(vp), loc.ctypes.data_as(vp),
+            out.ctypes.data_as(vp), err, len(err))
         if rc != _lib.APH_OK:
             raise ValueError(err.value.decode())
         return out
@@ -189,20 +217,143 @@ class TwoLevelMesh(AmrMesh):
     """An AmrMesh of two levels: the base blocks at the logical positions ``refined`` are
     replaced by their 2^ndim children (level 1), and regrids stay within the two levels."""
 
-    def __init__(self, ndim, nbase, block_nx, ng, xmin, xmax, refined, mesh_id=0):
+    def __init__(self, ndim, nbase, block_nx, ng, xmin, xmax, refined, mesh_id=0, bc=None):
         super().__init__(ndim, nbase, block_nx, ng, xmin, xmax,
-                         [(0, tuple(r)) for r in refined], mesh_id=mesh_id, max_level=1)
+                         [(0, tuple(r)) for r in refined], mesh_id=mesh_id, max_level=1,
+                         bc=bc)
+
+
+def _face_codes(bc, ndim):
+    """Six domain face codes (0 periodic, 1 outflow, 2 reflect) from codes or names."""
+    if bc is None:
+        return (BC_PERIODIC,) * 6
+    if len(bc) != 6:
+        raise ValueError("bc needs six face codes (-x1, +x1, -x2, +x2, -x3, +x3)")
+    codes = [_bc_code(b) for b in bc]
+    for d in range(3):
+        if d >= ndim:
+            codes[2 * d] = codes[2 * d + 1] = BC_PERIODIC
+            continue
+        lo, hi = codes[2 * d], codes[2 * d + 1]
+        if (lo == BC_PERIODIC) != (hi == BC_PERIODIC):
+            raise ValueError(f"axis {d}: a periodic face needs a periodic opposite face")
+        if any(c not in (BC_PERIODIC, BC_OUTFLOW, BC_REFLECT) for c in (lo, hi)):
+            raise ValueError(f"axis {d}: faces must be periodic, outflow or reflect")
+    return tuple(codes)
+
+
+def static_refinement(ndim, nbase, block_nx, xmin, xmax, regions, bc=None):
+    """The refined nodes of a mesh with static refinement regions, as Parthenon builds its
+    tree from ``<parthenon/static_refinementN>`` blocks (Mesh constructor; the rule
+    Athena++ introduced): for a region (rxmin, rxmax, level) every node of that level whose
+    right edge lies beyond rxmin, up to the first whose right edge reaches rxmax, on each
+    active axis, widened to whole sibling pairs (an odd first index steps down, an even
+    last one up), is added with its ancestors; then every leaf that touches a leaf two
+    levels finer is refined until none does (2:1 balance across faces, edges and corners;
+    no wrap across physical faces).  Node edges are xmin + (l + 1) (xmax - xmin) /
+    (nbase 2^level)."""
+    nd = int(ndim)
+    nb = tuple(int(v) for v in nbase) + (1,) * (3 - len(nbase))
+    codes = _face_codes(bc, nd)
+    per = [codes[2 * d] == BC_PERIODIC for d in range(3)]
+    ref = set()
+
+    def parent(lc):
+        return tuple(lc[d] // 2 if d < nd else 0 for d in range(3))
+
+    def add_node(lv, lc):  # the node and its ancestors become refined above it
+        for L in range(lv - 1, -1, -1):
+            lc = parent(lc)
+            ref.add((L, lc))
+
+    for rmin, rmax, level in regions:
+        level = int(level)
+        if level < 1:
+            continue
+        rng = []
+        for d in range(3):
+            if d >= nd:
+                rng.append((0, 0))
+                continue
+            n = nb[d] << level
+            w = (float(xmax[d]) - float(xmin[d])) / n
+            right = lambda l: float(xmin[d]) + (l + 1) * w
+            lo = 0
+            while lo < n and not right(lo) > float(rmin[d]):
+                lo += 1
+            hi = lo
+            while hi < n and not right(hi) >= float(rmax[d]):
+                hi += 1
+            if lo % 2 == 1:
+                lo -= 1
+            if hi % 2 == 0:
+                hi += 1
+            rng.append((lo, min(hi, n - 1)))
+        for z in range(rng[2][0], rng[2][1] + 1):
+            for y in range(rng[1][0], rng[1][1] + 1):
+                for x in range(rng[0][0], rng[0][1] + 1):
+                    add_node(level, (x, y, z))
+
+    offsets = [o for o in itertools.product((-1, 0, 1), repeat=3)
+               if o != (0, 0, 0) and all(o[d] == 0 for d in range(nd, 3))]
+
+    def shift(lv, lc, o):
+        out = []
+        for d in range(3):
+            n = nb[d] << lv if d < nd else 1
+            c = lc[d] + o[d]
+            if per[d]:
+                c %= n
+            elif not 0 <= c < n:
+                return None
+            out.append(c)
+        return tuple(out)
+
+    def children(lc):
+        return [(2 * lc[0] + cx, 2 * lc[1] + cy, 2 * lc[2] + cz if nd == 3 else 0)
+                for cz in ((0, 1) if nd == 3 else (0,)) for cy in (0, 1) for cx in (0, 1)]
+
+    def leaves():
+        stack = [(0, (x, y, z)) for z in range(nb[2]) for y in range(nb[1])
+                 for x in range(nb[0])]
+        while stack:
+            lv, lc = stack.pop()
+            if (lv, lc) in ref:
+                stack.extend((lv + 1, c) for c in children(lc))
+            else:
+                yield lv, lc
+
+    def touches_finer(lv, lc):
+        for o in offsets:
+            p = shift(lv, lc, o)
+            if p is None or (lv, p) not in ref:
+                continue
+            for c in children(p):
+                if all(o[d] == 0 or (c[d] % 2) == (1 if o[d] < 0 else 0) for d in range(nd)) \
+                        and (lv + 1, c) in ref:
+                    return True
+        return False
+
+    changed = True
+    while changed:
+        changed = False
+        for lv, lc in list(leaves()):
+            if touches_finer(lv, lc):
+                ref.add((lv, lc))
+                changed = True
+    return ref
 
 
 def block_table_py(mesh: AmrMesh):
     """The block table of a mesh's leaves, stated in Python (the library builds the same
     bytes, aph_amr_block_table): 2:1 balance checked; dx, areas, volume and xmin per
-    leaf; nbr to same-level leaves with periodic wrap; BC_AMR on faces without one."""
+    leaf; nbr to same-level leaves (wrapping on periodic axes); the domain's code on faces
+    on a physical domain face, BC_AMR on other faces without a same-level neighbour."""
     ndim = mesh.ndim
     for lv, lc in mesh.index:
         for o in mesh.offsets():
             p = mesh.shift(lv, lc, o)
-            if (lv, p) in mesh.index:
+            if p is None or (lv, p) in mesh.index:
                 continue
             if (lv, p) in mesh.refined:
                 for c in mesh.touching_children(p, o):
@@ -231,15 +382,18 @@ def block_table_py(mesh: AmrMesh):
         r["mesh_id"] = mesh.mesh_id
         nbr = np.full(27, -1, dtype=np.int32)
         for o in mesh.offsets():
-            nbr[(o[2] + 1) * 9 + (o[1] + 1) * 3 + (o[0] + 1)] = mesh.index.get(
-                (lv, mesh.shift(lv, lc, o)), -1)
+            p = mesh.shift(lv, lc, o)
+            if p is not None:
+                nbr[(o[2] + 1) * 9 + (o[1] + 1) * 3 + (o[0] + 1)] = mesh.index.get((lv, p), -1)
         r["nbr"] = nbr
         bc = [1] * 6  # outflow placeholder on faces with a same-level neighbour
         for d in range(ndim):
             for side in (0, 1):
                 o = [0, 0, 0]
                 o[d] = 1 if side else -1
-                if nbr[(o[2] + 1) * 9 + (o[1] + 1) * 3 + (o[0] + 1)] < 0:
+                if mesh.shift(lv, lc, o) is None:
+                    bc[2 * d + side] = mesh.bc[2 * d + side]  # a physical domain face
+                elif nbr[(o[2] + 1) * 9 + (o[1] + 1) * 3 + (o[0] + 1)] < 0:
                     bc[2 * d + side] = BC_AMR
         r["bc"] = bc
     return out
@@ -270,11 +424,14 @@ def _cell(bs, b, local, _cache={}):
 def amr_plan(mesh: AmrMesh, bs: BlockSet):
     """(ops, fixes, refs) for ``bs`` (a block set of ``mesh``'s table).
 
-    Every ghost cell whose direction has no same-level neighbour gets one op:
+    Every ghost cell inside the domain whose direction has no same-level neighbour gets
+    one op:
     * over a refined region (the leaves there are one level finer): MEAN of the 2^ndim
       children of the ghost cell;
     * over a coarser leaf: PROLONG from the coarse cell holding it, the slope stencil's
-      neighbours taken at the coarse level (a coarse cell, or the mean of its children).
+      neighbours taken at the coarse level (a coarse cell, or the mean of its children);
+      a neighbour beyond a physical domain face is the coarse cell itself, mirrored
+      (kind bit AMR_MIRROR_BIT + slot - 1) on a reflecting face.
     Every coarse face cell on a face to a refined region gets one flux fix (the mean of
     the 2^(ndim-1) fine faces across it).  2:1 balance keeps every value one level away."""
     nd, g, nx = mesh.ndim, mesh.ng, mesh.nx
@@ -282,7 +439,12 @@ def amr_plan(mesh: AmrMesh, bs: BlockSet):
     act = range(nd)
 
     def wrap(level, G):
-        return tuple(G[d] % mesh.ncells_axis(level, d) if d < nd else 0 for d in range(3))
+        return tuple((G[d] % mesh.ncells_axis(level, d) if mesh.periodic[d] else G[d])
+                     if d < nd else 0 for d in range(3))
+
+    def inside(level, G):
+        return all(mesh.periodic[d] or 0 <= G[d] < mesh.ncells_axis(level, d)
+                   for d in range(nd))
 
     def node(G):
         return tuple(G[d] // nx[d] if d < nd else 0 for d in range(3))
@@ -291,6 +453,8 @@ def amr_plan(mesh: AmrMesh, bs: BlockSet):
         """(block, ghost-inclusive local index) of the level-`level` leaf holding global
         cell G, or None when that region is not a leaf at this level."""
         G = wrap(level, G)
+        if not inside(level, G):
+            return None
         lc = node(G)
         b = mesh.index.get((level, lc))
         if b is None:
@@ -333,6 +497,8 @@ def amr_plan(mesh: AmrMesh, bs: BlockSet):
             loc = (i, j, k)
             Gw = wrap(lv, tuple(int(lc[d]) * nx[d] + loc[d] - g if d < nd else 0
                                 for d in range(3)))
+            if not inside(lv, Gw):
+                continue  # beyond a physical face: the boundary conditions fill it
             dst, stride = _cell(bs, b, loc)
             rec = np.zeros(1, dtype=AMR_OP_DTYPE)[0]
             rec["dst"], rec["dst_stride"] = dst, stride
@@ -344,14 +510,21 @@ def amr_plan(mesh: AmrMesh, bs: BlockSet):
                 if lv == 0 or interior(lv - 1, I) is None:
                     raise ValueError(f"block {b}: ghost cell {loc} is over neither a "
                                      "finer region nor a coarser leaf (2:1 balance)")
-                rec["kind"] = AMR_PROLONG
+                kind = AMR_PROLONG
                 rec["val"][0] = value(lv - 1, I)
                 for d in act:
-                    e = [0, 0, 0]
-                    e[d] = 1
                     rec["sign"][d] = -1 if Gw[d] % 2 == 0 else 1
-                    rec["val"][1 + 2 * d] = value(lv - 1, tuple(I[t] - e[t] for t in range(3)))
-                    rec["val"][2 + 2 * d] = value(lv - 1, tuple(I[t] + e[t] for t in range(3)))
+                    for side in (0, 1):
+                        J = tuple(I[t] + ((1 if side else -1) if t == d else 0)
+                                  for t in range(3))
+                        slot = 1 + 2 * d + side
+                        if inside(lv - 1, J):
+                            rec["val"][slot] = value(lv - 1, J)
+                        else:  # beyond a physical face: the coarse cell's boundary image
+                            rec["val"][slot] = value(lv - 1, I)
+                            if mesh.bc[2 * d + side] == BC_REFLECT:
+                                kind |= 1 << (AMR_MIRROR_BIT + slot - 1)
+                rec["kind"] = kind
             ops.append(rec)
     fixes = []
     for b in range(bs.nblocks):
@@ -462,16 +635,19 @@ def native_plan(mesh: AmrMesh, bs: BlockSet):
     nbase = np.array(mesh.nbase, dtype=np.int32)
     level = np.ascontiguousarray(mesh.level, dtype=np.int32)
     loc = np.ascontiguousarray(mesh.loc, dtype=np.int64)
+    bc = np.array(mesh.bc, dtype=np.int32)
     err = ctypes.create_string_buffer(512)
     n = [ctypes.c_int64(0) for _ in range(3)]
     args = lambda o, f, r: (blocks.ctypes.data_as(vp), nb, mesh.ndim, bs.nvar,
                             nbase.ctypes.data_as(vp), level.ctypes.data_as(vp),
-                            loc.ctypes.data_as(vp), o, ctypes.byref(n[0]), f,
-                            ctypes.byref(n[1]), r, ctypes.byref(n[2]), err, len(err))
+                            loc.ctypes.data_as(vp), bc.ctypes.data_as(vp), o,
+                            ctypes.byref(n[0]), f, ctypes.byref(n[1]), r,
+                            ctypes.byref(n[2]), err, len(err))
     # one build when the arrays are long enough: they are sized from the last build of a
     # mesh of this shape (a regrid changes the lists a little); a build into arrays too
     # short reports the exact lengths, and the second build uses them
-    shape = (mesh.ndim, bs.nvar, tuple(int(v) for v in mesh.nx), int(bs.blocks["ng"][0]))
+    shape = (mesh.ndim, bs.nvar, tuple(int(v) for v in mesh.nx), int(bs.blocks["ng"][0]),
+             mesh.bc)
     hint = _PLAN_HINT.get(shape)
     cap = None if hint is None else [int(1.25 * h) + 64 for h in hint]
     for _ in range(3):
@@ -622,6 +798,7 @@ class _Sampler:
 def regrid_tree_py(mesh: AmrMesh, tags):
     """The refined nodes after one regrid (the rules of ``regrid``), stated in Python; the
     library's aph_amr_regrid_tree gives the same tree."""
+    _need_periodic(mesh)
     tags = np.asarray(tags)
     ref = set(mesh.refined)
     leaves = [(int(mesh.level[b]), tuple(int(v) for v in mesh.loc[b]))
@@ -685,6 +862,12 @@ def _ancestors(levels, locs, nd):
     return ref
 
 
+def _need_periodic(mesh: AmrMesh):
+    if not all(mesh.periodic):
+        raise ValueError("adaptive regrids take periodic meshes; this mesh has physical "
+                         "faces (static refinement builds its tree once)")
+
+
 def regrid_tree(mesh: AmrMesh, tags):
     """The refined nodes after one regrid, from the library's tree (aph_amr_regrid_tree)."""
     level, loc = regrid_leaves(mesh, tags)
@@ -695,6 +878,7 @@ def regrid_leaves(mesh: AmrMesh, tags):
     """The leaves (level [n], loc [n, 3]) after one regrid, in the mesh's leaf order, from
     the library's tree (aph_amr_regrid_tree); equal to (mesh.level, mesh.loc) when the
     tags change nothing."""
+    _need_periodic(mesh)
     vp = ctypes.c_void_p
     L = _lib.lib()
     n = len(mesh.level)
@@ -736,6 +920,7 @@ def regrid(mesh: AmrMesh, bs: BlockSet, cons, tags, leaves=None):
     from the level above, as the ghost exchange fills them).  Ghost cells are left to the next ghost fill.  Totals are conserved to
     round-off and refining then merging returns the data."""
     nd = mesh.ndim
+    _need_periodic(mesh)
     if leaves is None:
         leaves = regrid_leaves(mesh, tags)
     ref = _ancestors(leaves[0], leaves[1], nd)

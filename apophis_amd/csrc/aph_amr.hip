@@ -21,6 +21,8 @@
 
 struct aph_amr_plan {
   aph_context *ctx = nullptr;
+  const aph_blockset *bs = nullptr;  // for the boundary conditions after the fill
+  bool physical = false;             // some block has a face on a physical domain face
   int32_t nvar = 0, ndim = 0;
   int64_t nops = 0, nfixes = 0, nrefs = 0;
   aph_amr_op *ops = nullptr;
@@ -57,14 +59,21 @@ __global__ void __launch_bounds__(kThreads)
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nops;
        t += (int64_t)gridDim.x * blockDim.x) {
     const aph_amr_op op = ops[t];
+    const int mirror = op.kind >> APH_AMR_MIRROR_BIT;  // slots 1..6 beyond a reflecting face
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
       const double c = amr_value<NV>(refs, op.val[0], v, u);
       double out = c;
-      if (op.kind == APH_AMR_PROLONG) {
+      if ((op.kind & 0xff) == APH_AMR_PROLONG) {
         for (int d = 0; d < ndim; ++d) {
-          const double a = c - amr_value<NV>(refs, op.val[1 + 2 * d], v, u);
-          const double b = amr_value<NV>(refs, op.val[2 + 2 * d], v, u) - c;
+          double lo = amr_value<NV>(refs, op.val[1 + 2 * d], v, u);
+          double hi = amr_value<NV>(refs, op.val[2 + 2 * d], v, u);
+          if (v == 1 + d) {  // the mirror image negates the momentum normal to the face
+            if (mirror & (1 << (2 * d))) lo = -lo;
+            if (mirror & (2 << (2 * d))) hi = -hi;
+          }
+          const double a = c - lo;
+          const double b = hi - c;
           const double g = (0.5 * (sgn1(a) + sgn1(b))) * fmin(fabs(a), fabs(b));
           out = out + (op.sign[d] * 0.25) * g;
         }
@@ -213,6 +222,8 @@ __global__ void __launch_bounds__(kThreads)
 // The cross-level fill of cons; when epoch is non-null the same launch bumps the fused
 // form's launch epoch (it is launched even without ops then).
 int amr_fill_epoch(aph_context *ctx, const aph_amr_plan *p, double *cons, unsigned *epoch);
+// The physical boundary conditions of aph_fill_ghosts alone (aph_plain.hip).
+int physical_bc(aph_context *ctx, const aph_blockset *bs, double *cons);
 
 }  // namespace aph
 
@@ -270,14 +281,16 @@ int aph_amr_plan_create(aph_context *ctx, const aph_blockset *bs, const aph_amr_
     if (b < 0 || loc.interior(b, op.dst))
       return set_error(ctx, APH_E_INVALID, "amr op %lld: destination is not a ghost cell",
                        (long long)t);
-    if (op.kind != APH_AMR_MEAN && op.kind != APH_AMR_PROLONG)
+    const int kind = op.kind & 0xff, mirror = op.kind >> APH_AMR_MIRROR_BIT;
+    if ((kind != APH_AMR_MEAN && kind != APH_AMR_PROLONG) || mirror < 0 ||
+        mirror >= (1 << (2 * bs->ndim)) || (kind == APH_AMR_MEAN && mirror))
       return set_error(ctx, APH_E_INVALID, "amr op %lld: unknown kind %d", (long long)t, op.kind);
-    const int nval = op.kind == APH_AMR_MEAN ? 1 : 1 + 2 * bs->ndim;
+    const int nval = kind == APH_AMR_MEAN ? 1 : 1 + 2 * bs->ndim;
     for (int k = 0; k < nval; ++k)
       if (!check_val(op.val[k], true, nullptr))
         return set_error(ctx, APH_E_INVALID, "amr op %lld: value %d does not reference "
                          "interior cells", (long long)t, k);
-    if (op.kind == APH_AMR_PROLONG)
+    if (kind == APH_AMR_PROLONG)
       for (int d = 0; d < bs->ndim; ++d)
         if (op.sign[d] != -1 && op.sign[d] != 1)
           return set_error(ctx, APH_E_INVALID, "amr op %lld: sign must be +-1", (long long)t);
@@ -305,6 +318,12 @@ int aph_amr_plan_create(aph_context *ctx, const aph_blockset *bs, const aph_amr_
   aph_amr_plan *p = new (std::nothrow) aph_amr_plan();
   if (!p) return set_error(ctx, APH_E_NOMEM, "host allocation failed");
   p->ctx = ctx;
+  p->bs = bs;
+  for (const aph_block &k : bs->h)
+    for (int f = 0; f < 2 * bs->ndim; ++f) {
+      const int sdn = (f >> 1) == 0 ? 1 : ((f >> 1) == 1 ? 3 : 9);
+      if (k.nbr[13 + ((f & 1) ? sdn : -sdn)] < 0 && k.bc[f] != APH_BC_AMR) p->physical = true;
+    }
   p->nvar = bs->nvar;
   p->ndim = bs->ndim;
   p->nops = nops;
@@ -347,7 +366,10 @@ int aph_amr_plan_destroy(aph_context *ctx, aph_amr_plan *p) {
 int aph_amr_fill(aph_context *ctx, const aph_amr_plan *p, double *cons) {
   if (!ctx) return APH_E_INVALID;
   if (!p || !cons) return set_error(ctx, APH_E_INVALID, "aph_amr_fill: null");
-  return aph::amr_fill_epoch(ctx, p, cons, nullptr);
+  const int rc = aph::amr_fill_epoch(ctx, p, cons, nullptr);
+  if (rc != APH_OK || !p->physical) return rc;
+  // the edge and corner ghost cells beyond the domain copy the cross-level ghosts beside them
+  return aph::physical_bc(ctx, p->bs, cons);
 }
 
 int aph_amr_flux_correct(aph_context *ctx, const aph_amr_plan *p, double *f1, double *f2,

@@ -1,6 +1,7 @@
 // aph_amr_tree.hip -- the multilevel mesh tree on the host: from the leaf list of a block
-// set (each block's level and logical position in a periodic base lattice) to the op lists
-// that aph_amr_plan_create takes (cross-level ghost values and flux fixes).
+// set (each block's level and logical position in a base lattice whose axes are periodic
+// or end in physical faces) to the op lists that aph_amr_plan_create takes (cross-level
+// ghost values and flux fixes).
 //
 // The reference gets these from Parthenon's mesh tree (neighbour finding across levels for
 // the boundary exchange and the flux correction, apophis_driver.cpp:68,74-76,94).  Here the
@@ -10,7 +11,10 @@
 //   * over a coarser leaf: PROLONG from the coarse cell holding it, the slope stencil's
 //     neighbours taken at the coarse level (a coarse cell, or the mean of its children);
 // and every coarse face cell on a block face whose neighbour node is refined becomes one
-// flux fix (the mean of the 2^(ndim-1) fine faces across it).  Lists are emitted in block
+// flux fix (the mean of the 2^(ndim-1) fine faces across it).  Ghost cells beyond a
+// physical domain face get no op (the boundary conditions fill them); a slope neighbour
+// beyond one is the coarse level's boundary image: the coarse cell itself (outflow) or
+// its mirror image (reflect, flagged in the op's kind).  Lists are emitted in block
 // order, ghost cells k, j, i ascending, children (cz, cy, cx) with x fastest, transverse
 // axes ascending with the first slowest -- the order the Python statement of the same
 // builder (apophis_amd/amr.py amr_plan) uses, so both give identical lists.
@@ -50,9 +54,30 @@ inline uint64_t key(int lv, int64_t x, int64_t y, int64_t z) {
   return ((uint64_t)lv << 58) | ((uint64_t)x << 38) | ((uint64_t)y << 19) | (uint64_t)z;
 }
 
+// Domain face codes (-x1, +x1, -x2, ...): NULL or APH_BC_PERIODIC on both faces of an
+// axis make it periodic; otherwise each face is APH_BC_OUTFLOW or APH_BC_REFLECT.
+void face_codes(const int32_t *bc, int nd, int per[3], int code[6]) {
+  for (int d = 0; d < 3; ++d) {
+    per[d] = 1;
+    code[2 * d] = code[2 * d + 1] = APH_BC_PERIODIC;
+    if (!bc || d >= nd) continue;
+    const int lo = bc[2 * d], hi = bc[2 * d + 1];
+    if (lo == APH_BC_PERIODIC && hi == APH_BC_PERIODIC) continue;
+    if (lo == APH_BC_PERIODIC || hi == APH_BC_PERIODIC)
+      fail("axis %d: a periodic face needs a periodic opposite face", d);
+    for (int f : {lo, hi})
+      if (f != APH_BC_OUTFLOW && f != APH_BC_REFLECT)
+        fail("axis %d: face code %d is not periodic, outflow or reflect", d, f);
+    per[d] = 0;
+    code[2 * d] = lo;
+    code[2 * d + 1] = hi;
+  }
+}
+
 struct Tree {
   int nd = 0, ng = 0, nvar = 0, nb = 0;
   int nx[3] = {1, 1, 1}, nbase[3] = {1, 1, 1}, E[3] = {1, 1, 1};
+  int per[3] = {1, 1, 1}, code[6] = {0, 0, 0, 0, 0, 0};
   const aph_block *blk = nullptr;
   const int32_t *level = nullptr;
   const int64_t *loc = nullptr;
@@ -63,10 +88,18 @@ struct Tree {
   std::vector<aph_amr_ref> refs;
 
   int64_t nlat(int lv, int d) const { return d < nd ? (int64_t)nbase[d] << lv : 1; }
+  // global cell index g of axis d at level lv, wrapped on a periodic axis (elsewhere it
+  // stays as it is and may lie beyond the domain)
   int64_t wrap(int64_t g, int lv, int d) const {
     if (d >= nd) return 0;
+    if (!per[d]) return g;
     const int64_t n = nlat(lv, d) * nx[d];
     return ((g % n) + n) % n;
+  }
+  bool inside(int lv, const int64_t G[3]) const {
+    for (int d = 0; d < nd; ++d)
+      if (!per[d] && (G[d] < 0 || G[d] >= nlat(lv, d) * nx[d])) return false;
+    return true;
   }
   bool is_refined(int lv, const int64_t node[3]) const {
     return refined.count(key(lv, node[0], node[1], node[2])) != 0;
@@ -82,6 +115,7 @@ struct Tree {
       W[d] = wrap(G[d], lv, d);
       node[d] = d < nd ? W[d] / nx[d] : 0;
     }
+    if (!inside(lv, W)) return false;
     auto it = leaf.find(key(lv, node[0], node[1], node[2]));
     if (it == leaf.end()) return false;
     b = it->second;
@@ -106,6 +140,7 @@ struct Tree {
     }
     int64_t W[3];
     for (int d = 0; d < 3; ++d) W[d] = wrap(I[d], lv, d);
+    if (!inside(lv, W)) fail("cell of level %d beyond the domain", lv);
     int64_t kids[8];
     int nk = 0;
     for (int cz = 0; cz < (nd == 3 ? 2 : 1); ++cz)
@@ -123,9 +158,11 @@ struct Tree {
 };
 
 void setup(Tree &t, const aph_block *blocks, int32_t nblocks, int32_t ndim, int32_t nvar,
-           const int32_t nbase[3], const int32_t *level, const int64_t *loc) {
+           const int32_t nbase[3], const int32_t *level, const int64_t *loc,
+           const int32_t *bc) {
   if (!blocks || nblocks < 1 || ndim < 2 || ndim > 3 || nvar < 1 || !nbase || !level || !loc)
     fail("bad arguments");
+  face_codes(bc, ndim, t.per, t.code);
   t.nd = ndim;
   t.nvar = nvar;
   t.nb = nblocks;
@@ -190,6 +227,7 @@ void build(Tree &t, std::vector<aph_amr_op> &ops, std::vector<aph_flux_fix> &fix
             Gw[d] = d < nd ? t.wrap(lc[d] * t.nx[d] + l[d] - g, lv, d) : 0;
             node[d] = d < nd ? Gw[d] / t.nx[d] : 0;
           }
+          if (!t.inside(lv, Gw)) continue;  // beyond a physical face: boundary conditions
           aph_amr_op op;
           std::memset(&op, 0, sizeof(op));
           op.dst = t.cell(b, l);
@@ -208,11 +246,18 @@ void build(Tree &t, std::vector<aph_amr_op> &ops, std::vector<aph_flux_fix> &fix
             op.val[0] = t.value(lv - 1, I);
             for (int d = 0; d < nd; ++d) {
               op.sign[d] = Gw[d] % 2 == 0 ? -1 : 1;
-              int64_t lo[3] = {I[0], I[1], I[2]}, hi[3] = {I[0], I[1], I[2]};
-              lo[d] -= 1;
-              hi[d] += 1;
-              op.val[1 + 2 * d] = t.value(lv - 1, lo);
-              op.val[2 + 2 * d] = t.value(lv - 1, hi);
+              for (int side = 0; side < 2; ++side) {
+                int64_t J[3] = {I[0], I[1], I[2]};
+                J[d] += side ? 1 : -1;
+                const int slot = 1 + 2 * d + side;
+                if (t.inside(lv - 1, J)) {
+                  op.val[slot] = t.value(lv - 1, J);
+                } else {  // beyond a physical face: the coarse cell's boundary image
+                  op.val[slot] = t.value(lv - 1, I);
+                  if (t.code[2 * d + side] == APH_BC_REFLECT)
+                    op.kind |= 1 << (APH_AMR_MIRROR_BIT + slot - 1);
+                }
+              }
             }
           }
           ops.push_back(op);
@@ -272,9 +317,9 @@ void build(Tree &t, std::vector<aph_amr_op> &ops, std::vector<aph_flux_fix> &fix
 
 extern "C" int aph_amr_build(const aph_block *blocks, int32_t nblocks, int32_t ndim,
                              int32_t nvar, const int32_t nbase[3], const int32_t *level,
-                             const int64_t *loc, aph_amr_op *ops, int64_t *nops,
-                             aph_flux_fix *fixes, int64_t *nfixes, aph_amr_ref *refs,
-                             int64_t *nrefs, char *err, size_t err_len) {
+                             const int64_t *loc, const int32_t *bc, aph_amr_op *ops,
+                             int64_t *nops, aph_flux_fix *fixes, int64_t *nfixes,
+                             aph_amr_ref *refs, int64_t *nrefs, char *err, size_t err_len) {
   auto report = [&](int code, const char *msg) {
     if (err && err_len) snprintf(err, err_len, "aph_amr_build: %s", msg);
     return code;
@@ -282,7 +327,7 @@ extern "C" int aph_amr_build(const aph_block *blocks, int32_t nblocks, int32_t n
   if (!nops || !nfixes || !nrefs) return report(APH_E_INVALID, "null length");
   try {
     Tree t;
-    setup(t, blocks, nblocks, ndim, nvar, nbase, level, loc);
+    setup(t, blocks, nblocks, ndim, nvar, nbase, level, loc, bc);
     std::vector<aph_amr_op> o;
     std::vector<aph_flux_fix> f;
     build(t, o, f);
@@ -306,24 +351,33 @@ extern "C" int aph_amr_build(const aph_block *blocks, int32_t nblocks, int32_t n
 }
 
 // ------------------------------------------------------------------ the tree itself
-// Leaves are (level, logical position); the refined nodes are their ancestors.  Every axis
-// is periodic.  Children are ordered (cz, cy, cx) with x fastest, and the depth-first leaf
-// order walks the base lattice z, y, x -- the order of apophis_amd/amr.py AmrMesh.
+// Leaves are (level, logical position); the refined nodes are their ancestors.  An axis is
+// periodic unless the block table is built with physical faces on it.  Children are
+// ordered (cz, cy, cx) with x fastest, and the depth-first leaf order walks the base
+// lattice z, y, x -- the order of apophis_amd/amr.py AmrMesh.
 namespace {
 
 struct Forest {
   int nd = 0;
   int64_t nbase[3] = {1, 1, 1};
+  int per[3] = {1, 1, 1};
   std::unordered_set<uint64_t> refined;
 
   int64_t nlat(int lv, int d) const { return d < nd ? nbase[d] << lv : 1; }
   uint64_t k(int lv, const int64_t *c) const { return key(lv, c[0], c[1], c[2]); }
   bool is_ref(int lv, const int64_t *c) const { return refined.count(k(lv, c)) != 0; }
-  void shift(int lv, const int64_t *c, const int *o, int64_t *out) const {
+  // the node at offset o from c (wrapped on periodic axes); false beyond a physical face
+  bool shift(int lv, const int64_t *c, const int *o, int64_t *out) const {
+    bool in = true;
     for (int d = 0; d < 3; ++d) {
       const int64_t n = nlat(lv, d);
-      out[d] = ((c[d] + o[d]) % n + n) % n;
+      out[d] = c[d] + o[d];
+      if (per[d])
+        out[d] = (out[d] % n + n) % n;
+      else if (out[d] < 0 || out[d] >= n)
+        in = false;
     }
+    return in;
   }
   int nchildren() const { return nd == 3 ? 8 : 4; }
   void child(const int64_t *c, int i, int64_t *out) const {  // i in (cz, cy, cx) order
@@ -348,8 +402,7 @@ struct Forest {
     offsets([&](const int *o) {
       if (hit) return;
       int64_t p[3];
-      shift(lv, c, o, p);
-      if (!is_ref(lv, p)) return;
+      if (!shift(lv, c, o, p) || !is_ref(lv, p)) return;
       for (int i = 0; i < nchildren() && !hit; ++i) {
         int64_t ch[3];
         child(p, i, ch);
@@ -507,9 +560,9 @@ extern "C" int aph_amr_regrid_tree(int32_t ndim, const int32_t nbase[3], int32_t
 
 extern "C" int aph_amr_block_table(int32_t ndim, const int32_t nbase[3], const int32_t nx[3],
                                    int32_t ng, const double xmin[3], const double xmax[3],
-                                   int32_t mesh_id, int32_t n, const int32_t *level,
-                                   const int64_t *loc, aph_block *out, char *err,
-                                   size_t err_len) {
+                                   const int32_t *bc, int32_t mesh_id, int32_t n,
+                                   const int32_t *level, const int64_t *loc, aph_block *out,
+                                   char *err, size_t err_len) {
   auto report = [&](int code, const char *msg) {
     if (err && err_len) snprintf(err, err_len, "aph_amr_block_table: %s", msg);
     return code;
@@ -518,6 +571,8 @@ extern "C" int aph_amr_block_table(int32_t ndim, const int32_t nbase[3], const i
   try {
     Forest t;
     forest_setup(t, ndim, nbase, n, level, loc);
+    int code[6];
+    face_codes(bc, ndim, t.per, code);
     std::unordered_map<uint64_t, int> leaf;
     for (int b = 0; b < n; ++b) {
       const int64_t c[3] = {loc[3 * b], loc[3 * b + 1], ndim == 3 ? loc[3 * b + 2] : 0};
@@ -532,7 +587,7 @@ extern "C" int aph_amr_block_table(int32_t ndim, const int32_t nbase[3], const i
       // this leaf are leaves, or lies inside a leaf one level coarser
       t.offsets([&](const int *o) {
         int64_t p[3];
-        t.shift(lv, c, o, p);
+        if (!t.shift(lv, c, o, p)) return;  // beyond a physical face
         if (leaf.count(t.k(lv, p))) return;
         if (t.is_ref(lv, p)) {
           for (int i = 0; i < t.nchildren(); ++i) {
@@ -572,7 +627,7 @@ extern "C" int aph_amr_block_table(int32_t ndim, const int32_t nbase[3], const i
       for (int q = 0; q < 27; ++q) r.nbr[q] = -1;
       t.offsets([&](const int *o) {
         int64_t p[3];
-        t.shift(lv, c, o, p);
+        if (!t.shift(lv, c, o, p)) return;  // beyond a physical face: nbr stays -1
         auto it = leaf.find(t.k(lv, p));
         r.nbr[(o[2] + 1) * 9 + (o[1] + 1) * 3 + (o[0] + 1)] = it == leaf.end() ? -1 : it->second;
       });
@@ -581,7 +636,11 @@ extern "C" int aph_amr_block_table(int32_t ndim, const int32_t nbase[3], const i
         for (int side = 0; side < 2; ++side) {
           int o[3] = {0, 0, 0};
           o[d] = side ? 1 : -1;
-          if (r.nbr[(o[2] + 1) * 9 + (o[1] + 1) * 3 + (o[0] + 1)] < 0) r.bc[2 * d + side] = APH_BC_AMR;
+          int64_t p[3];
+          if (!t.shift(lv, c, o, p))
+            r.bc[2 * d + side] = code[2 * d + side];  // a physical domain face
+          else if (r.nbr[(o[2] + 1) * 9 + (o[1] + 1) * 3 + (o[0] + 1)] < 0)
+            r.bc[2 * d + side] = APH_BC_AMR;
         }
     }
   } catch (const BuildError &e) {

@@ -482,6 +482,33 @@ int aph_fill_ghosts(aph_context *ctx, const aph_blockset *bs, int32_t first_bloc
   return APH_OK;
 }
 
+}  // extern "C"
+
+namespace aph {
+
+// The boundary-condition passes of aph_fill_ghosts alone, over every block (a multilevel
+// mesh runs them again after its cross-level fill).
+int physical_bc(aph_context *ctx, const aph_blockset *bs, double *cons) {
+  if (!bs->nblocks) return APH_OK;
+  const int cnt = bs->nblocks, nd = bs->ndim;
+  int64_t slab = 0;
+  for (int d = 0; d < 3; ++d) slab = std::max<int64_t>(slab, (int64_t)bs->max_ext[0] *
+                                                             bs->max_ext[1] * bs->max_ext[2] /
+                                                             std::max(1, bs->max_ext[d]) * 8);
+  dim3 gs = grid_for(slab, kThreads, cnt, 64);
+  APH_NV_DISPATCH(bs->nvar, ({
+    k_bc<NV, 0><<<gs, kThreads, 0, ctx->stream>>>(bs->d, 0, nd, cons);
+    if (nd >= 2) k_bc<NV, 1><<<gs, kThreads, 0, ctx->stream>>>(bs->d, 0, nd, cons);
+    if (nd >= 3) k_bc<NV, 2><<<gs, kThreads, 0, ctx->stream>>>(bs->d, 0, nd, cons);
+  }));
+  APH_LAUNCH_CHECK(ctx);
+  return APH_OK;
+}
+
+}  // namespace aph
+
+extern "C" {
+
 int aph_estimate_timestep(aph_context *ctx, const aph_blockset *bs, int32_t first_block,
                           int32_t nblocks, const aph_eos *eos, const double *prim,
                           const double *cons, double cfl, double *dt_out) {

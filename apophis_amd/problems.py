@@ -142,14 +142,30 @@ def _default_rmax(xmin, xmax):
     return float(np.sqrt(edge[0] * edge[0] + edge[1] * edge[1] + edge[2] * edge[2]))
 
 
+def _static_mesh(ndim, nb, block, ng, xmin, xmax, region, level):
+    """An outflow-bounded multilevel mesh (amr.AmrMesh) of nb^ndim base blocks of block^ndim
+    cells with the static refinement block (region, level) of a deck."""
+    from .amr import AmrMesh, static_refinement
+
+    nbase = tuple(nb if d < ndim else 1 for d in range(3))
+    bnx = tuple(block if d < ndim else 1 for d in range(3))
+    bc = ["outflow"] * 6
+    lo, hi = (-region,) * 3, (region,) * 3
+    ref = static_refinement(ndim, nbase[:ndim], bnx[:ndim], xmin, xmax, [(lo, hi, level)], bc)
+    return AmrMesh(ndim, nbase[:ndim], bnx[:ndim], ng, xmin, xmax, ref, max_level=level,
+                   bc=bc)
+
+
 def grav_sphere(res=32, block=None, ndim=3, rho0=1.0, r0=0.25, nrbin=None, G=1.0,
                 gm1=0.666666667, recon="dc", riemann="hlle", cfl=0.4, center=None,
-                solver="monopole", tol=1e-8):
+                solver="monopole", tol=1e-8, smr_level=0):
     """The gravity accuracy test (grav_test.cpp:15-55, pid 1; inputs/grav_test.in as
     tst/.../gravity/gravity.py runs it): rho = rho0 (1 - r^2/r0^2)^2 inside r0 and 0
     outside (ConsToPrim's floor then applies), at rest, E = 1, on [-1/2, 1/2]^ndim with
     outflow boundaries.  solver "monopole": centred at the origin, nrbin = 4 res;
-    "poisson": centred at (0.01, 0, 0), the residual tolerance `tol`."""
+    "poisson": centred at (0.01, 0, 0), the residual tolerance `tol`.  smr_level > 0: the
+    deck's static refinement block ([-0.25, 0.25]^3 at that level, gravity.py's SMR cases
+    with res 16 and 8^3 blocks) on a multilevel mesh, kept in meta["mesh"]."""
     if center is None:
         center = (0.01, 0.0, 0.0) if solver == "poisson" else (0.0, 0.0, 0.0)
     block = block or res
@@ -157,8 +173,15 @@ def grav_sphere(res=32, block=None, ndim=3, rho0=1.0, r0=0.25, nrbin=None, G=1.0
     nblocks = tuple(nb if d < ndim else 1 for d in range(3))
     bnx = tuple(block if d < ndim else 1 for d in range(3))
     xmin, xmax = (-0.5,) * 3, (0.5,) * 3
-    m = lattice_mesh(ndim, nblocks, bnx, NEED_GHOST[recon], xmin, xmax, ["outflow"] * 6)
-    bs = BlockSet(ndim, [m])
+    amesh = None
+    if smr_level:
+        if solver != "monopole":
+            raise ValueError("Poisson gravity on refined meshes is not built")
+        amesh = _static_mesh(ndim, nb, block, NEED_GHOST[recon], xmin, xmax, 0.25, smr_level)
+        bs = amesh.block_set()
+    else:
+        m = lattice_mesh(ndim, nblocks, bnx, NEED_GHOST[recon], xmin, xmax, ["outflow"] * 6)
+        bs = BlockSet(ndim, [m])
     cons = np.zeros(bs.field_size(bs.nvar))
     for b in range(bs.nblocks):
         x, y, z = bs.cell_centers(b)
@@ -172,23 +195,32 @@ def grav_sphere(res=32, block=None, ndim=3, rho0=1.0, r0=0.25, nrbin=None, G=1.0
     return Problem("grav_sphere", bs, cons, make_eos(gm1), recon, riemann, cfl,
                    integrator="rk1", **grav,
                    meta={"rho0": rho0, "r0": r0, "res": res, "box": (xmin, xmax),
-                         "center": center})
+                         "center": center, "mesh": amesh})
 
 
 def evrard(res=32, block=16, mass=1.0, radius=1.0, u_therm=0.05, nrbin=None, G=1.0,
            gm1=0.666666667, recon="plm", riemann="hllc", cfl=0.4, tlim=0.8,
-           solver="monopole", tol=1e-8):
+           solver="monopole", tol=1e-8, smr_level=0):
     """Evrard's collapse (evrards_collapse.cpp:15-49, inputs/evrards_collapse.in) with the
     monopole solver as tst/.../evrards_collapse/evrards_collapse.py runs it: rho =
     M / (2 pi R^2 r) inside R and 1e-5 outside, at rest, E = u_therm rho, on [-1, 1]^3
     (res^3 cells in blocks of block^3), outflow, PLM + HLLC, RK2, CFL 0.4, to t = 0.8;
     solver "monopole" (nrbin = 4 res) or "poisson" (the deck's own, residual tolerance
-    `tol`)."""
+    `tol`).  smr_level > 0: the deck's static refinement block ([-0.5, 0.5]^3 at that
+    level; the test's monopole_smr_32 is res 32, block 16, level 2) on a multilevel mesh,
+    kept in meta["mesh"]."""
     nb = res // block
     xmin, xmax = (-1.0,) * 3, (1.0,) * 3
-    m = lattice_mesh(3, (nb,) * 3, (block,) * 3, NEED_GHOST[recon], xmin, xmax,
-                     ["outflow"] * 6)
-    bs = BlockSet(3, [m])
+    amesh = None
+    if smr_level:
+        if solver != "monopole":
+            raise ValueError("Poisson gravity on refined meshes is not built")
+        amesh = _static_mesh(3, nb, block, NEED_GHOST[recon], xmin, xmax, 0.5, smr_level)
+        bs = amesh.block_set()
+    else:
+        m = lattice_mesh(3, (nb,) * 3, (block,) * 3, NEED_GHOST[recon], xmin, xmax,
+                         ["outflow"] * 6)
+        bs = BlockSet(3, [m])
     cons = np.zeros(bs.field_size(bs.nvar))
     for b in range(bs.nblocks):
         x, y, z = bs.cell_centers(b)
@@ -198,7 +230,8 @@ def evrard(res=32, block=16, mass=1.0, radius=1.0, u_therm=0.05, nrbin=None, G=1
     grav = {"monopole": (nrbin or 4 * res, _default_rmax(xmin, xmax), G)} \
         if solver == "monopole" else {"poisson": (G, 0.0, tol, 10000)}
     return Problem("evrard", bs, cons, make_eos(gm1), recon, riemann, cfl, tlim=tlim,
-                   **grav, meta={"res": res, "radius": radius, "box": (xmin, xmax)})
+                   **grav, meta={"res": res, "radius": radius, "box": (xmin, xmax),
+                                 "mesh": amesh})
 
 
 # ---------------------------------------------------------------- uneven blocks

@@ -59,8 +59,10 @@ enum aph_integrator { APH_INTEGRATOR_RK1 = 1, APH_INTEGRATOR_RK2 = 2, APH_INTEGR
 /* Physical boundary applied on a face whose neighbour entry is -1.
    Periodicity is expressed through the neighbour table instead. */
 /* APH_BC_AMR: the region beyond this face belongs to blocks of another refinement level;
-   aph_fill_ghosts leaves its ghost cells to aph_amr_fill (multilevel meshes below). */
-enum aph_bc { APH_BC_OUTFLOW = 1, APH_BC_REFLECT = 2, APH_BC_AMR = 3 };
+   aph_fill_ghosts leaves its ghost cells to aph_amr_fill (multilevel meshes below).
+   APH_BC_PERIODIC names a periodic domain face where a mesh's six face codes are given
+   (the multilevel tree below); a block's own bc[] never holds it. */
+enum aph_bc { APH_BC_PERIODIC = 0, APH_BC_OUTFLOW = 1, APH_BC_REFLECT = 2, APH_BC_AMR = 3 };
 
 /* One meshblock.  240 bytes, no implicit padding (numpy mirrors it field by field). */
 typedef struct aph_block {
@@ -278,13 +280,21 @@ typedef struct aph_amr_val {
   int32_t count;
 } aph_amr_val;
 enum aph_amr_kind { APH_AMR_MEAN = 0, APH_AMR_PROLONG = 1 };
+/* kind's low byte is an aph_amr_kind; bit APH_AMR_MIRROR_BIT + k - 1 (k = 1..6) marks value
+   slot k of a PROLONG op as the mirror image of a coarse cell across a reflecting domain
+   face normal to its axis d = (k - 1) / 2: its momentum component 1 + d is negated. */
+#define APH_AMR_MIRROR_BIT 8
 /* One ghost cell of cons (all components):
  *   APH_AMR_MEAN:    value(val[0]);
  *   APH_AMR_PROLONG: c = value(val[0]); for each active axis d in order,
  *                    g_d = minmod(c - value(val[1+2d]), value(val[2+2d]) - c),
  *                    minmod(a, b) = (SIGN(a) + SIGN(b))/2 * min(|a|, |b|), SIGN(0) = 1;
  *                    result = (((c + sign[0]/4 g_0) + sign[1]/4 g_1) + sign[2]/4 g_2),
- *                    sign[d] = -1 / +1 for the lower / upper child of the coarse cell. */
+ *                    sign[d] = -1 / +1 for the lower / upper child of the coarse cell.
+ *                    A slope neighbour beyond a physical domain face is the coarse level's
+ *                    boundary image (Parthenon applies the boundary conditions to the
+ *                    coarse buffer before it prolongates): outflow repeats the coarse cell,
+ *                    reflect mirrors it (the mirror bit above). */
 typedef struct aph_amr_op {
   int64_t dst;        /* component-0 element offset of the ghost cell   */
   int64_t dst_stride; /* its block's ncell                               */
@@ -310,20 +320,24 @@ int aph_amr_plan_create(aph_context *ctx, const aph_blockset *bs, const aph_amr_
                         int64_t nops, const aph_flux_fix *fixes, int64_t nfixes,
                         const aph_amr_ref *refs, int64_t nrefs, aph_amr_plan **out);
 int aph_amr_plan_destroy(aph_context *ctx, aph_amr_plan *plan);
-/* The op lists of a periodic multilevel mesh from its leaves (the Parthenon mesh tree's
-   neighbour finding across levels; host only, no context or device needed).  blocks: the
-   block set's table (every block the same nx, even, and ng; nbr / bc as described above);
+/* The op lists of a multilevel mesh from its leaves (the Parthenon mesh tree's neighbour
+   finding across levels; host only, no context or device needed).  blocks: the block
+   set's table (every block the same nx, even, and ng; nbr / bc as described above);
    level[b] and loc[3*b .. 3*b+2]: each block's level and logical position at its level in
-   a base lattice of nbase[0..ndim-1] blocks; the tree must be 2:1 balanced.  Ops: one per
-   ghost cell with no same-level neighbour in its direction (MEAN over a refined node,
-   PROLONG from a coarser leaf); fixes: one per coarse face cell on a face to a refined
-   node; in block order, ghost cells k, j, i ascending, children (cz, cy, cx) x fastest.
-   On input *nops, *nfixes, *nrefs are the capacities of the arrays (an array may be NULL:
-   then only its length is returned); on output the lengths.  On failure the message goes
-   to err (err_len bytes, may be NULL). */
-/* The mesh tree (host only, like aph_amr_build).  A periodic tree is given by its leaves:
+   a base lattice of nbase[0..ndim-1] blocks; the tree must be 2:1 balanced.  bc: the six
+   domain face codes (-x1, +x1, -x2, ...; APH_BC_PERIODIC on both faces of a periodic
+   axis, else APH_BC_OUTFLOW or APH_BC_REFLECT), NULL: every axis periodic.  Ops: one per
+   ghost cell inside the domain with no same-level neighbour in its direction (MEAN over a
+   refined node, PROLONG from a coarser leaf); ghost cells beyond a physical domain face
+   are left to the boundary conditions.  Fixes: one per coarse face cell on a face to a
+   refined node; in block order, ghost cells k, j, i ascending, children (cz, cy, cx) x
+   fastest.  On input *nops, *nfixes, *nrefs are the capacities of the arrays (an array may
+   be NULL: then only its length is returned); on output the lengths.  On failure the
+   message goes to err (err_len bytes, may be NULL). */
+/* The mesh tree (host only, like aph_amr_build).  A tree is given by its leaves:
    level[b] and loc[3*b .. 3*b+2] in a base lattice of nbase[0..ndim-1] nodes; the leaves
-   must cover the domain exactly once.
+   must cover the domain exactly once.  aph_amr_regrid_tree and aph_amr_remap take every
+   axis periodic (the adaptive path); aph_amr_block_table takes the domain face codes.
    aph_amr_regrid_tree: one regrid from per-leaf tags (-1 / 0 / 1), Parthenon's rules --
    leaves tagged 1 below max_level split; every leaf touching a leaf two levels finer splits
    (2:1 balance); sibling groups all tagged -1 (none split) merge, finest level first, then
@@ -332,16 +346,18 @@ int aph_amr_plan_destroy(aph_context *ctx, aph_amr_plan *plan);
    out_level / out_loc (cap entries; NULL: only *out_n is set).
    aph_amr_block_table: the aph_block table of the leaves (in their order) after checking
    2:1 balance: extents, ghosts, dx = (xmax - xmin) / (nbase nx 2^level) per active axis,
-   areas, volume, xmin, nbr to same-level leaves (periodic), bc = APH_BC_AMR on faces
-   without one (APH_BC_OUTFLOW elsewhere), cell_base 0 (the block set fills it). */
+   areas, volume, xmin, nbr to same-level leaves (wrapping on periodic axes only), bc =
+   the domain code on faces on a physical domain face, APH_BC_AMR on other faces without a
+   same-level neighbour (APH_BC_OUTFLOW elsewhere), cell_base 0 (the block set fills it).
+   bc: the six domain face codes as for aph_amr_build (NULL: periodic). */
 int aph_amr_regrid_tree(int32_t ndim, const int32_t nbase[3], int32_t max_level, int32_t n,
                         const int32_t *level, const int64_t *loc, const int32_t *tags,
                         int32_t *out_level, int64_t *out_loc, int32_t *out_n, int32_t cap,
                         char *err, size_t err_len);
 int aph_amr_block_table(int32_t ndim, const int32_t nbase[3], const int32_t nx[3], int32_t ng,
-                        const double xmin[3], const double xmax[3], int32_t mesh_id, int32_t n,
-                        const int32_t *level, const int64_t *loc, aph_block *out, char *err,
-                        size_t err_len);
+                        const double xmin[3], const double xmax[3], const int32_t *bc,
+                        int32_t mesh_id, int32_t n, const int32_t *level, const int64_t *loc,
+                        aph_block *out, char *err, size_t err_len);
 /* The interiors of a regridded mesh (host arrays in the pack layout, every block nx^ndim
    cells with ng ghosts, nvar components, blocks in leaf order): a kept leaf copies its
    cells; any other new leaf takes its box of the old mesh's level values -- a leaf's cells,
@@ -355,10 +371,15 @@ int aph_amr_remap(int32_t ndim, const int32_t nbase[3], const int32_t nx[3], int
                   const int64_t *new_loc, double *new_u, char *err, size_t err_len);
 int aph_amr_build(const aph_block *blocks, int32_t nblocks, int32_t ndim, int32_t nvar,
                   const int32_t nbase[3], const int32_t *level, const int64_t *loc,
-                  aph_amr_op *ops, int64_t *nops, aph_flux_fix *fixes, int64_t *nfixes,
-                  aph_amr_ref *refs, int64_t *nrefs, char *err, size_t err_len);
-/* Cross-level ghost values of cons; run after aph_fill_ghosts.  Reads interior cells
-   only, writes ghost cells only. */
+                  const int32_t *bc, aph_amr_op *ops, int64_t *nops, aph_flux_fix *fixes,
+                  int64_t *nfixes, aph_amr_ref *refs, int64_t *nrefs, char *err,
+                  size_t err_len);
+/* Cross-level ghost values of cons; run after aph_fill_ghosts.  The ops read interior
+   cells only and write ghost cells only.  On a mesh with physical domain faces the
+   boundary conditions of aph_fill_ghosts then run again over every block on one, so that
+   the edge and corner ghost cells beyond the domain take their values from the cross-level
+   ghost cells beside them (Parthenon fills the neighbour regions first and the physical
+   boundaries last).  The block set given to aph_amr_plan_create must outlive the plan. */
 int aph_amr_fill(aph_context *ctx, const aph_amr_plan *plan, double *cons);
 /* Flux correction before the update: every coarse face shared with finer blocks takes
    the mean of the fine fluxes across it.  flux_x2 / flux_x3 may be NULL when unused. */

@@ -17,6 +17,15 @@ linear fields, conservation, uniform states and an op-by-op cross-check of the h
 This module works from whole-level arrays (every cell of the domain at each level, the
 refined ones holding the restriction of their children), not from the plan's op lists.
 
+Physical domain faces (outflow, reflect): ghost cells beyond the domain are left to the
+boundary conditions, which run again after the cross-level values (Parthenon fills the
+neighbour regions first and the physical boundaries last, so an edge or corner ghost beyond
+the domain copies the cross-level ghost beside it); a prolongation's slope neighbour beyond
+the domain is the coarse level's boundary image (Parthenon applies the boundary conditions
+to the coarse buffer before it prolongates): the coarse cell itself for outflow, its mirror
+image with the normal momentum negated for reflect.  tests/test_amr_physical.py pins these
+by uniform and mirror-symmetric states and by linear fields away from the domain faces.
+
 refinement_tag restates the reference's own refinement criteria
 (src/refinement/gradient.cpp, other.cpp) with their index ranges; tests/test_refinement.py
 pins it by closed forms (linear pressure and velocity fields, uniform states) and the
@@ -28,7 +37,7 @@ import numpy as np
 
 from . import oracle as orc
 
-BC_AMR = 3
+BC_REFLECT, BC_AMR = 2, 3
 
 
 def _sgn(x):
@@ -94,10 +103,14 @@ def amr_fill(mesh, bs, u):
             o[d] = np.where(loc[d] < g, -1, np.where(loc[d] >= g + nx[d], 1, 0))
         ghost = (o[0] != 0) | (o[1] != 0) | (o[2] != 0)
         code = (o[2] + 1) * 9 + (o[1] + 1) * 3 + (o[0] + 1)
+        NL = [mesh.ncells_axis(lv, d) for d in range(3)]
         sel = ghost & (nbr[code] < 0)
+        for d in range(nd):  # beyond a physical face: the boundary conditions fill it
+            if not mesh.periodic[d]:
+                G = int(lc[d]) * nx[d] + loc[d] - g
+                sel &= (G >= 0) & (G < NL[d])
         if not sel.any():
             continue
-        NL = [mesh.ncells_axis(lv, d) for d in range(3)]
         Gw = [(int(lc[d]) * nx[d] + loc[d][sel] - g) % NL[d] if d < nd
               else np.zeros(sel.sum(), int) for d in range(3)]
         view = bs.block_view(u, b, nv)
@@ -114,10 +127,21 @@ def amr_fill(mesh, bs, u):
             at = lambda J: C[(slice(None),) + tuple(J[d] % NC[d] for d in (2, 1, 0))]
             c = at(I)
             pv = c.copy()
+            def nb_val(d, side):
+                """The slope neighbour of I along d; beyond a physical face, the coarse
+                cell's boundary image."""
+                J = [I[t] + ((1 if side else -1) if t == d else 0) for t in range(3)]
+                if mesh.periodic[d]:
+                    return at(J)
+                out = (J[d] < 0) | (J[d] >= NC[d])
+                J[d] = np.where(out, I[d], J[d])
+                v = at(J)
+                if mesh.bc[2 * d + side] == BC_REFLECT:
+                    v[1 + d] = np.where(out, -v[1 + d], v[1 + d])
+                return v
+
             for d in range(nd):
-                lo = [I[t] - (1 if t == d else 0) for t in range(3)]
-                hi = [I[t] + (1 if t == d else 0) for t in range(3)]
-                gd = _minmod(c - at(lo), at(hi) - c)
+                gd = _minmod(c - nb_val(d, 0), nb_val(d, 1) - c)
                 sign = np.where(Gc[d] % 2 == 0, -1.0, 1.0)
                 pv = pv + (sign * 0.25) * gd
             val[:, coarser] = pv
@@ -200,8 +224,9 @@ def flux_correct(mesh, bs, fx):
 
 class AmrState:
     """The oracle State of a problem on a multilevel mesh: the stage sequence of the plain
-    driver (fluxes, flux correction, update, ghost exchange + cross-level fill,
-    ConsToPrim) composed from the oracle's task functions."""
+    driver (monopole gravity when the problem has it, fluxes, flux correction, update,
+    gravity source, ghost exchange + cross-level fill + boundary conditions, ConsToPrim)
+    composed from the oracle's task functions, in the order orc_step_monopole runs them."""
 
     def __init__(self, prob, mesh, flux_correction=True):
         self.prob, self.mesh, self.bs = prob, mesh, prob.bs
@@ -211,12 +236,26 @@ class AmrState:
         self.prim = np.zeros_like(self.u0)
         self.gamma = np.zeros(self.bs.field_size(2))
         self.fx = [np.zeros_like(self.u0) for _ in range(3)]
+        self.mono = getattr(prob, "monopole", None)  # (nrbin, rmax, G) or None
+        if self.mono is not None:
+            self.grav = np.zeros(self.bs.field_size(3))
+        self.g = tuple(float(v) for v in getattr(prob, "g", (0.0, 0.0, 0.0)))
         self._ghosts()
 
     def _ghosts(self):
         orc.fill_ghosts(self.bs, self.u0)
         amr_fill(self.mesh, self.bs, self.u0)
+        if not all(self.mesh.periodic):  # corners beyond the domain: after the fill
+            orc.fill_ghosts(self.bs, self.u0)
         orc.cons_to_prim(self.bs, self.prob.eos, self.u0, self.prim, self.gamma)
+
+    def gravity_field(self):
+        """The monopole field of the current primitives (bins, tables, per-cell field)."""
+        nrbin, rmax, G = self.mono
+        mbin, _ = orc.mono_bin_masses(self.bs, self.prim, nrbin, rmax)
+        m_enc, phi_rad = orc.mono_tables(nrbin, self.bs.ndim, rmax, G, mbin)
+        self.grav, self.phi = orc.mono_field(self.bs, nrbin, rmax, G, m_enc, phi_rad)
+        return self.grav
 
     def estimate_dt(self):
         return orc.estimate_timestep(self.bs, self.prob.eos, self.prim, self.u0,
@@ -228,10 +267,16 @@ class AmrState:
         self.u1[:] = self.u0
         for s in range(len(g0)):
             recon = "dc" if (p.integrator == "vl2" and s == 0) else p.recon
+            if self.mono is not None:
+                self.gravity_field()
             orc.calculate_fluxes(bs, recon, p.riemann, self.prim, self.gamma, *self.fx)
             if self.correct:
                 flux_correct(self.mesh, bs, self.fx)
             orc.update(bs, self.u0, self.u1, *self.fx, g0[s], g1[s], be[s] * dt)
+            if self.mono is not None:
+                orc.apply_gravity_field(bs, self.u0, self.prim, self.grav, be[s] * dt)
+            elif any(v != 0.0 for v in self.g):
+                orc.apply_gravity(bs, self.u0, self.prim, np.array(self.g), be[s] * dt)
             self._ghosts()
         return self.estimate_dt()
 

@@ -56,13 +56,21 @@ def centres(bs):
 
 def sphere_errors(bs, grav, phi, rho0=1.0, r0=0.25, G=1.0):
     """(phi L2, g_x L2) of a gravity field against the sphere's analytic solution."""
+    return sphere_errors_max(bs, grav, phi, rho0, r0, G)[:2]
+
+
+def sphere_errors_max(bs, grav, phi, rho0=1.0, r0=0.25, G=1.0):
+    """(phi L2, g_x L2, phi max, g_x max) of a gravity field against the sphere's analytic
+    solution: gravity.py's l2_norm (the mean over every cell of every block, unweighted)
+    and max |error| (its REFINED_LIMITS quantities, gravity.py:48-53, 348-372)."""
     x, y, z = centres(bs)
     r = np.sqrt(x * x + y * y + z * z)
     ph = interior(bs, phi, 1, 0) - analytic_phi(r, rho0, r0, G)
     gr = analytic_g(r, rho0, r0, G)
     gx_exact = np.where(r > 0.0, gr * x / np.where(r > 0.0, r, 1.0), 0.0)
     gx = interior(bs, grav, 3, 0) - gx_exact
-    return float(np.sqrt(np.mean(ph ** 2))), float(np.sqrt(np.mean(gx ** 2)))
+    return (float(np.sqrt(np.mean(ph ** 2))), float(np.sqrt(np.mean(gx ** 2))),
+            float(np.max(np.abs(ph))), float(np.max(np.abs(gx))))
 
 
 def evrard_exact(gamma=5.0 / 3.0):

@@ -326,7 +326,7 @@ def test_native_tree_builds_the_same_lists(name):
     want = amr.amr_plan(m, bs)
     # no length hint (a counting build first), a hint too short (a second build at the
     # reported lengths) and the hint the last build left (one build)
-    shape = (m.ndim, bs.nvar, tuple(int(v) for v in m.nx), int(bs.blocks["ng"][0]))
+    shape = (m.ndim, bs.nvar, tuple(int(v) for v in m.nx), int(bs.blocks["ng"][0]), m.bc)
     for hint in (None, [0, 0, 0], "kept"):
         if hint is None:
             amr._PLAN_HINT.pop(shape, None)

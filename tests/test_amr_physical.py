@@ -1,0 +1,343 @@
+"""Multilevel meshes with physical domain faces (outflow, reflect) and static refinement:
+the host plan (apophis_amd/amr.py, the library's tree) and the oracle's restatement
+(oracle/amr_oracle.py).
+
+The reference takes the behaviour from Parthenon (not in the reference tree): neighbour
+regions are filled first (same level, restriction, prolongation), the physical boundary
+conditions last; the coarse buffer a prolongation reads gets the boundary conditions
+first, so a slope neighbour beyond the domain is the coarse cell's boundary image.  What
+pins the restatement here, beside plan == oracle:
+* a field that varies along the wall normal only: every prolongated ghost whose coarse
+  stencil reaches an outflow wall takes no slope into the wall (its value is the coarse
+  cell, whatever the profile) -- a wrap across the wall or a clamp to the wrong cell
+  would give a slope;
+* a normal momentum that is linear through zero at a reflecting wall: its mirror image is
+  the linear continuation, so those ghosts are exact -- without the sign flip they are not;
+* a state at rest and uniform is a fixed point of a step (outflow and reflect), bitwise;
+* a closed reflecting box keeps mass and energy to round-off through the level jumps;
+* the deck's static refinement blocks give the trees Parthenon's rule gives.
+"""
+import numpy as np
+import pytest
+
+from apophis_amd import amr, problems
+from apophis_amd import _lib
+from apophis_amd.mesh import make_eos
+from oracle import amr_oracle as ao
+from oracle import oracle as orc
+
+OUT6, REF6 = ["outflow"] * 6, ["reflect"] * 6
+
+MESHES = {
+    # a fine region touching the lower x wall, so fine ghosts beside coarse blocks reach it
+    "2d_out": dict(ndim=2, nbase=(4, 4), nx=(8, 8), ng=2, bc=OUT6,
+                   nodes=[(0, (0, 1)), (0, (0, 2)), (0, (1, 1))]),
+    "2d_ref": dict(ndim=2, nbase=(4, 4), nx=(8, 8), ng=3, bc=REF6,
+                   nodes=[(0, (0, 1)), (0, (0, 2)), (0, (1, 1)), (0, (3, 3))]),
+    "2d_ppm_mixed": dict(ndim=2, nbase=(4, 3), nx=(6, 6), ng=3,
+                         bc=["reflect", "outflow", "periodic", "periodic", "outflow", "outflow"],
+                         nodes=[(0, (0, 0)), (0, (0, 2)), (0, (3, 1))]),
+    "2d_3lev_ref": dict(ndim=2, nbase=(4, 4), nx=(8, 8), ng=2, bc=REF6,
+                        nodes=[(0, (0, 0)), (0, (1, 0)), (0, (0, 1)), (0, (1, 1)), (1, (0, 0)),
+                               (1, (1, 1))]),
+    "3d_out": dict(ndim=3, nbase=(3, 3, 2), nx=(6, 6, 6), ng=2, bc=OUT6,
+                   nodes=[(0, (0, 1, 0)), (0, (1, 1, 1))]),
+    "3d_ref": dict(ndim=3, nbase=(2, 2, 2), nx=(6, 6, 6), ng=3, bc=REF6,
+                   nodes=[(0, (0, 0, 0))]),
+}
+
+
+def _mesh(name):
+    k = MESHES[name]
+    return amr.AmrMesh(k["ndim"], k["nbase"], k["nx"], k["ng"], (0.0, 0.0, 0.0),
+                       (1.0, 1.0, 1.0), k["nodes"], max_level=2, bc=k["bc"])
+
+
+def _eval_plan(ops, refs, nd, u, nv):
+    """The plan op by op (include/apophis_hydro.h, aph_amr_op), mirror bits included."""
+    def val(v, comp):
+        r = refs[v["first"]: v["first"] + v["count"]]
+        s = u[r["offset"][0] + comp * r["stride"][0]]
+        for k in range(1, len(r)):
+            s = s + u[r["offset"][k] + comp * r["stride"][k]]
+        return s if len(r) == 1 else s * (1.0 / len(r))
+
+    sg = lambda x: -1.0 if x < 0 else 1.0
+    out = []
+    for op in ops:
+        kind, mirror = int(op["kind"]) & 0xFF, int(op["kind"]) >> amr.AMR_MIRROR_BIT
+        for c in range(nv):
+            v0 = val(op["val"][0], c)
+            o = v0
+            if kind == amr.AMR_PROLONG:
+                for d in range(nd):
+                    lo, hi = val(op["val"][1 + 2 * d], c), val(op["val"][2 + 2 * d], c)
+                    if c == 1 + d:
+                        lo = -lo if mirror & (1 << (2 * d)) else lo
+                        hi = -hi if mirror & (2 << (2 * d)) else hi
+                    a, b = v0 - lo, hi - v0
+                    o = o + (op["sign"][d] * 0.25) * ((0.5 * (sg(a) + sg(b))) * min(abs(a), abs(b)))
+            out.append((op["dst"] + c * op["dst_stride"], o))
+    for i, o in out:
+        u[i] = o
+
+
+@pytest.mark.parametrize("name", list(MESHES))
+def test_native_table_and_plan_equal_python(name):
+    m = _mesh(name)
+    assert amr.block_table_py(m).tobytes() == m.blocks.tobytes()
+    bs = m.block_set()
+    a = amr.amr_plan(m, bs)
+    b = amr.native_plan(m, bs)
+    for x, y in zip(a, b):
+        assert x.tobytes() == y.tobytes()
+    # physical faces carry the domain code and no neighbour
+    codes = {"outflow": 1, "reflect": 2}
+    for blk in range(bs.nblocks):
+        lv, lc = int(m.level[blk]), m.loc[blk]
+        for d in range(m.ndim):
+            for side in (0, 1):
+                n = m.nlat(lv, d)
+                wall = (lc[d] == 0 and side == 0) or (lc[d] == n - 1 and side == 1)
+                name_ = MESHES[name]["bc"][2 * d + side]
+                if wall and name_ != "periodic":
+                    assert m.blocks["bc"][blk][2 * d + side] == codes[name_]
+
+
+@pytest.mark.parametrize("name", list(MESHES))
+def test_plan_equals_oracle_fill(name):
+    m = _mesh(name)
+    bs = m.block_set()
+    u = np.random.default_rng(5).normal(size=bs.field_size(5))
+    a, b = u.copy(), u.copy()
+    orc.fill_ghosts(bs, a)
+    ao.amr_fill(m, bs, a)
+    orc.fill_ghosts(bs, b)
+    ops, _, refs = amr.amr_plan(m, bs)
+    _eval_plan(ops, refs, m.ndim, b, 5)
+    assert np.array_equal(a, b)
+    if "ref" in name:  # some op reads a mirror image
+        assert (ops["kind"] >> amr.AMR_MIRROR_BIT).any()
+    # no op writes a ghost cell beyond the domain; the boundary conditions do
+    ext = bs.extents()
+    for op in ops:
+        blk = int(np.searchsorted(5 * bs.blocks["cell_base"], op["dst"], side="right")) - 1
+        c = int(op["dst"] - 5 * bs.blocks["cell_base"][blk])
+        loc = (c % ext[blk][0], (c // ext[blk][0]) % ext[blk][1], c // (ext[blk][0] * ext[blk][1]))
+        lv = int(m.level[blk])
+        for d in range(m.ndim):
+            if not m.periodic[d]:
+                G = int(m.loc[blk][d]) * m.nx[d] + loc[d] - m.ng
+                assert 0 <= G < m.ncells_axis(lv, d)
+
+
+def _filled(m, field):
+    bs = m.block_set()
+    u = np.zeros(bs.field_size(5))
+    for b in range(bs.nblocks):
+        x, y, z = bs.cell_centers(b)
+        v = bs.interior(u, b, 5)
+        for n in range(5):
+            v[n] = field(n, x, y, z)
+    orc.fill_ghosts(bs, u)
+    ao.amr_fill(m, bs, u)
+    return bs, u
+
+
+def _ghost_centers(bs, m, op):
+    blk = int(np.searchsorted(5 * bs.blocks["cell_base"], op["dst"], side="right")) - 1
+    ext = bs.extents()[blk]
+    c = int(op["dst"] - 5 * bs.blocks["cell_base"][blk])
+    loc = (c % ext[0], (c // ext[0]) % ext[1], c // (ext[0] * ext[1]))
+    r = bs.blocks[blk]
+    return [r["xmin"][d] + (loc[d] - m.ng + 0.5) * r["dx"][d] if d < m.ndim else 0.0
+            for d in range(3)]
+
+
+@pytest.mark.parametrize("name", ["2d_out", "3d_out"])
+def test_outflow_wall_gives_no_slope_into_the_wall(name):
+    m = _mesh(name)
+    rng = np.random.default_rng(9)
+    prof = rng.normal(size=(5, 7))  # a random smooth profile along x only
+    field = lambda n, x, y, z: sum(prof[n, k] * np.cos((k + 1) * 1.7 * x + k) for k in range(7))
+    bs, u = _filled(m, field)
+    ops, _, refs = amr.amr_plan(m, bs)
+    # the plausible mistake: wrapping across the wall as on a periodic mesh
+    k = MESHES[name]
+    wrapped = amr.AmrMesh(k["ndim"], k["nbase"], k["nx"], k["ng"], (0.0, 0.0, 0.0),
+                          (1.0, 1.0, 1.0), k["nodes"], max_level=2)
+    opsw, _, refsw = amr.amr_plan(wrapped, wrapped.block_set())
+    uw = u.copy()
+    _eval_plan(opsw, refsw, m.ndim, uw, 5)
+    hit = differ = 0
+    for op in ops:
+        if int(op["kind"]) & 0xFF != amr.AMR_PROLONG:
+            continue
+        x = _ghost_centers(bs, m, op)
+        # the coarse cell holding the ghost, at the lower x wall (coarse dx = 2 fine dx)
+        blk = int(np.searchsorted(5 * bs.blocks["cell_base"], op["dst"], side="right")) - 1
+        cdx = 2.0 * bs.blocks["dx"][blk][0]
+        if x[0] >= cdx:
+            continue
+        cnt = op["val"]["count"]
+        if cnt[0] != 1 or any(cnt[1 + 2 * d] != 1 and cnt[2 + 2 * d] != 1
+                              for d in range(1, m.ndim)):
+            # a transverse slope is 0 when one transverse neighbour is a coarse cell at
+            # the same x (a difference of exactly 0); skip ghosts with means on both sides
+            continue
+        hit += 1
+        r = refs[op["val"][0]["first"]]
+        for n in range(5):
+            # no x slope at the wall, and no y / z slope for a field constant in y and z
+            c = u[r["offset"] + n * r["stride"]]
+            assert u[op["dst"] + n * op["dst_stride"]] == c
+            differ += uw[op["dst"] + n * op["dst_stride"]] != c
+    assert hit > 0 and differ > 0
+
+
+@pytest.mark.parametrize("name", ["2d_ref", "3d_ref"])
+def test_reflect_mirror_continues_the_normal_momentum_linearly(name):
+    m = _mesh(name)
+    k = 0.7
+    # m_x odd about the lower x wall (x = 0): its mirror image is its linear continuation
+    field = lambda n, x, y, z: (k * x if n == 1 else (2.0 + 0.0 * x if n in (0, 4) else 0.0 * x))
+    bs, u = _filled(m, field)
+    ops, _, refs = amr.amr_plan(m, bs)
+    hit = 0
+    for op in ops:
+        kind, mirror = int(op["kind"]) & 0xFF, int(op["kind"]) >> amr.AMR_MIRROR_BIT
+        if kind != amr.AMR_PROLONG or not mirror & 1:
+            continue
+        hit += 1
+        x = _ghost_centers(bs, m, op)
+        assert u[op["dst"] + 1 * op["dst_stride"]] == pytest.approx(k * x[0], rel=1e-14,
+                                                                   abs=1e-15)
+    assert hit > 0
+    # the same field without the sign flip would give no slope there: a plain copy
+    ops2 = ops.copy()
+    ops2["kind"] &= 0xFF
+    u2 = u.copy()
+    _eval_plan(ops2, refs, m.ndim, u2, 5)
+    assert not np.allclose(u2[ops["dst"] + ops["dst_stride"]], u[ops["dst"] + ops["dst_stride"]])
+
+
+def _problem(m, uniform=False, rest=False, recon="plm", seed=3):
+    bs = m.block_set()
+    u = np.zeros(bs.field_size(5))
+    for b in range(bs.nblocks):
+        x, y, z = bs.cell_centers(b)
+        v = bs.interior(u, b, 5)
+        r2 = (x - 0.3) ** 2 + (y - 0.6) ** 2 + ((z - 0.5) ** 2 if m.ndim == 3 else 0.0)
+        rho = 1.0 if uniform else 1.0 + 0.3 * np.exp(-r2 / 0.02)
+        p = 1.0 if uniform else 1.0 + 2.0 * np.exp(-r2 / 0.01)
+        vx = 0.0 if (uniform or rest) else 0.2 * np.sin(6.0 * y)
+        v[0] = rho
+        v[1] = rho * vx
+        v[2] = 0.0 * x
+        v[3] = 0.0 * x
+        v[4] = p / 0.4 + 0.5 * rho * vx * vx
+    return problems.Problem("amr_walls", bs, u, make_eos(0.4), recon, "hllc", 0.3)
+
+
+@pytest.mark.parametrize("name", ["2d_out", "2d_ref", "3d_out", "3d_ref"])
+def test_uniform_state_at_rest_is_a_fixed_point(name):
+    m = _mesh(name)
+    p = _problem(m, uniform=True)
+    st = ao.AmrState(p, m)
+    before = st.u0.copy()
+    st.step(st.estimate_dt())
+    mask = np.tile(p.bs.interior_mask(), 5)
+    assert np.array_equal(st.u0[mask], before[mask])
+
+
+def _totals(bs, u):
+    t = np.zeros(5)
+    for b in range(bs.nblocks):
+        t += bs.interior(u, b, 5).reshape(5, -1).sum(axis=1) * bs.blocks["vol"][b]
+    return t
+
+
+@pytest.mark.parametrize("name,recon", [("2d_ref", "plm"), ("2d_3lev_ref", "plm"),
+                                        ("3d_ref", "plm")])
+def test_closed_reflecting_box_keeps_mass_and_energy(name, recon):
+    m = _mesh(name)
+    p = _problem(m, recon=recon)
+    st = ao.AmrState(p, m)
+    t0 = _totals(p.bs, st.u0)
+    dt = st.estimate_dt()
+    for _ in range(4):
+        st.step(dt)
+    t1 = _totals(p.bs, st.u0)
+    assert abs(t1[0] - t0[0]) <= 1e-13 * abs(t0[0])
+    assert abs(t1[4] - t0[4]) <= 1e-13 * abs(t0[4])
+    # the walls push back: momentum is not a conserved total of a closed box
+    assert np.isfinite(st.u0).all()
+
+
+def test_static_refinement_of_the_reference_decks():
+    # inputs/evrards_collapse.in: 32^3 on [-1, 1]^3 in 16^3 blocks, [-0.5, 0.5]^3 at level 2
+    ref = amr.static_refinement(3, (2, 2, 2), (16, 16, 16), (-1.0,) * 3, (1.0,) * 3,
+                                [((-0.5,) * 3, (0.5,) * 3, 2)], OUT6)
+    m = amr.AmrMesh(3, (2, 2, 2), (16, 16, 16), 2, (-1.0,) * 3, (1.0,) * 3, ref, bc=OUT6)
+    assert np.bincount(m.level).tolist() == [0, 56, 64]
+    fine = m.loc[m.level == 2]
+    assert fine.min() == 2 and fine.max() == 5  # level-2 nodes 2..5: exactly [-0.5, 0.5]
+    # inputs/grav_test.in as gravity.py's SMR case runs it: 16^3 in 8^3 blocks on
+    # [-0.5, 0.5]^3, [-0.25, 0.25]^3 at level 2
+    ref = amr.static_refinement(3, (2, 2, 2), (8, 8, 8), (-0.5,) * 3, (0.5,) * 3,
+                                [((-0.25,) * 3, (0.25,) * 3, 2)], OUT6)
+    g = amr.AmrMesh(3, (2, 2, 2), (8, 8, 8), 2, (-0.5,) * 3, (0.5,) * 3, ref, bc=OUT6)
+    assert np.bincount(g.level).tolist() == [0, 56, 64]
+    # a region edge inside a node: that node and its sibling are refined; balance then
+    # refines the coarse neighbours two levels away
+    ref = amr.static_refinement(2, (4, 4), (8, 8), (0.0, 0.0), (1.0, 1.0),
+                                [((0.0, 0.0), (0.1, 0.1), 2)], OUT6)
+    t = amr.AmrMesh(2, (4, 4), (8, 8), 2, (0.0,) * 3, (1.0,) * 3, ref, bc=OUT6)
+    assert {(int(a), tuple(int(v) for v in b[:2])) for a, b in zip(t.level, t.loc)
+            if a == 2} == {(2, (x, y)) for x in (0, 1) for y in (0, 1)}
+    assert amr.block_table_py(t).tobytes() == t.blocks.tobytes()  # 2:1 balanced
+
+
+def test_refusals():
+    m = _mesh("2d_out")
+    bs = m.block_set()
+    with pytest.raises(ValueError, match="periodic"):
+        amr.regrid(m, bs, np.zeros(bs.field_size(5)), np.zeros(bs.nblocks, np.int32))
+    with pytest.raises(ValueError, match="periodic"):
+        amr.AmrMesh(2, (2, 2), (4, 4), 2, (0,) * 3, (1,) * 3, [],
+                    bc=["periodic", "outflow", "periodic", "periodic", "outflow", "outflow"])
+    # the library checks the face codes itself
+    import ctypes
+    vp = ctypes.c_void_p
+    out = np.zeros(1, dtype=m.blocks.dtype)
+    err = ctypes.create_string_buffer(256)
+    bad = np.array([0, 1, 0, 0, 0, 0], dtype=np.int32)
+    rc = _lib.lib().aph_amr_block_table(
+        2, np.array([1, 1, 1], np.int32).ctypes.data_as(vp),
+        np.array([4, 4, 1], np.int32).ctypes.data_as(vp), 2,
+        np.zeros(3).ctypes.data_as(vp), np.ones(3).ctypes.data_as(vp), bad.ctypes.data_as(vp),
+        0, 1, np.zeros(1, np.int32).ctypes.data_as(vp), np.zeros(3, np.int64).ctypes.data_as(vp),
+        out.ctypes.data_as(vp), err, len(err))
+    assert rc != _lib.APH_OK and b"periodic" in err.value
+
+
+def test_gravity_smr_oracle_meets_the_reference_limits():
+    """gravity.py's monopole_smr_16 configuration on the oracle: the field of the sphere
+    on the deck's three-level mesh meets REFINED_LIMITS, and the errors are below the
+    uniform 16^3 mesh's (the refined centre resolves the sphere)."""
+    from tests.monopole_checks import sphere_errors_max
+
+    p = problems.grav_sphere(res=16, block=8, smr_level=2)
+    st = ao.AmrState(p, p.meta["mesh"])
+    st.gravity_field()
+    l2p, l2g, mxp, mxg = sphere_errors_max(p.bs, st.grav, st.phi)
+    assert l2p <= 5e-3 and l2g <= 5e-2 and mxp <= 5e-2 and mxg <= 2e-1
+    u = problems.grav_sphere(res=16, block=8)
+    su = orc.State(u)
+    nrbin, rmax, G = u.monopole
+    mbin, _ = orc.mono_bin_masses(u.bs, su.prim, nrbin, rmax)
+    m_enc, phi_rad = orc.mono_tables(nrbin, 3, rmax, G, mbin)
+    grav, phi = orc.mono_field(u.bs, nrbin, rmax, G, m_enc, phi_rad)
+    ul2p, ul2g, _, _ = sphere_errors_max(u.bs, grav, phi)
+    assert l2g < ul2g
+

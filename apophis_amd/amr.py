@@ -344,6 +344,44 @@ def static_refinement(ndim, nbase, block_nx, xmin, xmax, regions, bc=None):
     return ref
 
 
+def mesh_from_pin(pin, ng):
+    """The mesh of an input deck's ``<parthenon/mesh>``, ``<parthenon/meshblock>`` and
+    ``<parthenon/static_refinementN>`` blocks (pin: nested dict {block: {key: value}} with
+    the deck's names and values): nx1..3, x1min..x3max, ix1_bc..ox3_bc (periodic, outflow,
+    reflecting) and numlevel of the mesh; the block size; each static refinement block's
+    x1min..x3max and level.  Returns an AmrMesh (with no static refinement block, its single
+    level).  ng: the ghost layers (the reconstruction's, hydro.cpp:56-67)."""
+    m = pin["parthenon/mesh"]
+    mb = pin.get("parthenon/meshblock", {})
+    nx = [int(m.get(f"nx{d + 1}", 1)) for d in range(3)]
+    nd = 3 if nx[2] > 1 else (2 if nx[1] > 1 else 1)
+    bnx = [int(mb.get(f"nx{d + 1}", nx[d])) for d in range(3)]
+    if any(nx[d] % bnx[d] for d in range(nd)):
+        raise ValueError("the mesh size must be a multiple of the block size")
+    nbase = [nx[d] // bnx[d] for d in range(nd)]
+    xmin = [float(m.get(f"x{d + 1}min", 0.0 if d < nd else -0.5)) for d in range(3)]
+    xmax = [float(m.get(f"x{d + 1}max", 1.0 if d < nd else 0.5)) for d in range(3)]
+    names = {"reflecting": "reflect"}
+    bc = []
+    for d in range(3):
+        for side in ("i", "o"):
+            b = str(m.get(f"{side}x{d + 1}_bc", "periodic"))
+            bc.append(names.get(b, b))
+    regions = []
+    for block, keys in pin.items():
+        if not block.startswith("parthenon/static_refinement"):
+            continue
+        lo = [float(keys.get(f"x{d + 1}min", xmin[d])) for d in range(3)]
+        hi = [float(keys.get(f"x{d + 1}max", xmax[d])) for d in range(3)]
+        regions.append((lo, hi, int(keys["level"])))
+    numlevel = int(m.get("numlevel", 1))
+    if any(r[2] > numlevel - 1 for r in regions):
+        raise ValueError("a static refinement level beyond numlevel - 1")
+    ref = static_refinement(nd, nbase, bnx[:nd], xmin, xmax, regions, bc)
+    return AmrMesh(nd, nbase, bnx[:nd], ng, xmin, xmax, ref, max_level=max(numlevel - 1, 0),
+                   bc=bc)
+
+
 def block_table_py(mesh: AmrMesh):
     """The block table of a mesh's leaves, stated in Python (the library builds the same
     bytes, aph_amr_block_table): 2:1 balance checked; dx, areas, volume and xmin per

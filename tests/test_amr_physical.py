@@ -341,3 +341,25 @@ def test_gravity_smr_oracle_meets_the_reference_limits():
     ul2p, ul2g, _, _ = sphere_errors_max(u.bs, grav, phi)
     assert l2g < ul2g
 
+
+
+def test_mesh_from_the_evrard_deck():
+    """The mesh blocks of inputs/evrards_collapse.in (:7-43) give the multilevel mesh the
+    problem generator's static refinement builds: the same leaves and block table."""
+    pin = {"parthenon/mesh": {"refinement": "static", "numlevel": 3, "nghost": 2,
+                              "nx1": 32, "x1min": -1.0, "x1max": 1.0, "ix1_bc": "outflow",
+                              "ox1_bc": "outflow", "nx2": 32, "x2min": -1.0, "x2max": 1.0,
+                              "ix2_bc": "outflow", "ox2_bc": "outflow", "nx3": 32,
+                              "x3min": -1.0, "x3max": 1.0, "ix3_bc": "outflow",
+                              "ox3_bc": "outflow"},
+           "parthenon/meshblock": {"nx1": 16, "nx2": 16, "nx3": 16},
+           "parthenon/static_refinement1": {"x1min": -0.5, "x1max": 0.5, "x2min": -0.5,
+                                            "x2max": 0.5, "x3min": -0.5, "x3max": 0.5,
+                                            "level": 2}}
+    m = amr.mesh_from_pin(pin, ng=2)
+    want = problems.evrard(smr_level=2).meta["mesh"]
+    assert m.blocks.tobytes() == want.blocks.tobytes()
+    assert np.array_equal(m.level, want.level) and np.array_equal(m.loc, want.loc)
+    pin["parthenon/static_refinement1"]["level"] = 3
+    with pytest.raises(ValueError, match="numlevel"):
+        amr.mesh_from_pin(pin, ng=2)

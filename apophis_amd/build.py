@@ -13,8 +13,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "_build")
 SOURCES = ["aph_fused.hip", "aph_fused_wide.hip", "aph_fused_amr.hip", "aph_adjoint.hip",
-           "aph_plain.hip", "aph_runtime.hip", "aph_amr.hip", "aph_amr_tree.hip",
-           "aph_gravity.hip", "aph_poisson.hip"]
+           "aph_adjoint_wide.hip", "aph_plain.hip", "aph_runtime.hip", "aph_amr.hip",
+           "aph_amr_tree.hip", "aph_gravity.hip", "aph_poisson.hip"]
 ARCH = os.environ.get("APH_OFFLOAD_ARCH", "gfx950")
 FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

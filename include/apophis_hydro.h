@@ -528,8 +528,9 @@ int aph_plan_set_wait_limit(aph_context *ctx, aph_plan *plan, int32_t polls);
  *                 lam_n = gam1 * lam_out
  * where J is the Jacobian of the flux-divergence (+ gravity) term with respect to the
  * stage input u_in (ConsToPrim, reconstruction through the limiter's active branch, the
- * Riemann solver's active wave branches; HLLE, HLLC and LHLLC).  Only interior cells of
- * the cotangent arrays are read or written.  lam_acc and lam_n may be NULL. */
+ * Riemann solver's active wave branches; HLLE, HLLC and LHLLC; nvar 5, 6, 12 or 14).  Only
+ * interior cells of the cotangent arrays are read or written.  lam_acc and lam_n may be
+ * NULL. */
 int aph_adjoint_stage(aph_context *ctx, aph_plan *plan, int32_t stage, const double *u_in,
                       const double *dt_dev, const double *lam_out, const double *lam_acc,
                       double *lam_in, double *lam_n);

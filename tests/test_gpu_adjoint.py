@@ -67,6 +67,13 @@ REF_CASES = {
                                                           seed=5),
     "wave_plm_hlle_1d": lambda: problems.linear_wave(ndim=1, res=32, recon="plm",
                                                      riemann="hlle"),
+    # the composition counts of kh-comp-advection.in: ONe + one level set (NV = 14) and
+    # ONe alone (NV = 12), in both adjoint forms (PLM one march, PPM split) and with vl2
+    "khcomp_plm_hllc_nv14": lambda: problems.kh_comp(nx=(16, 16), block=(16, 16)),
+    "khcomp_ppm_hlle_nv12": lambda: problems.kh_comp(nx=(16, 16), block=(16, 16), nlset=0,
+                                                     recon="ppm", riemann="hlle"),
+    "khcomp_plm_lhllc_vl2_nv14": lambda: _with(
+        problems.kh_comp(nx=(16, 16), block=(16, 16), riemann="lhllc"), integrator="vl2"),
 }
 
 

@@ -87,6 +87,21 @@ KFn pick_gather(int nv, int nd) {
   return nullptr;
 }
 
+// the multilevel kernels: which 0 = k_adj_amr_prim, 1 = k_adj_amr_ghost, 2 =
+// k_adj_amr_scatter (multilevel meshes are 2D or 3D)
+template <int NV>
+KFn amr_kernel(int nd, int which) {
+  if (which == 0) return nd == 3 ? k_adj_amr_prim<NV, 3> : k_adj_amr_prim<NV, 2>;
+  if (which == 1) return nd == 3 ? k_adj_amr_ghost<NV, 3> : k_adj_amr_ghost<NV, 2>;
+  return k_adj_amr_scatter<NV>;
+}
+
+KFn pick_amr(int nv, int nd, int which) {
+  if (nv == 5) return amr_kernel<5>(nd, which);
+  if (nv == 6) return amr_kernel<6>(nd, which);
+  return nullptr;
+}
+
 KFn pick_prim(int nv, int nd) {
   if (nv == 12 || nv == 14) return pick_prim_wide(nv, nd);
   if (nv == 5) return pick_nd(nd, k_adj_prim<5, 1>, k_adj_prim<5, 2>, k_adj_prim<5, 3>);
@@ -160,6 +175,9 @@ int adj_prepare(aph_context *ctx, const aph_blockset *bs, int recon, AdjScratch 
 }
 
 void adj_release(AdjScratch &s) {
+  if (s.contrib) (void)hipFree(s.contrib);
+  s.contrib = nullptr;
+  s.ncontrib = 0;
   for (double *p : {s.pbar, s.lam1, s.lam2, s.lam0p, s.prim, s.prim_t, s.lam_t, s.qlb, s.qrb})
     if (p) (void)hipFree(p);
   for (auto &p : s.spill)
@@ -171,10 +189,18 @@ void adj_release(AdjScratch &s) {
 int adj_stage(aph_context *ctx, const aph_blockset *bs, int recon, int rs, const aph_eos &eos,
               const double *g, int grav, AdjScratch &s, const AdjCoef &c, const double *u_in,
               const double *dt_dev, const double *lam_out, const double *lam_acc,
-              double *lam_in, double *lam_n, int lam_n_add) {
+              double *lam_in, double *lam_n, int lam_n_add, const AdjAmr *amr) {
   if (bs->nblocks == 0) return APH_OK;
   int rc = adj_prepare(ctx, bs, c.plan_recon, s);
   if (rc) return rc;
+  if (amr && amr->nrefs > s.ncontrib) {  // per ref of the ghost ops, NV cotangents
+    if (s.contrib) (void)hipFree(s.contrib);
+    s.contrib = nullptr;
+    s.ncontrib = 0;
+    if (hipMalloc(&s.contrib, sizeof(double) * bs->nvar * amr->nrefs) != hipSuccess)
+      return set_error(ctx, APH_E_NOMEM, "adjoint scratch allocation failed");
+    s.ncontrib = amr->nrefs;
+  }
   const int nd = bs->ndim;
   AxisKernels ks[3];
   for (int d = 0; d < nd; ++d) {
@@ -213,6 +239,8 @@ int adj_stage(aph_context *ctx, const aph_blockset *bs, int recon, int rs, const
   a.spill_base = s.spill_base;
   a.pbar_xt = ks[0].face != nullptr;  // the split form writes the x axis' Pbar transposed
   for (int d = 0; d < 3; ++d) a.segcap[d] = s.segcap[d];
+  if (amr) a.amr = *amr;
+  a.contrib = s.contrib;
   const int nf = bs->nvar + 2;
   // prim / gather: one 32 x TY (x, y) tile (32 x 1 in 1D) per workgroup pass
   auto tile_launch = [&](int ty, dim3 &grid, dim3 &block) {
@@ -228,6 +256,11 @@ int adj_stage(aph_context *ctx, const aph_blockset *bs, int recon, int rs, const
   tile_launch(prim_tile_y(nf), gp, bp);
   tile_launch(gather_tile_y(nf), gg, bg);
   if (ks[0].face) hipLaunchKernelGGL(pick_prim(bs->nvar, nd), gp, bp, 0, ctx->stream, a);
+  const unsigned gops = amr ? (unsigned)std::max<int64_t>(
+                                  1, std::min<int64_t>((amr->nops + 255) / 256, 4096))
+                            : 1u;
+  if (ks[0].face && amr && amr->nops)  // P of the cross-level ghost cells the sweeps read
+    hipLaunchKernelGGL(pick_amr(bs->nvar, nd, 0), dim3(gops), dim3(256), 0, ctx->stream, a);
   const unsigned nbk = (unsigned)bs->nblocks;
   int64_t maxu_all = 0;
   for (int d = 0; d < nd; ++d) {
@@ -250,6 +283,12 @@ int adj_stage(aph_context *ctx, const aph_blockset *bs, int recon, int rs, const
     hipLaunchKernelGGL(ks[0].axis, dim3((unsigned)((maxu_all + 63) / 64), nbk, (unsigned)nd),
                        dim3(64), 0, ctx->stream, a);
   hipLaunchKernelGGL(pick_gather(bs->nvar, nd), gg, bg, 0, ctx->stream, a);
+  if (amr && amr->nops) {  // the cross-level ghosts' cotangents back to the cells they read
+    hipLaunchKernelGGL(pick_amr(bs->nvar, nd, 1), dim3(gops), dim3(256), 0, ctx->stream, a);
+    const unsigned gt = (unsigned)std::max<int64_t>(
+        1, std::min<int64_t>((amr->ntgt + 255) / 256, 4096));
+    hipLaunchKernelGGL(pick_amr(bs->nvar, nd, 2), dim3(gt), dim3(256), 0, ctx->stream, a);
+  }
   APH_LAUNCH_CHECK(ctx);
   return APH_OK;
 }

@@ -89,6 +89,8 @@ struct AdjArgs {
   double *spill[3];            // per axis, [segment][slot][field][pencil] per block
   const int64_t *spill_base;   // [3][nblocks] element offsets
   int segcap[3];               // adj_seg cap per axis
+  AdjAmr amr;                  // multilevel mesh (faces == nullptr: single level)
+  double *contrib;             // multilevel: per ref of the ghost ops, NV cotangents
 };
 
 namespace {
@@ -112,6 +114,8 @@ __device__ __forceinline__ bool map_load(const AdjArgs &a, const AxisMap &m, int
     } else if (m.bclo == APH_BC_REFLECT) {
       idx = -1 - c;
       flip = true;
+    } else if (m.bclo == APH_BC_AMR) {
+      idx = c;  // the block's own ghost cells, set from the other level
     } else {
       idx = 0;
     }
@@ -122,6 +126,8 @@ __device__ __forceinline__ bool map_load(const AdjArgs &a, const AxisMap &m, int
     } else if (m.bchi == APH_BC_REFLECT) {
       idx = 2 * n_d - 1 - c;
       flip = true;
+    } else if (m.bchi == APH_BC_AMR) {
+      idx = c;
     } else {
       idx = n_d - 1;
     }
@@ -156,6 +162,8 @@ __device__ __forceinline__ int64_t map_cell(const CellMap &m, int n_d, int c, in
     } else if (m.bclo == APH_BC_REFLECT) {
       idx = -1 - c;
       flip = true;
+    } else if (m.bclo == APH_BC_AMR) {
+      idx = c;  // the block's own ghost cells (P from k_adj_amr_prim)
     } else {
       idx = 0;
     }
@@ -166,11 +174,52 @@ __device__ __forceinline__ int64_t map_cell(const CellMap &m, int n_d, int c, in
     } else if (m.bchi == APH_BC_REFLECT) {
       idx = 2 * n_d - 1 - c;
       flip = true;
+    } else if (m.bchi == APH_BC_AMR) {
+      idx = c;
     } else {
       idx = n_d - 1;
     }
   }
   return base;
+}
+
+// Flux correction in reverse, at a block's boundary face f = 0 (side 0) or n (side 1) on a
+// face to another level.  Forward: a face to a finer region (role 2) replaces its own flux
+// by (sum of the 2^(ndim-1) fine fluxes across it) * 2^(1-ndim), so its own Riemann solve
+// gets no cotangent; a face to a coarser leaf (role 1) is one of those fine fluxes, which
+// the coarse cell beside it uses too: its cotangent gains 2^(1-ndim) times the coarse
+// copy's, mu_c = beta dt A_c / V_c * (+-L_out of the coarse cell) (the coarse block's own
+// copy of the face: the cell below gives -, the cell above +).  pa, pb: the pencil's
+// transverse cells in this block.
+template <int NV, int NDIM, int D>
+__device__ __forceinline__ void amr_face_mu(const AdjArgs &a, int b, int side, int pa, int pb,
+                                            double *mu) {
+  constexpr int A = (D == 0) ? 1 : 0, B = (D == 2) ? 1 : 2;
+  const aph_amr_face af = a.amr.faces[b * 6 + 2 * D + side];
+  if (af.role == 2) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) mu[v] = 0.0;
+    return;
+  }
+  if (af.role != 1) return;
+  const aph_block &ck = a.blocks[a.amr.cblk[b * 6 + 2 * D + side]];
+  const int N0 = ext(ck, 0, NDIM), N1 = ext(ck, 1, NDIM), N2 = ext(ck, 2, NDIM);
+  const int64_t nc = (int64_t)N0 * N1 * N2;
+  const int g[3] = {gst(ck, 0, NDIM), gst(ck, 1, NDIM), gst(ck, 2, NDIM)};
+  int idx[3] = {0, 0, 0};
+  idx[D] = side ? 0 : ck.nx[D] - 1;  // the coarse cell beside the face
+  idx[A] = (af.qa + pa) >> 1;
+  if (B < NDIM) idx[B] = (af.qb + pb) >> 1;
+  const int64_t cell =
+      ((int64_t)(idx[2] + g[2]) * N1 + (idx[1] + g[1])) * N0 + (idx[0] + g[0]);
+  const double coef_c = a.beta * a.dt[0] * ck.area[D] / ck.vol;
+  const double share = (NDIM == 3) ? 0.25 : 0.5;
+  const double *lam = a.lam_out + (int64_t)NV * ck.cell_base + cell;
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    const double muc = side ? coef_c * (lam[v * nc] - 0.0) : coef_c * (0.0 - lam[v * nc]);
+    mu[v] = mu[v] + muc * share;
+  }
 }
 
 template <int D>
@@ -445,6 +494,8 @@ __device__ __forceinline__ void adj_face_body(const AdjArgs &a) {
       for (int v = 0; v < NV; ++v) mu[v] = coef * (lcur[v] - lprev[v]);
 #pragma unroll
       for (int v = 0; v < NV; ++v) lprev[v] = lcur[v];
+      if (a.amr.faces && ((f == 0 && m.bclo == APH_BC_AMR) || (f == n_d && m.bchi == APH_BC_AMR)))
+        amr_face_mu<NV, NDIM, D>(a, b, f == 0 ? 0 : 1, pa, pb, mu);
       const double wl[5] = {qlp[0], qlp[IVX], qlp[IVY], qlp[IVZ], qlp[4]};
       const double wr[5] = {qr[0], qr[IVX], qr[IVY], qr[IVZ], qr[4]};
       double flx[5];
@@ -796,6 +847,8 @@ __device__ __forceinline__ void adj_axis_body(const AdjArgs &a) {
       double mu[NV];
 #pragma unroll
       for (int v = 0; v < NV; ++v) mu[v] = coef * (lcur[v] - lprev[v]);
+      if (a.amr.faces && ((f == 0 && m.bclo == APH_BC_AMR) || (f == n_d && m.bchi == APH_BC_AMR)))
+        amr_face_mu<NV, NDIM, D>(a, b, f == 0 ? 0 : 1, pa, pb, mu);
       // forward Riemann flux (needed for the scalar upwind)
       const double wl[5] = {qlp[0], qlp[IVX], qlp[IVY], qlp[IVZ], qlp[4]};
       const double wr[5] = {qr[0], qr[IVX], qr[IVY], qr[IVZ], qr[4]};
@@ -986,7 +1039,7 @@ __global__ void __launch_bounds__(512) k_adj_gather(AdjArgs a) {
               add_slot<NF>(a, d, nb, pencil, nseg - 1, nseg, 3 + off, pbv);
             } else if (bk.bc[2 * d] == APH_BC_REFLECT) {
               add_slot<NF>(a, d, b, pencil, 0, nseg, 2 - off, pbv);
-            } else if (cc[d] == 0) {
+            } else if (bk.bc[2 * d] != APH_BC_AMR && cc[d] == 0) {  // outflow images
               for (int sl = 0; sl < 3; ++sl) add_slot<NF>(a, d, b, pencil, 0, nseg, sl, pbv);
             }
           }
@@ -1000,7 +1053,7 @@ __global__ void __launch_bounds__(512) k_adj_gather(AdjArgs a) {
               add_slot<NF>(a, d, nb, pencil, 0, nseg, off - (S - 3), pbv);
             } else if (bk.bc[2 * d + 1] == APH_BC_REFLECT) {
               add_slot<NF>(a, d, b, pencil, nseg - 1, nseg, 3 + (n_d - 1 - cc[d]), pbv);
-            } else if (cc[d] == n_d - 1) {
+            } else if (bk.bc[2 * d + 1] != APH_BC_AMR && cc[d] == n_d - 1) {
               for (int sl = 3; sl < 6; ++sl)
                 add_slot<NF>(a, d, b, pencil, nseg - 1, nseg, sl, pbv);
             }
@@ -1031,6 +1084,157 @@ __global__ void __launch_bounds__(512) k_adj_gather(AdjArgs a) {
       }
     }
     if (a.pbar_xt) __syncthreads();  // the tile is used before the next one is loaded
+  }
+}
+
+// Multilevel meshes ------------------------------------------------------------------
+// The ghost cells across a face to another level hold values the cross-level ops computed
+// from interior cells (the caller ran aph_amr_fill on u_in).  The sweeps read them like
+// any stencil cell and leave their primitive-space cotangents in the spill slots beyond
+// the block; these kernels take them back to the interior cells the ops read.
+
+// Block-local ghost-inclusive (i, j, k) of a component-0 offset of block b.
+__device__ __forceinline__ void local_ijk(const aph_block &bk, int64_t off, int ndim,
+                                          int &i, int &j, int &k) {
+  const int N0 = ext(bk, 0, ndim), N1 = ext(bk, 1, ndim);
+  i = (int)(off % N0);
+  off /= N0;
+  j = (int)(off % N1);
+  k = (int)(off / N1);
+}
+
+// Split form: P = ConsToPrim(U) of every op's ghost cell, in the pack and x-transposed
+// layouts (the sweeps read the face ghosts among them).
+template <int NV, int NDIM>
+__global__ void __launch_bounds__(256) k_adj_amr_prim(AdjArgs a) {
+  constexpr int NF = NV + 2;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < a.amr.nops;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const aph_amr_op op = a.amr.ops[t];
+    const aph_block &bk = a.blocks[a.amr.op_block[t]];
+    const int N0 = ext(bk, 0, NDIM), N1 = ext(bk, 1, NDIM), N2 = ext(bk, 2, NDIM);
+    const int64_t cell = op.dst - (int64_t)NV * bk.cell_base;
+    int i, j, k;
+    local_ijk(bk, cell, NDIM, i, j, k);
+    double u[NV], w[NV], gm[2];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) u[v] = a.u_in[op.dst + v * op.dst_stride];
+    cons_to_prim_pt<NV>(u, a.eos, w, gm);
+    double *pp = a.prim + (int64_t)NF * bk.cell_base + cell;
+    double *pt = a.prim_t + (int64_t)NF * bk.cell_base + ((int64_t)i * N2 + k) * N1 + j;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      const double x = f < NV ? w[f] : gm[f - NV];
+      pp[f * op.dst_stride] = x;
+      pt[f * op.dst_stride] = x;
+    }
+  }
+}
+
+__device__ __forceinline__ double sgn1(double x) { return x < 0.0 ? -1.0 : 1.0; }
+
+// Per op: the ghost cell's primitive-space cotangent (a face ghost within three cells of
+// the block: the spill slot of the one pencil that reads it; edge and corner ghosts are
+// never read), taken to conserved space at the ghost state (J_prim^T), then back through
+// the op: MEAN (sum * 1/count) gives each child 1/count of it; PROLONG c + sum_d
+// (sign_d / 4) minmod(c - lo_d, hi_d - c) follows minmod's active branch (equal signs;
+// the smaller magnitude, a on a tie; nothing when the signs differ) and the mirror bits.
+// Every ref of every op gets its NV contributions written (zeros included).
+template <int NV, int NDIM>
+__global__ void __launch_bounds__(256) k_adj_amr_ghost(AdjArgs a) {
+  constexpr int NF = NV + 2;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < a.amr.nops;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const aph_amr_op op = a.amr.ops[t];
+    const int b = a.amr.op_block[t];
+    const aph_block &bk = a.blocks[b];
+    const int g[3] = {gst(bk, 0, NDIM), gst(bk, 1, NDIM), gst(bk, 2, NDIM)};
+    int id[3];
+    local_ijk(bk, op.dst - (int64_t)NV * bk.cell_base, NDIM, id[0], id[1], id[2]);
+    int axis = -1, nout = 0, c = 0;
+    for (int d = 0; d < NDIM; ++d) {
+      const int q = id[d] - g[d];
+      if (q < 0 || q >= bk.nx[d]) {
+        ++nout;
+        axis = d;
+        c = q;
+      }
+    }
+    double pbv[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) pbv[f] = 0.0;
+    if (nout == 1 && c >= -3 && c < bk.nx[axis] + 3) {
+      const int d = axis;
+      const int Ad = (d == 0) ? 1 : 0, Bd = (d == 2) ? 1 : 2;
+      const int n_d = bk.nx[d];
+      const int nseg = n_d / adj_seg(n_d, a.segcap[d]);
+      const int pencil = (id[Ad] - g[Ad]) + (Bd < NDIM ? (id[Bd] - g[Bd]) * bk.nx[Ad] : 0);
+      if (c < 0)
+        add_slot<NF>(a, d, b, pencil, 0, nseg, c + 3, pbv);
+      else
+        add_slot<NF>(a, d, b, pencil, nseg - 1, nseg, 3 + (c - n_d), pbv);
+    }
+    double u[NV], ub[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      u[v] = a.u_in[op.dst + v * op.dst_stride];
+      ub[v] = 0.0;
+    }
+    cons_to_prim_adj<NV>(u, a.eos, pbv, pbv + NV, ub);
+    const int kind = op.kind & 0xff, mirror = op.kind >> APH_AMR_MIRROR_BIT;
+    auto value = [&](const aph_amr_val &vv, int comp) -> double {
+      const aph_amr_ref *r = a.amr.refs + vv.first;
+      double s = a.u_in[r[0].offset + comp * r[0].stride];
+      for (int q = 1; q < vv.count; ++q) s += a.u_in[r[q].offset + comp * r[q].stride];
+      return vv.count == 1 ? s : s * (1.0 / vv.count);
+    };
+    auto put = [&](const aph_amr_val &vv, int comp, double vb) {
+      const double each = vv.count == 1 ? vb : vb * (1.0 / vv.count);
+      for (int q = 0; q < vv.count; ++q) a.contrib[(int64_t)(vv.first + q) * NV + comp] = each;
+    };
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      if (kind == APH_AMR_MEAN) {
+        put(op.val[0], v, ub[v]);
+        continue;
+      }
+      const double cv = value(op.val[0], v);
+      double cb = ub[v];
+      for (int d = 0; d < NDIM; ++d) {
+        const bool mlo = (v == 1 + d) && (mirror & (1 << (2 * d)));
+        const bool mhi = (v == 1 + d) && (mirror & (2 << (2 * d)));
+        const double lo = mlo ? -value(op.val[1 + 2 * d], v) : value(op.val[1 + 2 * d], v);
+        const double hi = mhi ? -value(op.val[2 + 2 * d], v) : value(op.val[2 + 2 * d], v);
+        const double x = cv - lo, y = hi - cv;
+        const double gb = ub[v] * (op.sign[d] * 0.25);
+        double xb = 0.0, yb = 0.0;
+        if (sgn1(x) == sgn1(y)) {  // minmod = sgn * min(|x|, |y|): the smaller one
+          if (fabs(y) < fabs(x)) yb = gb;
+          else xb = gb;
+        }
+        cb += xb - yb;
+        put(op.val[1 + 2 * d], v, mlo ? xb : -xb);  // lo enters x with -1 (-lo mirrored)
+        put(op.val[2 + 2 * d], v, mhi ? -yb : yb);
+      }
+      put(op.val[0], v, cb);
+    }
+  }
+}
+
+// lam_in += the contributions of the ghost ops, per interior cell they read, summed over
+// its refs in ascending order (one lane per cell: no atomics, a fixed order).
+template <int NV>
+__global__ void __launch_bounds__(256) k_adj_amr_scatter(AdjArgs a) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < a.amr.ntgt;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t off = a.amr.tgt[t], st = a.amr.tgt_stride[t];
+    const int64_t r0 = a.amr.row[t], r1 = a.amr.row[t + 1];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      double s = a.contrib[(int64_t)a.amr.col[r0] * NV + v];
+      for (int64_t r = r0 + 1; r < r1; ++r) s += a.contrib[(int64_t)a.amr.col[r] * NV + v];
+      a.lam_in[off + v * st] += s;
+    }
   }
 }
 }  // namespace

@@ -16,6 +16,7 @@
 #include <utility>
 #include <vector>
 
+#include "aph_adjoint_host.hpp"
 #include "aph_device.hpp"
 #include "aph_runtime.hpp"
 
@@ -28,6 +29,12 @@ struct aph_amr_plan {
   aph_amr_op *ops = nullptr;
   aph_flux_fix *fixes = nullptr;
   aph_amr_ref *refs = nullptr;
+  // for the adjoint: the block of each op's ghost cell, and the cells the ops read as a
+  // transposed list (rows of ref indices per cell, ascending)
+  int32_t *op_block = nullptr;
+  int64_t *tgt = nullptr, *tgt_stride = nullptr, *row = nullptr;
+  int32_t *col = nullptr;
+  int64_t ntgt = 0;
 };
 
 namespace aph {
@@ -334,9 +341,39 @@ int aph_amr_plan_create(aph_context *ctx, const aph_blockset *bs, const aph_amr_
     if (hipMalloc(dst, bytes) != hipSuccess) return false;
     return hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
   };
+  // the adjoint's view: each op's block, and the refs of the ops grouped by the cell they
+  // read (a cell's contributions are then summed in one fixed order, with no atomics)
+  std::vector<int32_t> op_block((size_t)nops);
+  std::vector<std::pair<int64_t, int32_t>> pairs;  // (cell offset, ref index)
+  for (int64_t t = 0; t < nops; ++t) {
+    const aph_amr_op &op = ops[t];
+    op_block[t] = loc.find(op.dst, op.dst_stride);
+    const int nval = (op.kind & 0xff) == APH_AMR_MEAN ? 1 : 1 + 2 * bs->ndim;
+    for (int k = 0; k < nval; ++k)
+      for (int r = op.val[k].first; r < op.val[k].first + op.val[k].count; ++r)
+        pairs.emplace_back(refs[r].offset, r);
+  }
+  std::sort(pairs.begin(), pairs.end());
+  std::vector<int64_t> tgt, tstride, row;
+  std::vector<int32_t> col;
+  for (size_t q = 0; q < pairs.size(); ++q) {
+    if (q == 0 || pairs[q].first != pairs[q - 1].first) {
+      tgt.push_back(pairs[q].first);
+      tstride.push_back(refs[pairs[q].second].stride);
+      row.push_back((int64_t)q);
+    }
+    col.push_back(pairs[q].second);
+  }
+  row.push_back((int64_t)pairs.size());
+  p->ntgt = (int64_t)tgt.size();
   if (!up((void **)&p->ops, ops, sizeof(aph_amr_op) * nops) ||
       !up((void **)&p->fixes, fixes, sizeof(aph_flux_fix) * nfixes) ||
-      !up((void **)&p->refs, refs, sizeof(aph_amr_ref) * nrefs)) {
+      !up((void **)&p->refs, refs, sizeof(aph_amr_ref) * nrefs) ||
+      !up((void **)&p->op_block, op_block.data(), sizeof(int32_t) * op_block.size()) ||
+      !up((void **)&p->tgt, tgt.data(), sizeof(int64_t) * tgt.size()) ||
+      !up((void **)&p->tgt_stride, tstride.data(), sizeof(int64_t) * tstride.size()) ||
+      !up((void **)&p->row, row.data(), sizeof(int64_t) * row.size()) ||
+      !up((void **)&p->col, col.data(), sizeof(int32_t) * col.size())) {
     aph_amr_plan_destroy(ctx, p);
     return set_error(ctx, APH_E_NOMEM, "amr plan device allocation failed");
   }
@@ -347,9 +384,9 @@ int aph_amr_plan_create(aph_context *ctx, const aph_blockset *bs, const aph_amr_
 int aph_amr_plan_destroy(aph_context *ctx, aph_amr_plan *p) {
   if (!ctx) return APH_E_INVALID;
   if (!p) return APH_OK;
-  if (p->ops) (void)hipFree(p->ops);
-  if (p->fixes) (void)hipFree(p->fixes);
-  if (p->refs) (void)hipFree(p->refs);
+  for (void *q : {(void *)p->ops, (void *)p->fixes, (void *)p->refs, (void *)p->op_block,
+                  (void *)p->tgt, (void *)p->tgt_stride, (void *)p->row, (void *)p->col})
+    if (q) (void)hipFree(q);
   delete p;
   return APH_OK;
 }
@@ -388,6 +425,19 @@ int aph_amr_flux_correct(aph_context *ctx, const aph_amr_plan *p, double *f1, do
 }  // extern "C"
 
 namespace aph {
+
+void amr_adj_view(const aph_amr_plan *p, AdjAmr &v) {
+  v.ops = p->ops;
+  v.nops = p->nops;
+  v.refs = p->refs;
+  v.nrefs = p->nrefs;
+  v.op_block = p->op_block;
+  v.tgt = p->tgt;
+  v.tgt_stride = p->tgt_stride;
+  v.row = p->row;
+  v.col = p->col;
+  v.ntgt = p->ntgt;
+}
 
 int amr_fill_epoch(aph_context *ctx, const aph_amr_plan *p, double *cons, unsigned *epoch) {
   if (!p->nops && !epoch) return APH_OK;

@@ -114,6 +114,7 @@ struct aph_plan {
   // list, a running count, and the launch epoch)
   const aph_amr_plan *amr = nullptr;
   aph_amr_face *d_amr_face = nullptr;
+  int32_t *d_amr_cblk = nullptr;  // per block face: the coarse block of a role-1 face
   double *d_amr_buf = nullptr;
   std::vector<int32_t> level;
   std::vector<aph_amr_face> faces_h;
@@ -476,10 +477,9 @@ int adj_coef(const aph_plan *p, int s, AdjCoef &c, int &srecon) {
   return n;
 }
 
-// The adjoint entry points' checks: a single-level plan whose reconstruction / Riemann
-// pair (and vl2's first-order predictor) the adjoint kernels cover.
+// The adjoint entry points' checks: a plan whose reconstruction / Riemann pair (and vl2's
+// first-order predictor) the adjoint kernels cover, without self-gravity.
 int adj_plan_ok(aph_context *ctx, const aph_plan *p) {
-  if (p->amr) return set_error(ctx, APH_E_INVALID, "adjoint: multilevel meshes are not supported");
   if (p->mono || p->poisson)
     return set_error(ctx, APH_E_INVALID, "adjoint: monopole / Poisson gravity is not supported");
   if (!adj_supported(p->recon, p->rsolver, p->bs->nvar, p->bs->ndim) ||
@@ -489,6 +489,32 @@ int adj_plan_ok(aph_context *ctx, const aph_plan *p) {
   if (p->integrator == APH_INTEGRATOR_RK3 && !p->d_u2)
     return set_error(ctx, APH_E_STATE, "adjoint: rk3 plan without its third register");
   return APH_OK;
+}
+
+// The multilevel view of the adjoint (empty on a single-level plan).
+AdjAmr adj_amr(const aph_plan *p) {
+  AdjAmr v;
+  if (!p->amr) return v;
+  amr_adj_view(p->amr, v);
+  v.faces = p->d_amr_face;
+  v.cblk = p->d_amr_cblk;
+  return v;
+}
+
+// One adjoint stage of the plan; on a multilevel mesh the stage input's cross-level ghost
+// cells are filled first (the forward stage reads them; they hold no state).
+int plan_adj_stage(aph_context *ctx, aph_plan *p, int sr, const AdjCoef &c, const double *x,
+                   const double *dt_dev, const double *lo, const double *acc, double *li,
+                   double *ln, int add) {
+  if (p->amr) {
+    int rc = amr_fill_epoch(ctx, p->amr, const_cast<double *>(x), nullptr);
+    if (rc) return rc;
+    const AdjAmr v = adj_amr(p);
+    return adj_stage(ctx, p->bs, sr, p->rsolver, p->eos, p->g, p->grav, p->adj, c, x, dt_dev,
+                     lo, acc, li, ln, add, &v);
+  }
+  return adj_stage(ctx, p->bs, sr, p->rsolver, p->eos, p->g, p->grav, p->adj, c, x, dt_dev,
+                   lo, acc, li, ln, add);
 }
 
 // Candidate tiles for one block shape and workgroup size: divisors of the block edges,
@@ -590,6 +616,7 @@ int aph_plan_destroy(aph_context *ctx, aph_plan *p) {
   if (p->d_u2) (void)hipFree(p->d_u2);
   if (p->d_gfield) (void)hipFree(p->d_gfield);
   if (p->d_amr_face) (void)hipFree(p->d_amr_face);
+  if (p->d_amr_cblk) (void)hipFree(p->d_amr_cblk);
   if (p->d_amr_buf) (void)hipFree(p->d_amr_buf);
   for (void *q : {(void *)p->d_region, (void *)p->d_count, (void *)p->d_expect,
                   (void *)p->d_epoch, (void *)p->d_err})
@@ -660,6 +687,7 @@ int aph_plan_set_amr(aph_context *ctx, aph_plan *p, const aph_amr_plan *amr,
   }
   // a face to a coarser leaf (role 1) writes into the region of the opposite face of a
   // block one level coarser, which runs in the next phase
+  std::vector<int32_t> cblk((size_t)6 * bs->nblocks, -1);
   for (int b = 0; b < bs->nblocks; ++b) {
     for (int f = 0; f < 2 * nd; ++f) {
       const aph_amr_face &af = faces[b * 6 + f];
@@ -684,6 +712,7 @@ int aph_plan_set_amr(aph_context *ctx, aph_plan *p, const aph_amr_plan *amr,
       if (last >= r.end)
         return set_error(ctx, APH_E_INVALID, "block %d face %d: outside its coarse region", b,
                          f);
+      cblk[b * 6 + f] = r.block;
     }
   }
   // region index per block face: a role-2 face's own region, a role-1 face's target
@@ -696,10 +725,12 @@ int aph_plan_set_amr(aph_context *ctx, aph_plan *p, const aph_amr_plan *amr,
   p->faces_h.assign(faces, faces + (size_t)6 * bs->nblocks);
   p->nregions = (int)region_index.size();
   if (p->d_amr_face) (void)hipFree(p->d_amr_face);
+  if (p->d_amr_cblk) (void)hipFree(p->d_amr_cblk);
   if (p->d_amr_buf) (void)hipFree(p->d_amr_buf);
   for (void *q : {(void *)p->d_region, (void *)p->d_count, (void *)p->d_expect,
                   (void *)p->d_epoch, (void *)p->d_err})
     if (q) (void)hipFree(q);
+  p->d_amr_cblk = nullptr;
   p->d_amr_face = nullptr;
   p->d_amr_buf = nullptr;
   p->d_region = nullptr;
@@ -718,10 +749,13 @@ int aph_plan_set_amr(aph_context *ctx, aph_plan *p, const aph_amr_plan *amr,
                                sizeof(int32_t) * 6 * bs->nblocks, hipMemcpyHostToDevice));
   APH_HIP_CHECK(ctx, hipMemset(p->d_err, 0, sizeof(int)));  // read by aph_plan_status
   if (hipMalloc(&p->d_amr_face, sizeof(aph_amr_face) * 6 * bs->nblocks) != hipSuccess ||
+      hipMalloc(&p->d_amr_cblk, sizeof(int32_t) * 6 * bs->nblocks) != hipSuccess ||
       hipMalloc(&p->d_amr_buf, sizeof(double) * std::max<int64_t>(buffer_doubles, 1)) !=
           hipSuccess)
     return set_error(ctx, APH_E_NOMEM, "aph_plan_set_amr: device allocation failed");
   APH_HIP_CHECK(ctx, hipMemcpy(p->d_amr_face, faces, sizeof(aph_amr_face) * 6 * bs->nblocks,
+                               hipMemcpyHostToDevice));
+  APH_HIP_CHECK(ctx, hipMemcpy(p->d_amr_cblk, cblk.data(), sizeof(int32_t) * 6 * bs->nblocks,
                                hipMemcpyHostToDevice));
   APH_HIP_CHECK(ctx, hipMemset(p->d_amr_buf, 0, sizeof(double) * std::max<int64_t>(buffer_doubles, 1)));
   p->amr = amr;
@@ -1210,8 +1244,7 @@ int aph_adjoint_stage(aph_context *ctx, aph_plan *p, int32_t stage, const double
   int sr;
   if (!adj_coef(p, stage, c, sr))
     return set_error(ctx, APH_E_INVALID, "aph_adjoint_stage: stage %d out of range", stage);
-  return adj_stage(ctx, p->bs, sr, p->rsolver, p->eos, p->g, p->grav, p->adj, c, u_in,
-                   dt_dev, lam_out, lam_acc, lam_in, lam_n, 0);
+  return plan_adj_stage(ctx, p, sr, c, u_in, dt_dev, lam_out, lam_acc, lam_in, lam_n, 0);
 }
 
 int aph_adjoint_step(aph_context *ctx, aph_plan *p, double *u0, double *u1,
@@ -1248,8 +1281,7 @@ int aph_adjoint_step_stored(aph_context *ctx, aph_plan *p, const double *u0, con
   for (int s = 1; s <= n; ++s) adj_coef(p, s, c[s - 1], sr[s - 1]);
   auto stage = [&](int s, const double *x, const double *lo, const double *acc, double *li,
                    double *ln, int add) {
-    return adj_stage(ctx, bs, sr[s - 1], p->rsolver, p->eos, p->g, p->grav, A, c[s - 1], x,
-                     dt_dev, lo, acc, li, ln, add);
+    return plan_adj_stage(ctx, p, sr[s - 1], c[s - 1], x, dt_dev, lo, acc, li, ln, add);
   };
   if (n == 1) {  // rk1: U0' = U0 + dt L(U0); lam must not be both input and output
     APH_HIP_CHECK(ctx, hipMemcpyAsync(A.lam1, lam, sizeof(double) * bs->nvar * bs->ncells,

@@ -504,7 +504,8 @@ typedef struct aph_amr_face {
    role-2 face owns [base, base + its face cells x 2^(ndim-1) x nvar) inside the buffer,
    regions do not overlap, and each role-1 face lies inside the region of the opposite
    face of a block one level coarser.  The amr plan must outlive the plan.  The adjoint
-   entry points reject a plan with a multilevel mesh.  Synchronous. */
+   entry points run on such a plan as well (they fill the cross-level ghost cells of their
+   state inputs first, as a forward stage does).  Synchronous. */
 int aph_plan_set_amr(aph_context *ctx, aph_plan *plan, const aph_amr_plan *amr,
                      const aph_amr_face *faces, const int32_t *level,
                      int64_t buffer_doubles);
@@ -528,8 +529,10 @@ int aph_plan_set_wait_limit(aph_context *ctx, aph_plan *plan, int32_t polls);
  *                 lam_n = gam1 * lam_out
  * where J is the Jacobian of the flux-divergence (+ gravity) term with respect to the
  * stage input u_in (ConsToPrim, reconstruction through the limiter's active branch, the
- * Riemann solver's active wave branches; HLLE, HLLC and LHLLC; nvar 5, 6, 12 or 14).  Only
- * interior cells of the cotangent arrays are read or written.  lam_acc and lam_n may be
+ * Riemann solver's active wave branches; HLLE, HLLC and LHLLC; nvar 5, 6, 12 or 14; on a
+ * multilevel plan also the cross-level ghost values and the flux correction).  Only
+ * interior cells of the cotangent arrays are read or written; on a multilevel plan the
+ * cross-level ghost cells of the state inputs are filled.  lam_acc and lam_n may be
  * NULL. */
 int aph_adjoint_stage(aph_context *ctx, aph_plan *plan, int32_t stage, const double *u_in,
                       const double *dt_dev, const double *lam_out, const double *lam_acc,

@@ -132,12 +132,6 @@ def test_amr_error_paths(gpu_ctx):
     bad["qa"][r1] = bad["na"][r1] + 2  # past the end of its coarse region
     assert _lib.lib().aph_plan_set_amr(*args(bad, level, nbuf)) != 0
     assert _lib.lib().aph_plan_set_amr(*args(faces, level, nbuf - 1)) != 0
-    # and the adjoint refuses a multilevel plan
-    from apophis_amd import adjoint
-    fa = hydro.FusedDriver(gpu_ctx, p.bs, p.cons, "plm", "hllc", p.eos, p.cfl, amr_mesh=m)
-    u = torch.from_numpy(p.cons.copy()).cuda()
-    with pytest.raises(hydro._lib.HydroError, match="multilevel"):
-        adjoint.Stepper(fa, 1e-3).adjoint_step_(u, torch.zeros_like(u))
 
 
 @pytest.mark.parametrize("name,recon,riemann", [("2d_L", "plm", "hllc"),

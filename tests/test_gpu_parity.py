@@ -271,16 +271,18 @@ def test_unsupported_options_fail_loudly(gpu_ctx):
         hydro.FusedDriver(gpu_ctx, p.bs, p.cons, "ppm", "none", p.eos, p.cfl)
     u = torch.from_numpy(p.cons.copy()).cuda()
     lam = torch.zeros_like(u)
-    # the adjoint has no "none" solver (hydro switched off), and covers NV = 5, 6: the
-    # composition counts run forward only
+    # the adjoint has no "none" solver (hydro switched off); it covers the composition
+    # counts (NV = 12, 14; their gradients are checked in test_gpu_adjoint.py)
     fn = hydro.FusedDriver(gpu_ctx, p.bs, p.cons, "dc", "none", p.eos, p.cfl)
     with pytest.raises(hydro._lib.HydroError, match="unsupported"):
         adjoint.Stepper(fn, 1e-3).adjoint_step_(u, lam)
     q = problems.kh_comp(nx=(32, 32), block=(16, 16))
     fq = hydro.FusedDriver(gpu_ctx, q.bs, q.cons, q.recon, q.riemann, q.eos, q.cfl)
     uq = torch.from_numpy(q.cons.copy()).cuda()
-    with pytest.raises(hydro._lib.HydroError):
-        adjoint.Stepper(fq, 1e-3).adjoint_step_(uq, torch.zeros_like(uq))
+    lq = torch.ones_like(uq)
+    adjoint.Stepper(fq, 1e-3).adjoint_step_(uq, lq)
+    torch.cuda.synchronize()
+    assert torch.isfinite(lq).all()
     pin["parthenon/mesh"]["nghost"] = 2
     pin["hydro"]["reconstruction"] = "ppm"
     with pytest.raises(hydro.ApophisError):

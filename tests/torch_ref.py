@@ -312,7 +312,7 @@ def _pad(u, ax, G, kind, d):
         lo = u.narrow(ax, 0, 1).expand(*[G if a == ax else -1 for a in range(u.dim())])
         hi = u.narrow(ax, n - 1, 1).expand(*[G if a == ax else -1 for a in range(u.dim())])
     else:
-        sign = torch.ones(u.shape[0], 1, 1, 1, dtype=u.dtype)
+        sign = torch.ones(u.shape[0], 1, 1, 1, dtype=u.dtype, device=u.device)
         sign[1 + d] = -1.0
         lo = torch.flip(u.narrow(ax, 0, G), dims=[ax]) * sign
         hi = torch.flip(u.narrow(ax, n - G, G), dims=[ax]) * sign

@@ -322,6 +322,76 @@ def monopole_leg(ctx, hydro, problems, torch, nsteps=20, res=256):
     return out
 
 
+def evrard_smr_leg(ctx, hydro, problems, torch):
+    """The deck inputs/evrards_collapse.in on its own static-refinement mesh (32^3 base in
+    16^3 blocks, [-0.5, 0.5]^3 at level 2: 120 blocks over three levels, outflow) with the
+    monopole solver (the reference test's monopole_smr_32), batched fused form with the
+    device dt policy to t = 0.8: steps, wall time, zone-cycles/s, and the reference's
+    profile checks on the result."""
+    from tests.monopole_checks import evrard_exact, evrard_profiles, evrard_verdict
+
+    p = problems.evrard(smr_level=2)
+    cells = p.bs.interior_cells()
+    fd = hydro.FusedDriver(ctx, p.bs, p.cons, p.recon, p.riemann, p.eos, p.cfl, tlim=p.tlim,
+                           monopole=p.monopole, amr_mesh=p.meta["mesh"])
+    fd.initial_dt()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    n = 0
+    while float(fd.time.item()) < p.tlim * (1 - 1e-14):
+        fd.step()
+        n += 1
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    md = hydro.MeshData(ctx, p.bs, fd.u0.cpu().numpy())  # primitives of the final state
+    hydro.cons_to_prim(md, p.eos)
+    torch.cuda.synchronize()
+    fails, rms = evrard_verdict(evrard_profiles(p.bs, md.prim.cpu().numpy()), evrard_exact())
+    return {"blocks": p.bs.nblocks, "cells": cells, "levels": [int(v) for v in np.bincount(
+                p.meta["mesh"].level)], "steps": n, "wall_s": el,
+            "zone_cycles_per_s": cells * n / el, "profile_rms": rms,
+            "reference_checks_pass": not fails}
+
+
+def multilevel_adjoint_leg(ctx, hydro, problems, torch, reps=5):
+    """The adjoint on the two-level blast (2D PPM+HLLC, 48^2 base blocks of 32^2 with the
+    central 16^2 refined: 3.1 M cells): one forward step vs one adjoint step (stage 1
+    stored, and recomputed) at a frozen dt."""
+    from apophis_amd import adjoint
+
+    p, m = problems.blast2d_two_level()
+    fd = hydro.FusedDriver(ctx, p.bs, p.cons, p.recon, p.riemann, p.eos, p.cfl, amr_mesh=m)
+    dt = float(fd.initial_dt().item())
+    st = adjoint.Stepper(fd, dt)
+    u = torch.from_numpy(p.cons.copy()).cuda()
+    lam = torch.ones_like(u) * adjoint.interior_mask_tensor(p.bs, u.device)
+    st.step_(u)
+    st.stage1_(u)
+    s1 = st.stage1().clone()
+    st.adjoint_step_(u, lam, stage1=s1)
+    torch.cuda.synchronize()
+
+    def timed(fn):
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps * 1e3
+
+    f = timed(lambda: st.step_(u))
+    st.stage1_(u)
+    s1 = st.stage1().clone()
+    a1 = timed(lambda: st.adjoint_step_(u, lam, stage1=s1))
+    a2 = timed(lambda: st.adjoint_step_(u, lam))
+    out = {"cells": p.bs.interior_cells(), "blocks": p.bs.nblocks, "forward_step_ms": f,
+           "adjoint_step_stored_ms": a1, "adjoint_step_recompute_ms": a2,
+           "stored_over_forward": a1 / f, "recompute_over_forward": a2 / f,
+           "finite": bool(torch.isfinite(lam).all().item())}
+    del fd, st
+    torch.cuda.empty_cache()
+    return out
+
+
 def two_level_leg(ctx, hydro, problems, torch, nsteps=20):
     """A two-level mesh (2D blast, 48^2 base blocks of 32^2 with the central 16^2 refined:
     3.1 M cells, 3072 blocks): zone-cycles/s of the batched form (cross-level ghosts, fine
@@ -801,9 +871,15 @@ def main():
         print(f"# composition: {extras['kh_comp_ONe_lset']}", file=sys.stderr, flush=True)
         extras["blast_deck_adaptive"] = blast_deck_leg(ctx, hydro, problems, torch)
         print(f"# blast deck: {extras['blast_deck_adaptive']}", file=sys.stderr, flush=True)
+        extras["evrard_smr_monopole"] = evrard_smr_leg(ctx, hydro, problems, torch)
+        print(f"# evrard smr: {extras['evrard_smr_monopole']}", file=sys.stderr, flush=True)
     if not args.no_adjoint:
         extras["adjoint_kh"] = adjoint_leg(ctx, hydro, problems, torch)
         print(f"# adjoint: {extras['adjoint_kh']}", file=sys.stderr, flush=True)
+        extras["adjoint_two_level_blast2d"] = multilevel_adjoint_leg(ctx, hydro, problems,
+                                                                     torch)
+        print(f"# multilevel adjoint: {extras['adjoint_two_level_blast2d']}", file=sys.stderr,
+              flush=True)
     if not args.no_rt:
         extras["rt_batch_adjoint"] = rt_batch_leg(ctx, hydro, problems, torch)
         print(f"# rt batch: {extras['rt_batch_adjoint']}", file=sys.stderr, flush=True)

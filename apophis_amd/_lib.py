@@ -90,6 +90,11 @@ _SIGS = {
     "aph_plan_set_monopole": (_i32, [_vp, _vp, _vp]),
     "aph_poisson_create": (_i32, [_vp, _vp, _d, _d, _d, _i32, ctypes.POINTER(_vp)]),
     "aph_poisson_destroy": (_i32, [_vp, _vp]),
+    "aph_poisson_create_tree": (_i32, [_vp, _vp, _vp, _vp, _vp, _i32, _i32, _d, _d, _d, _i32,
+                                       ctypes.POINTER(_vp)]),
+    "aph_poisson_tree_matrix": (_i32, [_vp, _i32, _vp, _vp, _vp, _i32, _i32, _d, _vp, _vp, _vp,
+                                       _vp, _vp, _vp, _vp, _vp, ctypes.c_char_p,
+                                       ctypes.c_size_t]),
     "aph_poisson_solve": (_i32, [_vp, _vp, _vp, _vp, _vp]),
     "aph_poisson_get_info": (_i32, [_vp, _vp, _vp, ctypes.POINTER(_i32),
                                     ctypes.POINTER(_d)]),

@@ -711,12 +711,49 @@ def native_plan(mesh: AmrMesh, bs: BlockSet):
 _PLAN_HINT = {}  # (ndim, nvar, nx, ng) -> list lengths of the last native build
 
 
+def poisson_matrix(mesh: AmrMesh, bs: BlockSet, flux_correct=False, prolongation=0,
+                   alpha=0.0):
+    """The composite Poisson operator the device solver of a multilevel mesh uses, from the
+    library's host assembly (aph_poisson_tree_matrix): (rowptr, col, val) over the leaf
+    cells and (nb_rowptr, nb_col, nb_val), the six neighbour rows per cell."""
+    L = _lib.lib()
+    vp = ctypes.c_void_p
+    blocks = np.ascontiguousarray(bs.blocks)
+    nbase = np.array(mesh.nbase, dtype=np.int32)
+    level = np.ascontiguousarray(mesh.level, dtype=np.int32)
+    loc = np.ascontiguousarray(mesh.loc, dtype=np.int64)
+    err = ctypes.create_string_buffer(512)
+    nnz, nbn = ctypes.c_int64(0), ctypes.c_int64(0)
+    n = bs.nblocks * int(np.prod(mesh.nx))
+    head = (blocks.ctypes.data_as(vp), bs.nblocks, nbase.ctypes.data_as(vp),
+            level.ctypes.data_as(vp), loc.ctypes.data_as(vp), int(bool(flux_correct)),
+            int(prolongation), float(alpha))
+    rc = L.aph_poisson_tree_matrix(*head, None, None, None, ctypes.byref(nnz), None, None,
+                                   None, ctypes.byref(nbn), err, len(err))
+    if rc != _lib.APH_OK:
+        raise _lib.HydroError(rc, err.value.decode())
+    rowptr = np.zeros(n + 1, dtype=np.int64)
+    col = np.zeros(nnz.value, dtype=np.int32)
+    val = np.zeros(nnz.value)
+    nbptr = np.zeros(6 * n + 1, dtype=np.int64)
+    nbcol = np.zeros(nbn.value, dtype=np.int32)
+    nbval = np.zeros(nbn.value)
+    rc = L.aph_poisson_tree_matrix(*head, *(a.ctypes.data_as(vp) for a in (rowptr, col, val)),
+                                   ctypes.byref(nnz),
+                                   *(a.ctypes.data_as(vp) for a in (nbptr, nbcol, nbval)),
+                                   ctypes.byref(nbn), err, len(err))
+    if rc != _lib.APH_OK:
+        raise _lib.HydroError(rc, err.value.decode())
+    return rowptr, col, val, nbptr, nbcol, nbval
+
+
 class AmrPlan:
     """The plan on the device (aph_amr_plan_create), its lists built by the library's
     tree (aph_amr_build)."""
 
     def __init__(self, ctx, dbs, mesh: AmrMesh, bs: BlockSet):
         self.ctx = ctx
+        self.mesh = mesh
         self.ops, self.fixes, self.refs = native_plan(mesh, bs)
         self.h = ctypes.c_void_p()
         vp = ctypes.c_void_p

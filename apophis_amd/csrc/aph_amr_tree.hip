@@ -852,3 +852,334 @@ extern "C" int aph_amr_remap(int32_t ndim, const int32_t nbase[3], const int32_t
   if (err && err_len) err[0] = 0;
   return APH_OK;
 }
+
+// ------------------------------------------------------------------ Poisson operator
+// PoissonEquation (src/gravity/gsolvers/poisson_gravity.hpp:31-388, D = 1) on the leaf
+// cells of a 3D multilevel mesh whose every domain face is physical:
+//   F = (phi_low - phi_high) / dx through each face of a block (CalculateFluxes,
+//   :168-224), the value beyond a block face taken from the same level (the leaf cell, or
+//   the mean of the 2^3 children of a refined cell: RestrictAverage) or prolongated from
+//   the coarser leaf beyond (the phi field's refinement op, gravity.cpp:343-363: "Linear"
+//   is ProlongateSharedLinear, c + sum_d (+-1/4) (c_+d - c_-d) / 2 on the uniform coarse
+//   level; "Constant", c), and minus the cell inside beyond a domain face (the FixedFace
+//   zero boundary, SetZeroDirichletBoundary, on the coarse level too);
+//   with flux correction (:89-99) a coarse face over a finer region carries the mean of
+//   its 2^2 fine faces;
+//   (A phi)_c = -alpha phi_c + sum_d [F(c - 1/2) - F(c + 1/2)] A_d / V (FluxMultiplyMatrix,
+//   :299-349), each block with its own dx, areas and volume.
+// Rows are accumulated term by term in a fixed order (first appearance of a column keeps
+// its place), so the coefficients do not depend on hashing.
+#include "aph_poisson_tree.hpp"
+
+namespace {
+
+using Row = std::vector<std::pair<int64_t, double>>;
+
+void acc(Row &r, int64_t c, double w) {
+  for (auto &e : r)
+    if (e.first == c) {
+      e.second += w;
+      return;
+    }
+  r.emplace_back(c, w);
+}
+
+struct PoissonTree {
+  Tree t;
+  int64_t ncb = 0;
+  bool fc = false, constant = false;
+
+  int64_t lin(int b, const int64_t l[3]) const {  // l: ghost-inclusive local index
+    const int g = t.ng;
+    return (int64_t)b * ncb + ((l[2] - g) * t.nx[1] + (l[1] - g)) * t.nx[0] + (l[0] - g);
+  }
+  int64_t lin_int(int b, int i, int j, int k) const {
+    return (int64_t)b * ncb + ((int64_t)k * t.nx[1] + j) * t.nx[0] + i;
+  }
+  // level-lv value of global cell G (inside the domain): the leaf cell, or the mean of its
+  // children over a refined cell; false under a coarser leaf
+  bool value(int lv, const int64_t G[3], Row &out) const {
+    out.clear();
+    int b;
+    int64_t l[3];
+    if (t.interior(lv, G, b, l)) {
+      out.emplace_back(lin(b, l), 1.0);
+      return true;
+    }
+    const int64_t node[3] = {G[0] / t.nx[0], G[1] / t.nx[1], G[2] / t.nx[2]};
+    if (!t.is_refined(lv, node)) return false;
+    Row kid;
+    for (int cz = 0; cz < 2; ++cz)
+      for (int cy = 0; cy < 2; ++cy)
+        for (int cx = 0; cx < 2; ++cx) {
+          const int64_t F[3] = {2 * G[0] + cx, 2 * G[1] + cy, 2 * G[2] + cz};
+          if (!value(lv + 1, F, kid))
+            fail("cell (%lld, %lld, %lld) of level %d is neither a leaf cell nor the parent "
+                 "of leaf cells (2:1 balance)", (long long)G[0], (long long)G[1],
+                 (long long)G[2], lv);
+          for (auto &e : kid) acc(out, e.first, 0.125 * e.second);
+        }
+    return true;
+  }
+  // the value at global cell G of level lv seen from a block of level lv (G beyond it);
+  // inner: the cell inside next to G (for the domain-face image)
+  void ghost(int lv, const int64_t G[3], int64_t inner, Row &out) const {
+    out.clear();
+    if (!t.inside(lv, G)) {
+      out.emplace_back(inner, -1.0);
+      return;
+    }
+    if (value(lv, G, out)) return;
+    const int64_t I[3] = {G[0] / 2, G[1] / 2, G[2] / 2};
+    Row c;
+    if (lv == 0 || !value(lv - 1, I, c))
+      fail("cell (%lld, %lld, %lld) of level %d is over neither a finer region nor a coarser "
+           "leaf (2:1 balance)", (long long)G[0], (long long)G[1], (long long)G[2], lv);
+    out = c;
+    if (constant) return;
+    Row nb[2];
+    for (int d = 0; d < 3; ++d) {
+      const double sign = G[d] % 2 == 0 ? -1.0 : 1.0;
+      for (int s = 0; s < 2; ++s) {
+        int64_t J[3] = {I[0], I[1], I[2]};
+        J[d] += s ? 1 : -1;
+        if (t.inside(lv - 1, J)) {
+          value(lv - 1, J, nb[s]);
+        } else {
+          nb[s].clear();
+          for (auto &e : c) nb[s].emplace_back(e.first, -e.second);
+        }
+      }
+      // (hi - lo) / 2 * (+-1/4)
+      for (auto &e : nb[1]) acc(out, e.first, sign * 0.25 * 0.5 * 1.0 * e.second);
+      for (auto &e : nb[0]) acc(out, e.first, sign * 0.25 * 0.5 * -1.0 * e.second);
+    }
+  }
+  // the value at interior-or-ghost index c along d (transverse interior a, b) of block b
+  void cell(int b, int d, int64_t c, const int tt[2], Row &out) const {
+    const int A = d == 0 ? 1 : 0, B = d == 2 ? 1 : 2;
+    int64_t idx[3];
+    idx[d] = c;
+    idx[A] = tt[0];
+    idx[B] = tt[1];
+    if (c >= 0 && c < t.nx[d]) {
+      out.clear();
+      out.emplace_back(lin_int(b, (int)idx[0], (int)idx[1], (int)idx[2]), 1.0);
+      return;
+    }
+    const int lv = t.level[b];
+    int64_t G[3];
+    for (int q = 0; q < 3; ++q) G[q] = t.loc[3 * b + q] * t.nx[q] + idx[q];
+    int64_t in[3] = {idx[0], idx[1], idx[2]};
+    in[d] = c < 0 ? 0 : t.nx[d] - 1;
+    ghost(lv, G, lin_int(b, (int)in[0], (int)in[1], (int)in[2]), out);
+  }
+  // the plain flux through face f (0..nx) of block b along d at transverse (a, b)
+  void face_flux(int b, int d, int f, const int tt[2], Row &out) const {
+    Row lo, hi;
+    cell(b, d, f - 1, tt, lo);
+    cell(b, d, f, tt, hi);
+    out = lo;
+    for (auto &e : hi) acc(out, e.first, -e.second);
+    const double inv = 1.0 / t.blk[b].dx[d];
+    for (auto &e : out) e.second *= inv;
+  }
+  // the flux the operator uses: on a block face over a refined node with flux correction,
+  // the mean of the fine faces across it
+  void flux(int b, int d, int f, const int tt[2], Row &out) const {
+    if (fc && (f == 0 || f == t.nx[d])) {
+      const int side = f == t.nx[d];
+      const int lv = t.level[b];
+      int64_t p[3] = {t.loc[3 * b], t.loc[3 * b + 1], t.loc[3 * b + 2]};
+      p[d] += side ? 1 : -1;
+      if (p[d] >= 0 && p[d] < t.nlat(lv, d) && t.is_refined(lv, p)) {
+        const int A = d == 0 ? 1 : 0, B = d == 2 ? 1 : 2;
+        out.clear();
+        Row fine;
+        for (int ka = 0; ka < 2; ++ka)
+          for (int kb = 0; kb < 2; ++kb) {
+            int64_t G[3];
+            G[A] = 2 * (t.loc[3 * b + A] * t.nx[A] + tt[0]) + ka;
+            G[B] = 2 * (t.loc[3 * b + B] * t.nx[B] + tt[1]) + kb;
+            G[d] = 2 * (t.loc[3 * b + d] * t.nx[d] + f) - (side ? 0 : 1);
+            int fb;
+            int64_t fl[3];
+            if (!t.interior(lv + 1, G, fb, fl))
+              fail("block %d face %d: fine face outside the finer level", b, 2 * d + side);
+            const int ff = (int)(fl[d] - t.ng) + (side ? 0 : 1);
+            const int ft[2] = {(int)(fl[A] - t.ng), (int)(fl[B] - t.ng)};
+            face_flux(fb, d, ff, ft, fine);
+            for (auto &e : fine) acc(out, e.first, 0.25 * e.second);
+          }
+        return;
+      }
+    }
+    face_flux(b, d, f, tt, out);
+  }
+};
+
+void poisson_assemble(PoissonTree &pt, double alpha, aph::PoissonTreeCsr &out) {
+  const Tree &t = pt.t;
+  const int64_t n = (int64_t)t.nb * pt.ncb;
+  if (n > INT32_MAX) fail("too many cells for 32-bit column indices");
+  out.n = n;
+  out.rowptr.assign(1, 0);
+  out.col.clear();
+  out.val.clear();
+  Row row, f;
+  for (int b = 0; b < t.nb; ++b) {
+    const aph_block &blk = t.blk[b];
+    for (int k = 0; k < t.nx[2]; ++k)
+      for (int j = 0; j < t.nx[1]; ++j)
+        for (int i = 0; i < t.nx[0]; ++i) {
+          const int idx[3] = {i, j, k};
+          row.clear();
+          row.emplace_back(pt.lin_int(b, i, j, k), -alpha);
+          for (int d = 0; d < 3; ++d) {
+            const int A = d == 0 ? 1 : 0, B = d == 2 ? 1 : 2;
+            const double ar = blk.area[d] / blk.vol;
+            const int tt[2] = {idx[A], idx[B]};
+            pt.flux(b, d, idx[d], tt, f);
+            for (auto &e : f) acc(row, e.first, ar * e.second);
+            pt.flux(b, d, idx[d] + 1, tt, f);
+            for (auto &e : f) acc(row, e.first, -ar * e.second);
+          }
+          for (auto &e : row) {
+            out.col.push_back((int32_t)e.first);
+            out.val.push_back(e.second);
+          }
+          out.rowptr.push_back((int64_t)out.col.size());
+        }
+  }
+  // neighbour rows, (2 d + side) * n + c
+  out.nb_rowptr.assign(1, 0);
+  out.nb_col.clear();
+  out.nb_val.clear();
+  for (int d = 0; d < 3; ++d)
+    for (int s = 0; s < 2; ++s)
+      for (int b = 0; b < t.nb; ++b)
+        for (int k = 0; k < t.nx[2]; ++k)
+          for (int j = 0; j < t.nx[1]; ++j)
+            for (int i = 0; i < t.nx[0]; ++i) {
+              int q[3] = {i, j, k};
+              const int64_t r = pt.lin_int(b, i, j, k);
+              q[d] += s ? 1 : -1;
+              if (q[d] >= 0 && q[d] < t.nx[d]) {
+                out.nb_col.push_back((int32_t)pt.lin_int(b, q[0], q[1], q[2]));
+                out.nb_val.push_back(1.0);
+              } else {
+                int64_t G[3];
+                for (int a = 0; a < 3; ++a) G[a] = t.loc[3 * b + a] * t.nx[a] + q[a];
+                if (t.inside(t.level[b], G)) {
+                  pt.ghost(t.level[b], G, r, f);
+                  for (auto &e : f) {
+                    out.nb_col.push_back((int32_t)e.first);
+                    out.nb_val.push_back(e.second);
+                  }
+                }
+              }
+              out.nb_rowptr.push_back((int64_t)out.nb_col.size());
+            }
+  out.wall.assign(t.nb, 0);
+  for (int b = 0; b < t.nb; ++b)
+    for (int d = 0; d < 3; ++d) {
+      if (t.loc[3 * b + d] == 0) out.wall[b] |= 1 << (2 * d);
+      if (t.loc[3 * b + d] == t.nlat(t.level[b], d) - 1) out.wall[b] |= 1 << (2 * d + 1);
+    }
+}
+
+}  // namespace
+
+int aph::poisson_tree_csr(const aph_block *blocks, int32_t nblocks, const int32_t nbase[3],
+                          const int32_t *level, const int64_t *loc, int32_t flux_correct,
+                          int32_t prolongation, double alpha, PoissonTreeCsr &out, char *err,
+                          size_t err_len) {
+  auto report = [&](int code, const char *msg) {
+    if (err && err_len) snprintf(err, err_len, "%s", msg);
+    return code;
+  };
+  if (prolongation != APH_POISSON_PROLONG_LINEAR && prolongation != APH_POISSON_PROLONG_CONSTANT)
+    return report(APH_E_INVALID, "prolongation: APH_POISSON_PROLONG_LINEAR or _CONSTANT (the "
+                                 "quadratic form is not built)");
+  try {
+    PoissonTree pt;
+    const int32_t walls[6] = {APH_BC_OUTFLOW, APH_BC_OUTFLOW, APH_BC_OUTFLOW,
+                              APH_BC_OUTFLOW, APH_BC_OUTFLOW, APH_BC_OUTFLOW};
+    setup(pt.t, blocks, nblocks, 3, 1, nbase, level, loc, walls);
+    pt.ncb = (int64_t)pt.t.nx[0] * pt.t.nx[1] * pt.t.nx[2];
+    pt.fc = flux_correct != 0;
+    pt.constant = prolongation == APH_POISSON_PROLONG_CONSTANT;
+    // the block geometry must be the tree's: one base spacing, halved per level, and each
+    // block's corner where its logical position puts it
+    double lo[3], dx0[3];
+    for (int d = 0; d < 3; ++d) {
+      dx0[d] = blocks[0].dx[d] * (double)((int64_t)1 << level[0]);
+      lo[d] = blocks[0].xmin[d] - (double)loc[d] * pt.t.nx[d] * blocks[0].dx[d];
+    }
+    for (int b = 0; b < nblocks; ++b)
+      for (int d = 0; d < 3; ++d) {
+        const double dx = dx0[d] / (double)((int64_t)1 << level[b]);
+        const double x = lo[d] + (double)loc[3 * b + d] * pt.t.nx[d] * dx;
+        if (std::fabs(blocks[b].dx[d] - dx) > 1e-12 * dx ||
+            std::fabs(blocks[b].xmin[d] - x) > 1e-9 * dx)
+          fail("block %d: spacing or corner does not match its level and logical position", b);
+      }
+    // a block on a domain face must have no neighbour across it (the periodic problem
+    // needs a mean density subtracted; the tree itself has no wrap)
+    for (int b = 0; b < nblocks; ++b)
+      for (int d = 0; d < 3; ++d)
+        for (int side = 0; side < 2; ++side) {
+          const int64_t edge = side ? pt.t.nlat(level[b], d) - 1 : 0;
+          int o[3] = {0, 0, 0};
+          o[d] = side ? 1 : -1;
+          const int slot = (o[2] + 1) * 9 + (o[1] + 1) * 3 + o[0] + 1;
+          if (loc[3 * b + d] == edge && blocks[b].nbr[slot] >= 0)
+            fail("block %d: every face of the domain must be a physical boundary", b);
+        }
+    poisson_assemble(pt, alpha, out);
+  } catch (const BuildError &e) {
+    return report(APH_E_INVALID, e.msg);
+  } catch (const std::bad_alloc &) {
+    return report(APH_E_NOMEM, "host allocation failed");
+  }
+  if (err && err_len) err[0] = 0;
+  return APH_OK;
+}
+
+extern "C" int aph_poisson_tree_matrix(const aph_block *blocks, int32_t nblocks,
+                                       const int32_t nbase[3], const int32_t *level,
+                                       const int64_t *loc, int32_t flux_correct,
+                                       int32_t prolongation, double diagonal_alpha,
+                                       int64_t *rowptr, int32_t *col, double *val,
+                                       int64_t *nnz, int64_t *nb_rowptr, int32_t *nb_col,
+                                       double *nb_val, int64_t *nb_nnz, char *err,
+                                       size_t err_len) {
+  if (!nnz || !nb_nnz) {
+    if (err && err_len) snprintf(err, err_len, "aph_poisson_tree_matrix: null length");
+    return APH_E_INVALID;
+  }
+  aph::PoissonTreeCsr m;
+  char msg[256];
+  const int rc = aph::poisson_tree_csr(blocks, nblocks, nbase, level, loc, flux_correct,
+                                       prolongation, diagonal_alpha, m, msg, sizeof msg);
+  if (rc) {
+    if (err && err_len) snprintf(err, err_len, "aph_poisson_tree_matrix: %s", msg);
+    return rc;
+  }
+  const int64_t cap[2] = {*nnz, *nb_nnz};
+  *nnz = (int64_t)m.col.size();
+  *nb_nnz = (int64_t)m.nb_col.size();
+  if ((col && cap[0] < *nnz) || (nb_col && cap[1] < *nb_nnz)) {
+    if (err && err_len) snprintf(err, err_len, "aph_poisson_tree_matrix: arrays too short");
+    return APH_E_INVALID;
+  }
+  if (rowptr) std::memcpy(rowptr, m.rowptr.data(), sizeof(int64_t) * m.rowptr.size());
+  if (col) std::memcpy(col, m.col.data(), sizeof(int32_t) * m.col.size());
+  if (val) std::memcpy(val, m.val.data(), sizeof(double) * m.val.size());
+  if (nb_rowptr)
+    std::memcpy(nb_rowptr, m.nb_rowptr.data(), sizeof(int64_t) * m.nb_rowptr.size());
+  if (nb_col) std::memcpy(nb_col, m.nb_col.data(), sizeof(int32_t) * m.nb_col.size());
+  if (nb_val) std::memcpy(nb_val, m.nb_val.data(), sizeof(double) * m.nb_val.size());
+  if (err && err_len) err[0] = 0;
+  return APH_OK;
+}

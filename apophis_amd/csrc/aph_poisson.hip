@@ -20,8 +20,10 @@
 // Scalars of the iteration live on the device; once converged every later kernel of the
 // iteration returns at once, and the host reads the flag every few iterations.
 #include <climits>
+#include <type_traits>
 
 #include "aph_device.hpp"
+#include "aph_poisson_tree.hpp"
 #include "aph_runtime.hpp"
 
 namespace {
@@ -45,6 +47,14 @@ struct aph_poisson {
   double *coef = nullptr;  // nine multipole coefficients
   double *bmom = nullptr;  // per-block moments
   double *h_sc = nullptr;  // pinned host copy of the scalars
+  // multilevel mesh (aph_poisson_create_tree): the operator as CSR over the leaf cells,
+  // BiCGSTAB with the Jacobi preconditioner
+  bool tree = false;
+  int64_t nt = 0, ncb = 0;                  // leaf cells, cells per block
+  int64_t *d_ptr = nullptr, *d_nbptr = nullptr;
+  int32_t *d_col = nullptr, *d_nbcol = nullptr, *d_wall = nullptr;
+  double *d_val = nullptr, *d_nbval = nullptr, *d_dinv = nullptr;
+  double *rh = nullptr, *v = nullptr, *y = nullptr, *s = nullptr, *tv = nullptr;
 };
 
 namespace aph {
@@ -54,8 +64,10 @@ constexpr int kThreads = 256;
 constexpr double kOmega = 0.8;  // weighted Jacobi (two sweeps before and after)
 constexpr int kCoarseSweeps = 40;
 
-// device scalars
+// device scalars (conjugate gradients; BiCGSTAB reuses RR, BB, ALPHA, DONE and ITERS and
+// keeps its own in the other slots)
 enum Sc { RZ = 0, PQ, RR, BB, ALPHA, BETA, RZN, DONE, ITERS, NSC };
+enum ScBi { RHO = RZ, RV = PQ, OMEGA = BETA, RHO_OLD = RZN, TS = NSC, TT, RHO_NEW, FAILED, NSC_ALL };
 
 struct Grid {
   int nx, ny, nz;
@@ -435,6 +447,334 @@ void vcycle(aph_context *ctx, aph_poisson *ps, int l, const double *r, double *e
   hipLaunchKernelGGL(k_jacobi, gr, dim3(kThreads), 0, ctx->stream, g, ps->sc, r, t, e);
 }
 
+// ------------------------------------------------------------ multilevel meshes
+// The composite operator of a multilevel mesh as CSR over the leaf cells (block order,
+// cell (i, j, k) of block b at b ncb + (k ny + j) nx + i), assembled on the host from the
+// tree.  BiCGSTAB (the reference's MGBiCGSTAB without the multigrid preconditioner; the
+// operator is not symmetric once prolongation enters it) with x0 = 0, right-preconditioned
+// by the inverse diagonal, each dot product fused into the kernel that produces its
+// operand, reduced in two fixed-order passes.
+struct Csr {
+  const int64_t *ptr;
+  const int32_t *col;
+  const double *val;
+};
+
+__device__ __forceinline__ double row_dot(const Csr &A, int64_t r, const double *x) {
+  double s = 0.0;
+  for (int64_t q = A.ptr[r], e = A.ptr[r + 1]; q < e; ++q) s += A.val[q] * x[A.col[q]];
+  return s;
+}
+
+// K per-thread values into part[k * kRedBlocks + blockIdx.x]
+template <int K>
+__device__ __forceinline__ void block_partials(double (&v)[K], double *part) {
+  __shared__ double sh[K][kThreads];
+  for (int k = 0; k < K; ++k) sh[k][threadIdx.x] = v[k];
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int k = 0; k < K; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int k = 0; k < K; ++k) part[k * kRedBlocks + blockIdx.x] = sh[k][0];
+}
+
+#define APH_VEC_LOOP(n, c)                                                                 \
+  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < (n);               \
+       c += (int64_t)gridDim.x * blockDim.x)
+
+// x = 0, r = rh = b, p = v = 0; partial (b, b)
+__global__ void __launch_bounds__(kThreads) k_bi_init(int64_t n, const double *b, double *x,
+                                                      double *r, double *rh, double *p,
+                                                      double *v, double *part) {
+  double a[1] = {0.0};
+  APH_VEC_LOOP(n, c) {
+    const double bc = b[c];
+    x[c] = 0.0;
+    r[c] = bc;
+    rh[c] = bc;
+    p[c] = 0.0;
+    v[c] = 0.0;
+    a[0] += bc * bc;
+  }
+  block_partials<1>(a, part);
+}
+
+// phase 0: start; 1: alpha = rho / (rh, v); 2: omega = (t, s) / (t, t); 3: the residual test
+// and rho for the next iteration
+__global__ void __launch_bounds__(kThreads) k_bi_final(int nparts, int nk, int phase,
+                                                       double tol2, double *sc,
+                                                       const double *part) {
+  __shared__ double sh[2][kThreads];
+  if (phase > 0 && finished(sc)) return;
+  for (int k = 0; k < nk; ++k) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) v += part[k * kRedBlocks + i];
+    sh[k][threadIdx.x] = v;
+  }
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int k = 0; k < nk; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x) return;
+  const double p0 = sh[0][0], p1 = nk > 1 ? sh[1][0] : 0.0;
+  switch (phase) {
+    case 0:
+      for (int i = 0; i < NSC_ALL; ++i) sc[i] = 0.0;
+      sc[BB] = p0;
+      sc[RR] = p0;
+      sc[RHO] = p0;
+      sc[RHO_OLD] = sc[ALPHA] = sc[OMEGA] = 1.0;
+      if (p0 == 0.0) sc[DONE] = 1.0;  // a zero right-hand side is solved by x = 0
+      break;
+    case 1:
+      sc[RV] = p0;
+      if (p0 == 0.0) {  // breakdown
+        sc[FAILED] = 1.0;
+        sc[DONE] = 1.0;
+      } else {
+        sc[ALPHA] = sc[RHO] / p0;
+      }
+      break;
+    case 2:
+      sc[TS] = p0;
+      sc[TT] = p1;
+      if (p1 == 0.0) {  // t = A z = 0: s = 0, x is the solution
+        sc[RR] = 0.0;
+        sc[ITERS] += 1.0;
+        sc[DONE] = 1.0;
+      } else {
+        sc[OMEGA] = p0 / p1;
+      }
+      break;
+    default:
+      sc[RR] = p0;
+      sc[RHO_NEW] = p1;
+      sc[ITERS] += 1.0;
+      sc[RHO_OLD] = sc[RHO];
+      sc[RHO] = p1;
+      if (p0 <= tol2 * sc[BB]) {
+        sc[DONE] = 1.0;
+      } else if (p1 == 0.0 || sc[OMEGA] == 0.0) {  // breakdown
+        sc[FAILED] = 1.0;
+        sc[DONE] = 1.0;
+      }
+      break;
+  }
+}
+
+// p = r + beta (p - omega v), y = p / diag
+__global__ void __launch_bounds__(kThreads) k_bi_p(int64_t n, const double *sc, const double *r,
+                                                   const double *v, const double *dinv,
+                                                   double *p, double *y) {
+  if (finished(sc)) return;
+  const double om = sc[OMEGA];
+  const double beta = (sc[RHO] / sc[RHO_OLD]) * (sc[ALPHA] / om);
+  APH_VEC_LOOP(n, c) {
+    const double pc = r[c] + beta * (p[c] - om * v[c]);
+    p[c] = pc;
+    y[c] = pc * dinv[c];
+  }
+}
+
+// v = A y; partial (rh, v)
+__global__ void __launch_bounds__(kThreads) k_bi_av(int64_t n, Csr A, const double *sc,
+                                                    const double *y, const double *rh,
+                                                    double *v, double *part) {
+  if (finished(sc)) return;
+  double a[1] = {0.0};
+  APH_VEC_LOOP(n, c) {
+    const double vc = row_dot(A, c, y);
+    v[c] = vc;
+    a[0] += rh[c] * vc;
+  }
+  block_partials<1>(a, part);
+}
+
+// s = r - alpha v, x += alpha y, z = s / diag
+__global__ void __launch_bounds__(kThreads) k_bi_s(int64_t n, const double *sc, const double *r,
+                                                   const double *v, const double *y,
+                                                   const double *dinv, double *x, double *s,
+                                                   double *z) {
+  if (finished(sc)) return;
+  const double al = sc[ALPHA];
+  APH_VEC_LOOP(n, c) {
+    const double sv = r[c] - al * v[c];
+    s[c] = sv;
+    x[c] += al * y[c];
+    z[c] = sv * dinv[c];
+  }
+}
+
+// t = A z; partials (t, s), (t, t)
+__global__ void __launch_bounds__(kThreads) k_bi_at(int64_t n, Csr A, const double *sc,
+                                                    const double *z, const double *s,
+                                                    double *t, double *part) {
+  if (finished(sc)) return;
+  double a[2] = {0.0, 0.0};
+  APH_VEC_LOOP(n, c) {
+    const double tc = row_dot(A, c, z);
+    t[c] = tc;
+    a[0] += tc * s[c];
+    a[1] += tc * tc;
+  }
+  block_partials<2>(a, part);
+}
+
+// x += omega z, r = s - omega t; partials (r, r), (rh, r)
+__global__ void __launch_bounds__(kThreads) k_bi_x(int64_t n, const double *sc, const double *z,
+                                                   const double *s, const double *t,
+                                                   const double *rh, double *x, double *r,
+                                                   double *part) {
+  if (finished(sc)) return;
+  const double om = sc[OMEGA];
+  double a[2] = {0.0, 0.0};
+  APH_VEC_LOOP(n, c) {
+    x[c] += om * z[c];
+    const double rc = s[c] - om * t[c];
+    r[c] = rc;
+    a[0] += rc * rc;
+    a[1] += rh[c] * rc;
+  }
+  block_partials<2>(a, part);
+}
+
+__device__ __forceinline__ double face_potential(const double *coef, const aph_block &bk,
+                                                 int d, int side, double x, double y,
+                                                 double z) {
+  double p[3] = {x, y, z};
+  p[d] = bk.xmin[d] + (side ? bk.nx[d] : 0) * bk.dx[d];
+  return multipole(coef, p[0], p[1], p[2]);
+}
+
+// ComputeRhs over the leaf cells: 4 pi G rho minus 2 phi_b A_f / (dx V) on every face on
+// the domain boundary
+__global__ void __launch_bounds__(kThreads)
+    k_trhs(const aph_block *__restrict__ blocks, int nvar, const double *field, double dfloor,
+           const int32_t *__restrict__ wall, double four_pi_g, const double *coef,
+           int64_t ncb, double *b) {
+  const aph_block &bk = blocks[blockIdx.y];
+  const int N0 = bk.nx[0] + 2 * bk.ng, N1 = bk.nx[1] + 2 * bk.ng;
+  const int wm = wall[blockIdx.y];
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < ncb;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(t % bk.nx[0]), j = (int)((t / bk.nx[0]) % bk.nx[1]),
+              k = (int)(t / ((int64_t)bk.nx[0] * bk.nx[1]));
+    const int64_t cell = ((int64_t)(k + bk.ng) * N1 + (j + bk.ng)) * N0 + (i + bk.ng);
+    double rho = field[(int64_t)nvar * bk.cell_base + cell];
+    rho = (rho > dfloor) ? rho : dfloor;
+    double v = four_pi_g * rho;
+    if (wm) {
+      const double x = bk.xmin[0] + (i + 0.5) * bk.dx[0];
+      const double y = bk.xmin[1] + (j + 0.5) * bk.dx[1];
+      const double z = bk.xmin[2] + (k + 0.5) * bk.dx[2];
+      const int ijk[3] = {i, j, k};
+      for (int d = 0; d < 3; ++d)
+        for (int side = 0; side < 2; ++side)
+          if ((wm >> (2 * d + side) & 1) && ijk[d] == (side ? bk.nx[d] - 1 : 0))
+            v -= 2.0 * face_potential(coef, bk, d, side, x, y, z) *
+                 (bk.area[d] / (bk.dx[d] * bk.vol));
+    }
+    b[(int64_t)blockIdx.y * ncb + t] = v;
+  }
+}
+
+// ComputeGravityVector: g = -grad phi, centred, the neighbours as the operator sees them,
+// 2 phi_b - phi on the domain faces; phi and g into the pack layout
+__global__ void __launch_bounds__(kThreads)
+    k_tfield(const aph_block *__restrict__ blocks, const int32_t *__restrict__ wall, Csr NB,
+             int64_t n, int64_t ncb, const double *coef, const double *x, double *grav,
+             double *phi) {
+  const aph_block &bk = blocks[blockIdx.y];
+  const int N0 = bk.nx[0] + 2 * bk.ng, N1 = bk.nx[1] + 2 * bk.ng, N2 = bk.nx[2] + 2 * bk.ng;
+  const int64_t nc = (int64_t)N0 * N1 * N2;
+  const int wm = wall[blockIdx.y];
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < ncb;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(t % bk.nx[0]), j = (int)((t / bk.nx[0]) % bk.nx[1]),
+              k = (int)(t / ((int64_t)bk.nx[0] * bk.nx[1]));
+    const int ijk[3] = {i, j, k};
+    const int64_t c = (int64_t)blockIdx.y * ncb + t;
+    const double pc = x[c];
+    const double cx = bk.xmin[0] + (i + 0.5) * bk.dx[0];
+    const double cy = bk.xmin[1] + (j + 0.5) * bk.dx[1];
+    const double cz = bk.xmin[2] + (k + 0.5) * bk.dx[2];
+    const int64_t cell = ((int64_t)(k + bk.ng) * N1 + (j + bk.ng)) * N0 + (i + bk.ng);
+    double *gp = grav + 3 * bk.cell_base + cell;
+    for (int d = 0; d < 3; ++d) {
+      double nbv[2];
+      for (int side = 0; side < 2; ++side)
+        nbv[side] = ((wm >> (2 * d + side) & 1) && ijk[d] == (side ? bk.nx[d] - 1 : 0))
+                        ? 2.0 * face_potential(coef, bk, d, side, cx, cy, cz) - pc
+                        : row_dot(NB, (int64_t)(2 * d + side) * n + c, x);
+      gp[d * nc] = -((nbv[1] - nbv[0]) / (2.0 * bk.dx[d]));
+    }
+    if (phi) phi[bk.cell_base + cell] = pc;
+  }
+}
+
+int tree_pass(aph_context *ctx, aph_poisson *ps, const double *field, double dfloor,
+              double *grav, double *phi) {
+  const aph_blockset *bs = ps->bs;
+  hipStream_t st = ctx->stream;
+  hipLaunchKernelGGL(k_pmom_block, dim3((unsigned)bs->nblocks), dim3(kThreads), 0, st, bs->d,
+                     bs->nvar, field, dfloor, ps->bmom);
+  hipLaunchKernelGGL(k_pmom_final, dim3(1), dim3(64), 0, st, bs->nblocks, ps->bmom, ps->G,
+                     ps->coef);
+  const dim3 cg((unsigned)std::max<int64_t>(1, std::min<int64_t>(64, (ps->ncb + kThreads - 1) /
+                                                                         kThreads)),
+                (unsigned)bs->nblocks);
+  hipLaunchKernelGGL(k_trhs, cg, dim3(kThreads), 0, st, bs->d, bs->nvar, field, dfloor,
+                     ps->d_wall, 4.0 * M_PI * ps->G, ps->coef, ps->ncb, ps->b);
+  const int64_t n = ps->nt;
+  const int parts =
+      (int)std::max<int64_t>(1, std::min<int64_t>((n + kThreads - 1) / kThreads, kRedBlocks));
+  const dim3 gr((unsigned)parts), bl(kThreads);
+  const Csr A{ps->d_ptr, ps->d_col, ps->d_val};
+  const double tol2 = ps->tol * ps->tol;
+  hipLaunchKernelGGL(k_bi_init, gr, bl, 0, st, n, ps->b, ps->x, ps->r, ps->rh, ps->p, ps->v,
+                     ps->part);
+  hipLaunchKernelGGL(k_bi_final, dim3(1), bl, 0, st, parts, 1, 0, tol2, ps->sc, ps->part);
+  bool converged = false;
+  for (int it = 0; it < ps->max_it; ++it) {
+    hipLaunchKernelGGL(k_bi_p, gr, bl, 0, st, n, ps->sc, ps->r, ps->v, ps->d_dinv, ps->p, ps->y);
+    hipLaunchKernelGGL(k_bi_av, gr, bl, 0, st, n, A, ps->sc, ps->y, ps->rh, ps->v, ps->part);
+    hipLaunchKernelGGL(k_bi_final, dim3(1), bl, 0, st, parts, 1, 1, tol2, ps->sc, ps->part);
+    hipLaunchKernelGGL(k_bi_s, gr, bl, 0, st, n, ps->sc, ps->r, ps->v, ps->y, ps->d_dinv, ps->x,
+                       ps->s, ps->z);
+    hipLaunchKernelGGL(k_bi_at, gr, bl, 0, st, n, A, ps->sc, ps->z, ps->s, ps->tv, ps->part);
+    hipLaunchKernelGGL(k_bi_final, dim3(1), bl, 0, st, parts, 2, 2, tol2, ps->sc, ps->part);
+    hipLaunchKernelGGL(k_bi_x, gr, bl, 0, st, n, ps->sc, ps->z, ps->s, ps->tv, ps->rh, ps->x,
+                       ps->r, ps->part);
+    hipLaunchKernelGGL(k_bi_final, dim3(1), bl, 0, st, parts, 2, 3, tol2, ps->sc, ps->part);
+    if (it % 4 == 3 || it == ps->max_it - 1) {  // read the flag now and then; later
+                                                // kernels are no-ops once it is set
+      APH_HIP_CHECK(ctx, hipMemcpyAsync(ps->h_sc, ps->sc, sizeof(double) * NSC_ALL,
+                                        hipMemcpyDeviceToHost, st));
+      APH_HIP_CHECK(ctx, hipStreamSynchronize(st));
+      if (ps->h_sc[DONE] != 0.0) {
+        converged = ps->h_sc[FAILED] == 0.0;
+        break;
+      }
+    }
+  }
+  if (!converged)
+    return set_error(ctx, APH_E_STATE,
+                     "poisson gravity: %s (tolerance %g, %d iterations, relative residual %g)",
+                     ps->h_sc[FAILED] != 0.0 ? "BiCGSTAB breakdown" : "no convergence",
+                     ps->tol, (int)ps->h_sc[ITERS],
+                     ps->h_sc[BB] > 0.0 ? sqrt(ps->h_sc[RR] / ps->h_sc[BB]) : 0.0);
+  const Csr NB{ps->d_nbptr, ps->d_nbcol, ps->d_nbval};
+  hipLaunchKernelGGL(k_tfield, cg, bl, 0, st, bs->d, ps->d_wall, NB, n, ps->ncb, ps->coef,
+                     ps->x, grav, phi);
+  APH_LAUNCH_CHECK(ctx);
+  return APH_OK;
+}
+
 int check_lattice(aph_context *ctx, const aph_blockset *bs, aph_poisson *ps,
                   std::vector<int32_t> &boff) {
   if (bs->ndim != 3)
@@ -486,6 +826,7 @@ const aph_blockset *poisson_blockset(const aph_poisson *ps) { return ps->bs; }
 
 int poisson_pass(aph_context *ctx, aph_poisson *ps, const double *field, double dfloor,
                  double *grav, double *phi) {
+  if (ps->tree) return tree_pass(ctx, ps, field, dfloor, grav, phi);
   const aph_blockset *bs = ps->bs;
   hipLaunchKernelGGL(k_pmom_block, dim3((unsigned)bs->nblocks), dim3(kThreads), 0,
                      ctx->stream, bs->d, bs->nvar, field, dfloor, ps->bmom);
@@ -588,18 +929,89 @@ int aph_poisson_create(aph_context *ctx, const aph_blockset *bs, double gravity_
     if (e == hipSuccess) e = hipMalloc(&ps->t[l], sizeof(double) * nl);
   }
   if (e == hipSuccess) e = hipMalloc(&ps->part, sizeof(double) * kRedBlocks);
-  if (e == hipSuccess) e = hipMalloc(&ps->sc, sizeof(double) * NSC);
+  if (e == hipSuccess) e = hipMalloc(&ps->sc, sizeof(double) * NSC_ALL);
   if (e == hipSuccess) e = hipMalloc(&ps->coef, sizeof(double) * 9);
   if (e == hipSuccess) e = hipMalloc(&ps->bmom, sizeof(double) * 9 * bs->nblocks);
-  if (e == hipSuccess) e = hipHostMalloc(&ps->h_sc, sizeof(double) * NSC);
+  if (e == hipSuccess) e = hipHostMalloc(&ps->h_sc, sizeof(double) * NSC_ALL);
   if (e == hipSuccess)
     e = hipMemcpy(ps->d_boff, boff.data(), sizeof(int32_t) * boff.size(),
                   hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemset(ps->sc, 0, sizeof(double) * NSC);
+  if (e == hipSuccess) e = hipMemset(ps->sc, 0, sizeof(double) * NSC_ALL);
   if (e == hipSuccess) e = hipMemset(ps->coef, 0, sizeof(double) * 9);
   if (e != hipSuccess) {
     aph_poisson_destroy(ctx, ps);
     return set_error(ctx, APH_E_HIP, "aph_poisson_create: %s", hipGetErrorString(e));
+  }
+  *out = ps;
+  return APH_OK;
+}
+
+int aph_poisson_create_tree(aph_context *ctx, const aph_blockset *bs, const int32_t nbase[3],
+                            const int32_t *level, const int64_t *loc, int32_t flux_correct,
+                            int32_t prolongation, double gravity_constant,
+                            double diagonal_alpha, double tolerance, int32_t max_iterations,
+                            aph_poisson **out) {
+  if (!ctx) return APH_E_INVALID;
+  if (!bs || !out || !nbase || !level || !loc || !(tolerance > 0.0) || max_iterations < 1 ||
+      diagonal_alpha < 0.0)
+    return set_error(ctx, APH_E_INVALID,
+                     "aph_poisson_create_tree: null argument, or not tolerance > 0, "
+                     "max_iterations >= 1, alpha >= 0");
+  if (bs->ndim != 3)
+    return set_error(ctx, APH_E_INVALID, "aph_poisson_create_tree: 3D block sets only");
+  PoissonTreeCsr m;
+  char msg[256];
+  int rc = poisson_tree_csr(bs->h.data(), bs->nblocks, nbase, level, loc, flux_correct,
+                            prolongation, diagonal_alpha, m, msg, sizeof msg);
+  if (rc) return set_error(ctx, rc, "aph_poisson_create_tree: %s", msg);
+  std::vector<double> dinv((size_t)m.n);
+  for (int64_t r = 0; r < m.n; ++r) {
+    double dg = 0.0;
+    for (int64_t q = m.rowptr[r]; q < m.rowptr[r + 1]; ++q)
+      if (m.col[q] == r) dg = m.val[q];
+    if (dg == 0.0)
+      return set_error(ctx, APH_E_INVALID, "aph_poisson_create_tree: zero diagonal at cell %lld",
+                       (long long)r);
+    dinv[r] = 1.0 / dg;
+  }
+  aph_poisson *ps = new aph_poisson;
+  ps->bs = bs;
+  ps->G = gravity_constant;
+  ps->alpha = diagonal_alpha;
+  ps->tol = tolerance;
+  ps->max_it = max_iterations;
+  ps->tree = true;
+  ps->nt = m.n;
+  ps->ncb = m.n / bs->nblocks;
+  const size_t n = (size_t)m.n;
+  hipError_t e = hipSuccess;
+  auto put = [&](auto **dst, const auto &src) {
+    using T = typename std::decay_t<decltype(src)>::value_type;
+    if (e == hipSuccess) e = hipMalloc(dst, sizeof(T) * std::max<size_t>(1, src.size()));
+    if (e == hipSuccess && !src.empty())
+      e = hipMemcpy(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice);
+  };
+  put(&ps->d_ptr, m.rowptr);
+  put(&ps->d_col, m.col);
+  put(&ps->d_val, m.val);
+  put(&ps->d_nbptr, m.nb_rowptr);
+  put(&ps->d_nbcol, m.nb_col);
+  put(&ps->d_nbval, m.nb_val);
+  put(&ps->d_wall, m.wall);
+  put(&ps->d_dinv, dinv);
+  for (double **v : {&ps->x, &ps->b, &ps->r, &ps->rh, &ps->p, &ps->v, &ps->y, &ps->s, &ps->z,
+                     &ps->tv})
+    if (e == hipSuccess) e = hipMalloc(v, sizeof(double) * n);
+  if (e == hipSuccess) e = hipMalloc(&ps->part, sizeof(double) * 2 * kRedBlocks);
+  if (e == hipSuccess) e = hipMalloc(&ps->sc, sizeof(double) * NSC_ALL);
+  if (e == hipSuccess) e = hipMalloc(&ps->coef, sizeof(double) * 9);
+  if (e == hipSuccess) e = hipMalloc(&ps->bmom, sizeof(double) * 9 * bs->nblocks);
+  if (e == hipSuccess) e = hipHostMalloc(&ps->h_sc, sizeof(double) * NSC_ALL);
+  if (e == hipSuccess) e = hipMemset(ps->sc, 0, sizeof(double) * NSC_ALL);
+  if (e == hipSuccess) e = hipMemset(ps->coef, 0, sizeof(double) * 9);
+  if (e != hipSuccess) {
+    aph_poisson_destroy(ctx, ps);
+    return set_error(ctx, APH_E_HIP, "aph_poisson_create_tree: %s", hipGetErrorString(e));
   }
   *out = ps;
   return APH_OK;
@@ -610,7 +1022,11 @@ int aph_poisson_destroy(aph_context *ctx, aph_poisson *ps) {
   if (!ps) return APH_OK;
   for (void *v : {(void *)ps->d_boff, (void *)ps->x, (void *)ps->b, (void *)ps->r,
                   (void *)ps->z, (void *)ps->p, (void *)ps->q, (void *)ps->part,
-                  (void *)ps->sc, (void *)ps->coef, (void *)ps->bmom})
+                  (void *)ps->sc, (void *)ps->coef, (void *)ps->bmom, (void *)ps->d_ptr,
+                  (void *)ps->d_nbptr, (void *)ps->d_col, (void *)ps->d_nbcol,
+                  (void *)ps->d_wall, (void *)ps->d_val, (void *)ps->d_nbval,
+                  (void *)ps->d_dinv, (void *)ps->rh, (void *)ps->v, (void *)ps->y,
+                  (void *)ps->s, (void *)ps->tv})
     if (v) (void)hipFree(v);
   for (int l = 0; l < kMaxLevels; ++l)
     for (void *v : {(void *)ps->e[l], (void *)ps->rl[l], (void *)ps->t[l]})
@@ -634,7 +1050,7 @@ int aph_poisson_get_info(aph_context *ctx, aph_poisson *ps, double coeffs[9],
   if (!ps) return set_error(ctx, APH_E_INVALID, "aph_poisson_get_info: null");
   double c[9];
   APH_HIP_CHECK(ctx, hipMemcpyAsync(c, ps->coef, sizeof c, hipMemcpyDeviceToHost, ctx->stream));
-  APH_HIP_CHECK(ctx, hipMemcpyAsync(ps->h_sc, ps->sc, sizeof(double) * NSC,
+  APH_HIP_CHECK(ctx, hipMemcpyAsync(ps->h_sc, ps->sc, sizeof(double) * NSC_ALL,
                                     hipMemcpyDeviceToHost, ctx->stream));
   APH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   if (coeffs)

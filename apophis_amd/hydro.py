@@ -383,10 +383,11 @@ class Gravity(enum.IntEnum):  # main.hpp:55-61
 
 def initialize_gravity(pin):
     """InitializeGravity (gravity.cpp:220-370) for the solvers this library runs: "none",
-    "constant" (the field is the driver's g) and "monopole" (nrbin: gravity/nrbin, else
+    "constant" (the field is the driver's g), "monopole" (nrbin: gravity/nrbin, else
     the mesh's nx1, monopole_gravity.hpp:83-88; rmax: gravity/rmax, else the distance to
-    the farthest mesh corner, gravity.cpp:262-276).  "poisson" (multigrid / BiCGSTAB) is
-    refused."""
+    the farthest mesh corner, gravity.cpp:262-276) and "poisson" (gravity/max_iterations,
+    diagonal_alpha, solver, boundary_prolongation, err_tol, flux_correct, and the
+    solver_params block's residual_tolerance; PoissonGravitySolver runs it)."""
     pkg = StateDescriptor("Gravity")
     gtype = _get(pin, "gravity", "type", "none")
     pkg.add_param("gravity_constant", float(_get(pin, "gravity", "gravity_constant",
@@ -404,6 +405,11 @@ def initialize_gravity(pin):
         prolong = _get(pin, "gravity", "boundary_prolongation", "Linear")
         if prolong not in ("Linear", "Quadratic", "Constant"):
             raise ApophisError("Unknown prolongation method for gravity boundaries.")
+        pkg.add_param("boundary_prolongation", prolong)
+        fc = _get(pin, "gravity", "flux_correct", False)  # poisson_gravity.hpp:41
+        if isinstance(fc, str):
+            fc = fc.strip().lower() == "true"
+        pkg.add_param("flux_correct", bool(fc))
         err_tol = float(_get(pin, "gravity", "err_tol", 1.0e-8))
         pkg.add_param("err_tol", err_tol)
         pkg.add_param("residual_tolerance",
@@ -499,24 +505,49 @@ class MonopoleGravitySolver:
         return m_enc, phi_rad
 
 
+POISSON_PROLONGATION = {"Linear": 0, "Constant": 1}  # aph_poisson_prolongation
+
+
 class PoissonGravitySolver:
     """PoissonGravitySolver (poisson_gravity.hpp:390-625) on the device, for a 3D block set
-    tiling a box with physical boundaries on every face: the multipole coefficients, the
-    right-hand side, the solve (to the relative residual tolerance) and the "gravity" /
-    "gravity.phi" fields."""
+    with physical boundaries on every face: the multipole coefficients, the right-hand
+    side, the solve (to the relative residual tolerance) and the "gravity" / "gravity.phi"
+    fields.  mesh None: the blocks tile a box in one lattice (aph_poisson_create); mesh an
+    amr.AmrMesh whose block set is dbs's: the composite operator of the multilevel mesh
+    (aph_poisson_create_tree) with the deck's flux_correct and boundary_prolongation
+    ("Linear" or "Constant"; "Quadratic" is not built and is refused)."""
 
     def __init__(self, ctx, dbs, gravity_constant, diagonal_alpha=0.0, tolerance=1e-8,
-                 max_iterations=10000):
+                 max_iterations=10000, flux_correct=False, prolongation="Linear", mesh=None):
         self.ctx = ctx
         self.h = ctypes.c_void_p()
-        ctx.check(_lib.lib().aph_poisson_create(ctx.h, dbs, float(gravity_constant),
-                                                float(diagonal_alpha), float(tolerance),
-                                                int(max_iterations), ctypes.byref(self.h)))
+        if mesh is None:
+            ctx.check(_lib.lib().aph_poisson_create(ctx.h, dbs, float(gravity_constant),
+                                                    float(diagonal_alpha), float(tolerance),
+                                                    int(max_iterations),
+                                                    ctypes.byref(self.h)))
+            return
+        if prolongation not in POISSON_PROLONGATION:
+            raise ApophisError(f"gravity/boundary_prolongation {prolongation}: Linear or "
+                               "Constant on a multilevel mesh (the quadratic form is not "
+                               "built)")
+        vp = ctypes.c_void_p
+        self._tree = (np.array(mesh.nbase, dtype=np.int32),
+                      np.ascontiguousarray(mesh.level, dtype=np.int32),
+                      np.ascontiguousarray(mesh.loc, dtype=np.int64))
+        nbase, level, loc = self._tree
+        ctx.check(_lib.lib().aph_poisson_create_tree(
+            ctx.h, dbs, nbase.ctypes.data_as(vp), level.ctypes.data_as(vp),
+            loc.ctypes.data_as(vp), int(bool(flux_correct)),
+            POISSON_PROLONGATION[prolongation], float(gravity_constant),
+            float(diagonal_alpha), float(tolerance), int(max_iterations),
+            ctypes.byref(self.h)))
 
     @classmethod
-    def from_package(cls, ctx, pkg, dbs):
+    def from_package(cls, ctx, pkg, dbs, mesh=None):
         return cls(ctx, dbs, pkg.param("gravity_constant"), pkg.param("diagonal_alpha"),
-                   pkg.param("residual_tolerance"), pkg.param("max_iterations"))
+                   pkg.param("residual_tolerance"), pkg.param("max_iterations"),
+                   pkg.param("flux_correct"), pkg.param("boundary_prolongation"), mesh=mesh)
 
     def __del__(self):
         h = getattr(self, "h", None)
@@ -580,7 +611,8 @@ class ApophisDriver:
             if kind == Gravity.monopole:
                 self.mono = MonopoleGravitySolver.from_package(md.ctx, gravity, md.bs.ndim)
             else:
-                self.poisson = PoissonGravitySolver.from_package(md.ctx, gravity, md.dbs)
+                self.poisson = PoissonGravitySolver.from_package(
+                    md.ctx, gravity, md.dbs, mesh=amr.mesh if amr is not None else None)
             md.grav = torch.zeros(md.bs.field_size(3), dtype=torch.float64,
                                   device=md.cons.device)
             md.phi = torch.zeros(md.bs.field_size(1), dtype=torch.float64,
@@ -724,8 +756,10 @@ class FusedDriver:
             self.mono = MonopoleGravitySolver(ctx, bs.ndim, *monopole)
             ctx.check(_lib.lib().aph_plan_set_monopole(ctx.h, self.plan, self.mono.h))
         self.poisson = None
-        if poisson is not None:  # (gravity_constant, diagonal_alpha, tolerance, max_it)
-            self.poisson = PoissonGravitySolver(ctx, self.device_bs.h, *poisson)
+        if poisson is not None:  # (gravity_constant, diagonal_alpha, tolerance, max_it
+            # [, flux_correct, prolongation]); on a multilevel mesh the composite operator
+            self.poisson = PoissonGravitySolver(ctx, self.device_bs.h, *poisson,
+                                                mesh=amr_mesh)
             ctx.check(_lib.lib().aph_plan_set_poisson(ctx.h, self.plan, self.poisson.h))
         self.amr = None
         if amr_mesh is not None:  # a multilevel mesh (amr.AmrMesh) of this block set

@@ -163,9 +163,11 @@ def grav_sphere(res=32, block=None, ndim=3, rho0=1.0, r0=0.25, nrbin=None, G=1.0
     tst/.../gravity/gravity.py runs it): rho = rho0 (1 - r^2/r0^2)^2 inside r0 and 0
     outside (ConsToPrim's floor then applies), at rest, E = 1, on [-1/2, 1/2]^ndim with
     outflow boundaries.  solver "monopole": centred at the origin, nrbin = 4 res;
-    "poisson": centred at (0.01, 0, 0), the residual tolerance `tol`.  smr_level > 0: the
-    deck's static refinement block ([-0.25, 0.25]^3 at that level, gravity.py's SMR cases
-    with res 16 and 8^3 blocks) on a multilevel mesh, kept in meta["mesh"]."""
+    "poisson": centred at (0.01, 0, 0), the residual tolerance `tol`, flux_correct false
+    (the deck's).  smr_level > 0: the deck's static refinement block ([-0.25, 0.25]^3 at
+    that level, gravity.py's SMR cases with res 16 and 8^3 blocks) on a multilevel mesh,
+    kept in meta["mesh"]; Poisson's fine/coarse prolongation there is "Linear" (the
+    deck's boundary_prolongation = Quadratic is not built)."""
     if center is None:
         center = (0.01, 0.0, 0.0) if solver == "poisson" else (0.0, 0.0, 0.0)
     block = block or res
@@ -175,8 +177,6 @@ def grav_sphere(res=32, block=None, ndim=3, rho0=1.0, r0=0.25, nrbin=None, G=1.0
     xmin, xmax = (-0.5,) * 3, (0.5,) * 3
     amesh = None
     if smr_level:
-        if solver != "monopole":
-            raise ValueError("Poisson gravity on refined meshes is not built")
         amesh = _static_mesh(ndim, nb, block, NEED_GHOST[recon], xmin, xmax, 0.25, smr_level)
         bs = amesh.block_set()
     else:
@@ -191,7 +191,7 @@ def grav_sphere(res=32, block=None, ndim=3, rho0=1.0, r0=0.25, nrbin=None, G=1.0
     # the mesh box of the deck on every axis (gravity.cpp reads x2/x3 extents too)
     rmax = _default_rmax(xmin, xmax)
     grav = {"monopole": (nrbin or 4 * res, rmax, G)} if solver == "monopole" else \
-        {"poisson": (G, 0.0, tol, 10000)}
+        {"poisson": (G, 0.0, tol, 10000, False, "Linear")}
     return Problem("grav_sphere", bs, cons, make_eos(gm1), recon, riemann, cfl,
                    integrator="rk1", **grav,
                    meta={"rho0": rho0, "r0": r0, "res": res, "box": (xmin, xmax),
@@ -205,16 +205,15 @@ def evrard(res=32, block=16, mass=1.0, radius=1.0, u_therm=0.05, nrbin=None, G=1
     monopole solver as tst/.../evrards_collapse/evrards_collapse.py runs it: rho =
     M / (2 pi R^2 r) inside R and 1e-5 outside, at rest, E = u_therm rho, on [-1, 1]^3
     (res^3 cells in blocks of block^3), outflow, PLM + HLLC, RK2, CFL 0.4, to t = 0.8;
-    solver "monopole" (nrbin = 4 res) or "poisson" (the deck's own, residual tolerance
-    `tol`).  smr_level > 0: the deck's static refinement block ([-0.5, 0.5]^3 at that
-    level; the test's monopole_smr_32 is res 32, block 16, level 2) on a multilevel mesh,
-    kept in meta["mesh"]."""
+    solver "monopole" (nrbin = 4 res) or "poisson" (the deck's own: residual tolerance
+    `tol`, flux_correct true, Linear prolongation).  smr_level > 0: the deck's static
+    refinement block ([-0.5, 0.5]^3 at that level; the test's monopole_smr_32 is res 32,
+    block 16, level 2, its smr_16 the Poisson solver at res 16, block 8) on a multilevel
+    mesh, kept in meta["mesh"]."""
     nb = res // block
     xmin, xmax = (-1.0,) * 3, (1.0,) * 3
     amesh = None
     if smr_level:
-        if solver != "monopole":
-            raise ValueError("Poisson gravity on refined meshes is not built")
         amesh = _static_mesh(3, nb, block, NEED_GHOST[recon], xmin, xmax, 0.5, smr_level)
         bs = amesh.block_set()
     else:
@@ -228,7 +227,7 @@ def evrard(res=32, block=16, mass=1.0, radius=1.0, u_therm=0.05, nrbin=None, G=1
         rho = np.where(r < radius, mass / (2.0 * np.pi * (radius * radius) * r), 1e-5)
         _set_interior(bs, cons, b, rho, 0.0, 0.0, 0.0, u_therm * rho)
     grav = {"monopole": (nrbin or 4 * res, _default_rmax(xmin, xmax), G)} \
-        if solver == "monopole" else {"poisson": (G, 0.0, tol, 10000)}
+        if solver == "monopole" else {"poisson": (G, 0.0, tol, 10000, True, "Linear")}
     return Problem("evrard", bs, cons, make_eos(gm1), recon, riemann, cfl, tlim=tlim,
                    **grav, meta={"res": res, "radius": radius, "box": (xmin, xmax),
                                  "mesh": amesh})

@@ -322,18 +322,41 @@ def monopole_leg(ctx, hydro, problems, torch, nsteps=20, res=256):
     return out
 
 
-def evrard_smr_leg(ctx, hydro, problems, torch):
-    """The deck inputs/evrards_collapse.in on its own static-refinement mesh (32^3 base in
-    16^3 blocks, [-0.5, 0.5]^3 at level 2: 120 blocks over three levels, outflow) with the
-    monopole solver (the reference test's monopole_smr_32), batched fused form with the
-    device dt policy to t = 0.8: steps, wall time, zone-cycles/s, and the reference's
-    profile checks on the result."""
+def evrard_smr_leg(ctx, hydro, problems, torch, solver="monopole"):
+    """The deck inputs/evrards_collapse.in on a static-refinement mesh ([-0.5, 0.5]^3 at
+    level 2, outflow) to t = 0.8, batched fused form with the device dt policy: steps, wall
+    time, zone-cycles/s, and the reference's profile checks on the result.  solver
+    "monopole": the reference test's monopole_smr_32 (32^3 base in 16^3 blocks: 120 blocks
+    over three levels); "poisson": its smr_16 (16^3 base in 8^3 blocks, the deck's
+    MGBiCGSTAB problem with flux correction, solved by BiCGSTAB + Jacobi on the composite
+    operator), with the time of one solve of the initial state and its iterations."""
     from tests.monopole_checks import evrard_exact, evrard_profiles, evrard_verdict
 
-    p = problems.evrard(smr_level=2)
+    if solver == "monopole":
+        p = problems.evrard(smr_level=2)
+        grav = {"monopole": p.monopole}
+    else:
+        p = problems.evrard(res=16, block=8, smr_level=2, solver="poisson")
+        grav = {"poisson": p.poisson}
     cells = p.bs.interior_cells()
+    extra = {}
+    if solver == "poisson":  # one solve of the initial state, timed on its own
+        md = hydro.MeshData(ctx, p.bs, p.cons)
+        ps = hydro.PoissonGravitySolver(ctx, md.dbs, *p.poisson, mesh=p.meta["mesh"])
+        hydro.cons_to_prim(md, p.eos)
+        g = torch.zeros(p.bs.field_size(3), dtype=torch.float64, device=md.cons.device)
+        ps.solve(md, g)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(5):
+            ps.solve(md, g)
+        torch.cuda.synchronize()
+        _, iters, res = ps.info()
+        extra = {"solve_ms": (time.perf_counter() - t0) / 5 * 1e3, "solve_iterations": iters,
+                 "solve_rel_residual": res}
+        del ps, md
     fd = hydro.FusedDriver(ctx, p.bs, p.cons, p.recon, p.riemann, p.eos, p.cfl, tlim=p.tlim,
-                           monopole=p.monopole, amr_mesh=p.meta["mesh"])
+                           amr_mesh=p.meta["mesh"], **grav)
     fd.initial_dt()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -350,7 +373,7 @@ def evrard_smr_leg(ctx, hydro, problems, torch):
     return {"blocks": p.bs.nblocks, "cells": cells, "levels": [int(v) for v in np.bincount(
                 p.meta["mesh"].level)], "steps": n, "wall_s": el,
             "zone_cycles_per_s": cells * n / el, "profile_rms": rms,
-            "reference_checks_pass": not fails}
+            "reference_checks_pass": not fails, **extra}
 
 
 def multilevel_adjoint_leg(ctx, hydro, problems, torch, reps=5):
@@ -873,6 +896,9 @@ def main():
         print(f"# blast deck: {extras['blast_deck_adaptive']}", file=sys.stderr, flush=True)
         extras["evrard_smr_monopole"] = evrard_smr_leg(ctx, hydro, problems, torch)
         print(f"# evrard smr: {extras['evrard_smr_monopole']}", file=sys.stderr, flush=True)
+        extras["evrard_smr16_poisson"] = evrard_smr_leg(ctx, hydro, problems, torch, "poisson")
+        print(f"# evrard smr poisson: {extras['evrard_smr16_poisson']}", file=sys.stderr,
+              flush=True)
     if not args.no_adjoint:
         extras["adjoint_kh"] = adjoint_leg(ctx, hydro, problems, torch)
         print(f"# adjoint: {extras['adjoint_kh']}", file=sys.stderr, flush=True)

@@ -191,7 +191,8 @@ int aph_monopole_get_tables(aph_context *ctx, aph_monopole *m, double *m_enc,
    values of the multipole expansion moved to the right-hand side, ComputeRhs :502-589);
    it is solved by conjugate gradients preconditioned with a geometric multigrid V-cycle.
    The reference solves the same discrete problem with Parthenon's MG or BiCGSTAB, so the
-   two agree to the tolerance.  Refinement, periodic faces, 1D and 2D: refused. */
+   two agree to the tolerance.  Multilevel meshes: aph_poisson_create_tree; periodic faces,
+   1D and 2D: refused. */
 int aph_poisson_create(aph_context *ctx, const aph_blockset *bs, double gravity_constant,
                        double diagonal_alpha, double tolerance, int32_t max_iterations,
                        aph_poisson **out);
@@ -208,6 +209,44 @@ int aph_poisson_solve(aph_context *ctx, aph_poisson *ps, const double *prim, dou
    Synchronous; any output may be NULL. */
 int aph_poisson_get_info(aph_context *ctx, aph_poisson *ps, double coeffs[9],
                          int32_t *iterations, double *rel_residual);
+
+/* The phi field's prolongation at fine/coarse block faces (gravity/boundary_prolongation,
+   gravity.cpp:312-363): Parthenon's ProlongateSharedLinear (c + sum_d (+-1/4) (c_+d -
+   c_-d) / 2, unlimited) or ProlongatePiecewiseConstant.  "Quadratic" is not built. */
+enum aph_poisson_prolongation { APH_POISSON_PROLONG_LINEAR = 0, APH_POISSON_PROLONG_CONSTANT = 1 };
+
+/* Poisson self-gravity on a multilevel 3D mesh (the reference's static-refinement decks,
+   inputs/grav_test.in and inputs/evrards_collapse.in: PoissonEquation on the mesh's leaf
+   cells, poisson_gravity.hpp:31-388).  bs: the leaves' block set; level / loc (3 per
+   block) / nbase: the tree as for aph_amr_build, every domain face physical.  The
+   operator: per block face the flux (phi_low - phi_high) / dx, the value beyond a block
+   face from the same level (a leaf cell or the mean of a refined cell's children) or
+   prolongated from the coarser leaf beyond, minus the cell inside beyond a domain face;
+   flux_correct (gravity/flux_correct, :89-99) replaces a coarse face's flux over a finer
+   region by the mean of its fine faces.  Assembled once on the host from the tree; each
+   solve runs BiCGSTAB with the Jacobi preconditioner from phi = 0 to the relative
+   residual tolerance (the reference's MGBiCGSTAB reaches the same discrete solution).
+   aph_poisson_solve, aph_poisson_get_info, aph_poisson_destroy and aph_plan_set_poisson
+   take the solver as they take the lattice one; the right-hand side, the multipole
+   coefficients and the field are the lattice solver's, over the leaves.  Refused: 1D,
+   2D, periodic faces, blocks whose spacing or corner differs from their level and
+   position, prolongation other than the two above. */
+int aph_poisson_create_tree(aph_context *ctx, const aph_blockset *bs, const int32_t nbase[3],
+                            const int32_t *level, const int64_t *loc, int32_t flux_correct,
+                            int32_t prolongation, double gravity_constant,
+                            double diagonal_alpha, double tolerance, int32_t max_iterations,
+                            aph_poisson **out);
+/* Host only (no device work): the operator aph_poisson_create_tree assembles, as CSR over
+   the leaf cells (cell (i, j, k) of block b at b nx0 nx1 nx2 + (k nx1 + j) nx0 + i; rowptr
+   n + 1 entries), and the neighbour rows of the field (row (2 d + side) n + c: the value
+   beyond cell c's face on that side of axis d as a map of the cell values, empty on a
+   domain face; nb_rowptr 6 n + 1 entries).  Call with NULL arrays for nnz and nb_nnz. */
+int aph_poisson_tree_matrix(const aph_block *blocks, int32_t nblocks, const int32_t nbase[3],
+                            const int32_t *level, const int64_t *loc, int32_t flux_correct,
+                            int32_t prolongation, double diagonal_alpha, int64_t *rowptr,
+                            int32_t *col, double *val, int64_t *nnz, int64_t *nb_rowptr,
+                            int32_t *nb_col, double *nb_val, int64_t *nb_nnz, char *err,
+                            size_t err_len);
 
 /* Ghost exchange between neighbouring blocks followed by the physical boundary
    conditions (outflow / reflect, apophis_boundaries.cpp:17-92), axis by axis. */

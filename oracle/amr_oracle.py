@@ -224,9 +224,10 @@ def flux_correct(mesh, bs, fx):
 
 class AmrState:
     """The oracle State of a problem on a multilevel mesh: the stage sequence of the plain
-    driver (monopole gravity when the problem has it, fluxes, flux correction, update,
-    gravity source, ghost exchange + cross-level fill + boundary conditions, ConsToPrim)
-    composed from the oracle's task functions, in the order orc_step_monopole runs them."""
+    driver (monopole or Poisson gravity when the problem has it, fluxes, flux correction,
+    update, gravity source, ghost exchange + cross-level fill + boundary conditions,
+    ConsToPrim) composed from the oracle's task functions, in the order orc_step_monopole
+    runs them."""
 
     def __init__(self, prob, mesh, flux_correction=True):
         self.prob, self.mesh, self.bs = prob, mesh, prob.bs
@@ -237,8 +238,13 @@ class AmrState:
         self.gamma = np.zeros(self.bs.field_size(2))
         self.fx = [np.zeros_like(self.u0) for _ in range(3)]
         self.mono = getattr(prob, "monopole", None)  # (nrbin, rmax, G) or None
-        if self.mono is not None:
+        # (G, alpha, tolerance, max_it[, flux_correct, prolongation]) or None: the composite
+        # Poisson problem of the mesh (poisson_amr_oracle), solved to 1e-14
+        self.poisson = getattr(prob, "poisson", None)
+        self._pp = None
+        if self.mono is not None or self.poisson is not None:
             self.grav = np.zeros(self.bs.field_size(3))
+            self.phi = np.zeros(self.bs.field_size(1))
         self.g = tuple(float(v) for v in getattr(prob, "g", (0.0, 0.0, 0.0)))
         self._ghosts()
 
@@ -250,7 +256,26 @@ class AmrState:
         orc.cons_to_prim(self.bs, self.prob.eos, self.u0, self.prim, self.gamma)
 
     def gravity_field(self):
-        """The monopole field of the current primitives (bins, tables, per-cell field)."""
+        """The gravity field of the current primitives: monopole (bins, tables, per-cell
+        field) or Poisson (coefficients, right-hand side, solve, field)."""
+        if self.poisson is not None:
+            from . import poisson_amr_oracle as PA
+
+            G, alpha = self.poisson[0], self.poisson[1]
+            fc = self.poisson[4] if len(self.poisson) > 4 else False
+            prol = self.poisson[5] if len(self.poisson) > 5 else "Linear"
+            if self._pp is None:
+                self._pp = PA.Problem(self.mesh, self.bs, alpha, fc, prol)
+            pr = self._pp
+            _, phi, g = pr.gravity(self.prim, G)
+            nx = pr.nx
+            for b in range(self.bs.nblocks):
+                sl = slice(b * pr.ncb, (b + 1) * pr.ncb)
+                gv = self.bs.interior(self.grav, b, 3)
+                for d in range(3):
+                    gv[d] = g[d, sl].reshape(nx[2], nx[1], nx[0])
+                self.bs.interior(self.phi, b, 1)[0] = phi[sl].reshape(nx[2], nx[1], nx[0])
+            return self.grav
         nrbin, rmax, G = self.mono
         mbin, _ = orc.mono_bin_masses(self.bs, self.prim, nrbin, rmax)
         m_enc, phi_rad = orc.mono_tables(nrbin, self.bs.ndim, rmax, G, mbin)
@@ -267,13 +292,14 @@ class AmrState:
         self.u1[:] = self.u0
         for s in range(len(g0)):
             recon = "dc" if (p.integrator == "vl2" and s == 0) else p.recon
-            if self.mono is not None:
+            field = self.mono is not None or self.poisson is not None
+            if field:
                 self.gravity_field()
             orc.calculate_fluxes(bs, recon, p.riemann, self.prim, self.gamma, *self.fx)
             if self.correct:
                 flux_correct(self.mesh, bs, self.fx)
             orc.update(bs, self.u0, self.u1, *self.fx, g0[s], g1[s], be[s] * dt)
-            if self.mono is not None:
+            if field:
                 orc.apply_gravity_field(bs, self.u0, self.prim, self.grav, be[s] * dt)
             elif any(v != 0.0 for v in self.g):
                 orc.apply_gravity(bs, self.u0, self.prim, np.array(self.g), be[s] * dt)

@@ -59,11 +59,14 @@ def sphere_errors(bs, grav, phi, rho0=1.0, r0=0.25, G=1.0):
     return sphere_errors_max(bs, grav, phi, rho0, r0, G)[:2]
 
 
-def sphere_errors_max(bs, grav, phi, rho0=1.0, r0=0.25, G=1.0):
+def sphere_errors_max(bs, grav, phi, rho0=1.0, r0=0.25, G=1.0, center=(0.0, 0.0, 0.0)):
     """(phi L2, g_x L2, phi max, g_x max) of a gravity field against the sphere's analytic
-    solution: gravity.py's l2_norm (the mean over every cell of every block, unweighted)
-    and max |error| (its REFINED_LIMITS quantities, gravity.py:48-53, 348-372)."""
+    solution about `center` (gravity.py's get_errors: the Poisson runs' sphere sits at
+    X0_OFF = 0.01): gravity.py's l2_norm (the mean over every cell of every block,
+    unweighted) and max |error| (its REFINED_LIMITS quantities, gravity.py:48-53,
+    169-200, 348-372)."""
     x, y, z = centres(bs)
+    x, y, z = x - center[0], y - center[1], z - center[2]
     r = np.sqrt(x * x + y * y + z * z)
     ph = interior(bs, phi, 1, 0) - analytic_phi(r, rho0, r0, G)
     gr = analytic_g(r, rho0, r0, G)

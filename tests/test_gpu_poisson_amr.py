@@ -1,0 +1,193 @@
+"""GPU checks of Poisson gravity on multilevel meshes (aph_poisson_create_tree: the
+composite operator assembled on the host from the tree, BiCGSTAB with the Jacobi
+preconditioner on the device) against the multilevel restatement
+(oracle/poisson_amr_oracle.py) through the C ABI.  The device iterates to a relative
+residual tolerance, so the comparisons are to tolerances, stated per test:
+
+* the device's phi put through the oracle's own operator and right-hand side: a relative
+  residual within 2x the tolerance (the device's matrix equals the oracle's bitwise,
+  tests/test_oracle_poisson_amr.py, so this pins the right-hand side and the iteration),
+  and within 1e-7 (relative) of the oracle's 1e-14 solution at tolerance 1e-11;
+* multipole coefficients: 1e-13 relative;
+* the device's field against the oracle's field of the device's phi: 1e-12 of the
+  largest magnitude;
+* two plain and two fused steps of Evrard's collapse on the reference's smr_16 mesh
+  against the oracle's steps: 1e-9 relative per cell; fused and plain bitwise equal;
+* the reference's own checks on the GPU's results: gravity.py's REFINED_LIMITS
+  (poisson_smr_16) and evrards_collapse.py's profile checks (smr_16, to t = 0.8).
+"""
+import dataclasses
+import types
+
+import numpy as np
+import pytest
+import torch
+
+from apophis_amd import amr, hydro, problems
+from oracle import amr_oracle as ao
+from oracle import compare
+from oracle import oracle as orc
+from oracle import poisson_amr_oracle as PA
+from tests.monopole_checks import evrard_exact, evrard_profiles, evrard_verdict, sphere_errors_max
+from tests.test_oracle_poisson_amr import _mesh
+
+pytestmark = pytest.mark.gpu
+
+
+def _cloud(m, seed):
+    """A block set of the mesh with a smooth positive density (a displaced Gaussian over a
+    floor, with a seeded ripple) at rest."""
+    bs = m.block_set()
+    cons = np.zeros(bs.field_size(bs.nvar))
+    rng = np.random.default_rng(seed)
+    ctr = [m.xmin[d] + (0.4 + 0.2 * rng.random()) * (m.xmax[d] - m.xmin[d]) for d in range(3)]
+    k = rng.integers(1, 4, size=3)
+    for b in range(bs.nblocks):
+        x, y, z = bs.cell_centers(b)
+        r2 = (x - ctr[0]) ** 2 + (y - ctr[1]) ** 2 + (z - ctr[2]) ** 2
+        rho = 0.1 + np.exp(-r2 / 0.02) + 0.05 * np.sin(k[0] * x + k[1] * y + k[2] * z)
+        v = bs.interior(cons, b, bs.nvar)
+        v[0] = rho
+        v[4] = 1.0
+    return bs, cons
+
+
+CASES = [("grav16", False, "Linear"), ("three", True, "Linear"), ("three", False, "Constant"),
+         ("wallfine", True, "Linear"), ("wallfine", False, "Linear")]
+
+
+@pytest.mark.parametrize("name,fc,prol", CASES)
+def test_solution_matches_oracle(gpu_ctx, name, fc, prol):
+    print(f"[case] poisson tree solve {name} fc={fc} {prol}", flush=True)
+    m = _mesh(name)
+    if name == "grav16":
+        p = problems.grav_sphere(res=16, block=8, smr_level=2, solver="poisson")
+        bs, cons = p.bs, p.cons
+    else:
+        bs, cons = _cloud(m, 3)
+    G, tol = 0.9, 1e-11
+    dbs = hydro.DeviceBlockSet(gpu_ctx, bs)
+    ps = hydro.PoissonGravitySolver(gpu_ctx, dbs.h, G, 0.0, tol, 20000, fc, prol, mesh=m)
+    u = torch.as_tensor(cons, device="cuda")
+    grav = torch.zeros(bs.field_size(3), dtype=torch.float64, device="cuda")
+    phi = torch.zeros(bs.field_size(1), dtype=torch.float64, device="cuda")
+    ps.solve(types.SimpleNamespace(prim=u), grav, phi)  # density: component 0
+    coef_g, iters, resid = ps.info()
+    torch.cuda.synchronize()
+    print(f"  iterations {iters}, relative residual {resid:.2e}", flush=True)
+    assert resid <= tol and iters < 20000
+    pr = PA.Problem(m, bs, 0.0, fc, prol)
+    rho = pr.leaf_values(cons, bs.nvar, 0)
+    coef = pr.mp_coeffs(rho, G)
+    np.testing.assert_allclose(coef_g, coef, rtol=1e-13, atol=1e-16 * abs(coef).max())
+    b = pr.rhs(rho, G, coef)
+    phi_g = pr.leaf_values(phi.cpu().numpy(), 1, 0)
+    true_res = np.linalg.norm(pr.A @ phi_g - b) / np.linalg.norm(b)
+    phi_o = pr.solve(b)
+    dist = np.linalg.norm(phi_g - phi_o) / np.linalg.norm(phi_o)
+    print(f"  true residual {true_res:.2e}, distance {dist:.2e}", flush=True)
+    assert true_res <= 2 * tol and dist <= 1e-7
+    g_dev = np.stack([pr.leaf_values(grav.cpu().numpy(), 3, d) for d in range(3)])
+    g_ref = pr.field(phi_g, coef_g)
+    assert np.max(np.abs(g_dev - g_ref)) <= 1e-12 * np.max(np.abs(g_ref))
+    del ps, dbs
+
+
+def _pin(p, tol):
+    G, alpha, _, max_it, fc, prol = p.poisson
+    lo, hi = p.meta["box"]
+    mesh = {"nx1": 32, "nghost": p.meta["mesh"].ng}
+    for d in range(3):
+        mesh[f"x{d + 1}min"], mesh[f"x{d + 1}max"] = lo[d], hi[d]
+    return {"parthenon/time": {"cfl": p.cfl, "integrator": p.integrator},
+            "parthenon/mesh": mesh,
+            "hydro": {"reconstruction": p.recon, "riemann": p.riemann},
+            "eos": {"gm1": float(p.eos["gm1"][0])},
+            "gravity": {"type": "poisson", "gravity_constant": G, "solver": "MGBiCGSTAB",
+                        "flux_correct": "true" if fc else "false",
+                        "boundary_prolongation": prol, "max_iterations": max_it},
+            "gravity/solver_params": {"residual_tolerance": tol}}
+
+
+def _plain(ctx, p, tol):
+    m = p.meta["mesh"]
+    pin = _pin(p, tol)
+    md = hydro.MeshData(ctx, p.bs, p.cons)
+    plan = amr.AmrPlan(ctx, md.dbs, m, p.bs)
+    return hydro.ApophisDriver(hydro.initialize_hydro(pin), md, amr=plan,
+                               gravity=hydro.initialize_gravity(pin))
+
+
+def test_gravity_smr_meets_the_reference_limits(gpu_ctx):
+    """gravity.py's poisson_smr_16 (16^3 base in 8^3 blocks, [-0.25, 0.25]^3 at level 2,
+    the sphere at (0.01, 0, 0), flux_correct false, Linear prolongation): REFINED_LIMITS
+    (phi L2 5e-3, g_x L2 5e-2, phi max 5e-2, g_x max 0.2)."""
+    print("[case] poisson smr gravity limits", flush=True)
+    p = problems.grav_sphere(res=16, block=8, smr_level=2, solver="poisson")
+    drv = _plain(gpu_ctx, p, 1e-8)
+    drv.initialize()
+    drv.poisson.solve(drv.md, drv.md.grav, drv.md.phi)
+    torch.cuda.synchronize()
+    l2p, l2g, mxp, mxg = sphere_errors_max(p.bs, drv.md.grav.cpu().numpy(),
+                                           drv.md.phi.cpu().numpy(), center=p.meta["center"])
+    print(drv.poisson.info()[1:], l2p, l2g, mxp, mxg, flush=True)
+    assert l2p <= 5e-3 and l2g <= 5e-2 and mxp <= 5e-2 and mxg <= 2e-1
+
+
+def test_evrard_smr16_steps_match_oracle(gpu_ctx):
+    """evrards_collapse.py's smr_16 (16^3 base in 8^3 blocks, [-0.5, 0.5]^3 at level 2,
+    the deck's flux_correct true): two steps of the plain and fused forms against the
+    oracle's, at one dt."""
+    print("[case] poisson evrard smr_16 2 steps plain/fused", flush=True)
+    p = problems.evrard(res=16, block=8, smr_level=2, solver="poisson", tol=1e-11)
+    drv = _plain(gpu_ctx, p, 1e-11)
+    dt = drv.initialize()
+    fd = hydro.FusedDriver(gpu_ctx, p.bs, p.cons, p.recon, p.riemann, p.eos, p.cfl,
+                           tlim=p.tlim, poisson=p.poisson, amr_mesh=p.meta["mesh"])
+    fd.initial_dt()
+    fd.set_dt(dt)
+    ref = ao.AmrState(p, p.meta["mesh"])
+    for _ in range(2):
+        drv.step(dt)
+        fd.step(fixed_dt=True)
+        ref.step(dt)
+    torch.cuda.synchronize()
+    got = drv.md.cons.cpu().numpy()
+    worst = compare.worst_rel(got, ref.u0, p.bs)
+    print(f"  worst relative difference {worst:.2e}", flush=True)
+    assert worst <= 1e-9
+    assert compare.bitwise_equal_interior(fd.u0.cpu().numpy(), got, p.bs)
+
+
+def test_evrard_smr16_poisson_meets_the_reference_checks(gpu_ctx):
+    """smr_16 to t = 0.8 in the fused form with the device dt policy, and the reference's
+    profile checks (evrards_collapse.py) on the result."""
+    print("[case] poisson evrard smr_16 to t=0.8", flush=True)
+    p = problems.evrard(res=16, block=8, smr_level=2, solver="poisson")
+    fd = hydro.FusedDriver(gpu_ctx, p.bs, p.cons, p.recon, p.riemann, p.eos, p.cfl,
+                           tlim=p.tlim, poisson=p.poisson, amr_mesh=p.meta["mesh"])
+    fd.initial_dt()
+    n = 0
+    while float(fd.time.item()) < p.tlim * (1 - 1e-14):
+        fd.step()
+        n += 1
+    torch.cuda.synchronize()
+    st = orc.State(dataclasses.replace(p, cons=fd.u0.cpu().numpy(), poisson=None))
+    fails, rms = evrard_verdict(evrard_profiles(p.bs, st.prim), evrard_exact())
+    print(n, fd.poisson.info()[1:], rms, flush=True)
+    assert not fails, fails
+
+
+def test_refusals(gpu_ctx):
+    m = _mesh("three")
+    bs = m.block_set()
+    dbs = hydro.DeviceBlockSet(gpu_ctx, bs)
+    with pytest.raises(hydro.ApophisError, match="quadratic form is not built"):
+        hydro.PoissonGravitySolver(gpu_ctx, dbs.h, 1.0, prolongation="Quadratic", mesh=m)
+    pm = amr.AmrMesh(3, (2, 2, 2), (4, 4, 4), 1, (0, 0, 0), (1, 1, 1), {(0, (0, 0, 0))},
+                     max_level=1)  # periodic
+    pbs = pm.block_set()
+    pdbs = hydro.DeviceBlockSet(gpu_ctx, pbs)
+    with pytest.raises(hydro._lib.HydroError, match="physical boundary"):
+        hydro.PoissonGravitySolver(gpu_ctx, pdbs.h, 1.0, mesh=pm)
+    del dbs, pdbs

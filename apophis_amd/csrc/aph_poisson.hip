@@ -67,7 +67,7 @@ constexpr int kCoarseSweeps = 40;
 // device scalars (conjugate gradients; BiCGSTAB reuses RR, BB, ALPHA, DONE and ITERS and
 // keeps its own in the other slots)
 enum Sc { RZ = 0, PQ, RR, BB, ALPHA, BETA, RZN, DONE, ITERS, NSC };
-enum ScBi { RHO = RZ, RV = PQ, OMEGA = BETA, RHO_OLD = RZN, TS = NSC, TT, RHO_NEW, FAILED, NSC_ALL };
+enum ScBi { RV = PQ, OMEGA = BETA, TS = NSC, TT, FAILED, NSC_ALL };  // rho: RZ / RZN
 
 struct Grid {
   int nx, ny, nz;
@@ -452,13 +452,29 @@ void vcycle(aph_context *ctx, aph_poisson *ps, int l, const double *r, double *e
 // cell (i, j, k) of block b at b ncb + (k ny + j) nx + i), assembled on the host from the
 // tree.  BiCGSTAB (the reference's MGBiCGSTAB without the multigrid preconditioner; the
 // operator is not symmetric once prolongation enters it) with x0 = 0, right-preconditioned
-// by the inverse diagonal, each dot product fused into the kernel that produces its
-// operand, reduced in two fixed-order passes.
+// by the inverse diagonal.  Five launches per iteration, one per global dependency:
+//   k_bi_p   the residual test and rho = (rh, r) from k_bi_x's partials; p, y = p / diag
+//   k_bi_av  v = A y, partials of (rh, v)
+//   k_bi_s   alpha = rho / (rh, v); s = r - alpha v, x += alpha y, z = s / diag
+//   k_bi_at  t = A z, partials of (t, s), (t, t)
+//   k_bi_x   omega = (t, s) / (t, t); x += omega z, r = s - omega t, partials of (r, r),
+//            (rh, r)
+// Every workgroup of a kernel sums the previous kernel's partials itself, in one fixed
+// order (so all get the same bits), instead of a one-workgroup reduction launch; a scalar
+// a kernel needs comes from its own sum or from an earlier kernel (workgroup 0 writes the
+// scalars it forms, and rho alternates between two slots by iteration parity, so no kernel
+// reads a slot it writes).  A matrix row takes kLanes lanes (rows hold ~8 entries), summed
+// by a fixed shuffle tree.
 struct Csr {
   const int64_t *ptr;
   const int32_t *col;
   const double *val;
 };
+
+constexpr int kLanes = 8;  // lanes per matrix row in the products
+
+// the slot of rho for iterations of this parity
+__device__ __forceinline__ int rho_slot(int it) { return (it & 1) ? RZN : RZ; }
 
 __device__ __forceinline__ double row_dot(const Csr &A, int64_t r, const double *x) {
   double s = 0.0;
@@ -466,30 +482,77 @@ __device__ __forceinline__ double row_dot(const Csr &A, int64_t r, const double 
   return s;
 }
 
+// the sum over the workgroup of each of K per-thread values, in every thread: a butterfly
+// within each wave (partners add the same two values, so every lane holds the same bits),
+// then the waves' sums in wave order
+template <int K>
+__device__ __forceinline__ void block_sum(double (&v)[K]) {
+  constexpr int kWaves = kThreads / 64;
+  __shared__ double sh[K][kWaves];
+  for (int k = 0; k < K; ++k) {
+    double w = v[k];
+    for (int o = 32; o > 0; o >>= 1) w += __shfl_xor(w, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[k][threadIdx.x >> 6] = w;
+  }
+  __syncthreads();
+  for (int k = 0; k < K; ++k) {
+    double t = sh[k][0];
+    for (int i = 1; i < kWaves; ++i) t += sh[k][i];
+    v[k] = t;
+  }
+  __syncthreads();  // sh may be written again by the next call in the kernel
+}
+
 // K per-thread values into part[k * kRedBlocks + blockIdx.x]
 template <int K>
 __device__ __forceinline__ void block_partials(double (&v)[K], double *part) {
-  __shared__ double sh[K][kThreads];
-  for (int k = 0; k < K; ++k) sh[k][threadIdx.x] = v[k];
-  __syncthreads();
-  for (int w = kThreads / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w)
-      for (int k = 0; k < K; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
-    __syncthreads();
-  }
+  block_sum<K>(v);
   if (threadIdx.x == 0)
-    for (int k = 0; k < K; ++k) part[k * kRedBlocks + blockIdx.x] = sh[k][0];
+    for (int k = 0; k < K; ++k) part[k * kRedBlocks + blockIdx.x] = v[k];
 }
+
+// the sums of K partial arrays (nparts each), the same order in every workgroup; every
+// thread gets them
+template <int K>
+__device__ __forceinline__ void sum_partials(const double *part, int nparts, double (&out)[K]) {
+  for (int k = 0; k < K; ++k) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) v += part[k * kRedBlocks + i];
+    out[k] = v;
+  }
+  block_sum<K>(out);
+}
+
+__device__ __forceinline__ bool lead() { return blockIdx.x == 0 && threadIdx.x == 0; }
 
 #define APH_VEC_LOOP(n, c)                                                                 \
   for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < (n);               \
        c += (int64_t)gridDim.x * blockDim.x)
 
-// x = 0, r = rh = b, p = v = 0; partial (b, b)
+// y = A x over kLanes lanes per row; calls f(row, value) on the row's first lane
+template <class F>
+__device__ __forceinline__ void spmv(int64_t n, const Csr &A, const double *x, F f) {
+  const int lane = threadIdx.x & (kLanes - 1);
+  const int64_t step = (int64_t)gridDim.x * blockDim.x / kLanes;
+  for (int64_t c = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kLanes; c < n;
+       c += step) {
+    double s = 0.0;
+    for (int64_t q = A.ptr[c] + lane, e = A.ptr[c + 1]; q < e; q += kLanes)
+      s += A.val[q] * x[A.col[q]];
+    s += __shfl_xor(s, 4, kLanes);
+    s += __shfl_xor(s, 2, kLanes);
+    s += __shfl_xor(s, 1, kLanes);
+    if (lane == 0) f(c, s);
+  }
+}
+
+// x = 0, r = rh = b, p = v = 0; partials (b, b) twice (the first iteration's (r, r) and
+// (rh, r)); the scalars' starting values
 __global__ void __launch_bounds__(kThreads) k_bi_init(int64_t n, const double *b, double *x,
                                                       double *r, double *rh, double *p,
-                                                      double *v, double *part) {
-  double a[1] = {0.0};
+                                                      double *v, double *sc,
+                                                      double *part_x) {
+  double a[2] = {0.0, 0.0};
   APH_VEC_LOOP(n, c) {
     const double bc = b[c];
     x[c] = 0.0;
@@ -499,81 +562,45 @@ __global__ void __launch_bounds__(kThreads) k_bi_init(int64_t n, const double *b
     v[c] = 0.0;
     a[0] += bc * bc;
   }
-  block_partials<1>(a, part);
+  a[1] = a[0];
+  if (lead()) {
+    for (int i = 0; i < NSC_ALL; ++i) sc[i] = 0.0;
+    sc[ALPHA] = sc[OMEGA] = 1.0;
+    sc[rho_slot(1)] = 1.0;  // rho before the first iteration
+  }
+  block_partials<2>(a, part_x);
 }
 
-// phase 0: start; 1: alpha = rho / (rh, v); 2: omega = (t, s) / (t, t); 3: the residual test
-// and rho for the next iteration
-__global__ void __launch_bounds__(kThreads) k_bi_final(int nparts, int nk, int phase,
-                                                       double tol2, double *sc,
-                                                       const double *part) {
-  __shared__ double sh[2][kThreads];
-  if (phase > 0 && finished(sc)) return;
-  for (int k = 0; k < nk; ++k) {
-    double v = 0.0;
-    for (int i = threadIdx.x; i < nparts; i += blockDim.x) v += part[k * kRedBlocks + i];
-    sh[k][threadIdx.x] = v;
-  }
-  __syncthreads();
-  for (int w = kThreads / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w)
-      for (int k = 0; k < nk; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x) return;
-  const double p0 = sh[0][0], p1 = nk > 1 ? sh[1][0] : 0.0;
-  switch (phase) {
-    case 0:
-      for (int i = 0; i < NSC_ALL; ++i) sc[i] = 0.0;
-      sc[BB] = p0;
-      sc[RR] = p0;
-      sc[RHO] = p0;
-      sc[RHO_OLD] = sc[ALPHA] = sc[OMEGA] = 1.0;
-      if (p0 == 0.0) sc[DONE] = 1.0;  // a zero right-hand side is solved by x = 0
-      break;
-    case 1:
-      sc[RV] = p0;
-      if (p0 == 0.0) {  // breakdown
-        sc[FAILED] = 1.0;
-        sc[DONE] = 1.0;
-      } else {
-        sc[ALPHA] = sc[RHO] / p0;
-      }
-      break;
-    case 2:
-      sc[TS] = p0;
-      sc[TT] = p1;
-      if (p1 == 0.0) {  // t = A z = 0: s = 0, x is the solution
-        sc[RR] = 0.0;
-        sc[ITERS] += 1.0;
-        sc[DONE] = 1.0;
-      } else {
-        sc[OMEGA] = p0 / p1;
-      }
-      break;
-    default:
-      sc[RR] = p0;
-      sc[RHO_NEW] = p1;
-      sc[ITERS] += 1.0;
-      sc[RHO_OLD] = sc[RHO];
-      sc[RHO] = p1;
-      if (p0 <= tol2 * sc[BB]) {
-        sc[DONE] = 1.0;
-      } else if (p1 == 0.0 || sc[OMEGA] == 0.0) {  // breakdown
-        sc[FAILED] = 1.0;
-        sc[DONE] = 1.0;
-      }
-      break;
-  }
-}
-
-// p = r + beta (p - omega v), y = p / diag
-__global__ void __launch_bounds__(kThreads) k_bi_p(int64_t n, const double *sc, const double *r,
-                                                   const double *v, const double *dinv,
-                                                   double *p, double *y) {
+// iteration it: the residual test on r, rho = (rh, r), beta; p = r + beta (p - omega v),
+// y = p / diag.  check_only: the test alone (after the last iteration).
+__global__ void __launch_bounds__(kThreads)
+    k_bi_p(int it, int check_only, int64_t n, int nparts, double tol2, double *sc,
+           const double *part_x, const double *r, const double *v, const double *dinv,
+           double *p, double *y) {
   if (finished(sc)) return;
+  double s[2];
+  sum_partials<2>(part_x, nparts, s);
+  const double rr = s[0], rho = s[1];
+  const double bb = it == 0 ? rr : sc[BB];
+  if (lead()) {
+    sc[RR] = rr;
+    if (it == 0) sc[BB] = rr;
+  }
+  if (rr <= tol2 * bb) {  // converged (a zero right-hand side at once)
+    if (lead()) sc[DONE] = 1.0;
+    return;
+  }
+  if (check_only) return;
   const double om = sc[OMEGA];
-  const double beta = (sc[RHO] / sc[RHO_OLD]) * (sc[ALPHA] / om);
+  if (rho == 0.0 || om == 0.0) {  // breakdown
+    if (lead()) {
+      sc[FAILED] = 1.0;
+      sc[DONE] = 1.0;
+    }
+    return;
+  }
+  const double beta = (rho / sc[rho_slot(it + 1)]) * (sc[ALPHA] / om);
+  if (lead()) sc[rho_slot(it)] = rho;
   APH_VEC_LOOP(n, c) {
     const double pc = r[c] + beta * (p[c] - om * v[c]);
     p[c] = pc;
@@ -581,27 +608,39 @@ __global__ void __launch_bounds__(kThreads) k_bi_p(int64_t n, const double *sc, 
   }
 }
 
-// v = A y; partial (rh, v)
+// v = A y; partials (rh, v)
 __global__ void __launch_bounds__(kThreads) k_bi_av(int64_t n, Csr A, const double *sc,
                                                     const double *y, const double *rh,
-                                                    double *v, double *part) {
+                                                    double *v, double *part_v) {
   if (finished(sc)) return;
   double a[1] = {0.0};
-  APH_VEC_LOOP(n, c) {
-    const double vc = row_dot(A, c, y);
+  spmv(n, A, y, [&](int64_t c, double vc) {
     v[c] = vc;
     a[0] += rh[c] * vc;
-  }
-  block_partials<1>(a, part);
+  });
+  block_partials<1>(a, part_v);
 }
 
-// s = r - alpha v, x += alpha y, z = s / diag
-__global__ void __launch_bounds__(kThreads) k_bi_s(int64_t n, const double *sc, const double *r,
-                                                   const double *v, const double *y,
-                                                   const double *dinv, double *x, double *s,
-                                                   double *z) {
+// alpha = rho / (rh, v); s = r - alpha v, x += alpha y, z = s / diag
+__global__ void __launch_bounds__(kThreads)
+    k_bi_s(int it, int64_t n, int nparts, double *sc, const double *part_v, const double *r,
+           const double *v, const double *y, const double *dinv, double *x, double *s,
+           double *z) {
   if (finished(sc)) return;
-  const double al = sc[ALPHA];
+  double rv[1];
+  sum_partials<1>(part_v, nparts, rv);
+  if (rv[0] == 0.0) {  // breakdown
+    if (lead()) {
+      sc[FAILED] = 1.0;
+      sc[DONE] = 1.0;
+    }
+    return;
+  }
+  const double al = sc[rho_slot(it)] / rv[0];
+  if (lead()) {
+    sc[RV] = rv[0];
+    sc[ALPHA] = al;
+  }
   APH_VEC_LOOP(n, c) {
     const double sv = r[c] - al * v[c];
     s[c] = sv;
@@ -613,25 +652,40 @@ __global__ void __launch_bounds__(kThreads) k_bi_s(int64_t n, const double *sc, 
 // t = A z; partials (t, s), (t, t)
 __global__ void __launch_bounds__(kThreads) k_bi_at(int64_t n, Csr A, const double *sc,
                                                     const double *z, const double *s,
-                                                    double *t, double *part) {
+                                                    double *t, double *part_t) {
   if (finished(sc)) return;
   double a[2] = {0.0, 0.0};
-  APH_VEC_LOOP(n, c) {
-    const double tc = row_dot(A, c, z);
+  spmv(n, A, z, [&](int64_t c, double tc) {
     t[c] = tc;
     a[0] += tc * s[c];
     a[1] += tc * tc;
-  }
-  block_partials<2>(a, part);
+  });
+  block_partials<2>(a, part_t);
 }
 
-// x += omega z, r = s - omega t; partials (r, r), (rh, r)
-__global__ void __launch_bounds__(kThreads) k_bi_x(int64_t n, const double *sc, const double *z,
-                                                   const double *s, const double *t,
-                                                   const double *rh, double *x, double *r,
-                                                   double *part) {
+// omega = (t, s) / (t, t); x += omega z, r = s - omega t; partials (r, r), (rh, r)
+__global__ void __launch_bounds__(kThreads)
+    k_bi_x(int64_t n, int nparts, double *sc, const double *part_t, const double *z,
+           const double *s, const double *t, const double *rh, double *x, double *r,
+           double *part_x) {
   if (finished(sc)) return;
-  const double om = sc[OMEGA];
+  double tt[2];
+  sum_partials<2>(part_t, nparts, tt);
+  if (tt[1] == 0.0) {  // t = A z = 0: s = 0, x is the solution
+    if (lead()) {
+      sc[RR] = 0.0;
+      sc[ITERS] += 1.0;
+      sc[DONE] = 1.0;
+    }
+    return;
+  }
+  const double om = tt[0] / tt[1];
+  if (lead()) {
+    sc[TS] = tt[0];
+    sc[TT] = tt[1];
+    sc[OMEGA] = om;
+    sc[ITERS] += 1.0;
+  }
   double a[2] = {0.0, 0.0};
   APH_VEC_LOOP(n, c) {
     x[c] += om * z[c];
@@ -640,7 +694,7 @@ __global__ void __launch_bounds__(kThreads) k_bi_x(int64_t n, const double *sc, 
     a[0] += rc * rc;
     a[1] += rh[c] * rc;
   }
-  block_partials<2>(a, part);
+  block_partials<2>(a, part_x);
 }
 
 __device__ __forceinline__ double face_potential(const double *coef, const aph_block &bk,
@@ -733,35 +787,44 @@ int tree_pass(aph_context *ctx, aph_poisson *ps, const double *field, double dfl
   const int64_t n = ps->nt;
   const int parts =
       (int)std::max<int64_t>(1, std::min<int64_t>((n + kThreads - 1) / kThreads, kRedBlocks));
-  const dim3 gr((unsigned)parts), bl(kThreads);
+  const int mparts = (int)std::max<int64_t>(
+      1, std::min<int64_t>((n * kLanes + kThreads - 1) / kThreads, kRedBlocks));
+  const dim3 gr((unsigned)parts), gm((unsigned)mparts), bl(kThreads);
   const Csr A{ps->d_ptr, ps->d_col, ps->d_val};
   const double tol2 = ps->tol * ps->tol;
+  double *part_x = ps->part, *part_v = ps->part + 2 * kRedBlocks,
+         *part_t = ps->part + 3 * kRedBlocks;
   hipLaunchKernelGGL(k_bi_init, gr, bl, 0, st, n, ps->b, ps->x, ps->r, ps->rh, ps->p, ps->v,
-                     ps->part);
-  hipLaunchKernelGGL(k_bi_final, dim3(1), bl, 0, st, parts, 1, 0, tol2, ps->sc, ps->part);
-  bool converged = false;
-  for (int it = 0; it < ps->max_it; ++it) {
-    hipLaunchKernelGGL(k_bi_p, gr, bl, 0, st, n, ps->sc, ps->r, ps->v, ps->d_dinv, ps->p, ps->y);
-    hipLaunchKernelGGL(k_bi_av, gr, bl, 0, st, n, A, ps->sc, ps->y, ps->rh, ps->v, ps->part);
-    hipLaunchKernelGGL(k_bi_final, dim3(1), bl, 0, st, parts, 1, 1, tol2, ps->sc, ps->part);
-    hipLaunchKernelGGL(k_bi_s, gr, bl, 0, st, n, ps->sc, ps->r, ps->v, ps->y, ps->d_dinv, ps->x,
-                       ps->s, ps->z);
-    hipLaunchKernelGGL(k_bi_at, gr, bl, 0, st, n, A, ps->sc, ps->z, ps->s, ps->tv, ps->part);
-    hipLaunchKernelGGL(k_bi_final, dim3(1), bl, 0, st, parts, 2, 2, tol2, ps->sc, ps->part);
-    hipLaunchKernelGGL(k_bi_x, gr, bl, 0, st, n, ps->sc, ps->z, ps->s, ps->tv, ps->rh, ps->x,
-                       ps->r, ps->part);
-    hipLaunchKernelGGL(k_bi_final, dim3(1), bl, 0, st, parts, 2, 3, tol2, ps->sc, ps->part);
-    if (it % 4 == 3 || it == ps->max_it - 1) {  // read the flag now and then; later
-                                                // kernels are no-ops once it is set
-      APH_HIP_CHECK(ctx, hipMemcpyAsync(ps->h_sc, ps->sc, sizeof(double) * NSC_ALL,
-                                        hipMemcpyDeviceToHost, st));
-      APH_HIP_CHECK(ctx, hipStreamSynchronize(st));
-      if (ps->h_sc[DONE] != 0.0) {
-        converged = ps->h_sc[FAILED] == 0.0;
-        break;
-      }
+                     ps->sc, part_x);
+  bool done = false;
+  auto poll = [&]() -> int {  // later kernels are no-ops once the flag is set
+    APH_HIP_CHECK(ctx, hipMemcpyAsync(ps->h_sc, ps->sc, sizeof(double) * NSC_ALL,
+                                      hipMemcpyDeviceToHost, st));
+    APH_HIP_CHECK(ctx, hipStreamSynchronize(st));
+    done = ps->h_sc[DONE] != 0.0;
+    return APH_OK;
+  };
+  for (int it = 0; it < ps->max_it && !done; ++it) {
+    hipLaunchKernelGGL(k_bi_p, gr, bl, 0, st, it, 0, n, parts, tol2, ps->sc, part_x, ps->r,
+                       ps->v, ps->d_dinv, ps->p, ps->y);
+    hipLaunchKernelGGL(k_bi_av, gm, bl, 0, st, n, A, ps->sc, ps->y, ps->rh, ps->v, part_v);
+    hipLaunchKernelGGL(k_bi_s, gr, bl, 0, st, it, n, mparts, ps->sc, part_v, ps->r, ps->v,
+                       ps->y, ps->d_dinv, ps->x, ps->s, ps->z);
+    hipLaunchKernelGGL(k_bi_at, gm, bl, 0, st, n, A, ps->sc, ps->z, ps->s, ps->tv, part_t);
+    hipLaunchKernelGGL(k_bi_x, gr, bl, 0, st, n, mparts, ps->sc, part_t, ps->z, ps->s, ps->tv,
+                       ps->rh, ps->x, ps->r, part_x);
+    if (it % 4 == 3) {
+      int rc = poll();
+      if (rc) return rc;
     }
   }
+  if (!done) {  // the residual test of the last iteration
+    hipLaunchKernelGGL(k_bi_p, gr, bl, 0, st, ps->max_it, 1, n, parts, tol2, ps->sc, part_x,
+                       ps->r, ps->v, ps->d_dinv, ps->p, ps->y);
+    int rc = poll();
+    if (rc) return rc;
+  }
+  const bool converged = done && ps->h_sc[FAILED] == 0.0;
   if (!converged)
     return set_error(ctx, APH_E_STATE,
                      "poisson gravity: %s (tolerance %g, %d iterations, relative residual %g)",
@@ -1002,7 +1065,7 @@ int aph_poisson_create_tree(aph_context *ctx, const aph_blockset *bs, const int3
   for (double **v : {&ps->x, &ps->b, &ps->r, &ps->rh, &ps->p, &ps->v, &ps->y, &ps->s, &ps->z,
                      &ps->tv})
     if (e == hipSuccess) e = hipMalloc(v, sizeof(double) * n);
-  if (e == hipSuccess) e = hipMalloc(&ps->part, sizeof(double) * 2 * kRedBlocks);
+  if (e == hipSuccess) e = hipMalloc(&ps->part, sizeof(double) * 5 * kRedBlocks);
   if (e == hipSuccess) e = hipMalloc(&ps->sc, sizeof(double) * NSC_ALL);
   if (e == hipSuccess) e = hipMalloc(&ps->coef, sizeof(double) * 9);
   if (e == hipSuccess) e = hipMalloc(&ps->bmom, sizeof(double) * 9 * bs->nblocks);

@@ -594,6 +594,19 @@ def profile_run(args, ctx, hydro, problems, torch):
         _calibration_copies(torch)
         print(f"profile-run {args.profile_run}: done", flush=True)
         return
+    if args.profile_run == "poisson_smr":
+        # Evrard's smr_16 (120 blocks of 8^3 over levels 1 and 2) with the deck's Poisson
+        # problem on the multilevel mesh: 3 RK2 steps in the fused form, each stage preceded
+        # by the solver's moments, right-hand side, BiCGSTAB iterations and field
+        p = problems.evrard(res=16, block=8, smr_level=2, solver="poisson")
+        fd = hydro.FusedDriver(ctx, p.bs, p.cons, p.recon, p.riemann, p.eos, p.cfl,
+                               tlim=p.tlim, poisson=p.poisson, amr_mesh=p.meta["mesh"])
+        fd.initial_dt()
+        for _ in range(3):
+            fd.step()
+        torch.cuda.synchronize()
+        print(f"profile-run {args.profile_run}: done", flush=True)
+        return
     if args.profile_run == "occupancy":
         # the headline workload (256^3 blast, 8x8x8 tiles, 64 lanes) with more LDS reserved
         # per workgroup, so fewer workgroups are resident per CU: the time per stage launch
@@ -721,7 +734,7 @@ def main():
                     help="RT instances of --profile-run rt_adjoint")
     ap.add_argument("--profile-run", choices=["fused", "graph", "plain", "uneven", "autotune",
                                               "occupancy", "monopole", "adjoint",
-                                              "rt_adjoint", "amr"],
+                                              "rt_adjoint", "amr", "poisson_smr"],
                     default=None,
                     help="short eager run for rocprofv3 (no timing, no JSON)")
     args = ap.parse_args()

@@ -1183,3 +1183,177 @@ extern "C" int aph_poisson_tree_matrix(const aph_block *blocks, int32_t nblocks,
   if (err && err_len) err[0] = 0;
   return APH_OK;
 }
+
+// ------------------------------------------------------------ Poisson multigrid levels
+// Coarser meshes for the multigrid preconditioner of the composite problem: the tree cut at
+// levels Lmax-1, ..., 0 (every leaf deeper than L merged into its level-L ancestor), then
+// the base blocks with nx/2, nx/4, ... cells per axis while the count stays even and at
+// least 2.  Each is assembled by poisson_tree_csr from block tables made here (ghosts 1, no
+// neighbours), so the operator is the same discretisation on the coarser cells.
+namespace {
+
+struct MeshDesc {
+  int nx[3];
+  std::vector<int32_t> level;
+  std::vector<int64_t> loc;
+  std::unordered_map<uint64_t, int> index;  // (level, node) -> block
+};
+
+std::vector<aph_block> make_blocks(const MeshDesc &m, const double lo[3], const double H0[3]) {
+  std::vector<aph_block> out(m.level.size());
+  int64_t base = 0;
+  for (size_t b = 0; b < out.size(); ++b) {
+    aph_block &k = out[b];
+    std::memset(&k, 0, sizeof(k));
+    k.ng = 1;
+    double dx[3];
+    for (int d = 0; d < 3; ++d) {
+      k.nx[d] = m.nx[d];
+      dx[d] = H0[d] / ((double)m.nx[d] * (double)((int64_t)1 << m.level[b]));
+      k.dx[d] = dx[d];
+      k.xmin[d] = lo[d] + (double)m.loc[3 * b + d] * m.nx[d] * dx[d];
+    }
+    k.area[0] = dx[1] * dx[2];
+    k.area[1] = dx[0] * dx[2];
+    k.area[2] = dx[0] * dx[1];
+    k.vol = dx[0] * dx[1] * dx[2];
+    for (int q = 0; q < 27; ++q) k.nbr[q] = -1;
+    k.cell_base = base;
+    base += (int64_t)(m.nx[0] + 2) * (m.nx[1] + 2) * (m.nx[2] + 2);
+  }
+  return out;
+}
+
+// the cell of mesh c holding each cell of mesh f (c is f cut one level lower or with half
+// the cells per block axis)
+std::vector<int32_t> cell_map(const MeshDesc &f, const MeshDesc &c) {
+  const int64_t ncf = (int64_t)f.nx[0] * f.nx[1] * f.nx[2];
+  const int64_t ncc = (int64_t)c.nx[0] * c.nx[1] * c.nx[2];
+  std::vector<int32_t> out((size_t)(ncf * f.level.size()));
+  for (size_t b = 0; b < f.level.size(); ++b) {
+    const int lf = f.level[b];
+    // the coarse block: the leaf of c at this level or an ancestor of it
+    int lc = lf;
+    int64_t node[3] = {f.loc[3 * b], f.loc[3 * b + 1], f.loc[3 * b + 2]};
+    auto it = c.index.find(key(lc, node[0], node[1], node[2]));
+    while (it == c.index.end()) {
+      if (lc == 0) fail("multigrid level: a block has no coarse block");
+      --lc;
+      for (int d = 0; d < 3; ++d) node[d] >>= 1;
+      it = c.index.find(key(lc, node[0], node[1], node[2]));
+    }
+    const int cb = it->second;
+    for (int k = 0; k < f.nx[2]; ++k)
+      for (int j = 0; j < f.nx[1]; ++j)
+        for (int i = 0; i < f.nx[0]; ++i) {
+          const int ijk[3] = {i, j, k};
+          int64_t cl[3];
+          for (int d = 0; d < 3; ++d) {
+            // global index at level lf in f's cells, then at level lc in c's cells
+            int64_t g = f.loc[3 * b + d] * f.nx[d] + ijk[d];
+            g >>= (lf - lc);
+            if (c.nx[d] != f.nx[d]) g >>= 1;
+            cl[d] = g - c.loc[3 * cb + d] * c.nx[d];
+            if (cl[d] < 0 || cl[d] >= c.nx[d]) fail("multigrid level: cell outside its block");
+          }
+          out[(size_t)(b * ncf + ((int64_t)k * f.nx[1] + j) * f.nx[0] + i)] =
+              (int32_t)(cb * ncc + (cl[2] * c.nx[1] + cl[1]) * c.nx[0] + cl[0]);
+        }
+  }
+  return out;
+}
+
+}  // namespace
+
+int aph::poisson_tree_levels(const aph_block *blocks, int32_t nblocks, const int32_t nbase[3],
+                             const int32_t *level, const int64_t *loc, int32_t flux_correct,
+                             int32_t prolongation, double alpha, int64_t max_coarse,
+                             std::vector<PoissonMgLevel> &out, char *err, size_t err_len) {
+  auto report = [&](int code, const char *msg) {
+    if (err && err_len) snprintf(err, err_len, "%s", msg);
+    return code;
+  };
+  out.clear();
+  try {
+    if (nblocks < 1) fail("no blocks");
+    MeshDesc fine;
+    int lmax = 0;
+    for (int d = 0; d < 3; ++d) fine.nx[d] = blocks[0].nx[d];
+    fine.level.assign(level, level + nblocks);
+    fine.loc.assign(loc, loc + 3 * (size_t)nblocks);
+    for (int b = 0; b < nblocks; ++b) {
+      lmax = std::max(lmax, (int)level[b]);
+      fine.index[key(level[b], loc[3 * b], loc[3 * b + 1], loc[3 * b + 2])] = b;
+    }
+    double lo[3], H0[3];
+    for (int d = 0; d < 3; ++d) {
+      const double dx0 = blocks[0].dx[d] * (double)((int64_t)1 << level[0]);
+      H0[d] = dx0 * fine.nx[d];
+      lo[d] = blocks[0].xmin[d] - (double)loc[d] * fine.nx[d] * blocks[0].dx[d];
+    }
+    std::vector<MeshDesc> meshes;
+    for (int L = lmax - 1; L >= 0; --L) {  // the tree cut at L
+      MeshDesc m;
+      for (int d = 0; d < 3; ++d) m.nx[d] = fine.nx[d];
+      for (int b = 0; b < nblocks; ++b) {
+        const int l = std::min((int)level[b], L);
+        int64_t c[3];
+        for (int d = 0; d < 3; ++d) c[d] = loc[3 * b + d] >> (level[b] - l);
+        const uint64_t k = key(l, c[0], c[1], c[2]);
+        if (m.index.count(k)) continue;
+        m.index[k] = (int)m.level.size();
+        m.level.push_back(l);
+        for (int d = 0; d < 3; ++d) m.loc.push_back(c[d]);
+      }
+      meshes.push_back(std::move(m));
+    }
+    for (;;) {  // the base blocks with half the cells per axis
+      const MeshDesc &last = meshes.empty() ? fine : meshes.back();
+      const int64_t n = (int64_t)last.level.size() * last.nx[0] * last.nx[1] * last.nx[2];
+      if (n <= max_coarse) break;
+      bool ok = true;  // the halved count must stay even (the tree's prolongation)
+      for (int d = 0; d < 3; ++d) ok = ok && last.nx[d] % 4 == 0;
+      if (!ok) break;
+      MeshDesc m = last;
+      for (int d = 0; d < 3; ++d) m.nx[d] = last.nx[d] / 2;
+      meshes.push_back(std::move(m));
+    }
+    const MeshDesc *prev = &fine;
+    for (const MeshDesc &m : meshes) {
+      PoissonMgLevel lv;
+      std::vector<aph_block> bt = make_blocks(m, lo, H0);
+      PoissonTreeCsr csr;
+      char msg[256];
+      const int rc = poisson_tree_csr(bt.data(), (int32_t)bt.size(), nbase, m.level.data(),
+                                      m.loc.data(), flux_correct, prolongation, alpha, csr,
+                                      msg, sizeof msg);
+      if (rc) fail("multigrid level: %s", msg);
+      lv.n = csr.n;
+      lv.rowptr = std::move(csr.rowptr);
+      lv.col = std::move(csr.col);
+      lv.val = std::move(csr.val);
+      lv.cmap = cell_map(*prev, m);
+      std::vector<int64_t> cnt((size_t)lv.n + 1, 0);
+      for (int32_t c : lv.cmap) ++cnt[(size_t)c + 1];
+      lv.rptr.assign((size_t)lv.n + 1, 0);
+      for (int64_t c = 0; c < lv.n; ++c) lv.rptr[c + 1] = lv.rptr[c] + cnt[c + 1];
+      lv.rcol.assign(lv.cmap.size(), 0);
+      lv.rval.assign(lv.cmap.size(), 0.0);
+      std::vector<int64_t> at(lv.rptr.begin(), lv.rptr.end() - 1);
+      for (size_t f = 0; f < lv.cmap.size(); ++f) {  // finer cells in ascending order
+        const int32_t c = lv.cmap[f];
+        lv.rcol[at[c]] = (int32_t)f;
+        lv.rval[at[c]] = 1.0 / (double)cnt[(size_t)c + 1];
+        ++at[c];
+      }
+      out.push_back(std::move(lv));
+      prev = &m;
+    }
+  } catch (const BuildError &e) {
+    return report(APH_E_INVALID, e.msg);
+  } catch (const std::bad_alloc &) {
+    return report(APH_E_NOMEM, "host allocation failed");
+  }
+  if (err && err_len) err[0] = 0;
+  return APH_OK;
+}

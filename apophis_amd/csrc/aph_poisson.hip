@@ -1,5 +1,7 @@
 // aph_poisson.hip -- Poisson self-gravity (src/gravity/gsolvers/poisson_gravity.hpp) on a
-// uniform 3D lattice mesh whose outer faces are all physical boundaries.
+// 3D mesh whose outer faces are all physical boundaries: a uniform lattice (below) or a
+// multilevel mesh (the "multilevel meshes" section: the composite operator as CSR,
+// BiCGSTAB with a multigrid V-cycle over coarser meshes of the tree).
 //
 // Per call of the solver's tasks:
 //   k_pmom_block, k_pmom_final  ComputeMPCoeff + ScaleMPCoeff (:430-500): the nine
@@ -55,6 +57,17 @@ struct aph_poisson {
   int32_t *d_col = nullptr, *d_nbcol = nullptr, *d_wall = nullptr;
   double *d_val = nullptr, *d_nbval = nullptr, *d_dinv = nullptr;
   double *rh = nullptr, *v = nullptr, *y = nullptr, *s = nullptr, *tv = nullptr;
+  // multigrid preconditioner of the multilevel solve: level 0 is the operator above,
+  // levels 1.. the coarser meshes (aph_poisson_tree.hpp); empty = the Jacobi preconditioner
+  struct MgLevel {
+    int64_t n = 0;
+    int64_t *ptr = nullptr, *rptr = nullptr;
+    int32_t *col = nullptr, *cmap = nullptr, *rcol = nullptr;
+    double *val = nullptr, *dinv = nullptr, *rval = nullptr;
+    double *r = nullptr, *x = nullptr, *t = nullptr;
+  };
+  std::vector<MgLevel> mg;
+  double *d_cinv = nullptr;  // the coarsest level's inverse (dense, row-major), or NULL
 };
 
 namespace aph {
@@ -450,12 +463,14 @@ void vcycle(aph_context *ctx, aph_poisson *ps, int l, const double *r, double *e
 // ------------------------------------------------------------ multilevel meshes
 // The composite operator of a multilevel mesh as CSR over the leaf cells (block order,
 // cell (i, j, k) of block b at b ncb + (k ny + j) nx + i), assembled on the host from the
-// tree.  BiCGSTAB (the reference's MGBiCGSTAB without the multigrid preconditioner; the
-// operator is not symmetric once prolongation enters it) with x0 = 0, right-preconditioned
-// by the inverse diagonal.  Five launches per iteration, one per global dependency:
-//   k_bi_p   the residual test and rho = (rh, r) from k_bi_x's partials; p, y = p / diag
+// tree.  BiCGSTAB (the operator is not symmetric once prolongation enters it) with x0 = 0,
+// right-preconditioned by one multigrid V-cycle over coarser meshes of the same problem
+// (mg_vcycle below; the reference's MGBiCGSTAB), or by the inverse diagonal where the mesh
+// has no coarser level.  Five launches per iteration besides the two cycles, one per
+// global dependency:
+//   k_bi_p   the residual test and rho = (rh, r) from k_bi_x's partials; p (y = M^-1 p)
 //   k_bi_av  v = A y, partials of (rh, v)
-//   k_bi_s   alpha = rho / (rh, v); s = r - alpha v, x += alpha y, z = s / diag
+//   k_bi_s   alpha = rho / (rh, v); s = r - alpha v, x += alpha y (z = M^-1 s)
 //   k_bi_at  t = A z, partials of (t, s), (t, t)
 //   k_bi_x   omega = (t, s) / (t, t); x += omega z, r = s - omega t, partials of (r, r),
 //            (rh, r)
@@ -604,7 +619,7 @@ __global__ void __launch_bounds__(kThreads)
   APH_VEC_LOOP(n, c) {
     const double pc = r[c] + beta * (p[c] - om * v[c]);
     p[c] = pc;
-    y[c] = pc * dinv[c];
+    if (dinv) y[c] = pc * dinv[c];  // else the multigrid cycle forms y
   }
 }
 
@@ -645,7 +660,7 @@ __global__ void __launch_bounds__(kThreads)
     const double sv = r[c] - al * v[c];
     s[c] = sv;
     x[c] += al * y[c];
-    z[c] = sv * dinv[c];
+    if (dinv) z[c] = sv * dinv[c];  // else the multigrid cycle forms z
   }
 }
 
@@ -695,6 +710,143 @@ __global__ void __launch_bounds__(kThreads)
     a[1] += rh[c] * rc;
   }
   block_partials<2>(a, part_x);
+}
+
+// ---- the multigrid V-cycle over the coarser meshes (the tree cut one level at a time,
+// then the base blocks with half the cells per axis; each the same problem re-assembled):
+// two weighted-Jacobi sweeps before and after (w = 0.8), the residual's mean over each
+// coarse cell down, the coarse correction added back cell by cell, the coarsest level
+// solved with its inverse (or 40 sweeps when it is large).  Four launches per level: both
+// sweeps from zero in one (the first is pointwise), the residual with its restriction,
+// the correction with the first sweep after (also pointwise), the last sweep.  The cycle
+// is a fixed linear map, as BiCGSTAB's preconditioner must be.
+
+// sum_j A_cj f(j) over kLanes lanes per row; g(row, value) on the row's first lane
+template <class F, class G>
+__device__ __forceinline__ void spmv_f(int64_t n, const Csr &A, F f, G g) {
+  const int lane = threadIdx.x & (kLanes - 1);
+  const int64_t step = (int64_t)gridDim.x * blockDim.x / kLanes;
+  for (int64_t c = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kLanes; c < n;
+       c += step) {
+    double s = 0.0;
+    for (int64_t q = A.ptr[c] + lane, e = A.ptr[c + 1]; q < e; q += kLanes)
+      s += A.val[q] * f(A.col[q]);
+    s += __shfl_xor(s, 4, kLanes);
+    s += __shfl_xor(s, 2, kLanes);
+    s += __shfl_xor(s, 1, kLanes);
+    if (lane == 0) g(c, s);
+  }
+}
+
+// two sweeps from zero: x1 = w D^-1 r, x = x1 + w D^-1 (r - A x1)
+__global__ void __launch_bounds__(kThreads) k_mg_pre(int64_t n, Csr A, const double *dinv,
+                                                     const double *sc, const double *r,
+                                                     double *x) {
+  if (finished(sc)) return;
+  spmv_f(n, A, [&](int32_t j) { return kOmega * dinv[j] * r[j]; },
+         [&](int64_t c, double ax) {
+           const double x1 = kOmega * dinv[c] * r[c];
+           x[c] = x1 + kOmega * dinv[c] * (r[c] - ax);
+         });
+}
+
+// the coarse right-hand side: rc = mean over each coarse cell's fine cells of r - A x
+__global__ void __launch_bounds__(kThreads) k_mg_restrict(int64_t nc, Csr R, Csr A,
+                                                          const double *sc, const double *r,
+                                                          const double *x, double *rc) {
+  if (finished(sc)) return;
+  const int lane = threadIdx.x & (kLanes - 1);
+  const int64_t step = (int64_t)gridDim.x * blockDim.x / kLanes;
+  for (int64_t c = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kLanes; c < nc;
+       c += step) {
+    double v = 0.0;
+    const int64_t q = R.ptr[c] + lane;
+    if (q < R.ptr[c + 1]) {  // at most 2^3 fine cells
+      const int32_t f = R.col[q];
+      v = R.val[q] * (r[f] - row_dot(A, f, x));
+    }
+    v += __shfl_xor(v, 4, kLanes);
+    v += __shfl_xor(v, 2, kLanes);
+    v += __shfl_xor(v, 1, kLanes);
+    if (lane == 0) rc[c] = v;
+  }
+}
+
+// the coarse correction and the first sweep after: xn = x + ec[cmap],
+// out = xn + w D^-1 (r - A xn)
+__global__ void __launch_bounds__(kThreads)
+    k_mg_post1(int64_t n, Csr A, const double *dinv, const double *sc, const double *r,
+               const double *x, const int32_t *cmap, const double *ec, double *out) {
+  if (finished(sc)) return;
+  spmv_f(n, A, [&](int32_t j) { return x[j] + ec[cmap[j]]; },
+         [&](int64_t c, double ax) {
+           const double xn = x[c] + ec[cmap[c]];
+           out[c] = xn + kOmega * dinv[c] * (r[c] - ax);
+         });
+}
+
+// one sweep: out = t + w D^-1 (r - A t)
+__global__ void __launch_bounds__(kThreads) k_mg_sweep(int64_t n, Csr A, const double *dinv,
+                                                       const double *sc, const double *r,
+                                                       const double *t, double *out) {
+  if (finished(sc)) return;
+  spmv_f(n, A, [&](int32_t j) { return t[j]; },
+         [&](int64_t c, double ax) { out[c] = t[c] + kOmega * dinv[c] * (r[c] - ax); });
+}
+
+// the coarsest level solved: x = A^-1 r with the dense inverse, one wave per row
+__global__ void __launch_bounds__(kThreads) k_mg_dense(int64_t n, const double *inv,
+                                                       const double *sc, const double *r,
+                                                       double *x) {
+  if (finished(sc)) return;
+  const int lane = threadIdx.x & 63;
+  const int64_t step = (int64_t)gridDim.x * blockDim.x / 64;
+  for (int64_t c = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64; c < n; c += step) {
+    double v = 0.0;
+    for (int64_t j = lane; j < n; j += 64) v += inv[c * n + j] * r[j];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane == 0) x[c] = v;
+  }
+}
+
+constexpr int kCoarseDense = 512;   // largest coarsest level solved with its inverse
+constexpr int kCoarseJacobi = 40;   // sweeps on a larger coarsest level
+
+dim3 lane_grid(int64_t n, int lanes) {
+  return dim3((unsigned)std::max<int64_t>(
+      1, std::min<int64_t>((n * lanes + kThreads - 1) / kThreads, 4096)));
+}
+
+// x = M^-1 r on level l (level 0: the operator of the solve)
+void mg_vcycle(aph_poisson *ps, hipStream_t st, size_t l, const double *r, double *x) {
+  const aph_poisson::MgLevel &L = ps->mg[l];
+  const Csr A{L.ptr, L.col, L.val};
+  const dim3 bl(kThreads);
+  if (l + 1 == ps->mg.size()) {  // the coarsest level
+    if (ps->d_cinv) {
+      hipLaunchKernelGGL(k_mg_dense, lane_grid(L.n, 64), bl, 0, st, L.n, ps->d_cinv, ps->sc, r,
+                         x);
+      return;
+    }
+    hipLaunchKernelGGL(k_mg_pre, lane_grid(L.n, kLanes), bl, 0, st, L.n, A, L.dinv, ps->sc, r,
+                       x);
+    for (int k = 2; k < kCoarseJacobi; k += 2) {
+      hipLaunchKernelGGL(k_mg_sweep, lane_grid(L.n, kLanes), bl, 0, st, L.n, A, L.dinv,
+                         ps->sc, r, x, L.t);
+      hipLaunchKernelGGL(k_mg_sweep, lane_grid(L.n, kLanes), bl, 0, st, L.n, A, L.dinv,
+                         ps->sc, r, L.t, x);
+    }
+    return;
+  }
+  const aph_poisson::MgLevel &C = ps->mg[l + 1];
+  hipLaunchKernelGGL(k_mg_pre, lane_grid(L.n, kLanes), bl, 0, st, L.n, A, L.dinv, ps->sc, r, x);
+  hipLaunchKernelGGL(k_mg_restrict, lane_grid(C.n, kLanes), bl, 0, st, C.n,
+                     Csr{C.rptr, C.rcol, C.rval}, A, ps->sc, r, x, C.r);
+  mg_vcycle(ps, st, l + 1, C.r, C.x);
+  hipLaunchKernelGGL(k_mg_post1, lane_grid(L.n, kLanes), bl, 0, st, L.n, A, L.dinv, ps->sc, r,
+                     x, C.cmap, C.x, L.t);
+  hipLaunchKernelGGL(k_mg_sweep, lane_grid(L.n, kLanes), bl, 0, st, L.n, A, L.dinv, ps->sc, r,
+                     L.t, x);
 }
 
 __device__ __forceinline__ double face_potential(const double *coef, const aph_block &bk,
@@ -804,16 +956,21 @@ int tree_pass(aph_context *ctx, aph_poisson *ps, const double *field, double dfl
     done = ps->h_sc[DONE] != 0.0;
     return APH_OK;
   };
+  const bool mg = ps->mg.size() > 1;
+  const double *jac = mg ? nullptr : ps->d_dinv;  // the Jacobi preconditioner without levels
+  const int poll_every = mg ? 2 : 4;
   for (int it = 0; it < ps->max_it && !done; ++it) {
     hipLaunchKernelGGL(k_bi_p, gr, bl, 0, st, it, 0, n, parts, tol2, ps->sc, part_x, ps->r,
-                       ps->v, ps->d_dinv, ps->p, ps->y);
+                       ps->v, jac, ps->p, ps->y);
+    if (mg) mg_vcycle(ps, st, 0, ps->p, ps->y);
     hipLaunchKernelGGL(k_bi_av, gm, bl, 0, st, n, A, ps->sc, ps->y, ps->rh, ps->v, part_v);
     hipLaunchKernelGGL(k_bi_s, gr, bl, 0, st, it, n, mparts, ps->sc, part_v, ps->r, ps->v,
-                       ps->y, ps->d_dinv, ps->x, ps->s, ps->z);
+                       ps->y, jac, ps->x, ps->s, ps->z);
+    if (mg) mg_vcycle(ps, st, 0, ps->s, ps->z);
     hipLaunchKernelGGL(k_bi_at, gm, bl, 0, st, n, A, ps->sc, ps->z, ps->s, ps->tv, part_t);
     hipLaunchKernelGGL(k_bi_x, gr, bl, 0, st, n, mparts, ps->sc, part_t, ps->z, ps->s, ps->tv,
                        ps->rh, ps->x, ps->r, part_x);
-    if (it % 4 == 3) {
+    if (it % poll_every == poll_every - 1) {
       int rc = poll();
       if (rc) return rc;
     }
@@ -1072,6 +1229,85 @@ int aph_poisson_create_tree(aph_context *ctx, const aph_blockset *bs, const int3
   if (e == hipSuccess) e = hipHostMalloc(&ps->h_sc, sizeof(double) * NSC_ALL);
   if (e == hipSuccess) e = hipMemset(ps->sc, 0, sizeof(double) * NSC_ALL);
   if (e == hipSuccess) e = hipMemset(ps->coef, 0, sizeof(double) * 9);
+  // the multigrid levels: level 0 is the operator above, then the coarser meshes
+  std::vector<PoissonMgLevel> lv;
+  rc = poisson_tree_levels(bs->h.data(), bs->nblocks, nbase, level, loc, flux_correct,
+                           prolongation, diagonal_alpha, kCoarseDense, lv, msg, sizeof msg);
+  if (rc) {
+    aph_poisson_destroy(ctx, ps);
+    return set_error(ctx, rc, "aph_poisson_create_tree: %s", msg);
+  }
+  if (!lv.empty() && e == hipSuccess) {
+    ps->mg.resize(lv.size() + 1);
+    aph_poisson::MgLevel &m0 = ps->mg[0];
+    m0.n = m.n;
+    m0.ptr = ps->d_ptr;
+    m0.col = ps->d_col;
+    m0.val = ps->d_val;
+    m0.dinv = ps->d_dinv;
+    e = hipMalloc(&m0.t, sizeof(double) * n);
+    for (size_t l = 0; l < lv.size() && e == hipSuccess; ++l) {
+      const PoissonMgLevel &h = lv[l];
+      aph_poisson::MgLevel &d = ps->mg[l + 1];
+      d.n = h.n;
+      std::vector<double> di((size_t)h.n);
+      for (int64_t r = 0; r < h.n; ++r) {
+        double dg = 0.0;
+        for (int64_t q = h.rowptr[r]; q < h.rowptr[r + 1]; ++q)
+          if (h.col[q] == r) dg = h.val[q];
+        di[r] = dg != 0.0 ? 1.0 / dg : 0.0;
+      }
+      put(&d.ptr, h.rowptr);
+      put(&d.col, h.col);
+      put(&d.val, h.val);
+      put(&d.dinv, di);
+      put(&d.cmap, h.cmap);
+      put(&d.rptr, h.rptr);
+      put(&d.rcol, h.rcol);
+      put(&d.rval, h.rval);
+      for (double **v : {&d.r, &d.x, &d.t})
+        if (e == hipSuccess) e = hipMalloc(v, sizeof(double) * (size_t)h.n);
+    }
+    const PoissonMgLevel &c = lv.back();
+    if (e == hipSuccess && c.n <= kCoarseDense) {  // the coarsest level's inverse
+      const int64_t nc = c.n;
+      std::vector<double> a((size_t)(nc * nc), 0.0), inv((size_t)(nc * nc), 0.0);
+      for (int64_t r = 0; r < nc; ++r) {
+        for (int64_t q = c.rowptr[r]; q < c.rowptr[r + 1]; ++q) a[r * nc + c.col[q]] += c.val[q];
+        inv[r * nc + r] = 1.0;
+      }
+      for (int64_t k = 0; k < nc; ++k) {  // Gauss-Jordan with partial pivoting
+        int64_t pr = k;
+        for (int64_t r = k + 1; r < nc; ++r)
+          if (std::fabs(a[r * nc + k]) > std::fabs(a[pr * nc + k])) pr = r;
+        if (a[pr * nc + k] == 0.0) {
+          aph_poisson_destroy(ctx, ps);
+          return set_error(ctx, APH_E_INVALID,
+                           "aph_poisson_create_tree: singular coarsest multigrid level");
+        }
+        if (pr != k)
+          for (int64_t j = 0; j < nc; ++j) {
+            std::swap(a[k * nc + j], a[pr * nc + j]);
+            std::swap(inv[k * nc + j], inv[pr * nc + j]);
+          }
+        const double piv = 1.0 / a[k * nc + k];
+        for (int64_t j = 0; j < nc; ++j) {
+          a[k * nc + j] *= piv;
+          inv[k * nc + j] *= piv;
+        }
+        for (int64_t r = 0; r < nc; ++r) {
+          if (r == k) continue;
+          const double f = a[r * nc + k];
+          if (f == 0.0) continue;
+          for (int64_t j = 0; j < nc; ++j) {
+            a[r * nc + j] -= f * a[k * nc + j];
+            inv[r * nc + j] -= f * inv[k * nc + j];
+          }
+        }
+      }
+      put(&ps->d_cinv, inv);
+    }
+  }
   if (e != hipSuccess) {
     aph_poisson_destroy(ctx, ps);
     return set_error(ctx, APH_E_HIP, "aph_poisson_create_tree: %s", hipGetErrorString(e));
@@ -1094,6 +1330,16 @@ int aph_poisson_destroy(aph_context *ctx, aph_poisson *ps) {
   for (int l = 0; l < kMaxLevels; ++l)
     for (void *v : {(void *)ps->e[l], (void *)ps->rl[l], (void *)ps->t[l]})
       if (v) (void)hipFree(v);
+  for (size_t l = 0; l < ps->mg.size(); ++l) {  // level 0 borrows the solve's operator
+    aph_poisson::MgLevel &d = ps->mg[l];
+    if (l > 0)
+      for (void *v : {(void *)d.ptr, (void *)d.col, (void *)d.val, (void *)d.dinv,
+                      (void *)d.cmap, (void *)d.rptr, (void *)d.rcol, (void *)d.rval,
+                      (void *)d.r, (void *)d.x})
+        if (v) (void)hipFree(v);
+    if (d.t) (void)hipFree(d.t);
+  }
+  if (ps->d_cinv) (void)hipFree(ps->d_cinv);
   if (ps->h_sc) (void)hipHostFree(ps->h_sc);
   delete ps;
   return APH_OK;

@@ -26,4 +26,25 @@ int poisson_tree_csr(const aph_block *blocks, int32_t nblocks, const int32_t nba
                      int32_t prolongation, double alpha, PoissonTreeCsr &out, char *err,
                      size_t err_len);
 
+// One coarser level of the multigrid preconditioner: the same problem re-assembled on a
+// coarser mesh (the tree cut one level lower, or below the base level each block with half
+// its cells per axis), and the map from the next finer level's cells to its own.
+struct PoissonMgLevel {
+  int64_t n = 0;
+  std::vector<int64_t> rowptr;
+  std::vector<int32_t> col;
+  std::vector<double> val;
+  std::vector<int32_t> cmap;     // per cell of the finer level: the cell of this level
+  std::vector<int64_t> rptr;     // per cell of this level: its finer cells (CSR) ...
+  std::vector<int32_t> rcol;
+  std::vector<double> rval;      // ... weighted 1 / their count (the mean)
+};
+
+// The coarser levels below the mesh of poisson_tree_csr (the same arguments), finest first,
+// down to a level of at most max_coarse cells or blocks of 2 cells per axis.
+int poisson_tree_levels(const aph_block *blocks, int32_t nblocks, const int32_t nbase[3],
+                        const int32_t *level, const int64_t *loc, int32_t flux_correct,
+                        int32_t prolongation, double alpha, int64_t max_coarse,
+                        std::vector<PoissonMgLevel> &out, char *err, size_t err_len);
+
 }  // namespace aph

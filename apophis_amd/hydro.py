@@ -514,8 +514,9 @@ class PoissonGravitySolver:
     side, the solve (to the relative residual tolerance) and the "gravity" / "gravity.phi"
     fields.  mesh None: the blocks tile a box in one lattice (aph_poisson_create); mesh an
     amr.AmrMesh whose block set is dbs's: the composite operator of the multilevel mesh
-    (aph_poisson_create_tree) with the deck's flux_correct and boundary_prolongation
-    ("Linear" or "Constant"; "Quadratic" is not built and is refused)."""
+    (aph_poisson_create_tree: BiCGSTAB with a multigrid V-cycle over coarser meshes of
+    the tree) with the deck's flux_correct and boundary_prolongation ("Linear" or
+    "Constant"; "Quadratic" is not built and is refused)."""
 
     def __init__(self, ctx, dbs, gravity_constant, diagonal_alpha=0.0, tolerance=1e-8,
                  max_iterations=10000, flux_correct=False, prolongation="Linear", mesh=None):

@@ -328,8 +328,9 @@ def evrard_smr_leg(ctx, hydro, problems, torch, solver="monopole"):
     time, zone-cycles/s, and the reference's profile checks on the result.  solver
     "monopole": the reference test's monopole_smr_32 (32^3 base in 16^3 blocks: 120 blocks
     over three levels); "poisson": its smr_16 (16^3 base in 8^3 blocks, the deck's
-    MGBiCGSTAB problem with flux correction, solved by BiCGSTAB + Jacobi on the composite
-    operator), with the time of one solve of the initial state and its iterations."""
+    MGBiCGSTAB problem with flux correction, solved by BiCGSTAB with a multigrid V-cycle
+    on the composite operator), with the time of one solve of the initial state and its
+    iterations."""
     from tests.monopole_checks import evrard_exact, evrard_profiles, evrard_verdict
 
     if solver == "monopole":

@@ -223,9 +223,12 @@ enum aph_poisson_prolongation { APH_POISSON_PROLONG_LINEAR = 0, APH_POISSON_PROL
    face from the same level (a leaf cell or the mean of a refined cell's children) or
    prolongated from the coarser leaf beyond, minus the cell inside beyond a domain face;
    flux_correct (gravity/flux_correct, :89-99) replaces a coarse face's flux over a finer
-   region by the mean of its fine faces.  Assembled once on the host from the tree; each
-   solve runs BiCGSTAB with the Jacobi preconditioner from phi = 0 to the relative
-   residual tolerance (the reference's MGBiCGSTAB reaches the same discrete solution).
+   region by the mean of its fine faces.  Assembled once on the host from the tree, with
+   the same problem on coarser meshes (the tree cut one level at a time, then the base
+   blocks with half the cells per axis); each solve runs BiCGSTAB from phi = 0 to the
+   relative residual tolerance, preconditioned by one multigrid V-cycle over those
+   meshes (the reference's MGBiCGSTAB: the same discrete solution, a like number of
+   iterations; the decks' max_iterations = 15 suffices).
    aph_poisson_solve, aph_poisson_get_info, aph_poisson_destroy and aph_plan_set_poisson
    take the solver as they take the lattice one; the right-hand side, the multipole
    coefficients and the field are the lattice solver's, over the leaves.  Refused: 1D,

@@ -410,6 +410,10 @@ def initialize_gravity(pin):
         if isinstance(fc, str):
             fc = fc.strip().lower() == "true"
         pkg.add_param("flux_correct", bool(fc))
+        sfb = _get(pin, "gravity", "set_flux_boundary", False)  # :42, SetFluxBoundaries
+        if (sfb.strip().lower() == "true") if isinstance(sfb, str) else bool(sfb):
+            raise ApophisError("gravity/set_flux_boundary = true is not built (the fine/"
+                               "coarse flux boundary of PoissonEquation)")
         err_tol = float(_get(pin, "gravity", "err_tol", 1.0e-8))
         pkg.add_param("err_tol", err_tol)
         pkg.add_param("residual_tolerance",

@@ -49,6 +49,9 @@ def test_errors():
     with pytest.raises(hydro.ApophisError, match="Unknown prolongation method"):
         hydro.initialize_gravity({"gravity": {"type": "poisson",
                                               "boundary_prolongation": "Cubic"}})
+    with pytest.raises(hydro.ApophisError, match="set_flux_boundary"):
+        hydro.initialize_gravity({"gravity": {"type": "poisson",
+                                              "set_flux_boundary": "true"}})
 
 
 def test_poisson_defaults():
@@ -62,3 +65,8 @@ def test_poisson_defaults():
     pkg = hydro.initialize_gravity({"gravity": {"type": "poisson", "solver": "MGBiCGSTAB"},
                                     "gravity/solver_params": {"residual_tolerance": 1e-10}})
     assert pkg.param("residual_tolerance") == 1e-10
+    # the SMR decks' own settings (poisson_gravity.hpp:41-44, gravity.cpp:312-363)
+    assert (pkg.param("flux_correct"), pkg.param("boundary_prolongation")) == (False, "Linear")
+    pkg = hydro.initialize_gravity({"gravity": {"type": "poisson", "flux_correct": "true",
+                                                "boundary_prolongation": "Constant"}})
+    assert (pkg.param("flux_correct"), pkg.param("boundary_prolongation")) == (True, "Constant")

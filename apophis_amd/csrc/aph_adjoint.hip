@@ -47,7 +47,7 @@ namespace aph {
 
 // the composition counts NV 12 and 14 (aph_adjoint_wide.hip)
 AxisKernels pick_adj_axis_wide(int recon, int rs, int nv, int nd, int d);
-KFn pick_gather_wide(int nv, int nd);
+KFn pick_gather_wide(int nv, int nd, bool xt);
 KFn pick_prim_wide(int nv, int nd);
 
 AxisKernels pick_adj_axis(int recon, int rs, int nv, int nd, int d) {
@@ -80,10 +80,18 @@ KFn pick_nd(int nd, KFn k1, KFn k2, KFn k3) {
   return nd == 1 ? k1 : (nd == 2 ? k2 : k3);
 }
 
-KFn pick_gather(int nv, int nd) {
-  if (nv == 12 || nv == 14) return pick_gather_wide(nv, nd);
-  if (nv == 5) return pick_nd(nd, k_adj_gather<5, 1>, k_adj_gather<5, 2>, k_adj_gather<5, 3>);
-  if (nv == 6) return pick_nd(nd, k_adj_gather<6, 1>, k_adj_gather<6, 2>, k_adj_gather<6, 3>);
+KFn pick_gather(int nv, int nd, bool xt) {
+  if (nv == 12 || nv == 14) return pick_gather_wide(nv, nd, xt);
+  if (nv == 5)
+    return xt ? pick_nd(nd, k_adj_gather<5, 1, true>, k_adj_gather<5, 2, true>,
+                        k_adj_gather<5, 3, true>)
+              : pick_nd(nd, k_adj_gather<5, 1, false>, k_adj_gather<5, 2, false>,
+                        k_adj_gather<5, 3, false>);
+  if (nv == 6)
+    return xt ? pick_nd(nd, k_adj_gather<6, 1, true>, k_adj_gather<6, 2, true>,
+                        k_adj_gather<6, 3, true>)
+              : pick_nd(nd, k_adj_gather<6, 1, false>, k_adj_gather<6, 2, false>,
+                        k_adj_gather<6, 3, false>);
   return nullptr;
 }
 
@@ -254,7 +262,7 @@ int adj_stage(aph_context *ctx, const aph_blockset *bs, int recon, int rs, const
   };
   dim3 gp, bp, gg, bg;
   tile_launch(prim_tile_y(nf), gp, bp);
-  tile_launch(gather_tile_y(nf), gg, bg);
+  tile_launch(a.pbar_xt ? gather_tile_y(nf) : kGatherTileYMarch, gg, bg);
   if (ks[0].face) hipLaunchKernelGGL(pick_prim(bs->nvar, nd), gp, bp, 0, ctx->stream, a);
   const unsigned gops = amr ? (unsigned)std::max<int64_t>(
                                   1, std::min<int64_t>((amr->nops + 255) / 256, 4096))
@@ -282,7 +290,7 @@ int adj_stage(aph_context *ctx, const aph_blockset *bs, int recon, int rs, const
   if (ks[0].axis)  // one-march form: all axes in one launch, z = axis
     hipLaunchKernelGGL(ks[0].axis, dim3((unsigned)((maxu_all + 63) / 64), nbk, (unsigned)nd),
                        dim3(64), 0, ctx->stream, a);
-  hipLaunchKernelGGL(pick_gather(bs->nvar, nd), gg, bg, 0, ctx->stream, a);
+  hipLaunchKernelGGL(pick_gather(bs->nvar, nd, a.pbar_xt), gg, bg, 0, ctx->stream, a);
   if (amr && amr->nops) {  // the cross-level ghosts' cotangents back to the cells they read
     hipLaunchKernelGGL(pick_amr(bs->nvar, nd, 1), dim3(gops), dim3(256), 0, ctx->stream, a);
     const unsigned gt = (unsigned)std::max<int64_t>(

@@ -22,11 +22,10 @@ APH_HD void plm_adj(const double *q, double qlb, double qrb, double *qb) {
   const double dq2 = dql * dqr;
   qb[1] += qlb + qrb;
   const double dqmb = qlb - qrb;
-  if (dq2 > 0.0) {
-    const double s = dql + dqr;
-    const double dqm = dq2 / s;
-    const double dq2b = dqmb / s;
-    const double sb = -dqmb * dqm / s;
+  if (dq2 > 0.0) {  // dqm = dq2 / s: one reciprocal for its three quotients
+    const double is = 1.0 / (dql + dqr);
+    const double dq2b = dqmb * is;
+    const double sb = -dq2b * (dq2 * is);
     const double dqlb = dq2b * dqr + sb;
     const double dqrb = dq2b * dql + sb;
     qb[1] += dqlb - dqrb;
@@ -433,17 +432,21 @@ APH_HD void recon1_adj(const double *w, double qlb, double qrb, double *wb) {
 
 // ------------------------------------------------------------------ sound speed
 // c = C*sqrt(g / (1 + (ig + rho*C^2/p)));  adds cotangents of (g, ig, rho, p)
-APH_HD void sound_speed_adj(double g, double ig, double rho, double p, double cb, double &gb,
-                            double &igb, double &rhob, double &pb) {
-  const double t = rho * C_LIGHT2 / p;
-  const double dl = 1 + (ig + t);
-  const double s = g / dl;
-  const double sb = cb * C_LIGHT * 0.5 / sqrt(s);
-  gb += sb / dl;
-  const double dlb = -sb * s / dl;
+// c = C sqrt(s), s = g / dl, dl = 1 + (ig + t), t = rho C^2 / p; c is the forward value, so
+// ds = dc C^2 / (2 c) (sqrt(s) = c / C) needs no square root, and p and dl are inverted once
+// each.
+APH_HD void sound_speed_adj(double g, double ig, double rho, double p, double c, double cb,
+                            double &gb, double &igb, double &rhob, double &pb) {
+  const double ip = 1.0 / p;
+  const double t = rho * C_LIGHT2 * ip;
+  const double idl = 1.0 / (1 + (ig + t));
+  const double s = g * idl;
+  const double sb = cb * (0.5 * C_LIGHT2) / c;
+  gb += sb * idl;
+  const double dlb = -sb * s * idl;
   igb += dlb;
-  rhob += dlb * C_LIGHT2 / p;
-  pb -= dlb * t / p;
+  rhob += dlb * C_LIGHT2 * ip;
+  pb -= dlb * t * ip;
 }
 
 // ------------------------------------------------------------------ HLLC / LHLLC
@@ -531,13 +534,15 @@ APH_HD void hllc_adj(const double *wl, const double *wr, const double *ifl, cons
   ustb += flxb[1] * cp + flxb[4] * cp * sstar;
   cpb += flxb[1] * ust + flxb[4] * ust * sstar;
   sstarb += flxb[4] * ust * cp;
-  const double D2 = D * D;
+  // the reverse's quotients share one reciprocal per denominator (the cotangents are
+  // checked to a tolerance; the forward values above, which pick the branches, are exact)
+  const double iD2 = 1.0 / (D * D);
   if (left) {
-    sstarb += ulb * (-bm / D2) + ustb * (bm / D2);
-    bmb += ulb * (sstar / D2) + ustb * (-sstar / D2);
+    sstarb += (ustb - ulb) * bm * iD2;
+    bmb += (ulb - ustb) * sstar * iD2;
   } else {
-    sstarb += urb * (-bp / D2) + ustb * (bp / D2);
-    bpb += urb * (sstar / D2) + ustb * (-sstar / D2);
+    sstarb += (ustb - urb) * bp * iD2;
+    bpb += (urb - ustb) * sstar * iD2;
   }
   // fluxes along bm / bp
   double vxl2b = 0, vxr2b = 0, elb = 0, erb = 0;
@@ -563,12 +568,13 @@ APH_HD void hllc_adj(const double *wl, const double *wr, const double *ifl, cons
   bpb -= vxr2b;
   // contact pressure and speed
   double cp0b = (cp0 > 0.0) ? cpb : 0.0;
-  const double Nb = cp0b / M;
-  double Mb = -cp0b * cp0 / M;
+  const double iM = 1.0 / M;
+  const double Nb = cp0b * iM;
+  double Mb = -cp0b * cp0 * iM;
   double mlb = Nb * tr, mrb = Nb * tl, trb = Nb * ml, tlb = Nb * mr;
-  tlb += sstarb / M;
-  trb -= sstarb / M;
-  Mb -= sstarb * sstar / M;
+  tlb += sstarb * iM;
+  trb -= sstarb * iM;
+  Mb -= sstarb * sstar * iM;
   mlb += Mb;
   mrb += Mb;
   double vxlb = 0, vxrb = 0;
@@ -601,18 +607,23 @@ APH_HD void hllc_adj(const double *wl, const double *wr, const double *ifl, cons
   qrb += srb * cr;
   double pstarb = 0, Kb = 0;
   if (shl) {
-    const double Xb = qlb / (2.0 * ql);
-    Kb += Xb * (pstar / wl[4] - 1.0);
-    pstarb += Xb * K / wl[4];
-    wlb[4] -= Xb * K * pstar / (wl[4] * wl[4]);
+    const double Xb = qlb / (2.0 * ql), ip = 1.0 / wl[4];
+    Kb += Xb * (pstar * ip - 1.0);
+    pstarb += Xb * K * ip;
+    wlb[4] -= Xb * K * pstar * ip * ip;
   }
   if (shr) {
-    const double Xb = qrb / (2.0 * qr);
-    Kb += Xb * (pstar / wr[4] - 1.0);
-    pstarb += Xb * K / wr[4];
-    wrb[4] -= Xb * K * pstar / (wr[4] * wr[4]);
+    const double Xb = qrb / (2.0 * qr), ip = 1.0 / wr[4];
+    Kb += Xb * (pstar * ip - 1.0);
+    pstarb += Xb * K * ip;
+    wrb[4] -= Xb * K * pstar * ip * ip;
   }
-  double glb = Kb / (2 * gr), grb = -Kb * (gl + 1) / (2 * gr * gr);
+  double glb = 0.0, grb = 0.0;  // K = (gl + 1) / (2 gr) enters only through a shock
+  if (shl || shr) {
+    const double igr2 = 1.0 / (2 * gr);
+    glb = Kb * igr2;
+    grb = -glb * (gl + 1) * (2 * igr2);
+  }
   // pstar = .5 (pl + pr) - h * rhoa * ca [* phi]
   wlb[4] += .5 * pstarb;
   wrb[4] += .5 * pstarb;
@@ -622,10 +633,13 @@ APH_HD void hllc_adj(const double *wl, const double *wr, const double *ifl, cons
     rawb = rawb * phi;
     const double chib = phib * (2.0 - 2.0 * chi);
     if (machl < machr) {
-      if (machr > 0.0)
-        for (int k = 1; k <= 3; ++k) wrb[k] += chib * wr[k] / machr;
+      if (machr > 0.0) {
+        const double im = chib / machr;
+        for (int k = 1; k <= 3; ++k) wrb[k] += im * wr[k];
+      }
     } else if (machl > 0.0) {
-      for (int k = 1; k <= 3; ++k) wlb[k] += chib * wl[k] / machl;
+      const double im = chib / machl;
+      for (int k = 1; k <= 3; ++k) wlb[k] += im * wl[k];
     }
   }
   const double hb = rawb * rhoa * ca;
@@ -648,8 +662,8 @@ APH_HD void hllc_adj(const double *wl, const double *wr, const double *ifl, cons
     wrb[k] += erb * (0.5 * wr[0]) * 2.0 * wr[k];
   }
   // sound speeds
-  sound_speed_adj(gl, igl, wl[0], wl[4], clb, glb, iglb, wlb[0], wlb[4]);
-  sound_speed_adj(gr, igr, wr[0], wr[4], crb, grb, igrb, wrb[0], wrb[4]);
+  sound_speed_adj(gl, igl, wl[0], wl[4], cl, clb, glb, iglb, wlb[0], wlb[4]);
+  sound_speed_adj(gr, igr, wr[0], wr[4], cr, crb, grb, igrb, wrb[0], wrb[4]);
   iflb[1] += -iglb * igl * igl;
   ifrb[1] += -igrb * igr * igr;
   iflb[0] += glb;
@@ -704,10 +718,10 @@ APH_HD void hlle_adj(const double *wl, const double *wr, const double *ifl, cons
   }
   double bpb = 0, bmb = 0;
   if (split) {
-    const double den = bp - bm;
+    const double iden = 1.0 / (bp - bm);
     // tmp = 0.5 (bp + bm) / den
-    bpb += tmpb * (0.5 / den - tmp / den);
-    bmb += tmpb * (0.5 / den + tmp / den);
+    bpb += tmpb * ((0.5 - tmp) * iden);
+    bmb += tmpb * ((0.5 + tmp) * iden);
   }
   double vxlb = 0, vxrb = 0, elb = 0, erb = 0;
   wlb[0] += flb[0] * vxl + flb[1] * wl[1] * vxl + flb[2] * wl[2] * vxl + flb[3] * wl[3] * vxl;
@@ -741,18 +755,27 @@ APH_HD void hlle_adj(const double *wl, const double *wr, const double *ifl, cons
   qrb += arb * cr;
   double pmidb = 0, Flb = 0, Frb = 0;
   if (shl) {
-    Flb += qlb * (pmid / wl[4] - 1.0);
-    pmidb += qlb * Fl / wl[4];
-    wlb[4] -= qlb * Fl * pmid / (wl[4] * wl[4]);
+    const double ip = 1.0 / wl[4];
+    Flb += qlb * (pmid * ip - 1.0);
+    pmidb += qlb * Fl * ip;
+    wlb[4] -= qlb * Fl * pmid * ip * ip;
   }
   if (shr) {
-    Frb += qrb * (pmid / wr[4] - 1.0);
-    pmidb += qrb * Fr / wr[4];
-    wrb[4] -= qrb * Fr * pmid / (wr[4] * wr[4]);
+    const double ip = 1.0 / wr[4];
+    Frb += qrb * (pmid * ip - 1.0);
+    pmidb += qrb * Fr * ip;
+    wrb[4] -= qrb * Fr * pmid * ip * ip;
   }
-  // F = (g+1)/sqrt(2g): dF/dg = 1/sqrt(2g) - (g+1)/(2g sqrt(2g))
-  double glb = Flb * (1.0 / sqrt(2 * gl) - (gl + 1) / (2 * gl * sqrt(2 * gl)));
-  double grb = Frb * (1.0 / sqrt(2 * gr) - (gr + 1) / (2 * gr * sqrt(2 * gr)));
+  // F = (g+1)/sqrt(2g): dF/dg = (1 - (g+1)/(2g)) / sqrt(2g), only through a shock
+  double glb = 0.0, grb = 0.0;
+  if (shl) {
+    const double ir = 1.0 / sqrt(2 * gl);
+    glb = Flb * ((1.0 - (gl + 1) * (0.5 / gl)) * ir);
+  }
+  if (shr) {
+    const double ir = 1.0 / sqrt(2 * gr);
+    grb = Frb * ((1.0 - (gr + 1) * (0.5 / gr)) * ir);
+  }
   // pmid = .5 (pl + pr + (ul - ur) rhoa ca)
   wlb[4] += .5 * pmidb;
   wrb[4] += .5 * pmidb;
@@ -775,8 +798,8 @@ APH_HD void hlle_adj(const double *wl, const double *wr, const double *ifl, cons
     wlb[k] += elb * (0.5 * wl[0]) * 2.0 * wl[k];
     wrb[k] += erb * (0.5 * wr[0]) * 2.0 * wr[k];
   }
-  sound_speed_adj(gl, igl, wl[0], wl[4], clb, glb, iglb, wlb[0], wlb[4]);
-  sound_speed_adj(gr, igr, wr[0], wr[4], crb, grb, igrb, wrb[0], wrb[4]);
+  sound_speed_adj(gl, igl, wl[0], wl[4], cl, clb, glb, iglb, wlb[0], wlb[4]);
+  sound_speed_adj(gr, igr, wr[0], wr[4], cr, crb, grb, igrb, wrb[0], wrb[4]);
   iflb[1] += -iglb * igl * igl;
   ifrb[1] += -igrb * igr * igr;
   iflb[0] += glb;
@@ -822,14 +845,16 @@ APH_HD void cons_to_prim_adj(const double *u, const aph_eos &eos, const double *
     const bool bsmall = !(EOS_SMALL < braw);
     const double bulk = bsmall ? EOS_SMALL : braw;
     const double ee = u_e - e_k;
-    // gamma_e = w_p / ee + 1
-    double w_pb = gmb[1] / ee;
-    const double eeb = -gmb[1] * w_p / (ee * ee);
+    // gamma_e = w_p / ee + 1 (the reverse's quotients share one reciprocal per
+    // denominator; the forward values above, which pick the floor branches, are exact)
+    const double iee = 1.0 / ee, iwp = 1.0 / w_p;
+    double w_pb = gmb[1] * iee;
+    const double eeb = -w_pb * w_p * iee;
     u_eb += eeb;
     e_kb -= eeb;
     // gamma_c = bulk / w_p
-    const double bulkb = gmb[0] / w_p;
-    w_pb -= gmb[0] * bulk / (w_p * w_p);
+    const double bulkb = gmb[0] * iwp;
+    w_pb -= bulkb * bulk * iwp;
     if (!bsmall) {
       u_db += bulkb * g * sie;
       sieb += bulkb * g * u_d;
@@ -839,9 +864,10 @@ APH_HD void cons_to_prim_adj(const double *u, const aph_eos &eos, const double *
       u_db += w_pb * gm1e * sie;
       sieb += w_pb * gm1e * u_d;
     }
-    u_eb += sieb / u_d;
-    e_kb -= sieb / u_d;
-    u_db -= sieb * sie / u_d;
+    const double siebd = sieb * di;
+    u_eb += siebd;
+    e_kb -= siebd;
+    u_db -= siebd * sie;
   }
   dib += e_kb * 0.5 * S;
   for (int k = 1; k <= 3; ++k) {

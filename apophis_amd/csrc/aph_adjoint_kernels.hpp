@@ -308,6 +308,7 @@ __device__ __forceinline__ Lay axis_lay(int pa, int pb, int N0, int N1, int N2,
 // of every field within 64 KB.
 constexpr int prim_tile_y(int nf) { return nf > 8 ? 8 : 32; }
 constexpr int gather_tile_y(int nf) { return nf > 8 ? 8 : 16; }
+constexpr int kGatherTileYMarch = 8;  // the one-march form's gather (no LDS tile)
 
 template <int NV, int NDIM>
 __global__ void __launch_bounds__(1024) k_adj_prim(AdjArgs a) {
@@ -963,13 +964,45 @@ __device__ __forceinline__ void add_slot(const AdjArgs &a, int d, int blk, int p
   for (int v = 0; v < NF; ++v) acc[v] += p[v * fstr];
 }
 
-template <int NV, int NDIM>
+// A block's spill records along one axis, its layout resolved once (spill_at's arithmetic
+// without the per-call block-table reads and divisions).
+struct SpillView {
+  const double *base;
+  int64_t npen, nseg;
+  bool segfast;
+};
+
+template <int NF>
+__device__ __forceinline__ void add_slot_view(const SpillView &v, int pencil, int seg, int slot,
+                                              double *acc) {
+  const double *p;
+  int64_t fstr;
+  if (v.segfast) {
+    fstr = v.npen * v.nseg;
+    p = v.base + (int64_t)slot * NF * fstr + (int64_t)pencil * v.nseg + seg;
+  } else {
+    fstr = v.npen;
+    p = v.base + ((int64_t)(seg * 6 + slot) * NF) * v.npen + pencil;
+  }
+#pragma unroll
+  for (int f = 0; f < NF; ++f) acc[f] += p[f * fstr];
+}
+
+// XT: the x axis' Pbar is x-transposed (the split form) and goes through an LDS tile; the
+// one-march form's is in the pack layout and needs no LDS, so its workgroups are smaller
+// (32 x 8) and hold no tile.
+template <int NV, int NDIM, bool XT>
 __global__ void __launch_bounds__(512) k_adj_gather(AdjArgs a) {
   constexpr int NF = NV + 2;
-  constexpr int TX = 32, TY = (NDIM == 1) ? 1 : gather_tile_y(NF);
+  constexpr int TX = 32, TY = (NDIM == 1) ? 1 : (XT ? gather_tile_y(NF) : kGatherTileYMarch);
   // the x axis' Pbar of one 32 x 16 (x, y) tile, read along y from the x-transposed
   // layout and used along x (split form); every other array is read along x
-  __shared__ double xt[NF][TY][TX + 1];
+  using Tile = double[NF][TY][TX + 1];
+  Tile *xtp = nullptr;
+  if constexpr (XT) {
+    __shared__ Tile xt;
+    xtp = &xt;
+  }
   const int b = blockIdx.y;
   const aph_block &bk = a.blocks[b];
   const int N0 = ext(bk, 0, NDIM), N1 = ext(bk, 1, NDIM), N2 = ext(bk, 2, NDIM);
@@ -982,16 +1015,30 @@ __global__ void __launch_bounds__(512) k_adj_gather(AdjArgs a) {
   const double gsum = (a.stage == 1) ? a.gam0 + a.gam1 : a.gam0;
   const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
   const double *pbx = a.pbar + (int64_t)NF * bk.cell_base;
+  // per axis, once: the segment length (and its shift when a power of two), the segment
+  // count and this block's spill records
+  int segS[NDIM], segShift[NDIM];
+  SpillView own[NDIM];
+#pragma unroll
+  for (int d = 0; d < NDIM; ++d) {
+    const int A = (d == 0) ? 1 : 0, B = (d == 2) ? 1 : 2;
+    segS[d] = adj_seg(bk.nx[d], a.segcap[d]);
+    segShift[d] = (segS[d] & (segS[d] - 1)) == 0 ? __builtin_ctz(segS[d]) : -1;
+    own[d].base = a.spill[d] + a.spill_base[d * a.nblocks + b];
+    own[d].npen = (int64_t)bk.nx[A] * bk.nx[B];
+    own[d].nseg = bk.nx[d] / segS[d];
+    own[d].segfast = !XT && d == 0;
+  }
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const int kz = t / (nti * ntj), r = t % (nti * ntj);
     const int i0 = (r % nti) * TX, j0 = (r / nti) * TY;
-    if (a.pbar_xt) {  // thread q reads cell (i0 + q / TY, j0 + q % TY): adjacent y
+    if constexpr (XT) {  // thread q reads cell (i0 + q / TY, j0 + q % TY): adjacent y
       const int jl = threadIdx.x % TY, il = threadIdx.x / TY;
       const int i = i0 + il, j = j0 + jl;
       if (i < nx0 && j < nx1) {
         const int64_t cellt = ((int64_t)(i + g[0]) * N2 + (kz + g[2])) * N1 + (j + g[1]);
 #pragma unroll
-        for (int f = 0; f < NF; ++f) xt[f][jl][il] = pbx[f * nc + cellt];
+        for (int f = 0; f < NF; ++f) (*xtp)[f][jl][il] = pbx[f * nc + cellt];
       }
       __syncthreads();
     }
@@ -1010,9 +1057,9 @@ __global__ void __launch_bounds__(512) k_adj_gather(AdjArgs a) {
       // the axes' Pbar, summed x, then y, then z
       double pbv[NF];
       const double *pp = a.pbar + (int64_t)NF * bk.cell_base + cell;
-      if (a.pbar_xt) {
+      if constexpr (XT) {
 #pragma unroll
-        for (int f = 0; f < NF; ++f) pbv[f] = xt[f][ty][tx];
+        for (int f = 0; f < NF; ++f) pbv[f] = (*xtp)[f][ty][tx];
       } else {
 #pragma unroll
         for (int f = 0; f < NF; ++f) pbv[f] = pp[f * nc];
@@ -1023,39 +1070,40 @@ __global__ void __launch_bounds__(512) k_adj_gather(AdjArgs a) {
         for (int f = 0; f < NF; ++f) pbv[f] = pbv[f] + pp[d * a.pbar_axis + f * nc];
       // spills (primitive space) of the segments / blocks / boundary images that read
       // this cell
+#pragma unroll
       for (int d = 0; d < NDIM; ++d) {
         const int Ad = (d == 0) ? 1 : 0, Bd = (d == 2) ? 1 : 2;
         const int n_d = bk.nx[d];
-        const int S = adj_seg(n_d, a.segcap[d]), nseg = n_d / S;
+        const int S = segS[d], nseg = (int)own[d].nseg;
         const int pencil = cc[Ad] + cc[Bd] * bk.nx[Ad];
-        const int s = cc[d] / S, off = cc[d] % S;
+        const int s = segShift[d] >= 0 ? cc[d] >> segShift[d] : cc[d] / S;
+        const int off = cc[d] - s * S;
         const int sdn = (d == 0) ? 1 : (d == 1 ? 3 : 9);
         if (off < 3) {  // from below
           if (s > 0) {
-            add_slot<NF>(a, d, b, pencil, s - 1, nseg, 3 + off, pbv);
+            add_slot_view<NF>(own[d], pencil, s - 1, 3 + off, pbv);
           } else {
             const int nb = bk.nbr[13 - sdn];
             if (nb >= 0) {
               add_slot<NF>(a, d, nb, pencil, nseg - 1, nseg, 3 + off, pbv);
             } else if (bk.bc[2 * d] == APH_BC_REFLECT) {
-              add_slot<NF>(a, d, b, pencil, 0, nseg, 2 - off, pbv);
+              add_slot_view<NF>(own[d], pencil, 0, 2 - off, pbv);
             } else if (bk.bc[2 * d] != APH_BC_AMR && cc[d] == 0) {  // outflow images
-              for (int sl = 0; sl < 3; ++sl) add_slot<NF>(a, d, b, pencil, 0, nseg, sl, pbv);
+              for (int sl = 0; sl < 3; ++sl) add_slot_view<NF>(own[d], pencil, 0, sl, pbv);
             }
           }
         }
         if (off >= S - 3) {  // from above
           if (s < nseg - 1) {
-            add_slot<NF>(a, d, b, pencil, s + 1, nseg, off - (S - 3), pbv);
+            add_slot_view<NF>(own[d], pencil, s + 1, off - (S - 3), pbv);
           } else {
             const int nb = bk.nbr[13 + sdn];
             if (nb >= 0) {
               add_slot<NF>(a, d, nb, pencil, 0, nseg, off - (S - 3), pbv);
             } else if (bk.bc[2 * d + 1] == APH_BC_REFLECT) {
-              add_slot<NF>(a, d, b, pencil, nseg - 1, nseg, 3 + (n_d - 1 - cc[d]), pbv);
+              add_slot_view<NF>(own[d], pencil, nseg - 1, 3 + (n_d - 1 - cc[d]), pbv);
             } else if (bk.bc[2 * d + 1] != APH_BC_AMR && cc[d] == n_d - 1) {
-              for (int sl = 3; sl < 6; ++sl)
-                add_slot<NF>(a, d, b, pencil, nseg - 1, nseg, sl, pbv);
+              for (int sl = 3; sl < 6; ++sl) add_slot_view<NF>(own[d], pencil, nseg - 1, sl, pbv);
             }
           }
         }
@@ -1083,7 +1131,7 @@ __global__ void __launch_bounds__(512) k_adj_gather(AdjArgs a) {
                                             : a.gam1 * lo[v];
       }
     }
-    if (a.pbar_xt) __syncthreads();  // the tile is used before the next one is loaded
+    if constexpr (XT) __syncthreads();  // the tile is used before the next one is loaded
   }
 }
 

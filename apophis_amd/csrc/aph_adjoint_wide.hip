@@ -30,9 +30,17 @@ namespace {
 KFn nd_pick(int nd, KFn k1, KFn k2, KFn k3) { return nd == 1 ? k1 : (nd == 2 ? k2 : k3); }
 }  // namespace
 
-KFn pick_gather_wide(int nv, int nd) {
-  if (nv == 12) return nd_pick(nd, k_adj_gather<12, 1>, k_adj_gather<12, 2>, k_adj_gather<12, 3>);
-  if (nv == 14) return nd_pick(nd, k_adj_gather<14, 1>, k_adj_gather<14, 2>, k_adj_gather<14, 3>);
+KFn pick_gather_wide(int nv, int nd, bool xt) {
+  if (nv == 12)
+    return xt ? nd_pick(nd, k_adj_gather<12, 1, true>, k_adj_gather<12, 2, true>,
+                        k_adj_gather<12, 3, true>)
+              : nd_pick(nd, k_adj_gather<12, 1, false>, k_adj_gather<12, 2, false>,
+                        k_adj_gather<12, 3, false>);
+  if (nv == 14)
+    return xt ? nd_pick(nd, k_adj_gather<14, 1, true>, k_adj_gather<14, 2, true>,
+                        k_adj_gather<14, 3, true>)
+              : nd_pick(nd, k_adj_gather<14, 1, false>, k_adj_gather<14, 2, false>,
+                        k_adj_gather<14, 3, false>);
   return nullptr;
 }
 

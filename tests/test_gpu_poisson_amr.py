@@ -54,13 +54,15 @@ def _cloud(m, seed):
     return bs, cons
 
 
-CASES = [("grav16", False, "Linear"), ("three", True, "Linear"), ("three", False, "Constant"),
-         ("wallfine", True, "Linear"), ("wallfine", False, "Linear")]
+CASES = [("grav16", False, "Linear", 0.0), ("three", True, "Linear", 0.0),
+         ("three", False, "Constant", 0.0), ("wallfine", True, "Linear", 0.0),
+         ("wallfine", False, "Linear", 0.0), ("three", True, "Linear", 0.5)]
 
 
-@pytest.mark.parametrize("name,fc,prol", CASES)
-def test_solution_matches_oracle(gpu_ctx, name, fc, prol):
-    print(f"[case] poisson tree solve {name} fc={fc} {prol}", flush=True)
+@pytest.mark.parametrize("name,fc,prol,alpha", CASES)
+def test_solution_matches_oracle(gpu_ctx, name, fc, prol, alpha):
+    """alpha: diagonal_alpha (the screened problem), on every multigrid level too."""
+    print(f"[case] poisson tree solve {name} fc={fc} {prol} alpha={alpha}", flush=True)
     m = _mesh(name)
     if name == "grav16":
         p = problems.grav_sphere(res=16, block=8, smr_level=2, solver="poisson")
@@ -69,7 +71,7 @@ def test_solution_matches_oracle(gpu_ctx, name, fc, prol):
         bs, cons = _cloud(m, 3)
     G, tol = 0.9, 1e-11
     dbs = hydro.DeviceBlockSet(gpu_ctx, bs)
-    ps = hydro.PoissonGravitySolver(gpu_ctx, dbs.h, G, 0.0, tol, 20000, fc, prol, mesh=m)
+    ps = hydro.PoissonGravitySolver(gpu_ctx, dbs.h, G, alpha, tol, 20000, fc, prol, mesh=m)
     u = torch.as_tensor(cons, device="cuda")
     grav = torch.zeros(bs.field_size(3), dtype=torch.float64, device="cuda")
     phi = torch.zeros(bs.field_size(1), dtype=torch.float64, device="cuda")
@@ -80,7 +82,7 @@ def test_solution_matches_oracle(gpu_ctx, name, fc, prol):
     # the multigrid-preconditioned iteration: within the decks' max_iterations = 15 even at
     # this tolerance (the Jacobi-preconditioned one needs 40-130 here)
     assert resid <= tol and iters <= 15
-    pr = PA.Problem(m, bs, 0.0, fc, prol)
+    pr = PA.Problem(m, bs, alpha, fc, prol)
     rho = pr.leaf_values(cons, bs.nvar, 0)
     coef = pr.mp_coeffs(rho, G)
     np.testing.assert_allclose(coef_g, coef, rtol=1e-13, atol=1e-16 * abs(coef).max())

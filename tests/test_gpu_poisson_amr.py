@@ -56,7 +56,8 @@ def _cloud(m, seed):
 
 CASES = [("grav16", False, "Linear", 0.0), ("three", True, "Linear", 0.0),
          ("three", False, "Constant", 0.0), ("wallfine", True, "Linear", 0.0),
-         ("wallfine", False, "Linear", 0.0), ("three", True, "Linear", 0.5)]
+         ("wallfine", False, "Linear", 0.0), ("three", True, "Linear", 0.5),
+         ("six", True, "Linear", 0.0)]  # six: the coarsest level by Jacobi sweeps
 
 
 @pytest.mark.parametrize("name,fc,prol,alpha", CASES)

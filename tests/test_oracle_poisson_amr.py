@@ -47,6 +47,11 @@ def _mesh(name):
                                     [((0.0, 0.0, 0.0), (0.2, 0.2, 0.2), 1)], BC)
         return amr.AmrMesh(3, (3, 2, 2), (4, 4, 4), 2, (0, 0, 0), (1.5, 1, 1), ref,
                            max_level=1, bc=BC)
+    if name == "six":  # 6^3 blocks: no halving below the base level, a large coarsest level
+        ref = amr.static_refinement(3, (3, 2, 2), (6, 6, 6), (0, 0, 0), (1.5, 1, 1),
+                                    [((0.6, 0.4, 0.4), (0.8, 0.6, 0.6), 1)], BC)
+        return amr.AmrMesh(3, (3, 2, 2), (6, 6, 6), 1, (0, 0, 0), (1.5, 1, 1), ref,
+                           max_level=1, bc=BC)
     if name == "flat":
         return amr.AmrMesh(3, (2, 3, 2), (8, 4, 6), 1, (-0.5, -0.4, -0.6), (0.5, 0.8, 0.6),
                            set(), max_level=0, bc=BC)
@@ -171,7 +176,7 @@ def test_smr_accuracy_limits_of_the_reference():
 
 
 @pytest.mark.skipif(not _lib.available(), reason="native library not built")
-@pytest.mark.parametrize("name", ["grav16", "three", "wallfine", "flat"])
+@pytest.mark.parametrize("name", ["grav16", "three", "wallfine", "flat", "six"])
 def test_library_assembly_equals_the_restatement(name):
     m = _mesh(name)
     bs = m.block_set()

@@ -412,14 +412,20 @@ __device__ __forceinline__ void adj_face_body(const AdjArgs &a) {
   const double *lo_ptr = (XT ? a.lam_t : a.lam_out) + (int64_t)NV * bk.cell_base + L.base;
   double *qlb_ptr = a.qlb + D * a.qb_axis + (int64_t)NF * bk.cell_base + L.base;
   double *qrb_ptr = a.qrb + D * a.qb_axis + (int64_t)NF * bk.cell_base + L.base;
-  auto load = [&](int c, double *dst) {
+  // fetch issues a cell's loads and returns whether it is a reflected image (whose normal
+  // velocity changes sign); the sign is applied where the value is used, so the loads of
+  // the next step stay in flight over this step's Riemann adjoint
+  auto fetch = [&](int c, double *dst) -> bool {
     int idx;
     bool flip;
     const int64_t base = map_cell(m, n_d, c, idx, flip);
     const double *p = prim + (int64_t)NF * base + L.base + (int64_t)idx * L.sd;
 #pragma unroll
     for (int f = 0; f < NF; ++f) dst[f] = p[f * nc];
-    if (flip) dst[IVX] = -1.0 * dst[IVX];
+    return flip;
+  };
+  auto load = [&](int c, double *dst) {
+    if (fetch(c, dst)) dst[IVX] = -1.0 * dst[IVX];
   };
   double win[W][NF];  // win[k] = cell mm-GR+k
   // PPM as in the forward march: bit k of `plain` says every field of win[k] is +-0 or
@@ -458,12 +464,22 @@ __device__ __forceinline__ void adj_face_body(const AdjArgs &a) {
 #pragma unroll
   for (int v = 0; v < NV; ++v)
     lprev[v] = (a0 >= 1) ? lo_ptr[v * nc + (int64_t)(a0 - 1) * L.sd] : 0.0;
+  // the next step's cell and face cotangent, loaded one step ahead (the same addresses,
+  // cells a0-1+GR .. fend-1+GR and faces a0 .. fend-1)
+  double nxt[NF], lnxt[NV];
+  bool nflip = fetch(a0 - 1 + GR, nxt);
+#pragma unroll
+  for (int v = 0; v < NV; ++v)
+    lnxt[v] = (a0 < n_d) ? lo_ptr[v * nc + (int64_t)a0 * L.sd] : 0.0;
   for (int mm = a0 - 1; mm <= fend - 1; ++mm) {
 #pragma unroll
     for (int s = 0; s < W - 1; ++s)
 #pragma unroll
       for (int f = 0; f < NF; ++f) win[s][f] = win[s + 1][f];
-    load(mm + GR, win[W - 1]);
+#pragma unroll
+    for (int f = 0; f < NF; ++f) win[W - 1][f] = nxt[f];
+    if (nflip) win[W - 1][IVX] = -1.0 * win[W - 1][IVX];
+    if (mm + 1 <= fend - 1) nflip = fetch(mm + 1 + GR, nxt);
     double ql[NF], qr[NF];
     if constexpr (RECON == RECON_PPM) {
       plain = (plain >> 1) | (plain_cell(win[W - 1]) << (W - 1));
@@ -488,8 +504,12 @@ __device__ __forceinline__ void adj_face_body(const AdjArgs &a) {
       const int f = mm;
       double lcur[NV];
 #pragma unroll
-      for (int v = 0; v < NV; ++v)
-        lcur[v] = (f < n_d) ? lo_ptr[v * nc + (int64_t)f * L.sd] : 0.0;
+      for (int v = 0; v < NV; ++v) lcur[v] = lnxt[v];
+      if (f + 1 <= fend - 1) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+          lnxt[v] = (f + 1 < n_d) ? lo_ptr[v * nc + (int64_t)(f + 1) * L.sd] : 0.0;
+      }
       double mu[NV];
 #pragma unroll
       for (int v = 0; v < NV; ++v) mu[v] = coef * (lcur[v] - lprev[v]);

@@ -14,8 +14,11 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
 #include <any>
+#include <cctype>
 #include <cmath>
+#include <cstdio>
 #include <fstream>
 #include <sstream>
 #include <cstdint>
@@ -84,7 +87,11 @@ class ParameterInput {
     return GetString(b, k);
   }
   Real GetOrAddReal(const std::string &b, const std::string &k, Real d) {
-    if (!Has(b, k)) Set(b, k, std::to_string(d));
+    if (!Has(b, k)) {  // %.17g reads back as the same double (to_string keeps 6 decimals)
+      char txt[32];
+      std::snprintf(txt, sizeof txt, "%.17g", d);
+      Set(b, k, txt);
+    }
     return std::stod(GetString(b, k));
   }
   int GetInteger(const std::string &b, const std::string &k) const {
@@ -108,6 +115,7 @@ struct MeshData {
   aph_blockset *blocks = nullptr;
   T *cons = nullptr, *prim = nullptr, *gamma = nullptr;
   T *flux[3] = {nullptr, nullptr, nullptr};
+  T *grav = nullptr, *phi = nullptr;  // "gravity" (3 components) and "gravity.phi" (1)
   int32_t first = 0, count = -1;
   int32_t ndim = 1;
 };
@@ -131,11 +139,12 @@ class StateDescriptor {
 
 }  // namespace parthenon
 
-// reference enums (src/main.hpp:32-53)
+// reference enums (src/main.hpp:27-61)
 enum class Fluid { undefined, euler };
 enum class Reconstruction { undefined, dc, plm, ppm };
 enum class RiemannSolver { undefined, none, hlle, hllc, lhllc };
 enum class Integrator { undefined, rk1, rk2, rk3, vl2 };
+enum class Gravity { undefined, none, constant, monopole, poisson };
 
 namespace Apophis {
 
@@ -207,6 +216,16 @@ inline TaskStatus ApplyGravity(std::shared_ptr<MeshData<Real>> &md, const Real g
                                const Real dt) {
   check(md->ctx, aph_apply_gravity(md->ctx, md->blocks, md->first, md->count, md->cons,
                                    md->prim, g, dt));
+  return TaskStatus::complete;
+}
+
+// gravity.cpp:374-410, the reference's signature: the pack's "gravity" field (md->grav, 3
+// components, pack layout), filled by the stage's gravity solver; prim is the stage-start
+// primitive state
+inline TaskStatus ApplyGravity(std::shared_ptr<MeshData<Real>> &md, const Real dt) {
+  if (!md->grav) throw ApophisError("ApplyGravity: the pack has no gravity field");
+  check(md->ctx, aph_apply_gravity_field(md->ctx, md->blocks, md->first, md->count,
+                                         md->cons, md->prim, md->grav, dt));
   return TaskStatus::complete;
 }
 
@@ -298,6 +317,200 @@ inline std::shared_ptr<StateDescriptor> InitializeHydro(parthenon::ParameterInpu
   eos.pfloor = pin->GetOrAddReal("hydro", "pfloor", fl);
   pkg->AddParam<aph_eos>("eos", eos);
   return pkg;
+}
+
+// ----------------------------------------------------------------------- gravity package
+constexpr Real GRAVITY_G = 6.673e-8;  // constants.hpp:5
+
+inline bool deck_true(const std::string &v) {
+  std::string s;
+  for (char c : v) s += static_cast<char>(std::tolower(static_cast<unsigned char>(c)));
+  return s == "true" || s == "1";
+}
+
+// gravity.cpp:220-370 for the solvers this library runs.  Throws ApophisError with the
+// reference's messages where it calls PARTHENON_FAIL.  Parameters:
+//   "gravity_constant", "gravity" (the Gravity enum);
+//   monopole: "nrbin" (gravity/nrbin, else the mesh's nx1: monopole_gravity.hpp:83-88),
+//             "rmax" (gravity/rmax, else the distance to the farthest mesh corner);
+//   poisson:  "max_iterations", "diagonal_alpha", "solver", "boundary_prolongation",
+//             "flux_correct", "err_tol" as gravity.cpp:297-330 stores them, and the
+//             solver's own limits from its gravity/solver_params block (Parthenon's MG /
+//             BiCGSTAB read "residual_tolerance" and "max_iterations" there; the
+//             reference's gravity/max_iterations is stored but read by nothing):
+//             "residual_tolerance" (else err_tol) and "solver_max_iterations" (else
+//             gravity/max_iterations).
+inline std::shared_ptr<StateDescriptor> InitializeGravity(parthenon::ParameterInput *pin) {
+  auto pkg = std::make_shared<StateDescriptor>("Gravity");
+  const auto type = pin->GetOrAddString("gravity", "type", "none");
+  pkg->AddParam<Real>("gravity_constant",
+                      pin->GetOrAddReal("gravity", "gravity_constant", GRAVITY_G));
+  Gravity gravity = Gravity::undefined;
+  if (type == "none") {
+    gravity = Gravity::none;
+  } else if (type == "constant") {
+    gravity = Gravity::constant;
+  } else if (type == "monopole") {
+    gravity = Gravity::monopole;
+  } else if (type == "poisson") {
+    gravity = Gravity::poisson;
+  } else {
+    throw ApophisError("[Apophis]: Gravity not recognized. Exiting.");
+  }
+  pkg->AddParam<Gravity>("gravity", gravity);
+  if (gravity == Gravity::monopole) {
+    int nrbin = pin->Has("gravity", "nrbin") ? pin->GetInteger("gravity", "nrbin") : -1;
+    if (nrbin <= 0) nrbin = pin->GetInteger("parthenon/mesh", "nx1");
+    Real edge[3];
+    for (int a = 0; a < 3; ++a) {
+      const std::string n = std::to_string(a + 1);
+      const Real lo = a == 0 ? std::stod(pin->GetString("parthenon/mesh", "x1min"))
+                             : pin->GetOrAddReal("parthenon/mesh", "x" + n + "min", 0.0);
+      const Real hi = a == 0 ? std::stod(pin->GetString("parthenon/mesh", "x1max"))
+                             : pin->GetOrAddReal("parthenon/mesh", "x" + n + "max", 0.0);
+      edge[a] = std::max(std::abs(lo), std::abs(hi));
+    }
+    const Real default_rmax =
+        std::sqrt(edge[0] * edge[0] + edge[1] * edge[1] + edge[2] * edge[2]);
+    const Real rmax = pin->Has("gravity", "rmax") ? std::stod(pin->GetString("gravity", "rmax"))
+                                                  : default_rmax;
+    if (rmax <= 0.0)
+      throw ApophisError("[Apophis]: gravity/rmax must be positive for monopole gravity.");
+    pkg->AddParam<int>("nrbin", nrbin);
+    pkg->AddParam<Real>("rmax", rmax);
+  }
+  if (gravity == Gravity::poisson) {
+    const int max_it = pin->GetOrAddInteger("gravity", "max_iterations", 10000);
+    pkg->AddParam<int>("max_iterations", max_it);
+    pkg->AddParam<Real>("diagonal_alpha", pin->GetOrAddReal("gravity", "diagonal_alpha", 0.0));
+    const auto solver = pin->GetOrAddString("gravity", "solver", "MG");
+    if (solver != "MG" && solver != "MGBiCGSTAB")
+      throw ApophisError("Unknown solver type " + solver + ".");
+    pkg->AddParam<std::string>("solver", solver);
+    const auto prolong = pin->GetOrAddString("gravity", "boundary_prolongation", "Linear");
+    if (prolong != "Linear" && prolong != "Quadratic" && prolong != "Constant")
+      throw ApophisError("Unknown prolongation method for gravity boundaries.");
+    pkg->AddParam<std::string>("boundary_prolongation", prolong);
+    pkg->AddParam<bool>("flux_correct",
+                        deck_true(pin->GetOrAddString("gravity", "flux_correct", "false")));
+    if (deck_true(pin->GetOrAddString("gravity", "set_flux_boundary", "false")))
+      throw ApophisError("gravity/set_flux_boundary = true is not built (the fine/coarse flux "
+                         "boundary of PoissonEquation)");
+    const Real err_tol = pin->GetOrAddReal("gravity", "err_tol", 1.0e-8);
+    pkg->AddParam<Real>("err_tol", err_tol);
+    pkg->AddParam<Real>("residual_tolerance",
+                        pin->GetOrAddReal("gravity/solver_params", "residual_tolerance", err_tol));
+    pkg->AddParam<int>("solver_max_iterations",
+                       pin->GetOrAddInteger("gravity/solver_params", "max_iterations", max_it));
+  }
+  return pkg;
+}
+
+// A gravity solver (gravity.hpp:13-20) fills the "gravity" field (3 components, pack
+// layout) and, where it has one, "gravity.phi" (1 component), once per stage before the
+// flux tasks (apophis_driver.cpp:64-66).  The reference adds its PreComputeTasks and
+// AddTasks to the stage's task list per partition; AddTasks here runs both, in order,
+// over the mesh's partitions.
+struct GravitySolver {
+  virtual ~GravitySolver() = default;
+  virtual TaskStatus AddTasks(const std::vector<std::shared_ptr<MeshData<Real>>> &parts,
+                              Real *grav, Real *phi) = 0;
+  // whether the solver writes the field (the constant solver's field is the problem
+  // generator's, and "none" has none)
+  virtual bool Solves() const { return true; }
+};
+
+struct NoneGravitySolver : GravitySolver {  // none_gravity.hpp
+  TaskStatus AddTasks(const std::vector<std::shared_ptr<MeshData<Real>>> &, Real *,
+                      Real *) override {
+    return TaskStatus::complete;
+  }
+  bool Solves() const override { return false; }
+};
+
+struct ConstantGravitySolver : NoneGravitySolver {};  // constant_gravity.hpp
+
+// monopole_gravity.hpp:15-305 on the device: the tasks of aph_monopole_*.
+class MonopoleGravitySolver : public GravitySolver {
+ public:
+  MonopoleGravitySolver(aph_context *ctx, int ndim, int nrbin, Real rmax, Real G) : ctx_(ctx) {
+    check(ctx, aph_monopole_create(ctx, ndim, nrbin, rmax, G, &m_));
+  }
+  ~MonopoleGravitySolver() override { aph_monopole_destroy(ctx_, m_); }
+  MonopoleGravitySolver(const MonopoleGravitySolver &) = delete;
+  MonopoleGravitySolver &operator=(const MonopoleGravitySolver &) = delete;
+  TaskStatus AddTasks(const std::vector<std::shared_ptr<MeshData<Real>>> &parts, Real *grav,
+                      Real *phi) override {
+    const auto &all = parts.front();
+    check(ctx_, aph_monopole_zero_radial_bins(ctx_, m_, all->blocks, all->prim));  // :107-112
+    for (const auto &p : parts)                                                    // :114-153
+      check(ctx_, aph_monopole_accumulate_mass(ctx_, m_, p->blocks, p->first, p->count,
+                                               p->prim));
+    check(ctx_, aph_monopole_build_radial_tables(ctx_, m_));                        // :155-213
+    for (const auto &p : parts)                                                    // :215-302
+      check(ctx_, aph_monopole_gravity_field(ctx_, m_, p->blocks, p->first, p->count, grav,
+                                             phi));
+    return TaskStatus::complete;
+  }
+
+ private:
+  aph_context *ctx_;
+  aph_monopole *m_ = nullptr;
+};
+
+// poisson_gravity.hpp on the device, on one uniform lattice (aph_poisson_create): one
+// solve over the whole mesh per stage.
+class PoissonGravitySolver : public GravitySolver {
+ public:
+  PoissonGravitySolver(aph_context *ctx, const aph_blockset *bs, Real G, Real alpha, Real tol,
+                       int max_it)
+      : ctx_(ctx) {
+    check(ctx, aph_poisson_create(ctx, bs, G, alpha, tol, max_it, &ps_));
+  }
+  ~PoissonGravitySolver() override { aph_poisson_destroy(ctx_, ps_); }
+  PoissonGravitySolver(const PoissonGravitySolver &) = delete;
+  PoissonGravitySolver &operator=(const PoissonGravitySolver &) = delete;
+  TaskStatus AddTasks(const std::vector<std::shared_ptr<MeshData<Real>>> &parts, Real *grav,
+                      Real *phi) override {
+    check(ctx_, aph_poisson_solve(ctx_, ps_, parts.front()->prim, grav, phi));
+    return TaskStatus::complete;
+  }
+  // iterations and relative residual of the last solve
+  void Info(int *iterations, Real *residual) const {
+    int32_t it = 0;
+    check(ctx_, aph_poisson_get_info(ctx_, ps_, nullptr, &it, residual));
+    *iterations = it;
+  }
+
+ private:
+  aph_context *ctx_;
+  aph_poisson *ps_ = nullptr;
+};
+
+// The package's solver bound to a device block set (the reference keeps it in the
+// package as "gravity_solver", built from the deck; here it needs the device state).
+// Multilevel meshes take aph_poisson_create_tree, which this uniform-lattice adapter does
+// not build.
+inline std::shared_ptr<GravitySolver> MakeGravitySolver(const StateDescriptor &pkg,
+                                                        aph_context *ctx,
+                                                        const aph_blockset *bs, int ndim) {
+  switch (pkg.Param<Gravity>("gravity")) {
+    case Gravity::monopole:
+      return std::make_shared<MonopoleGravitySolver>(ctx, ndim, pkg.Param<int>("nrbin"),
+                                                     pkg.Param<Real>("rmax"),
+                                                     pkg.Param<Real>("gravity_constant"));
+    case Gravity::poisson:
+      if (pkg.Param<std::string>("boundary_prolongation") == "Quadratic")
+        throw ApophisError("gravity/boundary_prolongation = Quadratic: the quadratic form is "
+                           "not built");
+      return std::make_shared<PoissonGravitySolver>(
+          ctx, bs, pkg.Param<Real>("gravity_constant"), pkg.Param<Real>("diagonal_alpha"),
+          pkg.Param<Real>("residual_tolerance"), pkg.Param<int>("solver_max_iterations"));
+    case Gravity::constant:
+      return std::make_shared<ConstantGravitySolver>();
+    default:
+      return std::make_shared<NoneGravitySolver>();
+  }
 }
 
 }  // namespace Apophis

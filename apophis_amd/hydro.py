@@ -387,7 +387,8 @@ def initialize_gravity(pin):
     the mesh's nx1, monopole_gravity.hpp:83-88; rmax: gravity/rmax, else the distance to
     the farthest mesh corner, gravity.cpp:262-276) and "poisson" (gravity/max_iterations,
     diagonal_alpha, solver, boundary_prolongation, err_tol, flux_correct, and the
-    solver_params block's residual_tolerance; PoissonGravitySolver runs it)."""
+    solver_params block's residual_tolerance and max_iterations, the limits the solver
+    runs with; PoissonGravitySolver runs it)."""
     pkg = StateDescriptor("Gravity")
     gtype = _get(pin, "gravity", "type", "none")
     pkg.add_param("gravity_constant", float(_get(pin, "gravity", "gravity_constant",
@@ -416,8 +417,14 @@ def initialize_gravity(pin):
                                "coarse flux boundary of PoissonEquation)")
         err_tol = float(_get(pin, "gravity", "err_tol", 1.0e-8))
         pkg.add_param("err_tol", err_tol)
+        # the solver's own limits: Parthenon's MG / BiCGSTAB read them from the
+        # gravity/solver_params block (the reference stores gravity/max_iterations, :305-306,
+        # but nothing reads it), else gravity/err_tol and gravity/max_iterations
         pkg.add_param("residual_tolerance",
                       float(_get(pin, "gravity/solver_params", "residual_tolerance", err_tol)))
+        pkg.add_param("solver_max_iterations",
+                      int(_get(pin, "gravity/solver_params", "max_iterations",
+                               pkg.param("max_iterations"))))
     if gtype == "monopole":
         nrbin = int(_get(pin, "gravity", "nrbin", -1))
         if nrbin <= 0:
@@ -551,7 +558,7 @@ class PoissonGravitySolver:
     @classmethod
     def from_package(cls, ctx, pkg, dbs, mesh=None):
         return cls(ctx, dbs, pkg.param("gravity_constant"), pkg.param("diagonal_alpha"),
-                   pkg.param("residual_tolerance"), pkg.param("max_iterations"),
+                   pkg.param("residual_tolerance"), pkg.param("solver_max_iterations"),
                    pkg.param("flux_correct"), pkg.param("boundary_prolongation"), mesh=mesh)
 
     def __del__(self):

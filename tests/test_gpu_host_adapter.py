@@ -54,3 +54,67 @@ def test_cpp_driver_rejects_bad_config(tmp_path):
     assert "Not enough ghost zones for reconstruction" in r.stderr
     r, _ = _run(tmp_path, "hydro/riemann=roe")
     assert r.returncode == 1 and "Riemann solver not recognized" in r.stderr
+
+
+EVRARD = os.path.join(ROOT, "apophis_amd", "inputs", "evrards_collapse.in")
+
+
+def _evrard_pin(res, gtype, nrbin):
+    """The deck's values (apophis_amd/inputs/evrards_collapse.in) as the Python package's
+    parameter blocks."""
+    mesh = {"nghost": 2, "nx1": res}
+    for d in (1, 2, 3):
+        mesh[f"x{d}min"], mesh[f"x{d}max"] = -1.0, 1.0
+    grav = {"type": gtype, "gravity_constant": 1.0, "solver": "MGBiCGSTAB",
+            "flux_correct": "true", "diagonal_alpha": 0.0}
+    if gtype == "monopole":
+        grav["nrbin"] = nrbin
+    return {"parthenon/time": {"cfl": 0.4, "integrator": "rk2"}, "parthenon/mesh": mesh,
+            "hydro": {"reconstruction": "plm", "riemann": "hllc"},
+            "eos": {"gm1": 0.666666667, "cv": 1.5}, "gravity": grav,
+            "gravity/solver_params": {"max_iterations": 15, "residual_tolerance": 1e-8}}
+
+
+@pytest.mark.parametrize("gtype", ["monopole", "poisson"])
+def test_cpp_driver_evrard_matches_python_driver(tmp_path, gpu_ctx, gtype):
+    """evrards_collapse (the deck's 32^3 in blocks of 16^3, 3 steps; on a uniform 16^3 the
+    first CFL step, 0.21, is a fifth of the free-fall time and the 1/r centre collapses to
+    NaN within two steps) with monopole or Poisson
+    self-gravity: the C++ runner's state equals, bit for bit, the Python driver's
+    (ApophisDriver, itself checked against the oracle in tests/test_gpu_monopole.py and
+    tests/test_gpu_poisson.py) from the same deck values and dt policy.  The Poisson runs
+    converge within the deck's solver limit of 15 iterations."""
+    from apophis_amd import hydro
+
+    res, block, nlim, nrbin = 32, 16, 3, 128
+    over = ([f"parthenon/mesh/nx{a}={res}" for a in (1, 2, 3)] +
+            [f"parthenon/meshblock/nx{a}={block}" for a in (1, 2, 3)] +
+            [f"parthenon/time/nlim={nlim}", f"gravity/type={gtype}", f"gravity/nrbin={nrbin}"])
+    out = tmp_path / "u.bin"
+    print(f"[case] apophis_run evrards_collapse {gtype}", flush=True)
+    r = subprocess.run([EXE, EVRARD, *over, "--out", str(out)], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert f"steps={nlim} " in r.stdout
+    if gtype == "poisson":
+        it = int(r.stdout.split("poisson_iterations=")[1].split()[0])
+        assert 1 <= it <= 15
+    p = problems.evrard(res=res, block=block, solver=gtype)
+    pin = _evrard_pin(res, gtype, nrbin)
+    md = hydro.MeshData(gpu_ctx, p.bs, p.cons)
+    drv = hydro.ApophisDriver(hydro.initialize_hydro(pin), md,
+                              gravity=hydro.initialize_gravity(pin))
+    est = drv.initialize()
+    t, dt, tlim = 0.0, float("inf"), 0.8
+    for _ in range(nlim):
+        dt = min(2.0 * dt, est)
+        if t < tlim and tlim - t < dt:
+            dt = tlim - t
+        est = drv.step(dt)
+        t += dt
+    got = np.fromfile(out, dtype=np.float64)
+    ref = md.cons.cpu().numpy()
+    assert got.shape == ref.shape
+    assert np.isfinite(ref).all()  # ghosts included: the outflow copies of interiors
+    assert compare.bitwise_equal_interior(got, ref, p.bs)
+    assert float(r.stdout.split(" t=")[1].split()[0]) == t  # printed with %.17g

@@ -62,9 +62,15 @@ def test_poisson_defaults():
     assert (pkg.param("max_iterations"), pkg.param("diagonal_alpha"), pkg.param("solver"),
             pkg.param("err_tol"), pkg.param("residual_tolerance")) == (10000, 0.0, "MG", 1e-8,
                                                                        1e-8)
+    assert pkg.param("solver_max_iterations") == 10000
     pkg = hydro.initialize_gravity({"gravity": {"type": "poisson", "solver": "MGBiCGSTAB"},
                                     "gravity/solver_params": {"residual_tolerance": 1e-10}})
     assert pkg.param("residual_tolerance") == 1e-10
+    # the solver's iteration limit is its solver_params block's (the Evrard deck's 15), not
+    # the stored gravity/max_iterations
+    pkg = hydro.initialize_gravity({"gravity": {"type": "poisson", "max_iterations": 500},
+                                    "gravity/solver_params": {"max_iterations": 15}})
+    assert (pkg.param("max_iterations"), pkg.param("solver_max_iterations")) == (500, 15)
     # the SMR decks' own settings (poisson_gravity.hpp:41-44, gravity.cpp:312-363)
     assert (pkg.param("flux_correct"), pkg.param("boundary_prolongation")) == (False, "Linear")
     pkg = hydro.initialize_gravity({"gravity": {"type": "poisson", "flux_correct": "true",

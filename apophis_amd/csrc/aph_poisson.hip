@@ -50,7 +50,9 @@ struct aph_poisson {
   double *bmom = nullptr;  // per-block moments
   double *h_sc = nullptr;  // pinned host copy of the scalars
   // multilevel mesh (aph_poisson_create_tree): the operator as CSR over the leaf cells,
-  // BiCGSTAB with the Jacobi preconditioner
+  // BiCGSTAB preconditioned by the multigrid V-cycle below (d_dinv: the inverse diagonal,
+  // which level 0's smoother uses, and the Jacobi preconditioner when no coarser level
+  // exists)
   bool tree = false;
   int64_t nt = 0, ncb = 0;                  // leaf cells, cells per block
   int64_t *d_ptr = nullptr, *d_nbptr = nullptr;

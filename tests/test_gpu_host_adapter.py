@@ -1,6 +1,8 @@
 """The C++ host adapter (reference task shapes: CalculateFluxes<F,R,S>(shared_ptr<MeshData>&),
-FluxFun_t registry, InitializeHydro from an input deck) driving the Sod workload through
-the C ABI matches the oracle."""
+FluxFun_t registry, InitializeHydro and InitializeGravity from an input deck, the gravity
+solvers' AddTasks, ApplyGravity) driving its decks through the C ABI: the Sod workload
+matches the oracle bit for bit, and Evrard's collapse with monopole or Poisson gravity
+matches the Python driver bit for bit."""
 import dataclasses
 import os
 import subprocess

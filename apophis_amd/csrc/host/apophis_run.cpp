@@ -62,6 +62,8 @@ Lattice make_lattice(parthenon::ParameterInput &pin) {
     lo[d] = pin.GetOrAddReal("parthenon/mesh", "x" + n + "min", d == 0 ? 0.0 : -0.5);
     hi[d] = pin.GetOrAddReal("parthenon/mesh", "x" + n + "max", d == 0 ? 1.0 : 0.5);
     L.bn[d] = pin.GetOrAddInteger("parthenon/meshblock", "nx" + n, nx[d]);
+    if (nx[d] < 1 || L.bn[d] < 1)
+      throw ApophisError("mesh and meshblock nx" + n + " must be positive");
     if (nx[d] % L.bn[d]) throw ApophisError("meshblock nx" + n + " must divide mesh nx" + n);
     L.nb[d] = nx[d] / L.bn[d];
     code[2 * d] = bc_code(pin.GetOrAddString("parthenon/mesh", "ix" + n + "_bc", "outflow"));

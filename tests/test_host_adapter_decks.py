@@ -23,6 +23,7 @@ EVRARD = os.path.join(ROOT, "apophis_amd", "inputs", "evrards_collapse.in")
     (["gravity/type=constant"], "no problem generator here sets the field"),
     (["parthenon/mesh/refinement=static"], "one uniform lattice"),
     (["parthenon/meshblock/nx2=12"], "meshblock nx2 must divide mesh nx2"),
+    (["parthenon/meshblock/nx1=0"], "mesh and meshblock nx1 must be positive"),
     (["parthenon/mesh/ix3_bc=bogus"], "unknown boundary condition bogus"),
     (["hydro/reconstruction=ppm"], "Not enough ghost zones for reconstruction"),
     (["job/problem_id=blast"], "implements problem_id = sod and evrards_collapse"),
